@@ -1,0 +1,237 @@
+// aggregate.hip — scalar aggregates (no GROUP BY) and the host helpers shared
+// by every operator that evaluates predicates or aggregates.
+//
+// Replaces execute_aggregate (src/exec/aggregate.rs:154) / reduce_gpu_vec
+// (:1596) / the bolt_reduce kernels (src/jit/agg_kernels.rs:242,818) and
+// agg_with_pre::try_execute_resident (src/exec/agg_with_pre.rs:157).  The
+// reference launches one kernel per aggregate and finishes the per-block
+// partials on the host; here one pass evaluates the filter and up to four
+// aggregates per launch, and a second one-block kernel folds the partials in
+// a fixed order on the device.
+#include <cmath>
+
+#include "reduce_common.h"
+
+namespace bolt {
+
+// ---- host helpers -----------------------------------------------------------
+
+int make_dev_filter(const bolt_filter_t* f, const bolt_column_t* cols, int32_t n_cols, int64_t n_rows,
+                    DevFilter* out) {
+  memset(out, 0, sizeof *out);
+  if (f == nullptr || f->n_terms == 0) return BOLT_OK;
+  if (f->n_terms < 0 || f->n_terms > BOLT_MAX_PRED_TERMS)
+    return set_error(BOLT_ERR_PLAN, "filter: %d terms (max %d)", f->n_terms, BOLT_MAX_PRED_TERMS);
+  if (f->terms == nullptr) return set_error(BOLT_ERR_PLAN, "filter: terms is NULL");
+  for (int i = 0; i < f->n_terms; i++) {
+    const bolt_pred_term_t& t = f->terms[i];
+    if (t.col < 0 || t.col >= n_cols)
+      return set_error(BOLT_ERR_PLAN, "filter term %d: column %d out of range (%d columns)", i, t.col, n_cols);
+    const bolt_column_t& c = cols[t.col];
+    if (c.length != n_rows)
+      return set_error(BOLT_ERR_PLAN, "filter term %d: column length %lld != %lld rows", i,
+                       (long long)c.length, (long long)n_rows);
+    if (t.op < BOLT_CMP_LT || t.op > BOLT_CMP_IS_NOT_NULL)
+      return set_error(BOLT_ERR_PLAN, "filter term %d: unknown comparison %d", i, t.op);
+    if (dtype_size(c.dtype) == 0) return set_error(BOLT_ERR_TYPE, "filter term %d: unknown dtype %d", i, c.dtype);
+    DevTerm& d = out->t[i];
+    d.data = c.data;
+    d.validity = c.validity;
+    d.dtype = c.dtype;
+    d.op = t.op;
+    d.lit_i64 = t.lit_i64;
+    d.lit_f64 = t.lit_f64;
+    d.lit_hi = t.lit_hi;
+  }
+  out->n = f->n_terms;
+  return BOLT_OK;
+}
+
+int make_agg_list(const bolt_agg_t* aggs, int32_t n_aggs, const bolt_column_t* cols, int32_t n_cols,
+                  int32_t carry_dtype, AggList* out) {
+  memset(out, 0, sizeof *out);
+  if (n_aggs < 0 || n_aggs > kMaxAggs)
+    return set_error(BOLT_ERR_PLAN, "%d aggregates (max %d)", n_aggs, kMaxAggs);
+  for (int i = 0; i < n_aggs; i++) {
+    int op = aggs[i].op;
+    if (op < BOLT_AGG_SUM || op > BOLT_AGG_STDDEV_SAMP)
+      return set_error(BOLT_ERR_PLAN, "aggregate %d: unknown op %d", i, op);
+    AggDesc& d = out->a[i];
+    d.op = op;
+    if (op == BOLT_AGG_COUNT_STAR) {
+      d.dtype = BOLT_I64;
+      continue;
+    }
+    int dt;
+    if (aggs[i].col == BOLT_JOIN_CARRY) {
+      if (carry_dtype == 0) return set_error(BOLT_ERR_PLAN, "aggregate %d reads the join carry, but the table has none", i);
+      dt = carry_dtype;
+      d.src = 1;
+    } else {
+      if (aggs[i].col < 0 || aggs[i].col >= n_cols)
+        return set_error(BOLT_ERR_PLAN, "aggregate %d: input column %d out of range (%d columns)", i, aggs[i].col, n_cols);
+      const bolt_column_t& c = cols[aggs[i].col];
+      dt = c.dtype;
+      d.data = c.data;
+      d.validity = c.validity;
+    }
+    d.dtype = dt;
+    if (op == BOLT_AGG_COUNT) continue;
+    if (dt == BOLT_BOOL) return set_error(BOLT_ERR_TYPE, "aggregate %d: Bool inputs are not supported", i);
+    if (op == BOLT_AGG_SUM && dt == BOLT_DATE32)
+      // src/plan/logical_plan.rs:2135-2146
+      return set_error(BOLT_ERR_TYPE, "SUM over a temporal type is undefined in SQL (Date32)");
+    if ((op == BOLT_AGG_AVG || op >= BOLT_AGG_VAR_POP) && dt == BOLT_DATE32)
+      return set_error(BOLT_ERR_TYPE, "AVG/VAR/STDDEV over Date32 is not supported");
+    if (dt == BOLT_DEC128 && op != BOLT_AGG_SUM)
+      return set_error(BOLT_ERR_UNSUPPORTED, "aggregate %d: only SUM is supported over Decimal128", i);
+  }
+  out->n = n_aggs;
+  return BOLT_OK;
+}
+
+int acc_finalize(const Acc& a, int op, int dt, bolt_scalar_t* r) {
+  memset(r, 0, sizeof *r);
+  switch (op) {
+    case BOLT_AGG_COUNT:
+    case BOLT_AGG_COUNT_STAR:
+      r->dtype = BOLT_I64;
+      r->i64 = (int64_t)a.n;
+      return BOLT_OK;
+    case BOLT_AGG_SUM:
+      r->dtype = dt == BOLT_I32 ? BOLT_I64 : dt;
+      if (a.n == 0) { r->is_null = 1; return BOLT_OK; }
+      if (dt == BOLT_DEC128) {
+        if (a.ovf) return set_error(BOLT_ERR_TYPE, "SUM(Decimal128) overflow: accumulator exceeds i128 range");
+        r->i64 = (int64_t)a.lo;
+        r->hi = a.hi;
+      } else if (dtype_is_int(dt)) {
+        // exact 128-bit total must fit i64 (src/exec/aggregate.rs:2120)
+        if (a.ovf || a.hi != ((int64_t)a.lo >> 63))
+          return set_error(BOLT_ERR_TYPE, "SUM(integer) overflow: accumulator exceeds i64 range");
+        r->i64 = (int64_t)a.lo;
+      } else {
+        // Neumaier result; non-finite terms fall back to the plain fold,
+        // which is the running leading part (src/exec/aggregate.rs:2022)
+        double v = (std::isfinite(a.s) && std::isfinite(a.c)) ? a.s + a.c : a.s;
+        r->f64 = dt == BOLT_F32 ? (double)(float)v : v;
+      }
+      return BOLT_OK;
+    case BOLT_AGG_AVG:
+      r->dtype = BOLT_F64;
+      if (a.n == 0) { r->is_null = 1; return BOLT_OK; }
+      r->f64 = ((std::isfinite(a.s) && std::isfinite(a.c)) ? a.s + a.c : a.s) / (double)a.n;
+      return BOLT_OK;
+    case BOLT_AGG_MIN:
+    case BOLT_AGG_MAX:
+      r->dtype = dt;
+      if (a.n == 0) { r->is_null = 1; return BOLT_OK; }
+      if (dtype_is_float(dt)) r->f64 = f64_from_order_key(a.mm);
+      else r->i64 = (int64_t)(a.mm ^ 0x8000000000000000ull);
+      return BOLT_OK;
+    default: {
+      // Welford finalize (src/exec/welford.rs:104-134)
+      r->dtype = BOLT_F64;
+      bool samp = (op == BOLT_AGG_VAR_SAMP || op == BOLT_AGG_STDDEV_SAMP);
+      if (a.n == 0 || (samp && a.n <= 1)) { r->is_null = 1; return BOLT_OK; }
+      double var = a.c / (double)(samp ? a.n - 1 : a.n);
+      r->f64 = (op == BOLT_AGG_STDDEV_POP || op == BOLT_AGG_STDDEV_SAMP) ? std::sqrt(var) : var;
+      return BOLT_OK;
+    }
+  }
+}
+
+int launch_finish(int na, const Acc* parts, int n_parts, const AggList& aggs, Acc* out, hipStream_t s) {
+  switch (na) {
+    case 1: finish_partials_kernel<1><<<1, kReduceBlock, 0, s>>>(parts, n_parts, aggs, out); break;
+    case 2: finish_partials_kernel<2><<<1, kReduceBlock, 0, s>>>(parts, n_parts, aggs, out); break;
+    case 3: finish_partials_kernel<3><<<1, kReduceBlock, 0, s>>>(parts, n_parts, aggs, out); break;
+    case 4: finish_partials_kernel<4><<<1, kReduceBlock, 0, s>>>(parts, n_parts, aggs, out); break;
+    default: return set_error(BOLT_ERR_OTHER, "launch_finish: bad chunk size %d", na);
+  }
+  BOLT_HIP(hipGetLastError());
+  return BOLT_OK;
+}
+
+// ---- scalar aggregate kernel -------------------------------------------------
+
+// Grid-stride pass: each thread folds its rows into NA register accumulators;
+// one partial per block.  Rows are visited in a fixed thread mapping so the
+// float results are reproducible for a given grid.
+template <int NA>
+__global__ __launch_bounds__(kReduceBlock) void scalar_agg_kernel(DevFilter f, AggList aggs, int64_t n,
+                                                                 Acc* __restrict__ parts) {
+  Acc acc[NA];
+#pragma unroll
+  for (int a = 0; a < NA; a++) acc_init(acc[a], aggs.a[a].op);
+  int64_t stride = (int64_t)gridDim.x * kReduceBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kReduceBlock + threadIdx.x; i < n; i += stride) {
+    if (!eval_filter(f, i)) continue;
+#pragma unroll
+    for (int a = 0; a < NA; a++) {
+      const AggDesc& d = aggs.a[a];
+      if (d.op == BOLT_AGG_COUNT_STAR) { acc[a].n++; continue; }
+      if (!bit_valid(d.validity, i)) continue;
+      int64_t vi, vhi;
+      double vf;
+      load_value(d.data, d.dtype, i, vi, vhi, vf);
+      acc_update(acc[a], d.op, d.dtype, vi, vhi, vf);
+    }
+  }
+  block_reduce_store<NA>(acc, aggs, parts + (int64_t)blockIdx.x * NA);
+}
+
+static int launch_scalar_chunk(int na, const DevFilter& f, const AggList& chunk, int64_t n, unsigned grid,
+                               Acc* parts, hipStream_t s) {
+  switch (na) {
+    case 1: scalar_agg_kernel<1><<<grid, kReduceBlock, 0, s>>>(f, chunk, n, parts); break;
+    case 2: scalar_agg_kernel<2><<<grid, kReduceBlock, 0, s>>>(f, chunk, n, parts); break;
+    case 3: scalar_agg_kernel<3><<<grid, kReduceBlock, 0, s>>>(f, chunk, n, parts); break;
+    case 4: scalar_agg_kernel<4><<<grid, kReduceBlock, 0, s>>>(f, chunk, n, parts); break;
+    default: return set_error(BOLT_ERR_OTHER, "scalar aggregate: bad chunk size %d", na);
+  }
+  BOLT_HIP(hipGetLastError());
+  return BOLT_OK;
+}
+
+}  // namespace bolt
+
+using namespace bolt;
+
+extern "C" int bolt_aggregate(bolt_stream_t stream, const bolt_column_t* inputs, int32_t n_inputs,
+                              const bolt_agg_t* aggs, int32_t n_aggs, const bolt_filter_t* filter,
+                              bolt_scalar_t* results) {
+  BOLT_CHECK_DEVICE();
+  clear_error();
+  hipStream_t s = (hipStream_t)stream;
+  if (n_inputs < 1 || inputs == nullptr)
+    return set_error(BOLT_ERR_PLAN, "bolt_aggregate: at least one input column is required");
+  int64_t n = inputs[0].length;
+  for (int i = 1; i < n_inputs; i++)
+    if (inputs[i].length != n)
+      return set_error(BOLT_ERR_PLAN, "bolt_aggregate: column %d has %lld rows, column 0 has %lld", i,
+                       (long long)inputs[i].length, (long long)n);
+  DevFilter f;
+  BOLT_TRY(make_dev_filter(filter, inputs, n_inputs, n, &f));
+  AggList all;
+  BOLT_TRY(make_agg_list(aggs, n_aggs, inputs, n_inputs, 0, &all));
+  if (n_aggs == 0) return BOLT_OK;
+
+  unsigned grid = grid_for(n, kReduceBlock, 8, 2048);
+  int n_chunks = (n_aggs + 3) / 4;
+  DevBuf<Acc> parts, outs;
+  BOLT_TRY(parts.alloc((size_t)grid * 4 * n_chunks, s));
+  BOLT_TRY(outs.alloc((size_t)n_aggs, s));
+  for (int c = 0; c < n_chunks; c++) {
+    int first = c * 4, na = n_aggs - first < 4 ? n_aggs - first : 4;
+    AggList chunk = agg_chunk(all, first, na);
+    Acc* p = parts.ptr + (size_t)c * grid * 4;
+    BOLT_TRY(launch_scalar_chunk(na, f, chunk, n, grid, p, s));
+    BOLT_TRY(launch_finish(na, p, (int)grid, chunk, outs.ptr + first, s));
+  }
+  Acc host[kMaxAggs];
+  BOLT_HIP(hipMemcpyAsync(host, outs.ptr, sizeof(Acc) * n_aggs, hipMemcpyDeviceToHost, s));
+  BOLT_HIP(hipStreamSynchronize(s));
+  for (int i = 0; i < n_aggs; i++) BOLT_TRY(acc_finalize(host[i], all.a[i].op, all.a[i].dtype, &results[i]));
+  return BOLT_OK;
+}

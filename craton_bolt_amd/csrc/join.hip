@@ -1,0 +1,466 @@
+// join.hip — INNER hash join: build, probe (pair emission), and the fused
+// filter + probe + aggregate form.
+//
+// Replaces hash_join_indices_on_gpu_resident (src/exec/gpu_join.rs:1583),
+// the build / probe kernels (src/jit/hash_join_kernel.rs:484,711) and the
+// collision-list variants for duplicate keys (:1183,1368).
+//
+// Table layout (MI355X-first): the reference uses an SoA open-addressing
+// table (i64 keys with an i64::MIN empty sentinel + u32 row ids, load <= 0.5,
+// src/exec/gpu_join.rs:577).  Here a bucket is one 128-byte cache line:
+//   [0,4)    u32 claim count        (no sentinel: any int64 key is storable)
+//   [8,88)   10 x i64 keys
+//   [88,128) 10 x u32 values (build row id, or the 4-byte carried column)
+// A probe reads one line for a hit or a miss unless the bucket overflowed
+// (claim count > 10, then the chain continues in the next bucket), and the
+// table runs at ~80% load instead of 50%, so a 16M-row build side occupies
+// ~256 MB — about the size of the MI355X Infinity Cache.
+#include <cstdlib>
+
+#include "reduce_common.h"
+
+namespace bolt {
+
+constexpr int kSlots = 10;
+constexpr int kBucketBytes = 128;
+constexpr int kKeyOff = 8;
+constexpr int kValOff = 88;
+constexpr int kBuildBlock = 256;
+constexpr int kProbeBlock = 256;
+
+__host__ __device__ __forceinline__ uint64_t fmix64(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return x;
+}
+// Bucket index by multiply-shift range reduction of the mixed high word
+// (table sizes need not be powers of two).
+__host__ __device__ __forceinline__ uint32_t bucket_of(int64_t key, uint32_t nb) {
+  uint64_t h = fmix64((uint64_t)key);
+  return (uint32_t)(((h >> 32) * (uint64_t)nb) >> 32);
+}
+
+__device__ __forceinline__ int64_t load_key(const void* p, int dtype, int64_t i) {
+  return dtype == BOLT_I64 ? static_cast<const int64_t*>(p)[i] : (int64_t) static_cast<const int32_t*>(p)[i];
+}
+
+// ---- build -------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBuildBlock) void join_build_kernel(uint8_t* __restrict__ tbl, uint32_t nb,
+                                                                 const void* __restrict__ key, int kdt,
+                                                                 const uint8_t* __restrict__ kvalid,
+                                                                 const uint32_t* __restrict__ carry, int64_t n,
+                                                                 uint32_t* __restrict__ lost) {
+  int64_t stride = (int64_t)gridDim.x * kBuildBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBuildBlock + threadIdx.x; i < n; i += stride) {
+    if (!bit_valid(kvalid, i)) continue;  // NULL build keys never match (src/exec/join.rs:851)
+    int64_t k = load_key(key, kdt, i);
+    uint32_t v = carry ? carry[i] : (uint32_t)i;
+    uint32_t b = bucket_of(k, nb);
+    bool placed = false;
+    for (uint32_t step = 0; step < nb; step++) {
+      uint8_t* line = tbl + (size_t)b * kBucketBytes;
+      uint32_t s = atomicAdd(reinterpret_cast<uint32_t*>(line), 1u);
+      if (s < kSlots) {
+        reinterpret_cast<int64_t*>(line + kKeyOff)[s] = k;
+        reinterpret_cast<uint32_t*>(line + kValOff)[s] = v;
+        placed = true;
+        break;
+      }
+      b = (b + 1 == nb) ? 0 : b + 1;
+    }
+    if (!placed) atomicAdd(lost, 1u);
+  }
+}
+
+// Walk the chain of `k`: calls on_match(value) for every stored copy.
+// Returns after the first match when `first_only`.
+template <typename F>
+__device__ __forceinline__ void probe_chain(const uint8_t* __restrict__ tbl, uint32_t nb, int64_t k,
+                                            bool first_only, F&& on_match) {
+  uint32_t b = bucket_of(k, nb);
+  for (uint32_t step = 0; step < nb; step++) {
+    const uint8_t* line = tbl + (size_t)b * kBucketBytes;
+    // one 128-byte line: 8 x 16-byte loads issued together
+    const uint4* q = reinterpret_cast<const uint4*>(line);
+    uint4 w[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) w[j] = q[j];
+    const uint32_t* words = reinterpret_cast<const uint32_t*>(w);
+    uint32_t cnt = words[0];
+    uint32_t m = cnt < kSlots ? cnt : kSlots;
+    // fully unrolled so every word index is a compile-time constant and the
+    // line stays in registers
+#pragma unroll
+    for (int s = 0; s < kSlots; s++) {
+      int64_t kk = (int64_t)(((uint64_t)words[2 + 2 * s + 1] << 32) | words[2 + 2 * s]);
+      if ((uint32_t)s < m && kk == k) {
+        on_match(words[kValOff / 4 + s]);
+        if (first_only) return;
+      }
+    }
+    if (cnt <= kSlots) return;
+    b = (b + 1 == nb) ? 0 : b + 1;
+  }
+}
+
+// Marks the table non-unique if any build key is stored more than once.
+__global__ __launch_bounds__(kBuildBlock) void join_dup_kernel(const uint8_t* __restrict__ tbl, uint32_t nb,
+                                                               const void* __restrict__ key, int kdt,
+                                                               const uint8_t* __restrict__ kvalid, int64_t n,
+                                                               uint32_t* __restrict__ nonunique) {
+  int64_t stride = (int64_t)gridDim.x * kBuildBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBuildBlock + threadIdx.x; i < n; i += stride) {
+    if (!bit_valid(kvalid, i)) continue;
+    int64_t k = load_key(key, kdt, i);
+    int c = 0;
+    probe_chain(tbl, nb, k, false, [&](uint32_t) { c++; });
+    if (c > 1) *nonunique = 1u;
+  }
+}
+
+// ---- fused filter + probe + aggregate ------------------------------------------
+
+template <typename KeyT, int NA>
+__global__ __launch_bounds__(kProbeBlock) void join_probe_agg_kernel(
+    const uint8_t* __restrict__ tbl, uint32_t nb, const uint32_t* __restrict__ nonunique,
+    const KeyT* __restrict__ keys, const uint8_t* __restrict__ kvalid, int64_t n, DevFilter f, AggList aggs,
+    int carry_dtype, Acc* __restrict__ parts) {
+  Acc acc[NA];
+#pragma unroll
+  for (int a = 0; a < NA; a++) acc_init(acc[a], aggs.a[a].op);
+  const bool first_only = (*nonunique == 0u);
+  int64_t stride = (int64_t)gridDim.x * kProbeBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kProbeBlock + threadIdx.x; i < n; i += stride) {
+    if (!bit_valid(kvalid, i)) continue;
+    if (!eval_filter(f, i)) continue;
+    int64_t k = (int64_t)keys[i];
+    probe_chain(tbl, nb, k, first_only, [&](uint32_t w) {
+#pragma unroll
+      for (int a = 0; a < NA; a++) {
+        const AggDesc& d = aggs.a[a];
+        if (d.op == BOLT_AGG_COUNT_STAR) { acc[a].n++; continue; }
+        int64_t vi, vhi = 0;
+        double vf;
+        if (d.src == 1) {
+          carry_value(w, carry_dtype, vi, vf);
+        } else {
+          if (!bit_valid(d.validity, i)) continue;
+          load_value(d.data, d.dtype, i, vi, vhi, vf);
+        }
+        acc_update(acc[a], d.op, d.dtype, vi, vhi, vf);
+      }
+    });
+  }
+  block_reduce_store<NA>(acc, aggs, parts + (int64_t)blockIdx.x * NA);
+}
+
+// ---- probe emitting (probe_row, build_row) pairs -----------------------------
+
+// Wave-aggregated output allocation: one atomic per wave and pass.
+template <typename KeyT>
+__global__ __launch_bounds__(kProbeBlock) void join_probe_emit_kernel(
+    const uint8_t* __restrict__ tbl, uint32_t nb, const uint32_t* __restrict__ nonunique,
+    const KeyT* __restrict__ keys, const uint8_t* __restrict__ kvalid, int64_t n, DevFilter f,
+    unsigned long long* __restrict__ counter, int64_t capacity, uint32_t* __restrict__ out_probe,
+    uint32_t* __restrict__ out_build) {
+  const bool first_only = (*nonunique == 0u);
+  int lane = threadIdx.x & 63;
+  int64_t stride = (int64_t)gridDim.x * kProbeBlock;
+  int64_t base_i = (int64_t)blockIdx.x * kProbeBlock + (threadIdx.x & ~63);
+  // every lane of a wave iterates the same number of times so the wave-wide
+  // scan below is convergent
+  for (int64_t wi = base_i; wi < n; wi += stride) {
+    int64_t i = wi + lane;
+    bool active = i < n && bit_valid(kvalid, i) && eval_filter(f, i);
+    int64_t k = active ? (int64_t)keys[i] : 0;
+    uint32_t m = 0;
+    if (active) probe_chain(tbl, nb, k, first_only, [&](uint32_t) { m++; });
+    // inclusive scan of m across the wave
+    uint32_t incl = m;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      uint32_t t = __shfl_up(incl, d);
+      if (lane >= d) incl += t;
+    }
+    uint32_t total = __shfl(incl, 63);
+    unsigned long long base = 0;
+    if (lane == 63 && total) base = atomicAdd(counter, (unsigned long long)total);
+    base = __shfl(base, 63);
+    if (m) {
+      unsigned long long pos = base + incl - m;
+      probe_chain(tbl, nb, k, first_only, [&](uint32_t w) {
+        if ((int64_t)pos < capacity) {
+          out_probe[pos] = (uint32_t)i;
+          out_build[pos] = w;
+        }
+        pos++;
+      });
+    }
+  }
+}
+
+}  // namespace bolt
+
+using namespace bolt;
+
+struct bolt_join_table {
+  uint8_t* tbl = nullptr;
+  uint32_t nb = 0;
+  int64_t n_build = 0;
+  int32_t key_dtype = 0;
+  int32_t carry_dtype = 0;  // 0 = values are build row ids
+  uint32_t* flags = nullptr; // [0] nonunique, [1] lost rows
+  hipStream_t stream = nullptr;
+};
+
+static size_t join_table_cap_bytes() {
+  // BOLT_GPU_JOIN_TABLE_CAP_MB (src/exec/gpu_join.rs:393) overrides; the
+  // default is a quarter of device memory (the reference caps at 512 MiB on
+  // devices with >= 8 GiB, gpu_join.rs:378-388, sized for consumer cards).
+  const char* env = getenv("BOLT_GPU_JOIN_TABLE_CAP_MB");
+  if (env && *env) return (size_t)strtoull(env, nullptr, 10) << 20;
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+    (void)hipGetLastError();
+    return (size_t)512 << 20;
+  }
+  return tot / 4;
+}
+
+static int check_key_col(const bolt_column_t* c, const char* what) {
+  if (c == nullptr) return set_error(BOLT_ERR_PLAN, "%s: key column is NULL", what);
+  if (c->dtype != BOLT_I32 && c->dtype != BOLT_I64 && c->dtype != BOLT_DATE32)
+    return set_error(BOLT_ERR_TYPE, "%s: join key dtype %s not supported (Int32/Int64/Date32)", what,
+                     dtype_name(c->dtype));
+  if (c->length < 0 || c->length > 0xFFFFFFFFll)
+    return set_error(BOLT_ERR_PLAN, "%s: %lld rows exceeds the u32 row-id limit", what, (long long)c->length);
+  return BOLT_OK;
+}
+
+extern "C" int bolt_join_build(bolt_stream_t stream, const bolt_column_t* build_key, const bolt_column_t* carry,
+                               bolt_join_table_t** out) {
+  BOLT_CHECK_DEVICE();
+  clear_error();
+  *out = nullptr;
+  hipStream_t s = (hipStream_t)stream;
+  BOLT_TRY(check_key_col(build_key, "bolt_join_build"));
+  int32_t carry_dt = 0;
+  if (carry) {
+    if (dtype_size(carry->dtype) != 4)
+      return set_error(BOLT_ERR_TYPE, "bolt_join_build: carried column must be 4 bytes wide, got %s",
+                       dtype_name(carry->dtype));
+    if (carry->length != build_key->length)
+      return set_error(BOLT_ERR_PLAN, "bolt_join_build: carried column length differs from the key");
+    if (carry->validity)
+      return set_error(BOLT_ERR_UNSUPPORTED, "bolt_join_build: carried column must not have NULLs");
+    carry_dt = carry->dtype;
+  }
+  int64_t n = build_key->length;
+  uint64_t nb64 = (uint64_t)((n + 7) / 8);  // ~80% of 10 slots
+  if (nb64 < 1) nb64 = 1;
+  size_t bytes = (size_t)nb64 * kBucketBytes;
+  if (nb64 >= 0xFFFFFFFFull || bytes > join_table_cap_bytes())
+    return set_error(BOLT_DECLINE, "hash join: build side of %lld rows needs a %zu-byte table, over the cap",
+                     (long long)n, bytes);
+  auto* t = new bolt_join_table();
+  t->nb = (uint32_t)nb64;
+  t->n_build = n;
+  t->key_dtype = build_key->dtype == BOLT_I64 ? BOLT_I64 : BOLT_I32;
+  t->carry_dtype = carry_dt;
+  t->stream = s;
+  int st = dev_alloc(reinterpret_cast<void**>(&t->tbl), bytes, s);
+  if (st == BOLT_OK) st = dev_alloc(reinterpret_cast<void**>(&t->flags), 64, s);
+  if (st != BOLT_OK) {
+    bolt_join_table_release(t);
+    return st;
+  }
+  if ((reinterpret_cast<uintptr_t>(t->tbl) & (kBucketBytes - 1)) != 0) {
+    bolt_join_table_release(t);
+    return set_error(BOLT_ERR_MEMORY, "hash join: table allocation is not 128-byte aligned");
+  }
+  // zero the claim counts (whole table: the lines are touched by the build anyway)
+  if ((st = fill_u32(t->tbl, 0, bytes / 4, s)) != BOLT_OK || (st = fill_u32(t->flags, 0, 16, s)) != BOLT_OK) {
+    bolt_join_table_release(t);
+    return st;
+  }
+  if (n > 0) {
+    unsigned grid = grid_for(n, kBuildBlock, 4, 8192);
+    join_build_kernel<<<grid, kBuildBlock, 0, s>>>(t->tbl, t->nb, build_key->data, t->key_dtype,
+                                                   build_key->validity,
+                                                   carry ? static_cast<const uint32_t*>(carry->data) : nullptr, n,
+                                                   t->flags + 1);
+    join_dup_kernel<<<grid, kBuildBlock, 0, s>>>(t->tbl, t->nb, build_key->data, t->key_dtype, build_key->validity,
+                                                 n, t->flags);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      bolt_join_table_release(t);
+      return set_error(BOLT_ERR_DEVICE, "hash join build launch failed: %s", hipGetErrorString(e));
+    }
+  }
+  *out = t;
+  return BOLT_OK;
+}
+
+extern "C" int bolt_join_table_unique(const bolt_join_table_t* t, int32_t* unique) {
+  BOLT_CHECK_DEVICE();
+  uint32_t h[2] = {0, 0};
+  BOLT_HIP(hipMemcpyAsync(h, t->flags, 8, hipMemcpyDeviceToHost, t->stream));
+  BOLT_HIP(hipStreamSynchronize(t->stream));
+  if (h[1] != 0) return set_error(BOLT_ERR_OTHER, "hash join build lost %u rows (table full)", h[1]);
+  *unique = h[0] == 0 ? 1 : 0;
+  return BOLT_OK;
+}
+
+extern "C" void bolt_join_table_release(bolt_join_table_t* t) {
+  if (!t) return;
+  if (t->tbl) dev_free(t->tbl, t->stream);
+  if (t->flags) dev_free(t->flags, t->stream);
+  delete t;
+}
+
+static int check_probe_args(const bolt_join_table_t* t, const bolt_column_t* probe_cols, int32_t n_probe_cols,
+                            int32_t key_col, const char* what) {
+  if (t == nullptr) return set_error(BOLT_ERR_PLAN, "%s: table is NULL", what);
+  if (probe_cols == nullptr || key_col < 0 || key_col >= n_probe_cols)
+    return set_error(BOLT_ERR_PLAN, "%s: probe key column %d out of range (%d columns)", what, key_col,
+                     n_probe_cols);
+  BOLT_TRY(check_key_col(&probe_cols[key_col], what));
+  int64_t n = probe_cols[key_col].length;
+  for (int i = 0; i < n_probe_cols; i++)
+    if (probe_cols[i].length != n)
+      return set_error(BOLT_ERR_PLAN, "%s: probe column %d has %lld rows, key has %lld", what, i,
+                       (long long)probe_cols[i].length, (long long)n);
+  return BOLT_OK;
+}
+
+template <typename KeyT>
+static int launch_probe_agg(int na, const bolt_join_table_t* t, const bolt_column_t& key, const DevFilter& f,
+                            const AggList& chunk, unsigned grid, Acc* parts, hipStream_t s) {
+  const KeyT* kp = static_cast<const KeyT*>(key.data);
+#define PROBE_AGG_CASE(NA)                                                                          \
+  case NA:                                                                                          \
+    join_probe_agg_kernel<KeyT, NA><<<grid, kProbeBlock, 0, s>>>(t->tbl, t->nb, t->flags, kp,        \
+                                                                 key.validity, key.length, f, chunk, \
+                                                                 t->carry_dtype, parts);            \
+    break;
+  switch (na) {
+    PROBE_AGG_CASE(1)
+    PROBE_AGG_CASE(2)
+    PROBE_AGG_CASE(3)
+    PROBE_AGG_CASE(4)
+    default: return set_error(BOLT_ERR_OTHER, "join aggregate: bad chunk size %d", na);
+  }
+#undef PROBE_AGG_CASE
+  BOLT_HIP(hipGetLastError());
+  return BOLT_OK;
+}
+
+// Grid for the probe kernels; BOLT_JOIN_PROBE_GRID overrides (tuning).
+static unsigned probe_grid(int64_t n) {
+  const char* env = getenv("BOLT_JOIN_PROBE_GRID");
+  if (env && *env) return (unsigned)strtoul(env, nullptr, 10);
+  return grid_for(n, kProbeBlock, 4, 256 * 32);
+}
+
+extern "C" int bolt_join_probe_aggregate(bolt_stream_t stream, const bolt_join_table_t* t,
+                                         const bolt_column_t* probe_cols, int32_t n_probe_cols, int32_t key_col,
+                                         const bolt_filter_t* filter, const bolt_agg_t* aggs, int32_t n_aggs,
+                                         bolt_scalar_t* results) {
+  BOLT_CHECK_DEVICE();
+  clear_error();
+  hipStream_t s = (hipStream_t)stream;
+  BOLT_TRY(check_probe_args(t, probe_cols, n_probe_cols, key_col, "bolt_join_probe_aggregate"));
+  const bolt_column_t& key = probe_cols[key_col];
+  DevFilter f;
+  BOLT_TRY(make_dev_filter(filter, probe_cols, n_probe_cols, key.length, &f));
+  AggList all;
+  BOLT_TRY(make_agg_list(aggs, n_aggs, probe_cols, n_probe_cols, t->carry_dtype, &all));
+  if (n_aggs == 0) return BOLT_OK;
+  unsigned grid = probe_grid(key.length);
+  int n_chunks = (n_aggs + 3) / 4;
+  DevBuf<Acc> parts, outs;
+  BOLT_TRY(parts.alloc((size_t)grid * 4 * n_chunks, s));
+  BOLT_TRY(outs.alloc((size_t)n_aggs, s));
+  for (int c = 0; c < n_chunks; c++) {
+    int first = c * 4, na = n_aggs - first < 4 ? n_aggs - first : 4;
+    AggList chunk = agg_chunk(all, first, na);
+    Acc* p = parts.ptr + (size_t)c * grid * 4;
+    if (key.dtype == BOLT_I64) BOLT_TRY(launch_probe_agg<int64_t>(na, t, key, f, chunk, grid, p, s));
+    else BOLT_TRY(launch_probe_agg<int32_t>(na, t, key, f, chunk, grid, p, s));
+    BOLT_TRY(launch_finish(na, p, (int)grid, chunk, outs.ptr + first, s));
+  }
+  Acc host[kMaxAggs];
+  uint32_t fl[2];
+  BOLT_HIP(hipMemcpyAsync(host, outs.ptr, sizeof(Acc) * n_aggs, hipMemcpyDeviceToHost, s));
+  BOLT_HIP(hipMemcpyAsync(fl, t->flags, 8, hipMemcpyDeviceToHost, s));
+  BOLT_HIP(hipStreamSynchronize(s));
+  if (fl[1] != 0) return set_error(BOLT_ERR_OTHER, "hash join build lost %u rows (table full)", fl[1]);
+  for (int i = 0; i < n_aggs; i++) BOLT_TRY(acc_finalize(host[i], all.a[i].op, all.a[i].dtype, &results[i]));
+  return BOLT_OK;
+}
+
+extern "C" int bolt_join_probe(bolt_stream_t stream, const bolt_join_table_t* t, const bolt_column_t* probe_cols,
+                               int32_t n_probe_cols, int32_t key_col, const bolt_filter_t* filter,
+                               uint32_t* out_probe_rows, uint32_t* out_build_rows, int64_t capacity,
+                               int64_t* n_out) {
+  BOLT_CHECK_DEVICE();
+  clear_error();
+  hipStream_t s = (hipStream_t)stream;
+  *n_out = 0;
+  BOLT_TRY(check_probe_args(t, probe_cols, n_probe_cols, key_col, "bolt_join_probe"));
+  if (t->carry_dtype != 0)
+    return set_error(BOLT_ERR_PLAN, "bolt_join_probe: table stores a carried column, not build row ids");
+  const bolt_column_t& key = probe_cols[key_col];
+  DevFilter f;
+  BOLT_TRY(make_dev_filter(filter, probe_cols, n_probe_cols, key.length, &f));
+  DevBuf<unsigned long long> counter;
+  BOLT_TRY(counter.alloc(1, s));
+  BOLT_TRY(fill_u32(counter.ptr, 0, 2, s));
+  if (key.length > 0) {
+    unsigned grid = probe_grid(key.length);
+    if (key.dtype == BOLT_I64)
+      join_probe_emit_kernel<int64_t><<<grid, kProbeBlock, 0, s>>>(
+          t->tbl, t->nb, t->flags, static_cast<const int64_t*>(key.data), key.validity, key.length, f, counter.ptr,
+          capacity, out_probe_rows, out_build_rows);
+    else
+      join_probe_emit_kernel<int32_t><<<grid, kProbeBlock, 0, s>>>(
+          t->tbl, t->nb, t->flags, static_cast<const int32_t*>(key.data), key.validity, key.length, f, counter.ptr,
+          capacity, out_probe_rows, out_build_rows);
+    BOLT_HIP(hipGetLastError());
+  }
+  unsigned long long total = 0;
+  uint32_t fl[2];
+  BOLT_HIP(hipMemcpyAsync(&total, counter.ptr, 8, hipMemcpyDeviceToHost, s));
+  BOLT_HIP(hipMemcpyAsync(fl, t->flags, 8, hipMemcpyDeviceToHost, s));
+  BOLT_HIP(hipStreamSynchronize(s));
+  if (fl[1] != 0) return set_error(BOLT_ERR_OTHER, "hash join build lost %u rows (table full)", fl[1]);
+  *n_out = (int64_t)total;
+  if ((int64_t)total > capacity)
+    return set_error(BOLT_DECLINE, "hash join: %llu matches exceed the output capacity %lld", total,
+                     (long long)capacity);
+  return BOLT_OK;
+}
+
+extern "C" int bolt_inner_join(bolt_stream_t stream, const bolt_column_t* left_key, const bolt_column_t* right_key,
+                               uint32_t* out_left_rows, uint32_t* out_right_rows, int64_t capacity, int64_t* n_out) {
+  BOLT_CHECK_DEVICE();
+  clear_error();
+  *n_out = 0;
+  BOLT_TRY(check_key_col(left_key, "bolt_inner_join"));
+  BOLT_TRY(check_key_col(right_key, "bolt_inner_join"));
+  if (left_key->length == 0 || right_key->length == 0) return BOLT_OK;
+  // the smaller side builds (src/exec/join.rs:192)
+  bool build_is_left = left_key->length <= right_key->length;
+  const bolt_column_t* b = build_is_left ? left_key : right_key;
+  const bolt_column_t* p = build_is_left ? right_key : left_key;
+  bolt_join_table_t* t = nullptr;
+  BOLT_TRY(bolt_join_build(stream, b, nullptr, &t));
+  int st = build_is_left ? bolt_join_probe(stream, t, p, 1, 0, nullptr, out_right_rows, out_left_rows, capacity, n_out)
+                         : bolt_join_probe(stream, t, p, 1, 0, nullptr, out_left_rows, out_right_rows, capacity, n_out);
+  bolt_join_table_release(t);
+  return st;
+}

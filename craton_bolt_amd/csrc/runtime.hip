@@ -1,0 +1,471 @@
+// runtime.hip — the bolt_hip_* device-runtime C ABI over HIP.
+//
+// Mirrors the CUDA-driver entry points bound in src/cuda/cuda_sys.rs:89-224
+// one for one (see include/bolt_hip.h).  HIP has one primary context per
+// device, so the reference's context calls reduce to device selection.
+// Allocation goes through HIP's stream-ordered memory pool with the release
+// threshold raised, so freed blocks stay cached for the next query instead of
+// returning to the driver (the role of DeviceMemPool, src/cuda/mem_pool.rs:1144).
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+#include "bolt_internal.h"
+
+namespace bolt {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+void clear_error() { g_last_error.clear(); }
+
+static std::once_flag g_dev_once;
+static int g_dev_status = BOLT_ERR_NO_DEVICE;
+
+int device_status() {
+  std::call_once(g_dev_once, [] {
+    int n = 0;
+    if (hipGetDeviceCount(&n) == hipSuccess && n > 0) {
+      g_dev_status = 0;
+      // Keep freed pool memory cached across queries.
+      for (int d = 0; d < n; d++) {
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, d) == hipSuccess) {
+          uint64_t thr = ~0ull;
+          (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+        }
+      }
+    } else {
+      (void)hipGetLastError();
+    }
+  });
+  if (g_dev_status != 0) set_error(BOLT_ERR_NO_DEVICE, "no HIP device available");
+  return g_dev_status;
+}
+
+// ---- pool accounting --------------------------------------------------------
+
+static std::mutex g_pool_mu;
+static std::unordered_map<void*, size_t> g_live;
+static size_t g_in_use = 0, g_peak = 0;
+
+static void account_alloc(void* p, size_t bytes) {
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  g_live[p] = bytes;
+  g_in_use += bytes;
+  if (g_in_use > g_peak) g_peak = g_in_use;
+}
+static void account_free(void* p) {
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  auto it = g_live.find(p);
+  if (it != g_live.end()) {
+    g_in_use -= it->second;
+    g_live.erase(it);
+  }
+}
+
+int dev_alloc(void** p, size_t bytes, hipStream_t s) {
+  BOLT_CHECK_DEVICE();
+  *p = nullptr;
+  hipError_t e = hipMallocAsync(p, bytes ? bytes : 1, s);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(BOLT_ERR_MEMORY, "device allocation of %zu bytes failed: %s", bytes,
+                     hipGetErrorString(e));
+  }
+  account_alloc(*p, bytes);
+  return BOLT_OK;
+}
+
+int dev_free(void* p, hipStream_t s) {
+  if (!p) return BOLT_OK;
+  account_free(p);
+  BOLT_HIP(hipFreeAsync(p, s));
+  return BOLT_OK;
+}
+
+// ---- fill kernels (graph-capture safe) ----------------------------------------
+
+__global__ void fill_u32_kernel(uint32_t* __restrict__ dst, uint32_t v, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = v;
+}
+__global__ void fill_u8_kernel(uint8_t* __restrict__ dst, uint8_t v, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = v;
+}
+
+int fill_u32(void* dst, uint32_t value, size_t n_words, hipStream_t s) {
+  if (n_words == 0) return BOLT_OK;
+  unsigned grid = grid_for((int64_t)n_words, 256, 4);
+  fill_u32_kernel<<<grid, 256, 0, s>>>(static_cast<uint32_t*>(dst), value, n_words);
+  BOLT_HIP(hipGetLastError());
+  return BOLT_OK;
+}
+int fill_u8(void* dst, uint8_t value, size_t n, hipStream_t s) {
+  if (n == 0) return BOLT_OK;
+  // word-aligned bulk as u32, tail as bytes
+  uintptr_t a = reinterpret_cast<uintptr_t>(dst);
+  if ((a & 3) == 0 && n >= 4) {
+    uint32_t w = value * 0x01010101u;
+    BOLT_TRY(fill_u32(dst, w, n / 4, s));
+    size_t done = n & ~size_t(3);
+    if (done == n) return BOLT_OK;
+    dst = static_cast<uint8_t*>(dst) + done;
+    n -= done;
+  }
+  unsigned grid = grid_for((int64_t)n, 256, 4);
+  fill_u8_kernel<<<grid, 256, 0, s>>>(static_cast<uint8_t*>(dst), value, n);
+  BOLT_HIP(hipGetLastError());
+  return BOLT_OK;
+}
+
+const char* dtype_name(int32_t dt) {
+  switch (dt) {
+    case BOLT_BOOL: return "Bool";
+    case BOLT_I32: return "Int32";
+    case BOLT_I64: return "Int64";
+    case BOLT_F32: return "Float32";
+    case BOLT_F64: return "Float64";
+    case BOLT_DEC128: return "Decimal128";
+    case BOLT_DATE32: return "Date32";
+  }
+  return "Unknown";
+}
+
+// ---- stream pool (src/cuda/stream_pool.rs:136,170) ----------------------------
+
+static std::mutex g_stream_mu;
+static std::vector<hipStream_t> g_free_streams;
+
+}  // namespace bolt
+
+using namespace bolt;
+
+#define RT_CHECK() BOLT_CHECK_DEVICE()
+#define RT_HIP(expr)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess) {                                                               \
+      set_error((int)e_, "%s failed: %s", #expr, hipGetErrorString(e_));                 \
+      return (int)e_;                                                                     \
+    }                                                                                     \
+  } while (0)
+
+extern "C" {
+
+const char* bolt_last_error(void) { return g_last_error.c_str(); }
+
+const char* bolt_build_info(void) { return "craton_bolt_amd 0.1 gfx950"; }
+
+int bolt_hip_init(unsigned int flags) {
+  if (flags != 0) return set_error(hipErrorInvalidValue, "bolt_hip_init: flags must be 0");
+  RT_CHECK();
+  RT_HIP(hipInit(0));
+  return 0;
+}
+
+int bolt_hip_device_get_count(int* count) {
+  *count = 0;
+  RT_CHECK();
+  RT_HIP(hipGetDeviceCount(count));
+  return 0;
+}
+
+int bolt_hip_device_get_name(char* name, int len, int device) {
+  RT_CHECK();
+  hipDeviceProp_t p;
+  RT_HIP(hipGetDeviceProperties(&p, device));
+  snprintf(name, (size_t)len, "%s", p.name);
+  return 0;
+}
+
+int bolt_hip_device_get_attribute(int* value, int attrib, int device) {
+  RT_CHECK();
+  RT_HIP(hipDeviceGetAttribute(value, (hipDeviceAttribute_t)attrib, device));
+  return 0;
+}
+
+int bolt_hip_device_total_mem(size_t* bytes, int device) {
+  RT_CHECK();
+  RT_HIP(hipDeviceTotalMem(bytes, device));
+  return 0;
+}
+
+int bolt_hip_set_device(int device) {
+  RT_CHECK();
+  RT_HIP(hipSetDevice(device));
+  return 0;
+}
+
+int bolt_hip_get_device(int* device) {
+  RT_CHECK();
+  RT_HIP(hipGetDevice(device));
+  return 0;
+}
+
+int bolt_hip_device_synchronize(void) {
+  RT_CHECK();
+  RT_HIP(hipDeviceSynchronize());
+  return 0;
+}
+
+int bolt_hip_malloc(void** dptr, size_t bytes) {
+  RT_CHECK();
+  *dptr = nullptr;
+  RT_HIP(hipMallocAsync(dptr, bytes ? bytes : 1, nullptr));
+  RT_HIP(hipStreamSynchronize(nullptr));
+  account_alloc(*dptr, bytes);
+  return 0;
+}
+
+int bolt_hip_free(void* dptr) {
+  RT_CHECK();
+  if (!dptr) return 0;
+  account_free(dptr);
+  RT_HIP(hipFreeAsync(dptr, nullptr));
+  RT_HIP(hipStreamSynchronize(nullptr));
+  return 0;
+}
+
+int bolt_hip_malloc_async(void** dptr, size_t bytes, bolt_stream_t stream) {
+  RT_CHECK();
+  *dptr = nullptr;
+  RT_HIP(hipMallocAsync(dptr, bytes ? bytes : 1, (hipStream_t)stream));
+  account_alloc(*dptr, bytes);
+  return 0;
+}
+
+int bolt_hip_free_async(void* dptr, bolt_stream_t stream) {
+  RT_CHECK();
+  if (!dptr) return 0;
+  account_free(dptr);
+  RT_HIP(hipFreeAsync(dptr, (hipStream_t)stream));
+  return 0;
+}
+
+int bolt_hip_host_alloc(void** ptr, size_t bytes, unsigned int flags) {
+  RT_CHECK();
+  (void)flags;
+  RT_HIP(hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault));
+  return 0;
+}
+
+int bolt_hip_host_free(void* ptr) {
+  RT_CHECK();
+  RT_HIP(hipHostFree(ptr));
+  return 0;
+}
+
+int bolt_hip_mem_get_info(size_t* free_bytes, size_t* total_bytes) {
+  RT_CHECK();
+  RT_HIP(hipMemGetInfo(free_bytes, total_bytes));
+  return 0;
+}
+
+int bolt_hip_memcpy_htod(void* dst, const void* src, size_t bytes) {
+  RT_CHECK();
+  RT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return 0;
+}
+int bolt_hip_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+  RT_CHECK();
+  RT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return 0;
+}
+int bolt_hip_memcpy_dtod(void* dst, const void* src, size_t bytes) {
+  RT_CHECK();
+  RT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  return 0;
+}
+int bolt_hip_memcpy_htod_async(void* dst, const void* src, size_t bytes, bolt_stream_t stream) {
+  RT_CHECK();
+  RT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+  return 0;
+}
+int bolt_hip_memcpy_dtoh_async(void* dst, const void* src, size_t bytes, bolt_stream_t stream) {
+  RT_CHECK();
+  RT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  return 0;
+}
+int bolt_hip_memcpy_dtod_async(void* dst, const void* src, size_t bytes, bolt_stream_t stream) {
+  RT_CHECK();
+  RT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return 0;
+}
+
+int bolt_hip_memset_d8(void* dst, unsigned char value, size_t count) {
+  RT_CHECK();
+  int st = fill_u8(dst, value, count, nullptr);
+  if (st != BOLT_OK) return hipErrorLaunchFailure;
+  RT_HIP(hipStreamSynchronize(nullptr));
+  return 0;
+}
+int bolt_hip_memset_d8_async(void* dst, unsigned char value, size_t count, bolt_stream_t stream) {
+  RT_CHECK();
+  int st = fill_u8(dst, value, count, (hipStream_t)stream);
+  return st == BOLT_OK ? 0 : (int)hipErrorLaunchFailure;
+}
+
+int bolt_hip_module_load_data(bolt_module_t* module, const void* image) {
+  RT_CHECK();
+  hipModule_t m;
+  RT_HIP(hipModuleLoadData(&m, image));
+  *module = (bolt_module_t)m;
+  return 0;
+}
+int bolt_hip_module_unload(bolt_module_t module) {
+  RT_CHECK();
+  RT_HIP(hipModuleUnload((hipModule_t)module));
+  return 0;
+}
+int bolt_hip_module_get_function(bolt_function_t* func, bolt_module_t module, const char* name) {
+  RT_CHECK();
+  hipFunction_t f;
+  RT_HIP(hipModuleGetFunction(&f, (hipModule_t)module, name));
+  *func = (bolt_function_t)f;
+  return 0;
+}
+int bolt_hip_launch_kernel(bolt_function_t func, unsigned int gx, unsigned int gy, unsigned int gz,
+                           unsigned int bx, unsigned int by, unsigned int bz, unsigned int shmem,
+                           bolt_stream_t stream, void** kernel_params, void** extra) {
+  RT_CHECK();
+  RT_HIP(hipModuleLaunchKernel((hipFunction_t)func, gx, gy, gz, bx, by, bz, shmem, (hipStream_t)stream,
+                               kernel_params, extra));
+  return 0;
+}
+
+int bolt_hip_stream_create(bolt_stream_t* stream, unsigned int flags) {
+  RT_CHECK();
+  hipStream_t s;
+  RT_HIP(hipStreamCreateWithFlags(&s, flags ? hipStreamNonBlocking : hipStreamDefault));
+  *stream = (bolt_stream_t)s;
+  return 0;
+}
+int bolt_hip_stream_destroy(bolt_stream_t stream) {
+  RT_CHECK();
+  RT_HIP(hipStreamDestroy((hipStream_t)stream));
+  return 0;
+}
+int bolt_hip_stream_synchronize(bolt_stream_t stream) {
+  RT_CHECK();
+  RT_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return 0;
+}
+int bolt_hip_stream_pool_acquire(bolt_stream_t* stream) {
+  RT_CHECK();
+  {
+    std::lock_guard<std::mutex> lk(g_stream_mu);
+    if (!g_free_streams.empty()) {
+      *stream = (bolt_stream_t)g_free_streams.back();
+      g_free_streams.pop_back();
+      return 0;
+    }
+  }
+  hipStream_t s;
+  RT_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = (bolt_stream_t)s;
+  return 0;
+}
+int bolt_hip_stream_pool_release(bolt_stream_t stream) {
+  RT_CHECK();
+  std::lock_guard<std::mutex> lk(g_stream_mu);
+  g_free_streams.push_back((hipStream_t)stream);
+  return 0;
+}
+
+int bolt_hip_event_create(bolt_event_t* event, unsigned int flags) {
+  RT_CHECK();
+  hipEvent_t e;
+  RT_HIP(hipEventCreateWithFlags(&e, (flags & 2) ? hipEventDisableTiming : hipEventDefault));
+  *event = (bolt_event_t)e;
+  return 0;
+}
+int bolt_hip_event_record(bolt_event_t event, bolt_stream_t stream) {
+  RT_CHECK();
+  RT_HIP(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+  return 0;
+}
+int bolt_hip_event_query(bolt_event_t event) {
+  RT_CHECK();
+  hipError_t e = hipEventQuery((hipEvent_t)event);
+  if (e == hipErrorNotReady) {
+    (void)hipGetLastError();
+    return BOLT_HIP_ERROR_NOT_READY;
+  }
+  RT_HIP(e);
+  return 0;
+}
+int bolt_hip_event_synchronize(bolt_event_t event) {
+  RT_CHECK();
+  RT_HIP(hipEventSynchronize((hipEvent_t)event));
+  return 0;
+}
+int bolt_hip_event_elapsed_ms(float* ms, bolt_event_t start, bolt_event_t end) {
+  RT_CHECK();
+  RT_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)end));
+  return 0;
+}
+int bolt_hip_event_destroy(bolt_event_t event) {
+  RT_CHECK();
+  RT_HIP(hipEventDestroy((hipEvent_t)event));
+  return 0;
+}
+
+int bolt_hip_stream_begin_capture(bolt_stream_t stream, unsigned int mode) {
+  RT_CHECK();
+  RT_HIP(hipStreamBeginCapture((hipStream_t)stream, (hipStreamCaptureMode)mode));
+  return 0;
+}
+int bolt_hip_stream_end_capture(bolt_stream_t stream, bolt_graph_t* graph) {
+  RT_CHECK();
+  hipGraph_t g;
+  RT_HIP(hipStreamEndCapture((hipStream_t)stream, &g));
+  *graph = (bolt_graph_t)g;
+  return 0;
+}
+int bolt_hip_graph_instantiate(bolt_graph_exec_t* exec, bolt_graph_t graph) {
+  RT_CHECK();
+  hipGraphExec_t x;
+  RT_HIP(hipGraphInstantiate(&x, (hipGraph_t)graph, nullptr, nullptr, 0));
+  *exec = (bolt_graph_exec_t)x;
+  return 0;
+}
+int bolt_hip_graph_launch(bolt_graph_exec_t exec, bolt_stream_t stream) {
+  RT_CHECK();
+  RT_HIP(hipGraphLaunch((hipGraphExec_t)exec, (hipStream_t)stream));
+  return 0;
+}
+int bolt_hip_graph_exec_destroy(bolt_graph_exec_t exec) {
+  RT_CHECK();
+  RT_HIP(hipGraphExecDestroy((hipGraphExec_t)exec));
+  return 0;
+}
+int bolt_hip_graph_destroy(bolt_graph_t graph) {
+  RT_CHECK();
+  RT_HIP(hipGraphDestroy((hipGraph_t)graph));
+  return 0;
+}
+
+int bolt_hip_pool_stats(size_t* in_use_bytes, size_t* peak_bytes) {
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  *in_use_bytes = g_in_use;
+  *peak_bytes = g_peak;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,103 @@
+"""Seeded synthetic tables (numpy) from libbolt_datagen.so.
+
+Each generator consumes a fixed number of Xorshift64Star draws per row, so
+row r depends only on (seed, r) and the tables are filled by all host
+threads (see csrc/datagen.c).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libbolt_datagen.so"
+_lib = None
+
+# seeds of the operator workloads
+SEED_GROUPBY_SMALL = 0xB0170001
+SEED_JOIN = 0xB0170002
+SEED_SORT = 0xB0170003
+SEED_ZIPF = 0xB0170004
+SEED_BATCH = 0xB0170005
+SEED_SERVE = 0xB0170006
+
+_u64, _i64, _p = C.c_uint64, C.c_int64, C.c_void_p
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m craton_bolt_amd.build`")
+        L = C.CDLL(str(LIB_PATH))
+        L.bolt_gen_state_at.restype = _u64
+        L.bolt_gen_state_at.argtypes = [_u64, _u64]
+        L.bolt_gen_draws.argtypes = [_u64, _u64, _i64, _p]
+        L.bolt_scramble64.restype = _u64
+        L.bolt_scramble64.argtypes = [_u64]
+        L.bolt_scramble32.restype = C.c_uint32
+        L.bolt_scramble32.argtypes = [C.c_uint32]
+        L.bolt_gen_groupby_small.argtypes = [_u64, _i64, _u64, _u64, _p, _p]
+        L.bolt_gen_join_build.argtypes = [_u64, _i64, _u64, _p, _p]
+        L.bolt_gen_join_probe.argtypes = [_u64, _i64, _i64, C.c_double, _p, _p]
+        L.bolt_gen_sort_pairs.argtypes = [_u64, _i64, _p, _p]
+        L.bolt_gen_zipf_groupby.argtypes = [_u64, _i64, _i64, C.c_double, _p, _p]
+        L.bolt_gen_zipf_small.argtypes = [_u64, _i64, _i64, C.c_double, _p, _p, _p]
+        _lib = L
+    return _lib
+
+
+def draws(seed: int, n: int, offset: int = 0) -> np.ndarray:
+    out = np.empty(n, dtype=np.uint64)
+    lib().bolt_gen_draws(seed, offset, n, out.ctypes.data)
+    return out
+
+
+def groupby_small(n: int, card: int = 1000, vmod: int = 1000, seed: int = SEED_GROUPBY_SMALL):
+    """k int32 = draw % card, v int64 = draw % vmod (two draws per row)."""
+    k = np.empty(n, np.int32)
+    v = np.empty(n, np.int64)
+    lib().bolt_gen_groupby_small(seed, n, card, vmod, k.ctypes.data, v.ctypes.data)
+    return k, v
+
+
+def join_build(n_build: int, pmod: int = 1_000_000, seed: int = SEED_JOIN):
+    key = np.empty(n_build, np.int64)
+    payload = np.empty(n_build, np.int32)
+    lib().bolt_gen_join_build(seed, n_build, pmod, key.ctypes.data, payload.ctypes.data)
+    return key, payload
+
+
+def join_probe(n_build: int, n_probe: int, amount_scale: float = 100.0, seed: int = SEED_JOIN,
+               out_fk: np.ndarray | None = None, out_amount: np.ndarray | None = None):
+    fk = out_fk if out_fk is not None else np.empty(n_probe, np.int64)
+    amount = out_amount if out_amount is not None else np.empty(n_probe, np.float64)
+    lib().bolt_gen_join_probe(seed, n_build, n_probe, amount_scale, fk.ctypes.data, amount.ctypes.data)
+    return fk, amount
+
+
+def sort_pairs(n: int, seed: int = SEED_SORT):
+    key = np.empty(n, np.int64)
+    rid = np.empty(n, np.uint32)
+    lib().bolt_gen_sort_pairs(seed, n, key.ctypes.data, rid.ctypes.data)
+    return key, rid
+
+
+def zipf_groupby(n: int, n_ranks: int = 10_000_000, s: float = 1.1, seed: int = SEED_ZIPF):
+    key = np.empty(n, np.int64)
+    val = np.empty(n, np.float64)
+    lib().bolt_gen_zipf_groupby(seed, n, n_ranks, s, key.ctypes.data, val.ctypes.data)
+    return key, val
+
+
+def zipf_small(n: int, card: int, s: float = 1.1, seed: int = SEED_BATCH):
+    key = np.empty(n, np.int32)
+    vi = np.empty(n, np.int64)
+    vf = np.empty(n, np.float32)
+    lib().bolt_gen_zipf_small(seed, n, card, s, key.ctypes.data, vi.ctypes.data, vf.ctypes.data)
+    return key, vi, vf
+
+
+def scramble64(x: int) -> int:
+    return int(lib().bolt_scramble64(x))

@@ -1,0 +1,260 @@
+"""Host-side operator layer over the C ABI (include/bolt_ops.h).
+
+Function names and argument meaning follow the reference's executors
+(src/exec/*.rs): execute_aggregate, compact_columns, execute_groupby,
+execute_inner_join, execute_sort, plus the join build / probe pair of
+src/exec/gpu_join.rs.  Inputs and outputs are device columns; results that
+the reference returns as scalars are returned as Python values, with SQL NULL
+as None.  Errors surface as the typed exceptions of `_native` (GpuCapacity is
+the decline the caller answers by running its host path).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native as N
+from .device import DeviceBuffer, DeviceColumn, Stream, columns_array
+
+
+# ---- predicates -------------------------------------------------------------
+
+@dataclass
+class Term:
+    """`col <op> literal` (or IS [NOT] NULL) over the operator's input columns."""
+
+    col: int
+    op: int
+    value: float | int = 0
+
+    def c(self, dtype: int) -> N.BoltPredTerm:
+        t = N.BoltPredTerm(self.col, self.op, 0, 0.0, 0)
+        if dtype in (N.F32, N.F64):
+            t.lit_f64 = float(self.value)
+        elif dtype == N.DEC128:
+            v = int(self.value)
+            t.lit_i64 = C.c_int64(v & 0xFFFFFFFFFFFFFFFF).value
+            t.lit_hi = C.c_int64((v >> 64) & 0xFFFFFFFFFFFFFFFF).value
+        else:
+            t.lit_i64 = int(self.value)
+        return t
+
+
+def _filter(terms: list[Term] | None, cols: list[DeviceColumn]):
+    if not terms:
+        return None, None
+    arr = (N.BoltPredTerm * len(terms))()
+    for i, t in enumerate(terms):
+        arr[i] = t.c(cols[t.col].dtype)
+    f = N.BoltFilter(arr, len(terms))
+    return f, arr  # keep `arr` alive
+
+
+def _stream(s: Stream | None):
+    return s.handle if s is not None else None
+
+
+def scalar_value(s: N.BoltScalar):
+    """Python value of a bolt_scalar_t (None for SQL NULL)."""
+    if s.is_null:
+        return None
+    if s.dtype in (N.F32, N.F64):
+        return float(s.f64)
+    if s.dtype == N.DEC128:
+        return (int(s.hi) << 64) | (int(s.i64) & 0xFFFFFFFFFFFFFFFF)
+    return int(s.i64)
+
+
+def _aggs(aggs: list[tuple[int, int]]):
+    arr = (N.BoltAgg * max(len(aggs), 1))()
+    for i, (op, col) in enumerate(aggs):
+        arr[i] = N.BoltAgg(op, col)
+    return arr
+
+
+# ---- scalar aggregates (src/exec/aggregate.rs:154) ----------------------------
+
+def execute_aggregate(cols: list[DeviceColumn], aggs: list[tuple[int, int]], terms: list[Term] | None = None,
+                      stream: Stream | None = None) -> list:
+    """SELECT agg(col), ... FROM cols [WHERE terms]; one value per aggregate."""
+    ca = columns_array(cols)
+    f, keep = _filter(terms, cols)
+    res = (N.BoltScalar * max(len(aggs), 1))()
+    st = N.lib().bolt_aggregate(_stream(stream), ca, len(cols), _aggs(aggs), len(aggs),
+                                C.byref(f) if f else None, res)
+    N.check(st, "bolt_aggregate")
+    return [scalar_value(res[i]) for i in range(len(aggs))]
+
+
+# ---- filter + compaction (src/exec/gpu_compact.rs:1005) -----------------------
+
+def compact_columns(cols: list[DeviceColumn], terms: list[Term] | None = None, mask: DeviceBuffer | None = None,
+                    stream: Stream | None = None) -> tuple[list[DeviceColumn], int]:
+    """Rows passing `terms` (and `mask`, a u8 per row) gathered in input order."""
+    n = cols[0].length
+    outs = [DeviceColumn.empty(n, c.dtype, with_validity=c.validity is not None) for c in cols]
+    ca, oa = columns_array(cols), columns_array(outs)
+    f, keep = _filter(terms, cols)
+    n_out = C.c_int64()
+    st = N.lib().bolt_filter_compact(_stream(stream), ca, len(cols), C.byref(f) if f else None,
+                                     mask.ptr if mask else None, oa, C.byref(n_out))
+    N.check(st, "bolt_filter_compact")
+    for o in outs:
+        o.length = n_out.value
+    return outs, n_out.value
+
+
+def prefix_scan_mask(mask: DeviceBuffer, n: int, stream: Stream | None = None) -> tuple[DeviceBuffer, int]:
+    out = DeviceBuffer(max(n, 1) * 4)
+    total = C.c_int64()
+    N.check(N.lib().bolt_prefix_scan_mask(_stream(stream), mask.ptr, n, out.ptr, C.byref(total)),
+            "bolt_prefix_scan_mask")
+    return out, total.value
+
+
+# ---- GROUP BY (src/exec/groupby.rs:295) -----------------------------------------
+
+def execute_groupby(keys: list[DeviceColumn], inputs: list[DeviceColumn], aggs: list[tuple[int, int]],
+                    terms: list[Term] | None = None, sorted_output: bool = True, flags: int = 0,
+                    stream: Stream | None = None) -> dict:
+    """Returns {'n_groups', 'keys': [(values, valid)], 'aggs': [(values, valid)]} as numpy arrays."""
+    ka, ia = columns_array(keys), columns_array(inputs)
+    f, keep = _filter(terms, inputs)
+    tbl = C.c_void_p()
+    fl = flags | (N.GROUPBY_SORTED if sorted_output else 0)
+    st = N.lib().bolt_groupby(_stream(stream), ka, len(keys), ia, len(inputs), _aggs(aggs), len(aggs),
+                              C.byref(f) if f else None, fl, C.byref(tbl))
+    N.check(st, "bolt_groupby")
+    try:
+        nr = C.c_int64()
+        nc = C.c_int32()
+        N.check(N.lib().bolt_table_num_rows(tbl, C.byref(nr)))
+        N.check(N.lib().bolt_table_num_columns(tbl, C.byref(nc)))
+        cols = []
+        for i in range(nc.value):
+            bc = N.BoltColumn()
+            N.check(N.lib().bolt_table_column(tbl, i, C.byref(bc)))
+            cols.append(_download(bc, nr.value))
+    finally:
+        N.lib().bolt_table_release(tbl)
+    return {"n_groups": nr.value, "keys": cols[: len(keys)], "aggs": cols[len(keys):]}
+
+
+def _download(bc: N.BoltColumn, n: int):
+    from .device import _NP_DTYPE
+
+    if bc.dtype == N.DEC128:
+        vals = np.empty((n, 2), dtype=np.int64)
+    else:
+        vals = np.empty(n, dtype=_NP_DTYPE[bc.dtype])
+    if n:
+        N.check_rt(N.lib().bolt_hip_memcpy_dtoh(vals.ctypes.data, bc.data, vals.nbytes), "dtoh")
+    valid = None
+    if bc.validity:
+        bits = np.empty((n + 7) // 8, dtype=np.uint8)
+        if n:
+            N.check_rt(N.lib().bolt_hip_memcpy_dtoh(bits.ctypes.data, bc.validity, bits.nbytes), "dtoh")
+        valid = np.unpackbits(bits, bitorder="little")[:n].astype(bool)
+    return vals, valid
+
+
+# ---- hash join (src/exec/gpu_join.rs:1583, src/exec/join.rs:169) --------------------
+
+class JoinTable:
+    """A built hash table (build side resident on the device)."""
+
+    def __init__(self, key: DeviceColumn, carry: DeviceColumn | None = None, stream: Stream | None = None):
+        self._h = C.c_void_p()
+        self.stream = stream
+        kc = key.c()
+        cc = carry.c() if carry is not None else None
+        st = N.lib().bolt_join_build(_stream(stream), C.byref(kc), C.byref(cc) if cc is not None else None,
+                                     C.byref(self._h))
+        N.check(st, "bolt_join_build")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def unique(self) -> bool:
+        u = C.c_int32()
+        N.check(N.lib().bolt_join_table_unique(self._h, C.byref(u)), "bolt_join_table_unique")
+        return bool(u.value)
+
+    def probe(self, probe_cols: list[DeviceColumn], key_col: int, terms: list[Term] | None = None,
+              capacity: int | None = None) -> tuple[np.ndarray, np.ndarray]:
+        n = probe_cols[key_col].length
+        cap = n if capacity is None else capacity
+        op_, ob_ = DeviceBuffer(max(cap, 1) * 4), DeviceBuffer(max(cap, 1) * 4)
+        ca = columns_array(probe_cols)
+        f, keep = _filter(terms, probe_cols)
+        n_out = C.c_int64()
+        st = N.lib().bolt_join_probe(_stream(self.stream), self._h, ca, len(probe_cols), key_col,
+                                     C.byref(f) if f else None, op_.ptr, ob_.ptr, cap, C.byref(n_out))
+        N.check(st, "bolt_join_probe")
+        k = n_out.value
+        return op_.to_numpy(np.uint32, k), ob_.to_numpy(np.uint32, k)
+
+    def probe_aggregate(self, probe_cols: list[DeviceColumn], key_col: int, aggs: list[tuple[int, int]],
+                        terms: list[Term] | None = None) -> list:
+        ca = columns_array(probe_cols)
+        f, keep = _filter(terms, probe_cols)
+        res = (N.BoltScalar * max(len(aggs), 1))()
+        st = N.lib().bolt_join_probe_aggregate(_stream(self.stream), self._h, ca, len(probe_cols), key_col,
+                                               C.byref(f) if f else None, _aggs(aggs), len(aggs), res)
+        N.check(st, "bolt_join_probe_aggregate")
+        return [scalar_value(res[i]) for i in range(len(aggs))]
+
+    def release(self) -> None:
+        if self._h:
+            N.lib().bolt_join_table_release(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+def execute_inner_join(left_key: DeviceColumn, right_key: DeviceColumn, capacity: int | None = None,
+                       stream: Stream | None = None) -> tuple[np.ndarray, np.ndarray]:
+    """(left_row, right_row) pairs of the INNER equi-join (unordered)."""
+    cap = capacity if capacity is not None else max(left_key.length, right_key.length)
+    while True:
+        ol, orr = DeviceBuffer(max(cap, 1) * 4), DeviceBuffer(max(cap, 1) * 4)
+        lk, rk = left_key.c(), right_key.c()
+        n_out = C.c_int64()
+        st = N.lib().bolt_inner_join(_stream(stream), C.byref(lk), C.byref(rk), ol.ptr, orr.ptr, cap,
+                                     C.byref(n_out))
+        if st == N.BOLT_DECLINE and capacity is None and n_out.value > cap:
+            cap = n_out.value
+            continue
+        N.check(st, "bolt_inner_join")
+        k = n_out.value
+        return ol.to_numpy(np.uint32, k), orr.to_numpy(np.uint32, k)
+
+
+# ---- sort (src/exec/sort.rs:161, src/exec/gpu_sort.rs:944) -----------------------------
+
+def sort_pairs_i64(keys: DeviceBuffer, vals: DeviceBuffer, n: int, descending: bool = False,
+                   stream: Stream | None = None) -> tuple[DeviceBuffer, DeviceBuffer]:
+    ko, vo = DeviceBuffer(max(n, 1) * 8), DeviceBuffer(max(n, 1) * 4)
+    st = N.lib().bolt_sort_pairs_i64(_stream(stream), keys.ptr, vals.ptr, n, 1 if descending else 0, ko.ptr, vo.ptr)
+    N.check(st, "bolt_sort_pairs_i64")
+    return ko, vo
+
+
+def execute_sort(keys: list[DeviceColumn], descending: list[bool] | None = None,
+                 nulls_first: list[bool] | None = None, stream: Stream | None = None) -> np.ndarray:
+    """Stable row permutation for ORDER BY keys (numpy u32)."""
+    k = len(keys)
+    desc = (C.c_int32 * k)(*[int(bool(x)) for x in (descending or [False] * k)])
+    nf = (C.c_int32 * k)(*[int(bool(x)) for x in (nulls_first or [True] * k)])
+    n = keys[0].length
+    out = DeviceBuffer(max(n, 1) * 4)
+    st = N.lib().bolt_sort_indices(_stream(stream), columns_array(keys), k, desc, nf, out.ptr)
+    N.check(st, "bolt_sort_indices")
+    return out.to_numpy(np.uint32, n)

@@ -1,0 +1,262 @@
+/*
+ * bolt_ops.h — operator C ABI of the MI355X execution core.
+ *
+ * Entry points replace the reference's executor functions for the GPU
+ * relational-operator path (the dispatch arms of Engine::execute_leaf_whole,
+ * src/exec/engine.rs:3898-4130).  Data crosses the boundary as device
+ * column descriptors (plain pointers + lengths + a dtype code), never as
+ * framework types.
+ *
+ * Conventions (kept from the reference, src/error.rs and
+ * src/exec/engine.rs:3915-3925):
+ *   - every function returns an int status; BOLT_OK = 0;
+ *   - BOLT_DECLINE is the GpuCapacity analogue: the operator declines the
+ *     input (e.g. a build side beyond the table cap) and the caller reruns it
+ *     on its host path;
+ *   - BOLT_ERR_TYPE carries type errors, including the integer-SUM overflow
+ *     error ("SUM(integer) overflow: accumulator exceeds i64 range",
+ *     src/exec/aggregate.rs:2120);
+ *   - the message of the last failure on the calling thread is returned by
+ *     bolt_last_error() (declared in bolt_hip.h);
+ *   - when no GPU is usable every entry point returns BOLT_ERR_NO_DEVICE
+ *     (100000), the stub contract of tests/cuda_stub_contract_test.rs:40-60.
+ *
+ * Unlike the reference (src/exec/launch.rs:353,409), entry points do not
+ * synchronise after each launch: work is queued on the caller's stream and
+ * only the calls that must return a host-visible count or scalar wait for
+ * that stream.  Device outputs are caller-allocated unless a function says it
+ * returns a library-owned table (freed with the matching *_release).
+ */
+#ifndef BOLT_OPS_H
+#define BOLT_OPS_H
+
+#include <stdint.h>
+
+#include "bolt_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (src/error.rs BoltError variants) ---- */
+#define BOLT_OK 0
+#define BOLT_ERR_PLAN 1        /* BoltError::Plan */
+#define BOLT_ERR_TYPE 2        /* BoltError::Type */
+#define BOLT_ERR_MEMORY 3      /* BoltError::Memory */
+#define BOLT_DECLINE 4         /* BoltError::GpuCapacity: rerun on the host */
+#define BOLT_ERR_UNSUPPORTED 5 /* BoltError::Unsupported */
+#define BOLT_ERR_OTHER 6       /* BoltError::Other */
+#define BOLT_ERR_DEVICE 7      /* BoltError::CudaWithCode: a HIP call failed */
+#define BOLT_ERR_NO_DEVICE 100000
+
+/* ---- column dtypes (src/plan/logical_plan.rs:57 DataType) ---- */
+enum bolt_dtype {
+  BOLT_BOOL = 1,   /* one byte per value */
+  BOLT_I32 = 2,
+  BOLT_I64 = 3,
+  BOLT_F32 = 4,
+  BOLT_F64 = 5,
+  BOLT_DEC128 = 6, /* 16-byte little-endian two's complement, scale in `scale` */
+  BOLT_DATE32 = 7  /* days since the epoch, int32 */
+};
+
+/* A device column: Arrow layout (values + optional LSB-first validity
+ * bitmap, NULL = no nulls).  `data` and `validity` are device pointers. */
+typedef struct bolt_column {
+  void* data;
+  const uint8_t* validity;
+  int64_t length;
+  int32_t dtype;
+  int32_t scale; /* Decimal128 scale; precision is not needed by any kernel */
+} bolt_column_t;
+
+/* ---- aggregates (src/plan/logical_plan.rs:2056 AggregateExpr) ---- */
+enum bolt_agg_op {
+  BOLT_AGG_SUM = 1,
+  BOLT_AGG_COUNT = 2,      /* COUNT(col): non-NULL values */
+  BOLT_AGG_COUNT_STAR = 3, /* COUNT(*): rows */
+  BOLT_AGG_MIN = 4,
+  BOLT_AGG_MAX = 5,
+  BOLT_AGG_AVG = 6,
+  BOLT_AGG_VAR_POP = 7,
+  BOLT_AGG_VAR_SAMP = 8,
+  BOLT_AGG_STDDEV_POP = 9,
+  BOLT_AGG_STDDEV_SAMP = 10
+};
+
+typedef struct bolt_agg {
+  int32_t op;
+  int32_t col; /* index into the operator's input columns; ignored for COUNT(*) */
+} bolt_agg_t;
+
+/* A scalar result (src/exec/aggregate.rs Scalar).  Output dtypes follow
+ * sum_output_dtype / AggregateExpr::output_dtype
+ * (src/plan/logical_plan.rs:2126,2191): SUM(Int32) -> Int64,
+ * SUM(Decimal128(p,s)) -> Decimal128(38,s), COUNT -> Int64,
+ * AVG/VAR/STDDEV -> Float64, MIN/MAX keep the input type. */
+typedef struct bolt_scalar {
+  int32_t dtype;
+  int32_t is_null; /* SQL NULL: SUM/MIN/MAX/AVG over no values */
+  int64_t i64;     /* integer / date result, decimal low word */
+  double f64;      /* float result (Float32 results are widened) */
+  int64_t hi;      /* decimal high word */
+} bolt_scalar_t;
+
+/* ---- predicates: the filter form of a KernelSpec (src/plan/physical_plan.rs:537)
+ *      restricted to an AND of column-vs-literal comparisons.  A comparison
+ *      with a NULL value is false (the row is dropped), as in the reference's
+ *      predicate kernel (src/jit/scan_kernel.rs:265). ---- */
+enum bolt_cmp_op {
+  BOLT_CMP_LT = 1, BOLT_CMP_LE = 2, BOLT_CMP_GT = 3, BOLT_CMP_GE = 4,
+  BOLT_CMP_EQ = 5, BOLT_CMP_NE = 6, BOLT_CMP_IS_NULL = 7, BOLT_CMP_IS_NOT_NULL = 8
+};
+
+typedef struct bolt_pred_term {
+  int32_t col;     /* index into the operator's input columns */
+  int32_t op;      /* bolt_cmp_op */
+  int64_t lit_i64; /* literal for integer / date columns; decimal low word */
+  double lit_f64;  /* literal for float columns */
+  int64_t lit_hi;  /* decimal high word */
+} bolt_pred_term_t;
+
+#define BOLT_MAX_PRED_TERMS 8
+
+typedef struct bolt_filter {
+  const bolt_pred_term_t* terms; /* host array */
+  int32_t n_terms;               /* 0 = keep every row */
+} bolt_filter_t;
+
+/* ------------------------------------------------------------------------
+ * Scalar aggregates — replaces execute_aggregate (src/exec/aggregate.rs:154)
+ * and agg_with_pre::try_execute_resident (src/exec/agg_with_pre.rs:157).
+ * Computes n_aggs aggregates over the rows of `inputs` that pass `filter`
+ * (NULL filter = all rows) in one pass, and writes `results` (host array of
+ * n_aggs).  Synchronises the stream.  Integer SUM overflow -> BOLT_ERR_TYPE.
+ * ------------------------------------------------------------------------ */
+int bolt_aggregate(bolt_stream_t stream, const bolt_column_t* inputs, int32_t n_inputs,
+                   const bolt_agg_t* aggs, int32_t n_aggs, const bolt_filter_t* filter,
+                   bolt_scalar_t* results);
+
+/* ------------------------------------------------------------------------
+ * Filter + stream compaction — replaces launch_predicate_kernel
+ * (src/exec/compact.rs:183) + compact_columns_on_gpu
+ * (src/exec/gpu_compact.rs:1005).  Keeps the rows passing `filter` AND
+ * (if `mask` is non-NULL) with mask[i] != 0, in input order.  `out` holds
+ * n_cols caller-allocated device columns of capacity n_rows (their `validity`
+ * must be a writable bitmap when the input column has one).  Writes the
+ * survivor count to *n_out (host) and synchronises the stream.
+ * Single pass: predicate, decoupled look-back scan and gather in one kernel.
+ * ------------------------------------------------------------------------ */
+int bolt_filter_compact(bolt_stream_t stream, const bolt_column_t* cols, int32_t n_cols,
+                        const bolt_filter_t* filter, const uint8_t* mask, bolt_column_t* out,
+                        int64_t* n_out);
+
+/* Exclusive prefix sum of a u8 mask into u32 offsets, plus the total
+ * (prefix_scan_mask, src/exec/gpu_compact.rs:415).  n < 2^32. */
+int bolt_prefix_scan_mask(bolt_stream_t stream, const uint8_t* mask, int64_t n, uint32_t* offsets,
+                          int64_t* total);
+
+/* ------------------------------------------------------------------------
+ * Hash GROUP BY — replaces execute_groupby (src/exec/groupby.rs:295) and its
+ * Tier-1 / Tier-2 / classic cascade.  One or two key columns (Int32 / Int64 /
+ * Float32 / Float64 / Date32; two only when both are 32-bit, the packing of
+ * src/exec/groupby_common.rs:352), any number of aggregates.  A single-column
+ * key with NULLs forms one NULL group (src/exec/groupby.rs:528-606).
+ * Rows that fail `filter` are skipped.  The result is a library-owned table:
+ * n_keys key columns then one column per aggregate, groups in ascending
+ * packed-key order (the reference's output order, groupby.rs:800) when
+ * flags has BOLT_GROUPBY_SORTED, otherwise in hash order.
+ * ------------------------------------------------------------------------ */
+#define BOLT_GROUPBY_SORTED 1u
+#define BOLT_GROUPBY_FORCE_GLOBAL 2u /* skip the LDS tier (tests / autotuner) */
+#define BOLT_GROUPBY_FORCE_PARTITIONED 4u /* force the partitioned HBM tier */
+
+typedef struct bolt_table bolt_table_t;
+
+int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int32_t n_keys,
+                 const bolt_column_t* inputs, int32_t n_inputs, const bolt_agg_t* aggs,
+                 int32_t n_aggs, const bolt_filter_t* filter, uint32_t flags, bolt_table_t** out);
+int bolt_table_num_rows(const bolt_table_t* t, int64_t* n_rows);
+int bolt_table_num_columns(const bolt_table_t* t, int32_t* n_cols);
+/* Column i of a result table (device pointers owned by the table). */
+int bolt_table_column(const bolt_table_t* t, int32_t i, bolt_column_t* col);
+void bolt_table_release(bolt_table_t* t);
+
+/* ------------------------------------------------------------------------
+ * Hash join — replaces hash_join_indices_on_gpu_resident
+ * (src/exec/gpu_join.rs:1583) with its build (src/jit/hash_join_kernel.rs:484)
+ * and probe (:711) kernels, and execute_inner_join (src/exec/join.rs:169).
+ * Keys are Int32 / Int64 / Date32; NULL keys never match.  The table keeps
+ * every build row (duplicate keys allowed).  `carry` (optional, 4-byte
+ * Int32 / Float32 / Date32 column of the build side) is stored beside each
+ * key so probe-side aggregates read it without a gather.
+ * A build side beyond the table cap returns BOLT_DECLINE (gpu_join.rs:378-393).
+ * ------------------------------------------------------------------------ */
+typedef struct bolt_join_table bolt_join_table_t;
+
+int bolt_join_build(bolt_stream_t stream, const bolt_column_t* build_key, const bolt_column_t* carry,
+                    bolt_join_table_t** out);
+/* 1 if every non-NULL build key is distinct.  Synchronises. */
+int bolt_join_table_unique(const bolt_join_table_t* t, int32_t* unique);
+void bolt_join_table_release(bolt_join_table_t* t);
+
+/* INNER probe emitting (probe_row, build_row) pairs for the probe rows that
+ * pass `filter` (its columns index `probe_cols`, whose column `key_col` is the
+ * probe key).  Pairs are written in an unspecified order; the row set equals
+ * the reference's (compare after canonical ordering).  If more than
+ * `capacity` pairs match, nothing beyond capacity is written, *n_out is the
+ * full count and BOLT_DECLINE is returned.  Synchronises. */
+int bolt_join_probe(bolt_stream_t stream, const bolt_join_table_t* t, const bolt_column_t* probe_cols,
+                    int32_t n_probe_cols, int32_t key_col, const bolt_filter_t* filter,
+                    uint32_t* out_probe_rows, uint32_t* out_build_rows, int64_t capacity,
+                    int64_t* n_out);
+
+/* Fused filter + INNER join + aggregates: aggregates over the joined rows
+ * without materialising them (the reference composes this as a join query
+ * plus a scalar-aggregate query, src/plan/physical_plan.rs:3691-3712).
+ * An aggregate whose `col` is BOLT_JOIN_CARRY reads the build side's carried
+ * column; otherwise it indexes `probe_cols`.  Synchronises. */
+#define BOLT_JOIN_CARRY (-2)
+int bolt_join_probe_aggregate(bolt_stream_t stream, const bolt_join_table_t* t,
+                              const bolt_column_t* probe_cols, int32_t n_probe_cols, int32_t key_col,
+                              const bolt_filter_t* filter, const bolt_agg_t* aggs, int32_t n_aggs,
+                              bolt_scalar_t* results);
+
+/* execute_inner_join (src/exec/join.rs:169): the smaller side builds, the
+ * other probes; pairs are returned as (left_row, right_row).  Same capacity
+ * contract as bolt_join_probe. */
+int bolt_inner_join(bolt_stream_t stream, const bolt_column_t* left_key, const bolt_column_t* right_key,
+                    uint32_t* out_left_rows, uint32_t* out_right_rows, int64_t capacity,
+                    int64_t* n_out);
+
+/* ------------------------------------------------------------------------
+ * Radix sort — replaces sort_indices_on_gpu_radix{,_multi}
+ * (src/exec/gpu_sort.rs:944,995) and run_radix_pipeline_i64 (:1493).
+ * Stable LSD radix sort, one up-front histogram pass for all digits, then per
+ * digit one onesweep pass (LDS ranking + decoupled look-back), no host round
+ * trip between passes.  n < 2^30.
+ * ------------------------------------------------------------------------ */
+
+/* Sort (key, value) pairs by int64 key (ascending unless `descending`).
+ * keys_out / vals_out receive the sorted pairs; inputs are not modified. */
+int bolt_sort_pairs_i64(bolt_stream_t stream, const int64_t* keys_in, const uint32_t* vals_in,
+                        int64_t n, int32_t descending, int64_t* keys_out, uint32_t* vals_out);
+
+/* ORDER BY: stable row permutation for a lexicographic multi-column key
+ * (execute_sort, src/exec/sort.rs:161; key transforms of
+ * src/exec/gpu_sort.rs:1070-1180: signed ints by sign flip, floats by the
+ * IEEE-monotonic flip with NaN canonicalised last and -0.0 == +0.0).
+ * descending / nulls_first are host arrays of n_keys flags. */
+int bolt_sort_indices(bolt_stream_t stream, const bolt_column_t* keys, int32_t n_keys,
+                      const int32_t* descending, const int32_t* nulls_first, uint32_t* out_rows);
+
+/* ------------------------------------------------------------------------
+ * Library information
+ * ------------------------------------------------------------------------ */
+/* Build identifier string (target arch, version). */
+const char* bolt_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BOLT_OPS_H */

@@ -1,0 +1,226 @@
+"""CPU oracle (test infrastructure): ctypes binding of oracle/bolt_oracle.c.
+
+Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py
+use this package, and only as the checker.  Each wrapper names the
+reference file:line its C function restates (see bolt_oracle.h).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "build" / "libbolt_oracle.so"
+
+BOOL, I32, I64, F32, F64, DEC128, DATE32 = 1, 2, 3, 4, 5, 6, 7
+SUM, COUNT, COUNT_STAR, MIN, MAX, AVG, VAR_POP, VAR_SAMP, STDDEV_POP, STDDEV_SAMP = range(1, 11)
+LT, LE, GT, GE, EQ, NE, IS_NULL, IS_NOT_NULL = range(1, 9)
+OK, ERR_PLAN, ERR_TYPE = 0, 1, 2
+
+
+class Col(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("validity", C.c_void_p), ("n", C.c_int64), ("dtype", C.c_int32),
+                ("scale", C.c_int32)]
+
+
+class Scalar(C.Structure):
+    _fields_ = [("status", C.c_int32), ("is_null", C.c_int32), ("dtype", C.c_int32), ("pad", C.c_int32),
+                ("i64", C.c_int64), ("f64", C.c_double), ("hi", C.c_int64)]
+
+
+class TermS(C.Structure):
+    _fields_ = [("col", C.c_int32), ("op", C.c_int32), ("lit_i64", C.c_int64), ("lit_f64", C.c_double),
+                ("lit_hi", C.c_int64)]
+
+
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        L.bo_rng_seed.restype = C.c_uint64
+        L.bo_rng_seed.argtypes = [C.c_uint64]
+        L.bo_rng_next.restype = C.c_uint64
+        L.bo_rng_next.argtypes = [C.POINTER(C.c_uint64)]
+        L.bo_rng_unit.restype = C.c_double
+        L.bo_rng_unit.argtypes = [C.POINTER(C.c_uint64)]
+        L.bo_rng_fill.argtypes = [C.c_uint64, C.c_void_p, C.c_int64]
+        L.bo_shuffle_u32.argtypes = [C.c_void_p, C.c_int64, C.c_uint64]
+        L.bo_neumaier_sum.restype = C.c_double
+        L.bo_neumaier_sum.argtypes = [C.c_void_p, C.c_int64]
+        L.bo_scalar_agg.restype = Scalar
+        L.bo_scalar_agg.argtypes = [C.POINTER(Col), C.c_int32]
+        L.bo_filter_rows.restype = C.c_int64
+        L.bo_filter_rows.argtypes = [C.POINTER(Col), C.c_int32, C.POINTER(TermS), C.c_int32, C.c_int64, C.c_void_p]
+        L.bo_groupby.restype = C.c_int32
+        L.bo_groupby.argtypes = [C.POINTER(Col), C.c_int32, C.POINTER(Col), C.POINTER(C.c_int32),
+                                 C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int64), C.c_void_p, C.c_void_p,
+                                 C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                 C.POINTER(C.c_void_p)]
+        L.bo_inner_join.restype = C.c_int64
+        L.bo_inner_join.argtypes = [C.POINTER(Col), C.POINTER(Col), C.c_void_p, C.c_void_p, C.c_int64,
+                                    C.POINTER(C.c_int64)]
+        L.bo_sort_indices.argtypes = [C.POINTER(Col), C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                      C.c_int64, C.c_void_p]
+        L.bo_filter_join_agg.restype = C.c_int32
+        L.bo_filter_join_agg.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64,
+                                         C.c_double, C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        _lib = L
+    return _lib
+
+
+# ---- host column helper ---------------------------------------------------------
+
+class HostCol:
+    """Keeps numpy buffers alive behind a `Col` descriptor."""
+
+    def __init__(self, values: np.ndarray, dtype: int, valid: np.ndarray | None = None, scale: int = 0):
+        self.values = np.ascontiguousarray(values)
+        self.bitmap = np.packbits(np.asarray(valid, bool), bitorder="little") if valid is not None else None
+        n = self.values.shape[0]
+        self.col = Col(self.values.ctypes.data, self.bitmap.ctypes.data if self.bitmap is not None else None, n,
+                       dtype, scale)
+
+
+def _cols(cols: list[HostCol]):
+    arr = (Col * max(len(cols), 1))()
+    for i, c in enumerate(cols):
+        arr[i] = c.col
+    return arr
+
+
+# ---- wrappers ---------------------------------------------------------------------
+
+def rng_draws(seed: int, n: int) -> np.ndarray:
+    """tests/common/mod.rs:46-90 Xorshift64Star, n successive next_u64()."""
+    out = np.empty(n, np.uint64)
+    lib().bo_rng_fill(seed, out.ctypes.data, n)
+    return out
+
+
+def rng_unit(seed: int, n: int) -> np.ndarray:
+    s = C.c_uint64(lib().bo_rng_seed(seed))
+    return np.array([lib().bo_rng_unit(C.byref(s)) for _ in range(n)])
+
+
+def shuffle_u32(xs: np.ndarray, seed: int) -> np.ndarray:
+    """tests/common/mod.rs:102-110."""
+    a = np.ascontiguousarray(xs, dtype=np.uint32).copy()
+    lib().bo_shuffle_u32(a.ctypes.data, a.shape[0], seed)
+    return a
+
+
+def neumaier_sum(x: np.ndarray) -> float:
+    """src/exec/aggregate.rs:2001."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return float(lib().bo_neumaier_sum(x.ctypes.data, x.shape[0]))
+
+
+def scalar_value(s: Scalar):
+    if s.status != OK:
+        raise ValueError(f"oracle status {s.status}")
+    if s.is_null:
+        return None
+    if s.dtype in (F32, F64):
+        return float(s.f64)
+    if s.dtype == DEC128:
+        return (int(s.hi) << 64) | (int(s.i64) & 0xFFFFFFFFFFFFFFFF)
+    return int(s.i64)
+
+
+def scalar_agg(col: HostCol, op: int) -> Scalar:
+    """src/exec/aggregate.rs:154,2032-2262 (raw struct: check .status)."""
+    return lib().bo_scalar_agg(C.byref(col.col), op)
+
+
+def filter_rows(cols: list[HostCol], terms: list[tuple], n: int) -> np.ndarray:
+    """src/exec/compact.rs:146; terms are (col, op, literal)."""
+    ta = (TermS * max(len(terms), 1))()
+    for i, (c, op, lit) in enumerate(terms):
+        t = TermS(c, op, 0, 0.0, 0)
+        if cols[c].col.dtype in (F32, F64):
+            t.lit_f64 = float(lit)
+        else:
+            t.lit_i64 = int(lit)
+        ta[i] = t
+    out = np.empty(max(n, 1), np.uint32)
+    k = lib().bo_filter_rows(_cols(cols), len(cols), ta, len(terms), n, out.ctypes.data)
+    return out[:k]
+
+
+def groupby(keys: list[HostCol], inputs: list[HostCol], aggs: list[tuple[int, int]]):
+    """src/exec/groupby.rs:295 classic semantics. Returns dict or raises on error status."""
+    n = keys[0].col.n
+    cap = max(n, 1)
+    na = len(aggs)
+    ops = (C.c_int32 * max(na, 1))(*[a[0] for a in aggs])
+    colsi = (C.c_int32 * max(na, 1))(*[a[1] for a in aggs])
+    out_keys = np.empty(cap, np.int64)
+    out_kn = np.empty(cap, np.uint8)
+    oi = [np.empty(cap, np.int64) for _ in range(na)]
+    of = [np.empty(cap, np.float64) for _ in range(na)]
+    oh = [np.empty(cap, np.int64) for _ in range(na)]
+    on = [np.empty(cap, np.uint8) for _ in range(na)]
+
+    def ptrs(lst):
+        arr = (C.c_void_p * max(na, 1))()
+        for i, a in enumerate(lst):
+            arr[i] = a.ctypes.data
+        return arr
+
+    ng = C.c_int64()
+    st = lib().bo_groupby(_cols(keys), len(keys), _cols(inputs), ops, colsi, na, C.byref(ng), out_keys.ctypes.data,
+                          out_kn.ctypes.data, ptrs(oi), ptrs(of), ptrs(oh), ptrs(on))
+    if st != OK:
+        return {"status": st}
+    g = ng.value
+    return {"status": OK, "n_groups": g, "keys": out_keys[:g], "key_null": out_kn[:g].astype(bool),
+            "i64": [a[:g] for a in oi], "f64": [a[:g] for a in of], "hi": [a[:g] for a in oh],
+            "null": [a[:g].astype(bool) for a in on]}
+
+
+def inner_join(left: HostCol, right: HostCol) -> tuple[np.ndarray, np.ndarray]:
+    """src/exec/join.rs:169 (pairs in the reference's emission order)."""
+    need = C.c_int64()
+    lib().bo_inner_join(C.byref(left.col), C.byref(right.col), None, None, 0, C.byref(need))
+    k = need.value
+    ol, orr = np.empty(max(k, 1), np.uint32), np.empty(max(k, 1), np.uint32)
+    lib().bo_inner_join(C.byref(left.col), C.byref(right.col), ol.ctypes.data, orr.ctypes.data, k, C.byref(need))
+    return ol[:k], orr[:k]
+
+
+def sort_indices(keys: list[HostCol], descending: list[bool], nulls_first: list[bool]) -> np.ndarray:
+    """src/exec/sort.rs:161 + gpu_sort.rs:1070-1180 transforms; stable."""
+    k = len(keys)
+    n = keys[0].col.n
+    out = np.empty(max(n, 1), np.uint32)
+    d = (C.c_int32 * k)(*[int(x) for x in descending])
+    nf = (C.c_int32 * k)(*[int(x) for x in nulls_first])
+    lib().bo_sort_indices(_cols(keys), k, d, nf, n, out.ctypes.data)
+    return out[:n]
+
+
+def filter_join_agg(build_key, build_payload, probe_key, probe_amount, amount_gt: float):
+    """filter (compact.rs:146) -> host join (join.rs:169) -> SUM/SUM/COUNT (aggregate.rs:154)."""
+    bk = np.ascontiguousarray(build_key, np.int64)
+    bp = np.ascontiguousarray(build_payload, np.int32)
+    pk = np.ascontiguousarray(probe_key, np.int64)
+    pa = np.ascontiguousarray(probe_amount, np.float64)
+    sp, sn, sc, cnt = C.c_int64(), C.c_double(), C.c_double(), C.c_int64()
+    st = lib().bo_filter_join_agg(bk.ctypes.data, bp.ctypes.data, bk.shape[0], pk.ctypes.data, pa.ctypes.data,
+                                  pk.shape[0], amount_gt, C.byref(sp), C.byref(sn), C.byref(sc), C.byref(cnt))
+    return {"status": st, "sum_payload": sp.value, "sum_amount_naive": sn.value, "sum_amount": sc.value,
+            "count": cnt.value}

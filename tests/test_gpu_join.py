@@ -1,0 +1,124 @@
+"""GPU parity: hash join build / probe / fused filter+join+aggregate.
+
+Join cardinalities and row sets are bit-exact against the oracle's host
+INNER join (compared after canonical ordering of the pairs); integer sums and
+counts exact; the float sum within REL_TOL of the oracle's Neumaier sum."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from craton_bolt_amd import _native as N
+from craton_bolt_amd import datagen as D
+from craton_bolt_amd.device import DeviceColumn
+from craton_bolt_amd.ops import JoinTable, Term, execute_inner_join
+
+from gpu_util import both, rel_close
+
+REL_TOL = 1e-9
+pytestmark = pytest.mark.gpu
+
+
+def _canon(a, b):
+    order = np.lexsort((b, a))
+    return np.stack([a[order], b[order]])
+
+
+def test_gpu_join_e2e_fixture(gpu):
+    # tests/gpu_join_e2e.rs:68-142: 4K unique build keys, probe i % 8192
+    bk = np.arange(4096, dtype=np.int32)
+    pk = (np.arange(8192) % 8192).astype(np.int32)
+    db, hb = both(bk, N.I32)
+    dp, hp = both(pk, N.I32)
+    l, r = execute_inner_join(db, dp)
+    assert len(l) == 4096
+    assert np.all(bk[l] == pk[r])
+    assert np.all(1000 + bk[l] == 1000 + pk[r])
+    ol, orr = O.inner_join(hb, hp)
+    assert np.array_equal(_canon(l, r), _canon(ol, orr))
+
+
+@pytest.mark.parametrize("n_left,n_right,card", [(1, 1, 1), (300, 500, 40), (20_000, 5_000, 3_000),
+                                                 (100_000, 1_000_000, 200_000), (0, 10, 5)])
+def test_inner_join_matches_oracle_with_duplicates_and_nulls(gpu, n_left, n_right, card):
+    print(f"[case] inner_join {n_left}x{n_right} card={card}", flush=True)
+    rng = np.random.default_rng(n_left + n_right)
+    lk = rng.integers(0, card, n_left).astype(np.int64) * 1_000_003 - 7
+    rk = rng.integers(0, card, n_right).astype(np.int64) * 1_000_003 - 7
+    lv = rng.random(n_left) > 0.05
+    rv = rng.random(n_right) > 0.05
+    dl, hl = both(lk, N.I64, lv)
+    dr, hr = both(rk, N.I64, rv)
+    l, r = execute_inner_join(dl, dr)
+    ol, orr = O.inner_join(hl, hr)
+    assert len(l) == len(ol)
+    assert np.array_equal(_canon(l, r), _canon(ol, orr))
+
+
+def test_join_keys_at_int64_extremes(gpu):
+    # no sentinel: i64::MIN and i64::MAX are ordinary keys
+    bk = np.array([np.iinfo(np.int64).min, np.iinfo(np.int64).max, 0, -1], np.int64)
+    pk = np.array([0, np.iinfo(np.int64).min, 5, np.iinfo(np.int64).max, np.iinfo(np.int64).min], np.int64)
+    db, hb = both(bk, N.I64)
+    dp, hp = both(pk, N.I64)
+    l, r = execute_inner_join(db, dp)
+    ol, orr = O.inner_join(hb, hp)
+    assert np.array_equal(_canon(l, r), _canon(ol, orr))
+    assert len(l) == 4
+
+
+def test_build_uniqueness_flag(gpu):
+    t = JoinTable(DeviceColumn.from_numpy(np.arange(100_000, dtype=np.int64)))
+    assert t.unique()
+    t = JoinTable(DeviceColumn.from_numpy(np.array([1, 2, 3, 2], np.int64)))
+    assert not t.unique()
+
+
+def test_probe_capacity_declines(gpu):
+    t = JoinTable(DeviceColumn.from_numpy(np.zeros(10, np.int64)))
+    with pytest.raises(N.GpuCapacity):
+        t.probe([DeviceColumn.from_numpy(np.zeros(10, np.int64))], 0, capacity=50)
+
+
+@pytest.mark.parametrize("n_build,n_probe", [(1, 1), (1000, 10_000), (100_000, 2_000_000), (1 << 20, 3_000_001)])
+def test_filter_join_aggregate_matches_oracle(gpu, n_build, n_probe):
+    """The star-query shape: WHERE amount > 10, INNER JOIN, SUM(payload), SUM(amount), COUNT(*)."""
+    print(f"[case] filter_join_agg build={n_build} probe={n_probe}", flush=True)
+    bk, bp = D.join_build(n_build)
+    fk, amt = D.join_probe(n_build, n_probe)
+    want = O.filter_join_agg(bk, bp, fk, amt, 10.0)
+    t = JoinTable(DeviceColumn.from_numpy(bk), carry=DeviceColumn.from_numpy(bp))
+    assert t.unique()
+    cols = [DeviceColumn.from_numpy(fk), DeviceColumn.from_numpy(amt)]
+    got = t.probe_aggregate(cols, 0, [(N.SUM, N.JOIN_CARRY), (N.SUM, 1), (N.COUNT_STAR, 0)],
+                            [Term(1, N.GT, 10.0)])
+    assert got[0] == want["sum_payload"]
+    assert got[2] == want["count"]
+    if want["count"]:
+        assert rel_close(got[1], want["sum_amount"], REL_TOL)
+    else:
+        assert got[1] is None
+    # the pair-emitting probe agrees on the row set
+    t2 = JoinTable(DeviceColumn.from_numpy(bk))
+    pr, br = t2.probe(cols, 0, [Term(1, N.GT, 10.0)])
+    assert len(pr) == want["count"]
+    assert np.all(bk[br] == fk[pr]) and np.all(amt[pr] > 10.0)
+    assert len(np.unique(pr)) == len(pr)
+
+
+def test_join_aggregate_with_duplicate_build_keys(gpu):
+    rng = np.random.default_rng(77)
+    bk = rng.integers(0, 500, 5000).astype(np.int64)
+    bp = rng.integers(0, 100, 5000).astype(np.int32)
+    pk = rng.integers(0, 700, 50_000).astype(np.int64)
+    amt = rng.uniform(0, 100, 50_000)
+    t = JoinTable(DeviceColumn.from_numpy(bk), carry=DeviceColumn.from_numpy(bp))
+    assert not t.unique()
+    got = t.probe_aggregate([DeviceColumn.from_numpy(pk), DeviceColumn.from_numpy(amt)], 0,
+                            [(N.SUM, N.JOIN_CARRY), (N.COUNT_STAR, 0), (N.MIN, N.JOIN_CARRY), (N.SUM, 1)])
+    ol, orr = O.inner_join(O.HostCol(bk, O.I64), O.HostCol(pk, O.I64))
+    assert got[0] == int(bp[ol].astype(np.int64).sum())
+    assert got[1] == len(ol)
+    assert got[2] == int(bp[ol].min())
+    assert rel_close(got[3], math.fsum(amt[orr]), REL_TOL)
