@@ -92,49 +92,54 @@ int make_agg_list(const bolt_agg_t* aggs, int32_t n_aggs, const bolt_column_t* c
 
 int acc_finalize(const Acc& a, int op, int dt, bolt_scalar_t* r) {
   memset(r, 0, sizeof *r);
+  const uint64_t n = acc_count(a);
+  const bool ovf = (a.n & kOvfBit) != 0;
   switch (op) {
     case BOLT_AGG_COUNT:
     case BOLT_AGG_COUNT_STAR:
       r->dtype = BOLT_I64;
-      r->i64 = (int64_t)a.n;
+      r->i64 = (int64_t)n;
       return BOLT_OK;
     case BOLT_AGG_SUM:
       r->dtype = dt == BOLT_I32 ? BOLT_I64 : dt;
-      if (a.n == 0) { r->is_null = 1; return BOLT_OK; }
+      if (n == 0) { r->is_null = 1; return BOLT_OK; }
       if (dt == BOLT_DEC128) {
-        if (a.ovf) return set_error(BOLT_ERR_TYPE, "SUM(Decimal128) overflow: accumulator exceeds i128 range");
-        r->i64 = (int64_t)a.lo;
-        r->hi = a.hi;
+        if (ovf) return set_error(BOLT_ERR_TYPE, "SUM(Decimal128) overflow: accumulator exceeds i128 range");
+        r->i64 = (int64_t)a.a;
+        r->hi = (int64_t)a.b;
       } else if (dtype_is_int(dt)) {
-        // exact 128-bit total must fit i64 (src/exec/aggregate.rs:2120)
-        if (a.ovf || a.hi != ((int64_t)a.lo >> 63))
+        // the exact 128-bit total must fit i64 (src/exec/aggregate.rs:2120)
+        if (ovf || (int64_t)a.b != ((int64_t)a.a >> 63))
           return set_error(BOLT_ERR_TYPE, "SUM(integer) overflow: accumulator exceeds i64 range");
-        r->i64 = (int64_t)a.lo;
+        r->i64 = (int64_t)a.a;
       } else {
-        // Neumaier result; non-finite terms fall back to the plain fold,
-        // which is the running leading part (src/exec/aggregate.rs:2022)
-        double v = (std::isfinite(a.s) && std::isfinite(a.c)) ? a.s + a.c : a.s;
+        // Neumaier result; non-finite terms fall back to the plain fold, which
+        // is the running leading part (src/exec/aggregate.rs:2022)
+        double s = as_f64(a.a), c = as_f64(a.b);
+        double v = (std::isfinite(s) && std::isfinite(c)) ? s + c : s;
         r->f64 = dt == BOLT_F32 ? (double)(float)v : v;
       }
       return BOLT_OK;
-    case BOLT_AGG_AVG:
+    case BOLT_AGG_AVG: {
       r->dtype = BOLT_F64;
-      if (a.n == 0) { r->is_null = 1; return BOLT_OK; }
-      r->f64 = ((std::isfinite(a.s) && std::isfinite(a.c)) ? a.s + a.c : a.s) / (double)a.n;
+      if (n == 0) { r->is_null = 1; return BOLT_OK; }
+      double s = as_f64(a.a), c = as_f64(a.b);
+      r->f64 = ((std::isfinite(s) && std::isfinite(c)) ? s + c : s) / (double)n;
       return BOLT_OK;
+    }
     case BOLT_AGG_MIN:
     case BOLT_AGG_MAX:
       r->dtype = dt;
-      if (a.n == 0) { r->is_null = 1; return BOLT_OK; }
-      if (dtype_is_float(dt)) r->f64 = f64_from_order_key(a.mm);
-      else r->i64 = (int64_t)(a.mm ^ 0x8000000000000000ull);
+      if (n == 0) { r->is_null = 1; return BOLT_OK; }
+      if (dtype_is_float(dt)) r->f64 = f64_from_order_key(a.a);
+      else r->i64 = (int64_t)(a.a ^ 0x8000000000000000ull);
       return BOLT_OK;
     default: {
       // Welford finalize (src/exec/welford.rs:104-134)
       r->dtype = BOLT_F64;
       bool samp = (op == BOLT_AGG_VAR_SAMP || op == BOLT_AGG_STDDEV_SAMP);
-      if (a.n == 0 || (samp && a.n <= 1)) { r->is_null = 1; return BOLT_OK; }
-      double var = a.c / (double)(samp ? a.n - 1 : a.n);
+      if (n == 0 || (samp && n <= 1)) { r->is_null = 1; return BOLT_OK; }
+      double var = as_f64(a.b) / (double)(samp ? n - 1 : n);
       r->f64 = (op == BOLT_AGG_STDDEV_POP || op == BOLT_AGG_STDDEV_SAMP) ? std::sqrt(var) : var;
       return BOLT_OK;
     }
@@ -155,43 +160,73 @@ int launch_finish(int na, const Acc* parts, int n_parts, const AggList& aggs, Ac
 
 // ---- scalar aggregate kernel -------------------------------------------------
 
-// Grid-stride pass: each thread folds its rows into NA register accumulators;
-// one partial per block.  Rows are visited in a fixed thread mapping so the
-// float results are reproducible for a given grid.
-template <int NA>
+// Grid-stride pass, four rows per thread per iteration with every load
+// issued before any use (indices are clamped to the last row so the loads
+// need no branch); each thread folds its rows into NA register accumulators
+// and each block writes one partial.  The row-to-thread mapping is fixed by
+// the grid, so float results are reproducible.
+constexpr int kAggUnroll = 4;
+
+template <int NT, int NA>
 __global__ __launch_bounds__(kReduceBlock) void scalar_agg_kernel(DevFilter f, AggList aggs, int64_t n,
                                                                  Acc* __restrict__ parts) {
   Acc acc[NA];
 #pragma unroll
   for (int a = 0; a < NA; a++) acc_init(acc[a], aggs.a[a].op);
-  int64_t stride = (int64_t)gridDim.x * kReduceBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kReduceBlock + threadIdx.x; i < n; i += stride) {
-    if (!eval_filter(f, i)) continue;
+  const int64_t step = (int64_t)gridDim.x * kReduceBlock * kAggUnroll;
+  for (int64_t base = (int64_t)blockIdx.x * kReduceBlock * kAggUnroll + threadIdx.x; base < n; base += step) {
+    bool keep[kAggUnroll];
+    Raw v[kAggUnroll][NA];
 #pragma unroll
-    for (int a = 0; a < NA; a++) {
-      const AggDesc& d = aggs.a[a];
-      if (d.op == BOLT_AGG_COUNT_STAR) { acc[a].n++; continue; }
-      if (!bit_valid(d.validity, i)) continue;
-      int64_t vi, vhi;
-      double vf;
-      load_value(d.data, d.dtype, i, vi, vhi, vf);
-      acc_update(acc[a], d.op, d.dtype, vi, vhi, vf);
+    for (int u = 0; u < kAggUnroll; u++) {
+      int64_t i = base + (int64_t)u * kReduceBlock;
+      int64_t ic = i < n ? i : n - 1;
+      keep[u] = i < n && eval_filter_nt<NT>(f, ic);
+#pragma unroll
+      for (int a = 0; a < NA; a++)
+        if (aggs.a[a].op != BOLT_AGG_COUNT_STAR) v[u][a] = load_raw(aggs.a[a].data, aggs.a[a].dtype, ic);
+    }
+#pragma unroll
+    for (int u = 0; u < kAggUnroll; u++) {
+      if (!keep[u]) continue;
+      int64_t i = base + (int64_t)u * kReduceBlock;
+#pragma unroll
+      for (int a = 0; a < NA; a++) {
+        const AggDesc& d = aggs.a[a];
+        if (d.op == BOLT_AGG_COUNT_STAR) { acc[a].n++; continue; }
+        if (!bit_valid(d.validity, i)) continue;
+        int64_t vi, vhi;
+        double vf;
+        raw_value(v[u][a], d.dtype, vi, vhi, vf);
+        acc_update(acc[a], d.op, d.dtype, vi, vhi, vf);
+      }
     }
   }
   block_reduce_store<NA>(acc, aggs, parts + (int64_t)blockIdx.x * NA);
 }
 
-static int launch_scalar_chunk(int na, const DevFilter& f, const AggList& chunk, int64_t n, unsigned grid,
-                               Acc* parts, hipStream_t s) {
+template <int NT>
+static int launch_scalar_nt(int na, const DevFilter& f, const AggList& chunk, int64_t n, unsigned grid, Acc* parts,
+                            hipStream_t s) {
   switch (na) {
-    case 1: scalar_agg_kernel<1><<<grid, kReduceBlock, 0, s>>>(f, chunk, n, parts); break;
-    case 2: scalar_agg_kernel<2><<<grid, kReduceBlock, 0, s>>>(f, chunk, n, parts); break;
-    case 3: scalar_agg_kernel<3><<<grid, kReduceBlock, 0, s>>>(f, chunk, n, parts); break;
-    case 4: scalar_agg_kernel<4><<<grid, kReduceBlock, 0, s>>>(f, chunk, n, parts); break;
+    case 1: scalar_agg_kernel<NT, 1><<<grid, kReduceBlock, 0, s>>>(f, chunk, n, parts); break;
+    case 2: scalar_agg_kernel<NT, 2><<<grid, kReduceBlock, 0, s>>>(f, chunk, n, parts); break;
+    case 3: scalar_agg_kernel<NT, 3><<<grid, kReduceBlock, 0, s>>>(f, chunk, n, parts); break;
+    case 4: scalar_agg_kernel<NT, 4><<<grid, kReduceBlock, 0, s>>>(f, chunk, n, parts); break;
     default: return set_error(BOLT_ERR_OTHER, "scalar aggregate: bad chunk size %d", na);
   }
   BOLT_HIP(hipGetLastError());
   return BOLT_OK;
+}
+
+static int launch_scalar_chunk(int na, const DevFilter& f, const AggList& chunk, int64_t n, unsigned grid,
+                               Acc* parts, hipStream_t s) {
+  switch (f.n) {
+    case 0: return launch_scalar_nt<0>(na, f, chunk, n, grid, parts, s);
+    case 1: return launch_scalar_nt<1>(na, f, chunk, n, grid, parts, s);
+    case 2: return launch_scalar_nt<2>(na, f, chunk, n, grid, parts, s);
+    default: return launch_scalar_nt<-1>(na, f, chunk, n, grid, parts, s);
+  }
 }
 
 }  // namespace bolt
@@ -217,7 +252,7 @@ extern "C" int bolt_aggregate(bolt_stream_t stream, const bolt_column_t* inputs,
   BOLT_TRY(make_agg_list(aggs, n_aggs, inputs, n_inputs, 0, &all));
   if (n_aggs == 0) return BOLT_OK;
 
-  unsigned grid = grid_for(n, kReduceBlock, 8, 2048);
+  unsigned grid = grid_for(n, kReduceBlock, kAggUnroll * 4, 256 * 8);
   int n_chunks = (n_aggs + 3) / 4;
   DevBuf<Acc> parts, outs;
   BOLT_TRY(parts.alloc((size_t)grid * 4 * n_chunks, s));

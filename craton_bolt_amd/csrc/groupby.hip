@@ -1,0 +1,1022 @@
+// groupby.hip — hash GROUP BY.
+//
+// Replaces execute_groupby (src/exec/groupby.rs:295) with its cascade of
+// Tier-1 LDS kernels (src/jit/shmem_*_kernel.rs, groupby_shmem_*_exec.rs),
+// Tier-2 partitioned kernels (src/jit/partition_*.rs, groupby_tier2_*.rs) and
+// the classic global-atomic path (src/jit/hash_kernels.rs:246,764,
+// src/jit/float_atomics.rs:139), which launch one kernel per aggregate and
+// round-trip to the host between passes.  Here every aggregate of the query
+// is updated by the same pass:
+//
+//   pass A  LDS tier.  Each workgroup keeps an open-addressing table of up to
+//           a few thousand groups in LDS (keys + one 64-bit word per
+//           accumulator field) and updates it with LDS atomics.  Rows whose
+//           key finds no LDS slot within a short probe are spilled (row ids
+//           appended with one atomic per wave).  At the end the workgroup
+//           appends its groups as partial records.  A HyperLogLog sketch of
+//           every partial key and spilled key estimates the group count.
+//   pass B  HBM tier.  A global open-addressing table sized from the sketch
+//           (retried 4x larger if it fills) absorbs the partial records and
+//           the spilled rows with global atomics.
+//   pass C  Extraction: occupied slots are compacted with decoupled look-back
+//           into the output columns, finalising each aggregate on the way.
+//
+// Accumulators are exact where the reference's are: integer SUM keeps a
+// 128-bit total per group through a carry-propagating pair of 64-bit
+// atomics (overflow of i64 is reported at finalize, like checked_group_sum,
+// src/exec/groupby.rs:1775); Decimal128 SUM keeps 192 bits; MIN / MAX of
+// every type use native unsigned 64-bit atomic min / max on an
+// order-preserving image (floats in the NaN-largest order of
+// src/jit/float_atomics.rs:27-62), so no CAS loop is needed.  Float SUM uses
+// hardware f64 atomic adds (order-dependent, like the reference's atomics).
+//
+// Keys (src/exec/groupby_common.rs:352): one Int32 / Int64 / Float32 /
+// Float64 / Date32 column, or two 32-bit columns packed (a << 32) | b; float
+// keys canonicalise -0.0 and NaN (groupby_common.rs:302).  A single-column
+// key with NULLs forms one NULL group (groupby.rs:528-606); two-column keys
+// drop rows with a NULL key, as the reference does.  There is no sentinel
+// restriction: a key equal to the empty marker and the NULL group live in two
+// dedicated slots after the table.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <vector>
+
+#include "ops_common.h"
+#include "scan_common.h"
+
+namespace bolt {
+
+int radix_sort_u64_pairs(hipStream_t s, const uint64_t* keys_in, const uint32_t* vals_in, int64_t n,
+                         uint64_t* keys_out, uint32_t* vals_out, int key_bits);
+
+constexpr uint64_t kEmpty = ~0ull;
+constexpr int kGbBlock = 1024;
+constexpr int kHllBits = 11;
+constexpr int kHllRegs = 1 << kHllBits;
+constexpr int kLdsProbes = 16;
+constexpr int kGlobalProbes = 1 << 16;
+
+enum Kind { K_ISUM = 1, K_DSUM, K_FSUM, K_AVG, K_COUNT, K_MINMAX, K_VAR };
+
+struct GAgg {
+  const void* data;
+  const uint8_t* validity;
+  int32_t op;
+  int32_t dtype;
+  int32_t kind;
+  int32_t w;   // first word
+  int32_t nw;  // count word (-1 = none)
+  int32_t pad;
+};
+struct GPlan {
+  GAgg a[kMaxAggs];
+  int32_t n;
+  int32_t words;
+};
+
+struct KeySpec {
+  const void* d0;
+  const uint8_t* v0;
+  const void* d1;
+  const uint8_t* v1;
+  int32_t t0, t1;
+  int32_t n_keys;
+  int32_t null_group;
+};
+
+// ---- keys ---------------------------------------------------------------------
+
+__device__ __forceinline__ uint64_t key_bits(const void* d, int dt, int64_t row) {
+  switch (dt) {
+    case BOLT_I32: case BOLT_DATE32: return (uint64_t)static_cast<const uint32_t*>(d)[row];
+    case BOLT_I64: return static_cast<const uint64_t*>(d)[row];
+    case BOLT_F32: {
+      uint32_t b = static_cast<const uint32_t*>(d)[row];
+      if ((b & 0x7FFFFFFFu) > 0x7F800000u) b = 0x7FC00000u;  // NaN -> canonical
+      else if ((b & 0x7FFFFFFFu) == 0) b = 0;                // -0.0 -> +0.0
+      return b;
+    }
+    case BOLT_F64: {
+      uint64_t b = static_cast<const uint64_t*>(d)[row];
+      if ((b & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull) b = 0x7FF8000000000000ull;
+      else if ((b & 0x7FFFFFFFFFFFFFFFull) == 0) b = 0;
+      return b;
+    }
+  }
+  return 0;
+}
+
+// 0 = drop the row, 1 = regular key in *key, 2 = the NULL group
+__device__ __forceinline__ int row_key(const KeySpec& ks, int64_t row, uint64_t* key) {
+  bool v0 = bit_valid(ks.v0, row);
+  if (ks.n_keys == 1) {
+    if (!v0) return ks.null_group ? 2 : 0;
+    *key = key_bits(ks.d0, ks.t0, row);
+    return 1;
+  }
+  if (!v0 || !bit_valid(ks.v1, row)) return 0;
+  *key = (key_bits(ks.d0, ks.t0, row) << 32) | key_bits(ks.d1, ks.t1, row);
+  return 1;
+}
+
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return x;
+}
+
+// HyperLogLog register update from an independent hash of the key.
+__device__ __forceinline__ void hll_add(uint32_t* regs, uint64_t key) {
+  uint64_t h = mix64(key ^ 0x9E3779B97F4A7C15ull);
+  uint32_t idx = (uint32_t)(h >> (64 - kHllBits));
+  uint64_t rest = (h << kHllBits) | (1ull << (kHllBits - 1));
+  uint32_t rho = (uint32_t)__clzll(rest) + 1;
+  atomicMax(&regs[idx], rho);
+}
+
+// ---- accumulator updates -------------------------------------------------------
+
+// Carry-propagating 128-bit add: the carry out of the low word is computed
+// from the value the atomic returned, so concurrent adds compose exactly.
+__device__ __forceinline__ void atomic_add128(unsigned long long* lo, unsigned long long* hi, uint64_t vlo,
+                                              uint64_t vhi) {
+  unsigned long long old = atomicAdd(lo, (unsigned long long)vlo);
+  uint64_t carry = (old + vlo) < old;
+  uint64_t add_hi = vhi + carry;
+  if (add_hi) atomicAdd(hi, (unsigned long long)add_hi);
+}
+
+// 192-bit add of a sign-extended 128-bit value (Decimal128 SUM): detects
+// overflow of the i128 result exactly at finalize.
+__device__ __forceinline__ void atomic_add192(unsigned long long* w, size_t stride, uint64_t vlo, uint64_t vhi,
+                                              uint64_t vtop) {
+  unsigned long long old = atomicAdd(&w[0], (unsigned long long)vlo);
+  uint64_t c1 = (old + vlo) < old;
+  uint64_t mid_add = vhi + c1;
+  uint64_t ca = (c1 && vhi == ~0ull) ? 1 : 0;  // carry out of vhi + c1
+  uint64_t c2 = 0;
+  if (mid_add) {
+    unsigned long long old2 = atomicAdd(&w[stride], (unsigned long long)mid_add);
+    c2 = (old2 + mid_add) < old2;
+  }
+  uint64_t top_add = vtop + ca + c2;
+  if (top_add) atomicAdd(&w[2 * stride], (unsigned long long)top_add);
+}
+
+// Fold row `row`'s inputs into slot `slot` of a word array with `stride`
+// words per field (LDS or global: the same atomics, address space inferred).
+__device__ __forceinline__ void update_slot(const GPlan& P, unsigned long long* words, size_t stride, uint32_t slot,
+                                            int64_t row) {
+  for (int a = 0; a < P.n; a++) {
+    const GAgg& g = P.a[a];
+    unsigned long long* w = words + (size_t)g.w * stride + slot;
+    if (g.op == BOLT_AGG_COUNT_STAR) { atomicAdd(w, 1ull); continue; }
+    if (!bit_valid(g.validity, row)) continue;
+    if (g.kind == K_COUNT) { atomicAdd(w, 1ull); continue; }
+    if (g.nw >= 0) atomicAdd(words + (size_t)g.nw * stride + slot, 1ull);
+    switch (g.kind) {
+      case K_ISUM: {
+        int64_t v = g.dtype == BOLT_I64 ? static_cast<const int64_t*>(g.data)[row]
+                                        : (int64_t) static_cast<const int32_t*>(g.data)[row];
+        atomic_add128(w, w + stride, (uint64_t)v, (uint64_t)(v >> 63));
+        break;
+      }
+      case K_DSUM: {
+        const uint64_t* p = static_cast<const uint64_t*>(g.data) + 2 * row;
+        uint64_t lo = p[0], hi = p[1];
+        atomic_add192(w, stride, lo, hi, (uint64_t)((int64_t)hi >> 63));
+        break;
+      }
+      case K_FSUM:
+      case K_AVG:
+      case K_VAR: {
+        double v = g.dtype == BOLT_F64 ? static_cast<const double*>(g.data)[row]
+                 : g.dtype == BOLT_F32 ? (double)static_cast<const float*>(g.data)[row]
+                 : g.dtype == BOLT_I64 ? (double)static_cast<const int64_t*>(g.data)[row]
+                                       : (double)static_cast<const int32_t*>(g.data)[row];
+        unsafeAtomicAdd(reinterpret_cast<double*>(w), v);
+        break;
+      }
+      case K_MINMAX: {
+        uint64_t k;
+        switch (g.dtype) {
+          case BOLT_F64: k = f64_order_key(static_cast<const double*>(g.data)[row]); break;
+          case BOLT_F32: k = f64_order_key((double)static_cast<const float*>(g.data)[row]); break;
+          case BOLT_I64: k = (uint64_t) static_cast<const int64_t*>(g.data)[row] ^ 0x8000000000000000ull; break;
+          default: k = (uint64_t)(int64_t) static_cast<const int32_t*>(g.data)[row] ^ 0x8000000000000000ull; break;
+        }
+        if (g.op == BOLT_AGG_MIN) atomicMin(w, (unsigned long long)k);
+        else atomicMax(w, (unsigned long long)k);
+        break;
+      }
+    }
+  }
+}
+
+// Merge the words of a partial record (one value per field, `pstride` apart)
+// into a slot of the global table.
+__device__ __forceinline__ void merge_slot(const GPlan& P, unsigned long long* words, size_t stride, uint32_t slot,
+                                           const unsigned long long* pw, size_t pstride) {
+  for (int a = 0; a < P.n; a++) {
+    const GAgg& g = P.a[a];
+    unsigned long long* w = words + (size_t)g.w * stride + slot;
+    const unsigned long long* q = pw + (size_t)g.w * pstride;
+    if (g.nw >= 0 && g.nw != g.w) {
+      unsigned long long c = pw[(size_t)g.nw * pstride];
+      if (c) atomicAdd(words + (size_t)g.nw * stride + slot, c);
+    }
+    switch (g.kind) {
+      case K_COUNT: if (q[0]) atomicAdd(w, q[0]); break;
+      case K_ISUM: atomic_add128(w, w + stride, q[0], q[pstride]); break;
+      case K_DSUM: atomic_add192(w, stride, q[0], q[pstride], q[2 * pstride]); break;
+      case K_FSUM: case K_AVG: case K_VAR: unsafeAtomicAdd(reinterpret_cast<double*>(w), as_f64(q[0])); break;
+      case K_MINMAX:
+        if (g.op == BOLT_AGG_MIN) atomicMin(w, q[0]);
+        else atomicMax(w, q[0]);
+        break;
+    }
+  }
+}
+
+__device__ __forceinline__ unsigned long long word_identity(const GPlan& P, int w) {
+  for (int a = 0; a < P.n; a++)
+    if (P.a[a].kind == K_MINMAX && P.a[a].w == w) return P.a[a].op == BOLT_AGG_MIN ? ~0ull : 0ull;
+  return 0ull;
+}
+
+// Global table insert: returns the slot of `key` (claiming one if absent), or
+// 0xFFFFFFFF when the probe bound is exceeded (the table is too small).
+__device__ __forceinline__ uint32_t global_insert(unsigned long long* keys, uint64_t cap_mask, uint64_t key) {
+  uint64_t s = mix64(key) & cap_mask;
+  for (int p = 0; p < kGlobalProbes; p++) {
+    unsigned long long cur = keys[s];
+    if (cur == key) return (uint32_t)s;
+    if (cur == kEmpty) {
+      unsigned long long prev = atomicCAS(&keys[s], kEmpty, (unsigned long long)key);
+      if (prev == kEmpty || prev == key) return (uint32_t)s;
+    }
+    s = (s + 1) & cap_mask;
+  }
+  return 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ uint32_t global_find(const unsigned long long* keys, uint64_t cap_mask, uint64_t key) {
+  uint64_t s = mix64(key) & cap_mask;
+  for (int p = 0; p < kGlobalProbes; p++) {
+    unsigned long long cur = keys[s];
+    if (cur == key) return (uint32_t)s;
+    if (cur == kEmpty) return 0xFFFFFFFFu;
+    s = (s + 1) & cap_mask;
+  }
+  return 0xFFFFFFFFu;
+}
+
+// Wave-aggregated append: returns this lane's position for lanes with `want`.
+__device__ __forceinline__ unsigned long long wave_append(bool want, unsigned long long* counter) {
+  uint64_t m = __ballot(want);
+  if (m == 0) return 0;
+  int lane = threadIdx.x & 63;
+  int leader = __builtin_ctzll(m);
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+  base = __shfl(base, leader);
+  uint64_t below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
+  return base + (unsigned long long)__popcll(below);
+}
+
+// ---- pass A: LDS tier ------------------------------------------------------------
+
+struct PartOut {
+  unsigned long long* keys;  // [cap]
+  uint8_t* tag;              // [cap] 0 regular, 1 NULL group, 2 empty-marker key
+  unsigned long long* words; // [words][cap]
+  int64_t cap;
+  unsigned long long* count;
+  uint32_t* spill;           // [n] row ids
+  unsigned long long* spill_count;
+  uint32_t* hll;             // [kHllRegs]
+};
+
+__global__ __launch_bounds__(kGbBlock) void gb_lds_kernel(KeySpec ks, DevFilter f, GPlan P, int64_t n, int S,
+                                                         PartOut out) {
+  extern __shared__ unsigned long long sm[];
+  const int SS = S + 2;  // + NULL group + empty-marker key
+  unsigned long long* skeys = sm;
+  unsigned long long* swords = sm + SS;
+  uint32_t* shll = reinterpret_cast<uint32_t*>(swords + (size_t)P.words * SS);
+  uint32_t* sused = shll + kHllRegs;  // [2]
+  for (int i = threadIdx.x; i < SS; i += kGbBlock) skeys[i] = kEmpty;
+  for (int w = 0; w < P.words; w++) {
+    unsigned long long id = word_identity(P, w);
+    for (int i = threadIdx.x; i < SS; i += kGbBlock) swords[(size_t)w * SS + i] = id;
+  }
+  for (int i = threadIdx.x; i < kHllRegs; i += kGbBlock) shll[i] = 0;
+  if (threadIdx.x < 2) sused[threadIdx.x] = 0;
+  __syncthreads();
+
+  const int64_t stride = (int64_t)gridDim.x * kGbBlock;
+  const int64_t end = ((n + kGbBlock - 1) / kGbBlock) * kGbBlock;  // wave-uniform trip count
+  for (int64_t row = (int64_t)blockIdx.x * kGbBlock + threadIdx.x; row < end; row += stride) {
+    bool spill = false;
+    uint64_t key = 0;
+    if (row < n && eval_filter(f, row)) {
+      int kr = row_key(ks, row, &key);
+      int slot = -1;
+      if (kr == 2) {
+        slot = S;
+        sused[0] = 1;
+      } else if (kr == 1) {
+        if (key == kEmpty) {
+          slot = S + 1;
+          sused[1] = 1;
+        } else {
+          uint32_t s = (uint32_t)(((mix64(key) >> 32) * (uint64_t)S) >> 32);
+          for (int p = 0; p < kLdsProbes; p++) {
+            unsigned long long cur = skeys[s];
+            if (cur == key) { slot = (int)s; break; }
+            if (cur == kEmpty) {
+              unsigned long long prev = atomicCAS(&skeys[s], kEmpty, (unsigned long long)key);
+              if (prev == kEmpty || prev == key) { slot = (int)s; break; }
+            }
+            s = (s + 1 == (uint32_t)S) ? 0 : s + 1;
+          }
+          if (slot < 0) spill = true;
+        }
+      }
+      if (slot >= 0) update_slot(P, swords, SS, (uint32_t)slot, row);
+    }
+    unsigned long long pos = wave_append(spill, out.spill_count);
+    if (spill) {
+      out.spill[pos] = (uint32_t)row;
+      hll_add(shll, key);
+    }
+  }
+  __syncthreads();
+  // flush groups as partial records
+  for (int base = 0; base < SS; base += kGbBlock) {
+    int i = base + threadIdx.x;
+    bool used = false;
+    uint8_t tag = 0;
+    if (i < S) used = skeys[i] != kEmpty;
+    else if (i < SS) { used = sused[i - S] != 0; tag = (uint8_t)(i - S + 1); }
+    unsigned long long pos = wave_append(used, out.count);
+    if (used) {
+      out.keys[pos] = tag == 2 ? kEmpty : (tag == 1 ? 0ull : skeys[i]);
+      out.tag[pos] = tag;
+      for (int w = 0; w < P.words; w++) out.words[(size_t)w * out.cap + pos] = swords[(size_t)w * SS + i];
+      if (tag == 0) hll_add(shll, skeys[i]);
+    }
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < kHllRegs; r += kGbBlock)
+    if (shll[r]) atomicMax(&out.hll[r], shll[r]);
+}
+
+// ---- pass B: global table ------------------------------------------------------------
+
+struct GTable {
+  unsigned long long* keys;   // [cap]
+  unsigned long long* words;  // [words][cap + 2]
+  uint32_t* used_special;     // [2]
+  uint32_t* err;              // [0] probe overflow, [1] lookback, [2] integer overflow
+  uint64_t cap;               // power of two
+};
+
+__global__ void gb_init_kernel(GTable t, GPlan P) {
+  const size_t total = t.cap + 2;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+    if (i < t.cap) t.keys[i] = kEmpty;
+    for (int w = 0; w < P.words; w++) t.words[(size_t)w * total + i] = word_identity(P, w);
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 2) t.used_special[threadIdx.x] = 0;
+}
+
+__device__ __forceinline__ int64_t gslot(const GTable& t, int kind_tag, uint64_t key) {
+  if (kind_tag == 1) { t.used_special[0] = 1; return (int64_t)t.cap; }
+  if (kind_tag == 2 || key == kEmpty) { t.used_special[1] = 1; return (int64_t)t.cap + 1; }
+  uint32_t s = global_insert(t.keys, t.cap - 1, key);
+  if (s == 0xFFFFFFFFu) { atomicOr(&t.err[0], 1u); return -1; }
+  return s;
+}
+
+__global__ __launch_bounds__(256) void gb_merge_parts_kernel(GTable t, GPlan P, const unsigned long long* pkeys,
+                                                            const uint8_t* ptag, const unsigned long long* pwords,
+                                                            int64_t pcap, int64_t np) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < np; i += stride) {
+    int64_t s = gslot(t, ptag[i], pkeys[i]);
+    if (s < 0) continue;
+    merge_slot(P, t.words, t.cap + 2, (uint32_t)s, pwords + i, (size_t)pcap);
+  }
+}
+
+// Rows processed straight against the global table: the spilled rows
+// (rows != nullptr) or every row of the input (rows == nullptr).
+__global__ __launch_bounds__(256) void gb_rows_kernel(GTable t, GPlan P, KeySpec ks, DevFilter f,
+                                                     const uint32_t* rows, int64_t nrows) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nrows; i += stride) {
+    int64_t row = rows ? (int64_t)rows[i] : i;
+    if (!rows && !eval_filter(f, row)) continue;
+    uint64_t key = 0;
+    int kr = row_key(ks, row, &key);
+    if (kr == 0) continue;
+    int64_t s = gslot(t, kr == 2 ? 1 : 0, key);
+    if (s < 0) continue;
+    update_slot(P, t.words, t.cap + 2, (uint32_t)s, row);
+  }
+}
+
+// Second pass for VAR / STDDEV: M2 = sum over the group of (x - mean)^2, with
+// the mean taken from the finished first pass (two-pass variance).
+__global__ __launch_bounds__(256) void gb_m2_kernel(GTable t, GPlan P, KeySpec ks, DevFilter f, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const size_t total = t.cap + 2;
+  for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < n; row += stride) {
+    if (!eval_filter(f, row)) continue;
+    uint64_t key = 0;
+    int kr = row_key(ks, row, &key);
+    if (kr == 0) continue;
+    int64_t s;
+    if (kr == 2) s = (int64_t)t.cap;
+    else if (key == kEmpty) s = (int64_t)t.cap + 1;
+    else {
+      uint32_t q = global_find(t.keys, t.cap - 1, key);
+      if (q == 0xFFFFFFFFu) continue;
+      s = q;
+    }
+    for (int a = 0; a < P.n; a++) {
+      const GAgg& g = P.a[a];
+      if (g.kind != K_VAR || !bit_valid(g.validity, row)) continue;
+      double v = g.dtype == BOLT_F64 ? static_cast<const double*>(g.data)[row]
+               : g.dtype == BOLT_F32 ? (double)static_cast<const float*>(g.data)[row]
+               : g.dtype == BOLT_I64 ? (double)static_cast<const int64_t*>(g.data)[row]
+                                     : (double)static_cast<const int32_t*>(g.data)[row];
+      double sum = as_f64(t.words[(size_t)g.w * total + s]);
+      double cnt = (double)t.words[(size_t)g.nw * total + s];
+      double d = v - sum / cnt;
+      unsafeAtomicAdd(reinterpret_cast<double*>(&t.words[(size_t)(g.w + 1) * total + s]), d * d);
+    }
+  }
+}
+
+// ---- pass C: extraction ---------------------------------------------------------------
+
+struct OutCol {
+  uint8_t* data;
+  uint32_t* valid;  // zeroed bitmap words or nullptr
+  int32_t dtype;
+  int32_t pad;
+};
+struct OutSpec {
+  OutCol key[2];
+  OutCol agg[kMaxAggs];
+  unsigned long long* packed;  // packed key per group (for ordering), i64 order; NULL group = i64::MAX
+  uint32_t* slot_of;           // slot per output row
+};
+
+__device__ __forceinline__ void set_valid(uint32_t* bm, uint64_t pos) { atomicOr(&bm[pos >> 5], 1u << (pos & 31)); }
+
+__device__ __forceinline__ void write_key(const OutCol& c, uint64_t pos, uint64_t bits) {
+  if (c.dtype == BOLT_I64 || c.dtype == BOLT_F64) reinterpret_cast<uint64_t*>(c.data)[pos] = bits;
+  else reinterpret_cast<uint32_t*>(c.data)[pos] = (uint32_t)bits;
+}
+
+__device__ __forceinline__ void finalize_to(const GTable& t, const GAgg& g, const OutCol& c, size_t total, uint64_t s,
+                                            uint64_t pos) {
+  auto W = [&](int w) { return t.words[(size_t)w * total + s]; };
+  uint64_t n = g.nw >= 0 ? W(g.nw) : 1;
+  bool valid = true;
+  switch (g.kind) {
+    case K_COUNT: reinterpret_cast<int64_t*>(c.data)[pos] = (int64_t)W(g.w); break;
+    case K_ISUM: {
+      uint64_t lo = W(g.w);
+      int64_t hi = (int64_t)W(g.w + 1);
+      if (hi != ((int64_t)lo >> 63)) atomicOr(&t.err[2], 1u);
+      reinterpret_cast<int64_t*>(c.data)[pos] = (int64_t)lo;
+      valid = n > 0;
+      break;
+    }
+    case K_DSUM: {
+      uint64_t lo = W(g.w), mid = W(g.w + 1);
+      int64_t top = (int64_t)W(g.w + 2);
+      if (top != ((int64_t)mid >> 63)) atomicOr(&t.err[2], 2u);
+      reinterpret_cast<uint64_t*>(c.data)[2 * pos] = lo;
+      reinterpret_cast<uint64_t*>(c.data)[2 * pos + 1] = mid;
+      valid = n > 0;
+      break;
+    }
+    case K_FSUM: {
+      double v = as_f64(W(g.w));
+      if (g.dtype == BOLT_F32) reinterpret_cast<float*>(c.data)[pos] = (float)v;
+      else reinterpret_cast<double*>(c.data)[pos] = v;
+      valid = n > 0;
+      break;
+    }
+    case K_AVG: {
+      double v = n ? as_f64(W(g.w)) / (double)n : 0.0;
+      reinterpret_cast<double*>(c.data)[pos] = v;
+      valid = n > 0;
+      break;
+    }
+    case K_MINMAX: {
+      uint64_t k = W(g.w);
+      if (g.dtype == BOLT_F64) reinterpret_cast<double*>(c.data)[pos] = f64_from_order_key(k);
+      else if (g.dtype == BOLT_F32) reinterpret_cast<float*>(c.data)[pos] = (float)f64_from_order_key(k);
+      else if (g.dtype == BOLT_I64) reinterpret_cast<int64_t*>(c.data)[pos] = (int64_t)(k ^ 0x8000000000000000ull);
+      else reinterpret_cast<int32_t*>(c.data)[pos] = (int32_t)(int64_t)(k ^ 0x8000000000000000ull);
+      valid = n > 0;
+      break;
+    }
+    case K_VAR: {
+      bool samp = g.op == BOLT_AGG_VAR_SAMP || g.op == BOLT_AGG_STDDEV_SAMP;
+      double m2 = as_f64(W(g.w + 1));
+      valid = samp ? n > 1 : n > 0;
+      double var = valid ? m2 / (double)(samp ? n - 1 : n) : 0.0;
+      if (g.op == BOLT_AGG_STDDEV_POP || g.op == BOLT_AGG_STDDEV_SAMP) var = sqrt(var);
+      reinterpret_cast<double*>(c.data)[pos] = var;
+      break;
+    }
+  }
+  if (c.valid && valid) set_valid(c.valid, pos);
+}
+
+constexpr int kExBlock = 256;
+constexpr int kExItems = 16;
+
+__global__ __launch_bounds__(kExBlock) void gb_extract_kernel(GTable t, GPlan P, KeySpec ks, OutSpec o,
+                                                             uint64_t* __restrict__ status,
+                                                             uint32_t* __restrict__ ticket,
+                                                             unsigned long long* __restrict__ total_out) {
+  __shared__ uint32_t sh_tile;
+  __shared__ uint32_t sh_cnt[kExItems * 4];
+  __shared__ uint64_t sh_prefix;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) sh_tile = atomicAdd(ticket, 1u);
+  __syncthreads();
+  const int64_t tile = sh_tile;
+  const uint64_t total = t.cap + 2;
+  const uint64_t base = (uint64_t)tile * kExBlock * kExItems;
+  uint32_t keep = 0;
+#pragma unroll
+  for (int j = 0; j < kExItems; j++) {
+    uint64_t s = base + (uint64_t)j * kExBlock + threadIdx.x;
+    bool k = false;
+    if (s < t.cap) k = t.keys[s] != kEmpty;
+    else if (s < total) k = t.used_special[s - t.cap] != 0;
+    keep |= (uint32_t)k << j;
+    uint64_t b = __ballot(k);
+    if (lane == 0) sh_cnt[j * 4 + wave] = (uint32_t)__popcll(b);
+  }
+  __syncthreads();
+  if (wave == 0) {
+    uint32_t c = sh_cnt[lane];
+    uint32_t incl = c;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      uint32_t v = __shfl_up(incl, d);
+      if (lane >= d) incl += v;
+    }
+    uint32_t tile_total = __shfl(incl, 63);
+    sh_cnt[lane] = incl - c;
+    uint64_t excl = lookback_wave(status, tile, tile_total, &t.err[1]);
+    if (lane == 0) {
+      sh_prefix = excl;
+      if (tile == (int64_t)gridDim.x - 1) *total_out = excl + tile_total;
+    }
+  }
+  __syncthreads();
+  const uint64_t prefix = sh_prefix;
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  for (int j = 0; j < kExItems; j++) {
+    bool k = (keep >> j) & 1u;
+    uint64_t b = __ballot(k);
+    if (!k) continue;
+    uint64_t s = base + (uint64_t)j * kExBlock + threadIdx.x;
+    uint64_t pos = prefix + sh_cnt[j * 4 + wave] + (uint64_t)__popcll(b & below);
+    bool null_group = s == t.cap;
+    uint64_t packed = s < t.cap ? t.keys[s] : (null_group ? 0x7FFFFFFFFFFFFFFFull : kEmpty);
+    o.packed[pos] = packed;
+    o.slot_of[pos] = (uint32_t)s;
+    if (ks.n_keys == 1) {
+      write_key(o.key[0], pos, null_group ? 0 : packed);
+      if (o.key[0].valid && !null_group) set_valid(o.key[0].valid, pos);
+    } else {
+      write_key(o.key[0], pos, packed >> 32);
+      write_key(o.key[1], pos, packed & 0xFFFFFFFFull);
+    }
+    for (int a = 0; a < P.n; a++) finalize_to(t, P.a[a], o.agg[a], total, s, pos);
+  }
+}
+
+// Gather for the sorted output order: out[i] = in[perm[i]].
+__global__ void gather_kernel(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, int width,
+                              const uint32_t* __restrict__ perm, int64_t n, const uint32_t* __restrict__ vsrc,
+                              uint32_t* __restrict__ vdst) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    int64_t p = perm[i];
+    switch (width) {
+      case 4: reinterpret_cast<uint32_t*>(dst)[i] = reinterpret_cast<const uint32_t*>(src)[p]; break;
+      case 8: reinterpret_cast<uint64_t*>(dst)[i] = reinterpret_cast<const uint64_t*>(src)[p]; break;
+      case 16: reinterpret_cast<uint4*>(dst)[i] = reinterpret_cast<const uint4*>(src)[p]; break;
+    }
+    if (vsrc && ((vsrc[p >> 5] >> (p & 31)) & 1u)) atomicOr(&vdst[i >> 5], 1u << (i & 31));
+  }
+}
+
+// i64 order of the packed key as an unsigned radix key
+__global__ void flip_sign_kernel(const unsigned long long* __restrict__ in, uint64_t* __restrict__ out,
+                                 uint32_t* __restrict__ idx, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    out[i] = in[i] ^ 0x8000000000000000ull;
+    idx[i] = (uint32_t)i;
+  }
+}
+
+}  // namespace bolt
+
+using namespace bolt;
+
+// ---- result tables ------------------------------------------------------------------------
+
+struct bolt_table {
+  std::vector<bolt_column_t> cols;
+  std::vector<void*> owned;
+  int64_t n_rows = 0;
+  hipStream_t stream = nullptr;
+};
+
+extern "C" int bolt_table_num_rows(const bolt_table_t* t, int64_t* n) {
+  if (!t) return set_error(BOLT_ERR_PLAN, "table is NULL");
+  *n = t->n_rows;
+  return BOLT_OK;
+}
+extern "C" int bolt_table_num_columns(const bolt_table_t* t, int32_t* n) {
+  if (!t) return set_error(BOLT_ERR_PLAN, "table is NULL");
+  *n = (int32_t)t->cols.size();
+  return BOLT_OK;
+}
+extern "C" int bolt_table_column(const bolt_table_t* t, int32_t i, bolt_column_t* col) {
+  if (!t || i < 0 || i >= (int32_t)t->cols.size()) return set_error(BOLT_ERR_PLAN, "column index out of range");
+  *col = t->cols[i];
+  return BOLT_OK;
+}
+extern "C" void bolt_table_release(bolt_table_t* t) {
+  if (!t) return;
+  for (void* p : t->owned) dev_free(p, t->stream);
+  delete t;
+}
+
+static int table_alloc_col(bolt_table_t* t, int32_t dtype, int64_t cap, bool with_valid, hipStream_t s,
+                           bolt_column_t* out) {
+  void* d = nullptr;
+  BOLT_TRY(dev_alloc(&d, (size_t)(cap > 0 ? cap : 1) * dtype_size(dtype), s));
+  t->owned.push_back(d);
+  void* v = nullptr;
+  if (with_valid) {
+    size_t words = (size_t)(cap + 31) / 32 + 1;
+    BOLT_TRY(dev_alloc(&v, words * 4, s));
+    t->owned.push_back(v);
+    BOLT_TRY(fill_u32(v, 0, words, s));
+  }
+  out->data = d;
+  out->validity = static_cast<const uint8_t*>(v);
+  out->length = 0;
+  out->dtype = dtype;
+  out->scale = 0;
+  return BOLT_OK;
+}
+
+// ---- plan construction ---------------------------------------------------------------------
+
+static int make_gplan(const bolt_agg_t* aggs, int32_t n_aggs, const bolt_column_t* inputs, int32_t n_inputs,
+                      int64_t n, GPlan* P, int32_t* out_dtype) {
+  memset(P, 0, sizeof *P);
+  if (n_aggs < 0 || n_aggs > kMaxAggs) return set_error(BOLT_ERR_PLAN, "%d aggregates (max %d)", n_aggs, kMaxAggs);
+  int w = 0;
+  for (int i = 0; i < n_aggs; i++) {
+    GAgg& g = P->a[i];
+    g.op = aggs[i].op;
+    g.nw = -1;
+    if (g.op < BOLT_AGG_SUM || g.op > BOLT_AGG_STDDEV_SAMP) return set_error(BOLT_ERR_PLAN, "aggregate %d: unknown op", i);
+    if (g.op == BOLT_AGG_COUNT_STAR) {
+      g.kind = K_COUNT;
+      g.dtype = BOLT_I64;
+      g.w = w++;
+      out_dtype[i] = BOLT_I64;
+      continue;
+    }
+    if (aggs[i].col < 0 || aggs[i].col >= n_inputs)
+      return set_error(BOLT_ERR_PLAN, "aggregate %d: input column %d out of range (%d inputs)", i, aggs[i].col, n_inputs);
+    const bolt_column_t& c = inputs[aggs[i].col];
+    if (c.length != n)
+      return set_error(BOLT_ERR_PLAN, "aggregate %d: input has %lld rows, keys have %lld", i, (long long)c.length,
+                       (long long)n);
+    g.data = c.data;
+    g.validity = c.validity;
+    g.dtype = c.dtype;
+    bool nullable = c.validity != nullptr;
+    int dt = c.dtype;
+    if (g.op == BOLT_AGG_COUNT) {
+      g.kind = K_COUNT;
+      g.w = w++;
+      out_dtype[i] = BOLT_I64;
+      continue;
+    }
+    if (dt == BOLT_BOOL) return set_error(BOLT_ERR_TYPE, "aggregate %d: Bool inputs are not supported", i);
+    switch (g.op) {
+      case BOLT_AGG_SUM:
+        if (dt == BOLT_DATE32) return set_error(BOLT_ERR_TYPE, "SUM over a temporal type is undefined in SQL (Date32)");
+        if (dt == BOLT_DEC128) { g.kind = K_DSUM; g.w = w; w += 3; out_dtype[i] = BOLT_DEC128; }
+        else if (dtype_is_float(dt)) { g.kind = K_FSUM; g.w = w++; out_dtype[i] = dt; }
+        else { g.kind = K_ISUM; g.w = w; w += 2; out_dtype[i] = BOLT_I64; }
+        if (nullable) g.nw = w++;
+        break;
+      case BOLT_AGG_MIN:
+      case BOLT_AGG_MAX:
+        if (dt == BOLT_DEC128) return set_error(BOLT_ERR_UNSUPPORTED, "GROUP BY MIN/MAX over Decimal128");
+        g.kind = K_MINMAX;
+        g.w = w++;
+        out_dtype[i] = dt;
+        if (nullable) g.nw = w++;
+        break;
+      case BOLT_AGG_AVG:
+        if (dt == BOLT_DATE32 || dt == BOLT_DEC128) return set_error(BOLT_ERR_TYPE, "AVG over %s is not supported", dtype_name(dt));
+        g.kind = K_AVG;
+        g.w = w++;
+        g.nw = w++;
+        out_dtype[i] = BOLT_F64;
+        break;
+      default:
+        if (dt == BOLT_DATE32 || dt == BOLT_DEC128) return set_error(BOLT_ERR_TYPE, "VAR/STDDEV over %s is not supported", dtype_name(dt));
+        g.kind = K_VAR;
+        g.w = w;
+        w += 2;
+        g.nw = w++;
+        out_dtype[i] = BOLT_F64;
+        break;
+    }
+  }
+  P->n = n_aggs;
+  P->words = w;
+  return BOLT_OK;
+}
+
+static double hll_estimate(const uint32_t* regs) {
+  const double m = kHllRegs;
+  double sum = 0;
+  int zeros = 0;
+  for (int i = 0; i < kHllRegs; i++) {
+    sum += std::ldexp(1.0, -(int)regs[i]);
+    zeros += regs[i] == 0;
+  }
+  double alpha = 0.7213 / (1.0 + 1.079 / m);
+  double e = alpha * m * m / sum;
+  if (e <= 2.5 * m && zeros > 0) e = m * std::log(m / zeros);
+  return e;
+}
+
+static uint64_t next_pow2(uint64_t x) {
+  uint64_t p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+static int lds_budget_bytes() {
+  static int cached = -1;
+  if (cached < 0) {
+    int dev = 0, v = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) {
+      (void)hipGetLastError();
+      v = 65536;
+    }
+    cached = std::min(v, 96 * 1024);
+  }
+  return cached;
+}
+
+extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int32_t n_keys,
+                            const bolt_column_t* inputs, int32_t n_inputs, const bolt_agg_t* aggs, int32_t n_aggs,
+                            const bolt_filter_t* filter, uint32_t flags, bolt_table_t** out) {
+  BOLT_CHECK_DEVICE();
+  clear_error();
+  *out = nullptr;
+  hipStream_t s = (hipStream_t)stream;
+  if (n_keys < 1 || n_keys > 2 || keys == nullptr)
+    return set_error(BOLT_ERR_PLAN, "bolt_groupby: 1 or 2 key columns required, got %d", n_keys);
+  const int64_t n = keys[0].length;
+  if (n > 0xFFFFFFFFll) return set_error(BOLT_ERR_PLAN, "bolt_groupby: %lld rows exceeds the u32 row-id limit", (long long)n);
+  KeySpec ks;
+  memset(&ks, 0, sizeof ks);
+  ks.n_keys = n_keys;
+  for (int k = 0; k < n_keys; k++) {
+    int dt = keys[k].dtype;
+    if (!(dt == BOLT_I32 || dt == BOLT_I64 || dt == BOLT_F32 || dt == BOLT_F64 || dt == BOLT_DATE32))
+      // src/exec/groupby_common.rs:117 key_bit_width
+      return set_error(BOLT_ERR_TYPE, "GROUP BY key dtype %s not supported", dtype_name(dt));
+    if (keys[k].length != n) return set_error(BOLT_ERR_PLAN, "bolt_groupby: key columns differ in length");
+    if (n_keys == 2 && dtype_size(dt) != 4)
+      return set_error(BOLT_ERR_UNSUPPORTED,
+                       "multi-column GROUP BY with > 64 bits of key width not yet supported");
+  }
+  ks.d0 = keys[0].data;
+  ks.v0 = keys[0].validity;
+  ks.t0 = keys[0].dtype;
+  if (n_keys == 2) {
+    ks.d1 = keys[1].data;
+    ks.v1 = keys[1].validity;
+    ks.t1 = keys[1].dtype;
+  }
+  ks.null_group = (n_keys == 1 && keys[0].validity != nullptr) ? 1 : 0;
+  GPlan P;
+  int32_t out_dt[kMaxAggs];
+  BOLT_TRY(make_gplan(aggs, n_aggs, inputs, n_inputs, n, &P, out_dt));
+  DevFilter f;
+  BOLT_TRY(make_dev_filter(filter, inputs, n_inputs, n, &f));
+
+  // ---- pass A ----
+  const int W = P.words;
+  const int budget = lds_budget_bytes();
+  int S = (int)((budget - (kHllRegs + 2) * 4) / (8 * (1 + W))) - 2;
+  if (S > 4096) S = 4096;
+  const bool use_lds = !(flags & BOLT_GROUPBY_FORCE_GLOBAL) && S >= 256 && n > 0;
+  int64_t n_parts = 0, n_spill = 0;
+  double est = 0;
+  DevBuf<unsigned long long> pkeys, pwords, counters;
+  DevBuf<uint8_t> ptag;
+  DevBuf<uint32_t> spill, hll;
+  if (use_lds) {
+    const size_t lds = (size_t)(S + 2) * 8 * (1 + W) + (kHllRegs + 2) * 4;
+    BOLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(gb_lds_kernel),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    unsigned grid = grid_for(n, kGbBlock, 16, 256);
+    const int64_t pcap = (int64_t)grid * (S + 2);
+    BOLT_TRY(pkeys.alloc((size_t)pcap, s));
+    BOLT_TRY(ptag.alloc((size_t)pcap, s));
+    BOLT_TRY(pwords.alloc((size_t)pcap * (W > 0 ? W : 1), s));
+    BOLT_TRY(spill.alloc((size_t)n, s));
+    BOLT_TRY(hll.alloc(kHllRegs, s));
+    BOLT_TRY(counters.alloc(2, s));
+    BOLT_TRY(fill_u32(hll.ptr, 0, kHllRegs, s));
+    BOLT_TRY(fill_u32(counters.ptr, 0, 4, s));
+    PartOut po{pkeys.ptr, ptag.ptr, pwords.ptr, pcap, counters.ptr, spill.ptr, counters.ptr + 1, hll.ptr};
+    gb_lds_kernel<<<grid, kGbBlock, lds, s>>>(ks, f, P, n, S, po);
+    BOLT_HIP(hipGetLastError());
+    unsigned long long hc[2];
+    std::vector<uint32_t> regs(kHllRegs);
+    BOLT_HIP(hipMemcpyAsync(hc, counters.ptr, 16, hipMemcpyDeviceToHost, s));
+    BOLT_HIP(hipMemcpyAsync(regs.data(), hll.ptr, kHllRegs * 4, hipMemcpyDeviceToHost, s));
+    BOLT_HIP(hipStreamSynchronize(s));
+    n_parts = (int64_t)hc[0];
+    n_spill = (int64_t)hc[1];
+    est = hll_estimate(regs.data());
+  } else {
+    n_spill = n;  // every row goes to the global table
+    est = (double)n;
+  }
+
+  // ---- pass B ----
+  const int64_t upper = n_parts + n_spill;  // exact bound on the group count
+  uint64_t want = (uint64_t)std::min<double>((double)upper, est * 1.25 + 64.0);
+  uint64_t cap = next_pow2(std::max<uint64_t>(1024, 2 * want));
+  GTable t;
+  DevBuf<unsigned long long> tkeys, twords;
+  DevBuf<uint32_t> tmisc;
+  BOLT_TRY(tmisc.alloc(8, s));
+  for (int attempt = 0;; attempt++) {
+    BOLT_TRY(tkeys.alloc(cap, s));
+    BOLT_TRY(twords.alloc((cap + 2) * (W > 0 ? W : 1), s));
+    BOLT_TRY(fill_u32(tmisc.ptr, 0, 8, s));
+    t.keys = tkeys.ptr;
+    t.words = twords.ptr;
+    t.used_special = tmisc.ptr;
+    t.err = tmisc.ptr + 4;
+    t.cap = cap;
+    gb_init_kernel<<<grid_for((int64_t)cap + 2, 256, 4), 256, 0, s>>>(t, P);
+    if (n_parts > 0)
+      gb_merge_parts_kernel<<<grid_for(n_parts, 256, 4), 256, 0, s>>>(t, P, pkeys.ptr, ptag.ptr, pwords.ptr,
+                                                                      (int64_t)pkeys.count, n_parts);
+    if (n_spill > 0)
+      gb_rows_kernel<<<grid_for(n_spill, 256, 4), 256, 0, s>>>(t, P, ks, f, use_lds ? spill.ptr : nullptr, n_spill);
+    BOLT_HIP(hipGetLastError());
+    uint32_t e0 = 0;
+    BOLT_HIP(hipMemcpyAsync(&e0, t.err, 4, hipMemcpyDeviceToHost, s));
+    BOLT_HIP(hipStreamSynchronize(s));
+    if (e0 == 0) break;
+    if (attempt >= 3 || cap >= (1ull << 33))
+      return set_error(BOLT_ERR_MEMORY, "GROUP BY hash table overflow at %llu slots", (unsigned long long)cap);
+    cap *= 4;
+  }
+  bool has_var = false;
+  for (int a = 0; a < P.n; a++) has_var |= P.a[a].kind == K_VAR;
+  if (has_var && n > 0) {
+    gb_m2_kernel<<<grid_for(n, 256, 4), 256, 0, s>>>(t, P, ks, f, n);
+    BOLT_HIP(hipGetLastError());
+  }
+
+  // ---- pass C ----
+  const uint64_t total = cap + 2;
+  const int64_t out_cap = (int64_t)std::min<uint64_t>(total, (uint64_t)std::max<int64_t>(upper, 0) + 2);
+  auto* tbl = new bolt_table();
+  tbl->stream = s;
+  std::vector<bolt_column_t> cols(n_keys + n_aggs);
+  OutSpec o;
+  memset(&o, 0, sizeof o);
+  int st = BOLT_OK;
+  for (int k = 0; k < n_keys && st == BOLT_OK; k++) {
+    st = table_alloc_col(tbl, keys[k].dtype, out_cap, k == 0 && ks.null_group, s, &cols[k]);
+    o.key[k] = OutCol{static_cast<uint8_t*>(cols[k].data), (uint32_t*)cols[k].validity, keys[k].dtype, 0};
+  }
+  for (int a = 0; a < n_aggs && st == BOLT_OK; a++) {
+    const GAgg& g = P.a[a];
+    bool can_null = (g.nw >= 0 && g.kind != K_COUNT) ||
+                    g.op == BOLT_AGG_VAR_SAMP || g.op == BOLT_AGG_STDDEV_SAMP;
+    st = table_alloc_col(tbl, out_dt[a], out_cap, can_null, s, &cols[n_keys + a]);
+    o.agg[a] = OutCol{static_cast<uint8_t*>(cols[n_keys + a].data), (uint32_t*)cols[n_keys + a].validity, out_dt[a], 0};
+  }
+  DevBuf<unsigned long long> packed;
+  DevBuf<uint32_t> slot_of;
+  if (st == BOLT_OK) st = packed.alloc((size_t)out_cap, s);
+  if (st == BOLT_OK) st = slot_of.alloc((size_t)out_cap, s);
+  if (st != BOLT_OK) {
+    bolt_table_release(tbl);
+    return st;
+  }
+  o.packed = packed.ptr;
+  o.slot_of = slot_of.ptr;
+  const int64_t tiles = (int64_t)((total + kExBlock * kExItems - 1) / (kExBlock * kExItems));
+  DevBuf<uint64_t> status;
+  DevBuf<uint32_t> emisc;
+  if ((st = status.alloc((size_t)tiles, s)) != BOLT_OK || (st = emisc.alloc(4, s)) != BOLT_OK ||
+      (st = fill_u32(status.ptr, 0, (size_t)tiles * 2, s)) != BOLT_OK || (st = fill_u32(emisc.ptr, 0, 4, s)) != BOLT_OK) {
+    bolt_table_release(tbl);
+    return st;
+  }
+  gb_extract_kernel<<<(unsigned)tiles, kExBlock, 0, s>>>(t, P, ks, o, status.ptr, emisc.ptr,
+                                                         reinterpret_cast<unsigned long long*>(emisc.ptr + 2));
+  hipError_t le = hipGetLastError();
+  uint32_t eh[4] = {0, 0, 0, 0}, terr[4] = {0, 0, 0, 0};
+  if (le == hipSuccess) le = hipMemcpyAsync(eh, emisc.ptr, 16, hipMemcpyDeviceToHost, s);
+  if (le == hipSuccess) le = hipMemcpyAsync(terr, t.err, 16, hipMemcpyDeviceToHost, s);
+  if (le == hipSuccess) le = hipStreamSynchronize(s);
+  if (le != hipSuccess) {
+    bolt_table_release(tbl);
+    return set_error(BOLT_ERR_DEVICE, "GROUP BY extraction failed: %s", hipGetErrorString(le));
+  }
+  const int64_t n_groups = (int64_t)(((uint64_t)eh[3] << 32) | eh[2]);
+  if (terr[1]) {
+    bolt_table_release(tbl);
+    return set_error(BOLT_ERR_OTHER, "GROUP BY extraction look-back timed out");
+  }
+  if (terr[2] & 1u) {
+    bolt_table_release(tbl);
+    return set_error(BOLT_ERR_TYPE, "SUM(integer) overflow: accumulator exceeds i64 range");
+  }
+  if (terr[2] & 2u) {
+    bolt_table_release(tbl);
+    return set_error(BOLT_ERR_TYPE, "SUM(Decimal128) overflow: accumulator exceeds i128 range");
+  }
+
+  // ---- ordering by packed key (src/exec/groupby.rs:800) ----
+  if ((flags & BOLT_GROUPBY_SORTED) && n_groups > 1) {
+    DevBuf<uint64_t> kin, kout;
+    DevBuf<uint32_t> vin, perm;
+    st = kin.alloc((size_t)n_groups, s);
+    if (st == BOLT_OK) st = kout.alloc((size_t)n_groups, s);
+    if (st == BOLT_OK) st = vin.alloc((size_t)n_groups, s);
+    if (st == BOLT_OK) st = perm.alloc((size_t)n_groups, s);
+    if (st == BOLT_OK) {
+      flip_sign_kernel<<<grid_for(n_groups, 256, 4), 256, 0, s>>>(packed.ptr, kin.ptr, vin.ptr, n_groups);
+      st = radix_sort_u64_pairs(s, kin.ptr, vin.ptr, n_groups, kout.ptr, perm.ptr, 64);
+    }
+    for (size_t c = 0; c < cols.size() && st == BOLT_OK; c++) {
+      bolt_column_t sorted;
+      st = table_alloc_col(tbl, cols[c].dtype, n_groups, cols[c].validity != nullptr, s, &sorted);
+      if (st != BOLT_OK) break;
+      gather_kernel<<<grid_for(n_groups, 256, 4), 256, 0, s>>>(
+          static_cast<const uint8_t*>(cols[c].data), static_cast<uint8_t*>(sorted.data), dtype_size(cols[c].dtype),
+          perm.ptr, n_groups, reinterpret_cast<const uint32_t*>(cols[c].validity),
+          reinterpret_cast<uint32_t*>(const_cast<uint8_t*>(sorted.validity)));
+      cols[c] = sorted;
+    }
+    if (st == BOLT_OK && hipGetLastError() != hipSuccess) st = set_error(BOLT_ERR_DEVICE, "GROUP BY gather failed");
+    if (st == BOLT_OK && hipStreamSynchronize(s) != hipSuccess) st = set_error(BOLT_ERR_DEVICE, "GROUP BY sort failed");
+    if (st != BOLT_OK) {
+      bolt_table_release(tbl);
+      return st;
+    }
+  }
+  for (auto& c : cols) c.length = n_groups;
+  tbl->cols = cols;
+  tbl->n_rows = n_groups;
+  *out = tbl;
+  return BOLT_OK;
+}

@@ -124,7 +124,27 @@ __global__ __launch_bounds__(kBuildBlock) void join_dup_kernel(const uint8_t* __
 
 // ---- fused filter + probe + aggregate ------------------------------------------
 
-template <typename KeyT, int NA>
+// Slot index held in the first / second key half of 16-byte chunk j of a
+// bucket line (chunk 0: count + key0; chunks 1-4: keys 2j-1, 2j; chunk 5:
+// key9 + values 0-1; chunks 6-7: values 2-9).
+__device__ __forceinline__ uint32_t expand_slot_bits(uint32_t first_half, uint32_t second_half) {
+  // first_half bit j -> slot (j == 0 ? 0 : j <= 4 ? 2j-1 : 9); second_half bit j (1..4) -> slot 2j
+  uint32_t m = first_half & 1u;
+#pragma unroll
+  for (int j = 1; j <= 4; j++) {
+    m |= ((first_half >> j) & 1u) << (2 * j - 1);
+    m |= ((second_half >> j) & 1u) << (2 * j);
+  }
+  m |= ((first_half >> 5) & 1u) << 9;
+  return m;
+}
+
+// Cooperative probe: a wave takes 64 probe rows; in each of 8 rounds the 8
+// lanes of a group read the 8 16-byte chunks of one row's bucket line, so one
+// load instruction fetches 8 whole lines.  Every round's load is issued
+// before the first compare.  Matches come back to the row's lane as a slot
+// mask; the rare overflowed bucket (claim count > 10) continues per lane.
+template <typename KeyT, int NT, int NA>
 __global__ __launch_bounds__(kProbeBlock) void join_probe_agg_kernel(
     const uint8_t* __restrict__ tbl, uint32_t nb, const uint32_t* __restrict__ nonunique,
     const KeyT* __restrict__ keys, const uint8_t* __restrict__ kvalid, int64_t n, DevFilter f, AggList aggs,
@@ -133,12 +153,53 @@ __global__ __launch_bounds__(kProbeBlock) void join_probe_agg_kernel(
 #pragma unroll
   for (int a = 0; a < NA; a++) acc_init(acc[a], aggs.a[a].op);
   const bool first_only = (*nonunique == 0u);
-  int64_t stride = (int64_t)gridDim.x * kProbeBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kProbeBlock + threadIdx.x; i < n; i += stride) {
-    if (!bit_valid(kvalid, i)) continue;
-    if (!eval_filter(f, i)) continue;
-    int64_t k = (int64_t)keys[i];
-    probe_chain(tbl, nb, k, first_only, [&](uint32_t w) {
+  const int lane = threadIdx.x & 63, g = lane >> 3, j = lane & 7;
+  const int64_t stride = (int64_t)gridDim.x * kProbeBlock;
+  for (int64_t base = (int64_t)blockIdx.x * kProbeBlock + (threadIdx.x & ~63); base < n; base += stride) {
+    const int64_t i = base + lane;
+    const int64_t ic = i < n ? i : n - 1;
+    int64_t k = (int64_t)__builtin_nontemporal_load(keys + ic);
+    Raw ra[NA];
+#pragma unroll
+    for (int a = 0; a < NA; a++)
+      if (aggs.a[a].src == 0 && aggs.a[a].op != BOLT_AGG_COUNT_STAR) ra[a] = load_raw(aggs.a[a].data, aggs.a[a].dtype, ic);
+    const bool act = i < n && bit_valid(kvalid, i) && eval_filter_nt<NT>(f, ic);
+    const uint64_t actmask = __ballot(act);
+    const uint32_t bkt = bucket_of(k, nb);
+
+    uint4 ch[8];
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      const int src = r * 8 + g;
+      const uint32_t bb = __shfl(bkt, src);
+      if ((actmask >> src) & 1ull)
+        ch[r] = *reinterpret_cast<const uint4*>(tbl + (size_t)bb * kBucketBytes + j * 16);
+      else
+        ch[r] = make_uint4(0, 0, 0, 0);
+    }
+    uint32_t my_mask = 0, my_cnt = 0;
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      const int src = r * 8 + g;
+      const int64_t kk = __shfl(k, src);
+      const uint32_t cnt = __shfl(ch[r].x, lane & ~7);
+      const uint32_t m_slots = cnt < kSlots ? cnt : kSlots;
+      const uint64_t kx = ((uint64_t)ch[r].y << 32) | ch[r].x;
+      const uint64_t kz = ((uint64_t)ch[r].w << 32) | ch[r].z;
+      const uint32_t sA = j == 0 ? 0u : (j <= 4 ? 2u * j - 1u : 9u);
+      bool mA = (j <= 5) && sA < m_slots && (int64_t)(j == 0 ? kz : kx) == kk;
+      bool mB = (j >= 1 && j <= 4) && 2u * j < m_slots && (int64_t)kz == kk;
+      const uint64_t bA = __ballot(mA), bB = __ballot(mB);
+      const uint32_t cnt_owner = __shfl(ch[r].x, (lane & 7) << 3);
+      if ((lane >> 3) == r) {
+        const int og = lane & 7;
+        my_mask = expand_slot_bits((uint32_t)(bA >> (8 * og)) & 0xFFu, (uint32_t)(bB >> (8 * og)) & 0xFFu);
+        my_cnt = cnt_owner;
+      }
+    }
+    if (!act) continue;
+    const uint8_t* line = tbl + (size_t)bkt * kBucketBytes;
+    auto consume = [&](uint32_t w) {
 #pragma unroll
       for (int a = 0; a < NA; a++) {
         const AggDesc& d = aggs.a[a];
@@ -149,11 +210,38 @@ __global__ __launch_bounds__(kProbeBlock) void join_probe_agg_kernel(
           carry_value(w, carry_dtype, vi, vf);
         } else {
           if (!bit_valid(d.validity, i)) continue;
-          load_value(d.data, d.dtype, i, vi, vhi, vf);
+          raw_value(ra[a], d.dtype, vi, vhi, vf);
         }
         acc_update(acc[a], d.op, d.dtype, vi, vhi, vf);
       }
-    });
+    };
+    bool found = false;
+    while (my_mask) {
+      const int s = __builtin_ctz(my_mask);
+      my_mask &= my_mask - 1;
+      consume(reinterpret_cast<const uint32_t*>(line + kValOff)[s]);
+      found = true;
+      if (first_only) break;
+    }
+    if (my_cnt > kSlots && !(found && first_only)) {
+      // overflowed bucket: walk the rest of the chain from the next bucket
+      const uint32_t nxt = (bkt + 1 == nb) ? 0 : bkt + 1;
+      uint32_t b2 = nxt;
+      for (uint32_t step = 1; step < nb; step++) {
+        const uint8_t* l2 = tbl + (size_t)b2 * kBucketBytes;
+        const uint32_t c2 = *reinterpret_cast<const uint32_t*>(l2);
+        const uint32_t m2 = c2 < kSlots ? c2 : kSlots;
+        bool stop = false;
+        for (uint32_t s2 = 0; s2 < m2; s2++) {
+          if (reinterpret_cast<const int64_t*>(l2 + kKeyOff)[s2] == k) {
+            consume(reinterpret_cast<const uint32_t*>(l2 + kValOff)[s2]);
+            if (first_only) { stop = true; break; }
+          }
+        }
+        if (stop || c2 <= kSlots) break;
+        b2 = (b2 + 1 == nb) ? 0 : b2 + 1;
+      }
+    }
   }
   block_reduce_store<NA>(acc, aggs, parts + (int64_t)blockIdx.x * NA);
 }
@@ -337,15 +425,15 @@ static int check_probe_args(const bolt_join_table_t* t, const bolt_column_t* pro
   return BOLT_OK;
 }
 
-template <typename KeyT>
-static int launch_probe_agg(int na, const bolt_join_table_t* t, const bolt_column_t& key, const DevFilter& f,
-                            const AggList& chunk, unsigned grid, Acc* parts, hipStream_t s) {
+template <typename KeyT, int NT>
+static int launch_probe_agg_nt(int na, const bolt_join_table_t* t, const bolt_column_t& key, const DevFilter& f,
+                               const AggList& chunk, unsigned grid, Acc* parts, hipStream_t s) {
   const KeyT* kp = static_cast<const KeyT*>(key.data);
-#define PROBE_AGG_CASE(NA)                                                                          \
-  case NA:                                                                                          \
-    join_probe_agg_kernel<KeyT, NA><<<grid, kProbeBlock, 0, s>>>(t->tbl, t->nb, t->flags, kp,        \
-                                                                 key.validity, key.length, f, chunk, \
-                                                                 t->carry_dtype, parts);            \
+#define PROBE_AGG_CASE(NA)                                                                              \
+  case NA:                                                                                              \
+    join_probe_agg_kernel<KeyT, NT, NA><<<grid, kProbeBlock, 0, s>>>(t->tbl, t->nb, t->flags, kp,        \
+                                                                     key.validity, key.length, f, chunk, \
+                                                                     t->carry_dtype, parts);            \
     break;
   switch (na) {
     PROBE_AGG_CASE(1)
@@ -357,6 +445,17 @@ static int launch_probe_agg(int na, const bolt_join_table_t* t, const bolt_colum
 #undef PROBE_AGG_CASE
   BOLT_HIP(hipGetLastError());
   return BOLT_OK;
+}
+
+template <typename KeyT>
+static int launch_probe_agg(int na, const bolt_join_table_t* t, const bolt_column_t& key, const DevFilter& f,
+                            const AggList& chunk, unsigned grid, Acc* parts, hipStream_t s) {
+  switch (f.n) {
+    case 0: return launch_probe_agg_nt<KeyT, 0>(na, t, key, f, chunk, grid, parts, s);
+    case 1: return launch_probe_agg_nt<KeyT, 1>(na, t, key, f, chunk, grid, parts, s);
+    case 2: return launch_probe_agg_nt<KeyT, 2>(na, t, key, f, chunk, grid, parts, s);
+    default: return launch_probe_agg_nt<KeyT, -1>(na, t, key, f, chunk, grid, parts, s);
+  }
 }
 
 // Grid for the probe kernels; BOLT_JOIN_PROBE_GRID overrides (tuning).

@@ -40,6 +40,12 @@ __device__ __forceinline__ bool cmp_apply(int op, int c) {
   return false;
 }
 
+__host__ __device__ __forceinline__ double as_f64_raw(uint64_t u) {
+  double d;
+  memcpy(&d, &u, 8);
+  return d;
+}
+
 __device__ __forceinline__ bool eval_term(const DevTerm& t, int64_t row) {
   bool valid = bit_valid(t.validity, row);
   if (t.op == BOLT_CMP_IS_NULL) return !valid;
@@ -85,35 +91,117 @@ __device__ __forceinline__ bool eval_filter(const DevFilter& f, int64_t row) {
   return true;
 }
 
+// Raw 8- or 16-byte image of one column value, loaded once per row so that
+// the predicate and the aggregates read registers, not memory.
+struct Raw {
+  uint64_t lo;
+  int64_t hi;
+};
+
+__device__ __forceinline__ Raw load_raw(const void* p, int dtype, int64_t row) {
+  Raw r;
+  r.hi = 0;
+  switch (dtype) {
+    case BOLT_I64: case BOLT_F64: r.lo = __builtin_nontemporal_load(static_cast<const uint64_t*>(p) + row); break;
+    case BOLT_I32: case BOLT_F32: case BOLT_DATE32:
+      r.lo = __builtin_nontemporal_load(static_cast<const uint32_t*>(p) + row);
+      break;
+    case BOLT_BOOL: r.lo = static_cast<const uint8_t*>(p)[row]; break;
+    case BOLT_DEC128: {
+      const uint64_t* w = static_cast<const uint64_t*>(p) + 2 * row;
+      r.lo = w[0];
+      r.hi = (int64_t)w[1];
+      break;
+    }
+    default: r.lo = 0;
+  }
+  return r;
+}
+
+// Decode a raw image as (int, decimal-high, float).
+__device__ __forceinline__ void raw_value(Raw r, int dtype, int64_t& vi, int64_t& vhi, double& vf) {
+  vhi = 0;
+  switch (dtype) {
+    case BOLT_I32: case BOLT_DATE32: vi = (int64_t)(int32_t)(uint32_t)r.lo; vf = (double)vi; break;
+    case BOLT_I64: vi = (int64_t)r.lo; vf = (double)vi; break;
+    case BOLT_F32: vi = 0; vf = (double)__uint_as_float((uint32_t)r.lo); break;
+    case BOLT_F64: vi = 0; vf = as_f64_raw(r.lo); break;
+    case BOLT_BOOL: vi = (int64_t)r.lo; vf = (double)vi; break;
+    case BOLT_DEC128: vi = (int64_t)r.lo; vhi = r.hi; vf = 0; break;
+    default: vi = 0; vf = 0;
+  }
+}
+
+__device__ __forceinline__ bool eval_term_raw(const DevTerm& t, Raw r, bool valid) {
+  if (t.op == BOLT_CMP_IS_NULL) return !valid;
+  if (t.op == BOLT_CMP_IS_NOT_NULL) return valid;
+  if (!valid) return false;
+  int64_t vi, vhi;
+  double vf;
+  raw_value(r, t.dtype, vi, vhi, vf);
+  if (t.dtype == BOLT_F32 || t.dtype == BOLT_F64) {
+    double l = t.lit_f64;
+    switch (t.op) {
+      case BOLT_CMP_LT: return vf < l;
+      case BOLT_CMP_LE: return vf <= l;
+      case BOLT_CMP_GT: return vf > l;
+      case BOLT_CMP_GE: return vf >= l;
+      case BOLT_CMP_EQ: return vf == l;
+      case BOLT_CMP_NE: return vf != l;
+    }
+    return false;
+  }
+  int c;
+  if (t.dtype == BOLT_DEC128)
+    c = vhi < t.lit_hi ? -1 : vhi > t.lit_hi ? 1 : ((uint64_t)vi < (uint64_t)t.lit_i64 ? -1 : (uint64_t)vi > (uint64_t)t.lit_i64 ? 1 : 0);
+  else
+    c = vi < t.lit_i64 ? -1 : (vi > t.lit_i64 ? 1 : 0);
+  return cmp_apply(t.op, c);
+}
+
+// Evaluate the first NT terms (compile-time count) of `f` at `row`; with
+// NT < 0 the runtime count f.n is used.
+template <int NT>
+__device__ __forceinline__ bool eval_filter_nt(const DevFilter& f, int64_t row) {
+  if constexpr (NT < 0) {
+    return eval_filter(f, row);
+  } else {
+    Raw r[NT > 0 ? NT : 1];
+#pragma unroll
+    for (int t = 0; t < NT; t++) r[t] = load_raw(f.t[t].data, f.t[t].dtype, row);
+    bool ok = true;
+#pragma unroll
+    for (int t = 0; t < NT; t++) ok = ok && eval_term_raw(f.t[t], r[t], bit_valid(f.t[t].validity, row));
+    return ok;
+  }
+}
+
 // Host: validate a bolt_filter_t against `cols` and build the kernel form.
 int make_dev_filter(const bolt_filter_t* f, const bolt_column_t* cols, int32_t n_cols, int64_t n_rows,
                     DevFilter* out);
 
 // ---------------------------------------------------------------------------
-// Exact accumulators.
-//   integer / decimal SUM : 128-bit two's-complement (lo, hi) + an overflow
-//                           flag for carries out of bit 127; integer SUM is
+// Exact accumulators, three 64-bit words each:
+//   integer / decimal SUM : a,b = 128-bit two's-complement (lo, hi); bit 63 of
+//                           n flags a carry out of bit 127.  Integer SUM is
 //                           range-checked against i64 at finalize, so the
 //                           overflow error does not depend on row order;
-//   float SUM / AVG       : Neumaier pair (s, c) in f64, merged with TwoSum;
-//   MIN / MAX             : an order-preserving u64 image, so the merge is
-//                           one unsigned min / max (ints: sign flip; floats:
-//                           the NaN-largest total order of
+//   float SUM / AVG       : a,b = Neumaier pair (s, c) in f64, merged with
+//                           TwoSum;
+//   MIN / MAX             : a = an order-preserving u64 image, so the merge
+//                           is one unsigned min / max (ints: sign flip;
+//                           floats: the NaN-largest total order of
 //                           src/exec/aggregate.rs:2165);
-//   VAR / STDDEV          : Welford (n, mean, M2), merged with Chan's formula
-//                           (src/exec/welford.rs:84);
-//   COUNT                 : u64.
+//   VAR / STDDEV          : a,b = Welford (mean, M2), merged with Chan's
+//                           formula (src/exec/welford.rs:84);
+//   n                     : values seen (COUNT).
 // ---------------------------------------------------------------------------
 struct Acc {
-  uint64_t lo;
-  int64_t hi;
-  double s;   // float sum / Welford mean
-  double c;   // Neumaier compensation / Welford M2
-  uint64_t n; // values seen
-  uint64_t mm;
-  uint32_t ovf;
-  uint32_t pad;
+  uint64_t a;
+  uint64_t b;
+  uint64_t n;  // count in bits 0..62, overflow flag in bit 63
 };
+constexpr uint64_t kOvfBit = 1ull << 63;
 
 struct AggDesc {
   const void* data;
@@ -129,47 +217,57 @@ struct AggList {
 };
 
 __host__ __device__ __forceinline__ uint64_t mm_identity(int op) { return op == BOLT_AGG_MIN ? ~0ull : 0ull; }
+__host__ __device__ __forceinline__ double as_f64(uint64_t u) { double d; memcpy(&d, &u, 8); return d; }
+__host__ __device__ __forceinline__ uint64_t as_u64(double d) { uint64_t u; memcpy(&u, &d, 8); return u; }
+__host__ __device__ __forceinline__ uint64_t acc_count(const Acc& a) { return a.n & ~kOvfBit; }
 
 __device__ __forceinline__ void acc_init(Acc& a, int op) {
-  a.lo = 0; a.hi = 0; a.s = 0; a.c = 0; a.n = 0; a.ovf = 0; a.pad = 0;
-  a.mm = mm_identity(op);
+  a.a = (op == BOLT_AGG_MIN || op == BOLT_AGG_MAX) ? mm_identity(op) : 0;
+  a.b = 0;
+  a.n = 0;
 }
 
 __device__ __forceinline__ void add128(Acc& a, uint64_t lo, int64_t hi) {
-  uint64_t nlo = a.lo + lo;
-  uint64_t carry = nlo < a.lo;
-  int64_t h;
-  // overflow of the signed 128-bit sum: operands of equal sign, result of the other
-  int64_t t;
-  bool o1 = __builtin_add_overflow(a.hi, hi, &t);
+  uint64_t nlo = a.a + lo;
+  uint64_t carry = nlo < a.a;
+  int64_t t, h;
+  // signed 128-bit overflow: exactly one of the two partial adds of the high
+  // word overflows
+  bool o1 = __builtin_add_overflow((int64_t)a.b, hi, &t);
   bool o2 = __builtin_add_overflow(t, (int64_t)carry, &h);
-  a.ovf |= (uint32_t)(o1 ^ o2);
-  a.lo = nlo;
-  a.hi = h;
+  a.n |= (uint64_t)(o1 ^ o2) << 63;
+  a.a = nlo;
+  a.b = (uint64_t)h;
 }
 
 __device__ __forceinline__ void neum_add(Acc& a, double v) {
-  double t = a.s + v;
-  if (fabs(a.s) >= fabs(v)) a.c += (a.s - t) + v;
-  else a.c += (v - t) + a.s;
-  a.s = t;
+  double s = as_f64(a.a), c = as_f64(a.b);
+  double t = s + v;
+  if (fabs(s) >= fabs(v)) c += (s - t) + v;
+  else c += (v - t) + s;
+  a.a = as_u64(t);
+  a.b = as_u64(c);
 }
 
 __device__ __forceinline__ void welford_push(Acc& a, double x) {
   a.n += 1;
-  double d = x - a.s;
-  a.s += d / (double)a.n;
-  a.c += d * (x - a.s);
+  double mean = as_f64(a.a), m2 = as_f64(a.b);
+  double d = x - mean;
+  mean += d / (double)acc_count(a);
+  m2 += d * (x - mean);
+  a.a = as_u64(mean);
+  a.b = as_u64(m2);
 }
 
-// Fold one value (already known non-NULL) into `a` for aggregate `op`.
-// `v_i` is the integer value (or decimal low word), `v_hi` the decimal high
-// word, `v_f` the float value, `is_float` selects the float form.
+// Fold one (non-NULL) value into `a` for aggregate `op`: `v_i` is the
+// integer value (decimal low word), `v_hi` the decimal high word, `v_f` the
+// value as f64.
 __device__ __forceinline__ void acc_update(Acc& a, int op, int dtype, int64_t v_i, int64_t v_hi, double v_f) {
   switch (op) {
     case BOLT_AGG_SUM:
-      if (dtype == BOLT_F32 || dtype == BOLT_F64) { neum_add(a, v_f); a.n++; }
-      else { add128(a, (uint64_t)v_i, dtype == BOLT_DEC128 ? v_hi : (v_i >> 63)); a.n++; }
+      if (dtype == BOLT_F32 || dtype == BOLT_F64) neum_add(a, v_f);
+      else add128(a, (uint64_t)v_i, dtype == BOLT_DEC128 ? v_hi : (v_i >> 63));
+      a.n++;
       break;
     case BOLT_AGG_AVG: neum_add(a, v_f); a.n++; break;
     case BOLT_AGG_COUNT: case BOLT_AGG_COUNT_STAR: a.n++; break;
@@ -177,7 +275,7 @@ __device__ __forceinline__ void acc_update(Acc& a, int op, int dtype, int64_t v_
       uint64_t k;
       if (dtype == BOLT_F32 || dtype == BOLT_F64) k = f64_order_key(v_f);
       else k = (uint64_t)v_i ^ 0x8000000000000000ull;
-      a.mm = op == BOLT_AGG_MIN ? (k < a.mm ? k : a.mm) : (k > a.mm ? k : a.mm);
+      a.a = op == BOLT_AGG_MIN ? (k < a.a ? k : a.a) : (k > a.a ? k : a.a);
       a.n++;
       break;
     }
@@ -186,30 +284,35 @@ __device__ __forceinline__ void acc_update(Acc& a, int op, int dtype, int64_t v_
 }
 
 // Merge `b` into `a` (both for aggregate `op`).
-__device__ __forceinline__ void acc_merge(Acc& a, const Acc& b, int op) {
+__device__ __forceinline__ void acc_merge(Acc& a, const Acc& b, int op, int dtype) {
   switch (op) {
-    case BOLT_AGG_SUM: case BOLT_AGG_AVG: {
-      add128(a, b.lo, b.hi);
-      a.ovf |= b.ovf;
-      // TwoSum of the leading parts, compensations added
-      double s = a.s + b.s;
-      double bp = s - a.s;
-      double err = (a.s - (s - bp)) + (b.s - bp);
-      a.c = a.c + b.c + err;
-      a.s = s;
-      a.n += b.n;
+    case BOLT_AGG_SUM: case BOLT_AGG_AVG:
+      if (op == BOLT_AGG_SUM && !(dtype == BOLT_F32 || dtype == BOLT_F64)) {
+        add128(a, b.a, (int64_t)b.b);
+        a.n = ((a.n | b.n) & kOvfBit) | (acc_count(a) + acc_count(b));
+      } else {
+        // TwoSum of the leading parts, compensations added
+        double as = as_f64(a.a), bs = as_f64(b.a);
+        double s = as + bs;
+        double bp = s - as;
+        double err = (as - (s - bp)) + (bs - bp);
+        a.b = as_u64(as_f64(a.b) + as_f64(b.b) + err);
+        a.a = as_u64(s);
+        a.n += b.n;
+      }
       break;
-    }
     case BOLT_AGG_COUNT: case BOLT_AGG_COUNT_STAR: a.n += b.n; break;
-    case BOLT_AGG_MIN: a.mm = b.mm < a.mm ? b.mm : a.mm; a.n += b.n; break;
-    case BOLT_AGG_MAX: a.mm = b.mm > a.mm ? b.mm : a.mm; a.n += b.n; break;
+    case BOLT_AGG_MIN: a.a = b.a < a.a ? b.a : a.a; a.n += b.n; break;
+    case BOLT_AGG_MAX: a.a = b.a > a.a ? b.a : a.a; a.n += b.n; break;
     default: {
-      if (b.n == 0) break;
-      if (a.n == 0) { a = b; break; }
-      double na = (double)a.n, nb = (double)b.n, n = na + nb;
-      double d = b.s - a.s;
-      a.s = a.s + d * (nb / n);
-      a.c = a.c + b.c + d * d * (na * nb / n);
+      uint64_t nb_ = acc_count(b), na_ = acc_count(a);
+      if (nb_ == 0) break;
+      if (na_ == 0) { a = b; break; }
+      double na = (double)na_, nb = (double)nb_, n = na + nb;
+      double ma = as_f64(a.a), mb = as_f64(b.a);
+      double d = mb - ma;
+      a.a = as_u64(ma + d * (nb / n));
+      a.b = as_u64(as_f64(a.b) + as_f64(b.b) + d * d * (na * nb / n));
       a.n += b.n;
       break;
     }
@@ -218,23 +321,18 @@ __device__ __forceinline__ void acc_merge(Acc& a, const Acc& b, int op) {
 
 __device__ __forceinline__ Acc shfl_xor_acc(const Acc& a, int m) {
   Acc r;
-  r.lo = __shfl_xor(a.lo, m);
-  r.hi = __shfl_xor(a.hi, m);
-  r.s = __shfl_xor(a.s, m);
-  r.c = __shfl_xor(a.c, m);
+  r.a = __shfl_xor(a.a, m);
+  r.b = __shfl_xor(a.b, m);
   r.n = __shfl_xor(a.n, m);
-  r.mm = __shfl_xor(a.mm, m);
-  r.ovf = __shfl_xor(a.ovf, m);
-  r.pad = 0;
   return r;
 }
 
 // Reduce `a` across the 64 lanes of the wave (result valid in every lane).
-__device__ __forceinline__ void wave_reduce_acc(Acc& a, int op) {
+__device__ __forceinline__ void wave_reduce_acc(Acc& a, int op, int dtype) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) {
     Acc b = shfl_xor_acc(a, m);
-    acc_merge(a, b, op);
+    acc_merge(a, b, op, dtype);
   }
 }
 

@@ -17,7 +17,7 @@ __device__ __forceinline__ void block_reduce_store(Acc (&acc)[NA], const AggList
   __shared__ Acc sh[kWaves][NA];
   int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int a = 0; a < NA; a++) wave_reduce_acc(acc[a], aggs.a[a].op);
+  for (int a = 0; a < NA; a++) wave_reduce_acc(acc[a], aggs.a[a].op, aggs.a[a].dtype);
   if (lane == 0) {
 #pragma unroll
     for (int a = 0; a < NA; a++) sh[wave][a] = acc[a];
@@ -27,7 +27,7 @@ __device__ __forceinline__ void block_reduce_store(Acc (&acc)[NA], const AggList
 #pragma unroll
     for (int a = 0; a < NA; a++) {
       Acc r = sh[0][a];
-      for (int w = 1; w < kWaves; w++) acc_merge(r, sh[w][a], aggs.a[a].op);
+      for (int w = 1; w < kWaves; w++) acc_merge(r, sh[w][a], aggs.a[a].op, aggs.a[a].dtype);
       out[a] = r;
     }
   }
@@ -43,7 +43,7 @@ __global__ __launch_bounds__(kReduceBlock) void finish_partials_kernel(const Acc
   for (int a = 0; a < NA; a++) acc_init(acc[a], aggs.a[a].op);
   for (int p = threadIdx.x; p < n_parts; p += kReduceBlock) {
 #pragma unroll
-    for (int a = 0; a < NA; a++) acc_merge(acc[a], parts[(int64_t)p * NA + a], aggs.a[a].op);
+    for (int a = 0; a < NA; a++) acc_merge(acc[a], parts[(int64_t)p * NA + a], aggs.a[a].op, aggs.a[a].dtype);
   }
   block_reduce_store<NA>(acc, aggs, out);
 }

@@ -1,0 +1,402 @@
+// sort.hip — stable LSD radix sort of (64-bit key, u32 value) pairs, and
+// the multi-column ORDER BY permutation built on it.
+//
+// Replaces run_radix_pipeline_i64 (src/exec/gpu_sort.rs:1493) and the
+// histogram / scatter kernels (src/jit/sort_kernel_radix.rs:489,885), which
+// use 4-bit digits (16 passes for int64) with a device-to-host histogram
+// copy, a host scan and a host-to-device offset copy between passes.  Here:
+//   * one up-front pass histograms all eight 8-bit digits at once (LDS
+//     histograms, one global atomic per bin per block);
+//   * each digit is then one "onesweep" pass (Adinets & Merrill, 2022): a
+//     4096-key tile ranks its keys with wave64 ballot matching (8 ballots
+//     per key give each key its set of same-digit peers), looks back across
+//     earlier tiles per digit for its global offsets (decoupled look-back,
+//     one digit per thread), reorders the tile by digit in LDS and writes
+//     runs of equal digits with coalesced stores;
+//   * passes whose digit is constant over the whole input are skipped.
+// Key transforms follow src/exec/gpu_sort.rs:1070-1180: signed integers flip
+// the sign bit, floats use the IEEE-monotonic flip after canonicalising NaN
+// (sorted last) and -0.0 (== +0.0), DESC is the bitwise NOT.
+#include <algorithm>
+#include <vector>
+
+#include "bolt_internal.h"
+
+namespace bolt {
+
+constexpr int kRadixBits = 8;
+constexpr int kBins = 1 << kRadixBits;
+constexpr int kSortBlock = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortBlock * kSortItems;  // 4096 keys
+constexpr int kSortWaves = kSortBlock / 64;
+constexpr int kMaxPasses = 9;  // up to 72 key bits
+
+constexpr uint32_t kSFlagAgg = 1u << 30;
+constexpr uint32_t kSFlagInc = 2u << 30;
+constexpr uint32_t kSValue = (1u << 30) - 1;
+
+// ---- histogram of every digit ---------------------------------------------------
+
+__global__ __launch_bounds__(kSortBlock) void radix_hist_kernel(const uint64_t* __restrict__ keys, int64_t n,
+                                                                int passes, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[kMaxPasses][kBins];
+  for (int i = threadIdx.x; i < passes * kBins; i += kSortBlock) (&h[0][0])[i] = 0;
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * kSortBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kSortBlock + threadIdx.x; i < n; i += stride) {
+    uint64_t k = __builtin_nontemporal_load(keys + i);
+    for (int p = 0; p < passes; p++) atomicAdd(&h[p][(k >> (p * kRadixBits)) & (kBins - 1)], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < passes * kBins; i += kSortBlock) {
+    uint32_t v = (&h[0][0])[i];
+    if (v) atomicAdd(&hist[i], v);
+  }
+}
+
+// exclusive scan of each pass's 256 bins (one block per pass)
+__global__ __launch_bounds__(kBins) void radix_scan_kernel(const uint32_t* __restrict__ hist,
+                                                           uint32_t* __restrict__ offs) {
+  __shared__ uint32_t s[kBins];
+  const int p = blockIdx.x, t = threadIdx.x;
+  s[t] = hist[p * kBins + t];
+  __syncthreads();
+  for (int d = 1; d < kBins; d <<= 1) {
+    uint32_t v = t >= d ? s[t - d] : 0;
+    __syncthreads();
+    s[t] += v;
+    __syncthreads();
+  }
+  offs[p * kBins + t] = s[t] - hist[p * kBins + t];
+}
+
+// ---- one onesweep pass ------------------------------------------------------------
+
+__global__ __launch_bounds__(kSortBlock) void radix_pass_kernel(
+    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
+    uint32_t* __restrict__ vout, int64_t n, int shift, const uint32_t* __restrict__ digit_base,
+    uint32_t* __restrict__ status, uint32_t* __restrict__ ticket, uint32_t* __restrict__ err) {
+  __shared__ uint64_t sk[kSortTile];
+  __shared__ uint32_t sv[kSortTile];
+  __shared__ uint32_t wh[kSortWaves][kBins];  // per-wave digit counts, then per-wave prefixes
+  __shared__ uint32_t lex[kBins];             // tile-local exclusive digit offsets
+  __shared__ uint32_t gb[kBins];              // global base of each digit's run in this tile
+  __shared__ uint32_t sh_tile;
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) sh_tile = atomicAdd(ticket, 1u);
+  for (int i = threadIdx.x; i < kSortWaves * kBins; i += kSortBlock) (&wh[0][0])[i] = 0;
+  __syncthreads();
+  const int64_t tile = sh_tile;
+  const int64_t tbase = tile * kSortTile;
+  // wave w owns tile keys [w*64*ITEMS, (w+1)*64*ITEMS), item j covers 64
+  // consecutive keys: input order = (wave, item, lane)
+  uint64_t key[kSortItems];
+  uint32_t val[kSortItems];
+  uint16_t rank[kSortItems];
+#pragma unroll
+  for (int j = 0; j < kSortItems; j++) {
+    int64_t i = tbase + (int64_t)wave * 64 * kSortItems + j * 64 + lane;
+    if (i < n) {
+      key[j] = __builtin_nontemporal_load(kin + i);
+      val[j] = __builtin_nontemporal_load(vin + i);
+    } else {
+      key[j] = ~0ull;  // out-of-range lanes sort to the last digit and are never stored
+      val[j] = 0;
+    }
+  }
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+  for (int j = 0; j < kSortItems; j++) {
+    const uint32_t d = (uint32_t)(key[j] >> shift) & (kBins - 1);
+    uint64_t peers = ~0ull;
+#pragma unroll
+    for (int b = 0; b < kRadixBits; b++) {
+      uint64_t m = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? m : ~m;
+    }
+    const int leader = __builtin_ctzll(peers);
+    uint32_t old = 0;
+    if (lane == leader) {
+      old = wh[wave][d];
+      wh[wave][d] = old + (uint32_t)__popcll(peers);
+    }
+    old = __shfl(old, leader);
+    rank[j] = (uint16_t)(old + (uint32_t)__popcll(peers & lt));
+  }
+  __syncthreads();
+  // thread t owns digit t: wave prefixes, tile count, local offsets
+  const int t = threadIdx.x;
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int w = 0; w < kSortWaves; w++) {
+    uint32_t c = wh[w][t];
+    wh[w][t] = cnt;
+    cnt += c;
+  }
+  // exclusive scan of the 256 digit counts across the block
+  lex[t] = cnt;
+  __syncthreads();
+  for (int d = 1; d < kBins; d <<= 1) {
+    uint32_t v = t >= d ? lex[t - d] : 0;
+    __syncthreads();
+    lex[t] += v;
+    __syncthreads();
+  }
+  const uint32_t my_excl = lex[t] - cnt;
+  __syncthreads();
+  lex[t] = my_excl;
+  // decoupled look-back for digit t (one digit per thread)
+  uint32_t* st = status + (size_t)tile * kBins + t;
+  uint32_t prefix = 0;
+  if (tile == 0) {
+    __hip_atomic_store(st, kSFlagInc | cnt, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    __hip_atomic_store(st, kSFlagAgg | cnt, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    int64_t pt = tile - 1;
+    uint32_t spins = 0;
+    while (pt >= 0) {
+      uint32_t v = __hip_atomic_load(status + (size_t)pt * kBins + t, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+      uint32_t fl = v & ~kSValue;
+      if (fl == 0) {
+        if (++spins > (1u << 24)) {
+          atomicOr(err, 1u);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        continue;
+      }
+      prefix += v & kSValue;
+      if (fl == kSFlagInc) break;
+      pt--;
+    }
+    __hip_atomic_store(st, kSFlagInc | (prefix + cnt), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  gb[t] = digit_base[t] + prefix;
+  __syncthreads();
+  // reorder the tile by digit in LDS (stable: rank follows input order)
+#pragma unroll
+  for (int j = 0; j < kSortItems; j++) {
+    int64_t i = tbase + (int64_t)wave * 64 * kSortItems + j * 64 + lane;
+    if (i < n) {
+      const uint32_t d = (uint32_t)(key[j] >> shift) & (kBins - 1);
+      const uint32_t pos = lex[d] + wh[wave][d] + rank[j];
+      sk[pos] = key[j];
+      sv[pos] = val[j];
+    }
+  }
+  __syncthreads();
+  const int64_t valid = n - tbase < kSortTile ? n - tbase : kSortTile;
+#pragma unroll
+  for (int j = 0; j < kSortItems; j++) {
+    const int i = j * kSortBlock + threadIdx.x;
+    if (i < valid) {
+      const uint64_t k = sk[i];
+      const uint32_t d = (uint32_t)(k >> shift) & (kBins - 1);
+      const uint64_t dst = (uint64_t)gb[d] + (uint32_t)i - lex[d];
+      kout[dst] = k;
+      vout[dst] = sv[i];
+    }
+  }
+}
+
+__global__ void copy_pairs_kernel(const uint64_t* __restrict__ ki, const uint32_t* __restrict__ vi,
+                                  uint64_t* __restrict__ ko, uint32_t* __restrict__ vo, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    ko[i] = ki[i];
+    vo[i] = vi[i];
+  }
+}
+
+// Sort (keys_in, vals_in) ascending by the low `key_bits` bits of the key
+// (stable), writing keys_out / vals_out.  Inputs are not modified.
+int radix_sort_u64_pairs(hipStream_t s, const uint64_t* keys_in, const uint32_t* vals_in, int64_t n,
+                         uint64_t* keys_out, uint32_t* vals_out, int key_bits) {
+  if (n <= 0) return BOLT_OK;
+  if (n >= (1ll << 30)) return set_error(BOLT_ERR_PLAN, "radix sort: %lld keys exceeds the 2^30 limit", (long long)n);
+  const int passes = (key_bits + kRadixBits - 1) / kRadixBits;
+  if (passes < 1 || passes > kMaxPasses) return set_error(BOLT_ERR_PLAN, "radix sort: bad key width %d", key_bits);
+  DevBuf<uint32_t> hist, offs, misc;
+  BOLT_TRY(hist.alloc((size_t)passes * kBins, s));
+  BOLT_TRY(offs.alloc((size_t)passes * kBins, s));
+  BOLT_TRY(misc.alloc(4, s));
+  BOLT_TRY(fill_u32(hist.ptr, 0, (size_t)passes * kBins, s));
+  radix_hist_kernel<<<grid_for(n, kSortBlock, 16, 2048), kSortBlock, 0, s>>>(keys_in, n, passes, hist.ptr);
+  radix_scan_kernel<<<passes, kBins, 0, s>>>(hist.ptr, offs.ptr);
+  BOLT_HIP(hipGetLastError());
+  std::vector<uint32_t> h((size_t)passes * kBins);
+  BOLT_HIP(hipMemcpyAsync(h.data(), hist.ptr, h.size() * 4, hipMemcpyDeviceToHost, s));
+  BOLT_HIP(hipStreamSynchronize(s));
+  std::vector<int> todo;
+  for (int p = 0; p < passes; p++) {
+    bool trivial = false;
+    for (int b = 0; b < kBins; b++)
+      if (h[(size_t)p * kBins + b] == (uint32_t)n) trivial = true;
+    if (!trivial) todo.push_back(p);
+  }
+  if (todo.empty()) {
+    copy_pairs_kernel<<<grid_for(n, 256, 4), 256, 0, s>>>(keys_in, vals_in, keys_out, vals_out, n);
+    BOLT_HIP(hipGetLastError());
+    return BOLT_OK;
+  }
+  const int64_t tiles = (n + kSortTile - 1) / kSortTile;
+  DevBuf<uint64_t> ktmp;
+  DevBuf<uint32_t> vtmp, status;
+  BOLT_TRY(ktmp.alloc((size_t)n, s));
+  BOLT_TRY(vtmp.alloc((size_t)n, s));
+  BOLT_TRY(status.alloc((size_t)tiles * kBins, s));
+  // ping-pong so that the last pass writes the output buffers
+  const int np = (int)todo.size();
+  const uint64_t* ksrc = keys_in;
+  const uint32_t* vsrc = vals_in;
+  for (int q = 0; q < np; q++) {
+    const bool to_out = ((np - 1 - q) % 2) == 0;
+    uint64_t* kdst = to_out ? keys_out : ktmp.ptr;
+    uint32_t* vdst = to_out ? vals_out : vtmp.ptr;
+    BOLT_TRY(fill_u32(status.ptr, 0, (size_t)tiles * kBins, s));
+    BOLT_TRY(fill_u32(misc.ptr, 0, 4, s));
+    const int p = todo[q];
+    radix_pass_kernel<<<(unsigned)tiles, kSortBlock, 0, s>>>(ksrc, vsrc, kdst, vdst, n, p * kRadixBits,
+                                                             offs.ptr + (size_t)p * kBins, status.ptr, misc.ptr,
+                                                             misc.ptr + 1);
+    BOLT_HIP(hipGetLastError());
+    ksrc = kdst;
+    vsrc = vdst;
+  }
+  uint32_t e = 0;
+  BOLT_HIP(hipMemcpyAsync(&e, misc.ptr + 1, 4, hipMemcpyDeviceToHost, s));
+  BOLT_HIP(hipStreamSynchronize(s));
+  if (e) return set_error(BOLT_ERR_OTHER, "radix sort look-back timed out");
+  return BOLT_OK;
+}
+
+// ---- key transforms ------------------------------------------------------------------
+
+__global__ void i64_to_radix_kernel(const int64_t* __restrict__ in, uint64_t* __restrict__ out, int64_t n, int desc) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    uint64_t k = (uint64_t)in[i] ^ 0x8000000000000000ull;  // gpu_sort.rs:1091
+    out[i] = desc ? ~k : k;
+  }
+}
+__global__ void radix_to_i64_kernel(const uint64_t* __restrict__ in, int64_t* __restrict__ out, int64_t n, int desc) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    uint64_t k = desc ? ~in[i] : in[i];
+    out[i] = (int64_t)(k ^ 0x8000000000000000ull);
+  }
+}
+
+// Ascending radix image of column value `row` (bits used: 32 or 64).
+__device__ __forceinline__ uint64_t radix_image(const void* d, int dt, int64_t row) {
+  switch (dt) {
+    case BOLT_I32: case BOLT_DATE32: return (uint64_t)(static_cast<const uint32_t*>(d)[row] ^ 0x80000000u);
+    case BOLT_I64: return static_cast<const uint64_t*>(d)[row] ^ 0x8000000000000000ull;
+    case BOLT_F32: {
+      uint32_t b = static_cast<const uint32_t*>(d)[row];
+      if ((b & 0x7FFFFFFFu) > 0x7F800000u) b = 0x7FC00000u;  // canonical NaN
+      else if ((b & 0x7FFFFFFFu) == 0) b = 0;
+      uint32_t m = (uint32_t)((int32_t)b >> 31) | 0x80000000u;  // gpu_sort.rs:1150
+      return (uint64_t)(b ^ m);
+    }
+    case BOLT_F64: {
+      uint64_t b = static_cast<const uint64_t*>(d)[row];
+      if ((b & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull) b = 0x7FF8000000000000ull;
+      else if ((b & 0x7FFFFFFFFFFFFFFFull) == 0) b = 0;
+      uint64_t m = (uint64_t)((int64_t)b >> 63) | 0x8000000000000000ull;  // gpu_sort.rs:1169
+      return b ^ m;
+    }
+    case BOLT_BOOL: return static_cast<const uint8_t*>(d)[row] ? 1 : 0;
+  }
+  return 0;
+}
+
+// key[i] = image of column value at perm[i]; mode 0 = value (DESC = NOT),
+// mode 1 = null rank only (0 first / 1 last per nulls_first).
+__global__ void order_key_kernel(const void* __restrict__ d, const uint8_t* __restrict__ valid, int dt, int bits,
+                                 int desc, int nulls_first, int mode, const uint32_t* __restrict__ perm,
+                                 uint64_t* __restrict__ key, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const uint64_t mask = bits >= 64 ? ~0ull : ((1ull << bits) - 1);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    int64_t row = perm[i];
+    if (mode == 1) {
+      bool isnull = !bit_valid(valid, row);
+      key[i] = isnull ? (nulls_first ? 0 : 1) : (nulls_first ? 1 : 0);
+    } else {
+      uint64_t k = bit_valid(valid, row) ? radix_image(d, dt, row) : 0;
+      key[i] = (desc ? ~k : k) & mask;
+    }
+  }
+}
+
+__global__ void iota_kernel(uint32_t* __restrict__ out, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = (uint32_t)i;
+}
+
+}  // namespace bolt
+
+using namespace bolt;
+
+extern "C" int bolt_sort_pairs_i64(bolt_stream_t stream, const int64_t* keys_in, const uint32_t* vals_in, int64_t n,
+                                   int32_t descending, int64_t* keys_out, uint32_t* vals_out) {
+  BOLT_CHECK_DEVICE();
+  clear_error();
+  hipStream_t s = (hipStream_t)stream;
+  if (n < 0) return set_error(BOLT_ERR_PLAN, "bolt_sort_pairs_i64: negative length");
+  if (n == 0) return BOLT_OK;
+  // radix images live in keys_out's memory only after the sort, so stage them
+  DevBuf<uint64_t> img, sorted;
+  BOLT_TRY(img.alloc((size_t)n, s));
+  BOLT_TRY(sorted.alloc((size_t)n, s));
+  i64_to_radix_kernel<<<grid_for(n, 256, 4), 256, 0, s>>>(keys_in, img.ptr, n, descending);
+  BOLT_HIP(hipGetLastError());
+  BOLT_TRY(radix_sort_u64_pairs(s, img.ptr, vals_in, n, sorted.ptr, vals_out, 64));
+  radix_to_i64_kernel<<<grid_for(n, 256, 4), 256, 0, s>>>(sorted.ptr, keys_out, n, descending);
+  BOLT_HIP(hipGetLastError());
+  return BOLT_OK;
+}
+
+extern "C" int bolt_sort_indices(bolt_stream_t stream, const bolt_column_t* keys, int32_t n_keys,
+                                 const int32_t* descending, const int32_t* nulls_first, uint32_t* out_rows) {
+  BOLT_CHECK_DEVICE();
+  clear_error();
+  hipStream_t s = (hipStream_t)stream;
+  if (n_keys < 1 || keys == nullptr) return set_error(BOLT_ERR_PLAN, "bolt_sort_indices: no sort keys");
+  const int64_t n = keys[0].length;
+  for (int k = 0; k < n_keys; k++) {
+    if (keys[k].length != n) return set_error(BOLT_ERR_PLAN, "bolt_sort_indices: key columns differ in length");
+    int dt = keys[k].dtype;
+    if (!(dt == BOLT_I32 || dt == BOLT_I64 || dt == BOLT_F32 || dt == BOLT_F64 || dt == BOLT_DATE32 || dt == BOLT_BOOL))
+      return set_error(BOLT_ERR_UNSUPPORTED, "bolt_sort_indices: sort key dtype %s not supported", dtype_name(dt));
+  }
+  if (n == 0) return BOLT_OK;
+  if (n >= (1ll << 30)) return set_error(BOLT_ERR_PLAN, "bolt_sort_indices: %lld rows exceeds 2^30", (long long)n);
+  DevBuf<uint64_t> k0, k1;
+  DevBuf<uint32_t> p0;
+  BOLT_TRY(k0.alloc((size_t)n, s));
+  BOLT_TRY(k1.alloc((size_t)n, s));
+  BOLT_TRY(p0.alloc((size_t)n, s));
+  iota_kernel<<<grid_for(n, 256, 4), 256, 0, s>>>(out_rows, n);
+  BOLT_HIP(hipGetLastError());
+  // least significant key first; each step is a stable sort of the current
+  // permutation, so earlier (more significant) keys win ties last
+  for (int k = n_keys - 1; k >= 0; k--) {
+    const bolt_column_t& c = keys[k];
+    const int bits = c.dtype == BOLT_BOOL ? 1 : dtype_size(c.dtype) * 8;
+    const int desc = descending ? descending[k] : 0;
+    const int nf = nulls_first ? nulls_first[k] : 1;
+    for (int mode = 0; mode < (c.validity ? 2 : 1); mode++) {
+      order_key_kernel<<<grid_for(n, 256, 4), 256, 0, s>>>(c.data, c.validity, c.dtype, bits, desc, nf, mode, out_rows,
+                                                          k0.ptr, n);
+      BOLT_HIP(hipGetLastError());
+      BOLT_TRY(radix_sort_u64_pairs(s, k0.ptr, out_rows, n, k1.ptr, p0.ptr, mode == 1 ? 1 : bits));
+      BOLT_HIP(hipMemcpyAsync(out_rows, p0.ptr, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+    }
+  }
+  BOLT_HIP(hipStreamSynchronize(s));
+  return BOLT_OK;
+}
