@@ -5,26 +5,24 @@
 // the build / probe kernels (src/jit/hash_join_kernel.rs:484,711) and the
 // collision-list variants for duplicate keys (:1183,1368).
 //
-// Table layout (MI355X-first): the reference uses an SoA open-addressing
-// table (i64 keys with an i64::MIN empty sentinel + u32 row ids, load <= 0.5,
-// src/exec/gpu_join.rs:577).  Here a bucket is one 128-byte cache line:
-//   [0,4)    u32 claim count        (no sentinel: any int64 key is storable)
-//   [8,88)   10 x i64 keys
-//   [88,128) 10 x u32 values (build row id, or the 4-byte carried column)
-// A probe reads one line for a hit or a miss unless the bucket overflowed
-// (claim count > 10, then the chain continues in the next bucket), and the
-// table runs at ~80% load instead of 50%, so a 16M-row build side occupies
-// ~256 MB — about the size of the MI355X Infinity Cache.
+// Table layout (MI355X-first).  The reference uses an SoA open-addressing
+// table (i64 keys with an i64::MIN empty sentinel + a separate u32 row-id
+// array, load <= 0.5, src/exec/gpu_join.rs:577), so a hit costs two scattered
+// loads.  On MI355X a scattered probe is limited by the number of distinct
+// 16-byte requests it issues, not by HBM bytes (a per-lane probe costs the
+// same against a 1 MB and a 256 MB table), so here a slot is one 16-byte
+// word that one load returns whole:
+//   { i64 key, u32 value (build row id or a carried 4-byte column), u32 state }
+// with state 0 = empty, so any int64 key is storable (no sentinel).  Linear
+// probing at a low load factor (default 0.25, set by BOLT_JOIN_LOAD_FACTOR)
+// makes the expected probe about 1.2 requests for a hit and 1.4 for a miss;
+// the 288 GB of HBM makes the larger table (1 GiB for 16M build rows) cheap.
 #include <cstdlib>
 
 #include "reduce_common.h"
 
 namespace bolt {
 
-constexpr int kSlots = 10;
-constexpr int kBucketBytes = 128;
-constexpr int kKeyOff = 8;
-constexpr int kValOff = 88;
 constexpr int kBuildBlock = 256;
 constexpr int kProbeBlock = 256;
 
@@ -36,170 +34,96 @@ __host__ __device__ __forceinline__ uint64_t fmix64(uint64_t x) {
   x ^= x >> 33;
   return x;
 }
-// Bucket index by multiply-shift range reduction of the mixed high word
-// (table sizes need not be powers of two).
-__host__ __device__ __forceinline__ uint32_t bucket_of(int64_t key, uint32_t nb) {
-  uint64_t h = fmix64((uint64_t)key);
-  return (uint32_t)(((h >> 32) * (uint64_t)nb) >> 32);
-}
 
 __device__ __forceinline__ int64_t load_key(const void* p, int dtype, int64_t i) {
   return dtype == BOLT_I64 ? static_cast<const int64_t*>(p)[i] : (int64_t) static_cast<const int32_t*>(p)[i];
 }
 
+__device__ __forceinline__ int64_t slot_key(const uint4& e) { return (int64_t)(((uint64_t)e.y << 32) | e.x); }
+
 // ---- build -------------------------------------------------------------------
 
-__global__ __launch_bounds__(kBuildBlock) void join_build_kernel(uint8_t* __restrict__ tbl, uint32_t nb,
+__global__ __launch_bounds__(kBuildBlock) void join_build_kernel(uint4* __restrict__ tbl, uint64_t mask,
                                                                  const void* __restrict__ key, int kdt,
                                                                  const uint8_t* __restrict__ kvalid,
                                                                  const uint32_t* __restrict__ carry, int64_t n,
                                                                  uint32_t* __restrict__ lost) {
-  int64_t stride = (int64_t)gridDim.x * kBuildBlock;
+  const int64_t stride = (int64_t)gridDim.x * kBuildBlock;
   for (int64_t i = (int64_t)blockIdx.x * kBuildBlock + threadIdx.x; i < n; i += stride) {
     if (!bit_valid(kvalid, i)) continue;  // NULL build keys never match (src/exec/join.rs:851)
-    int64_t k = load_key(key, kdt, i);
-    uint32_t v = carry ? carry[i] : (uint32_t)i;
-    uint32_t b = bucket_of(k, nb);
+    const int64_t k = load_key(key, kdt, i);
+    const uint32_t v = carry ? carry[i] : (uint32_t)i;
+    uint64_t s = fmix64((uint64_t)k) & mask;
     bool placed = false;
-    for (uint32_t step = 0; step < nb; step++) {
-      uint8_t* line = tbl + (size_t)b * kBucketBytes;
-      uint32_t s = atomicAdd(reinterpret_cast<uint32_t*>(line), 1u);
-      if (s < kSlots) {
-        reinterpret_cast<int64_t*>(line + kKeyOff)[s] = k;
-        reinterpret_cast<uint32_t*>(line + kValOff)[s] = v;
+    for (uint64_t step = 0; step <= mask; step++) {
+      uint32_t* st = &tbl[s].w;
+      if (*st == 0u && atomicCAS(st, 0u, 1u) == 0u) {
+        tbl[s].x = (uint32_t)k;
+        tbl[s].y = (uint32_t)((uint64_t)k >> 32);
+        tbl[s].z = v;
         placed = true;
         break;
       }
-      b = (b + 1 == nb) ? 0 : b + 1;
+      s = (s + 1) & mask;
     }
     if (!placed) atomicAdd(lost, 1u);
   }
 }
 
-// Walk the chain of `k`: calls on_match(value) for every stored copy.
-// Returns after the first match when `first_only`.
+// Walk the probe sequence of `k`: on_match(value) for every stored copy;
+// stops at the first empty slot, or after the first match when `first_only`.
 template <typename F>
-__device__ __forceinline__ void probe_chain(const uint8_t* __restrict__ tbl, uint32_t nb, int64_t k,
-                                            bool first_only, F&& on_match) {
-  uint32_t b = bucket_of(k, nb);
-  for (uint32_t step = 0; step < nb; step++) {
-    const uint8_t* line = tbl + (size_t)b * kBucketBytes;
-    // one 128-byte line: 8 x 16-byte loads issued together
-    const uint4* q = reinterpret_cast<const uint4*>(line);
-    uint4 w[8];
-#pragma unroll
-    for (int j = 0; j < 8; j++) w[j] = q[j];
-    const uint32_t* words = reinterpret_cast<const uint32_t*>(w);
-    uint32_t cnt = words[0];
-    uint32_t m = cnt < kSlots ? cnt : kSlots;
-    // fully unrolled so every word index is a compile-time constant and the
-    // line stays in registers
-#pragma unroll
-    for (int s = 0; s < kSlots; s++) {
-      int64_t kk = (int64_t)(((uint64_t)words[2 + 2 * s + 1] << 32) | words[2 + 2 * s]);
-      if ((uint32_t)s < m && kk == k) {
-        on_match(words[kValOff / 4 + s]);
-        if (first_only) return;
-      }
+__device__ __forceinline__ void probe_chain(const uint4* __restrict__ tbl, uint64_t mask, int64_t k, bool first_only,
+                                            F&& on_match) {
+  uint64_t s = fmix64((uint64_t)k) & mask;
+  for (uint64_t step = 0; step <= mask; step++) {
+    const uint4 e = tbl[s];
+    if (e.w == 0u) return;
+    if (slot_key(e) == k) {
+      on_match(e.z);
+      if (first_only) return;
     }
-    if (cnt <= kSlots) return;
-    b = (b + 1 == nb) ? 0 : b + 1;
+    s = (s + 1) & mask;
   }
 }
 
 // Marks the table non-unique if any build key is stored more than once.
-__global__ __launch_bounds__(kBuildBlock) void join_dup_kernel(const uint8_t* __restrict__ tbl, uint32_t nb,
+__global__ __launch_bounds__(kBuildBlock) void join_dup_kernel(const uint4* __restrict__ tbl, uint64_t mask,
                                                                const void* __restrict__ key, int kdt,
                                                                const uint8_t* __restrict__ kvalid, int64_t n,
                                                                uint32_t* __restrict__ nonunique) {
-  int64_t stride = (int64_t)gridDim.x * kBuildBlock;
+  const int64_t stride = (int64_t)gridDim.x * kBuildBlock;
   for (int64_t i = (int64_t)blockIdx.x * kBuildBlock + threadIdx.x; i < n; i += stride) {
     if (!bit_valid(kvalid, i)) continue;
-    int64_t k = load_key(key, kdt, i);
+    const int64_t k = load_key(key, kdt, i);
     int c = 0;
-    probe_chain(tbl, nb, k, false, [&](uint32_t) { c++; });
+    probe_chain(tbl, mask, k, false, [&](uint32_t) { c++; });
     if (c > 1) *nonunique = 1u;
   }
 }
 
-// ---- fused filter + probe + aggregate ------------------------------------------
+// ---- fused filter + probe + aggregate (generic: predicate and aggregates
+// are read from kernel arguments; instantiated for 0-2 predicate terms and
+// 1-4 aggregates) ---------------------------------------------------------------
 
-// Slot index held in the first / second key half of 16-byte chunk j of a
-// bucket line (chunk 0: count + key0; chunks 1-4: keys 2j-1, 2j; chunk 5:
-// key9 + values 0-1; chunks 6-7: values 2-9).
-__device__ __forceinline__ uint32_t expand_slot_bits(uint32_t first_half, uint32_t second_half) {
-  // first_half bit j -> slot (j == 0 ? 0 : j <= 4 ? 2j-1 : 9); second_half bit j (1..4) -> slot 2j
-  uint32_t m = first_half & 1u;
-#pragma unroll
-  for (int j = 1; j <= 4; j++) {
-    m |= ((first_half >> j) & 1u) << (2 * j - 1);
-    m |= ((second_half >> j) & 1u) << (2 * j);
-  }
-  m |= ((first_half >> 5) & 1u) << 9;
-  return m;
-}
-
-// Cooperative probe: a wave takes 64 probe rows; in each of 8 rounds the 8
-// lanes of a group read the 8 16-byte chunks of one row's bucket line, so one
-// load instruction fetches 8 whole lines.  Every round's load is issued
-// before the first compare.  Matches come back to the row's lane as a slot
-// mask; the rare overflowed bucket (claim count > 10) continues per lane.
 template <typename KeyT, int NT, int NA>
 __global__ __launch_bounds__(kProbeBlock) void join_probe_agg_kernel(
-    const uint8_t* __restrict__ tbl, uint32_t nb, const uint32_t* __restrict__ nonunique,
+    const uint4* __restrict__ tbl, uint64_t mask, const uint32_t* __restrict__ nonunique,
     const KeyT* __restrict__ keys, const uint8_t* __restrict__ kvalid, int64_t n, DevFilter f, AggList aggs,
     int carry_dtype, Acc* __restrict__ parts) {
   Acc acc[NA];
 #pragma unroll
   for (int a = 0; a < NA; a++) acc_init(acc[a], aggs.a[a].op);
   const bool first_only = (*nonunique == 0u);
-  const int lane = threadIdx.x & 63, g = lane >> 3, j = lane & 7;
   const int64_t stride = (int64_t)gridDim.x * kProbeBlock;
-  for (int64_t base = (int64_t)blockIdx.x * kProbeBlock + (threadIdx.x & ~63); base < n; base += stride) {
-    const int64_t i = base + lane;
-    const int64_t ic = i < n ? i : n - 1;
-    int64_t k = (int64_t)__builtin_nontemporal_load(keys + ic);
+  for (int64_t i = (int64_t)blockIdx.x * kProbeBlock + threadIdx.x; i < n; i += stride) {
+    const int64_t k = (int64_t)__builtin_nontemporal_load(keys + i);
     Raw ra[NA];
 #pragma unroll
     for (int a = 0; a < NA; a++)
-      if (aggs.a[a].src == 0 && aggs.a[a].op != BOLT_AGG_COUNT_STAR) ra[a] = load_raw(aggs.a[a].data, aggs.a[a].dtype, ic);
-    const bool act = i < n && bit_valid(kvalid, i) && eval_filter_nt<NT>(f, ic);
-    const uint64_t actmask = __ballot(act);
-    const uint32_t bkt = bucket_of(k, nb);
-
-    uint4 ch[8];
-#pragma unroll
-    for (int r = 0; r < 8; r++) {
-      const int src = r * 8 + g;
-      const uint32_t bb = __shfl(bkt, src);
-      if ((actmask >> src) & 1ull)
-        ch[r] = *reinterpret_cast<const uint4*>(tbl + (size_t)bb * kBucketBytes + j * 16);
-      else
-        ch[r] = make_uint4(0, 0, 0, 0);
-    }
-    uint32_t my_mask = 0, my_cnt = 0;
-#pragma unroll
-    for (int r = 0; r < 8; r++) {
-      const int src = r * 8 + g;
-      const int64_t kk = __shfl(k, src);
-      const uint32_t cnt = __shfl(ch[r].x, lane & ~7);
-      const uint32_t m_slots = cnt < kSlots ? cnt : kSlots;
-      const uint64_t kx = ((uint64_t)ch[r].y << 32) | ch[r].x;
-      const uint64_t kz = ((uint64_t)ch[r].w << 32) | ch[r].z;
-      const uint32_t sA = j == 0 ? 0u : (j <= 4 ? 2u * j - 1u : 9u);
-      bool mA = (j <= 5) && sA < m_slots && (int64_t)(j == 0 ? kz : kx) == kk;
-      bool mB = (j >= 1 && j <= 4) && 2u * j < m_slots && (int64_t)kz == kk;
-      const uint64_t bA = __ballot(mA), bB = __ballot(mB);
-      const uint32_t cnt_owner = __shfl(ch[r].x, (lane & 7) << 3);
-      if ((lane >> 3) == r) {
-        const int og = lane & 7;
-        my_mask = expand_slot_bits((uint32_t)(bA >> (8 * og)) & 0xFFu, (uint32_t)(bB >> (8 * og)) & 0xFFu);
-        my_cnt = cnt_owner;
-      }
-    }
-    if (!act) continue;
-    const uint8_t* line = tbl + (size_t)bkt * kBucketBytes;
-    auto consume = [&](uint32_t w) {
+      if (aggs.a[a].src == 0 && aggs.a[a].op != BOLT_AGG_COUNT_STAR) ra[a] = load_raw(aggs.a[a].data, aggs.a[a].dtype, i);
+    if (!bit_valid(kvalid, i) || !eval_filter_nt<NT>(f, i)) continue;
+    probe_chain(tbl, mask, k, first_only, [&](uint32_t w) {
 #pragma unroll
       for (int a = 0; a < NA; a++) {
         const AggDesc& d = aggs.a[a];
@@ -214,34 +138,7 @@ __global__ __launch_bounds__(kProbeBlock) void join_probe_agg_kernel(
         }
         acc_update(acc[a], d.op, d.dtype, vi, vhi, vf);
       }
-    };
-    bool found = false;
-    while (my_mask) {
-      const int s = __builtin_ctz(my_mask);
-      my_mask &= my_mask - 1;
-      consume(reinterpret_cast<const uint32_t*>(line + kValOff)[s]);
-      found = true;
-      if (first_only) break;
-    }
-    if (my_cnt > kSlots && !(found && first_only)) {
-      // overflowed bucket: walk the rest of the chain from the next bucket
-      const uint32_t nxt = (bkt + 1 == nb) ? 0 : bkt + 1;
-      uint32_t b2 = nxt;
-      for (uint32_t step = 1; step < nb; step++) {
-        const uint8_t* l2 = tbl + (size_t)b2 * kBucketBytes;
-        const uint32_t c2 = *reinterpret_cast<const uint32_t*>(l2);
-        const uint32_t m2 = c2 < kSlots ? c2 : kSlots;
-        bool stop = false;
-        for (uint32_t s2 = 0; s2 < m2; s2++) {
-          if (reinterpret_cast<const int64_t*>(l2 + kKeyOff)[s2] == k) {
-            consume(reinterpret_cast<const uint32_t*>(l2 + kValOff)[s2]);
-            if (first_only) { stop = true; break; }
-          }
-        }
-        if (stop || c2 <= kSlots) break;
-        b2 = (b2 + 1 == nb) ? 0 : b2 + 1;
-      }
-    }
+    });
   }
   block_reduce_store<NA>(acc, aggs, parts + (int64_t)blockIdx.x * NA);
 }
@@ -251,22 +148,22 @@ __global__ __launch_bounds__(kProbeBlock) void join_probe_agg_kernel(
 // Wave-aggregated output allocation: one atomic per wave and pass.
 template <typename KeyT>
 __global__ __launch_bounds__(kProbeBlock) void join_probe_emit_kernel(
-    const uint8_t* __restrict__ tbl, uint32_t nb, const uint32_t* __restrict__ nonunique,
+    const uint4* __restrict__ tbl, uint64_t mask, const uint32_t* __restrict__ nonunique,
     const KeyT* __restrict__ keys, const uint8_t* __restrict__ kvalid, int64_t n, DevFilter f,
     unsigned long long* __restrict__ counter, int64_t capacity, uint32_t* __restrict__ out_probe,
     uint32_t* __restrict__ out_build) {
   const bool first_only = (*nonunique == 0u);
-  int lane = threadIdx.x & 63;
-  int64_t stride = (int64_t)gridDim.x * kProbeBlock;
-  int64_t base_i = (int64_t)blockIdx.x * kProbeBlock + (threadIdx.x & ~63);
+  const int lane = threadIdx.x & 63;
+  const int64_t stride = (int64_t)gridDim.x * kProbeBlock;
+  const int64_t base_i = (int64_t)blockIdx.x * kProbeBlock + (threadIdx.x & ~63);
   // every lane of a wave iterates the same number of times so the wave-wide
   // scan below is convergent
   for (int64_t wi = base_i; wi < n; wi += stride) {
-    int64_t i = wi + lane;
-    bool active = i < n && bit_valid(kvalid, i) && eval_filter(f, i);
-    int64_t k = active ? (int64_t)keys[i] : 0;
+    const int64_t i = wi + lane;
+    const bool active = i < n && bit_valid(kvalid, i) && eval_filter(f, i);
+    const int64_t k = active ? (int64_t)keys[i] : 0;
     uint32_t m = 0;
-    if (active) probe_chain(tbl, nb, k, first_only, [&](uint32_t) { m++; });
+    if (active) probe_chain(tbl, mask, k, first_only, [&](uint32_t) { m++; });
     // inclusive scan of m across the wave
     uint32_t incl = m;
 #pragma unroll
@@ -274,13 +171,13 @@ __global__ __launch_bounds__(kProbeBlock) void join_probe_emit_kernel(
       uint32_t t = __shfl_up(incl, d);
       if (lane >= d) incl += t;
     }
-    uint32_t total = __shfl(incl, 63);
+    const uint32_t total = __shfl(incl, 63);
     unsigned long long base = 0;
     if (lane == 63 && total) base = atomicAdd(counter, (unsigned long long)total);
     base = __shfl(base, 63);
     if (m) {
       unsigned long long pos = base + incl - m;
-      probe_chain(tbl, nb, k, first_only, [&](uint32_t w) {
+      probe_chain(tbl, mask, k, first_only, [&](uint32_t w) {
         if ((int64_t)pos < capacity) {
           out_probe[pos] = (uint32_t)i;
           out_build[pos] = w;
@@ -296,8 +193,8 @@ __global__ __launch_bounds__(kProbeBlock) void join_probe_emit_kernel(
 using namespace bolt;
 
 struct bolt_join_table {
-  uint8_t* tbl = nullptr;
-  uint32_t nb = 0;
+  uint4* tbl = nullptr;
+  uint64_t mask = 0;  // capacity - 1 (power of two)
   int64_t n_build = 0;
   int32_t key_dtype = 0;
   int32_t carry_dtype = 0;  // 0 = values are build row ids
@@ -348,14 +245,20 @@ extern "C" int bolt_join_build(bolt_stream_t stream, const bolt_column_t* build_
     carry_dt = carry->dtype;
   }
   int64_t n = build_key->length;
-  uint64_t nb64 = (uint64_t)((n + 7) / 8);  // ~80% of 10 slots
-  if (nb64 < 1) nb64 = 1;
-  size_t bytes = (size_t)nb64 * kBucketBytes;
-  if (nb64 >= 0xFFFFFFFFull || bytes > join_table_cap_bytes())
+  // capacity: next power of two >= n / load factor
+  double load = 0.25;
+  if (const char* env = getenv("BOLT_JOIN_LOAD_FACTOR")) {
+    double v = atof(env);
+    if (v > 0.01 && v <= 0.9) load = v;
+  }
+  uint64_t cap = 64;
+  while ((double)cap * load < (double)n) cap <<= 1;
+  size_t bytes = (size_t)cap * sizeof(uint4);
+  if (bytes > join_table_cap_bytes())
     return set_error(BOLT_DECLINE, "hash join: build side of %lld rows needs a %zu-byte table, over the cap",
                      (long long)n, bytes);
   auto* t = new bolt_join_table();
-  t->nb = (uint32_t)nb64;
+  t->mask = cap - 1;
   t->n_build = n;
   t->key_dtype = build_key->dtype == BOLT_I64 ? BOLT_I64 : BOLT_I32;
   t->carry_dtype = carry_dt;
@@ -366,22 +269,18 @@ extern "C" int bolt_join_build(bolt_stream_t stream, const bolt_column_t* build_
     bolt_join_table_release(t);
     return st;
   }
-  if ((reinterpret_cast<uintptr_t>(t->tbl) & (kBucketBytes - 1)) != 0) {
-    bolt_join_table_release(t);
-    return set_error(BOLT_ERR_MEMORY, "hash join: table allocation is not 128-byte aligned");
-  }
-  // zero the claim counts (whole table: the lines are touched by the build anyway)
+  // state words must start at 0 (the fill writes the whole table)
   if ((st = fill_u32(t->tbl, 0, bytes / 4, s)) != BOLT_OK || (st = fill_u32(t->flags, 0, 16, s)) != BOLT_OK) {
     bolt_join_table_release(t);
     return st;
   }
   if (n > 0) {
     unsigned grid = grid_for(n, kBuildBlock, 4, 8192);
-    join_build_kernel<<<grid, kBuildBlock, 0, s>>>(t->tbl, t->nb, build_key->data, t->key_dtype,
+    join_build_kernel<<<grid, kBuildBlock, 0, s>>>(t->tbl, t->mask, build_key->data, t->key_dtype,
                                                    build_key->validity,
                                                    carry ? static_cast<const uint32_t*>(carry->data) : nullptr, n,
                                                    t->flags + 1);
-    join_dup_kernel<<<grid, kBuildBlock, 0, s>>>(t->tbl, t->nb, build_key->data, t->key_dtype, build_key->validity,
+    join_dup_kernel<<<grid, kBuildBlock, 0, s>>>(t->tbl, t->mask, build_key->data, t->key_dtype, build_key->validity,
                                                  n, t->flags);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -431,7 +330,7 @@ static int launch_probe_agg_nt(int na, const bolt_join_table_t* t, const bolt_co
   const KeyT* kp = static_cast<const KeyT*>(key.data);
 #define PROBE_AGG_CASE(NA)                                                                              \
   case NA:                                                                                              \
-    join_probe_agg_kernel<KeyT, NT, NA><<<grid, kProbeBlock, 0, s>>>(t->tbl, t->nb, t->flags, kp,        \
+    join_probe_agg_kernel<KeyT, NT, NA><<<grid, kProbeBlock, 0, s>>>(t->tbl, t->mask, t->flags, kp,        \
                                                                      key.validity, key.length, f, chunk, \
                                                                      t->carry_dtype, parts);            \
     break;
@@ -523,11 +422,11 @@ extern "C" int bolt_join_probe(bolt_stream_t stream, const bolt_join_table_t* t,
     unsigned grid = probe_grid(key.length);
     if (key.dtype == BOLT_I64)
       join_probe_emit_kernel<int64_t><<<grid, kProbeBlock, 0, s>>>(
-          t->tbl, t->nb, t->flags, static_cast<const int64_t*>(key.data), key.validity, key.length, f, counter.ptr,
+          t->tbl, t->mask, t->flags, static_cast<const int64_t*>(key.data), key.validity, key.length, f, counter.ptr,
           capacity, out_probe_rows, out_build_rows);
     else
       join_probe_emit_kernel<int32_t><<<grid, kProbeBlock, 0, s>>>(
-          t->tbl, t->nb, t->flags, static_cast<const int32_t*>(key.data), key.validity, key.length, f, counter.ptr,
+          t->tbl, t->mask, t->flags, static_cast<const int32_t*>(key.data), key.validity, key.length, f, counter.ptr,
           capacity, out_probe_rows, out_build_rows);
     BOLT_HIP(hipGetLastError());
   }

@@ -19,11 +19,15 @@ constexpr uint64_t kFlagInc = 2ull << 62;
 constexpr uint64_t kValueMask = (1ull << 62) - 1;
 constexpr uint32_t kSpinLimit = 1u << 24;
 
-__device__ __forceinline__ uint64_t ld_acquire(const uint64_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+// Status words carry their whole payload (flag + count in one 64-bit word),
+// and no other memory is read across tiles, so relaxed device-scope atomics
+// suffice.  Acquire / release at agent scope would invalidate or write back
+// the per-XCD L2 on every poll and publish.
+__device__ __forceinline__ uint64_t ld_status(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void st_release(uint64_t* p, uint64_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void st_status(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Called by one whole wave (64 lanes) of tile `tile` whose own count is
@@ -33,18 +37,18 @@ __device__ __forceinline__ uint64_t lookback_wave(uint64_t* __restrict__ status,
                                                   uint32_t* __restrict__ err) {
   int lane = threadIdx.x & 63;
   if (tile == 0) {
-    if (lane == 0) st_release(&status[0], kFlagInc | total);
+    if (lane == 0) st_status(&status[0], kFlagInc | total);
     return 0;
   }
-  if (lane == 0) st_release(&status[tile], kFlagAgg | total);
+  if (lane == 0) st_status(&status[tile], kFlagAgg | total);
   uint64_t excl = 0;
   int64_t base = tile - 1;
   for (;;) {
     int64_t idx = base - lane;
-    uint64_t v = idx >= 0 ? ld_acquire(&status[idx]) : (kFlagInc | 0);
+    uint64_t v = idx >= 0 ? ld_status(&status[idx]) : (kFlagInc | 0);
     uint32_t spins = 0;
     while (__ballot((v >> 62) == 0) != 0) {
-      if ((v >> 62) == 0) v = ld_acquire(&status[idx]);
+      if ((v >> 62) == 0) v = ld_status(&status[idx]);
       if (++spins > kSpinLimit) {
         if (lane == 0) atomicOr(err, 1u);
         v = kFlagInc;  // give up: the host reports the error
@@ -61,7 +65,7 @@ __device__ __forceinline__ uint64_t lookback_wave(uint64_t* __restrict__ status,
     if (inc) break;
     base -= 64;
   }
-  if (lane == 0) st_release(&status[tile], kFlagInc | (excl + total));
+  if (lane == 0) st_status(&status[tile], kFlagInc | (excl + total));
   return excl;
 }
 

@@ -147,17 +147,19 @@ __global__ __launch_bounds__(kSortBlock) void radix_pass_kernel(
   const uint32_t my_excl = lex[t] - cnt;
   __syncthreads();
   lex[t] = my_excl;
-  // decoupled look-back for digit t (one digit per thread)
+  // decoupled look-back for digit t (one digit per thread).  The status word
+  // holds its whole payload, so relaxed device-scope atomics suffice (acquire
+  // / release at agent scope would invalidate / write back the per-XCD L2).
   uint32_t* st = status + (size_t)tile * kBins + t;
   uint32_t prefix = 0;
   if (tile == 0) {
-    __hip_atomic_store(st, kSFlagInc | cnt, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(st, kSFlagInc | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   } else {
-    __hip_atomic_store(st, kSFlagAgg | cnt, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(st, kSFlagAgg | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     int64_t pt = tile - 1;
     uint32_t spins = 0;
     while (pt >= 0) {
-      uint32_t v = __hip_atomic_load(status + (size_t)pt * kBins + t, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+      uint32_t v = __hip_atomic_load(status + (size_t)pt * kBins + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       uint32_t fl = v & ~kSValue;
       if (fl == 0) {
         if (++spins > (1u << 24)) {
@@ -171,7 +173,7 @@ __global__ __launch_bounds__(kSortBlock) void radix_pass_kernel(
       if (fl == kSFlagInc) break;
       pt--;
     }
-    __hip_atomic_store(st, kSFlagInc | (prefix + cnt), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(st, kSFlagInc | (prefix + cnt), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   gb[t] = digit_base[t] + prefix;
   __syncthreads();

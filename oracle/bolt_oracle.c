@@ -362,7 +362,9 @@ static uint64_t hmap_slot(hmap_t* m, int64_t key, int* found) {
 /* groupby_common.rs:302,325 canonicalisation then bit pattern */
 static uint64_t key_bits(const bo_col* c, int64_t i) {
   switch (c->dtype) {
-    case BO_I32: return (uint64_t)(uint32_t)((const int32_t*)c->data)[i];
+    case BO_I32: case BO_DATE32: /* Date32 keys (rejected by the reference, groupby_common.rs:117)
+                                    are packed like Int32 */
+      return (uint64_t)(uint32_t)((const int32_t*)c->data)[i];
     case BO_I64: return (uint64_t)((const int64_t*)c->data)[i];
     case BO_F32: {
       float f = ((const float*)c->data)[i];

@@ -296,6 +296,14 @@ def test_groupby_float_keys_canonicalised():
     assert sorted(r["i64"][0].tolist()) == [1, 2, 2]
 
 
+def test_groupby_date32_keys_pack_like_int32():
+    k = np.array([-3, 5, -3, 7], np.int32)
+    r32 = O.groupby([O.HostCol(k, O.I32)], [O.HostCol(np.ones(4, np.int64), O.I64)], [(O.SUM, 0)])
+    rd = O.groupby([O.HostCol(k, O.DATE32)], [O.HostCol(np.ones(4, np.int64), O.I64)], [(O.SUM, 0)])
+    assert rd["keys"].tolist() == r32["keys"].tolist() == [5, 7, 2 ** 32 - 3]
+    assert rd["i64"][0].tolist() == [1, 1, 2]
+
+
 def test_groupby_integer_sum_overflow():
     # groupby.rs:1775 checked_group_sum
     k = np.array([1, 1, 2], np.int32)
