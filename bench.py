@@ -29,7 +29,8 @@ def _parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="join", choices=["join"])
+    p.add_argument("--workload", default="join", choices=["join", "sort", "groupby", "groupby-zipf", "calib"])
+    p.add_argument("--n", type=int, default=256 << 20, help="rows for the sort / groupby / calib workloads")
     p.add_argument("--n-build", type=int, default=16 << 20)
     p.add_argument("--n-probe", type=int, default=256 << 20)
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -157,13 +158,123 @@ def bench_join(args) -> dict:
     return line
 
 
+def _timed(stream, fn, steps, warmup):
+    from craton_bolt_amd.device import Event
+
+    for _ in range(warmup):
+        fn()
+    stream.synchronize()
+    e0, e1 = Event(), Event()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    e1.synchronize()
+    wall = time.perf_counter() - t0
+    return e0.elapsed_ms(e1) / steps, wall * 1000.0 / steps
+
+
+def _line(metric_value, unit, args, ms, dtype, config, roofline):
+    return {
+        "metric": "rows/s + % HBM peak per operator; batched/served vs plain queries/s, p99 ms",
+        "value": metric_value, "unit": unit, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+        "data": "synthetic", "config": config, "roofline": roofline,
+    }
+
+
+def _roof(bytes_per_launch, kernel_ms, kernel):
+    achieved = bytes_per_launch / (kernel_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel, "bytes_per_launch": bytes_per_launch}
+
+
+def bench_sort(args) -> dict:
+    """256M (int64 key, u32 row id) pairs; 200 B/key model for 64-bit LSD."""
+    from craton_bolt_amd import datagen as D
+    from craton_bolt_amd.device import DeviceBuffer, Stream
+    from craton_bolt_amd.ops import sort_pairs_i64
+
+    n = args.n
+    key, rid = D.sort_pairs(n)
+    dk, dv = DeviceBuffer.from_numpy(key), DeviceBuffer.from_numpy(rid)
+    s = Stream()
+    ms, wall = _timed(s, lambda: sort_pairs_i64(dk, dv, n, stream=s), args.steps, args.warmup)
+    model = n * 200
+    return _line(n / (ms * 1e-3), "keys/s", args, ms, "int64",
+                 {"workload": f"ORDER BY radix sort: {n} (int64 key uniform over the full range, u32 row id)",
+                  "n": n, "compulsory_bytes": n * 24, "model_bytes": model},
+                 _roof(model, ms, "radix sort (histogram + 8 onesweep passes), 200 B/key model"))
+
+
+def bench_groupby(args, zipf: bool) -> dict:
+    from craton_bolt_amd import _native as N
+    from craton_bolt_amd import datagen as D
+    from craton_bolt_amd.device import DeviceColumn, Stream
+    from craton_bolt_amd.ops import execute_groupby
+
+    n = args.n
+    s = Stream()
+    if zipf:
+        k, v = D.zipf_groupby(n, n_ranks=10_000_000)
+        aggs = [(N.SUM, 0), (N.COUNT_STAR, 0)]
+        desc = f"high-cardinality GROUP BY: {n} rows, int64 key Zipf(1.1) over 10M ranks, SUM(float64), COUNT(*)"
+        per_row, dtype = 16, "f64"
+    else:
+        k, v = D.groupby_small(n)
+        aggs = [(N.SUM, 0), (N.COUNT_STAR, 0), (N.MIN, 0), (N.MAX, 0)]
+        desc = f"plain GROUP BY: {n} rows, int32 key over 1000 values, int64 value; SUM, COUNT(*), MIN, MAX"
+        per_row, dtype = 12, "int64"
+    dk, dv = DeviceColumn.from_numpy(k), DeviceColumn.from_numpy(v)
+    res = {}
+
+    def run():
+        res["r"] = execute_groupby([dk], [dv], aggs, sorted_output=not zipf, stream=s)
+
+    ms, wall = _timed(s, run, args.steps, args.warmup)
+    g = res["r"]["n_groups"]
+    algo = n * per_row + g * (8 + 8 * len(aggs))
+    return _line(n / (ms * 1e-3), "rows/s", args, ms, dtype, {"workload": desc, "n": n, "groups": g},
+                 _roof(algo, ms, "whole GROUP BY (all passes)"))
+
+
+def bench_calib(args) -> dict:
+    """Known-size kernels for scaling the FETCH_SIZE / WRITE_SIZE counters:
+    SUM over 2^28 float64 reads 2 GiB; an always-true compaction of the same
+    column reads 2 GiB and writes 2 GiB."""
+    from craton_bolt_amd import _native as N
+    from craton_bolt_amd.device import DeviceColumn, Stream
+    from craton_bolt_amd.ops import Term, compact_columns, execute_aggregate
+
+    n = 1 << 28
+    x = np.arange(n, dtype=np.float64)
+    d = DeviceColumn.from_numpy(x)
+    s = Stream()
+    ms_sum, _ = _timed(s, lambda: execute_aggregate([d], [(N.SUM, 0)], stream=s), args.steps, args.warmup)
+    ms_cp, _ = _timed(s, lambda: compact_columns([d], [Term(0, N.IS_NOT_NULL)], stream=s), args.steps, args.warmup)
+    return _line(n * 8 / (ms_sum * 1e-3), "bytes/s", args, ms_sum, "f64",
+                 {"workload": "calibration: SUM over 2^28 float64 (2 GiB read), always-true compaction (2 GiB read "
+                              "+ 2 GiB write)", "sum_ms": ms_sum, "copy_ms": ms_cp, "bytes": n * 8},
+                 _roof(n * 8, ms_sum, "scalar_agg_kernel"))
+
+
 def main():
     args = _parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each box has one MI355X); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    line = bench_join(args)
+    if args.workload == "join":
+        line = bench_join(args)
+    elif args.workload == "sort":
+        line = bench_sort(args)
+    elif args.workload == "groupby":
+        line = bench_groupby(args, zipf=False)
+    elif args.workload == "groupby-zipf":
+        line = bench_groupby(args, zipf=True)
+    else:
+        line = bench_calib(args)
     print(json.dumps(line), flush=True)
 
 

@@ -32,6 +32,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
     f"-I{ROOT / 'include'}",
+    f"-I{ROOT / '_build'}",
 ]
 
 LIB_HIP = PKG / "libbolt_hip.so"
@@ -49,6 +50,16 @@ def _run(cmd: list[str], quiet: bool) -> None:
         print(out)
 
 
+def _write_jit_prelude() -> None:
+    """Embed device_acc.h as a string literal: it is prepended to the kernels
+    the library generates per query and compiles with hipRTC."""
+    text = (CSRC / "device_acc.h").read_text().replace("#pragma once\n", "")
+    out = OBJ / "jit_prelude.inc"
+    body = 'static const char kJitPrelude[] = R"BOLTJIT(\n' + text + '\n)BOLTJIT";\n'
+    if not out.exists() or out.read_text() != body:
+        out.write_text(body)
+
+
 def _compile_hip(src: Path, quiet: bool) -> Path:
     obj = OBJ / (src.stem + ".o")
     _run([HIPCC, *HIP_FLAGS, "-c", str(src), "-o", str(obj)], quiet)
@@ -57,6 +68,7 @@ def _compile_hip(src: Path, quiet: bool) -> Path:
 
 def build_hip(quiet: bool = True, jobs: int | None = None) -> Path:
     OBJ.mkdir(exist_ok=True)
+    _write_jit_prelude()
     srcs = sorted(CSRC.glob("*.hip"))
     jobs = jobs or min(len(srcs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4)), 16)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:

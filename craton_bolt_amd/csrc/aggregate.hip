@@ -10,6 +10,7 @@
 // a fixed order on the device.
 #include <cmath>
 
+#include "jit.h"
 #include "reduce_common.h"
 
 namespace bolt {
@@ -146,12 +147,13 @@ int acc_finalize(const Acc& a, int op, int dt, bolt_scalar_t* r) {
   }
 }
 
-int launch_finish(int na, const Acc* parts, int n_parts, const AggList& aggs, Acc* out, hipStream_t s) {
+int launch_finish(int na, const Acc* parts, int n_parts, int stride, int first, const AggList& aggs, Acc* out,
+                  hipStream_t s) {
   switch (na) {
-    case 1: finish_partials_kernel<1><<<1, kReduceBlock, 0, s>>>(parts, n_parts, aggs, out); break;
-    case 2: finish_partials_kernel<2><<<1, kReduceBlock, 0, s>>>(parts, n_parts, aggs, out); break;
-    case 3: finish_partials_kernel<3><<<1, kReduceBlock, 0, s>>>(parts, n_parts, aggs, out); break;
-    case 4: finish_partials_kernel<4><<<1, kReduceBlock, 0, s>>>(parts, n_parts, aggs, out); break;
+    case 1: finish_partials_kernel<1><<<1, kReduceBlock, 0, s>>>(parts, n_parts, stride, first, aggs, out); break;
+    case 2: finish_partials_kernel<2><<<1, kReduceBlock, 0, s>>>(parts, n_parts, stride, first, aggs, out); break;
+    case 3: finish_partials_kernel<3><<<1, kReduceBlock, 0, s>>>(parts, n_parts, stride, first, aggs, out); break;
+    case 4: finish_partials_kernel<4><<<1, kReduceBlock, 0, s>>>(parts, n_parts, stride, first, aggs, out); break;
     default: return set_error(BOLT_ERR_OTHER, "launch_finish: bad chunk size %d", na);
   }
   BOLT_HIP(hipGetLastError());
@@ -253,16 +255,36 @@ extern "C" int bolt_aggregate(bolt_stream_t stream, const bolt_column_t* inputs,
   if (n_aggs == 0) return BOLT_OK;
 
   unsigned grid = grid_for(n, kReduceBlock, kAggUnroll * 4, 256 * 8);
-  int n_chunks = (n_aggs + 3) / 4;
   DevBuf<Acc> parts, outs;
-  BOLT_TRY(parts.alloc((size_t)grid * 4 * n_chunks, s));
   BOLT_TRY(outs.alloc((size_t)n_aggs, s));
-  for (int c = 0; c < n_chunks; c++) {
-    int first = c * 4, na = n_aggs - first < 4 ? n_aggs - first : 4;
-    AggList chunk = agg_chunk(all, first, na);
-    Acc* p = parts.ptr + (size_t)c * grid * 4;
-    BOLT_TRY(launch_scalar_chunk(na, f, chunk, n, grid, p, s));
-    BOLT_TRY(launch_finish(na, p, (int)grid, chunk, outs.ptr + first, s));
+  bool done = false;
+  if (jit_enabled() && n > 0) {
+    JitSpec sp;
+    JitLits lits;
+    JitCols cols;
+    hipFunction_t fn;
+    if (jit_spec_from(f, all, false, 0, false, 0, &sp, &lits, &cols) == BOLT_OK &&
+        jit_get(jit_row_agg_source(sp), &fn) == BOLT_OK) {
+      BOLT_TRY(parts.alloc((size_t)grid * n_aggs, s));
+      BOLT_TRY(jit_launch_row_agg(fn, grid, s, nullptr, 0, nullptr, n, nullptr, nullptr, lits, cols, parts.ptr));
+      for (int first = 0; first < n_aggs; first += 4) {
+        int na = n_aggs - first < 4 ? n_aggs - first : 4;
+        BOLT_TRY(launch_finish(na, parts.ptr, (int)grid, n_aggs, first, agg_chunk(all, first, na), outs.ptr + first, s));
+      }
+      done = true;
+    }
+    clear_error();
+  }
+  if (!done) {
+    int n_chunks = (n_aggs + 3) / 4;
+    BOLT_TRY(parts.alloc((size_t)grid * 4 * n_chunks, s));
+    for (int c = 0; c < n_chunks; c++) {
+      int first = c * 4, na = n_aggs - first < 4 ? n_aggs - first : 4;
+      AggList chunk = agg_chunk(all, first, na);
+      Acc* p = parts.ptr + (size_t)c * grid * 4;
+      BOLT_TRY(launch_scalar_chunk(na, f, chunk, n, grid, p, s));
+      BOLT_TRY(launch_finish(na, p, (int)grid, na, 0, chunk, outs.ptr + first, s));
+    }
   }
   Acc host[kMaxAggs];
   BOLT_HIP(hipMemcpyAsync(host, outs.ptr, sizeof(Acc) * n_aggs, hipMemcpyDeviceToHost, s));

@@ -10,6 +10,7 @@
 #include <string>
 
 #include "../../include/bolt_ops.h"
+#include "device_acc.h"
 
 namespace bolt {
 
@@ -115,24 +116,6 @@ inline unsigned grid_for(int64_t work_items, int block, int per_thread, int max_
 
 __device__ __forceinline__ bool bit_valid(const uint8_t* validity, int64_t i) {
   return validity == nullptr || ((validity[i >> 3] >> (i & 7)) & 1);
-}
-
-// Order-preserving map of a double to u64 under the NaN-largest total order
-// of src/exec/aggregate.rs:2165 (all NaNs -> one top value, -0.0 < +0.0).
-// Native unsigned 64-bit atomic min/max on the image implements float
-// MIN/MAX without a CAS loop.
-__host__ __device__ __forceinline__ uint64_t f64_order_key(double d) {
-  uint64_t b;
-  memcpy(&b, &d, 8);
-  if ((b & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull) return ~0ull;  // any NaN -> top
-  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
-}
-__host__ __device__ __forceinline__ double f64_from_order_key(uint64_t k) {
-  uint64_t b = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
-  if (k == ~0ull) b = 0x7FF8000000000000ull;
-  double d;
-  memcpy(&d, &b, 8);
-  return d;
 }
 
 }  // namespace bolt

@@ -1,0 +1,47 @@
+// jit.h — per-query fused row kernels generated as HIP source and compiled
+// with hipRTC (see jit.hip).
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "ops_common.h"
+
+namespace bolt {
+
+// The shape a generated kernel is specialised for (everything that changes
+// the emitted source; pointers and literals are arguments).
+struct JitSpec {
+  struct Col {
+    int dtype;
+    bool has_valid;
+  };
+  struct Term {
+    int col;
+    int op;
+  };
+  struct Agg {
+    int op;
+    int src;  // 0 column, 1 join carry, 2 COUNT(*)
+    int col;
+    int dtype;
+  };
+  bool probe = false;
+  int key_dtype = 0;
+  bool key_valid = false;
+  int carry_dtype = 0;
+  std::vector<Col> cols;
+  std::vector<Term> terms;
+  std::vector<Agg> aggs;
+};
+
+bool jit_enabled();
+std::string jit_row_agg_source(const JitSpec& sp);
+int jit_get(const std::string& src, hipFunction_t* fn);
+int jit_spec_from(const DevFilter& f, const AggList& aggs, bool probe, int key_dtype, bool key_valid, int carry_dtype,
+                  JitSpec* sp, JitLits* lits, JitCols* cols);
+int jit_launch_row_agg(hipFunction_t fn, unsigned grid, hipStream_t s, const uint4* tbl, uint64_t mask,
+                       const uint32_t* flags, int64_t n, const void* key, const uint8_t* kvalid, const JitLits& lits,
+                       const JitCols& cols, Acc* parts);
+
+}  // namespace bolt

@@ -19,21 +19,13 @@
 // the 288 GB of HBM makes the larger table (1 GiB for 16M build rows) cheap.
 #include <cstdlib>
 
+#include "jit.h"
 #include "reduce_common.h"
 
 namespace bolt {
 
 constexpr int kBuildBlock = 256;
 constexpr int kProbeBlock = 256;
-
-__host__ __device__ __forceinline__ uint64_t fmix64(uint64_t x) {
-  x ^= x >> 33;
-  x *= 0xff51afd7ed558ccdull;
-  x ^= x >> 33;
-  x *= 0xc4ceb9fe1a85ec53ull;
-  x ^= x >> 33;
-  return x;
-}
 
 __device__ __forceinline__ int64_t load_key(const void* p, int dtype, int64_t i) {
   return dtype == BOLT_I64 ? static_cast<const int64_t*>(p)[i] : (int64_t) static_cast<const int32_t*>(p)[i];
@@ -379,17 +371,39 @@ extern "C" int bolt_join_probe_aggregate(bolt_stream_t stream, const bolt_join_t
   BOLT_TRY(make_agg_list(aggs, n_aggs, probe_cols, n_probe_cols, t->carry_dtype, &all));
   if (n_aggs == 0) return BOLT_OK;
   unsigned grid = probe_grid(key.length);
-  int n_chunks = (n_aggs + 3) / 4;
   DevBuf<Acc> parts, outs;
-  BOLT_TRY(parts.alloc((size_t)grid * 4 * n_chunks, s));
   BOLT_TRY(outs.alloc((size_t)n_aggs, s));
-  for (int c = 0; c < n_chunks; c++) {
-    int first = c * 4, na = n_aggs - first < 4 ? n_aggs - first : 4;
-    AggList chunk = agg_chunk(all, first, na);
-    Acc* p = parts.ptr + (size_t)c * grid * 4;
-    if (key.dtype == BOLT_I64) BOLT_TRY(launch_probe_agg<int64_t>(na, t, key, f, chunk, grid, p, s));
-    else BOLT_TRY(launch_probe_agg<int32_t>(na, t, key, f, chunk, grid, p, s));
-    BOLT_TRY(launch_finish(na, p, (int)grid, chunk, outs.ptr + first, s));
+  bool done = false;
+  if (jit_enabled() && key.length > 0) {
+    JitSpec sp;
+    JitLits lits;
+    JitCols cols;
+    hipFunction_t fn;
+    if (jit_spec_from(f, all, true, key.dtype == BOLT_I64 ? BOLT_I64 : BOLT_I32, key.validity != nullptr,
+                      t->carry_dtype, &sp, &lits, &cols) == BOLT_OK &&
+        jit_get(jit_row_agg_source(sp), &fn) == BOLT_OK) {
+      BOLT_TRY(parts.alloc((size_t)grid * n_aggs, s));
+      BOLT_TRY(jit_launch_row_agg(fn, grid, s, t->tbl, t->mask, t->flags, key.length, key.data, key.validity, lits, cols,
+                                  parts.ptr));
+      for (int first = 0; first < n_aggs; first += 4) {
+        int na = n_aggs - first < 4 ? n_aggs - first : 4;
+        BOLT_TRY(launch_finish(na, parts.ptr, (int)grid, n_aggs, first, agg_chunk(all, first, na), outs.ptr + first, s));
+      }
+      done = true;
+    }
+    clear_error();
+  }
+  if (!done) {
+    int n_chunks = (n_aggs + 3) / 4;
+    BOLT_TRY(parts.alloc((size_t)grid * 4 * n_chunks, s));
+    for (int c = 0; c < n_chunks; c++) {
+      int first = c * 4, na = n_aggs - first < 4 ? n_aggs - first : 4;
+      AggList chunk = agg_chunk(all, first, na);
+      Acc* p = parts.ptr + (size_t)c * grid * 4;
+      if (key.dtype == BOLT_I64) BOLT_TRY(launch_probe_agg<int64_t>(na, t, key, f, chunk, grid, p, s));
+      else BOLT_TRY(launch_probe_agg<int32_t>(na, t, key, f, chunk, grid, p, s));
+      BOLT_TRY(launch_finish(na, p, (int)grid, na, 0, chunk, outs.ptr + first, s));
+    }
   }
   Acc host[kMaxAggs];
   uint32_t fl[2];

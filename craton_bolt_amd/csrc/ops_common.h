@@ -40,12 +40,6 @@ __device__ __forceinline__ bool cmp_apply(int op, int c) {
   return false;
 }
 
-__host__ __device__ __forceinline__ double as_f64_raw(uint64_t u) {
-  double d;
-  memcpy(&d, &u, 8);
-  return d;
-}
-
 __device__ __forceinline__ bool eval_term(const DevTerm& t, int64_t row) {
   bool valid = bit_valid(t.validity, row);
   if (t.op == BOLT_CMP_IS_NULL) return !valid;
@@ -125,7 +119,7 @@ __device__ __forceinline__ void raw_value(Raw r, int dtype, int64_t& vi, int64_t
     case BOLT_I32: case BOLT_DATE32: vi = (int64_t)(int32_t)(uint32_t)r.lo; vf = (double)vi; break;
     case BOLT_I64: vi = (int64_t)r.lo; vf = (double)vi; break;
     case BOLT_F32: vi = 0; vf = (double)__uint_as_float((uint32_t)r.lo); break;
-    case BOLT_F64: vi = 0; vf = as_f64_raw(r.lo); break;
+    case BOLT_F64: vi = 0; vf = as_f64(r.lo); break;
     case BOLT_BOOL: vi = (int64_t)r.lo; vf = (double)vi; break;
     case BOLT_DEC128: vi = (int64_t)r.lo; vhi = r.hi; vf = 0; break;
     default: vi = 0; vf = 0;
@@ -180,28 +174,12 @@ __device__ __forceinline__ bool eval_filter_nt(const DevFilter& f, int64_t row) 
 int make_dev_filter(const bolt_filter_t* f, const bolt_column_t* cols, int32_t n_cols, int64_t n_rows,
                     DevFilter* out);
 
-// ---------------------------------------------------------------------------
-// Exact accumulators, three 64-bit words each:
-//   integer / decimal SUM : a,b = 128-bit two's-complement (lo, hi); bit 63 of
-//                           n flags a carry out of bit 127.  Integer SUM is
-//                           range-checked against i64 at finalize, so the
-//                           overflow error does not depend on row order;
-//   float SUM / AVG       : a,b = Neumaier pair (s, c) in f64, merged with
-//                           TwoSum;
-//   MIN / MAX             : a = an order-preserving u64 image, so the merge
-//                           is one unsigned min / max (ints: sign flip;
-//                           floats: the NaN-largest total order of
-//                           src/exec/aggregate.rs:2165);
-//   VAR / STDDEV          : a,b = Welford (mean, M2), merged with Chan's
-//                           formula (src/exec/welford.rs:84);
-//   n                     : values seen (COUNT).
-// ---------------------------------------------------------------------------
-struct Acc {
-  uint64_t a;
-  uint64_t b;
-  uint64_t n;  // count in bits 0..62, overflow flag in bit 63
-};
-constexpr uint64_t kOvfBit = 1ull << 63;
+// Accumulators (Acc, acc_update, acc_merge, wave_reduce_acc, ...) live in
+// device_acc.h, shared with the kernels generated per query.
+static_assert(AG_SUM == BOLT_AGG_SUM && AG_STDDEV_SAMP == BOLT_AGG_STDDEV_SAMP && AG_COUNT_STAR == BOLT_AGG_COUNT_STAR,
+              "aggregate codes");
+static_assert(DT_I32 == BOLT_I32 && DT_F64 == BOLT_F64 && DT_DEC128 == BOLT_DEC128 && DT_DATE32 == BOLT_DATE32,
+              "dtype codes");
 
 struct AggDesc {
   const void* data;
@@ -215,126 +193,6 @@ struct AggList {
   AggDesc a[kMaxAggs];
   int32_t n;
 };
-
-__host__ __device__ __forceinline__ uint64_t mm_identity(int op) { return op == BOLT_AGG_MIN ? ~0ull : 0ull; }
-__host__ __device__ __forceinline__ double as_f64(uint64_t u) { double d; memcpy(&d, &u, 8); return d; }
-__host__ __device__ __forceinline__ uint64_t as_u64(double d) { uint64_t u; memcpy(&u, &d, 8); return u; }
-__host__ __device__ __forceinline__ uint64_t acc_count(const Acc& a) { return a.n & ~kOvfBit; }
-
-__device__ __forceinline__ void acc_init(Acc& a, int op) {
-  a.a = (op == BOLT_AGG_MIN || op == BOLT_AGG_MAX) ? mm_identity(op) : 0;
-  a.b = 0;
-  a.n = 0;
-}
-
-__device__ __forceinline__ void add128(Acc& a, uint64_t lo, int64_t hi) {
-  uint64_t nlo = a.a + lo;
-  uint64_t carry = nlo < a.a;
-  int64_t t, h;
-  // signed 128-bit overflow: exactly one of the two partial adds of the high
-  // word overflows
-  bool o1 = __builtin_add_overflow((int64_t)a.b, hi, &t);
-  bool o2 = __builtin_add_overflow(t, (int64_t)carry, &h);
-  a.n |= (uint64_t)(o1 ^ o2) << 63;
-  a.a = nlo;
-  a.b = (uint64_t)h;
-}
-
-__device__ __forceinline__ void neum_add(Acc& a, double v) {
-  double s = as_f64(a.a), c = as_f64(a.b);
-  double t = s + v;
-  if (fabs(s) >= fabs(v)) c += (s - t) + v;
-  else c += (v - t) + s;
-  a.a = as_u64(t);
-  a.b = as_u64(c);
-}
-
-__device__ __forceinline__ void welford_push(Acc& a, double x) {
-  a.n += 1;
-  double mean = as_f64(a.a), m2 = as_f64(a.b);
-  double d = x - mean;
-  mean += d / (double)acc_count(a);
-  m2 += d * (x - mean);
-  a.a = as_u64(mean);
-  a.b = as_u64(m2);
-}
-
-// Fold one (non-NULL) value into `a` for aggregate `op`: `v_i` is the
-// integer value (decimal low word), `v_hi` the decimal high word, `v_f` the
-// value as f64.
-__device__ __forceinline__ void acc_update(Acc& a, int op, int dtype, int64_t v_i, int64_t v_hi, double v_f) {
-  switch (op) {
-    case BOLT_AGG_SUM:
-      if (dtype == BOLT_F32 || dtype == BOLT_F64) neum_add(a, v_f);
-      else add128(a, (uint64_t)v_i, dtype == BOLT_DEC128 ? v_hi : (v_i >> 63));
-      a.n++;
-      break;
-    case BOLT_AGG_AVG: neum_add(a, v_f); a.n++; break;
-    case BOLT_AGG_COUNT: case BOLT_AGG_COUNT_STAR: a.n++; break;
-    case BOLT_AGG_MIN: case BOLT_AGG_MAX: {
-      uint64_t k;
-      if (dtype == BOLT_F32 || dtype == BOLT_F64) k = f64_order_key(v_f);
-      else k = (uint64_t)v_i ^ 0x8000000000000000ull;
-      a.a = op == BOLT_AGG_MIN ? (k < a.a ? k : a.a) : (k > a.a ? k : a.a);
-      a.n++;
-      break;
-    }
-    default: welford_push(a, v_f); break;
-  }
-}
-
-// Merge `b` into `a` (both for aggregate `op`).
-__device__ __forceinline__ void acc_merge(Acc& a, const Acc& b, int op, int dtype) {
-  switch (op) {
-    case BOLT_AGG_SUM: case BOLT_AGG_AVG:
-      if (op == BOLT_AGG_SUM && !(dtype == BOLT_F32 || dtype == BOLT_F64)) {
-        add128(a, b.a, (int64_t)b.b);
-        a.n = ((a.n | b.n) & kOvfBit) | (acc_count(a) + acc_count(b));
-      } else {
-        // TwoSum of the leading parts, compensations added
-        double as = as_f64(a.a), bs = as_f64(b.a);
-        double s = as + bs;
-        double bp = s - as;
-        double err = (as - (s - bp)) + (bs - bp);
-        a.b = as_u64(as_f64(a.b) + as_f64(b.b) + err);
-        a.a = as_u64(s);
-        a.n += b.n;
-      }
-      break;
-    case BOLT_AGG_COUNT: case BOLT_AGG_COUNT_STAR: a.n += b.n; break;
-    case BOLT_AGG_MIN: a.a = b.a < a.a ? b.a : a.a; a.n += b.n; break;
-    case BOLT_AGG_MAX: a.a = b.a > a.a ? b.a : a.a; a.n += b.n; break;
-    default: {
-      uint64_t nb_ = acc_count(b), na_ = acc_count(a);
-      if (nb_ == 0) break;
-      if (na_ == 0) { a = b; break; }
-      double na = (double)na_, nb = (double)nb_, n = na + nb;
-      double ma = as_f64(a.a), mb = as_f64(b.a);
-      double d = mb - ma;
-      a.a = as_u64(ma + d * (nb / n));
-      a.b = as_u64(as_f64(a.b) + as_f64(b.b) + d * d * (na * nb / n));
-      a.n += b.n;
-      break;
-    }
-  }
-}
-
-__device__ __forceinline__ Acc shfl_xor_acc(const Acc& a, int m) {
-  Acc r;
-  r.a = __shfl_xor(a.a, m);
-  r.b = __shfl_xor(a.b, m);
-  r.n = __shfl_xor(a.n, m);
-  return r;
-}
-
-// Reduce `a` across the 64 lanes of the wave (result valid in every lane).
-__device__ __forceinline__ void wave_reduce_acc(Acc& a, int op, int dtype) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) {
-    Acc b = shfl_xor_acc(a, m);
-    acc_merge(a, b, op, dtype);
-  }
-}
 
 // Host-side: turn a final Acc into a scalar result following the output
 // dtype rules of src/plan/logical_plan.rs:2126,2191.

@@ -35,21 +35,26 @@ __device__ __forceinline__ void block_reduce_store(Acc (&acc)[NA], const AggList
 
 // One block folds `n_parts` per-block partials (layout [part][NA]) in a fixed
 // order, so float results do not depend on scheduling.
+// (layout [part][stride], this chunk's aggregates at offset `first`).
 template <int NA>
 __global__ __launch_bounds__(kReduceBlock) void finish_partials_kernel(const Acc* __restrict__ parts, int n_parts,
-                                                                      AggList aggs, Acc* __restrict__ out) {
+                                                                      int stride, int first, AggList aggs,
+                                                                      Acc* __restrict__ out) {
   Acc acc[NA];
 #pragma unroll
   for (int a = 0; a < NA; a++) acc_init(acc[a], aggs.a[a].op);
   for (int p = threadIdx.x; p < n_parts; p += kReduceBlock) {
 #pragma unroll
-    for (int a = 0; a < NA; a++) acc_merge(acc[a], parts[(int64_t)p * NA + a], aggs.a[a].op, aggs.a[a].dtype);
+    for (int a = 0; a < NA; a++)
+      acc_merge(acc[a], parts[(int64_t)p * stride + first + a], aggs.a[a].op, aggs.a[a].dtype);
   }
   block_reduce_store<NA>(acc, aggs, out);
 }
 
-// Launch the finishing kernel for a chunk of NA aggregates.
-int launch_finish(int na, const Acc* parts, int n_parts, const AggList& aggs, Acc* out, hipStream_t s);
+// Launch the finishing kernel for a chunk of `na` (<= 4) aggregates whose
+// partials sit at offset `first` of rows of `stride` accumulators.
+int launch_finish(int na, const Acc* parts, int n_parts, int stride, int first, const AggList& chunk, Acc* out,
+                  hipStream_t s);
 
 // Split an AggList into chunks of at most 4 (kernels are instantiated for
 // 1..4 aggregates so that accumulators stay in registers).

@@ -17,6 +17,14 @@ from gpu_util import both, dec_array, rel_close
 REL_TOL = 1e-9
 pytestmark = pytest.mark.gpu
 
+
+@pytest.fixture(autouse=True, params=["jit", "generic"])
+def kernel_path(request, monkeypatch):
+    """Every case runs through the per-query hipRTC kernel and through the
+    precompiled generic kernel (BOLT_JIT=0)."""
+    monkeypatch.setenv("BOLT_JIT", "1" if request.param == "jit" else "0")
+    return request.param
+
 ALL_OPS = [N.SUM, N.COUNT, N.COUNT_STAR, N.MIN, N.MAX, N.AVG, N.VAR_POP, N.VAR_SAMP, N.STDDEV_POP, N.STDDEV_SAMP]
 FLOAT_RESULT = {N.AVG, N.VAR_POP, N.VAR_SAMP, N.STDDEV_POP, N.STDDEV_SAMP}
 

@@ -20,6 +20,14 @@ REL_TOL = 1e-9
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(autouse=True, params=["jit", "generic"])
+def kernel_path(request, monkeypatch):
+    """Every case runs through the per-query hipRTC kernel and through the
+    precompiled generic kernel (BOLT_JIT=0)."""
+    monkeypatch.setenv("BOLT_JIT", "1" if request.param == "jit" else "0")
+    return request.param
+
+
 def _canon(a, b):
     order = np.lexsort((b, a))
     return np.stack([a[order], b[order]])
