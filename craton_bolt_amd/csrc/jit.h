@@ -27,6 +27,7 @@ struct JitSpec {
     int dtype;
   };
   bool probe = false;
+  int unroll = 4;  // rows per lane per block iteration (1..8)
   int key_dtype = 0;
   bool key_valid = false;
   int carry_dtype = 0;

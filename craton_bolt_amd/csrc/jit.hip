@@ -151,102 +151,133 @@ std::string jit_row_agg_source(const JitSpec& sp) {
     if (sp.cols[c].has_valid) o << "  const unsigned char* __restrict__ v" << c << " = cols.v[" << c << "];\n";
   }
   const int na = (int)sp.aggs.size();
+  const int R = sp.unroll;
   for (int a = 0; a < na; a++) o << "  Acc acc" << a << "; acc_init(acc" << a << ", " << sp.aggs[a].op << ");\n";
   if (sp.probe) o << "  const bool first_only = flags[0] == 0u;\n";
-  o << "  const long long stride = (long long)gridDim.x * 256;\n"
-       "  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {\n";
-  if (sp.probe)
-    o << "    const " << ctype(sp.key_dtype) << " kraw = __builtin_nontemporal_load((const " << ctype(sp.key_dtype)
-      << "*)keyp + i);\n";
-  for (int c = 0; c < nc; c++) {
-    if (sp.cols[c].dtype == BOLT_DEC128)
-      o << "    const unsigned long long x" << c << "_lo = c" << c << "[2 * i], x" << c << "_hi = c" << c
-        << "[2 * i + 1];\n";
-    else if (sp.cols[c].dtype == BOLT_BOOL)
-      o << "    const unsigned char x" << c << " = c" << c << "[i];\n";
-    else
-      o << "    const " << ctype(sp.cols[c].dtype) << " x" << c << " = __builtin_nontemporal_load(c" << c << " + i);\n";
-  }
-  o << "    bool keep = true;\n";
-  if (sp.probe && sp.key_valid) o << "    keep = keep && vbit(kvalid, i);\n";
-  for (int t = 0; t < (int)sp.terms.size(); t++) {
-    const JitSpec::Term& tm = sp.terms[t];
-    const int c = tm.col;
-    const bool hv = sp.cols[c].has_valid;
-    const int dt = sp.cols[c].dtype;
-    std::string valid = hv ? "vbit(v" + std::to_string(c) + ", i)" : "true";
-    std::string expr;
-    if (tm.op == BOLT_CMP_IS_NULL) expr = "!" + valid;
-    else if (tm.op == BOLT_CMP_IS_NOT_NULL) expr = valid;
-    else {
-      std::ostringstream e;
-      if (dt == BOLT_F32 || dt == BOLT_F64)
-        e << "((double)x" << c << " " << cmp_op(tm.op) << " lits.f[" << t << "])";
-      else if (dt == BOLT_DEC128)
-        e << "(cmp128((long long)x" << c << "_hi, x" << c << "_lo, lits.hi[" << t << "], (unsigned long long)lits.i["
-          << t << "]) " << cmp_op(tm.op) << " 0)";
+  // Each block iteration covers R * 256 rows; lane t owns rows base + r * 256
+  // (r < R), so every load is a coalesced 256-row stripe and the R rows' first
+  // table reads are independent and in flight together.
+  o << "  const long long tile = 256LL * " << R << ";\n"
+       "  for (long long base = (long long)blockIdx.x * tile + threadIdx.x; base < n; base += (long long)gridDim.x * tile) {\n";
+  for (int r = 0; r < R; r++) {
+    const std::string i = "i" + std::to_string(r);
+    const std::string in = "in" + std::to_string(r);
+    o << "    const long long " << i << " = base + " << 256 * r << ";\n"
+      << "    const bool " << in << " = " << i << " < n;\n";
+    if (sp.probe)
+      o << "    const " << ctype(sp.key_dtype) << " kraw" << r << " = " << in << " ? __builtin_nontemporal_load((const "
+        << ctype(sp.key_dtype) << "*)keyp + " << i << ") : 0;\n";
+    for (int c = 0; c < nc; c++) {
+      const std::string x = "x" + std::to_string(c) + "_" + std::to_string(r);
+      if (sp.cols[c].dtype == BOLT_DEC128)
+        o << "    const unsigned long long " << x << "_lo = " << in << " ? c" << c << "[2 * " << i << "] : 0, " << x
+          << "_hi = " << in << " ? c" << c << "[2 * " << i << " + 1] : 0;\n";
+      else if (sp.cols[c].dtype == BOLT_BOOL)
+        o << "    const unsigned char " << x << " = " << in << " ? c" << c << "[" << i << "] : 0;\n";
       else
-        e << "((long long)x" << c << " " << cmp_op(tm.op) << " lits.i[" << t << "])";
-      expr = hv ? "(" + valid + " && " + e.str() + ")" : e.str();
+        o << "    const " << ctype(sp.cols[c].dtype) << " " << x << " = " << in << " ? __builtin_nontemporal_load(c" << c
+          << " + " << i << ") : 0;\n";
     }
-    o << "    keep = keep && " << expr << ";\n";
   }
-  o << "    if (!keep) continue;\n";
-  // aggregate updates
-  std::ostringstream up;
-  for (int a = 0; a < na; a++) {
-    const JitSpec::Agg& g = sp.aggs[a];
-    const std::string A = "acc" + std::to_string(a);
-    if (g.op == BOLT_AGG_COUNT_STAR) {
-      up << "      " << A << ".n++;\n";
-      continue;
-    }
-    std::string V;
-    int dt = g.dtype;
-    std::string guard;
-    if (g.src == 1) {
-      V = dt == BOLT_F32 ? "(double)__uint_as_float(w)" : "(long long)(int)w";
-    } else {
-      V = "x" + std::to_string(g.col);
-      if (sp.cols[g.col].has_valid) guard = "vbit(v" + std::to_string(g.col) + ", i)";
-    }
-    std::ostringstream s;
-    switch (g.op) {
-      case BOLT_AGG_COUNT: s << A << ".n++;"; break;
-      case BOLT_AGG_SUM:
-        if (dt == BOLT_DEC128) s << "add128(" << A << ", " << V << "_lo, (long long)" << V << "_hi); " << A << ".n++;";
-        else if (dt == BOLT_F32 || dt == BOLT_F64) s << "neum_add(" << A << ", (double)" << V << "); " << A << ".n++;";
-        else s << "{ const long long v_ = (long long)" << V << "; add128(" << A << ", (unsigned long long)v_, v_ >> 63); "
-               << A << ".n++; }";
-        break;
-      case BOLT_AGG_AVG: s << "neum_add(" << A << ", (double)" << V << "); " << A << ".n++;"; break;
-      case BOLT_AGG_MIN:
-      case BOLT_AGG_MAX: {
-        const char* is_min = g.op == BOLT_AGG_MIN ? "true" : "false";
-        if (dt == BOLT_F32 || dt == BOLT_F64) s << "minmax_push(" << A << ", " << is_min << ", f64_order_key((double)" << V << "));";
-        else s << "minmax_push(" << A << ", " << is_min << ", (unsigned long long)(long long)" << V << " ^ 0x8000000000000000ull);";
-        break;
+  for (int r = 0; r < R; r++) {
+    const std::string i = "i" + std::to_string(r);
+    const std::string keep = "keep" + std::to_string(r);
+    o << "    bool " << keep << " = in" << r << ";\n";
+    if (sp.probe && sp.key_valid) o << "    " << keep << " = " << keep << " && vbit(kvalid, " << i << ");\n";
+    for (int t = 0; t < (int)sp.terms.size(); t++) {
+      const JitSpec::Term& tm = sp.terms[t];
+      const int c = tm.col;
+      const bool hv = sp.cols[c].has_valid;
+      const int dt = sp.cols[c].dtype;
+      const std::string x = "x" + std::to_string(c) + "_" + std::to_string(r);
+      std::string valid = hv ? "vbit(v" + std::to_string(c) + ", " + i + ")" : "true";
+      std::string expr;
+      if (tm.op == BOLT_CMP_IS_NULL) expr = "!" + valid;
+      else if (tm.op == BOLT_CMP_IS_NOT_NULL) expr = valid;
+      else {
+        std::ostringstream e;
+        if (dt == BOLT_F32 || dt == BOLT_F64)
+          e << "((double)" << x << " " << cmp_op(tm.op) << " lits.f[" << t << "])";
+        else if (dt == BOLT_DEC128)
+          e << "(cmp128((long long)" << x << "_hi, " << x << "_lo, lits.hi[" << t << "], (unsigned long long)lits.i["
+            << t << "]) " << cmp_op(tm.op) << " 0)";
+        else
+          e << "((long long)" << x << " " << cmp_op(tm.op) << " lits.i[" << t << "])";
+        expr = hv ? "(" + valid + " && " + e.str() + ")" : e.str();
       }
-      default: s << "welford_push(" << A << ", (double)" << V << ");"; break;
+      o << "    " << keep << " = " << keep << " && " << expr << ";\n";
     }
-    if (!guard.empty()) up << "      if (" << guard << ") { " << s.str() << " }\n";
-    else up << "      " << s.str() << "\n";
   }
+  // aggregate updates of row r
+  auto updates = [&](int r, const char* ind) {
+    std::ostringstream up;
+    const std::string i = "i" + std::to_string(r);
+    for (int a = 0; a < na; a++) {
+      const JitSpec::Agg& g = sp.aggs[a];
+      const std::string A = "acc" + std::to_string(a);
+      if (g.op == BOLT_AGG_COUNT_STAR) {
+        up << ind << A << ".n++;\n";
+        continue;
+      }
+      std::string V;
+      int dt = g.dtype;
+      std::string guard;
+      if (g.src == 1) {
+        V = dt == BOLT_F32 ? "(double)__uint_as_float(w)" : "(long long)(int)w";
+      } else {
+        V = "x" + std::to_string(g.col) + "_" + std::to_string(r);
+        if (sp.cols[g.col].has_valid) guard = "vbit(v" + std::to_string(g.col) + ", " + i + ")";
+      }
+      std::ostringstream s;
+      switch (g.op) {
+        case BOLT_AGG_COUNT: s << A << ".n++;"; break;
+        case BOLT_AGG_SUM:
+          if (dt == BOLT_DEC128) s << "add128(" << A << ", " << V << "_lo, (long long)" << V << "_hi); " << A << ".n++;";
+          else if (dt == BOLT_F32 || dt == BOLT_F64) s << "neum_add(" << A << ", (double)" << V << "); " << A << ".n++;";
+          else s << "{ const long long v_ = (long long)" << V << "; add128(" << A << ", (unsigned long long)v_, v_ >> 63); "
+                 << A << ".n++; }";
+          break;
+        case BOLT_AGG_AVG: s << "neum_add(" << A << ", (double)" << V << "); " << A << ".n++;"; break;
+        case BOLT_AGG_MIN:
+        case BOLT_AGG_MAX: {
+          const char* is_min = g.op == BOLT_AGG_MIN ? "true" : "false";
+          if (dt == BOLT_F32 || dt == BOLT_F64)
+            s << "minmax_push(" << A << ", " << is_min << ", f64_order_key((double)" << V << "));";
+          else
+            s << "minmax_push(" << A << ", " << is_min << ", (unsigned long long)(long long)" << V
+              << " ^ 0x8000000000000000ull);";
+          break;
+        }
+        default: s << "welford_push(" << A << ", (double)" << V << ");"; break;
+      }
+      if (!guard.empty()) up << ind << "if (" << guard << ") { " << s.str() << " }\n";
+      else up << ind << s.str() << "\n";
+    }
+    return up.str();
+  };
   if (sp.probe) {
-    o << "    const long long k = (long long)kraw;\n"
-         "    unsigned long long s = fmix64((unsigned long long)k) & mask;\n"
-         "    for (unsigned long long step = 0; step <= mask; step++) {\n"
-         "      const uint4 e = tbl[s];\n"
-         "      if (e.w == 0u) break;\n"
-         "      if ((long long)(((unsigned long long)e.y << 32) | e.x) == k) {\n"
-         "      const unsigned int w = e.z;\n"
-      << up.str()
-      << "      if (first_only) break;\n"
-         "      }\n"
-         "      s = (s + 1) & mask;\n"
-         "    }\n";
+    // first reads of all R rows, then each row's chain
+    for (int r = 0; r < R; r++)
+      o << "    const long long k" << r << " = (long long)kraw" << r << ";\n"
+        << "    unsigned long long s" << r << " = fmix64((unsigned long long)k" << r << ") & mask;\n"
+        << "    uint4 e" << r << " = make_uint4(0u, 0u, 0u, 0u);\n"
+        << "    if (keep" << r << ") e" << r << " = tbl[s" << r << "];\n";
+    for (int r = 0; r < R; r++) {
+      const std::string e = "e" + std::to_string(r), sv = "s" + std::to_string(r);
+      o << "    if (keep" << r << ") {\n"
+        << "      for (unsigned long long step = 0; " << e << ".w != 0u && step <= mask; step++) {\n"
+        << "        if ((long long)(((unsigned long long)" << e << ".y << 32) | " << e << ".x) == k" << r << ") {\n"
+        << "          const unsigned int w = " << e << ".z;\n"
+        << updates(r, "          ")
+        << "          if (first_only) break;\n"
+        << "        }\n"
+        << "        " << sv << " = (" << sv << " + 1) & mask;\n"
+        << "        " << e << " = tbl[" << sv << "];\n"
+        << "      }\n"
+        << "    }\n";
+    }
   } else {
-    o << "    {\n" << up.str() << "    }\n";
+    for (int r = 0; r < R; r++) o << "    if (keep" << r << ") {\n" << updates(r, "      ") << "    }\n";
   }
   o << "  }\n  __shared__ Acc sh[4];\n";
   for (int a = 0; a < na; a++)
@@ -265,6 +296,10 @@ int jit_spec_from(const DevFilter& f, const AggList& aggs, bool probe, int key_d
   sp->key_dtype = key_dtype;
   sp->key_valid = key_valid;
   sp->carry_dtype = carry_dtype;
+  if (const char* u = getenv("BOLT_JIT_UNROLL")) {
+    int v = atoi(u);
+    if (v >= 1 && v <= 8) sp->unroll = v;
+  }
   auto col_index = [&](const void* d, const uint8_t* v, int dt) -> int {
     for (int c = 0; c < (int)sp->cols.size(); c++)
       if (cols->d[c] == d && cols->v[c] == v && sp->cols[c].dtype == dt) return c;
