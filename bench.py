@@ -29,7 +29,9 @@ def _parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="join", choices=["join", "sort", "groupby", "groupby-zipf", "calib"])
+    p.add_argument("--workload", default="join", choices=["join", "sort", "groupby", "groupby-zipf", "calib", "batch"])
+    p.add_argument("--queries", type=int, default=512, help="queries of the batch workload")
+    p.add_argument("--no-autotune", action="store_true", help="batch workload: keep the default parameters")
     p.add_argument("--n", type=int, default=256 << 20, help="rows for the sort / groupby / calib workloads")
     p.add_argument("--n-build", type=int, default=16 << 20)
     p.add_argument("--n-probe", type=int, default=256 << 20)
@@ -259,6 +261,91 @@ def bench_calib(args) -> dict:
                  _roof(n * 8, ms_sum, "scalar_agg_kernel"))
 
 
+def bench_batch(args) -> dict:
+    """512 independent GROUP BY k, SUM(v_i64), SUM(v_f32), COUNT(*) queries
+    (rows log-uniform 1e4..4e6, Zipf(1.1) int32 keys over a log-uniform
+    16..1e6 cardinality): one graph-captured persistent batch versus the same
+    queries as 512 plain per-query bolt_groupby calls."""
+    import ctypes as C
+
+    from craton_bolt_amd import _native as N
+    from craton_bolt_amd import datagen as D
+    from craton_bolt_amd.device import DeviceColumn, Event, Stream, columns_array
+    from craton_bolt_amd.ops import GroupByBatch, _aggs
+
+    shapes = D.batch_shapes(args.queries)
+    t0 = time.perf_counter()
+    queries = []
+    total_rows = 0
+    for q, (n, card) in enumerate(shapes):
+        k, vi, vf = D.zipf_small(n, card, seed=D.SEED_BATCH + q)
+        queries.append(([DeviceColumn.from_numpy(k)], [DeviceColumn.from_numpy(vi), DeviceColumn.from_numpy(vf)]))
+        total_rows += n
+    print(f"[bench] generated + uploaded {args.queries} queries ({total_rows} rows) in {time.perf_counter() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    aggs = [(N.SUM, 0), (N.SUM, 1), (N.COUNT_STAR, 0)]
+    s = Stream()
+
+    # plain path: one bolt_groupby per query (each plans, sizes and extracts on its own)
+    ag = _aggs(aggs)
+    plain_args = [(columns_array(k), columns_array(i)) for k, i in queries]
+    L = N.lib()
+
+    def plain_all(collect=None):
+        for q, (ka, ia) in enumerate(plain_args):
+            tbl = C.c_void_p()
+            N.check(L.bolt_groupby(s.handle, ka, 1, ia, 2, ag, len(aggs), None, 0, C.byref(tbl)), "bolt_groupby")
+            if collect is not None:
+                nr = C.c_int64()
+                L.bolt_table_num_rows(tbl, C.byref(nr))
+                collect.append(nr.value)
+            L.bolt_table_release(tbl)
+        s.synchronize()
+
+    plain_groups = []
+    plain_all(plain_groups)
+    t0 = time.perf_counter()
+    plain_reps = max(1, min(3, args.steps))
+    for _ in range(plain_reps):
+        plain_all()
+    plain_ms = (time.perf_counter() - t0) * 1000.0 / plain_reps
+    print(f"[bench] plain: {plain_ms:.1f} ms per {args.queries} queries", file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    b = GroupByBatch(queries, aggs, stream=s)
+    create_ms = (time.perf_counter() - t0) * 1000.0
+    tuned = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tuned, tuned_ms = b.autotune(reps=2)
+        print(f"[bench] autotune: {tuned} {tuned_ms:.2f} ms ({time.perf_counter() - t0:.1f}s)", file=sys.stderr,
+              flush=True)
+    for _ in range(args.warmup):
+        b.run()
+    e0, e1 = Event(), Event()
+    t0 = time.perf_counter()
+    e0.record(s)
+    for _ in range(args.steps):
+        b.run()
+    e1.record(s)
+    e1.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1000.0 / args.steps
+    dev_ms = e0.elapsed_ms(e1) / args.steps
+    mism = sum(1 for q in range(args.queries) if b.num_groups(q) != plain_groups[q])
+    tune, arena = b.tune()
+    algo = total_rows * 16 + sum(plain_groups) * (4 + 8 + 4 + 8)
+    line = _line(args.queries / (wall_ms * 1e-3), "queries/s", args, wall_ms, "int64",
+                 {"workload": f"batched GROUP BY: {args.queries} queries x (GROUP BY int32 Zipf(1.1) key, SUM(int64), "
+                              f"SUM(float32), COUNT(*)), rows log-uniform 1e4..4e6, cardinality log-uniform 16..1e6",
+                  "total_rows": total_rows, "groups": int(sum(plain_groups)),
+                  "plain_ms_per_batch": plain_ms, "plain_queries_per_s": args.queries / (plain_ms * 1e-3),
+                  "speedup_vs_plain": plain_ms / wall_ms, "group_count_mismatches": mism,
+                  "create_ms": create_ms, "tune": tune, "arena_bytes": arena},
+                 _roof(algo, dev_ms, "whole batched pipeline (sketch, plan, aggregate, extract)"))
+    b.release()
+    return line
+
+
 def main():
     args = _parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -273,6 +360,8 @@ def main():
         line = bench_groupby(args, zipf=False)
     elif args.workload == "groupby-zipf":
         line = bench_groupby(args, zipf=True)
+    elif args.workload == "batch":
+        line = bench_batch(args)
     else:
         line = bench_calib(args)
     print(json.dumps(line), flush=True)

@@ -78,6 +78,10 @@ class BoltFilter(C.Structure):
     _fields_ = [("terms", C.POINTER(BoltPredTerm)), ("n_terms", C.c_int32)]
 
 
+class BoltGbTune(C.Structure):
+    _fields_ = [("block", C.c_int32), ("tile_rows", C.c_int32), ("lds_slots", C.c_int32), ("grid", C.c_int32)]
+
+
 # ---- errors ------------------------------------------------------------------
 
 class BoltError(RuntimeError):
@@ -206,6 +210,16 @@ _SIGS = {
                                       C.c_void_p]),
     "bolt_sort_indices": (C.c_int, [C.c_void_p, C.POINTER(BoltColumn), C.c_int32, C.POINTER(C.c_int32),
                                     C.POINTER(C.c_int32), C.c_void_p]),
+    # batched GROUP BY (bolt_batch.h)
+    "bolt_gb_batch_create": (C.c_int, [C.c_void_p, C.POINTER(BoltColumn), C.c_int32, C.POINTER(BoltColumn),
+                                       C.c_int32, C.c_int32, C.POINTER(BoltAgg), C.c_int32, C.POINTER(BoltGbTune),
+                                       C.POINTER(C.c_void_p)]),
+    "bolt_gb_batch_run": (C.c_int, [C.c_void_p]),
+    "bolt_gb_batch_num_groups": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64)]),
+    "bolt_gb_batch_column": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(BoltColumn)]),
+    "bolt_gb_batch_get_tune": (C.c_int, [C.c_void_p, C.POINTER(BoltGbTune), C.POINTER(C.c_int64)]),
+    "bolt_gb_batch_release": (None, [C.c_void_p]),
+    "bolt_gb_batch_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(BoltGbTune), C.POINTER(C.c_float)]),
 }
 
 

@@ -1,0 +1,351 @@
+// groupby_dev.h — device-side pieces of hash GROUP BY shared by the
+// per-query operator (groupby.hip) and the batched form (batch.hip): key
+// encoding, the accumulator word layout, LDS / HBM slot updates with exact
+// integer atomics, global open-addressing table and finalisation.
+#pragma once
+
+#include "ops_common.h"
+
+namespace bolt {
+
+constexpr uint64_t kEmpty = ~0ull;
+constexpr int kGbBlock = 1024;
+constexpr int kHllBits = 11;
+constexpr int kHllRegs = 1 << kHllBits;
+constexpr int kLdsProbes = 16;
+constexpr int kGlobalProbes = 1 << 16;
+
+enum Kind { K_ISUM = 1, K_DSUM, K_FSUM, K_AVG, K_COUNT, K_MINMAX, K_VAR };
+
+struct GAgg {
+  const void* data;
+  const uint8_t* validity;
+  int32_t op;
+  int32_t dtype;
+  int32_t kind;
+  int32_t w;   // first word
+  int32_t nw;  // count word (-1 = none)
+  int32_t pad;
+};
+struct GPlan {
+  GAgg a[kMaxAggs];
+  int32_t n;
+  int32_t words;
+};
+
+struct KeySpec {
+  const void* d0;
+  const uint8_t* v0;
+  const void* d1;
+  const uint8_t* v1;
+  int32_t t0, t1;
+  int32_t n_keys;
+  int32_t null_group;
+};
+
+// ---- keys ---------------------------------------------------------------------
+
+__device__ __forceinline__ uint64_t key_bits(const void* d, int dt, int64_t row) {
+  switch (dt) {
+    case BOLT_I32: case BOLT_DATE32: return (uint64_t)static_cast<const uint32_t*>(d)[row];
+    case BOLT_I64: return static_cast<const uint64_t*>(d)[row];
+    case BOLT_F32: {
+      uint32_t b = static_cast<const uint32_t*>(d)[row];
+      if ((b & 0x7FFFFFFFu) > 0x7F800000u) b = 0x7FC00000u;  // NaN -> canonical
+      else if ((b & 0x7FFFFFFFu) == 0) b = 0;                // -0.0 -> +0.0
+      return b;
+    }
+    case BOLT_F64: {
+      uint64_t b = static_cast<const uint64_t*>(d)[row];
+      if ((b & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull) b = 0x7FF8000000000000ull;
+      else if ((b & 0x7FFFFFFFFFFFFFFFull) == 0) b = 0;
+      return b;
+    }
+  }
+  return 0;
+}
+
+// 0 = drop the row, 1 = regular key in *key, 2 = the NULL group
+__device__ __forceinline__ int row_key(const KeySpec& ks, int64_t row, uint64_t* key) {
+  bool v0 = bit_valid(ks.v0, row);
+  if (ks.n_keys == 1) {
+    if (!v0) return ks.null_group ? 2 : 0;
+    *key = key_bits(ks.d0, ks.t0, row);
+    return 1;
+  }
+  if (!v0 || !bit_valid(ks.v1, row)) return 0;
+  *key = (key_bits(ks.d0, ks.t0, row) << 32) | key_bits(ks.d1, ks.t1, row);
+  return 1;
+}
+
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return x;
+}
+
+// HyperLogLog register update from an independent hash of the key.
+__device__ __forceinline__ void hll_add(uint32_t* regs, uint64_t key) {
+  uint64_t h = mix64(key ^ 0x9E3779B97F4A7C15ull);
+  uint32_t idx = (uint32_t)(h >> (64 - kHllBits));
+  uint64_t rest = (h << kHllBits) | (1ull << (kHllBits - 1));
+  uint32_t rho = (uint32_t)__clzll(rest) + 1;
+  atomicMax(&regs[idx], rho);
+}
+
+// ---- accumulator updates -------------------------------------------------------
+
+// Carry-propagating 128-bit add: the carry out of the low word is computed
+// from the value the atomic returned, so concurrent adds compose exactly.
+__device__ __forceinline__ void atomic_add128(unsigned long long* lo, unsigned long long* hi, uint64_t vlo,
+                                              uint64_t vhi) {
+  unsigned long long old = atomicAdd(lo, (unsigned long long)vlo);
+  uint64_t carry = (old + vlo) < old;
+  uint64_t add_hi = vhi + carry;
+  if (add_hi) atomicAdd(hi, (unsigned long long)add_hi);
+}
+
+// 192-bit add of a sign-extended 128-bit value (Decimal128 SUM): detects
+// overflow of the i128 result exactly at finalize.
+__device__ __forceinline__ void atomic_add192(unsigned long long* w, size_t stride, uint64_t vlo, uint64_t vhi,
+                                              uint64_t vtop) {
+  unsigned long long old = atomicAdd(&w[0], (unsigned long long)vlo);
+  uint64_t c1 = (old + vlo) < old;
+  uint64_t mid_add = vhi + c1;
+  uint64_t ca = (c1 && vhi == ~0ull) ? 1 : 0;  // carry out of vhi + c1
+  uint64_t c2 = 0;
+  if (mid_add) {
+    unsigned long long old2 = atomicAdd(&w[stride], (unsigned long long)mid_add);
+    c2 = (old2 + mid_add) < old2;
+  }
+  uint64_t top_add = vtop + ca + c2;
+  if (top_add) atomicAdd(&w[2 * stride], (unsigned long long)top_add);
+}
+
+// Fold row `row`'s inputs into slot `slot` of a word array with `stride`
+// words per field (LDS or global: the same atomics, address space inferred).
+__device__ __forceinline__ void update_slot(const GPlan& P, unsigned long long* words, size_t stride, uint32_t slot,
+                                            int64_t row) {
+  for (int a = 0; a < P.n; a++) {
+    const GAgg& g = P.a[a];
+    unsigned long long* w = words + (size_t)g.w * stride + slot;
+    if (g.op == BOLT_AGG_COUNT_STAR) { atomicAdd(w, 1ull); continue; }
+    if (!bit_valid(g.validity, row)) continue;
+    if (g.kind == K_COUNT) { atomicAdd(w, 1ull); continue; }
+    if (g.nw >= 0) atomicAdd(words + (size_t)g.nw * stride + slot, 1ull);
+    switch (g.kind) {
+      case K_ISUM: {
+        int64_t v = g.dtype == BOLT_I64 ? static_cast<const int64_t*>(g.data)[row]
+                                        : (int64_t) static_cast<const int32_t*>(g.data)[row];
+        atomic_add128(w, w + stride, (uint64_t)v, (uint64_t)(v >> 63));
+        break;
+      }
+      case K_DSUM: {
+        const uint64_t* p = static_cast<const uint64_t*>(g.data) + 2 * row;
+        uint64_t lo = p[0], hi = p[1];
+        atomic_add192(w, stride, lo, hi, (uint64_t)((int64_t)hi >> 63));
+        break;
+      }
+      case K_FSUM:
+      case K_AVG:
+      case K_VAR: {
+        double v = g.dtype == BOLT_F64 ? static_cast<const double*>(g.data)[row]
+                 : g.dtype == BOLT_F32 ? (double)static_cast<const float*>(g.data)[row]
+                 : g.dtype == BOLT_I64 ? (double)static_cast<const int64_t*>(g.data)[row]
+                                       : (double)static_cast<const int32_t*>(g.data)[row];
+        unsafeAtomicAdd(reinterpret_cast<double*>(w), v);
+        break;
+      }
+      case K_MINMAX: {
+        uint64_t k;
+        switch (g.dtype) {
+          case BOLT_F64: k = f64_order_key(static_cast<const double*>(g.data)[row]); break;
+          case BOLT_F32: k = f64_order_key((double)static_cast<const float*>(g.data)[row]); break;
+          case BOLT_I64: k = (uint64_t) static_cast<const int64_t*>(g.data)[row] ^ 0x8000000000000000ull; break;
+          default: k = (uint64_t)(int64_t) static_cast<const int32_t*>(g.data)[row] ^ 0x8000000000000000ull; break;
+        }
+        if (g.op == BOLT_AGG_MIN) atomicMin(w, (unsigned long long)k);
+        else atomicMax(w, (unsigned long long)k);
+        break;
+      }
+    }
+  }
+}
+
+// Merge the words of a partial record (one value per field, `pstride` apart)
+// into a slot of the global table.
+__device__ __forceinline__ void merge_slot(const GPlan& P, unsigned long long* words, size_t stride, uint32_t slot,
+                                           const unsigned long long* pw, size_t pstride) {
+  for (int a = 0; a < P.n; a++) {
+    const GAgg& g = P.a[a];
+    unsigned long long* w = words + (size_t)g.w * stride + slot;
+    const unsigned long long* q = pw + (size_t)g.w * pstride;
+    if (g.nw >= 0 && g.nw != g.w) {
+      unsigned long long c = pw[(size_t)g.nw * pstride];
+      if (c) atomicAdd(words + (size_t)g.nw * stride + slot, c);
+    }
+    switch (g.kind) {
+      case K_COUNT: if (q[0]) atomicAdd(w, q[0]); break;
+      case K_ISUM: atomic_add128(w, w + stride, q[0], q[pstride]); break;
+      case K_DSUM: atomic_add192(w, stride, q[0], q[pstride], q[2 * pstride]); break;
+      case K_FSUM: case K_AVG: case K_VAR: unsafeAtomicAdd(reinterpret_cast<double*>(w), as_f64(q[0])); break;
+      case K_MINMAX:
+        if (g.op == BOLT_AGG_MIN) atomicMin(w, q[0]);
+        else atomicMax(w, q[0]);
+        break;
+    }
+  }
+}
+
+__device__ __forceinline__ unsigned long long word_identity(const GPlan& P, int w) {
+  for (int a = 0; a < P.n; a++)
+    if (P.a[a].kind == K_MINMAX && P.a[a].w == w) return P.a[a].op == BOLT_AGG_MIN ? ~0ull : 0ull;
+  return 0ull;
+}
+
+// Global table insert: returns the slot of `key` (claiming one if absent), or
+// 0xFFFFFFFF when the probe bound is exceeded (the table is too small).
+__device__ __forceinline__ uint32_t global_insert(unsigned long long* keys, uint64_t cap_mask, uint64_t key) {
+  uint64_t s = mix64(key) & cap_mask;
+  for (int p = 0; p < kGlobalProbes; p++) {
+    unsigned long long cur = keys[s];
+    if (cur == key) return (uint32_t)s;
+    if (cur == kEmpty) {
+      unsigned long long prev = atomicCAS(&keys[s], kEmpty, (unsigned long long)key);
+      if (prev == kEmpty || prev == key) return (uint32_t)s;
+    }
+    s = (s + 1) & cap_mask;
+  }
+  return 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ uint32_t global_find(const unsigned long long* keys, uint64_t cap_mask, uint64_t key) {
+  uint64_t s = mix64(key) & cap_mask;
+  for (int p = 0; p < kGlobalProbes; p++) {
+    unsigned long long cur = keys[s];
+    if (cur == key) return (uint32_t)s;
+    if (cur == kEmpty) return 0xFFFFFFFFu;
+    s = (s + 1) & cap_mask;
+  }
+  return 0xFFFFFFFFu;
+}
+
+// Wave-aggregated append: returns this lane's position for lanes with `want`.
+__device__ __forceinline__ unsigned long long wave_append(bool want, unsigned long long* counter) {
+  uint64_t m = __ballot(want);
+  if (m == 0) return 0;
+  int lane = threadIdx.x & 63;
+  int leader = __builtin_ctzll(m);
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+  base = __shfl(base, leader);
+  uint64_t below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
+  return base + (unsigned long long)__popcll(below);
+}
+
+
+struct GTable {
+  unsigned long long* keys;   // [cap]
+  unsigned long long* words;  // [words][cap + 2]
+  uint32_t* used_special;     // [2]
+  uint32_t* err;              // [0] probe overflow, [1] lookback, [2] integer overflow
+  uint64_t cap;               // power of two
+};
+
+__device__ __forceinline__ int64_t gslot(const GTable& t, int kind_tag, uint64_t key) {
+  if (kind_tag == 1) { t.used_special[0] = 1; return (int64_t)t.cap; }
+  if (kind_tag == 2 || key == kEmpty) { t.used_special[1] = 1; return (int64_t)t.cap + 1; }
+  uint32_t s = global_insert(t.keys, t.cap - 1, key);
+  if (s == 0xFFFFFFFFu) { atomicOr(&t.err[0], 1u); return -1; }
+  return s;
+}
+
+struct OutCol {
+  uint8_t* data;
+  uint32_t* valid;  // zeroed bitmap words or nullptr
+  int32_t dtype;
+  int32_t pad;
+};
+struct OutSpec {
+  OutCol key[2];
+  OutCol agg[kMaxAggs];
+  unsigned long long* packed;  // packed key per group (for ordering), i64 order; NULL group = i64::MAX
+  uint32_t* slot_of;           // slot per output row
+};
+
+__device__ __forceinline__ void set_valid(uint32_t* bm, uint64_t pos) { atomicOr(&bm[pos >> 5], 1u << (pos & 31)); }
+
+__device__ __forceinline__ void write_key(const OutCol& c, uint64_t pos, uint64_t bits) {
+  if (c.dtype == BOLT_I64 || c.dtype == BOLT_F64) reinterpret_cast<uint64_t*>(c.data)[pos] = bits;
+  else reinterpret_cast<uint32_t*>(c.data)[pos] = (uint32_t)bits;
+}
+
+__device__ __forceinline__ void finalize_to(const GTable& t, const GAgg& g, const OutCol& c, size_t total, uint64_t s,
+                                            uint64_t pos) {
+  auto W = [&](int w) { return t.words[(size_t)w * total + s]; };
+  uint64_t n = g.nw >= 0 ? W(g.nw) : 1;
+  bool valid = true;
+  switch (g.kind) {
+    case K_COUNT: reinterpret_cast<int64_t*>(c.data)[pos] = (int64_t)W(g.w); break;
+    case K_ISUM: {
+      uint64_t lo = W(g.w);
+      int64_t hi = (int64_t)W(g.w + 1);
+      if (hi != ((int64_t)lo >> 63)) atomicOr(&t.err[2], 1u);
+      reinterpret_cast<int64_t*>(c.data)[pos] = (int64_t)lo;
+      valid = n > 0;
+      break;
+    }
+    case K_DSUM: {
+      uint64_t lo = W(g.w), mid = W(g.w + 1);
+      int64_t top = (int64_t)W(g.w + 2);
+      if (top != ((int64_t)mid >> 63)) atomicOr(&t.err[2], 2u);
+      reinterpret_cast<uint64_t*>(c.data)[2 * pos] = lo;
+      reinterpret_cast<uint64_t*>(c.data)[2 * pos + 1] = mid;
+      valid = n > 0;
+      break;
+    }
+    case K_FSUM: {
+      double v = as_f64(W(g.w));
+      if (g.dtype == BOLT_F32) reinterpret_cast<float*>(c.data)[pos] = (float)v;
+      else reinterpret_cast<double*>(c.data)[pos] = v;
+      valid = n > 0;
+      break;
+    }
+    case K_AVG: {
+      double v = n ? as_f64(W(g.w)) / (double)n : 0.0;
+      reinterpret_cast<double*>(c.data)[pos] = v;
+      valid = n > 0;
+      break;
+    }
+    case K_MINMAX: {
+      uint64_t k = W(g.w);
+      if (g.dtype == BOLT_F64) reinterpret_cast<double*>(c.data)[pos] = f64_from_order_key(k);
+      else if (g.dtype == BOLT_F32) reinterpret_cast<float*>(c.data)[pos] = (float)f64_from_order_key(k);
+      else if (g.dtype == BOLT_I64) reinterpret_cast<int64_t*>(c.data)[pos] = (int64_t)(k ^ 0x8000000000000000ull);
+      else reinterpret_cast<int32_t*>(c.data)[pos] = (int32_t)(int64_t)(k ^ 0x8000000000000000ull);
+      valid = n > 0;
+      break;
+    }
+    case K_VAR: {
+      bool samp = g.op == BOLT_AGG_VAR_SAMP || g.op == BOLT_AGG_STDDEV_SAMP;
+      double m2 = as_f64(W(g.w + 1));
+      valid = samp ? n > 1 : n > 0;
+      double var = valid ? m2 / (double)(samp ? n - 1 : n) : 0.0;
+      if (g.op == BOLT_AGG_STDDEV_POP || g.op == BOLT_AGG_STDDEV_SAMP) var = sqrt(var);
+      reinterpret_cast<double*>(c.data)[pos] = var;
+      break;
+    }
+  }
+  if (c.valid && valid) set_valid(c.valid, pos);
+}
+
+// ---- host helpers (groupby.hip) ----
+int make_gplan(const bolt_agg_t* aggs, int32_t n_aggs, const bolt_column_t* inputs, int32_t n_inputs, int64_t n,
+               GPlan* P, int32_t* out_dtype);
+double hll_estimate(const uint32_t* regs);
+uint64_t next_pow2(uint64_t x);
+
+}  // namespace bolt

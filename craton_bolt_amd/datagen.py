@@ -99,5 +99,19 @@ def zipf_small(n: int, card: int, s: float = 1.1, seed: int = SEED_BATCH):
     return key, vi, vf
 
 
+def batch_shapes(n_queries: int = 512, seed: int = SEED_BATCH) -> list[tuple[int, int]]:
+    """(rows, key cardinality) of each batched query: both log-uniform, rows over
+    [1e4, 4e6] and cardinality over [16, 1e6], from two draws of seed + q."""
+    import math
+
+    out = []
+    for q in range(n_queries):
+        u = (draws(seed + q, 2) >> np.uint64(11)).astype(np.float64) / 2.0 ** 53
+        n = int(round(math.exp(math.log(1e4) + u[0] * (math.log(4e6) - math.log(1e4)))))
+        c = int(round(math.exp(math.log(16) + u[1] * (math.log(1e6) - math.log(16)))))
+        out.append((n, c))
+    return out
+
+
 def scramble64(x: int) -> int:
     return int(lib().bolt_scramble64(x))

@@ -160,6 +160,79 @@ def _download(bc: N.BoltColumn, n: int):
     return vals, valid
 
 
+# ---- batched GROUP BY (bolt_batch.h) ----------------------------------------------------
+
+class GroupByBatch:
+    """Many GROUP BY queries with one aggregate list, run as one graph-captured
+    persistent pipeline.  queries: list of (key_columns, input_columns)."""
+
+    def __init__(self, queries: list[tuple[list[DeviceColumn], list[DeviceColumn]]], aggs: list[tuple[int, int]],
+                 tune: dict | None = None, stream: Stream | None = None):
+        self._h = C.c_void_p()
+        self.n_queries = len(queries)
+        self.n_keys = len(queries[0][0])
+        self.n_inputs = len(queries[0][1])
+        self.n_aggs = len(aggs)
+        self._cols = queries  # keep the device columns alive
+        ka = (N.BoltColumn * (self.n_queries * self.n_keys))()
+        ia = (N.BoltColumn * max(1, self.n_queries * self.n_inputs))()
+        for q, (ks, ins) in enumerate(queries):
+            if len(ks) != self.n_keys or len(ins) != self.n_inputs:
+                raise ValueError("every query needs the same number of key and input columns")
+            for j, c in enumerate(ks):
+                ka[q * self.n_keys + j] = c.c()
+            for j, c in enumerate(ins):
+                ia[q * self.n_inputs + j] = c.c()
+        t = None
+        if tune:
+            t = N.BoltGbTune(tune.get("block", 0), tune.get("tile_rows", 0), tune.get("lds_slots", 0),
+                             tune.get("grid", 0))
+        st = N.lib().bolt_gb_batch_create(_stream(stream), ka, self.n_keys, ia, self.n_inputs, self.n_queries,
+                                          _aggs(aggs), len(aggs), C.byref(t) if t else None, C.byref(self._h))
+        N.check(st, "bolt_gb_batch_create")
+
+    def run(self) -> None:
+        N.check(N.lib().bolt_gb_batch_run(self._h), "bolt_gb_batch_run")
+
+    def num_groups(self, q: int) -> int:
+        n = C.c_int64()
+        N.check(N.lib().bolt_gb_batch_num_groups(self._h, q, C.byref(n)))
+        return n.value
+
+    def result(self, q: int) -> dict:
+        """Query q's groups (hash order), in execute_groupby's dict layout."""
+        n = self.num_groups(q)
+        cols = []
+        for i in range(self.n_keys + self.n_aggs):
+            bc = N.BoltColumn()
+            N.check(N.lib().bolt_gb_batch_column(self._h, q, i, C.byref(bc)))
+            cols.append(_download(bc, n))
+        return {"n_groups": n, "keys": cols[: self.n_keys], "aggs": cols[self.n_keys:]}
+
+    def tune(self) -> tuple[dict, int]:
+        t = N.BoltGbTune()
+        nb = C.c_int64()
+        N.check(N.lib().bolt_gb_batch_get_tune(self._h, C.byref(t), C.byref(nb)))
+        return {"block": t.block, "tile_rows": t.tile_rows, "lds_slots": t.lds_slots, "grid": t.grid}, nb.value
+
+    def autotune(self, reps: int = 2) -> tuple[dict, float]:
+        t = N.BoltGbTune()
+        ms = C.c_float()
+        N.check(N.lib().bolt_gb_batch_autotune(self._h, reps, C.byref(t), C.byref(ms)), "bolt_gb_batch_autotune")
+        return {"block": t.block, "tile_rows": t.tile_rows, "lds_slots": t.lds_slots, "grid": t.grid}, ms.value
+
+    def release(self) -> None:
+        if self._h:
+            N.lib().bolt_gb_batch_release(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
 # ---- hash join (src/exec/gpu_join.rs:1583, src/exec/join.rs:169) --------------------
 
 class JoinTable:
