@@ -29,7 +29,10 @@ def _parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="join", choices=["join", "sort", "groupby", "groupby-zipf", "calib", "batch"])
+    p.add_argument("--workload", default="join", choices=["join", "sort", "groupby", "groupby-zipf", "calib", "batch", "serve"])
+    p.add_argument("--fact-rows", type=int, default=200_000_000, help="serve workload: fact table rows")
+    p.add_argument("--serve-queries", type=int, default=64, help="serve workload: queries per measurement")
+    p.add_argument("--slots", type=int, default=8, help="serve workload: concurrent queries")
     p.add_argument("--queries", type=int, default=512, help="queries of the batch workload")
     p.add_argument("--no-autotune", action="store_true", help="batch workload: keep the default parameters")
     p.add_argument("--n", type=int, default=256 << 20, help="rows for the sort / groupby / calib workloads")
@@ -346,6 +349,142 @@ def bench_batch(args) -> dict:
     return line
 
 
+def serve_mix(srv, n: int, seed: int):
+    """The served query mix (filter -> 1..3 joins -> GROUP BY 10K..100K groups ->
+    ORDER BY + LIMIT 100) with seeded parameters; fact columns are
+    fk1, fk2, fk3, price, qty, date."""
+    from craton_bolt_amd import _native as N
+    from craton_bolt_amd import datagen as D
+    from craton_bolt_amd.ops import Term
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        t = i % 3
+        d0 = D.STAR_DATE0 + int(rng.integers(0, 2557 - 120))
+        if t == 0:  # revenue by city, one quarter
+            spec = dict(terms=[Term(5, N.GE, d0), Term(5, N.LT, d0 + 90)], joins=[(1, 1)], group=[(0, 0)],
+                        aggs=[(N.SUM, 3), (N.SUM, 4), (N.COUNT_STAR, 0)], order_agg=0, limit=100, est_groups=10_000)
+        elif t == 1:  # brand ranking, two months of expensive items
+            p0 = int(rng.integers(100_000, 600_000))
+            spec = dict(terms=[Term(5, N.GE, d0), Term(5, N.LT, d0 + 60), Term(3, N.GE, p0)], joins=[(0, 0)],
+                        group=[(0, 0)], aggs=[(N.SUM, 3), (N.COUNT_STAR, 0)], order_agg=0, limit=100,
+                        est_groups=100_000)
+        else:  # category x zone over three joins, four months
+            spec = dict(terms=[Term(5, N.GE, d0), Term(5, N.LT, d0 + 120)], joins=[(0, 0), (1, 1), (2, 2)],
+                        group=[(2, 0), (1, 1)], aggs=[(N.COUNT_STAR, 0), (N.SUM, 4), (N.SUM, 3)], order_agg=2,
+                        limit=100, est_groups=25_000)
+        out.append(srv.query(**spec))
+    return out
+
+
+def bench_serve(args) -> dict:
+    """Served form over a device-resident star schema: plain capacity (one
+    query at a time, one launch per query) versus continuous batching with
+    `slots` overlapping queries, plus a Poisson-arrival sweep with p50 / p99."""
+    import threading
+
+    from craton_bolt_amd import _native as N
+    from craton_bolt_amd import datagen as D
+    from craton_bolt_amd.device import DeviceColumn
+    from craton_bolt_amd.ops import Server
+
+    t0 = time.perf_counter()
+    n = args.fact_rows
+    f = D.star_fact(n)
+    dts = [N.I32, N.I32, N.I32, N.DEC128, N.F64, N.DATE32]
+    fact = [DeviceColumn.from_numpy(f[k], dt) for k, dt in zip(["fk1", "fk2", "fk3", "price", "qty", "date"], dts)]
+    del f
+    dims = []
+    for d, nd in enumerate(D.STAR_DIMS):
+        k, attrs = D.star_dim(d, nd)
+        dims.append((DeviceColumn.from_numpy(k), [DeviceColumn.from_numpy(a) for a in attrs]))
+    print(f"[bench] star schema {n} fact rows ready in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    nq = args.serve_queries
+
+    def closed_loop(srv, queries):
+        t = time.perf_counter()
+        tickets = [srv.submit(q) for q in queries]
+        res = [srv.wait(tk) for tk in tickets]
+        return time.perf_counter() - t, res
+
+    plain = Server(fact, dims, slots=1, chunk_rows=n)
+    closed_loop(plain, serve_mix(plain, 3, seed=1))  # warm-up (module loads, pools)
+    pq = serve_mix(plain, max(6, nq // 4), seed=D.SEED_SERVE)
+    p_t, p_res = closed_loop(plain, pq)
+    plain_qps = len(pq) / p_t
+    print(f"[bench] plain: {plain_qps:.2f} queries/s", file=sys.stderr, flush=True)
+    plain.close()
+
+    srv = Server(fact, dims, slots=args.slots)
+    closed_loop(srv, serve_mix(srv, args.slots, seed=2))
+    sq = serve_mix(srv, nq, seed=D.SEED_SERVE)
+    steps0 = srv.stats()["steps"]
+    s_t, s_res = closed_loop(srv, sq)
+    laps = (srv.stats()["steps"] - steps0) / max(1, -(-n // (8 << 20)))
+    served_qps = nq / s_t
+    bad = sum(1 for r in s_res if r["status"] != 0)
+    print(f"[bench] served: {served_qps:.2f} queries/s ({bad} errors)", file=sys.stderr, flush=True)
+    # identical results: the plain queries against their served twins (same
+    # seeds -> same queries); floats within 1e-9 relative (atomic order)
+    def agree(a, b):
+        if a["status"] != b["status"] or a["n_groups"] != b["n_groups"] or len(a["rows"]) != len(b["rows"]):
+            return False
+        for ra, rb in zip(a["rows"], b["rows"]):
+            for x, y in zip(ra, rb):
+                if isinstance(y, float):
+                    if abs(x - y) > 1e-9 * max(abs(y), 1.0):
+                        return False
+                elif x != y:
+                    return False
+        return True
+
+    same = all(agree(a, b) for a, b in zip(p_res, s_res[: len(p_res)]))
+
+    sweep = []
+    rng = np.random.default_rng(D.SEED_SERVE + 7)
+    for frac in (0.25, 0.5, 0.75, 0.9):
+        qps = served_qps * frac
+        queries = serve_mix(srv, nq, seed=D.SEED_SERVE + int(frac * 100))
+        gaps = rng.exponential(1.0 / qps, size=nq)
+        tickets = [None] * nq
+        start = time.perf_counter()
+
+        def feeder():
+            at = start
+            for i, q in enumerate(queries):
+                at += gaps[i]
+                d = at - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                tickets[i] = srv.submit(q)
+
+        th = threading.Thread(target=feeder)
+        th.start()
+        th.join()
+        res = [srv.wait(tk) for tk in tickets]
+        lat = np.array([r["done_ms"] - r["submit_ms"] for r in res])
+        span = (max(r["done_ms"] for r in res) - min(r["submit_ms"] for r in res)) / 1000.0
+        sweep.append({"offered_qps": qps, "achieved_qps": nq / span, "p50_ms": float(np.percentile(lat, 50)),
+                      "p99_ms": float(np.percentile(lat, 99))})
+        print(f"[bench] offered {qps:.1f} qps: p50 {sweep[-1]['p50_ms']:.1f} ms p99 {sweep[-1]['p99_ms']:.1f} ms",
+              file=sys.stderr, flush=True)
+    st = srv.stats()
+    srv.close()
+    lat_closed = np.array([r["done_ms"] - r["submit_ms"] for r in s_res])
+    fact_bytes = n * (3 * 4 + 16 + 8 + 4)
+    line = _line(served_qps, "queries/s", args, s_t * 1000.0 / nq, "decimal128/int64/f64",
+                 {"workload": f"served star queries (filter -> 1..3 joins -> GROUP BY 10K..100K groups -> ORDER BY "
+                              f"+ LIMIT 100) over {n} fact rows, {args.slots} slots, continuous batching",
+                  "fact_rows": n, "slots": args.slots, "plain_queries_per_s": plain_qps,
+                  "speedup_vs_plain": served_qps / plain_qps, "saturation_p50_ms": float(np.percentile(lat_closed, 50)),
+                  "saturation_p99_ms": float(np.percentile(lat_closed, 99)), "poisson_sweep": sweep,
+                  "errors": bad, "plain_and_served_agree": bool(same), "server_stats": st},
+                 _roof(fact_bytes, s_t * 1000.0 / max(laps, 1e-9),
+                       "shared scan step kernel: fact bytes per lap / mean lap time at saturation"))
+    return line
+
+
 def main():
     args = _parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -362,6 +501,8 @@ def main():
         line = bench_groupby(args, zipf=True)
     elif args.workload == "batch":
         line = bench_batch(args)
+    elif args.workload == "serve":
+        line = bench_serve(args)
     else:
         line = bench_calib(args)
     print(json.dumps(line), flush=True)

@@ -82,6 +82,50 @@ class BoltGbTune(C.Structure):
     _fields_ = [("block", C.c_int32), ("tile_rows", C.c_int32), ("lds_slots", C.c_int32), ("grid", C.c_int32)]
 
 
+class BoltServeConfig(C.Structure):
+    _fields_ = [("slots", C.c_int32), ("hbm_budget", C.c_int64), ("chunk_rows", C.c_int64)]
+
+
+class BoltServeDim(C.Structure):
+    _fields_ = [("key", BoltColumn), ("attrs", C.POINTER(BoltColumn)), ("n_attrs", C.c_int32)]
+
+
+SERVE_MAX_DIMS, SERVE_MAX_AGGS, SERVE_MAX_TERMS, SERVE_MAX_LIMIT = 3, 4, 4, 1024
+
+
+class BoltServeQuery(C.Structure):
+    _fields_ = [
+        ("n_terms", C.c_int32),
+        ("terms", BoltPredTerm * SERVE_MAX_TERMS),
+        ("n_joins", C.c_int32),
+        ("join_fk", C.c_int32 * SERVE_MAX_DIMS),
+        ("join_dim", C.c_int32 * SERVE_MAX_DIMS),
+        ("n_group", C.c_int32),
+        ("group_dim", C.c_int32 * 2),
+        ("group_attr", C.c_int32 * 2),
+        ("n_aggs", C.c_int32),
+        ("aggs", BoltAgg * SERVE_MAX_AGGS),
+        ("order_agg", C.c_int32),
+        ("limit", C.c_int32),
+        ("est_groups", C.c_int64),
+    ]
+
+
+class BoltServeResult(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32),
+        ("n_rows", C.c_int32),
+        ("n_groups", C.c_int64),
+        ("hbm_bytes", C.c_int64),
+        ("n_cols", C.c_int32),
+        ("cells", C.POINTER(BoltScalar)),
+        ("submit_ms", C.c_double),
+        ("admit_ms", C.c_double),
+        ("done_ms", C.c_double),
+        ("error", C.c_char_p),
+    ]
+
+
 # ---- errors ------------------------------------------------------------------
 
 class BoltError(RuntimeError):
@@ -220,6 +264,16 @@ _SIGS = {
     "bolt_gb_batch_get_tune": (C.c_int, [C.c_void_p, C.POINTER(BoltGbTune), C.POINTER(C.c_int64)]),
     "bolt_gb_batch_release": (None, [C.c_void_p]),
     "bolt_gb_batch_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(BoltGbTune), C.POINTER(C.c_float)]),
+    # served form (bolt_serve.h)
+    "bolt_server_create": (C.c_int, [C.c_void_p, C.POINTER(BoltColumn), C.c_int32, C.POINTER(BoltServeDim), C.c_int32,
+                                     C.POINTER(BoltServeConfig), C.POINTER(C.c_void_p)]),
+    "bolt_server_submit": (C.c_int, [C.c_void_p, C.POINTER(BoltServeQuery), C.POINTER(C.c_int64)]),
+    "bolt_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.POINTER(BoltServeResult)]),
+    "bolt_server_result_release": (None, [C.c_void_p, C.c_int64]),
+    "bolt_server_now_ms": (C.c_double, [C.c_void_p]),
+    "bolt_server_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                    C.POINTER(C.c_int64)]),
+    "bolt_server_destroy": (None, [C.c_void_p]),
 }
 
 

@@ -230,3 +230,21 @@ void bolt_gen_zipf_small(uint64_t seed, int64_t n, int64_t card, double s, int32
   });
   zipf_free(&z);
 }
+
+/* ---- served-form star schema fact table: three int32 foreign keys uniform
+ *      over the dimension sizes, price Decimal128(12,2) = cents uniform over
+ *      [100, 1000000) as (lo, hi) int64 pairs, qty float64 uniform [1, 50),
+ *      Date32 uniform over 2557 days (7 years) from day `date0`.  Six draws
+ *      per row. ---- */
+void bolt_gen_star_fact(uint64_t seed, int64_t n, int64_t n1, int64_t n2, int64_t n3, int32_t date0, int32_t* fk1,
+                        int32_t* fk2, int32_t* fk3, int64_t* price, double* qty, int32_t* date) {
+  PARALLEL_ROWS(seed, 0, n, 6, {
+    fk1[r] = (int32_t)(xs_next(S) % (uint64_t)n1);
+    fk2[r] = (int32_t)(xs_next(S) % (uint64_t)n2);
+    fk3[r] = (int32_t)(xs_next(S) % (uint64_t)n3);
+    price[2 * r] = (int64_t)(100 + xs_next(S) % (1000000 - 100));
+    price[2 * r + 1] = 0;
+    qty[r] = 1.0 + xs_unit(S) * 49.0;
+    date[r] = date0 + (int32_t)(xs_next(S) % 2557);
+  });
+}

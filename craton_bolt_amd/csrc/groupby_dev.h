@@ -210,7 +210,7 @@ __device__ __forceinline__ unsigned long long word_identity(const GPlan& P, int 
 // 0xFFFFFFFF when the probe bound is exceeded (the table is too small).
 __device__ __forceinline__ uint32_t global_insert(unsigned long long* keys, uint64_t cap_mask, uint64_t key) {
   uint64_t s = mix64(key) & cap_mask;
-  for (int p = 0; p < kGlobalProbes; p++) {
+  for (uint64_t p = 0; p < (uint64_t)kGlobalProbes && p <= cap_mask; p++) {
     unsigned long long cur = keys[s];
     if (cur == key) return (uint32_t)s;
     if (cur == kEmpty) {
@@ -224,7 +224,7 @@ __device__ __forceinline__ uint32_t global_insert(unsigned long long* keys, uint
 
 __device__ __forceinline__ uint32_t global_find(const unsigned long long* keys, uint64_t cap_mask, uint64_t key) {
   uint64_t s = mix64(key) & cap_mask;
-  for (int p = 0; p < kGlobalProbes; p++) {
+  for (uint64_t p = 0; p < (uint64_t)kGlobalProbes && p <= cap_mask; p++) {
     unsigned long long cur = keys[s];
     if (cur == key) return (uint32_t)s;
     if (cur == kEmpty) return 0xFFFFFFFFu;

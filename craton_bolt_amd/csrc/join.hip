@@ -20,6 +20,7 @@
 #include <cstdlib>
 
 #include "jit.h"
+#include "join_dev.h"
 #include "reduce_common.h"
 
 namespace bolt {
@@ -31,7 +32,6 @@ __device__ __forceinline__ int64_t load_key(const void* p, int dtype, int64_t i)
   return dtype == BOLT_I64 ? static_cast<const int64_t*>(p)[i] : (int64_t) static_cast<const int32_t*>(p)[i];
 }
 
-__device__ __forceinline__ int64_t slot_key(const uint4& e) { return (int64_t)(((uint64_t)e.y << 32) | e.x); }
 
 // ---- build -------------------------------------------------------------------
 
@@ -59,23 +59,6 @@ __global__ __launch_bounds__(kBuildBlock) void join_build_kernel(uint4* __restri
       s = (s + 1) & mask;
     }
     if (!placed) atomicAdd(lost, 1u);
-  }
-}
-
-// Walk the probe sequence of `k`: on_match(value) for every stored copy;
-// stops at the first empty slot, or after the first match when `first_only`.
-template <typename F>
-__device__ __forceinline__ void probe_chain(const uint4* __restrict__ tbl, uint64_t mask, int64_t k, bool first_only,
-                                            F&& on_match) {
-  uint64_t s = fmix64((uint64_t)k) & mask;
-  for (uint64_t step = 0; step <= mask; step++) {
-    const uint4 e = tbl[s];
-    if (e.w == 0u) return;
-    if (slot_key(e) == k) {
-      on_match(e.z);
-      if (first_only) return;
-    }
-    s = (s + 1) & mask;
   }
 }
 
@@ -184,15 +167,6 @@ __global__ __launch_bounds__(kProbeBlock) void join_probe_emit_kernel(
 
 using namespace bolt;
 
-struct bolt_join_table {
-  uint4* tbl = nullptr;
-  uint64_t mask = 0;  // capacity - 1 (power of two)
-  int64_t n_build = 0;
-  int32_t key_dtype = 0;
-  int32_t carry_dtype = 0;  // 0 = values are build row ids
-  uint32_t* flags = nullptr; // [0] nonunique, [1] lost rows
-  hipStream_t stream = nullptr;
-};
 
 static size_t join_table_cap_bytes() {
   // BOLT_GPU_JOIN_TABLE_CAP_MB (src/exec/gpu_join.rs:393) overrides; the

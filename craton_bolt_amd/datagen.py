@@ -44,6 +44,7 @@ def lib() -> C.CDLL:
         L.bolt_gen_sort_pairs.argtypes = [_u64, _i64, _p, _p]
         L.bolt_gen_zipf_groupby.argtypes = [_u64, _i64, _i64, C.c_double, _p, _p]
         L.bolt_gen_zipf_small.argtypes = [_u64, _i64, _i64, C.c_double, _p, _p, _p]
+        L.bolt_gen_star_fact.argtypes = [_u64, _i64, _i64, _i64, _i64, C.c_int32, _p, _p, _p, _p, _p, _p]
         _lib = L
     return _lib
 
@@ -111,6 +112,37 @@ def batch_shapes(n_queries: int = 512, seed: int = SEED_BATCH) -> list[tuple[int
         c = int(round(math.exp(math.log(16) + u[1] * (math.log(1e6) - math.log(16)))))
         out.append((n, c))
     return out
+
+
+STAR_DIMS = (1_000_000, 100_000, 2_500)
+STAR_DATE0 = 17532  # 2018-01-01 as days since the epoch
+
+
+def star_fact(n: int, dims=STAR_DIMS, seed: int = SEED_SERVE) -> dict:
+    """Fact table of the served workload: fk1/fk2/fk3 int32, price Decimal128(12,2)
+    as an (n, 2) int64 [lo, hi] array of cents, qty float64, date Date32."""
+    out = {"fk1": np.empty(n, np.int32), "fk2": np.empty(n, np.int32), "fk3": np.empty(n, np.int32),
+           "price": np.empty((n, 2), np.int64), "qty": np.empty(n, np.float64), "date": np.empty(n, np.int32)}
+    lib().bolt_gen_star_fact(seed, n, dims[0], dims[1], dims[2], STAR_DATE0, out["fk1"].ctypes.data,
+                             out["fk2"].ctypes.data, out["fk3"].ctypes.data, out["price"].ctypes.data,
+                             out["qty"].ctypes.data, out["date"].ctypes.data)
+    return out
+
+
+STAR_ATTRS = (("brand", 100_000, "a1000", 1000), ("city", 10_000, "zone", 1000, "region", 50),
+              ("cat", 25, "sub", 250))
+
+
+def star_dim(d: int, n: int) -> tuple[np.ndarray, list[np.ndarray]]:
+    """Dimension d (0, 1, 2) with n rows: int32 keys are the permutation
+    (r * 1000003 + 12345) mod n of the row ids (so the join is a real key
+    lookup), attributes are key mod a fixed modulus (dim 0: brand = key % 100000,
+    key % 1000; dim 1: city = key % 10000, zone = key % 1000, region = key % 50;
+    dim 2: cat = key % 25, sub = key % 250)."""
+    r = np.arange(n, dtype=np.int64)
+    key = ((r * 1000003 + 12345) % n).astype(np.int32)
+    mods = STAR_ATTRS[d][1::2]
+    return key, [(key.astype(np.int64) % m).astype(np.int32) for m in mods]
 
 
 def scramble64(x: int) -> int:

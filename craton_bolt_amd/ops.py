@@ -233,6 +233,89 @@ class GroupByBatch:
             pass
 
 
+# ---- served form (bolt_serve.h) ------------------------------------------------------
+
+class Server:
+    """Request loop over a device-resident star schema (continuous batching,
+    HBM-budget admission).  dims: list of (key column, [attribute columns])."""
+
+    def __init__(self, fact: list[DeviceColumn], dims: list[tuple[DeviceColumn, list[DeviceColumn]]],
+                 slots: int = 8, hbm_budget: int = 4 << 30, chunk_rows: int = 0):
+        self._h = C.c_void_p()
+        self._keep = (fact, dims)
+        fa = columns_array(fact)
+        self._attr_arrays = [columns_array(attrs) for _, attrs in dims]
+        da = (N.BoltServeDim * len(dims))()
+        for i, (key, attrs) in enumerate(dims):
+            da[i] = N.BoltServeDim(key.c(), C.cast(self._attr_arrays[i], C.POINTER(N.BoltColumn)), len(attrs))
+        cfg = N.BoltServeConfig(slots, hbm_budget, chunk_rows)
+        N.check(N.lib().bolt_server_create(None, fa, len(fact), da, len(dims), C.byref(cfg), C.byref(self._h)),
+                "bolt_server_create")
+
+    def query(self, terms: list[Term], joins: list[tuple[int, int]], group: list[tuple[int, int]],
+              aggs: list[tuple[int, int]], order_agg: int = 0, limit: int = 100, est_groups: int = 0):
+        """joins: (fact fk column, dimension); group: (join index, attribute index)."""
+        q = N.BoltServeQuery()
+        q.n_terms = len(terms)
+        for i, t in enumerate(terms):
+            q.terms[i] = t.c(self._keep[0][t.col].dtype)
+        q.n_joins = len(joins)
+        for i, (fk, dim) in enumerate(joins):
+            q.join_fk[i] = fk
+            q.join_dim[i] = dim
+        q.n_group = len(group)
+        for i, (j, a) in enumerate(group):
+            q.group_dim[i] = j
+            q.group_attr[i] = a
+        q.n_aggs = len(aggs)
+        for i, (op, col) in enumerate(aggs):
+            q.aggs[i] = N.BoltAgg(op, col)
+        q.order_agg = order_agg
+        q.limit = limit
+        q.est_groups = est_groups
+        return q
+
+    def submit(self, q: N.BoltServeQuery) -> int:
+        t = C.c_int64()
+        N.check(N.lib().bolt_server_submit(self._h, C.byref(q), C.byref(t)), "bolt_server_submit")
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: float = -1.0) -> dict | None:
+        r = N.BoltServeResult()
+        st = N.lib().bolt_server_wait(self._h, ticket, timeout_ms, C.byref(r))
+        if st == N.BOLT_HIP_ERROR_NOT_READY:
+            return None
+        N.check(st, "bolt_server_wait")
+        rows = []
+        for i in range(r.n_rows):
+            rows.append(tuple(scalar_value(r.cells[i * r.n_cols + c]) for c in range(r.n_cols)))
+        out = {"status": r.status, "rows": rows, "n_groups": r.n_groups, "hbm_bytes": r.hbm_bytes,
+               "submit_ms": r.submit_ms,
+               "admit_ms": r.admit_ms, "done_ms": r.done_ms,
+               "error": r.error.decode(errors="replace") if r.error else ""}
+        N.lib().bolt_server_result_release(self._h, ticket)
+        return out
+
+    def now_ms(self) -> float:
+        return N.lib().bolt_server_now_ms(self._h)
+
+    def stats(self) -> dict:
+        st, done, pa, pb = C.c_int64(), C.c_int64(), C.c_int32(), C.c_int64()
+        N.check(N.lib().bolt_server_stats(self._h, C.byref(st), C.byref(done), C.byref(pa), C.byref(pb)))
+        return {"steps": st.value, "completed": done.value, "peak_active": pa.value, "peak_bytes": pb.value}
+
+    def close(self) -> None:
+        if self._h:
+            N.lib().bolt_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ---- hash join (src/exec/gpu_join.rs:1583, src/exec/join.rs:169) --------------------
 
 class JoinTable:
