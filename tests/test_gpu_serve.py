@@ -58,8 +58,8 @@ def reference(star, spec):
     """Oracle filter + host INNER join + oracle GROUP BY, then ORDER BY / LIMIT."""
     host, dts = star["host"], star["dts"]
     n = len(host[0])
-    hc = [O.HostCol(a, {N.I32: O.I32, N.DEC128: O.DEC128, N.F64: O.F64, N.DATE32: O.DATE32}[dt])
-          for a, dt in zip(host, dts)]
+    odt = [{N.I32: O.I32, N.DEC128: O.DEC128, N.F64: O.F64, N.DATE32: O.DATE32}[dt] for dt in dts]
+    hc = [O.HostCol(a, o) for a, o in zip(host, odt)]
     rows = O.filter_rows(hc, [(t.col, t.op, t.value) for t in spec["terms"]], n) if spec["terms"] else np.arange(n)
     drow = []
     for fk, d in spec["joins"]:
@@ -74,7 +74,7 @@ def reference(star, spec):
     for j, a in spec["group"]:
         d = spec["joins"][j][1]
         gk.append(star["dims_h"][d][1][a][drow[j]])
-    inputs = [O.HostCol(np.ascontiguousarray(host[c][rows]), hc[c].dtype) for c in range(len(host))]
+    inputs = [O.HostCol(np.ascontiguousarray(host[c][rows]), odt[c]) for c in range(len(host))]
     want = O.groupby([O.HostCol(np.ascontiguousarray(k), O.I32) for k in gk], inputs, spec["aggs"])
     assert want["status"] == O.OK
     g = want["n_groups"]
