@@ -48,7 +48,8 @@
 namespace bolt {
 
 int radix_sort_u64_pairs(hipStream_t s, const uint64_t* keys_in, const uint32_t* vals_in, int64_t n,
-                         uint64_t* keys_out, uint32_t* vals_out, int key_bits);
+                         uint64_t* keys_out, uint32_t* vals_out, int key_bits, uint64_t in_xor = 0,
+                         uint64_t out_xor = 0);
 
 // ---- pass A: LDS tier ------------------------------------------------------------
 

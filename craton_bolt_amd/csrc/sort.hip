@@ -13,11 +13,22 @@
 //     earlier tiles per digit for its global offsets (decoupled look-back,
 //     one digit per thread), reorders the tile by digit in LDS and writes
 //     runs of equal digits with coalesced stores;
-//   * passes whose digit is constant over the whole input are skipped.
+//   * passes whose digit is constant over the whole input are skipped;
+//   * for large inputs with three or more varying digits the sort is hybrid
+//     (MSD-grouped, LSD-finished): only the two most significant varying
+//     digits get global onesweep passes; they leave up to 65536 contiguous
+//     buckets of ~n/65536 keys, which one workgroup each sorts by the
+//     remaining bytes entirely in LDS and writes back once (Stehle &
+//     Jacobsen's hybrid radix sort, 2017).  Three HBM round trips replace
+//     eight for 64-bit keys.  Skewed inputs whose largest bucket exceeds the
+//     LDS capacity fall back to plain LSD.
 // Key transforms follow src/exec/gpu_sort.rs:1070-1180: signed integers flip
 // the sign bit, floats use the IEEE-monotonic flip after canonicalising NaN
-// (sorted last) and -0.0 (== +0.0), DESC is the bitwise NOT.
+// (sorted last) and -0.0 (== +0.0), DESC is the bitwise NOT; for int64
+// pairs the transform is an xor applied as the first pass loads and undone
+// as the last one stores.
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "bolt_internal.h"
@@ -39,13 +50,14 @@ constexpr uint32_t kSValue = (1u << 30) - 1;
 // ---- histogram of every digit ---------------------------------------------------
 
 __global__ __launch_bounds__(kSortBlock) void radix_hist_kernel(const uint64_t* __restrict__ keys, int64_t n,
-                                                                int passes, uint32_t* __restrict__ hist) {
+                                                                int passes, uint32_t* __restrict__ hist,
+                                                                uint64_t in_xor) {
   __shared__ uint32_t h[kMaxPasses][kBins];
   for (int i = threadIdx.x; i < passes * kBins; i += kSortBlock) (&h[0][0])[i] = 0;
   __syncthreads();
   const int64_t stride = (int64_t)gridDim.x * kSortBlock;
   for (int64_t i = (int64_t)blockIdx.x * kSortBlock + threadIdx.x; i < n; i += stride) {
-    uint64_t k = __builtin_nontemporal_load(keys + i);
+    uint64_t k = __builtin_nontemporal_load(keys + i) ^ in_xor;
     for (int p = 0; p < passes; p++) atomicAdd(&h[p][(k >> (p * kRadixBits)) & (kBins - 1)], 1u);
   }
   __syncthreads();
@@ -76,7 +88,8 @@ __global__ __launch_bounds__(kBins) void radix_scan_kernel(const uint32_t* __res
 __global__ __launch_bounds__(kSortBlock) void radix_pass_kernel(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
     uint32_t* __restrict__ vout, int64_t n, int shift, const uint32_t* __restrict__ digit_base,
-    uint32_t* __restrict__ status, uint32_t* __restrict__ ticket, uint32_t* __restrict__ err) {
+    uint32_t* __restrict__ status, uint32_t* __restrict__ ticket, uint32_t* __restrict__ err, uint64_t in_xor,
+    uint64_t out_xor) {
   __shared__ uint64_t sk[kSortTile];
   __shared__ uint32_t sv[kSortTile];
   __shared__ uint32_t wh[kSortWaves][kBins];  // per-wave digit counts, then per-wave prefixes
@@ -99,7 +112,7 @@ __global__ __launch_bounds__(kSortBlock) void radix_pass_kernel(
   for (int j = 0; j < kSortItems; j++) {
     int64_t i = tbase + (int64_t)wave * 64 * kSortItems + j * 64 + lane;
     if (i < n) {
-      key[j] = __builtin_nontemporal_load(kin + i);
+      key[j] = __builtin_nontemporal_load(kin + i) ^ in_xor;
       val[j] = __builtin_nontemporal_load(vin + i);
     } else {
       key[j] = ~0ull;  // out-of-range lanes sort to the last digit and are never stored
@@ -197,25 +210,188 @@ __global__ __launch_bounds__(kSortBlock) void radix_pass_kernel(
       const uint64_t k = sk[i];
       const uint32_t d = (uint32_t)(k >> shift) & (kBins - 1);
       const uint64_t dst = (uint64_t)gb[d] + (uint32_t)i - lex[d];
-      kout[dst] = k;
+      kout[dst] = k ^ out_xor;
       vout[dst] = sv[i];
     }
   }
 }
 
 __global__ void copy_pairs_kernel(const uint64_t* __restrict__ ki, const uint32_t* __restrict__ vi,
-                                  uint64_t* __restrict__ ko, uint32_t* __restrict__ vo, int64_t n) {
+                                  uint64_t* __restrict__ ko, uint32_t* __restrict__ vo, int64_t n, uint64_t x) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    ko[i] = ki[i];
+    ko[i] = ki[i] ^ x;
     vo[i] = vi[i];
   }
 }
 
-// Sort (keys_in, vals_in) ascending by the low `key_bits` bits of the key
-// (stable), writing keys_out / vals_out.  Inputs are not modified.
+// ---- hybrid tail: per-bucket LDS sort ------------------------------------------------
+//
+// After two stable global passes on the two most significant varying digits
+// (p1 < p2), the keys form up to 65536 contiguous buckets, one per (digit p2,
+// digit p1) pair, and every bit above byte p1 other than those two digits is
+// constant.  One workgroup then sorts each bucket entirely in LDS by the
+// remaining low bytes (stable LSD, skipping bytes that are constant within the
+// bucket) and writes it once: three HBM round trips instead of one per digit.
+
+constexpr int kLocBlock = 512;
+constexpr int kLocItems = 11;
+constexpr int kLocMax = kLocBlock * kLocItems;  // 5632 keys per bucket
+constexpr int kLocWaves = kLocBlock / 64;
+constexpr int kBuckets = 1 << 16;
+
+__device__ __forceinline__ uint32_t bucket_of(uint64_t k, int s1, int s2) {
+  return (uint32_t)((((k >> s2) & 255) << 8) | ((k >> s1) & 255));
+}
+
+// start[b] = first index whose bucket is >= b (b = 0 .. 65536); *max_size =
+// the largest bucket.  keys are sorted by bucket.
+__global__ void bucket_bounds_kernel(const uint64_t* __restrict__ keys, int64_t n, int s1, int s2,
+                                     uint32_t* __restrict__ start, uint32_t* __restrict__ max_size) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > kBuckets) return;
+  auto lower = [&](uint32_t target) -> int64_t {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (bucket_of(keys[mid], s1, s2) < target) lo = mid + 1;
+      else hi = mid;
+    }
+    return lo;
+  };
+  const int64_t s0 = b < kBuckets ? lower((uint32_t)b) : n;
+  start[b] = (uint32_t)s0;
+  if (b < kBuckets) {
+    const int64_t e = b + 1 < kBuckets ? lower((uint32_t)(b + 1)) : n;
+    atomicMax(max_size, (uint32_t)(e - s0));
+  }
+}
+
+__global__ __launch_bounds__(kLocBlock) void local_sort_kernel(
+    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
+    uint32_t* __restrict__ vout, const uint32_t* __restrict__ start, int low_bits, uint32_t byte_mask,
+    uint64_t out_xor, uint32_t* __restrict__ err) {
+  __shared__ uint64_t sk[kLocMax];
+  __shared__ uint32_t sv[kLocMax];
+  __shared__ uint32_t wh[kLocWaves][kBins];
+  __shared__ uint32_t lex[kBins];
+  __shared__ uint64_t red[2][kLocWaves];
+  const uint32_t s0 = start[blockIdx.x];
+  const int size = (int)(start[blockIdx.x + 1] - s0);
+  if (size <= 0) return;
+  if (size > kLocMax) {  // the host checks the largest bucket first; never write out of range
+    if (threadIdx.x == 0) atomicOr(err, 2u);
+    return;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t low_mask = (1ull << low_bits) - 1;  // low_bits <= 48
+  const uint64_t top = kin[s0] & ~low_mask;           // constant over the bucket
+  // element order: (wave, item, lane); w = (low bits << 16) | element index
+  uint64_t w[kLocItems];
+  uint64_t a = ~0ull, o = 0;
+#pragma unroll
+  for (int j = 0; j < kLocItems; j++) {
+    const int i = wave * 64 * kLocItems + j * 64 + lane;
+    if (i < size) {
+      const uint64_t low = kin[s0 + i] & low_mask;
+      w[j] = (low << 16) | (uint64_t)i;
+      a &= low;
+      o |= low;
+      sv[i] = vin[s0 + i];
+    } else {
+      w[j] = ~0ull;  // sorts after every element of the bucket
+    }
+  }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    a &= __shfl_xor(a, m);
+    o |= __shfl_xor(o, m);
+  }
+  if (lane == 0) {
+    red[0][wave] = a;
+    red[1][wave] = o;
+  }
+  __syncthreads();
+  uint64_t varying = 0;
+  {
+    uint64_t ra = ~0ull, ro = 0;
+    for (int k = 0; k < kLocWaves; k++) {
+      ra &= red[0][k];
+      ro |= red[1][k];
+    }
+    varying = ra ^ ro;
+  }
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  const int t = threadIdx.x;
+  for (int q = 0; q * 8 < low_bits; q++) {
+    if (!((byte_mask >> q) & 1u) || ((varying >> (8 * q)) & 255) == 0) continue;  // block-uniform
+    const int shift = 16 + 8 * q;
+    for (int i = t; i < kLocWaves * kBins; i += kLocBlock) (&wh[0][0])[i] = 0;
+    __syncthreads();
+    uint16_t rank[kLocItems];
+#pragma unroll
+    for (int j = 0; j < kLocItems; j++) {
+      const uint32_t d = (uint32_t)(w[j] >> shift) & (kBins - 1);
+      uint64_t peers = ~0ull;
+#pragma unroll
+      for (int bit = 0; bit < kRadixBits; bit++) {
+        const uint64_t m = __ballot((d >> bit) & 1u);
+        peers &= ((d >> bit) & 1u) ? m : ~m;
+      }
+      const int leader = __builtin_ctzll(peers);
+      uint32_t old = 0;
+      if (lane == leader) {
+        old = wh[wave][d];
+        wh[wave][d] = old + (uint32_t)__popcll(peers);
+      }
+      old = __shfl(old, leader);
+      rank[j] = (uint16_t)(old + (uint32_t)__popcll(peers & lt));
+    }
+    __syncthreads();
+    uint32_t cnt = 0;
+    if (t < kBins) {
+#pragma unroll
+      for (int k = 0; k < kLocWaves; k++) {
+        const uint32_t c = wh[k][t];
+        wh[k][t] = cnt;
+        cnt += c;
+      }
+      lex[t] = cnt;
+    }
+    __syncthreads();
+    for (int d = 1; d < kBins; d <<= 1) {
+      uint32_t v = (t < kBins && t >= d) ? lex[t - d] : 0;
+      __syncthreads();
+      if (t < kBins) lex[t] += v;
+      __syncthreads();
+    }
+    if (t < kBins) lex[t] -= cnt;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kLocItems; j++) {
+      const uint32_t d = (uint32_t)(w[j] >> shift) & (kBins - 1);
+      sk[lex[d] + wh[wave][d] + rank[j]] = w[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kLocItems; j++) w[j] = sk[wave * 64 * kLocItems + j * 64 + lane];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < kLocItems; j++) {
+    const int i = wave * 64 * kLocItems + j * 64 + lane;
+    if (i < size) {
+      kout[s0 + i] = (top | (w[j] >> 16)) ^ out_xor;
+      vout[s0 + i] = sv[(uint32_t)(w[j] & 0xFFFF)];
+    }
+  }
+}
+
+// Sort (keys_in, vals_in) ascending by the low `key_bits` bits of
+// (key ^ in_xor), stable, writing (key image ^ out_xor, value) to keys_out /
+// vals_out.  Inputs are not modified and must not alias the outputs.
 int radix_sort_u64_pairs(hipStream_t s, const uint64_t* keys_in, const uint32_t* vals_in, int64_t n,
-                         uint64_t* keys_out, uint32_t* vals_out, int key_bits) {
+                         uint64_t* keys_out, uint32_t* vals_out, int key_bits, uint64_t in_xor, uint64_t out_xor) {
   if (n <= 0) return BOLT_OK;
   if (n >= (1ll << 30)) return set_error(BOLT_ERR_PLAN, "radix sort: %lld keys exceeds the 2^30 limit", (long long)n);
   const int passes = (key_bits + kRadixBits - 1) / kRadixBits;
@@ -225,7 +401,7 @@ int radix_sort_u64_pairs(hipStream_t s, const uint64_t* keys_in, const uint32_t*
   BOLT_TRY(offs.alloc((size_t)passes * kBins, s));
   BOLT_TRY(misc.alloc(4, s));
   BOLT_TRY(fill_u32(hist.ptr, 0, (size_t)passes * kBins, s));
-  radix_hist_kernel<<<grid_for(n, kSortBlock, 16, 2048), kSortBlock, 0, s>>>(keys_in, n, passes, hist.ptr);
+  radix_hist_kernel<<<grid_for(n, kSortBlock, 16, 2048), kSortBlock, 0, s>>>(keys_in, n, passes, hist.ptr, in_xor);
   radix_scan_kernel<<<passes, kBins, 0, s>>>(hist.ptr, offs.ptr);
   BOLT_HIP(hipGetLastError());
   std::vector<uint32_t> h((size_t)passes * kBins);
@@ -239,7 +415,7 @@ int radix_sort_u64_pairs(hipStream_t s, const uint64_t* keys_in, const uint32_t*
     if (!trivial) todo.push_back(p);
   }
   if (todo.empty()) {
-    copy_pairs_kernel<<<grid_for(n, 256, 4), 256, 0, s>>>(keys_in, vals_in, keys_out, vals_out, n);
+    copy_pairs_kernel<<<grid_for(n, 256, 4), 256, 0, s>>>(keys_in, vals_in, keys_out, vals_out, n, in_xor ^ out_xor);
     BOLT_HIP(hipGetLastError());
     return BOLT_OK;
   }
@@ -249,47 +425,69 @@ int radix_sort_u64_pairs(hipStream_t s, const uint64_t* keys_in, const uint32_t*
   BOLT_TRY(ktmp.alloc((size_t)n, s));
   BOLT_TRY(vtmp.alloc((size_t)n, s));
   BOLT_TRY(status.alloc((size_t)tiles * kBins, s));
-  // ping-pong so that the last pass writes the output buffers
+  auto pass = [&](const uint64_t* ksrc, const uint32_t* vsrc, uint64_t* kdst, uint32_t* vdst, int p, uint64_t ix,
+                  uint64_t ox) -> int {
+    BOLT_TRY(fill_u32(status.ptr, 0, (size_t)tiles * kBins, s));
+    BOLT_TRY(fill_u32(misc.ptr, 0, 4, s));
+    radix_pass_kernel<<<(unsigned)tiles, kSortBlock, 0, s>>>(ksrc, vsrc, kdst, vdst, n, p * kRadixBits,
+                                                             offs.ptr + (size_t)p * kBins, status.ptr, misc.ptr,
+                                                             misc.ptr + 1, ix, ox);
+    BOLT_HIP(hipGetLastError());
+    return BOLT_OK;
+  };
+  auto check_lookback = [&]() -> int {
+    uint32_t e = 0;
+    BOLT_HIP(hipMemcpyAsync(&e, misc.ptr + 1, 4, hipMemcpyDeviceToHost, s));
+    BOLT_HIP(hipStreamSynchronize(s));
+    if (e) return set_error(BOLT_ERR_OTHER, "radix sort look-back timed out");
+    return BOLT_OK;
+  };
   const int np = (int)todo.size();
+  // hybrid: two global passes on the top varying digits, then LDS bucket sorts
+  if (np >= 3 && n >= (1 << 20) && getenv("BOLT_SORT_LSD") == nullptr) {
+    const int p1 = todo[np - 2], p2 = todo[np - 1];
+    BOLT_TRY(pass(keys_in, vals_in, keys_out, vals_out, p1, in_xor, 0));
+    BOLT_TRY(pass(keys_out, vals_out, ktmp.ptr, vtmp.ptr, p2, 0, 0));
+    DevBuf<uint32_t> starts;
+    BOLT_TRY(starts.alloc(kBuckets + 1, s));
+    BOLT_TRY(fill_u32(misc.ptr + 2, 0, 2, s));
+    bucket_bounds_kernel<<<(kBuckets + 1 + 255) / 256, 256, 0, s>>>(ktmp.ptr, n, p1 * kRadixBits, p2 * kRadixBits,
+                                                                  starts.ptr, misc.ptr + 2);
+    BOLT_HIP(hipGetLastError());
+    uint32_t mx[2] = {0, 0};
+    BOLT_HIP(hipMemcpyAsync(mx, misc.ptr + 1, 8, hipMemcpyDeviceToHost, s));
+    BOLT_HIP(hipStreamSynchronize(s));
+    if (mx[0]) return set_error(BOLT_ERR_OTHER, "radix sort look-back timed out");
+    if (mx[1] <= (uint32_t)kLocMax) {
+      uint32_t byte_mask = 0;
+      for (int q = 0; q < np - 2; q++) byte_mask |= 1u << todo[q];
+      BOLT_TRY(fill_u32(misc.ptr + 2, 0, 1, s));
+      local_sort_kernel<<<kBuckets, kLocBlock, 0, s>>>(ktmp.ptr, vtmp.ptr, keys_out, vals_out, starts.ptr,
+                                                       p1 * kRadixBits, byte_mask, out_xor, misc.ptr + 2);
+      BOLT_HIP(hipGetLastError());
+      uint32_t e2 = 0;
+      BOLT_HIP(hipMemcpyAsync(&e2, misc.ptr + 2, 4, hipMemcpyDeviceToHost, s));
+      BOLT_HIP(hipStreamSynchronize(s));
+      if (e2) return set_error(BOLT_ERR_OTHER, "radix sort: bucket larger than the LDS sort");
+      return BOLT_OK;
+    }
+    // a bucket too large for LDS (skewed keys): plain LSD from the input
+  }
+  // LSD: one pass per varying digit, ping-pong so the last pass writes the output
   const uint64_t* ksrc = keys_in;
   const uint32_t* vsrc = vals_in;
   for (int q = 0; q < np; q++) {
     const bool to_out = ((np - 1 - q) % 2) == 0;
     uint64_t* kdst = to_out ? keys_out : ktmp.ptr;
     uint32_t* vdst = to_out ? vals_out : vtmp.ptr;
-    BOLT_TRY(fill_u32(status.ptr, 0, (size_t)tiles * kBins, s));
-    BOLT_TRY(fill_u32(misc.ptr, 0, 4, s));
-    const int p = todo[q];
-    radix_pass_kernel<<<(unsigned)tiles, kSortBlock, 0, s>>>(ksrc, vsrc, kdst, vdst, n, p * kRadixBits,
-                                                             offs.ptr + (size_t)p * kBins, status.ptr, misc.ptr,
-                                                             misc.ptr + 1);
-    BOLT_HIP(hipGetLastError());
+    BOLT_TRY(pass(ksrc, vsrc, kdst, vdst, todo[q], q == 0 ? in_xor : 0, q == np - 1 ? out_xor : 0));
     ksrc = kdst;
     vsrc = vdst;
   }
-  uint32_t e = 0;
-  BOLT_HIP(hipMemcpyAsync(&e, misc.ptr + 1, 4, hipMemcpyDeviceToHost, s));
-  BOLT_HIP(hipStreamSynchronize(s));
-  if (e) return set_error(BOLT_ERR_OTHER, "radix sort look-back timed out");
-  return BOLT_OK;
+  return check_lookback();
 }
 
 // ---- key transforms ------------------------------------------------------------------
-
-__global__ void i64_to_radix_kernel(const int64_t* __restrict__ in, uint64_t* __restrict__ out, int64_t n, int desc) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    uint64_t k = (uint64_t)in[i] ^ 0x8000000000000000ull;  // gpu_sort.rs:1091
-    out[i] = desc ? ~k : k;
-  }
-}
-__global__ void radix_to_i64_kernel(const uint64_t* __restrict__ in, int64_t* __restrict__ out, int64_t n, int desc) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    uint64_t k = desc ? ~in[i] : in[i];
-    out[i] = (int64_t)(k ^ 0x8000000000000000ull);
-  }
-}
 
 // Ascending radix image of column value `row` (bits used: 32 or 64).
 __device__ __forceinline__ uint64_t radix_image(const void* d, int dt, int64_t row) {
@@ -350,15 +548,23 @@ extern "C" int bolt_sort_pairs_i64(bolt_stream_t stream, const int64_t* keys_in,
   hipStream_t s = (hipStream_t)stream;
   if (n < 0) return set_error(BOLT_ERR_PLAN, "bolt_sort_pairs_i64: negative length");
   if (n == 0) return BOLT_OK;
-  // radix images live in keys_out's memory only after the sort, so stage them
-  DevBuf<uint64_t> img, sorted;
-  BOLT_TRY(img.alloc((size_t)n, s));
-  BOLT_TRY(sorted.alloc((size_t)n, s));
-  i64_to_radix_kernel<<<grid_for(n, 256, 4), 256, 0, s>>>(keys_in, img.ptr, n, descending);
-  BOLT_HIP(hipGetLastError());
-  BOLT_TRY(radix_sort_u64_pairs(s, img.ptr, vals_in, n, sorted.ptr, vals_out, 64));
-  radix_to_i64_kernel<<<grid_for(n, 256, 4), 256, 0, s>>>(sorted.ptr, keys_out, n, descending);
-  BOLT_HIP(hipGetLastError());
+  // radix image: sign bit flipped (src/exec/gpu_sort.rs:1091), all bits
+  // inverted for DESC; applied on load by the first pass and undone on the
+  // final store, so no separate transform kernels run
+  const uint64_t x = descending ? 0x7FFFFFFFFFFFFFFFull : 0x8000000000000000ull;
+  const uint64_t* kin = reinterpret_cast<const uint64_t*>(keys_in);
+  const uint32_t* vin = vals_in;
+  DevBuf<uint64_t> kcopy;
+  DevBuf<uint32_t> vcopy;
+  if ((const void*)keys_in == (const void*)keys_out || (const void*)vals_in == (const void*)vals_out) {
+    BOLT_TRY(kcopy.alloc((size_t)n, s));
+    BOLT_TRY(vcopy.alloc((size_t)n, s));
+    BOLT_HIP(hipMemcpyAsync(kcopy.ptr, keys_in, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+    BOLT_HIP(hipMemcpyAsync(vcopy.ptr, vals_in, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+    kin = kcopy.ptr;
+    vin = vcopy.ptr;
+  }
+  BOLT_TRY(radix_sort_u64_pairs(s, kin, vin, n, reinterpret_cast<uint64_t*>(keys_out), vals_out, 64, x, x));
   return BOLT_OK;
 }
 
@@ -395,7 +601,7 @@ extern "C" int bolt_sort_indices(bolt_stream_t stream, const bolt_column_t* keys
       order_key_kernel<<<grid_for(n, 256, 4), 256, 0, s>>>(c.data, c.validity, c.dtype, bits, desc, nf, mode, out_rows,
                                                           k0.ptr, n);
       BOLT_HIP(hipGetLastError());
-      BOLT_TRY(radix_sort_u64_pairs(s, k0.ptr, out_rows, n, k1.ptr, p0.ptr, mode == 1 ? 1 : bits));
+      BOLT_TRY(radix_sort_u64_pairs(s, k0.ptr, out_rows, n, k1.ptr, p0.ptr, mode == 1 ? 1 : bits, 0, 0));
       BOLT_HIP(hipMemcpyAsync(out_rows, p0.ptr, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
     }
   }

@@ -17,6 +17,43 @@ from gpu_util import both
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(autouse=True, params=["hybrid", "lsd"])
+def sort_path(request, monkeypatch):
+    """Large sorts run the hybrid (two global passes + LDS bucket sorts) and,
+    with BOLT_SORT_LSD set, the plain LSD sort."""
+    if request.param == "lsd":
+        monkeypatch.setenv("BOLT_SORT_LSD", "1")
+    else:
+        monkeypatch.delenv("BOLT_SORT_LSD", raising=False)
+    return request.param
+
+
+def _check_pairs(key, rid, desc=False):
+    n = len(key)
+    ko, vo = sort_pairs_i64(DeviceBuffer.from_numpy(key), DeviceBuffer.from_numpy(rid), n, desc)
+    order = np.argsort(~key if desc else key, kind="stable")
+    assert np.array_equal(ko.to_numpy(np.int64, n), key[order])
+    assert np.array_equal(vo.to_numpy(np.uint32, n), rid[order])
+
+
+def test_sort_hybrid_shapes(gpu):
+    """Key sets that exercise the hybrid's choices: 40-bit keys (top varying
+    digits are bytes 3 and 4), a skewed set whose buckets overflow LDS (LSD
+    fallback), heavy duplicates, and two varying bytes only."""
+    rng = np.random.default_rng(21)
+    n = 3_000_000
+    rid = np.arange(n, dtype=np.uint32)
+    print("[case] 40-bit keys", flush=True)
+    _check_pairs(rng.integers(0, 2 ** 40, n).astype(np.int64), rid)
+    print("[case] skewed: two top-16 values", flush=True)
+    k = rng.integers(0, 2 ** 48, n).astype(np.int64) | (rng.integers(0, 2, n).astype(np.int64) << 56)
+    _check_pairs(k, rid)
+    print("[case] duplicates", flush=True)
+    _check_pairs((rng.integers(0, 5000, n).astype(np.int64) << 20) - (1 << 40), rid, desc=True)
+    print("[case] two varying bytes", flush=True)
+    _check_pairs(rng.integers(0, 65536, n).astype(np.int64), rid)
+
+
 @pytest.mark.parametrize("n", [1, 2, 1000, 4096, 4097, 1_000_003, 20_000_000])
 @pytest.mark.parametrize("desc", [False, True])
 def test_sort_pairs_i64(gpu, n, desc):
