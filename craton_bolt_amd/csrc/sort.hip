@@ -95,6 +95,7 @@ __global__ __launch_bounds__(kSortBlock) void radix_pass_kernel(
   __shared__ uint32_t wh[kSortWaves][kBins];  // per-wave digit counts, then per-wave prefixes
   __shared__ uint32_t lex[kBins];             // tile-local exclusive digit offsets
   __shared__ uint32_t gb[kBins];              // global base of each digit's run in this tile
+  __shared__ uint32_t wtot[kSortWaves];
   __shared__ uint32_t sh_tile;
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -108,16 +109,19 @@ __global__ __launch_bounds__(kSortBlock) void radix_pass_kernel(
   uint64_t key[kSortItems];
   uint32_t val[kSortItems];
   uint16_t rank[kSortItems];
+  // branch-free loads (clamped index) so all 2 x ITEMS loads are in flight at
+  // once; out-of-range lanes become ~0 (last digit) and are never stored
 #pragma unroll
   for (int j = 0; j < kSortItems; j++) {
-    int64_t i = tbase + (int64_t)wave * 64 * kSortItems + j * 64 + lane;
-    if (i < n) {
-      key[j] = __builtin_nontemporal_load(kin + i) ^ in_xor;
-      val[j] = __builtin_nontemporal_load(vin + i);
-    } else {
-      key[j] = ~0ull;  // out-of-range lanes sort to the last digit and are never stored
-      val[j] = 0;
-    }
+    const int64_t i = tbase + (int64_t)wave * 64 * kSortItems + j * 64 + lane;
+    const int64_t ic = i < n ? i : n - 1;
+    key[j] = __builtin_nontemporal_load(kin + ic);
+    val[j] = __builtin_nontemporal_load(vin + ic);
+  }
+#pragma unroll
+  for (int j = 0; j < kSortItems; j++) {
+    const int64_t i = tbase + (int64_t)wave * 64 * kSortItems + j * 64 + lane;
+    key[j] = i < n ? key[j] ^ in_xor : ~0ull;
   }
   const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
@@ -139,7 +143,8 @@ __global__ __launch_bounds__(kSortBlock) void radix_pass_kernel(
     rank[j] = (uint16_t)(old + (uint32_t)__popcll(peers & lt));
   }
   __syncthreads();
-  // thread t owns digit t: wave prefixes, tile count, local offsets
+  // thread t owns digit t: wave prefixes and the tile's count, published at
+  // once so that later tiles' look-backs find it early
   const int t = threadIdx.x;
   uint32_t cnt = 0;
 #pragma unroll
@@ -148,43 +153,60 @@ __global__ __launch_bounds__(kSortBlock) void radix_pass_kernel(
     wh[w][t] = cnt;
     cnt += c;
   }
-  // exclusive scan of the 256 digit counts across the block
-  lex[t] = cnt;
-  __syncthreads();
-  for (int d = 1; d < kBins; d <<= 1) {
-    uint32_t v = t >= d ? lex[t - d] : 0;
-    __syncthreads();
-    lex[t] += v;
-    __syncthreads();
-  }
-  const uint32_t my_excl = lex[t] - cnt;
-  __syncthreads();
-  lex[t] = my_excl;
-  // decoupled look-back for digit t (one digit per thread).  The status word
-  // holds its whole payload, so relaxed device-scope atomics suffice (acquire
-  // / release at agent scope would invalidate / write back the per-XCD L2).
+  // The status word holds its whole payload, so relaxed device-scope atomics
+  // suffice (acquire / release at agent scope would invalidate / write back
+  // the per-XCD L2).
   uint32_t* st = status + (size_t)tile * kBins + t;
+  __hip_atomic_store(st, (tile == 0 ? kSFlagInc : kSFlagAgg) | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // exclusive scan of the 256 digit counts (4 waves x 64 digits, shuffles)
+  {
+    uint32_t inc = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t v = __shfl_up(inc, d);
+      if (lane >= d) inc += v;
+    }
+    if (lane == 63) wtot[wave] = inc;
+    lex[t] = inc - cnt;
+  }
+  __syncthreads();
+  {
+    uint32_t add = 0;
+    for (int k = 0; k < wave; k++) add += wtot[k];
+    lex[t] += add;
+  }
+  // decoupled look-back for digit t, four predecessors per round trip
   uint32_t prefix = 0;
-  if (tile == 0) {
-    __hip_atomic_store(st, kSFlagInc | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  } else {
-    __hip_atomic_store(st, kSFlagAgg | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (tile > 0) {
     int64_t pt = tile - 1;
     uint32_t spins = 0;
-    while (pt >= 0) {
-      uint32_t v = __hip_atomic_load(status + (size_t)pt * kBins + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      uint32_t fl = v & ~kSValue;
-      if (fl == 0) {
+    bool done = false;
+    while (!done) {
+      uint32_t v[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        v[q] = pt - q >= 0 ? __hip_atomic_load(status + (size_t)(pt - q) * kBins + t, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT)
+                           : kSFlagInc;  // before tile 0: an inclusive zero
+      int q = 0;
+      for (; q < 4; q++) {
+        const uint32_t fl = v[q] & ~kSValue;
+        if (fl == 0) break;  // not published yet: wait on this one
+        prefix += v[q] & kSValue;
+        if (fl == kSFlagInc) {
+          done = true;
+          break;
+        }
+      }
+      if (done) break;
+      pt -= q;
+      if (q < 4) {
         if (++spins > (1u << 24)) {
           atomicOr(err, 1u);
           break;
         }
         __builtin_amdgcn_s_sleep(1);
-        continue;
       }
-      prefix += v & kSValue;
-      if (fl == kSFlagInc) break;
-      pt--;
     }
     __hip_atomic_store(st, kSFlagInc | (prefix + cnt), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
@@ -275,6 +297,7 @@ __global__ __launch_bounds__(kLocBlock) void local_sort_kernel(
   __shared__ uint32_t sv[kLocMax];
   __shared__ uint32_t wh[kLocWaves][kBins];
   __shared__ uint32_t lex[kBins];
+  __shared__ uint32_t wtot[4];
   __shared__ uint64_t red[2][kLocWaves];
   const uint32_t s0 = start[blockIdx.x];
   const int size = (int)(start[blockIdx.x + 1] - s0);
@@ -288,19 +311,25 @@ __global__ __launch_bounds__(kLocBlock) void local_sort_kernel(
   const uint64_t top = kin[s0] & ~low_mask;           // constant over the bucket
   // element order: (wave, item, lane); w = (low bits << 16) | element index
   uint64_t w[kLocItems];
+  uint32_t vv[kLocItems];
   uint64_t a = ~0ull, o = 0;
+  // branch-free (clamped) loads first, so they are all in flight together
 #pragma unroll
   for (int j = 0; j < kLocItems; j++) {
     const int i = wave * 64 * kLocItems + j * 64 + lane;
-    if (i < size) {
-      const uint64_t low = kin[s0 + i] & low_mask;
-      w[j] = (low << 16) | (uint64_t)i;
-      a &= low;
-      o |= low;
-      sv[i] = vin[s0 + i];
-    } else {
-      w[j] = ~0ull;  // sorts after every element of the bucket
-    }
+    const int ic = i < size ? i : size - 1;
+    w[j] = kin[s0 + ic];
+    vv[j] = vin[s0 + ic];
+  }
+#pragma unroll
+  for (int j = 0; j < kLocItems; j++) {
+    const int i = wave * 64 * kLocItems + j * 64 + lane;
+    const uint64_t low = w[j] & low_mask;
+    const bool in = i < size;
+    a &= in ? low : ~0ull;
+    o |= in ? low : 0ull;
+    w[j] = in ? ((low << 16) | (uint64_t)i) : ~0ull;  // out-of-range: after every element of the bucket
+    if (in) sv[i] = vv[j];
   }
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) {
@@ -328,7 +357,12 @@ __global__ __launch_bounds__(kLocBlock) void local_sort_kernel(
     const int shift = 16 + 8 * q;
     for (int i = t; i < kLocWaves * kBins; i += kLocBlock) (&wh[0][0])[i] = 0;
     __syncthreads();
-    uint16_t rank[kLocItems];
+    // rank within the wave: peers by ballot matching; the per-wave digit
+    // counters are bumped with returning LDS atomics issued for all items
+    // back to back (LDS keeps a wave's operations in order, so ranks stay
+    // stable) and read once afterwards
+    uint32_t rank[kLocItems], oldv[kLocItems];
+    int lead[kLocItems];
 #pragma unroll
     for (int j = 0; j < kLocItems; j++) {
       const uint32_t d = (uint32_t)(w[j] >> shift) & (kBins - 1);
@@ -338,34 +372,39 @@ __global__ __launch_bounds__(kLocBlock) void local_sort_kernel(
         const uint64_t m = __ballot((d >> bit) & 1u);
         peers &= ((d >> bit) & 1u) ? m : ~m;
       }
-      const int leader = __builtin_ctzll(peers);
-      uint32_t old = 0;
-      if (lane == leader) {
-        old = wh[wave][d];
-        wh[wave][d] = old + (uint32_t)__popcll(peers);
-      }
-      old = __shfl(old, leader);
-      rank[j] = (uint16_t)(old + (uint32_t)__popcll(peers & lt));
+      lead[j] = __builtin_ctzll(peers);
+      rank[j] = (uint32_t)__popcll(peers & lt);
+      oldv[j] = 0;
+      if (lane == lead[j]) oldv[j] = atomicAdd(&wh[wave][d], (uint32_t)__popcll(peers));
     }
+#pragma unroll
+    for (int j = 0; j < kLocItems; j++) rank[j] += __shfl(oldv[j], lead[j]);
     __syncthreads();
-    uint32_t cnt = 0;
+    // digit t: exclusive offsets per wave, then an exclusive scan over the
+    // 256 digits (waves 0..3 scan 64 digits each with shuffles)
     if (t < kBins) {
+      uint32_t cnt = 0;
 #pragma unroll
       for (int k = 0; k < kLocWaves; k++) {
         const uint32_t c = wh[k][t];
         wh[k][t] = cnt;
         cnt += c;
       }
-      lex[t] = cnt;
+      uint32_t inc = cnt;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t v = __shfl_up(inc, d);
+        if (lane >= d) inc += v;
+      }
+      if (lane == 63) wtot[wave] = inc;
+      lex[t] = inc - cnt;
     }
     __syncthreads();
-    for (int d = 1; d < kBins; d <<= 1) {
-      uint32_t v = (t < kBins && t >= d) ? lex[t - d] : 0;
-      __syncthreads();
-      if (t < kBins) lex[t] += v;
-      __syncthreads();
+    if (t < kBins) {
+      uint32_t add = 0;
+      for (int k = 0; k < (t >> 6); k++) add += wtot[k];
+      lex[t] += add;
     }
-    if (t < kBins) lex[t] -= cnt;
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kLocItems; j++) {
