@@ -298,6 +298,8 @@ __global__ __launch_bounds__(kLocBlock) void local_sort_kernel(
   __shared__ uint32_t wh[kLocWaves][kBins];
   __shared__ uint32_t lex[kBins];
   __shared__ uint32_t wtot[4];
+  __shared__ uint32_t wtot8[kLocWaves];
+  __shared__ uint32_t sh_max;
   __shared__ uint64_t red[2][kLocWaves];
   const uint32_t s0 = start[blockIdx.x];
   const int size = (int)(start[blockIdx.x + 1] - s0);
@@ -350,8 +352,75 @@ __global__ __launch_bounds__(kLocBlock) void local_sort_kernel(
     }
     varying = ra ^ ro;
   }
-  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   const int t = threadIdx.x;
+  // ---- fast path: one counting pass on the 11 bits below the bucket's top
+  // varying bit (sub-buckets of ~size/2048 keys), then an insertion sort of
+  // each sub-bucket by the whole packed word; its low 16 bits are the
+  // element's position, so equal keys keep their input order (stable) even
+  // though the atomic scatter is not.  Buckets with a sub-bucket above 64
+  // keys (heavy duplicates) take the LSD passes below instead.
+  if (varying != 0) {
+    const int h = 63 - __clzll(varying);
+    const int dsh = 16 + (h >= 10 ? h - 10 : 0);
+    uint32_t* cnt = &wh[0][0];  // 2048 counters
+    for (int i = t; i < kLocWaves * kBins; i += kLocBlock) cnt[i] = 0;
+    if (t == 0) sh_max = 0;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kLocItems; j++)
+      if (w[j] != ~0ull) atomicAdd(&cnt[(uint32_t)(w[j] >> dsh) & 2047u], 1u);
+    __syncthreads();
+    {
+      const uint32_t c0 = cnt[4 * t], c1 = cnt[4 * t + 1], c2 = cnt[4 * t + 2], c3 = cnt[4 * t + 3];
+      const uint32_t local = c0 + c1 + c2 + c3;
+      uint32_t inc = local;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t v = __shfl_up(inc, d);
+        if (lane >= d) inc += v;
+      }
+      if (lane == 63) wtot8[wave] = inc;
+      const uint32_t mx = max(max(c0, c1), max(c2, c3));
+      if (mx) atomicMax(&sh_max, mx);
+      __syncthreads();
+      uint32_t carry = 0;
+      for (int k = 0; k < wave; k++) carry += wtot8[k];
+      const uint32_t ex = carry + inc - local;
+      cnt[4 * t] = ex;
+      cnt[4 * t + 1] = ex + c0;
+      cnt[4 * t + 2] = ex + c0 + c1;
+      cnt[4 * t + 3] = ex + c0 + c1 + c2;
+    }
+    __syncthreads();
+    if (sh_max <= 64u) {  // block-uniform
+#pragma unroll
+      for (int j = 0; j < kLocItems; j++)
+        if (w[j] != ~0ull) sk[atomicAdd(&cnt[(uint32_t)(w[j] >> dsh) & 2047u], 1u)] = w[j];
+      __syncthreads();
+      // cnt[d] is now the end of sub-bucket d (= the start of d + 1)
+      for (int d = t; d < 2048; d += kLocBlock) {
+        const int b = d ? (int)cnt[d - 1] : 0, e = (int)cnt[d];
+        for (int x = b + 1; x < e; x++) {
+          const uint64_t v = sk[x];
+          int y = x - 1;
+          while (y >= b && sk[y] > v) {
+            sk[y + 1] = sk[y];
+            y--;
+          }
+          sk[y + 1] = v;
+        }
+      }
+      __syncthreads();
+      for (int i = t; i < size; i += kLocBlock) {
+        const uint64_t ww = sk[i];
+        kout[s0 + i] = (top | (ww >> 16)) ^ out_xor;
+        vout[s0 + i] = sv[(uint32_t)(ww & 0xFFFF)];
+      }
+      return;
+    }
+  }
+  // ---- general path: stable LSD over the bytes that vary in this bucket
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   for (int q = 0; q * 8 < low_bits; q++) {
     if (!((byte_mask >> q) & 1u) || ((varying >> (8 * q)) & 255) == 0) continue;  // block-uniform
     const int shift = 16 + 8 * q;
