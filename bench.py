@@ -90,7 +90,8 @@ def bench_join(args) -> dict:
     aggs = [(N.SUM, N.JOIN_CARRY), (N.SUM, 1), (N.COUNT_STAR, 0)]
     terms = [Term(1, N.GT, 10.0)]
     cols = [d_fk, d_amt]
-    ev = [Event() for _ in range(4)]
+    ev = [Event() for _ in range(3)]
+    kev = [Event(), Event()]  # recorded by the library around the fused probe kernel itself
 
     def step(record: bool):
         if record:
@@ -98,7 +99,7 @@ def bench_join(args) -> dict:
         t = JoinTable(d_bk, carry=d_bp, stream=stream)
         if record:
             ev[1].record(stream)
-        res = t.probe_aggregate(cols, 0, aggs, terms)
+        res = t.probe_aggregate(cols, 0, aggs, terms)  # synchronises the stream
         if record:
             ev[2].record(stream)
         t.release()
@@ -107,22 +108,27 @@ def bench_join(args) -> dict:
     for _ in range(args.warmup):
         res = step(False)
     stream.synchronize()
-    build_ms, probe_ms = [], []
+    build_ms, probe_ms, kernel_ms = [], [], []
+    N.lib().bolt_profile_events(kev[0].handle, kev[1].handle)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         res = step(True)
         ev[2].synchronize()
         build_ms.append(ev[0].elapsed_ms(ev[1]))
         probe_ms.append(ev[1].elapsed_ms(ev[2]))
+        kernel_ms.append(kev[0].elapsed_ms(kev[1]))
     stream.synchronize()
     wall = time.perf_counter() - t0
+    N.lib().bolt_profile_events(None, None)
     ms_per_step = wall * 1000.0 / args.steps
-    probe_avg = float(np.mean(probe_ms))
+    kern = float(np.mean(kernel_ms))
     # algorithmic bytes of the probe launch: fk (8 B) + amount (8 B) per probe row
     probe_bytes = npb * 16
-    achieved = probe_bytes / (probe_avg * 1e-3) / 1e9
+    achieved = probe_bytes / (kern * 1e-3) / 1e9
     print(f"[bench] result sum_payload={res[0]} sum_amount={res[1]:.6f} count={res[2]} "
-          f"build_ms={np.mean(build_ms):.3f} probe_ms={probe_avg:.3f}", file=sys.stderr, flush=True)
+          f"build_ms={np.mean(build_ms):.3f} probe_call_ms={np.mean(probe_ms):.3f} probe_kernel_ms={kern:.3f}",
+          file=sys.stderr, flush=True)
+    traffic = _traffic("join", kern)
 
     line = {
         "metric": "rows/s + % HBM peak per operator; batched/served vs plain queries/s, p99 ms",
@@ -140,11 +146,12 @@ def bench_join(args) -> dict:
         "config": {
             "workload": "fused filter + hash join + aggregate: build 16M (unique int64 key, int32 payload), "
                         "probe 256M (int64 fk 50% hits, float64 amount); WHERE amount > 10, SUM(payload), "
-                        "SUM(amount), COUNT(*)",
+                        "SUM(amount), COUNT(*); one step = build + probe",
             "n_build": nb,
             "n_probe": npb,
             "build_ms": float(np.mean(build_ms)),
-            "probe_ms": probe_avg,
+            "probe_call_ms": float(np.mean(probe_ms)),
+            "probe_kernel_ms": kern,
             "result": {"sum_payload": res[0], "sum_amount": res[1], "count": res[2]},
         },
         "roofline": {
@@ -153,14 +160,27 @@ def bench_join(args) -> dict:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "join_probe_agg_kernel (+ one-block finishing kernel)",
+            "traffic": traffic,
+            "kernel": "bolt_jit_kernel (hipRTC fused filter + probe + aggregate), timed by HIP events on its stream",
             "bytes_per_launch": probe_bytes,
         },
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline_join(bk, bp, fk, amt, npb, min(args.cpu_sample, npb))
     return line
+
+
+def _traffic(workload: str, kernel_ms: float):
+    """HBM bytes per launch of the workload's dominant kernel, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (profiles/traffic.json,
+    counters already scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)[workload]
+    except (OSError, KeyError, ValueError):
+        return None
+    return float(t["traffic_bytes"])
 
 
 def _timed(stream, fn, steps, warmup):
@@ -205,12 +225,27 @@ def bench_sort(args) -> dict:
     key, rid = D.sort_pairs(n)
     dk, dv = DeviceBuffer.from_numpy(key), DeviceBuffer.from_numpy(rid)
     s = Stream()
-    ms, wall = _timed(s, lambda: sort_pairs_i64(dk, dv, n, stream=s), args.steps, args.warmup)
+    from craton_bolt_amd import _native as N
+    from craton_bolt_amd.device import Event
+
+    kev = [Event(), Event()]
+    kms = []
+
+    def one():
+        sort_pairs_i64(dk, dv, n, stream=s)
+        kev[1].synchronize()
+        kms.append(kev[0].elapsed_ms(kev[1]))
+
+    N.lib().bolt_profile_events(kev[0].handle, kev[1].handle)
+    ms, wall = _timed(s, one, args.steps, args.warmup)
+    N.lib().bolt_profile_events(None, None)
+    ms = float(np.mean(kms[args.warmup:]))  # the sort itself, on its stream
     model = n * 200
     return _line(n / (ms * 1e-3), "keys/s", args, ms, "int64",
                  {"workload": f"ORDER BY radix sort: {n} (int64 key uniform over the full range, u32 row id)",
-                  "n": n, "compulsory_bytes": n * 24, "model_bytes": model},
-                 _roof(model, ms, "radix sort (histogram + 8 onesweep passes), 200 B/key model"))
+                  "n": n, "compulsory_bytes": n * 24, "model_bytes": model,
+                  "sort_ms_on_stream": ms, "wall_ms_per_call": wall},
+                 _roof(model, ms, "hybrid radix sort (histogram, 2 onesweep passes, LDS bucket sorts), 200 B/key model"))
 
 
 def bench_groupby(args, zipf: bool) -> dict:

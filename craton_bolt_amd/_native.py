@@ -180,6 +180,7 @@ _lib: C.CDLL | None = None
 _SIGS = {
     "bolt_last_error": (C.c_char_p, []),
     "bolt_build_info": (C.c_char_p, []),
+    "bolt_profile_events": (C.c_int, [C.c_void_p, C.c_void_p]),
     "bolt_hip_init": (C.c_int, [C.c_uint]),
     "bolt_hip_device_get_count": (C.c_int, [C.POINTER(C.c_int)]),
     "bolt_hip_device_get_name": (C.c_int, [C.c_char_p, C.c_int, C.c_int]),

@@ -266,7 +266,9 @@ extern "C" int bolt_aggregate(bolt_stream_t stream, const bolt_column_t* inputs,
     if (jit_spec_from(f, all, false, 0, false, 0, &sp, &lits, &cols) == BOLT_OK &&
         jit_get(jit_row_agg_source(sp), &fn) == BOLT_OK) {
       BOLT_TRY(parts.alloc((size_t)grid * n_aggs, s));
-      BOLT_TRY(jit_launch_row_agg(fn, grid, s, nullptr, 0, nullptr, n, nullptr, nullptr, lits, cols, parts.ptr));
+      prof_begin(s);
+      BOLT_TRY(jit_launch_row_agg(fn, grid, s, nullptr, 0, nullptr, n, nullptr, nullptr, lits, cols, parts.ptr, nullptr, 0));
+      prof_end(s);
       for (int first = 0; first < n_aggs; first += 4) {
         int na = n_aggs - first < 4 ? n_aggs - first : 4;
         BOLT_TRY(launch_finish(na, parts.ptr, (int)grid, n_aggs, first, agg_chunk(all, first, na), outs.ptr + first, s));

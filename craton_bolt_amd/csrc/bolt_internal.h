@@ -46,6 +46,12 @@ int device_status();
     if (s_ != BOLT_OK) return s_; \
   } while (0)
 
+// ---- measurement hook -----------------------------------------------------
+
+// Events registered with bolt_profile_events on this thread (nullptr = none).
+void prof_begin(hipStream_t s);
+void prof_end(hipStream_t s);
+
 // ---- memory ---------------------------------------------------------------
 
 // Stream-ordered allocation from the device pool (accounted in pool stats).

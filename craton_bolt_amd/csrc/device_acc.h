@@ -53,6 +53,16 @@ __host__ __device__ __forceinline__ u64_t fmix64(u64_t x) {
   return x;
 }
 
+// Join-side Bloom filter block test: the 128-bit block must have one bit set
+// in each 32-bit word at the positions given by four 5-bit fields of h.
+__device__ __forceinline__ uint4 bloom_pattern(u64_t h) {
+  return make_uint4(1u << ((h >> 20) & 31), 1u << ((h >> 25) & 31), 1u << ((h >> 30) & 31), 1u << ((h >> 35) & 31));
+}
+__device__ __forceinline__ bool bloom_hit(const uint4& b, u64_t h) {
+  const uint4 p = bloom_pattern(h);
+  return (b.x & p.x) && (b.y & p.y) && (b.z & p.z) && (b.w & p.w);
+}
+
 __device__ __forceinline__ bool vbit(const unsigned char* validity, i64_t i) {
   return validity == nullptr || ((validity[i >> 3] >> (i & 7)) & 1);
 }

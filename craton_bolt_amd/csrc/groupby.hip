@@ -513,6 +513,7 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
   DevFilter f;
   BOLT_TRY(make_dev_filter(filter, inputs, n_inputs, n, &f));
 
+  prof_begin(s);
   // ---- pass A ----
   const int W = P.words;
   const int budget = lds_budget_bytes();
@@ -685,6 +686,7 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
       return st;
     }
   }
+  prof_end(s);
   for (auto& c : cols) c.length = n_groups;
   tbl->cols = cols;
   tbl->n_rows = n_groups;

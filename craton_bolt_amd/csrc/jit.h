@@ -27,6 +27,7 @@ struct JitSpec {
     int dtype;
   };
   bool probe = false;
+  bool bloom = false;  // probe through the build side's Bloom filter first
   int unroll = 4;  // rows per lane per block iteration (1..8)
   int key_dtype = 0;
   bool key_valid = false;
@@ -43,6 +44,6 @@ int jit_spec_from(const DevFilter& f, const AggList& aggs, bool probe, int key_d
                   JitSpec* sp, JitLits* lits, JitCols* cols);
 int jit_launch_row_agg(hipFunction_t fn, unsigned grid, hipStream_t s, const uint4* tbl, uint64_t mask,
                        const uint32_t* flags, int64_t n, const void* key, const uint8_t* kvalid, const JitLits& lits,
-                       const JitCols& cols, Acc* parts);
+                       const JitCols& cols, Acc* parts, const uint4* bloom, uint64_t bloom_mask);
 
 }  // namespace bolt

@@ -139,7 +139,8 @@ std::string jit_row_agg_source(const JitSpec& sp) {
   o << "extern \"C\" __global__ void __launch_bounds__(256) bolt_jit_kernel(\n"
        "    const uint4* __restrict__ tbl, unsigned long long mask, const unsigned int* __restrict__ flags,\n"
        "    long long n, const void* __restrict__ keyp, const unsigned char* __restrict__ kvalid,\n"
-       "    bolt::JitLits lits, bolt::JitCols cols, bolt::Acc* __restrict__ parts) {\n"
+       "    bolt::JitLits lits, bolt::JitCols cols, bolt::Acc* __restrict__ parts,\n"
+       "    const uint4* __restrict__ bloom, unsigned long long bmask) {\n"
        "  using namespace bolt;\n";
   const int nc = (int)sp.cols.size();
   for (int c = 0; c < nc; c++) {
@@ -259,8 +260,17 @@ std::string jit_row_agg_source(const JitSpec& sp) {
     // first reads of all R rows, then each row's chain
     for (int r = 0; r < R; r++)
       o << "    const long long k" << r << " = (long long)kraw" << r << ";\n"
-        << "    unsigned long long s" << r << " = fmix64((unsigned long long)k" << r << ") & mask;\n"
-        << "    uint4 e" << r << " = make_uint4(0u, 0u, 0u, 0u);\n"
+        << "    const unsigned long long h" << r << " = fmix64((unsigned long long)k" << r << ");\n"
+        << "    unsigned long long s" << r << " = h" << r << " & mask;\n";
+    if (sp.bloom) {
+      // Bloom blocks of all R rows first (cache resident), then the table
+      for (int r = 0; r < R; r++)
+        o << "    const uint4 b" << r << " = bloom[keep" << r << " ? ((h" << r << " >> 40) & bmask) : 0];\n";
+      for (int r = 0; r < R; r++)
+        o << "    keep" << r << " = keep" << r << " && bloom_hit(b" << r << ", h" << r << ");\n";
+    }
+    for (int r = 0; r < R; r++)
+      o << "    uint4 e" << r << " = make_uint4(0u, 0u, 0u, 0u);\n"
         << "    if (keep" << r << ") e" << r << " = tbl[s" << r << "];\n";
     for (int r = 0; r < R; r++) {
       const std::string e = "e" + std::to_string(r), sv = "s" + std::to_string(r);
@@ -333,12 +343,12 @@ int jit_spec_from(const DevFilter& f, const AggList& aggs, bool probe, int key_d
 
 int jit_launch_row_agg(hipFunction_t fn, unsigned grid, hipStream_t s, const uint4* tbl, uint64_t mask,
                        const uint32_t* flags, int64_t n, const void* key, const uint8_t* kvalid, const JitLits& lits,
-                       const JitCols& cols, Acc* parts) {
-  unsigned long long m = mask;
+                       const JitCols& cols, Acc* parts, const uint4* bloom, uint64_t bloom_mask) {
+  unsigned long long m = mask, bm = bloom_mask;
   long long nn = n;
   JitLits l = lits;
   JitCols c = cols;
-  void* args[] = {(void*)&tbl, &m, (void*)&flags, &nn, (void*)&key, (void*)&kvalid, &l, &c, &parts};
+  void* args[] = {(void*)&tbl, &m, (void*)&flags, &nn, (void*)&key, (void*)&kvalid, &l, &c, &parts, (void*)&bloom, &bm};
   BOLT_HIP(hipModuleLaunchKernel(fn, grid, 1, 1, 256, 1, 1, 0, s, args, nullptr));
   return BOLT_OK;
 }

@@ -14,9 +14,26 @@ struct bolt_join_table {
   int32_t carry_dtype = 0;  // 0 = values are build row ids
   uint32_t* flags = nullptr; // [0] nonunique, [1] lost rows
   hipStream_t stream = nullptr;
+  uint4* bloom = nullptr;     // split-block Bloom filter of the build keys (nullptr = none)
+  uint64_t bloom_mask = 0;    // blocks - 1 (power of two)
 };
 
 namespace bolt {
+
+// Optional split-block Bloom filter over the build keys (BOLT_JOIN_BLOOM=1),
+// 16 bits per key: one 128-bit
+// block per key, chosen by bits 40..63 of fmix64(key), with one bit set in
+// each of its four 32-bit words (bit positions from four 5-bit fields of bits
+// 20..39); the table slot uses the low bits of the same hash.  About 0.25% of
+// absent keys pass.  A probe reads one 16-byte block, which stays in the
+// Infinity Cache for build sides of tens of millions of rows, and only
+// possible hits go on to the table in HBM.  (bloom_pattern / bloom_hit live in
+// device_acc.h, shared with the generated probe kernels.)
+__device__ __forceinline__ bool bloom_maybe(const uint4* bloom, uint64_t bmask, int64_t k) {
+  if (!bloom) return true;
+  const uint64_t h = fmix64((uint64_t)k);
+  return bloom_hit(bloom[(h >> 40) & bmask], h);
+}
 
 __device__ __forceinline__ int64_t slot_key(const uint4& e) { return (int64_t)(((uint64_t)e.y << 32) | e.x); }
 

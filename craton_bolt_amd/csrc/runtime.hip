@@ -19,6 +19,8 @@
 namespace bolt {
 
 static thread_local std::string g_last_error;
+static thread_local hipEvent_t g_prof_start = nullptr;
+static thread_local hipEvent_t g_prof_end = nullptr;
 
 int set_error(int code, const char* fmt, ...) {
   char buf[1024];
@@ -30,6 +32,13 @@ int set_error(int code, const char* fmt, ...) {
   return code;
 }
 void clear_error() { g_last_error.clear(); }
+
+void prof_begin(hipStream_t s) {
+  if (g_prof_start) (void)hipEventRecord(g_prof_start, s);
+}
+void prof_end(hipStream_t s) {
+  if (g_prof_end) (void)hipEventRecord(g_prof_end, s);
+}
 
 static std::once_flag g_dev_once;
 static int g_dev_status = BOLT_ERR_NO_DEVICE;
@@ -171,6 +180,12 @@ extern "C" {
 const char* bolt_last_error(void) { return g_last_error.c_str(); }
 
 const char* bolt_build_info(void) { return "craton_bolt_amd 0.1 gfx950"; }
+
+int bolt_profile_events(bolt_event_t start, bolt_event_t end) {
+  g_prof_start = reinterpret_cast<hipEvent_t>(start);
+  g_prof_end = reinterpret_cast<hipEvent_t>(end);
+  return 0;
+}
 
 int bolt_hip_init(unsigned int flags) {
   if (flags != 0) return set_error(hipErrorInvalidValue, "bolt_hip_init: flags must be 0");

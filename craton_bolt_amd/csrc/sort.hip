@@ -672,7 +672,9 @@ extern "C" int bolt_sort_pairs_i64(bolt_stream_t stream, const int64_t* keys_in,
     kin = kcopy.ptr;
     vin = vcopy.ptr;
   }
+  prof_begin(s);
   BOLT_TRY(radix_sort_u64_pairs(s, kin, vin, n, reinterpret_cast<uint64_t*>(keys_out), vals_out, 64, x, x));
+  prof_end(s);
   return BOLT_OK;
 }
 

@@ -256,6 +256,15 @@ int bolt_sort_indices(bolt_stream_t stream, const bolt_column_t* keys, int32_t n
 /* Build identifier string (target arch, version). */
 const char* bolt_build_info(void);
 
+/* Measurement hook (no reference counterpart; for benchmarks).  While set on
+ * the calling thread, the operators record `start` immediately before and
+ * `end` immediately after their dominant kernel, on the stream that kernel
+ * runs on (bolt_join_probe_aggregate: the fused probe; bolt_aggregate: the
+ * row pass; bolt_sort_pairs_i64: the whole sort; bolt_groupby: the whole
+ * operator), so that bolt_hip_event_elapsed_ms gives the kernel's own time.
+ * Pass NULL, NULL to clear. */
+int bolt_profile_events(bolt_event_t start, bolt_event_t end);
+
 #ifdef __cplusplus
 }
 #endif

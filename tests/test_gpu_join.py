@@ -20,11 +20,13 @@ REL_TOL = 1e-9
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(autouse=True, params=["jit", "generic"])
+@pytest.fixture(autouse=True, params=["jit", "generic", "jit+bloom"])
 def kernel_path(request, monkeypatch):
-    """Every case runs through the per-query hipRTC kernel and through the
-    precompiled generic kernel (BOLT_JIT=0)."""
-    monkeypatch.setenv("BOLT_JIT", "1" if request.param == "jit" else "0")
+    """Every case runs through the per-query hipRTC kernel, the precompiled
+    generic kernel (BOLT_JIT=0), and the hipRTC kernel behind the build
+    side's Bloom filter (BOLT_JOIN_BLOOM=1)."""
+    monkeypatch.setenv("BOLT_JIT", "0" if request.param == "generic" else "1")
+    monkeypatch.setenv("BOLT_JOIN_BLOOM", "1" if request.param == "jit+bloom" else "0")
     return request.param
 
 
