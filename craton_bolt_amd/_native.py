@@ -273,7 +273,7 @@ _SIGS = {
     "bolt_server_result_release": (None, [C.c_void_p, C.c_int64]),
     "bolt_server_now_ms": (C.c_double, [C.c_void_p]),
     "bolt_server_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32),
-                                    C.POINTER(C.c_int64)]),
+                                    C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "bolt_server_destroy": (None, [C.c_void_p]),
 }
 

@@ -56,8 +56,20 @@ struct SGroupKey {
   int32_t dt;
   int32_t join;
 };
+// Up to kPreCols fact columns that the active queries' predicates read are
+// loaded once per row for all of them (the union, rebuilt when the active
+// set changes); a query's term then reads registers, so rows that fail every
+// predicate cost a few ALU operations per query and no dependent loads.
+constexpr int kPreCols = 4;
+struct SPre {
+  const void* d[kPreCols];
+  const uint8_t* v[kPreCols];
+  int32_t dt[kPreCols];
+  int32_t n;
+};
 struct SQ {
   DevFilter f;
+  int8_t term_slot[BOLT_MAX_PRED_TERMS];  // preloaded column of each term, -1 = read it directly
   GPlan P;
   SJoin j[BOLT_SERVE_MAX_DIMS];
   SGroupKey g[2];
@@ -73,14 +85,93 @@ __device__ __forceinline__ int64_t fk_value(const void* p, int dt, int64_t row) 
   return dt == BOLT_I64 ? static_cast<const int64_t*>(p)[row] : (int64_t) static_cast<const int32_t*>(p)[row];
 }
 
-// One chunk of the fact table for every active query.
-__global__ __launch_bounds__(kServeBlock) void serve_step_kernel(const SQ* __restrict__ qs, int nq, SDims dims,
-                                                                 int64_t r0, int64_t r1) {
-  const int64_t stride = (int64_t)gridDim.x * kServeBlock;
-  for (int64_t row = r0 + (int64_t)blockIdx.x * kServeBlock + threadIdx.x; row < r1; row += stride) {
+constexpr int kServeRows = 4;  // rows per lane in the grid sizing
+
+// Step, part 1: every active query's predicate over one chunk.  Block b
+// owns rows [r0 + b*per, r0 + (b+1)*per) of the chunk and appends the rows
+// that pass query q, in order, to its own segment (q, b) of the row lists,
+// counting in LDS: no global atomics.  The predicate columns of all queries
+// are loaded once per row.  Part 2 then runs the joins and aggregation over
+// those dense lists with every lane busy, instead of the few lanes per wave
+// (a few percent of rows) that pass a selective predicate.
+constexpr int kServeGrid = 2048;
+constexpr int kServeWaves = kServeBlock / 64;
+
+__global__ __launch_bounds__(kServeBlock) void serve_filter_kernel(const SQ* __restrict__ qs, int nq, SPre pre,
+                                                                   int64_t r0, int64_t r1, int64_t per,
+                                                                   uint32_t* __restrict__ lists,
+                                                                   uint32_t* __restrict__ counts) {
+  __shared__ uint32_t wcnt[BOLT_SERVE_MAX_SLOTS][kServeWaves];
+  __shared__ uint32_t running[BOLT_SERVE_MAX_SLOTS];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  for (int q = threadIdx.x; q < nq; q += kServeBlock) running[q] = 0;
+  __syncthreads();
+  const int64_t b0 = r0 + (int64_t)blockIdx.x * per;
+  const int64_t b1 = min(r1, b0 + per);
+  for (int64_t base = b0; base < b1; base += kServeBlock) {  // block-uniform trip count
+    const int64_t row = base + threadIdx.x;
+    const bool in = row < b1;
+    const int64_t rc = in ? row : b1 - 1;  // clamped: every load is in range
+    // named registers (no array, so nothing is indexed at run time)
+    Raw c0{0, 0}, c1{0, 0}, c2{0, 0}, c3{0, 0};
+    bool v0 = true, v1 = true, v2 = true, v3 = true;
+    if (pre.n > 0) { c0 = load_raw(pre.d[0], pre.dt[0], rc); v0 = bit_valid(pre.v[0], rc); }
+    if (pre.n > 1) { c1 = load_raw(pre.d[1], pre.dt[1], rc); v1 = bit_valid(pre.v[1], rc); }
+    if (pre.n > 2) { c2 = load_raw(pre.d[2], pre.dt[2], rc); v2 = bit_valid(pre.v[2], rc); }
+    if (pre.n > 3) { c3 = load_raw(pre.d[3], pre.dt[3], rc); v3 = bit_valid(pre.v[3], rc); }
+    uint64_t passmask = 0;  // bit q: the row passes query q (nq <= 64)
     for (int qi = 0; qi < nq; qi++) {
       const SQ& Q = qs[qi];
-      if (!eval_filter(Q.f, row)) continue;
+      bool pass = in;
+      for (int t = 0; t < Q.f.n && pass; t++) {
+        const int sl = Q.term_slot[t];
+        if (sl < 0) {
+          pass = eval_term(Q.f.t[t], rc);
+        } else {
+          const Raw x = sl == 0 ? c0 : sl == 1 ? c1 : sl == 2 ? c2 : c3;
+          const bool xv = sl == 0 ? v0 : sl == 1 ? v1 : sl == 2 ? v2 : v3;
+          pass = eval_term_raw(Q.f.t[t], x, xv);
+        }
+      }
+      const uint64_t m = __ballot(pass);
+      if (lane == 0) wcnt[qi][wave] = (uint32_t)__popcll(m);
+      if (pass) passmask |= 1ull << qi;
+    }
+    __syncthreads();
+    for (int qi = 0; qi < nq; qi++) {
+      const bool pass = (passmask >> qi) & 1ull;
+      const uint64_t m = __ballot(pass);
+      if (pass) {
+        uint32_t off = running[qi];
+        for (int w = 0; w < wave; w++) off += wcnt[qi][w];
+        off += (uint32_t)__popcll(m & lt);
+        lists[((uint64_t)qi * gridDim.x + blockIdx.x) * (uint64_t)per + off] = (uint32_t)(row - r0);
+      }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < nq; q += kServeBlock) {
+      uint32_t add = 0;
+      for (int w = 0; w < kServeWaves; w++) add += wcnt[q][w];
+      running[q] += add;
+    }
+    __syncthreads();
+  }
+  for (int q = threadIdx.x; q < nq; q += kServeBlock) counts[(uint64_t)q * gridDim.x + blockIdx.x] = running[q];
+}
+
+// Step, part 2: block b runs the joins and aggregation for the rows in the
+// segments (q, b) that filter block b wrote.
+__global__ __launch_bounds__(kServeBlock) void serve_probe_kernel(const SQ* __restrict__ qs, int nq, SDims dims,
+                                                                  int64_t r0, int64_t per,
+                                                                  const uint32_t* __restrict__ lists,
+                                                                  const uint32_t* __restrict__ counts) {
+  for (int qi = 0; qi < nq; qi++) {
+    const SQ& Q = qs[qi];
+    const uint64_t seg = (uint64_t)qi * gridDim.x + blockIdx.x;
+    const uint32_t cnt = counts[seg];
+    for (uint32_t i = threadIdx.x; i < cnt; i += kServeBlock) {
+      const int64_t row = r0 + (int64_t)lists[seg * (uint64_t)per + i];
       uint32_t d0 = 0, d1 = 0, d2 = 0;
       bool ok = true;
       for (int j = 0; j < Q.n_joins && ok; j++) {
@@ -102,8 +193,8 @@ __global__ __launch_bounds__(kServeBlock) void serve_step_kernel(const SQ* __res
       uint64_t key = attr_bits(Q.g[0].attr, Q.g[0].dt, drow(Q.g[0].join));
       if (Q.n_group == 2)
         key = (key << 32) | (attr_bits(Q.g[1].attr, Q.g[1].dt, drow(Q.g[1].join)) & 0xFFFFFFFFull);
-      const int64_t s = gslot(Q.t, 0, key);
-      if (s >= 0) update_slot(Q.P, Q.t.words, Q.stride, (uint32_t)s, row);
+      const int64_t sl = gslot(Q.t, 0, key);
+      if (sl >= 0) update_slot(Q.P, Q.t.words, Q.stride, (uint32_t)sl, row);
     }
   }
 }
@@ -231,8 +322,20 @@ struct bolt_server {
   int64_t bytes_in_use = 0;
   SQ* h_desc = nullptr;
   DevBuf<SQ> d_desc;
+  DevBuf<uint32_t> d_lists;   // [slots][grid][per] chunk-local rows passing each query's predicate
+  DevBuf<uint32_t> d_counts;  // [slots][grid]
+  SPre pre{};
   bool desc_dirty = true;
   std::thread th;
+  // finisher: extraction / ORDER BY / copy-out of queries that completed
+  // their lap run here, on their own stream, while the scan goes on
+  hipStream_t fs = nullptr;
+  std::thread fin_th;
+  std::deque<std::unique_ptr<SActive>> fin_q;  // under mu
+  int fin_pending = 0;                          // under mu
+  bool fin_stop = false;                        // under mu
+  std::condition_variable cv_fin;
+  std::atomic<int64_t> step_us{0}, finish_us{0};
   // stats
   std::atomic<int64_t> steps{0}, peak_bytes{0};
   int64_t completed = 0;  // under mu
@@ -327,7 +430,7 @@ struct bolt_server {
 
   // Extract, order, limit and copy out the result of a query that has seen
   // every chunk.  Returns 1 when the query must be re-run with larger tables.
-  int finish(SActive& A, SDone& out) {
+  int finish(SActive& A, SDone& out, hipStream_t s) {
     const bolt_serve_query_t& q = A.r.q;
     uint32_t err[4];
     BOLT_HIP(hipMemcpyAsync(err, A.misc.ptr + 2, 16, hipMemcpyDeviceToHost, s));
@@ -495,8 +598,8 @@ struct bolt_server {
       {
         std::unique_lock<std::mutex> lk(mu);
         if (active.empty() && queue.empty()) {
-          if (stop) return;
-          cv_work.wait(lk, [&] { return stop || !queue.empty(); });
+          if (stop && fin_pending == 0) return;
+          cv_work.wait(lk, [&] { return !queue.empty() || (stop && fin_pending == 0); });
           if (queue.empty()) continue;
         }
         // FIFO admission at this chunk boundary: a slot and the HBM budget
@@ -504,7 +607,9 @@ struct bolt_server {
         size_t n_act = active.size();
         while (!queue.empty() && (int)(n_act + admitted.size()) < cfg.slots) {
           const SReq& r = queue.front();
-          if (bytes + r.need > cfg.hbm_budget && (n_act + admitted.size()) > 0) break;
+          // queries still being finished hold their memory too (bytes_in_use
+          // counts them); submit already declined any query over the budget
+          if (bytes + r.need > cfg.hbm_budget) break;
           bytes += r.need;
           admitted.push_back(r);
           queue.pop_front();
@@ -521,7 +626,10 @@ struct bolt_server {
           complete(*A, st, std::move(d));
           continue;
         }
-        bytes_in_use += r.need;
+        {
+          std::lock_guard<std::mutex> lk(mu);
+          bytes_in_use += r.need;
+        }
         active.push_back(std::move(A));
         desc_dirty = true;
       }
@@ -530,16 +638,40 @@ struct bolt_server {
       if (active.empty()) continue;
       if (n_chunks > 0) {
         if (desc_dirty) {
+          // union of the predicate columns of the active queries
+          memset(&pre, 0, sizeof pre);
           int i = 0;
-          for (auto& a : active) h_desc[i++] = a->desc;
+          for (auto& a : active) {
+            SQ& d = a->desc;
+            for (int t = 0; t < BOLT_MAX_PRED_TERMS; t++) d.term_slot[t] = -1;
+            for (int t = 0; t < d.f.n; t++) {
+              const DevTerm& tm = d.f.t[t];
+              int slot = -1;
+              for (int c = 0; c < pre.n; c++)
+                if (pre.d[c] == tm.data) slot = c;
+              if (slot < 0 && pre.n < kPreCols) {
+                slot = pre.n++;
+                pre.d[slot] = tm.data;
+                pre.v[slot] = tm.validity;
+                pre.dt[slot] = tm.dtype;
+              }
+              d.term_slot[t] = (int8_t)slot;
+            }
+            h_desc[i++] = d;
+          }
           (void)hipMemcpyAsync(d_desc.ptr, h_desc, sizeof(SQ) * active.size(), hipMemcpyHostToDevice, s);
           desc_dirty = false;
         }
         const int64_t r0 = cur_chunk * cfg.chunk_rows;
         const int64_t r1 = std::min(n_rows, r0 + cfg.chunk_rows);
-        serve_step_kernel<<<grid_for(r1 - r0, kServeBlock, 4, 8192), kServeBlock, 0, s>>>(
-            d_desc.ptr, (int)active.size(), sdims, r0, r1);
+        const auto t_step = SClock::now();
+        const int64_t per = ((cfg.chunk_rows + kServeGrid - 1) / kServeGrid + kServeBlock - 1) / kServeBlock * kServeBlock;
+        serve_filter_kernel<<<kServeGrid, kServeBlock, 0, s>>>(d_desc.ptr, (int)active.size(), pre, r0, r1, per,
+                                                               d_lists.ptr, d_counts.ptr);
+        serve_probe_kernel<<<kServeGrid, kServeBlock, 0, s>>>(d_desc.ptr, (int)active.size(), sdims, r0, per,
+                                                              d_lists.ptr, d_counts.ptr);
         (void)hipStreamSynchronize(s);
+        step_us += std::chrono::duration_cast<std::chrono::microseconds>(SClock::now() - t_step).count();
         cur_chunk = (cur_chunk + 1) % n_chunks;
         steps++;
       }
@@ -550,29 +682,56 @@ struct bolt_server {
           ++it;
           continue;
         }
-        clear_error();
-        SDone d;
-        int st = finish(A, d);
-        bytes_in_use -= A.r.need;
-        if (st == 1 && A.r.attempt < 3) {
-          // a group table filled up: run the query again with 4x the table
-          SReq r = A.r;
-          r.attempt++;
-          r.min_cap = A.r.cap * 4;
-          int pst = plan_request(r);
-          if (pst == BOLT_OK) {
-            std::lock_guard<std::mutex> lk(mu);
-            queue.push_front(r);
-          } else {
-            fail_request(r, pst, bolt_last_error());
-          }
-        } else {
-          if (st == 1) st = set_error(BOLT_ERR_MEMORY, "serve: group table overflow (est_groups far too low)");
-          complete(A, st, std::move(d));
+        {
+          std::lock_guard<std::mutex> lk(mu);
+          fin_q.push_back(std::move(*it));
+          fin_pending++;
         }
+        cv_fin.notify_one();
         it = active.erase(it);
         desc_dirty = true;
       }
+    }
+  }
+
+  void finisher() {
+    (void)hipSetDevice(dev);
+    for (;;) {
+      std::unique_ptr<SActive> Ap;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_fin.wait(lk, [&] { return !fin_q.empty() || fin_stop; });
+        if (fin_q.empty()) return;
+        Ap = std::move(fin_q.front());
+        fin_q.pop_front();
+      }
+      SActive& A = *Ap;
+      const auto t0f = SClock::now();
+      clear_error();
+      SDone d;
+      int st = finish(A, d, fs);
+      bool retry = false;
+      SReq r = A.r;
+      if (st == 1 && A.r.attempt < 3) {
+        // a group table filled up: run the query again with 4x the table
+        r.attempt++;
+        r.min_cap = A.r.cap * 4;
+        const int pst = plan_request(r);
+        if (pst == BOLT_OK) retry = true;
+        else st = pst;
+      } else if (st == 1) {
+        st = set_error(BOLT_ERR_MEMORY, "serve: group table overflow (est_groups far too low)");
+      }
+      if (!retry) complete(A, st, std::move(d));
+      Ap.reset();  // frees the query's tables (stream-ordered on the scan stream)
+      finish_us += std::chrono::duration_cast<std::chrono::microseconds>(SClock::now() - t0f).count();
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        bytes_in_use -= r.need;
+        if (retry) queue.push_front(r);
+        fin_pending--;
+      }
+      cv_work.notify_all();
     }
   }
 };
@@ -599,13 +758,19 @@ extern "C" int bolt_server_create(bolt_stream_t stream, const bolt_column_t* fac
   srv->cfg.slots = cfg && cfg->slots > 0 ? cfg->slots : 8;
   srv->cfg.hbm_budget = cfg && cfg->hbm_budget > 0 ? cfg->hbm_budget : (int64_t)4 << 30;
   srv->cfg.chunk_rows = cfg && cfg->chunk_rows > 0 ? cfg->chunk_rows : (int64_t)8 << 20;
-  if (srv->cfg.slots > 64) return set_error(BOLT_ERR_PLAN, "bolt_server_create: at most 64 slots");
+  if (srv->cfg.slots > BOLT_SERVE_MAX_SLOTS)
+    return set_error(BOLT_ERR_PLAN, "bolt_server_create: at most %d slots", BOLT_SERVE_MAX_SLOTS);
+  // a chunk never exceeds the table (its row lists are allocated per slot) nor 2^31 rows
+  srv->cfg.chunk_rows = std::min<int64_t>(srv->cfg.chunk_rows, std::max<int64_t>(srv->n_rows, 1));
+  srv->cfg.chunk_rows = std::min<int64_t>(srv->cfg.chunk_rows, (int64_t)1 << 31);
   srv->n_chunks = (srv->n_rows + srv->cfg.chunk_rows - 1) / srv->cfg.chunk_rows;
   memset(&srv->sdims, 0, sizeof srv->sdims);
   auto cleanup = [&]() {
     for (auto* t : srv->dim_tables) bolt_join_table_release(t);
     srv->dim_tables.clear();
     srv->d_desc.reset();
+    srv->d_lists.reset();
+    srv->d_counts.reset();
     (void)hipStreamSynchronize(srv->s);
     (void)hipStreamDestroy(srv->s);
   };
@@ -647,6 +812,12 @@ extern "C" int bolt_server_create(bolt_stream_t stream, const bolt_column_t* fac
     }
   }
   int st = srv->d_desc.alloc(srv->cfg.slots, srv->s);
+  {
+    const int64_t per =
+        ((srv->cfg.chunk_rows + kServeGrid - 1) / kServeGrid + kServeBlock - 1) / kServeBlock * kServeBlock;
+    if (st == BOLT_OK) st = srv->d_lists.alloc((size_t)srv->cfg.slots * kServeGrid * (size_t)per, srv->s);
+    if (st == BOLT_OK) st = srv->d_counts.alloc((size_t)srv->cfg.slots * kServeGrid, srv->s);
+  }
   if (st == BOLT_OK && hipHostMalloc(reinterpret_cast<void**>(&srv->h_desc), sizeof(SQ) * srv->cfg.slots,
                                      hipHostMallocDefault) != hipSuccess)
     st = set_error(BOLT_ERR_MEMORY, "bolt_server_create: pinned descriptor buffer");
@@ -656,8 +827,13 @@ extern "C" int bolt_server_create(bolt_stream_t stream, const bolt_column_t* fac
     cleanup();
     return st;
   }
+  if (hipStreamCreateWithFlags(&srv->fs, hipStreamNonBlocking) != hipSuccess) {
+    cleanup();
+    return set_error(BOLT_ERR_DEVICE, "bolt_server_create: finisher stream");
+  }
   bolt_server* p = srv.release();
   p->th = std::thread([p] { p->loop(); });
+  p->fin_th = std::thread([p] { p->finisher(); });
   *out = p;
   return BOLT_OK;
 }
@@ -723,7 +899,7 @@ extern "C" void bolt_server_result_release(bolt_server_t* srv, int64_t ticket) {
 extern "C" double bolt_server_now_ms(const bolt_server_t* srv) { return srv ? srv->now_ms() : 0.0; }
 
 extern "C" int bolt_server_stats(const bolt_server_t* srv, int64_t* steps, int64_t* completed, int32_t* peak_active,
-                                 int64_t* peak_bytes) {
+                                 int64_t* peak_bytes, double* step_ms, double* finish_ms) {
   if (!srv) return set_error(BOLT_ERR_PLAN, "bolt_server_stats: server is NULL");
   bolt_server* s = const_cast<bolt_server*>(srv);
   std::lock_guard<std::mutex> lk(s->mu);
@@ -731,6 +907,8 @@ extern "C" int bolt_server_stats(const bolt_server_t* srv, int64_t* steps, int64
   if (completed) *completed = s->completed;
   if (peak_active) *peak_active = s->peak_active.load();
   if (peak_bytes) *peak_bytes = s->peak_bytes.load();
+  if (step_ms) *step_ms = s->step_us.load() / 1000.0;
+  if (finish_ms) *finish_ms = s->finish_us.load() / 1000.0;
   return BOLT_OK;
 }
 
@@ -741,11 +919,20 @@ extern "C" void bolt_server_destroy(bolt_server_t* srv) {
     srv->stop = true;
   }
   srv->cv_work.notify_all();
-  if (srv->th.joinable()) srv->th.join();
+  if (srv->th.joinable()) srv->th.join();  // returns once the queue, the scan and the finisher are drained
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    srv->fin_stop = true;
+  }
+  srv->cv_fin.notify_all();
+  if (srv->fin_th.joinable()) srv->fin_th.join();
   srv->active.clear();
   srv->d_desc.reset();
+  srv->d_lists.reset();
+  srv->d_counts.reset();
   for (auto* t : srv->dim_tables) bolt_join_table_release(t);
   (void)hipStreamSynchronize(srv->s);
   (void)hipStreamDestroy(srv->s);
+  if (srv->fs) (void)hipStreamDestroy(srv->fs);
   delete srv;
 }

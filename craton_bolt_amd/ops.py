@@ -301,8 +301,11 @@ class Server:
 
     def stats(self) -> dict:
         st, done, pa, pb = C.c_int64(), C.c_int64(), C.c_int32(), C.c_int64()
-        N.check(N.lib().bolt_server_stats(self._h, C.byref(st), C.byref(done), C.byref(pa), C.byref(pb)))
-        return {"steps": st.value, "completed": done.value, "peak_active": pa.value, "peak_bytes": pb.value}
+        sm, fm = C.c_double(), C.c_double()
+        N.check(N.lib().bolt_server_stats(self._h, C.byref(st), C.byref(done), C.byref(pa), C.byref(pb),
+                                          C.byref(sm), C.byref(fm)))
+        return {"steps": st.value, "completed": done.value, "peak_active": pa.value, "peak_bytes": pb.value,
+                "step_ms": sm.value, "finish_ms": fm.value}
 
     def close(self) -> None:
         if self._h:

@@ -43,11 +43,12 @@ extern "C" {
 #define BOLT_SERVE_MAX_AGGS 4
 #define BOLT_SERVE_MAX_TERMS 4
 #define BOLT_SERVE_MAX_LIMIT 1024
+#define BOLT_SERVE_MAX_SLOTS 64
 
 typedef struct bolt_server bolt_server_t;
 
 typedef struct bolt_serve_config {
-  int32_t slots;          /* queries overlapping on the device (>= 1) */
+  int32_t slots;          /* queries overlapping on the device (1 .. BOLT_SERVE_MAX_SLOTS) */
   int64_t hbm_budget;     /* bytes of hash tables + scratch the admitted queries may hold */
   int64_t chunk_rows;     /* fact rows per step (0 = default) */
 } bolt_serve_config_t;
@@ -108,9 +109,11 @@ void bolt_server_result_release(bolt_server_t* srv, int64_t ticket);
 /* Milliseconds on the server clock (for arrival schedules). */
 double bolt_server_now_ms(const bolt_server_t* srv);
 /* Counters: steps run, queries completed, peak concurrent queries, peak
- * admitted HBM bytes. */
+ * admitted HBM bytes, and the milliseconds spent in scan steps (launch to
+ * completion) and in finishing queries (extraction, ORDER BY, copy-out; done
+ * on a second stream, overlapping the scan). */
 int bolt_server_stats(const bolt_server_t* srv, int64_t* steps, int64_t* completed, int32_t* peak_active,
-                      int64_t* peak_bytes);
+                      int64_t* peak_bytes, double* step_ms, double* finish_ms);
 /* Drains the queue, stops the thread and frees the device state. */
 void bolt_server_destroy(bolt_server_t* srv);
 
