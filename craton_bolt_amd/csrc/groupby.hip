@@ -59,8 +59,10 @@ struct PartOut {
   unsigned long long* words; // [words][cap]
   int64_t cap;
   unsigned long long* count;
-  uint32_t* spill;           // [n] row ids
-  unsigned long long* spill_count;
+  uint32_t* spill;           // [grid][per] row ids: each workgroup spills into its own segment
+  unsigned long long* spill_count;  // total spilled rows (one atomic per workgroup)
+  uint32_t* spill_seg;       // [grid] rows in each workgroup's segment
+  int64_t per;               // segment capacity
   uint32_t* hll;             // [kHllRegs]
 };
 
@@ -71,14 +73,14 @@ __global__ __launch_bounds__(kGbBlock) void gb_lds_kernel(KeySpec ks, DevFilter 
   unsigned long long* skeys = sm;
   unsigned long long* swords = sm + SS;
   uint32_t* shll = reinterpret_cast<uint32_t*>(swords + (size_t)P.words * SS);
-  uint32_t* sused = shll + kHllRegs;  // [2]
+  uint32_t* sused = shll + kHllRegs;  // [2] specials used, [2] spill count
   for (int i = threadIdx.x; i < SS; i += kGbBlock) skeys[i] = kEmpty;
   for (int w = 0; w < P.words; w++) {
     unsigned long long id = word_identity(P, w);
     for (int i = threadIdx.x; i < SS; i += kGbBlock) swords[(size_t)w * SS + i] = id;
   }
   for (int i = threadIdx.x; i < kHllRegs; i += kGbBlock) shll[i] = 0;
-  if (threadIdx.x < 2) sused[threadIdx.x] = 0;
+  if (threadIdx.x < 3) sused[threadIdx.x] = 0;
   __syncthreads();
 
   const int64_t stride = (int64_t)gridDim.x * kGbBlock;
@@ -112,13 +114,27 @@ __global__ __launch_bounds__(kGbBlock) void gb_lds_kernel(KeySpec ks, DevFilter 
       }
       if (slot >= 0) update_slot(P, swords, SS, (uint32_t)slot, row);
     }
-    unsigned long long pos = wave_append(spill, out.spill_count);
-    if (spill) {
-      out.spill[pos] = (uint32_t)row;
-      hll_add(shll, key);
+    // append to this workgroup's own segment: one LDS atomic per wave, no
+    // global counter (a shared one serialises every wave on one address)
+    const uint64_t m = __ballot(spill);
+    if (m) {
+      const int lane = threadIdx.x & 63;
+      const int leader = __builtin_ctzll(m);
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(&sused[2], (uint32_t)__popcll(m));
+      base = __shfl(base, leader);
+      if (spill) {
+        const uint64_t below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
+        out.spill[(int64_t)blockIdx.x * out.per + base + (uint32_t)__popcll(below)] = (uint32_t)row;
+        hll_add(shll, key);
+      }
     }
   }
   __syncthreads();
+  if (threadIdx.x == 0) {
+    out.spill_seg[blockIdx.x] = sused[2];
+    if (sused[2]) atomicAdd(out.spill_count, (unsigned long long)sused[2]);
+  }
   // flush groups as partial records
   for (int base = 0; base < SS; base += kGbBlock) {
     int i = base + threadIdx.x;
@@ -178,6 +194,23 @@ __global__ __launch_bounds__(256) void gb_rows_kernel(GTable t, GPlan P, KeySpec
     int64_t s = gslot(t, kr == 2 ? 1 : 0, key);
     if (s < 0) continue;
     update_slot(P, t.words, t.cap + 2, (uint32_t)s, row);
+  }
+}
+
+// The spilled rows, segment by segment: blocks (b, k) for k < K share
+// workgroup b's segment.
+__global__ __launch_bounds__(256) void gb_rows_seg_kernel(GTable t, GPlan P, KeySpec ks, const uint32_t* spill,
+                                                         int64_t per, const uint32_t* seg_count, int K) {
+  const int b = blockIdx.x / K, k = blockIdx.x % K;
+  const uint32_t cnt = seg_count[b];
+  for (uint32_t i = (uint32_t)k * blockDim.x + threadIdx.x; i < cnt; i += (uint32_t)K * blockDim.x) {
+    const int64_t row = (int64_t)spill[(int64_t)b * per + i];
+    uint64_t key = 0;
+    const int kr = row_key(ks, row, &key);
+    if (kr == 0) continue;
+    const int64_t sl = gslot(t, kr == 2 ? 1 : 0, key);
+    if (sl < 0) continue;
+    update_slot(P, t.words, t.cap + 2, (uint32_t)sl, row);
   }
 }
 
@@ -517,29 +550,34 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
   // ---- pass A ----
   const int W = P.words;
   const int budget = lds_budget_bytes();
-  int S = (int)((budget - (kHllRegs + 2) * 4) / (8 * (1 + W))) - 2;
+  int S = (int)((budget - (kHllRegs + 4) * 4) / (8 * (1 + W))) - 2;
   if (S > 4096) S = 4096;
   const bool use_lds = !(flags & BOLT_GROUPBY_FORCE_GLOBAL) && S >= 256 && n > 0;
   int64_t n_parts = 0, n_spill = 0;
   double est = 0;
   DevBuf<unsigned long long> pkeys, pwords, counters;
   DevBuf<uint8_t> ptag;
-  DevBuf<uint32_t> spill, hll;
+  DevBuf<uint32_t> spill, hll, spill_seg;
+  unsigned grid = 0;
+  int64_t spill_per = 0;
   if (use_lds) {
-    const size_t lds = (size_t)(S + 2) * 8 * (1 + W) + (kHllRegs + 2) * 4;
+    const size_t lds = (size_t)(S + 2) * 8 * (1 + W) + (kHllRegs + 4) * 4;
     BOLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(gb_lds_kernel),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    unsigned grid = grid_for(n, kGbBlock, 16, 256);
+    grid = grid_for(n, kGbBlock, 16, 256);
     const int64_t pcap = (int64_t)grid * (S + 2);
+    spill_per = (n + (int64_t)grid * kGbBlock - 1) / ((int64_t)grid * kGbBlock) * kGbBlock;
     BOLT_TRY(pkeys.alloc((size_t)pcap, s));
     BOLT_TRY(ptag.alloc((size_t)pcap, s));
     BOLT_TRY(pwords.alloc((size_t)pcap * (W > 0 ? W : 1), s));
-    BOLT_TRY(spill.alloc((size_t)n, s));
+    BOLT_TRY(spill.alloc((size_t)grid * spill_per, s));
+    BOLT_TRY(spill_seg.alloc(grid, s));
     BOLT_TRY(hll.alloc(kHllRegs, s));
     BOLT_TRY(counters.alloc(2, s));
     BOLT_TRY(fill_u32(hll.ptr, 0, kHllRegs, s));
     BOLT_TRY(fill_u32(counters.ptr, 0, 4, s));
-    PartOut po{pkeys.ptr, ptag.ptr, pwords.ptr, pcap, counters.ptr, spill.ptr, counters.ptr + 1, hll.ptr};
+    PartOut po{pkeys.ptr, ptag.ptr, pwords.ptr, pcap, counters.ptr, spill.ptr, counters.ptr + 1, spill_seg.ptr,
+               spill_per, hll.ptr};
     gb_lds_kernel<<<grid, kGbBlock, lds, s>>>(ks, f, P, n, S, po);
     BOLT_HIP(hipGetLastError());
     unsigned long long hc[2];
@@ -576,8 +614,14 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
     if (n_parts > 0)
       gb_merge_parts_kernel<<<grid_for(n_parts, 256, 4), 256, 0, s>>>(t, P, pkeys.ptr, ptag.ptr, pwords.ptr,
                                                                       (int64_t)pkeys.count, n_parts);
-    if (n_spill > 0)
-      gb_rows_kernel<<<grid_for(n_spill, 256, 4), 256, 0, s>>>(t, P, ks, f, use_lds ? spill.ptr : nullptr, n_spill);
+    if (n_spill > 0) {
+      if (use_lds) {
+        const int K = 16;  // blocks per spill segment
+        gb_rows_seg_kernel<<<grid * K, 256, 0, s>>>(t, P, ks, spill.ptr, spill_per, spill_seg.ptr, K);
+      } else {
+        gb_rows_kernel<<<grid_for(n_spill, 256, 4), 256, 0, s>>>(t, P, ks, f, nullptr, n_spill);
+      }
+    }
     BOLT_HIP(hipGetLastError());
     uint32_t e0 = 0;
     BOLT_HIP(hipMemcpyAsync(&e0, t.err, 4, hipMemcpyDeviceToHost, s));
