@@ -22,6 +22,7 @@
 #include "jit.h"
 #include "join_dev.h"
 #include "reduce_common.h"
+#include "scan_common.h"
 
 namespace bolt {
 
@@ -135,41 +136,84 @@ __global__ __launch_bounds__(kProbeBlock) void join_probe_agg_kernel(
 }
 
 // ---- probe emitting (probe_row, build_row) pairs -----------------------------
+//
+// Tiled with a decoupled look-back, like the compaction: a tile of
+// kEmitTile probe rows counts its matches, one wave looks back across
+// earlier tiles for the tile's output offset, and the pairs are written
+// densely.  No output counter is shared (one per wave serialises every wave
+// on a single address).  Each lane owns kEmitItems consecutive rows, so the
+// pairs come out in probe-row order.  With unique build keys one walk finds each row's
+// match and keeps it in a register; with duplicates the chain is walked
+// twice (count, then emit).
+constexpr int kEmitItems = 8;
+constexpr int kEmitTile = kProbeBlock * kEmitItems;  // 2048 rows
 
-// Wave-aggregated output allocation: one atomic per wave and pass.
 template <typename KeyT>
 __global__ __launch_bounds__(kProbeBlock) void join_probe_emit_kernel(
     const uint4* __restrict__ tbl, uint64_t mask, const uint32_t* __restrict__ nonunique,
     const KeyT* __restrict__ keys, const uint8_t* __restrict__ kvalid, int64_t n, DevFilter f,
-    unsigned long long* __restrict__ counter, int64_t capacity, uint32_t* __restrict__ out_probe,
+    uint64_t* __restrict__ status, uint32_t* __restrict__ ticket, uint32_t* __restrict__ err,
+    unsigned long long* __restrict__ total_out, int64_t capacity, uint32_t* __restrict__ out_probe,
     uint32_t* __restrict__ out_build, const uint4* __restrict__ bloom, uint64_t bmask) {
+  __shared__ uint32_t sh_tile;
+  __shared__ uint32_t wsum[kProbeBlock / 64];
+  __shared__ unsigned long long sh_base;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) sh_tile = atomicAdd(ticket, 1u);
+  __syncthreads();
+  const int64_t tile = sh_tile;
+  const int64_t tbase = tile * kEmitTile;
   const bool first_only = (*nonunique == 0u);
-  const int lane = threadIdx.x & 63;
-  const int64_t stride = (int64_t)gridDim.x * kProbeBlock;
-  const int64_t base_i = (int64_t)blockIdx.x * kProbeBlock + (threadIdx.x & ~63);
-  // every lane of a wave iterates the same number of times so the wave-wide
-  // scan below is convergent
-  for (int64_t wi = base_i; wi < n; wi += stride) {
-    const int64_t i = wi + lane;
-    bool active = i < n && bit_valid(kvalid, i) && eval_filter(f, i);
-    const int64_t k = active ? (int64_t)keys[i] : 0;
-    active = active && bloom_maybe(bloom, bmask, k);
-    uint32_t m = 0;
-    if (active) probe_chain(tbl, mask, k, first_only, [&](uint32_t) { m++; });
-    // inclusive scan of m across the wave
-    uint32_t incl = m;
+  uint32_t m[kEmitItems], bv[kEmitItems];
+  uint32_t mine = 0;
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      uint32_t t = __shfl_up(incl, d);
-      if (lane >= d) incl += t;
+  for (int j = 0; j < kEmitItems; j++) {
+    const int64_t i = tbase + ((int64_t)threadIdx.x * kEmitItems + j);
+    m[j] = 0;
+    bv[j] = 0;
+    if (i < n && bit_valid(kvalid, i) && eval_filter(f, i)) {
+      const int64_t k = (int64_t)keys[i];
+      if (bloom_maybe(bloom, bmask, k))
+        probe_chain(tbl, mask, k, first_only, [&](uint32_t w) {
+          m[j]++;
+          bv[j] = w;
+        });
     }
-    const uint32_t total = __shfl(incl, 63);
-    unsigned long long base = 0;
-    if (lane == 63 && total) base = atomicAdd(counter, (unsigned long long)total);
-    base = __shfl(base, 63);
-    if (m) {
-      unsigned long long pos = base + incl - m;
-      probe_chain(tbl, mask, k, first_only, [&](uint32_t w) {
+    mine += m[j];
+  }
+  // lane prefix within the wave, wave prefix within the tile
+  uint32_t inc = mine;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t v = __shfl_up(inc, d);
+    if (lane >= d) inc += v;
+  }
+  if (lane == 63) wsum[wave] = inc;
+  __syncthreads();
+  if (wave == 0) {
+    uint32_t tile_total = 0;
+    for (int w = 0; w < kProbeBlock / 64; w++) tile_total += wsum[w];
+    const uint64_t excl = lookback_wave(status, tile, tile_total, err);
+    if (lane == 0) {
+      sh_base = excl;
+      if (tile == (int64_t)gridDim.x - 1) *total_out = excl + tile_total;
+    }
+  }
+  __syncthreads();
+  unsigned long long pos = sh_base + inc - mine;
+  for (int w = 0; w < wave; w++) pos += wsum[w];
+#pragma unroll
+  for (int j = 0; j < kEmitItems; j++) {
+    if (!m[j]) continue;
+    const int64_t i = tbase + ((int64_t)threadIdx.x * kEmitItems + j);
+    if (first_only) {
+      if ((int64_t)pos < capacity) {
+        out_probe[pos] = (uint32_t)i;
+        out_build[pos] = bv[j];
+      }
+      pos++;
+    } else {
+      probe_chain(tbl, mask, (int64_t)keys[i], false, [&](uint32_t w) {
         if ((int64_t)pos < capacity) {
           out_probe[pos] = (uint32_t)i;
           out_build[pos] = w;
@@ -449,26 +493,34 @@ extern "C" int bolt_join_probe(bolt_stream_t stream, const bolt_join_table_t* t,
   DevFilter f;
   BOLT_TRY(make_dev_filter(filter, probe_cols, n_probe_cols, key.length, &f));
   DevBuf<unsigned long long> counter;
+  DevBuf<uint64_t> status;
+  DevBuf<uint32_t> misc;  // [0] ticket, [1] look-back error
   BOLT_TRY(counter.alloc(1, s));
   BOLT_TRY(fill_u32(counter.ptr, 0, 2, s));
+  BOLT_TRY(misc.alloc(4, s));
+  BOLT_TRY(fill_u32(misc.ptr, 0, 4, s));
   if (key.length > 0) {
-    unsigned grid = probe_grid(key.length);
+    const int64_t tiles = (key.length + kEmitTile - 1) / kEmitTile;
+    BOLT_TRY(status.alloc((size_t)tiles, s));
+    BOLT_TRY(fill_u32(status.ptr, 0, (size_t)tiles * 2, s));
     if (key.dtype == BOLT_I64)
-      join_probe_emit_kernel<int64_t><<<grid, kProbeBlock, 0, s>>>(
-          t->tbl, t->mask, t->flags, static_cast<const int64_t*>(key.data), key.validity, key.length, f, counter.ptr,
-          capacity, out_probe_rows, out_build_rows, t->bloom, t->bloom_mask);
+      join_probe_emit_kernel<int64_t><<<(unsigned)tiles, kProbeBlock, 0, s>>>(
+          t->tbl, t->mask, t->flags, static_cast<const int64_t*>(key.data), key.validity, key.length, f, status.ptr,
+          misc.ptr, misc.ptr + 1, counter.ptr, capacity, out_probe_rows, out_build_rows, t->bloom, t->bloom_mask);
     else
-      join_probe_emit_kernel<int32_t><<<grid, kProbeBlock, 0, s>>>(
-          t->tbl, t->mask, t->flags, static_cast<const int32_t*>(key.data), key.validity, key.length, f, counter.ptr,
-          capacity, out_probe_rows, out_build_rows, t->bloom, t->bloom_mask);
+      join_probe_emit_kernel<int32_t><<<(unsigned)tiles, kProbeBlock, 0, s>>>(
+          t->tbl, t->mask, t->flags, static_cast<const int32_t*>(key.data), key.validity, key.length, f, status.ptr,
+          misc.ptr, misc.ptr + 1, counter.ptr, capacity, out_probe_rows, out_build_rows, t->bloom, t->bloom_mask);
     BOLT_HIP(hipGetLastError());
   }
   unsigned long long total = 0;
-  uint32_t fl[2];
+  uint32_t fl[2], hm[2];
   BOLT_HIP(hipMemcpyAsync(&total, counter.ptr, 8, hipMemcpyDeviceToHost, s));
   BOLT_HIP(hipMemcpyAsync(fl, t->flags, 8, hipMemcpyDeviceToHost, s));
+  BOLT_HIP(hipMemcpyAsync(hm, misc.ptr, 8, hipMemcpyDeviceToHost, s));
   BOLT_HIP(hipStreamSynchronize(s));
   if (fl[1] != 0) return set_error(BOLT_ERR_OTHER, "hash join build lost %u rows (table full)", fl[1]);
+  if (hm[1] != 0) return set_error(BOLT_ERR_OTHER, "hash join probe: output offset look-back timed out");
   *n_out = (int64_t)total;
   if ((int64_t)total > capacity)
     return set_error(BOLT_DECLINE, "hash join: %llu matches exceed the output capacity %lld", total,

@@ -202,8 +202,9 @@ void bolt_join_table_release(bolt_join_table_t* t);
 
 /* INNER probe emitting (probe_row, build_row) pairs for the probe rows that
  * pass `filter` (its columns index `probe_cols`, whose column `key_col` is the
- * probe key).  Pairs are written in an unspecified order; the row set equals
- * the reference's (compare after canonical ordering).  If more than
+ * probe key).  Pairs come out in ascending probe-row order (the build rows of
+ * one probe row in hash-chain order); the row set equals the reference's,
+ * whose order is unspecified (compare after canonical ordering).  If more than
  * `capacity` pairs match, nothing beyond capacity is written, *n_out is the
  * full count and BOLT_DECLINE is returned.  Synchronises. */
 int bolt_join_probe(bolt_stream_t stream, const bolt_join_table_t* t, const bolt_column_t* probe_cols,

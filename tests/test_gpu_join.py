@@ -91,6 +91,26 @@ def test_probe_capacity_declines(gpu):
         t.probe([DeviceColumn.from_numpy(np.zeros(10, np.int64))], 0, capacity=50)
 
 
+@pytest.mark.parametrize("n_probe", [2047, 2048, 2049, 300_001])
+def test_probe_pairs_in_probe_row_order(gpu, n_probe):
+    """Pairs come out sorted by probe row across the 2048-row tiles, with
+    duplicate build keys, NULL probe keys and a filter."""
+    print(f"[case] probe order n_probe={n_probe}", flush=True)
+    rng = np.random.default_rng(n_probe)
+    bk = rng.integers(0, 900, 3000).astype(np.int64)
+    pk = rng.integers(0, 1200, n_probe).astype(np.int64)
+    pv = rng.random(n_probe) > 0.1
+    amt = rng.uniform(0, 100, n_probe)
+    t = JoinTable(DeviceColumn.from_numpy(bk))
+    pr, br = t.probe([DeviceColumn.from_numpy(pk, valid=pv), DeviceColumn.from_numpy(amt)], 0,
+                     [Term(1, N.GT, 25.0)], capacity=8 * n_probe)
+    assert np.all(np.diff(pr.astype(np.int64)) >= 0)
+    keep = pv & (amt > 25.0)
+    ol, orr = O.inner_join(O.HostCol(bk, O.I64), O.HostCol(pk, O.I64, pv))
+    m = keep[orr]
+    assert np.array_equal(_canon(pr, br), _canon(orr[m].astype(np.uint32), ol[m].astype(np.uint32)))
+
+
 @pytest.mark.parametrize("n_build,n_probe", [(1, 1), (1000, 10_000), (100_000, 2_000_000), (1 << 20, 3_000_001)])
 def test_filter_join_aggregate_matches_oracle(gpu, n_build, n_probe):
     """The star-query shape: WHERE amount > 10, INNER JOIN, SUM(payload), SUM(amount), COUNT(*)."""
