@@ -43,6 +43,7 @@
 #include <vector>
 
 #include "groupby_dev.h"
+#include "jit.h"
 #include "scan_common.h"
 
 namespace bolt {
@@ -52,19 +53,6 @@ int radix_sort_u64_pairs(hipStream_t s, const uint64_t* keys_in, const uint32_t*
                          uint64_t out_xor = 0);
 
 // ---- pass A: LDS tier ------------------------------------------------------------
-
-struct PartOut {
-  unsigned long long* keys;  // [cap]
-  uint8_t* tag;              // [cap] 0 regular, 1 NULL group, 2 empty-marker key
-  unsigned long long* words; // [words][cap]
-  int64_t cap;
-  unsigned long long* count;
-  uint32_t* spill;           // [grid][per] row ids: each workgroup spills into its own segment
-  unsigned long long* spill_count;  // total spilled rows (one atomic per workgroup)
-  uint32_t* spill_seg;       // [grid] rows in each workgroup's segment
-  int64_t per;               // segment capacity
-  uint32_t* hll;             // [kHllRegs]
-};
 
 __global__ __launch_bounds__(kGbBlock) void gb_lds_kernel(KeySpec ks, DevFilter f, GPlan P, int64_t n, int S,
                                                          PartOut out) {
@@ -99,60 +87,15 @@ __global__ __launch_bounds__(kGbBlock) void gb_lds_kernel(KeySpec ks, DevFilter 
           slot = S + 1;
           sused[1] = 1;
         } else {
-          uint32_t s = (uint32_t)(((mix64(key) >> 32) * (uint64_t)S) >> 32);
-          for (int p = 0; p < kLdsProbes; p++) {
-            unsigned long long cur = skeys[s];
-            if (cur == key) { slot = (int)s; break; }
-            if (cur == kEmpty) {
-              unsigned long long prev = atomicCAS(&skeys[s], kEmpty, (unsigned long long)key);
-              if (prev == kEmpty || prev == key) { slot = (int)s; break; }
-            }
-            s = (s + 1 == (uint32_t)S) ? 0 : s + 1;
-          }
-          if (slot < 0) spill = true;
+          slot = gb_lds_slot(skeys, S, key);
+          spill = slot < 0;
         }
       }
       if (slot >= 0) update_slot(P, swords, SS, (uint32_t)slot, row);
     }
-    // append to this workgroup's own segment: one LDS atomic per wave, no
-    // global counter (a shared one serialises every wave on one address)
-    const uint64_t m = __ballot(spill);
-    if (m) {
-      const int lane = threadIdx.x & 63;
-      const int leader = __builtin_ctzll(m);
-      uint32_t base = 0;
-      if (lane == leader) base = atomicAdd(&sused[2], (uint32_t)__popcll(m));
-      base = __shfl(base, leader);
-      if (spill) {
-        const uint64_t below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
-        out.spill[(int64_t)blockIdx.x * out.per + base + (uint32_t)__popcll(below)] = (uint32_t)row;
-        hll_add(shll, key);
-      }
-    }
+    gb_spill(spill, row, key, sused, shll, out);
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    out.spill_seg[blockIdx.x] = sused[2];
-    if (sused[2]) atomicAdd(out.spill_count, (unsigned long long)sused[2]);
-  }
-  // flush groups as partial records
-  for (int base = 0; base < SS; base += kGbBlock) {
-    int i = base + threadIdx.x;
-    bool used = false;
-    uint8_t tag = 0;
-    if (i < S) used = skeys[i] != kEmpty;
-    else if (i < SS) { used = sused[i - S] != 0; tag = (uint8_t)(i - S + 1); }
-    unsigned long long pos = wave_append(used, out.count);
-    if (used) {
-      out.keys[pos] = tag == 2 ? kEmpty : (tag == 1 ? 0ull : skeys[i]);
-      out.tag[pos] = tag;
-      for (int w = 0; w < P.words; w++) out.words[(size_t)w * out.cap + pos] = swords[(size_t)w * SS + i];
-      if (tag == 0) hll_add(shll, skeys[i]);
-    }
-  }
-  __syncthreads();
-  for (int r = threadIdx.x; r < kHllRegs; r += kGbBlock)
-    if (shll[r]) atomicMax(&out.hll[r], shll[r]);
+  gb_lds_flush(skeys, swords, S, P.words, shll, sused, out);
 }
 
 // ---- pass B: global table ------------------------------------------------------------
@@ -493,6 +436,45 @@ uint64_t bolt::next_pow2(uint64_t x) {
   return p;
 }
 
+// The generated LDS-tier kernel's shape for this query (filter, keys, plan).
+static int gb_jit_spec(const KeySpec& ks, const GPlan& P, const DevFilter& f, GbJitSpec* sp, JitLits* lits,
+                       JitCols* cols) {
+  *sp = GbJitSpec();
+  memset(lits, 0, sizeof *lits);
+  memset(cols, 0, sizeof *cols);
+  if (const char* u = getenv("BOLT_GB_UNROLL")) {
+    const int v = atoi(u);
+    if (v >= 1 && v <= 8) sp->unroll = v;
+  }
+  sp->n_keys = ks.n_keys;
+  sp->t0 = ks.t0;
+  sp->t1 = ks.t1;
+  sp->kv0 = ks.v0 != nullptr;
+  sp->kv1 = ks.n_keys == 2 && ks.v1 != nullptr;
+  sp->words = P.words;
+  sp->ident.assign(P.words, 0ull);
+  for (int a = 0; a < P.n; a++)
+    if (P.a[a].kind == K_MINMAX) sp->ident[P.a[a].w] = P.a[a].op == BOLT_AGG_MIN ? ~0ull : 0ull;
+  for (int t = 0; t < f.n; t++) {
+    const int c = jit_row_agg_col(f.t[t].data, f.t[t].validity, f.t[t].dtype, &sp->cols, cols);
+    if (c < 0) return set_error(BOLT_ERR_UNSUPPORTED, "jit: too many columns");
+    sp->terms.push_back({c, f.t[t].op});
+    lits->i[t] = f.t[t].lit_i64;
+    lits->f[t] = f.t[t].lit_f64;
+    lits->hi[t] = f.t[t].lit_hi;
+  }
+  for (int a = 0; a < P.n; a++) {
+    const GAgg& g = P.a[a];
+    GbJitSpec::Agg ja{g.op, g.kind, g.dtype, -1, g.w, g.nw};
+    if (g.op != BOLT_AGG_COUNT_STAR) {
+      ja.col = jit_row_agg_col(g.data, g.validity, g.dtype, &sp->cols, cols);
+      if (ja.col < 0) return set_error(BOLT_ERR_UNSUPPORTED, "jit: too many columns");
+    }
+    sp->aggs.push_back(ja);
+  }
+  return BOLT_OK;
+}
+
 static int lds_budget_bytes() {
   static int cached = -1;
   if (cached < 0) {
@@ -562,11 +544,24 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
   int64_t spill_per = 0;
   if (use_lds) {
     const size_t lds = (size_t)(S + 2) * 8 * (1 + W) + (kHllRegs + 4) * 4;
-    BOLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(gb_lds_kernel),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    // the per-query generated kernel when hipRTC is available, else the
+    // precompiled one; rows_per_iter is what one workgroup iteration covers
+    hipFunction_t gfn = nullptr;
+    GbJitSpec gs;
+    JitLits glits;
+    JitCols gcols;
+    if (jit_enabled() && gb_jit_spec(ks, P, f, &gs, &glits, &gcols) == BOLT_OK &&
+        jit_get(jit_gb_lds_source(gs), &gfn) != BOLT_OK)
+      gfn = nullptr;
+    clear_error();
+    const int64_t rows_per_iter = (int64_t)kGbBlock * (gfn ? gs.unroll : 1);
+    if (!gfn)
+      BOLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(gb_lds_kernel),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     grid = grid_for(n, kGbBlock, 16, 256);
     const int64_t pcap = (int64_t)grid * (S + 2);
-    spill_per = (n + (int64_t)grid * kGbBlock - 1) / ((int64_t)grid * kGbBlock) * kGbBlock;
+    // a workgroup handles at most ceil(n / (grid * rows_per_iter)) iterations
+    spill_per = (n + (int64_t)grid * rows_per_iter - 1) / ((int64_t)grid * rows_per_iter) * rows_per_iter;
     BOLT_TRY(pkeys.alloc((size_t)pcap, s));
     BOLT_TRY(ptag.alloc((size_t)pcap, s));
     BOLT_TRY(pwords.alloc((size_t)pcap * (W > 0 ? W : 1), s));
@@ -578,7 +573,14 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
     BOLT_TRY(fill_u32(counters.ptr, 0, 4, s));
     PartOut po{pkeys.ptr, ptag.ptr, pwords.ptr, pcap, counters.ptr, spill.ptr, counters.ptr + 1, spill_seg.ptr,
                spill_per, hll.ptr};
-    gb_lds_kernel<<<grid, kGbBlock, lds, s>>>(ks, f, P, n, S, po);
+    if (gfn) {
+      long long nn = n;
+      int SSz = S;
+      void* args[] = {&nn, &SSz, (void*)&ks.d0, (void*)&ks.v0, (void*)&ks.d1, (void*)&ks.v1, &glits, &gcols, &po};
+      BOLT_HIP(hipModuleLaunchKernel(gfn, grid, 1, 1, kGbBlock, 1, 1, (unsigned)lds, s, args, nullptr));
+    } else {
+      gb_lds_kernel<<<grid, kGbBlock, lds, s>>>(ks, f, P, n, S, po);
+    }
     BOLT_HIP(hipGetLastError());
     unsigned long long hc[2];
     std::vector<uint32_t> regs(kHllRegs);

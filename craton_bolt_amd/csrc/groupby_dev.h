@@ -5,14 +5,11 @@
 #pragma once
 
 #include "ops_common.h"
+// after ops_common.h: gb_lds.h builds on device_acc.h
+#include "gb_lds.h"
 
 namespace bolt {
 
-constexpr uint64_t kEmpty = ~0ull;
-constexpr int kGbBlock = 1024;
-constexpr int kHllBits = 11;
-constexpr int kHllRegs = 1 << kHllBits;
-constexpr int kLdsProbes = 16;
 constexpr int kGlobalProbes = 1 << 16;
 
 enum Kind { K_ISUM = 1, K_DSUM, K_FSUM, K_AVG, K_COUNT, K_MINMAX, K_VAR };
@@ -49,18 +46,8 @@ __device__ __forceinline__ uint64_t key_bits(const void* d, int dt, int64_t row)
   switch (dt) {
     case BOLT_I32: case BOLT_DATE32: return (uint64_t)static_cast<const uint32_t*>(d)[row];
     case BOLT_I64: return static_cast<const uint64_t*>(d)[row];
-    case BOLT_F32: {
-      uint32_t b = static_cast<const uint32_t*>(d)[row];
-      if ((b & 0x7FFFFFFFu) > 0x7F800000u) b = 0x7FC00000u;  // NaN -> canonical
-      else if ((b & 0x7FFFFFFFu) == 0) b = 0;                // -0.0 -> +0.0
-      return b;
-    }
-    case BOLT_F64: {
-      uint64_t b = static_cast<const uint64_t*>(d)[row];
-      if ((b & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull) b = 0x7FF8000000000000ull;
-      else if ((b & 0x7FFFFFFFFFFFFFFFull) == 0) b = 0;
-      return b;
-    }
+    case BOLT_F32: return canon_f32_bits(static_cast<const uint32_t*>(d)[row]);
+    case BOLT_F64: return canon_f64_bits(static_cast<const uint64_t*>(d)[row]);
   }
   return 0;
 }
@@ -85,44 +72,6 @@ __host__ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
   x *= 0xc4ceb9fe1a85ec53ull;
   x ^= x >> 33;
   return x;
-}
-
-// HyperLogLog register update from an independent hash of the key.
-__device__ __forceinline__ void hll_add(uint32_t* regs, uint64_t key) {
-  uint64_t h = mix64(key ^ 0x9E3779B97F4A7C15ull);
-  uint32_t idx = (uint32_t)(h >> (64 - kHllBits));
-  uint64_t rest = (h << kHllBits) | (1ull << (kHllBits - 1));
-  uint32_t rho = (uint32_t)__clzll(rest) + 1;
-  atomicMax(&regs[idx], rho);
-}
-
-// ---- accumulator updates -------------------------------------------------------
-
-// Carry-propagating 128-bit add: the carry out of the low word is computed
-// from the value the atomic returned, so concurrent adds compose exactly.
-__device__ __forceinline__ void atomic_add128(unsigned long long* lo, unsigned long long* hi, uint64_t vlo,
-                                              uint64_t vhi) {
-  unsigned long long old = atomicAdd(lo, (unsigned long long)vlo);
-  uint64_t carry = (old + vlo) < old;
-  uint64_t add_hi = vhi + carry;
-  if (add_hi) atomicAdd(hi, (unsigned long long)add_hi);
-}
-
-// 192-bit add of a sign-extended 128-bit value (Decimal128 SUM): detects
-// overflow of the i128 result exactly at finalize.
-__device__ __forceinline__ void atomic_add192(unsigned long long* w, size_t stride, uint64_t vlo, uint64_t vhi,
-                                              uint64_t vtop) {
-  unsigned long long old = atomicAdd(&w[0], (unsigned long long)vlo);
-  uint64_t c1 = (old + vlo) < old;
-  uint64_t mid_add = vhi + c1;
-  uint64_t ca = (c1 && vhi == ~0ull) ? 1 : 0;  // carry out of vhi + c1
-  uint64_t c2 = 0;
-  if (mid_add) {
-    unsigned long long old2 = atomicAdd(&w[stride], (unsigned long long)mid_add);
-    c2 = (old2 + mid_add) < old2;
-  }
-  uint64_t top_add = vtop + ca + c2;
-  if (top_add) atomicAdd(&w[2 * stride], (unsigned long long)top_add);
 }
 
 // Fold row `row`'s inputs into slot `slot` of a word array with `stride`
@@ -232,20 +181,6 @@ __device__ __forceinline__ uint32_t global_find(const unsigned long long* keys, 
   }
   return 0xFFFFFFFFu;
 }
-
-// Wave-aggregated append: returns this lane's position for lanes with `want`.
-__device__ __forceinline__ unsigned long long wave_append(bool want, unsigned long long* counter) {
-  uint64_t m = __ballot(want);
-  if (m == 0) return 0;
-  int lane = threadIdx.x & 63;
-  int leader = __builtin_ctzll(m);
-  unsigned long long base = 0;
-  if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
-  base = __shfl(base, leader);
-  uint64_t below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
-  return base + (unsigned long long)__popcll(below);
-}
-
 
 struct GTable {
   unsigned long long* keys;   // [cap]

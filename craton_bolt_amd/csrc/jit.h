@@ -37,11 +37,35 @@ struct JitSpec {
   std::vector<Agg> aggs;
 };
 
+// The shape of a generated GROUP BY LDS-tier kernel: key encoding, filter,
+// and each aggregate's accumulator words (the layout of the precompiled
+// kernel's plan, so both produce the same partial records).
+struct GbJitSpec {
+  struct Agg {
+    int op;
+    int kind;   // accumulator kind of the plan (integer / decimal / float sum, count, min-max, ...)
+    int dtype;
+    int col;    // input column (-1 for COUNT(*))
+    int w, nw;  // first word, count word (-1 = none)
+  };
+  int unroll = 8;  // rows per lane per block iteration (1..8)
+  int n_keys = 1;
+  int t0 = 0, t1 = 0;
+  bool kv0 = false, kv1 = false;
+  int words = 0;
+  std::vector<unsigned long long> ident;  // identity of each word
+  std::vector<JitSpec::Col> cols;
+  std::vector<JitSpec::Term> terms;
+  std::vector<Agg> aggs;
+};
+
 bool jit_enabled();
 std::string jit_row_agg_source(const JitSpec& sp);
 int jit_get(const std::string& src, hipFunction_t* fn);
 int jit_spec_from(const DevFilter& f, const AggList& aggs, bool probe, int key_dtype, bool key_valid, int carry_dtype,
                   JitSpec* sp, JitLits* lits, JitCols* cols);
+std::string jit_gb_lds_source(const GbJitSpec& sp);
+int jit_row_agg_col(const void* d, const uint8_t* v, int dt, std::vector<JitSpec::Col>* cols, JitCols* jc);
 int jit_launch_row_agg(hipFunction_t fn, unsigned grid, hipStream_t s, const uint4* tbl, uint64_t mask,
                        const uint32_t* flags, int64_t n, const void* key, const uint8_t* kvalid, const JitLits& lits,
                        const JitCols& cols, Acc* parts, const uint4* bloom, uint64_t bloom_mask);

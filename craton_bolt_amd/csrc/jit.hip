@@ -27,6 +27,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "groupby_dev.h"
 #include "jit.h"
 #include "jit_prelude.inc"
 
@@ -76,6 +77,39 @@ const char* cmp_op(int op) {
     case BOLT_CMP_NE: return "!=";
   }
   return "==";
+}
+
+// Predicate term t of row r (row index variable `i`, loaded value x<col>_<r>).
+std::string term_expr(const std::vector<JitSpec::Col>& cols, const JitSpec::Term& tm, int t, int r,
+                      const std::string& i) {
+  const int c = tm.col;
+  const bool hv = cols[c].has_valid;
+  const int dt = cols[c].dtype;
+  const std::string x = "x" + std::to_string(c) + "_" + std::to_string(r);
+  const std::string valid = hv ? "vbit(v" + std::to_string(c) + ", " + i + ")" : "true";
+  if (tm.op == BOLT_CMP_IS_NULL) return "!" + valid;
+  if (tm.op == BOLT_CMP_IS_NOT_NULL) return valid;
+  std::ostringstream e;
+  if (dt == BOLT_F32 || dt == BOLT_F64)
+    e << "((double)" << x << " " << cmp_op(tm.op) << " lits.f[" << t << "])";
+  else if (dt == BOLT_DEC128)
+    e << "(cmp128((long long)" << x << "_hi, " << x << "_lo, lits.hi[" << t << "], (unsigned long long)lits.i[" << t
+      << "]) " << cmp_op(tm.op) << " 0)";
+  else
+    e << "((long long)" << x << " " << cmp_op(tm.op) << " lits.i[" << t << "])";
+  return hv ? "(" + valid + " && " + e.str() + ")" : e.str();
+}
+
+// Pointer declarations c<k> (and v<k> for nullable columns) of the column set.
+void emit_col_ptrs(std::ostringstream& o, const std::vector<JitSpec::Col>& cols) {
+  for (int c = 0; c < (int)cols.size(); c++) {
+    if (cols[c].dtype == BOLT_DEC128)
+      o << "  const unsigned long long* __restrict__ c" << c << " = (const unsigned long long*)cols.d[" << c << "];\n";
+    else
+      o << "  const " << ctype(cols[c].dtype) << "* __restrict__ c" << c << " = (const " << ctype(cols[c].dtype)
+        << "*)cols.d[" << c << "];\n";
+    if (cols[c].has_valid) o << "  const unsigned char* __restrict__ v" << c << " = cols.v[" << c << "];\n";
+  }
 }
 
 }  // namespace
@@ -143,14 +177,7 @@ std::string jit_row_agg_source(const JitSpec& sp) {
        "    const uint4* __restrict__ bloom, unsigned long long bmask) {\n"
        "  using namespace bolt;\n";
   const int nc = (int)sp.cols.size();
-  for (int c = 0; c < nc; c++) {
-    if (sp.cols[c].dtype == BOLT_DEC128)
-      o << "  const unsigned long long* __restrict__ c" << c << " = (const unsigned long long*)cols.d[" << c << "];\n";
-    else
-      o << "  const " << ctype(sp.cols[c].dtype) << "* __restrict__ c" << c << " = (const " << ctype(sp.cols[c].dtype)
-        << "*)cols.d[" << c << "];\n";
-    if (sp.cols[c].has_valid) o << "  const unsigned char* __restrict__ v" << c << " = cols.v[" << c << "];\n";
-  }
+  emit_col_ptrs(o, sp.cols);
   const int na = (int)sp.aggs.size();
   const int R = sp.unroll;
   for (int a = 0; a < na; a++) o << "  Acc acc" << a << "; acc_init(acc" << a << ", " << sp.aggs[a].op << ");\n";
@@ -185,29 +212,8 @@ std::string jit_row_agg_source(const JitSpec& sp) {
     const std::string keep = "keep" + std::to_string(r);
     o << "    bool " << keep << " = in" << r << ";\n";
     if (sp.probe && sp.key_valid) o << "    " << keep << " = " << keep << " && vbit(kvalid, " << i << ");\n";
-    for (int t = 0; t < (int)sp.terms.size(); t++) {
-      const JitSpec::Term& tm = sp.terms[t];
-      const int c = tm.col;
-      const bool hv = sp.cols[c].has_valid;
-      const int dt = sp.cols[c].dtype;
-      const std::string x = "x" + std::to_string(c) + "_" + std::to_string(r);
-      std::string valid = hv ? "vbit(v" + std::to_string(c) + ", " + i + ")" : "true";
-      std::string expr;
-      if (tm.op == BOLT_CMP_IS_NULL) expr = "!" + valid;
-      else if (tm.op == BOLT_CMP_IS_NOT_NULL) expr = valid;
-      else {
-        std::ostringstream e;
-        if (dt == BOLT_F32 || dt == BOLT_F64)
-          e << "((double)" << x << " " << cmp_op(tm.op) << " lits.f[" << t << "])";
-        else if (dt == BOLT_DEC128)
-          e << "(cmp128((long long)" << x << "_hi, " << x << "_lo, lits.hi[" << t << "], (unsigned long long)lits.i["
-            << t << "]) " << cmp_op(tm.op) << " 0)";
-        else
-          e << "((long long)" << x << " " << cmp_op(tm.op) << " lits.i[" << t << "])";
-        expr = hv ? "(" + valid + " && " + e.str() + ")" : e.str();
-      }
-      o << "    " << keep << " = " << keep << " && " << expr << ";\n";
-    }
+    for (int t = 0; t < (int)sp.terms.size(); t++)
+      o << "    " << keep << " = " << keep << " && " << term_expr(sp.cols, sp.terms[t], t, r, i) << ";\n";
   }
   // aggregate updates of row r
   auto updates = [&](int r, const char* ind) {
@@ -297,6 +303,153 @@ std::string jit_row_agg_source(const JitSpec& sp) {
   return o.str();
 }
 
+// GROUP BY LDS tier (the generated form of gb_lds_kernel in groupby.hip):
+// each lane owns R rows per block iteration, 1024 apart, whose key, filter
+// and input loads are all issued before the first row touches the LDS table,
+// so R rows' HBM reads are in flight per lane instead of one.  Slot lookup,
+// spilling and the flush are the shared gb_lds.h helpers; the accumulator
+// updates are the same LDS atomics as update_slot, specialised to the plan.
+std::string jit_gb_lds_source(const GbJitSpec& sp) {
+  std::ostringstream o;
+  const int R = sp.unroll;
+  const int W = sp.words;
+  o << kJitPrelude << "\n";
+  o << "extern \"C\" __global__ void __launch_bounds__(1024) bolt_jit_kernel(\n"
+       "    long long n, int S, const void* __restrict__ k0p, const unsigned char* __restrict__ k0v,\n"
+       "    const void* __restrict__ k1p, const unsigned char* __restrict__ k1v, bolt::JitLits lits,\n"
+       "    bolt::JitCols cols, bolt::PartOut out) {\n"
+       "  using namespace bolt;\n"
+       "  extern __shared__ u64_t sm[];\n"
+       "  const int SS = S + 2;\n"
+       "  u64_t* skeys = sm;\n"
+       "  u64_t* swords = sm + SS;\n"
+       "  u32_t* shll = (u32_t*)(swords + (u64_t)" << W << " * SS);\n"
+       "  u32_t* sused = shll + kHllRegs;\n"
+       "  for (int i = threadIdx.x; i < SS; i += 1024) {\n"
+       "    skeys[i] = kEmpty;\n";
+  for (int w = 0; w < W; w++) o << "    swords[" << w << " * SS + i] = " << sp.ident[w] << "ull;\n";
+  o << "  }\n"
+       "  for (int i = threadIdx.x; i < kHllRegs; i += 1024) shll[i] = 0;\n"
+       "  if (threadIdx.x < 3) sused[threadIdx.x] = 0;\n"
+       "  __syncthreads();\n";
+  auto kty = [](int dt) { return dtype_size(dt) == 4 ? "u32_t" : "u64_t"; };
+  auto kcanon = [](int dt, const std::string& v) -> std::string {
+    if (dt == BOLT_F32) return "(u64_t)canon_f32_bits(" + v + ")";
+    if (dt == BOLT_F64) return "canon_f64_bits(" + v + ")";
+    return "(u64_t)" + v;
+  };
+  o << "  const " << kty(sp.t0) << "* __restrict__ k0 = (const " << kty(sp.t0) << "*)k0p;\n";
+  if (sp.n_keys == 2) o << "  const " << kty(sp.t1) << "* __restrict__ k1 = (const " << kty(sp.t1) << "*)k1p;\n";
+  emit_col_ptrs(o, sp.cols);
+  o << "  const long long tile = 1024LL * " << R << ";\n"
+       "  const long long end = ((n + tile - 1) / tile) * tile;  // wave-uniform trip count\n"
+       "  for (long long base = (long long)blockIdx.x * tile + threadIdx.x; base < end; base += (long long)gridDim.x * tile) {\n";
+  for (int r = 0; r < R; r++) {
+    const std::string rs = std::to_string(r), i = "i" + rs, j = "j" + rs;
+    o << "    const long long " << i << " = base + " << 1024 * r << ";\n"
+      << "    const bool in" << rs << " = " << i << " < n;\n"
+      << "    const long long " << j << " = in" << rs << " ? " << i << " : n - 1;  // clamped: loads stay branch-free\n"
+      << "    const " << kty(sp.t0) << " ka" << rs << " = __builtin_nontemporal_load(k0 + " << j << ");\n";
+    if (sp.n_keys == 2) o << "    const " << kty(sp.t1) << " kb" << rs << " = __builtin_nontemporal_load(k1 + " << j << ");\n";
+    for (int c = 0; c < (int)sp.cols.size(); c++) {
+      const std::string x = "x" + std::to_string(c) + "_" + rs;
+      if (sp.cols[c].dtype == BOLT_DEC128)
+        o << "    const unsigned long long " << x << "_lo = c" << c << "[2 * " << j << "], " << x << "_hi = c" << c
+          << "[2 * " << j << " + 1];\n";
+      else if (sp.cols[c].dtype == BOLT_BOOL)
+        o << "    const unsigned char " << x << " = c" << c << "[" << j << "];\n";
+      else
+        o << "    const " << ctype(sp.cols[c].dtype) << " " << x << " = __builtin_nontemporal_load(c" << c << " + " << j
+          << ");\n";
+    }
+  }
+  for (int r = 0; r < R; r++) {
+    const std::string rs = std::to_string(r), i = "i" + rs;
+    o << "    bool keep" << rs << " = in" << rs << ";\n";
+    for (int t = 0; t < (int)sp.terms.size(); t++)
+      o << "    keep" << rs << " = keep" << rs << " && " << term_expr(sp.cols, sp.terms[t], t, r, i) << ";\n";
+    // 0 drop, 1 regular key, 2 the NULL group (single nullable key)
+    if (sp.n_keys == 1) {
+      o << "    const u64_t key" << rs << " = " << kcanon(sp.t0, "ka" + rs) << ";\n";
+      if (sp.kv0) o << "    const int kind" << rs << " = keep" << rs << " ? (vbit(k0v, " << i << ") ? 1 : 2) : 0;\n";
+      else o << "    const int kind" << rs << " = keep" << rs << " ? 1 : 0;\n";
+    } else {
+      o << "    const u64_t key" << rs << " = (" << kcanon(sp.t0, "ka" + rs) << " << 32) | " << kcanon(sp.t1, "kb" + rs)
+        << ";\n";
+      std::string kv = "keep" + rs;
+      if (sp.kv0) kv += " && vbit(k0v, " + i + ")";
+      if (sp.kv1) kv += " && vbit(k1v, " + i + ")";
+      o << "    const int kind" << rs << " = (" << kv << ") ? 1 : 0;\n";
+    }
+  }
+  for (int r = 0; r < R; r++) {
+    const std::string rs = std::to_string(r), i = "i" + rs;
+    o << "    {\n"
+         "      int slot = -1;\n"
+         "      bool spill = false;\n"
+         "      if (kind" << rs << " == 2) { slot = S; sused[0] = 1; }\n"
+         "      else if (kind" << rs << " == 1) {\n"
+         "        if (key" << rs << " == kEmpty) { slot = S + 1; sused[1] = 1; }\n"
+         "        else { slot = gb_lds_slot(skeys, S, key" << rs << "); spill = slot < 0; }\n"
+         "      }\n"
+         "      if (slot >= 0) {\n";
+    for (const GbJitSpec::Agg& g : sp.aggs) {
+      auto word = [&](int w) { return "swords[" + std::to_string(w) + " * SS + slot]"; };
+      if (g.op == BOLT_AGG_COUNT_STAR) {
+        o << "        atomicAdd(&" << word(g.w) << ", 1ull);\n";
+        continue;
+      }
+      const std::string x = "x" + std::to_string(g.col) + "_" + rs;
+      std::ostringstream u;
+      if (g.kind == K_COUNT) {
+        u << "atomicAdd(&" << word(g.w) << ", 1ull);";
+      } else {
+        if (g.nw >= 0) u << "atomicAdd(&" << word(g.nw) << ", 1ull); ";
+        switch (g.kind) {
+          case K_ISUM:
+            u << "{ const long long v_ = (long long)" << x << "; atomic_add128(&" << word(g.w) << ", &" << word(g.w + 1)
+              << ", (u64_t)v_, (u64_t)(v_ >> 63)); }";
+            break;
+          case K_DSUM:
+            u << "atomic_add192(&" << word(g.w) << ", (u64_t)SS, " << x << "_lo, " << x << "_hi, (u64_t)((long long)"
+              << x << "_hi >> 63));";
+            break;
+          case K_MINMAX: {
+            std::string k = (g.dtype == BOLT_F32 || g.dtype == BOLT_F64)
+                                ? "f64_order_key((double)" + x + ")"
+                                : "((u64_t)(long long)" + x + " ^ 0x8000000000000000ull)";
+            u << (g.op == BOLT_AGG_MIN ? "atomicMin(&" : "atomicMax(&") << word(g.w) << ", " << k << ");";
+            break;
+          }
+          default:  // float SUM, AVG, VAR / STDDEV (sum and count here, M2 in a second pass)
+            u << "unsafeAtomicAdd((double*)&" << word(g.w) << ", (double)" << x << ");";
+            break;
+        }
+      }
+      if (sp.cols[g.col].has_valid) o << "        if (vbit(v" << g.col << ", " << i << ")) { " << u.str() << " }\n";
+      else o << "        " << u.str() << "\n";
+    }
+    o << "      }\n"
+         "      gb_spill(spill, " << i << ", key" << rs << ", sused, shll, out);\n"
+         "    }\n";
+  }
+  o << "  }\n"
+       "  gb_lds_flush(skeys, swords, S, " << W << ", shll, sused, out);\n"
+       "}\n";
+  return o.str();
+}
+
+int jit_row_agg_col(const void* d, const uint8_t* v, int dt, std::vector<JitSpec::Col>* cols, JitCols* jc) {
+  for (int c = 0; c < (int)cols->size(); c++)
+    if (jc->d[c] == d && jc->v[c] == v && (*cols)[c].dtype == dt) return c;
+  if ((int)cols->size() >= kJitMaxCols) return -1;
+  const int c = (int)cols->size();
+  cols->push_back({dt, v != nullptr});
+  jc->d[c] = d;
+  jc->v[c] = v;
+  return c;
+}
+
 int jit_spec_from(const DevFilter& f, const AggList& aggs, bool probe, int key_dtype, bool key_valid, int carry_dtype,
                   JitSpec* sp, JitLits* lits, JitCols* cols) {
   *sp = JitSpec();
@@ -311,14 +464,7 @@ int jit_spec_from(const DevFilter& f, const AggList& aggs, bool probe, int key_d
     if (v >= 1 && v <= 8) sp->unroll = v;
   }
   auto col_index = [&](const void* d, const uint8_t* v, int dt) -> int {
-    for (int c = 0; c < (int)sp->cols.size(); c++)
-      if (cols->d[c] == d && cols->v[c] == v && sp->cols[c].dtype == dt) return c;
-    if ((int)sp->cols.size() >= kJitMaxCols) return -1;
-    int c = (int)sp->cols.size();
-    sp->cols.push_back({dt, v != nullptr});
-    cols->d[c] = d;
-    cols->v[c] = v;
-    return c;
+    return jit_row_agg_col(d, v, dt, &sp->cols, cols);
   };
   for (int t = 0; t < f.n; t++) {
     int c = col_index(f.t[t].data, f.t[t].validity, f.t[t].dtype);

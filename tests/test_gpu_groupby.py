@@ -22,6 +22,14 @@ pytestmark = pytest.mark.gpu
 REL_TOL = 1e-9
 
 
+@pytest.fixture(autouse=True, params=["jit", "generic"])
+def kernel_path(request, monkeypatch):
+    """Every case runs through the LDS-tier kernel generated per query with
+    hipRTC and through the precompiled generic one (BOLT_JIT=0)."""
+    monkeypatch.setenv("BOLT_JIT", "0" if request.param == "generic" else "1")
+    return request.param
+
+
 def _packed_from_output(res, key_dtypes):
     """Rebuild the reference's packed i64 key from the output key columns."""
     cols = res["keys"]
@@ -125,6 +133,18 @@ def test_cardinality_sweep(gpu, card, flags):
     run_both([(k, N.I64, None)], [(v, N.I64, None), (f, N.F64, None)],
              [(N.SUM, 0), (N.COUNT_STAR, 0), (N.MIN, 1), (N.MAX, 0), (N.SUM, 1), (N.AVG, 1)], [N.I64],
              flags=flags)
+
+
+@pytest.mark.parametrize("n", [1, 1023, 8191, 8193, 300_001])
+def test_ragged_sizes_with_spills(gpu, n):
+    """Row counts around the 1024-row and 8192-row workgroup iterations, with
+    about as many groups as rows so most rows spill out of the LDS tables."""
+    print(f"[case] groupby ragged n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    k = rng.integers(0, max(n, 2), n).astype(np.int64)
+    v = rng.integers(-1000, 1000, n).astype(np.int64)
+    kv = rng.random(n) > 0.05
+    run_both([(k, N.I64, kv)], [(v, N.I64, None)], [(N.SUM, 0), (N.COUNT_STAR, 0), (N.MIN, 0)], [N.I64])
 
 
 @pytest.mark.parametrize("kdt", [N.I32, N.I64, N.F32, N.F64, N.DATE32])
