@@ -249,10 +249,15 @@ def bench_sort(args) -> dict:
 
 
 def bench_groupby(args, zipf: bool) -> dict:
+    """One bolt_groupby call per step, result left in HBM (the table is
+    released, not downloaded).  The roofline is the LDS-tier pass (pass A),
+    timed by the library's own events on the operator's stream."""
+    import ctypes as C
+
     from craton_bolt_amd import _native as N
     from craton_bolt_amd import datagen as D
-    from craton_bolt_amd.device import DeviceColumn, Stream
-    from craton_bolt_amd.ops import execute_groupby
+    from craton_bolt_amd.device import DeviceColumn, Event, Stream, columns_array
+    from craton_bolt_amd.ops import _aggs
 
     n = args.n
     s = Stream()
@@ -260,23 +265,52 @@ def bench_groupby(args, zipf: bool) -> dict:
         k, v = D.zipf_groupby(n, n_ranks=10_000_000)
         aggs = [(N.SUM, 0), (N.COUNT_STAR, 0)]
         desc = f"high-cardinality GROUP BY: {n} rows, int64 key Zipf(1.1) over 10M ranks, SUM(float64), COUNT(*)"
-        per_row, dtype = 16, "f64"
+        per_row, dtype, flags = 16, "f64", 0
     else:
         k, v = D.groupby_small(n)
         aggs = [(N.SUM, 0), (N.COUNT_STAR, 0), (N.MIN, 0), (N.MAX, 0)]
-        desc = f"plain GROUP BY: {n} rows, int32 key over 1000 values, int64 value; SUM, COUNT(*), MIN, MAX"
-        per_row, dtype = 12, "int64"
+        desc = (f"plain GROUP BY: {n} rows, int32 key over 1000 values, int64 value; SUM, COUNT(*), MIN, MAX; "
+                f"output ordered by key")
+        per_row, dtype, flags = 12, "int64", N.GROUPBY_SORTED
     dk, dv = DeviceColumn.from_numpy(k), DeviceColumn.from_numpy(v)
-    res = {}
+    ka, ia, ag = columns_array([dk]), columns_array([dv]), _aggs(aggs)
+    L = N.lib()
+    ev = [Event(), Event()]
+    kev = [Event(), Event()]  # recorded by the library around pass A
+    groups = {}
 
-    def run():
-        res["r"] = execute_groupby([dk], [dv], aggs, sorted_output=not zipf, stream=s)
+    def one(record: bool):
+        tbl = C.c_void_p()
+        if record:
+            ev[0].record(s)
+        N.check(L.bolt_groupby(s.handle, ka, 1, ia, 1, ag, len(aggs), None, flags, C.byref(tbl)), "bolt_groupby")
+        if record:
+            ev[1].record(s)
+        nr = C.c_int64()
+        L.bolt_table_num_rows(tbl, C.byref(nr))
+        groups["g"] = nr.value
+        L.bolt_table_release(tbl)
 
-    ms, wall = _timed(s, run, args.steps, args.warmup)
-    g = res["r"]["n_groups"]
-    algo = n * per_row + g * (8 + 8 * len(aggs))
-    return _line(n / (ms * 1e-3), "rows/s", args, ms, dtype, {"workload": desc, "n": n, "groups": g},
-                 _roof(algo, ms, "whole GROUP BY (all passes)"))
+    for _ in range(args.warmup):
+        one(False)
+    s.synchronize()
+    op_ms, k_ms = [], []
+    L.bolt_profile_events(kev[0].handle, kev[1].handle)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        one(True)
+        ev[1].synchronize()
+        op_ms.append(ev[0].elapsed_ms(ev[1]))
+        k_ms.append(kev[0].elapsed_ms(kev[1]))
+    s.synchronize()
+    wall = (time.perf_counter() - t0) * 1000.0 / args.steps
+    L.bolt_profile_events(None, None)
+    ms, kern = float(np.mean(op_ms)), float(np.mean(k_ms))
+    g = groups["g"]
+    return _line(n / (ms * 1e-3), "rows/s", args, ms, dtype,
+                 {"workload": desc, "n": n, "groups": g, "operator_ms_on_stream": ms, "pass_a_kernel_ms": kern,
+                  "wall_ms_per_call": wall},
+                 _roof(n * per_row, kern, "GROUP BY LDS-tier pass (keys + inputs read once)"))
 
 
 def bench_calib(args) -> dict:

@@ -90,11 +90,17 @@ __device__ __forceinline__ u64_t wave_append(bool want, u64_t* counter) {
   return base + (u64_t)__popcll(below);
 }
 
-// Slot of `key` (never kEmpty) in an S-slot LDS table, claiming an empty slot
-// on the way; -1 if no slot turns up within kLdsProbes.
-__device__ __forceinline__ int gb_lds_slot(u64_t* skeys, int S, u64_t key) {
-  u32_t s = (u32_t)(((fmix64(key) >> 32) * (u64_t)S) >> 32);
-  for (int p = 0; p < kLdsProbes; p++) {
+// Home slot of `key` in an S-slot LDS table: the top 32 bits of the key
+// times a 64-bit odd constant (every key bit reaches them), scaled to S.
+__device__ __forceinline__ u32_t gb_lds_home(u64_t key, int S) {
+  const u32_t h = (u32_t)((key * 0x9E3779B97F4A7C15ull) >> 32);
+  return (u32_t)(((u64_t)h * (u32_t)S) >> 32);
+}
+
+// Linear probe for `key` (never kEmpty) from slot `s`, claiming an empty slot
+// on the way; -1 if no slot turns up within `probes` slots.
+__device__ __forceinline__ int gb_lds_probe(u64_t* skeys, int S, u64_t key, u32_t s, int probes) {
+  for (int p = 0; p < probes; p++) {
     const u64_t cur = skeys[s];
     if (cur == key) return (int)s;
     if (cur == kEmpty) {
@@ -104,6 +110,11 @@ __device__ __forceinline__ int gb_lds_slot(u64_t* skeys, int S, u64_t key) {
     s = (s + 1 == (u32_t)S) ? 0 : s + 1;
   }
   return -1;
+}
+
+// Slot of `key` within kLdsProbes slots of its home, or -1.
+__device__ __forceinline__ int gb_lds_slot(u64_t* skeys, int S, u64_t key) {
+  return gb_lds_probe(skeys, S, key, gb_lds_home(key, S), kLdsProbes);
 }
 
 // Append `row` to this workgroup's spill segment (called by the whole wave;

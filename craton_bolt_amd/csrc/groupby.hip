@@ -123,6 +123,54 @@ __global__ __launch_bounds__(256) void gb_merge_parts_kernel(GTable t, GPlan P, 
   }
 }
 
+// Partial records merged in two levels when the groups are few: each
+// workgroup folds its share of the records into an LDS table (LDS atomics),
+// then merges that table into the global one, so a group receives one
+// global update per merging workgroup instead of one per LDS-tier workgroup
+// (the latter all hit the same few words).  Records that find no LDS slot go
+// to the global table directly.
+__global__ __launch_bounds__(kGbBlock) void gb_merge_lds_kernel(GTable t, GPlan P, const unsigned long long* pkeys,
+                                                               const uint8_t* ptag, const unsigned long long* pwords,
+                                                               int64_t pcap, int64_t np, int S) {
+  extern __shared__ unsigned long long sm[];
+  const int SS = S + 2;
+  unsigned long long* skeys = sm;
+  unsigned long long* swords = sm + SS;
+  uint32_t* sused = reinterpret_cast<uint32_t*>(swords + (size_t)P.words * SS);
+  for (int i = threadIdx.x; i < SS; i += kGbBlock) skeys[i] = kEmpty;
+  for (int w = 0; w < P.words; w++) {
+    const unsigned long long id = word_identity(P, w);
+    for (int i = threadIdx.x; i < SS; i += kGbBlock) swords[(size_t)w * SS + i] = id;
+  }
+  if (threadIdx.x < 2) sused[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * kGbBlock;
+  for (int64_t r = (int64_t)blockIdx.x * kGbBlock + threadIdx.x; r < np; r += stride) {
+    const uint8_t tag = ptag[r];
+    const unsigned long long key = pkeys[r];
+    int slot;
+    if (tag == 1) { slot = S; sused[0] = 1; }
+    else if (tag == 2) { slot = S + 1; sused[1] = 1; }
+    else slot = gb_lds_slot(skeys, S, key);
+    if (slot >= 0) {
+      merge_slot(P, swords, SS, (uint32_t)slot, pwords + r, (size_t)pcap);
+    } else {
+      const int64_t g = gslot(t, tag, key);
+      if (g >= 0) merge_slot(P, t.words, t.cap + 2, (uint32_t)g, pwords + r, (size_t)pcap);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < SS; i += kGbBlock) {
+    int64_t g = -1;
+    if (i < S) {
+      if (skeys[i] != kEmpty) g = gslot(t, 0, skeys[i]);
+    } else if (sused[i - S]) {
+      g = gslot(t, i - S + 1, 0);
+    }
+    if (g >= 0) merge_slot(P, t.words, t.cap + 2, (uint32_t)g, swords + i, (size_t)SS);
+  }
+}
+
 // Rows processed straight against the global table: the spilled rows
 // (rows != nullptr) or every row of the input (rows == nullptr).
 __global__ __launch_bounds__(256) void gb_rows_kernel(GTable t, GPlan P, KeySpec ks, DevFilter f,
@@ -442,10 +490,6 @@ static int gb_jit_spec(const KeySpec& ks, const GPlan& P, const DevFilter& f, Gb
   *sp = GbJitSpec();
   memset(lits, 0, sizeof *lits);
   memset(cols, 0, sizeof *cols);
-  if (const char* u = getenv("BOLT_GB_UNROLL")) {
-    const int v = atoi(u);
-    if (v >= 1 && v <= 8) sp->unroll = v;
-  }
   sp->n_keys = ks.n_keys;
   sp->t0 = ks.t0;
   sp->t1 = ks.t1;
@@ -472,6 +516,15 @@ static int gb_jit_spec(const KeySpec& ks, const GPlan& P, const DevFilter& f, Gb
     }
     sp->aggs.push_back(ja);
   }
+  // rows in flight per lane: as many as the registers hold at 1024 threads
+  // (32-bit words loaded per row: keys and every column)
+  int words = ks.n_keys;
+  for (const JitSpec::Col& c : sp->cols) words += c.dtype == BOLT_DEC128 ? 4 : (dtype_size(c.dtype) + 3) / 4;
+  sp->unroll = words <= 6 ? 8 : words <= 12 ? 4 : 2;
+  if (const char* u = getenv("BOLT_GB_UNROLL")) {
+    const int v = atoi(u);
+    if (v >= 1 && v <= 8) sp->unroll = v;
+  }
   return BOLT_OK;
 }
 
@@ -484,7 +537,12 @@ static int lds_budget_bytes() {
       (void)hipGetLastError();
       v = 65536;
     }
-    cached = std::min(v, 96 * 1024);
+    int kb = 96;  // one 1024-thread workgroup per CU either way; BOLT_GB_LDS_KB overrides
+    if (const char* e = getenv("BOLT_GB_LDS_KB")) {
+      const int x = atoi(e);
+      if (x >= 16 && x <= 160) kb = x;
+    }
+    cached = std::min(v, kb * 1024);
   }
   return cached;
 }
@@ -528,7 +586,6 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
   DevFilter f;
   BOLT_TRY(make_dev_filter(filter, inputs, n_inputs, n, &f));
 
-  prof_begin(s);
   // ---- pass A ----
   const int W = P.words;
   const int budget = lds_budget_bytes();
@@ -573,6 +630,7 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
     BOLT_TRY(fill_u32(counters.ptr, 0, 4, s));
     PartOut po{pkeys.ptr, ptag.ptr, pwords.ptr, pcap, counters.ptr, spill.ptr, counters.ptr + 1, spill_seg.ptr,
                spill_per, hll.ptr};
+    prof_begin(s);
     if (gfn) {
       long long nn = n;
       int SSz = S;
@@ -581,6 +639,7 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
     } else {
       gb_lds_kernel<<<grid, kGbBlock, lds, s>>>(ks, f, P, n, S, po);
     }
+    prof_end(s);
     BOLT_HIP(hipGetLastError());
     unsigned long long hc[2];
     std::vector<uint32_t> regs(kHllRegs);
@@ -613,15 +672,28 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
     t.err = tmisc.ptr + 4;
     t.cap = cap;
     gb_init_kernel<<<grid_for((int64_t)cap + 2, 256, 4), 256, 0, s>>>(t, P);
-    if (n_parts > 0)
-      gb_merge_parts_kernel<<<grid_for(n_parts, 256, 4), 256, 0, s>>>(t, P, pkeys.ptr, ptag.ptr, pwords.ptr,
-                                                                      (int64_t)pkeys.count, n_parts);
+    if (n_parts > 0) {
+      if (est * 2.0 < (double)S && n_parts > 4096) {
+        // few groups: two-level merge through LDS tables of the same size
+        const unsigned mgrid = (unsigned)std::min<int64_t>(64, (n_parts + 4095) / 4096);
+        const size_t mlds = (size_t)(S + 2) * 8 * (1 + W) + 16;
+        BOLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(gb_merge_lds_kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)mlds));
+        gb_merge_lds_kernel<<<mgrid, kGbBlock, mlds, s>>>(t, P, pkeys.ptr, ptag.ptr, pwords.ptr,
+                                                          (int64_t)pkeys.count, n_parts, S);
+      } else {
+        gb_merge_parts_kernel<<<grid_for(n_parts, 256, 4), 256, 0, s>>>(t, P, pkeys.ptr, ptag.ptr, pwords.ptr,
+                                                                        (int64_t)pkeys.count, n_parts);
+      }
+    }
     if (n_spill > 0) {
       if (use_lds) {
         const int K = 16;  // blocks per spill segment
         gb_rows_seg_kernel<<<grid * K, 256, 0, s>>>(t, P, ks, spill.ptr, spill_per, spill_seg.ptr, K);
       } else {
+        if (attempt == 0) prof_begin(s);
         gb_rows_kernel<<<grid_for(n_spill, 256, 4), 256, 0, s>>>(t, P, ks, f, nullptr, n_spill);
+        if (attempt == 0) prof_end(s);
       }
     }
     BOLT_HIP(hipGetLastError());
@@ -732,7 +804,6 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
       return st;
     }
   }
-  prof_end(s);
   for (auto& c : cols) c.length = n_groups;
   tbl->cols = cols;
   tbl->n_rows = n_groups;

@@ -382,6 +382,16 @@ std::string jit_gb_lds_source(const GbJitSpec& sp) {
       o << "    const int kind" << rs << " = (" << kv << ") ? 1 : 0;\n";
     }
   }
+  // home slots of all R rows read together (one LDS round trip); the rows
+  // found at home, the common case once the table is warm, skip the probe loop
+  for (int r = 0; r < R; r++) {
+    const std::string rs = std::to_string(r);
+    o << "    const u32_t home" << rs << " = gb_lds_home(key" << rs << ", S);\n";
+  }
+  for (int r = 0; r < R; r++) {
+    const std::string rs = std::to_string(r);
+    o << "    const u64_t cur" << rs << " = skeys[home" << rs << "];\n";
+  }
   for (int r = 0; r < R; r++) {
     const std::string rs = std::to_string(r), i = "i" + rs;
     o << "    {\n"
@@ -390,7 +400,13 @@ std::string jit_gb_lds_source(const GbJitSpec& sp) {
          "      if (kind" << rs << " == 2) { slot = S; sused[0] = 1; }\n"
          "      else if (kind" << rs << " == 1) {\n"
          "        if (key" << rs << " == kEmpty) { slot = S + 1; sused[1] = 1; }\n"
-         "        else { slot = gb_lds_slot(skeys, S, key" << rs << "); spill = slot < 0; }\n"
+         "        else if (cur" << rs << " == key" << rs << ") slot = (int)home" << rs << ";\n"
+         "        else {\n"
+         "          const bool taken = cur" << rs << " != kEmpty;\n"
+         "          const u32_t from = taken ? (home" << rs << " + 1 == (u32_t)S ? 0u : home" << rs << " + 1) : home" << rs << ";\n"
+         "          slot = gb_lds_probe(skeys, S, key" << rs << ", from, taken ? kLdsProbes - 1 : kLdsProbes);\n"
+         "          spill = slot < 0;\n"
+         "        }\n"
          "      }\n"
          "      if (slot >= 0) {\n";
     for (const GbJitSpec::Agg& g : sp.aggs) {
@@ -407,8 +423,11 @@ std::string jit_gb_lds_source(const GbJitSpec& sp) {
         if (g.nw >= 0) u << "atomicAdd(&" << word(g.nw) << ", 1ull); ";
         switch (g.kind) {
           case K_ISUM:
-            u << "{ const long long v_ = (long long)" << x << "; atomic_add128(&" << word(g.w) << ", &" << word(g.w + 1)
-              << ", (u64_t)v_, (u64_t)(v_ >> 63)); }";
+            // exact without returning atomics: word w sums the low 32 bits
+            // (unsigned), word w + 1 the high 32 bits (signed); fewer than
+            // 2^32 rows cannot overflow either.  Recombined before the flush.
+            u << "{ const long long v_ = (long long)" << x << "; atomicAdd(&" << word(g.w)
+              << ", (u64_t)(u32_t)v_); if (v_ >> 32) atomicAdd(&" << word(g.w + 1) << ", (u64_t)(v_ >> 32)); }";
             break;
           case K_DSUM:
             u << "atomic_add192(&" << word(g.w) << ", (u64_t)SS, " << x << "_lo, " << x << "_hi, (u64_t)((long long)"
@@ -433,8 +452,24 @@ std::string jit_gb_lds_source(const GbJitSpec& sp) {
          "      gb_spill(spill, " << i << ", key" << rs << ", sused, shll, out);\n"
          "    }\n";
   }
-  o << "  }\n"
-       "  gb_lds_flush(skeys, swords, S, " << W << ", shll, sused, out);\n"
+  o << "  }\n";
+  // integer sums back to the (lo, hi) 128-bit pair the partial records carry
+  bool any_isum = false;
+  for (const GbJitSpec::Agg& g : sp.aggs) any_isum |= g.kind == K_ISUM;
+  if (any_isum) {
+    o << "  __syncthreads();\n"
+         "  for (int i = threadIdx.x; i < SS; i += 1024) {\n";
+    for (const GbJitSpec::Agg& g : sp.aggs) {
+      if (g.kind != K_ISUM) continue;
+      const std::string lo = "swords[" + std::to_string(g.w) + " * SS + i]";
+      const std::string hi = "swords[" + std::to_string(g.w + 1) + " * SS + i]";
+      o << "    { const u64_t u_ = " << lo << "; const long long t_ = (long long)" << hi << ";\n"
+        << "      const u64_t lo0 = (u64_t)t_ << 32, lo1 = lo0 + u_;\n"
+        << "      " << lo << " = lo1; " << hi << " = (u64_t)(t_ >> 32) + (lo1 < lo0 ? 1ull : 0ull); }\n";
+    }
+    o << "  }\n";
+  }
+  o << "  gb_lds_flush(skeys, swords, S, " << W << ", shll, sused, out);\n"
        "}\n";
   return o.str();
 }

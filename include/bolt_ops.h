@@ -261,8 +261,9 @@ const char* bolt_build_info(void);
  * the calling thread, the operators record `start` immediately before and
  * `end` immediately after their dominant kernel, on the stream that kernel
  * runs on (bolt_join_probe_aggregate: the fused probe; bolt_aggregate: the
- * row pass; bolt_sort_pairs_i64: the whole sort; bolt_groupby: the whole
- * operator), so that bolt_hip_event_elapsed_ms gives the kernel's own time.
+ * row pass; bolt_sort_pairs_i64: the whole sort; bolt_groupby: the pass over
+ * the input rows, i.e. the LDS tier, or the global-table pass when the LDS
+ * tier is off), so that bolt_hip_event_elapsed_ms gives the kernel's own time.
  * Pass NULL, NULL to clear. */
 int bolt_profile_events(bolt_event_t start, bolt_event_t end);
 

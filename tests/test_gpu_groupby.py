@@ -216,8 +216,10 @@ def test_integer_sum_overflow_is_an_error(gpu):
         execute_groupby([both(k, N.I32, None)[0]], [both(v, N.I64, None)[0]], [(N.SUM, 0)])
 
 
-def test_decimal_sum_by_group(gpu):
-    n = 50_000
+@pytest.mark.parametrize("n", [50_000, 1_500_000])
+def test_decimal_sum_by_group(gpu, n):
+    # 1.5M rows: enough LDS-tier workgroups that the partial records take the
+    # two-level (LDS, then global) merge
     rng = np.random.default_rng(11)
     k = rng.integers(0, 100, n).astype(np.int32)
     vals = [int(x) * 10 ** 18 + int(y) for x, y in zip(rng.integers(-10 ** 12, 10 ** 12, n), rng.integers(0, 10 ** 18, n))]
