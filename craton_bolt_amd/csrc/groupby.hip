@@ -244,22 +244,25 @@ __global__ __launch_bounds__(256) void gb_m2_kernel(GTable t, GPlan P, KeySpec k
 constexpr int kExBlock = 256;
 constexpr int kExItems = 16;
 
+// ITEMS slots per thread: 16 for large tables, 1 for small ones (more tiles,
+// so a table of a few thousand slots is not left to one or two workgroups).
+template <int ITEMS>
 __global__ __launch_bounds__(kExBlock) void gb_extract_kernel(GTable t, GPlan P, KeySpec ks, OutSpec o,
                                                              uint64_t* __restrict__ status,
                                                              uint32_t* __restrict__ ticket,
                                                              unsigned long long* __restrict__ total_out) {
   __shared__ uint32_t sh_tile;
-  __shared__ uint32_t sh_cnt[kExItems * 4];
+  __shared__ uint32_t sh_cnt[ITEMS * 4];
   __shared__ uint64_t sh_prefix;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (threadIdx.x == 0) sh_tile = atomicAdd(ticket, 1u);
   __syncthreads();
   const int64_t tile = sh_tile;
   const uint64_t total = t.cap + 2;
-  const uint64_t base = (uint64_t)tile * kExBlock * kExItems;
+  const uint64_t base = (uint64_t)tile * kExBlock * ITEMS;
   uint32_t keep = 0;
 #pragma unroll
-  for (int j = 0; j < kExItems; j++) {
+  for (int j = 0; j < ITEMS; j++) {
     uint64_t s = base + (uint64_t)j * kExBlock + threadIdx.x;
     bool k = false;
     if (s < t.cap) k = t.keys[s] != kEmpty;
@@ -270,7 +273,7 @@ __global__ __launch_bounds__(kExBlock) void gb_extract_kernel(GTable t, GPlan P,
   }
   __syncthreads();
   if (wave == 0) {
-    uint32_t c = sh_cnt[lane];
+    uint32_t c = lane < ITEMS * 4 ? sh_cnt[lane] : 0u;
     uint32_t incl = c;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -278,7 +281,7 @@ __global__ __launch_bounds__(kExBlock) void gb_extract_kernel(GTable t, GPlan P,
       if (lane >= d) incl += v;
     }
     uint32_t tile_total = __shfl(incl, 63);
-    sh_cnt[lane] = incl - c;
+    if (lane < ITEMS * 4) sh_cnt[lane] = incl - c;
     uint64_t excl = lookback_wave(status, tile, tile_total, &t.err[1]);
     if (lane == 0) {
       sh_prefix = excl;
@@ -288,7 +291,7 @@ __global__ __launch_bounds__(kExBlock) void gb_extract_kernel(GTable t, GPlan P,
   __syncthreads();
   const uint64_t prefix = sh_prefix;
   const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  for (int j = 0; j < kExItems; j++) {
+  for (int j = 0; j < ITEMS; j++) {
     bool k = (keep >> j) & 1u;
     uint64_t b = __ballot(k);
     if (!k) continue;
@@ -333,6 +336,44 @@ __global__ void flip_sign_kernel(const unsigned long long* __restrict__ in, uint
     out[i] = in[i] ^ 0x8000000000000000ull;
     idx[i] = (uint32_t)i;
   }
+}
+
+// Ascending order of a few signed 64-bit keys (ties by position): the rank
+// of a key is the number of keys that precede it.  A workgroup ranks 64 keys;
+// its four waves count over interleaved quarters of all keys (staged through
+// LDS, read as broadcasts) and add their counts.  One launch and no host
+// round trip, where the radix sort needs a histogram read-back; O(n^2) work,
+// so only for small n.
+constexpr int kRankChunk = 2048;
+constexpr int64_t kRankSortMax = 4096;
+__global__ __launch_bounds__(256) void rank_sort_kernel(const unsigned long long* __restrict__ keys, int64_t n,
+                                                        uint32_t* __restrict__ perm) {
+  __shared__ long long sk[kRankChunk];
+  __shared__ uint32_t part[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+  const long long ki = i < n ? (long long)keys[i] : 0;
+  uint32_t r0 = 0, r1 = 0;
+  for (int64_t base = 0; base < n; base += kRankChunk) {
+    const int m = (int)(n - base < kRankChunk ? n - base : kRankChunk);
+    __syncthreads();
+    for (int j = threadIdx.x; j < m; j += 256) sk[j] = (long long)keys[base + j];
+    __syncthreads();
+    const int64_t li = i - base;  // position of key i relative to this chunk
+    int j = wave;
+    for (; j + 4 < m; j += 8) {
+      const long long a = sk[j], b = sk[j + 4];
+      r0 += (a < ki || (a == ki && j < li)) ? 1u : 0u;
+      r1 += (b < ki || (b == ki && j + 4 < li)) ? 1u : 0u;
+    }
+    if (j < m) {
+      const long long a = sk[j];
+      r0 += (a < ki || (a == ki && j < li)) ? 1u : 0u;
+    }
+  }
+  part[wave][lane] = r0 + r1;
+  __syncthreads();
+  if (wave == 0 && i < n) perm[part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]] = (uint32_t)i;
 }
 
 }  // namespace bolt
@@ -594,7 +635,7 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
   const bool use_lds = !(flags & BOLT_GROUPBY_FORCE_GLOBAL) && S >= 256 && n > 0;
   int64_t n_parts = 0, n_spill = 0;
   double est = 0;
-  DevBuf<unsigned long long> pkeys, pwords, counters;
+  DevBuf<unsigned long long> pkeys, pwords;
   DevBuf<uint8_t> ptag;
   DevBuf<uint32_t> spill, hll, spill_seg;
   unsigned grid = 0;
@@ -608,7 +649,7 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
     JitLits glits;
     JitCols gcols;
     if (jit_enabled() && gb_jit_spec(ks, P, f, &gs, &glits, &gcols) == BOLT_OK &&
-        jit_get(jit_gb_lds_source(gs), &gfn) != BOLT_OK)
+        jit_get_keyed(gb_jit_key(gs), [&] { return jit_gb_lds_source(gs); }, &gfn) != BOLT_OK)
       gfn = nullptr;
     clear_error();
     const int64_t rows_per_iter = (int64_t)kGbBlock * (gfn ? gs.unroll : 1);
@@ -624,11 +665,11 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
     BOLT_TRY(pwords.alloc((size_t)pcap * (W > 0 ? W : 1), s));
     BOLT_TRY(spill.alloc((size_t)grid * spill_per, s));
     BOLT_TRY(spill_seg.alloc(grid, s));
-    BOLT_TRY(hll.alloc(kHllRegs, s));
-    BOLT_TRY(counters.alloc(2, s));
-    BOLT_TRY(fill_u32(hll.ptr, 0, kHllRegs, s));
-    BOLT_TRY(fill_u32(counters.ptr, 0, 4, s));
-    PartOut po{pkeys.ptr, ptag.ptr, pwords.ptr, pcap, counters.ptr, spill.ptr, counters.ptr + 1, spill_seg.ptr,
+    // sketch registers and the two counters in one buffer: one read-back
+    BOLT_TRY(hll.alloc(kHllRegs + 4, s));
+    BOLT_TRY(fill_u32(hll.ptr, 0, kHllRegs + 4, s));
+    unsigned long long* counters = reinterpret_cast<unsigned long long*>(hll.ptr + kHllRegs);
+    PartOut po{pkeys.ptr, ptag.ptr, pwords.ptr, pcap, counters, spill.ptr, counters + 1, spill_seg.ptr,
                spill_per, hll.ptr};
     prof_begin(s);
     if (gfn) {
@@ -641,11 +682,11 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
     }
     prof_end(s);
     BOLT_HIP(hipGetLastError());
-    unsigned long long hc[2];
-    std::vector<uint32_t> regs(kHllRegs);
-    BOLT_HIP(hipMemcpyAsync(hc, counters.ptr, 16, hipMemcpyDeviceToHost, s));
-    BOLT_HIP(hipMemcpyAsync(regs.data(), hll.ptr, kHllRegs * 4, hipMemcpyDeviceToHost, s));
+    std::vector<uint32_t> regs(kHllRegs + 4);
+    BOLT_HIP(hipMemcpyAsync(regs.data(), hll.ptr, (kHllRegs + 4) * 4, hipMemcpyDeviceToHost, s));
     BOLT_HIP(hipStreamSynchronize(s));
+    unsigned long long hc[2];
+    memcpy(hc, regs.data() + kHllRegs, 16);
     n_parts = (int64_t)hc[0];
     n_spill = (int64_t)hc[1];
     est = hll_estimate(regs.data());
@@ -654,14 +695,28 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
     est = (double)n;
   }
 
-  // ---- pass B ----
+  // ---- pass B (global table) and pass C (extraction) ----
+  // One host round trip for both: a table that overflowed is detected with
+  // the extraction's results and the two passes rerun 4x larger.
   const int64_t upper = n_parts + n_spill;  // exact bound on the group count
   uint64_t want = (uint64_t)std::min<double>((double)upper, est * 1.25 + 64.0);
   uint64_t cap = next_pow2(std::max<uint64_t>(1024, 2 * want));
+  if (const char* e = getenv("BOLT_GB_INITIAL_CAP")) {  // testing: force the overflow-and-rerun path
+    const long long c = atoll(e);
+    if (c >= 1024) cap = next_pow2((uint64_t)c);
+  }
+  bool has_var = false;
+  for (int a = 0; a < P.n; a++) has_var |= P.a[a].kind == K_VAR;
   GTable t;
   DevBuf<unsigned long long> tkeys, twords;
   DevBuf<uint32_t> tmisc;
   BOLT_TRY(tmisc.alloc(8, s));
+  bolt_table* tbl = nullptr;
+  std::vector<bolt_column_t> cols(n_keys + n_aggs);
+  DevBuf<unsigned long long> packed;
+  DevBuf<uint32_t> slot_of;
+  int64_t n_groups = 0;
+  int st = BOLT_OK;
   for (int attempt = 0;; attempt++) {
     BOLT_TRY(tkeys.alloc(cap, s));
     BOLT_TRY(twords.alloc((cap + 2) * (W > 0 ? W : 1), s));
@@ -671,9 +726,9 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
     t.used_special = tmisc.ptr;
     t.err = tmisc.ptr + 4;
     t.cap = cap;
-    gb_init_kernel<<<grid_for((int64_t)cap + 2, 256, 4), 256, 0, s>>>(t, P);
+    gb_init_kernel<<<grid_for((int64_t)cap + 2, 256, 1), 256, 0, s>>>(t, P);
     if (n_parts > 0) {
-      if (est * 2.0 < (double)S && n_parts > 4096) {
+      if (est < 0.75 * (double)S && n_parts > 4096) {
         // few groups: two-level merge through LDS tables of the same size
         const unsigned mgrid = (unsigned)std::min<int64_t>(64, (n_parts + 4095) / 4096);
         const size_t mlds = (size_t)(S + 2) * 8 * (1 + W) + 16;
@@ -696,94 +751,97 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
         if (attempt == 0) prof_end(s);
       }
     }
+    if (has_var && n > 0) gb_m2_kernel<<<grid_for(n, 256, 4), 256, 0, s>>>(t, P, ks, f, n);
     BOLT_HIP(hipGetLastError());
-    uint32_t e0 = 0;
-    BOLT_HIP(hipMemcpyAsync(&e0, t.err, 4, hipMemcpyDeviceToHost, s));
-    BOLT_HIP(hipStreamSynchronize(s));
-    if (e0 == 0) break;
-    if (attempt >= 3 || cap >= (1ull << 33))
-      return set_error(BOLT_ERR_MEMORY, "GROUP BY hash table overflow at %llu slots", (unsigned long long)cap);
-    cap *= 4;
-  }
-  bool has_var = false;
-  for (int a = 0; a < P.n; a++) has_var |= P.a[a].kind == K_VAR;
-  if (has_var && n > 0) {
-    gb_m2_kernel<<<grid_for(n, 256, 4), 256, 0, s>>>(t, P, ks, f, n);
-    BOLT_HIP(hipGetLastError());
-  }
 
-  // ---- pass C ----
-  const uint64_t total = cap + 2;
-  const int64_t out_cap = (int64_t)std::min<uint64_t>(total, (uint64_t)std::max<int64_t>(upper, 0) + 2);
-  auto* tbl = new bolt_table();
-  tbl->stream = s;
-  std::vector<bolt_column_t> cols(n_keys + n_aggs);
-  OutSpec o;
-  memset(&o, 0, sizeof o);
-  int st = BOLT_OK;
-  for (int k = 0; k < n_keys && st == BOLT_OK; k++) {
-    st = table_alloc_col(tbl, keys[k].dtype, out_cap, k == 0 && ks.null_group, s, &cols[k]);
-    o.key[k] = OutCol{static_cast<uint8_t*>(cols[k].data), (uint32_t*)cols[k].validity, keys[k].dtype, 0};
-  }
-  for (int a = 0; a < n_aggs && st == BOLT_OK; a++) {
-    const GAgg& g = P.a[a];
-    bool can_null = (g.nw >= 0 && g.kind != K_COUNT) ||
-                    g.op == BOLT_AGG_VAR_SAMP || g.op == BOLT_AGG_STDDEV_SAMP;
-    st = table_alloc_col(tbl, out_dt[a], out_cap, can_null, s, &cols[n_keys + a]);
-    o.agg[a] = OutCol{static_cast<uint8_t*>(cols[n_keys + a].data), (uint32_t*)cols[n_keys + a].validity, out_dt[a], 0};
-  }
-  DevBuf<unsigned long long> packed;
-  DevBuf<uint32_t> slot_of;
-  if (st == BOLT_OK) st = packed.alloc((size_t)out_cap, s);
-  if (st == BOLT_OK) st = slot_of.alloc((size_t)out_cap, s);
-  if (st != BOLT_OK) {
-    bolt_table_release(tbl);
-    return st;
-  }
-  o.packed = packed.ptr;
-  o.slot_of = slot_of.ptr;
-  const int64_t tiles = (int64_t)((total + kExBlock * kExItems - 1) / (kExBlock * kExItems));
-  DevBuf<uint64_t> status;
-  DevBuf<uint32_t> emisc;
-  if ((st = status.alloc((size_t)tiles, s)) != BOLT_OK || (st = emisc.alloc(4, s)) != BOLT_OK ||
-      (st = fill_u32(status.ptr, 0, (size_t)tiles * 2, s)) != BOLT_OK || (st = fill_u32(emisc.ptr, 0, 4, s)) != BOLT_OK) {
-    bolt_table_release(tbl);
-    return st;
-  }
-  gb_extract_kernel<<<(unsigned)tiles, kExBlock, 0, s>>>(t, P, ks, o, status.ptr, emisc.ptr,
-                                                         reinterpret_cast<unsigned long long*>(emisc.ptr + 2));
-  hipError_t le = hipGetLastError();
-  uint32_t eh[4] = {0, 0, 0, 0}, terr[4] = {0, 0, 0, 0};
-  if (le == hipSuccess) le = hipMemcpyAsync(eh, emisc.ptr, 16, hipMemcpyDeviceToHost, s);
-  if (le == hipSuccess) le = hipMemcpyAsync(terr, t.err, 16, hipMemcpyDeviceToHost, s);
-  if (le == hipSuccess) le = hipStreamSynchronize(s);
-  if (le != hipSuccess) {
-    bolt_table_release(tbl);
-    return set_error(BOLT_ERR_DEVICE, "GROUP BY extraction failed: %s", hipGetErrorString(le));
-  }
-  const int64_t n_groups = (int64_t)(((uint64_t)eh[3] << 32) | eh[2]);
-  if (terr[1]) {
-    bolt_table_release(tbl);
-    return set_error(BOLT_ERR_OTHER, "GROUP BY extraction look-back timed out");
-  }
-  if (terr[2] & 1u) {
-    bolt_table_release(tbl);
-    return set_error(BOLT_ERR_TYPE, "SUM(integer) overflow: accumulator exceeds i64 range");
-  }
-  if (terr[2] & 2u) {
-    bolt_table_release(tbl);
-    return set_error(BOLT_ERR_TYPE, "SUM(Decimal128) overflow: accumulator exceeds i128 range");
+    const uint64_t total = cap + 2;
+    const int64_t out_cap = (int64_t)std::min<uint64_t>(total, (uint64_t)std::max<int64_t>(upper, 0) + 2);
+    tbl = new bolt_table();
+    tbl->stream = s;
+    OutSpec o;
+    memset(&o, 0, sizeof o);
+    for (int k = 0; k < n_keys && st == BOLT_OK; k++) {
+      st = table_alloc_col(tbl, keys[k].dtype, out_cap, k == 0 && ks.null_group, s, &cols[k]);
+      o.key[k] = OutCol{static_cast<uint8_t*>(cols[k].data), (uint32_t*)cols[k].validity, keys[k].dtype, 0};
+    }
+    for (int a = 0; a < n_aggs && st == BOLT_OK; a++) {
+      const GAgg& g = P.a[a];
+      bool can_null = (g.nw >= 0 && g.kind != K_COUNT) ||
+                      g.op == BOLT_AGG_VAR_SAMP || g.op == BOLT_AGG_STDDEV_SAMP;
+      st = table_alloc_col(tbl, out_dt[a], out_cap, can_null, s, &cols[n_keys + a]);
+      o.agg[a] = OutCol{static_cast<uint8_t*>(cols[n_keys + a].data), (uint32_t*)cols[n_keys + a].validity, out_dt[a], 0};
+    }
+    if (st == BOLT_OK) st = packed.alloc((size_t)out_cap, s);
+    if (st == BOLT_OK) st = slot_of.alloc((size_t)out_cap, s);
+    if (st != BOLT_OK) {
+      bolt_table_release(tbl);
+      return st;
+    }
+    o.packed = packed.ptr;
+    o.slot_of = slot_of.ptr;
+    const bool small = total <= (1u << 20);
+    const int64_t per_tile = (int64_t)kExBlock * (small ? 1 : kExItems);
+    const int64_t tiles = (int64_t)((total + per_tile - 1) / per_tile);
+    // look-back status words and the ticket / total in one buffer, one fill
+    DevBuf<uint64_t> status;
+    if ((st = status.alloc((size_t)tiles + 2, s)) != BOLT_OK ||
+        (st = fill_u32(status.ptr, 0, ((size_t)tiles + 2) * 2, s)) != BOLT_OK) {
+      bolt_table_release(tbl);
+      return st;
+    }
+    uint32_t* emisc = reinterpret_cast<uint32_t*>(status.ptr + tiles);
+    if (small)
+      gb_extract_kernel<1><<<(unsigned)tiles, kExBlock, 0, s>>>(t, P, ks, o, status.ptr, emisc,
+                                                               reinterpret_cast<unsigned long long*>(emisc + 2));
+    else
+      gb_extract_kernel<kExItems><<<(unsigned)tiles, kExBlock, 0, s>>>(
+          t, P, ks, o, status.ptr, emisc, reinterpret_cast<unsigned long long*>(emisc + 2));
+    hipError_t le = hipGetLastError();
+    uint32_t eh[4] = {0, 0, 0, 0}, terr[4] = {0, 0, 0, 0};
+    if (le == hipSuccess) le = hipMemcpyAsync(eh, emisc, 16, hipMemcpyDeviceToHost, s);
+    if (le == hipSuccess) le = hipMemcpyAsync(terr, t.err, 16, hipMemcpyDeviceToHost, s);
+    if (le == hipSuccess) le = hipStreamSynchronize(s);
+    if (le != hipSuccess) {
+      bolt_table_release(tbl);
+      return set_error(BOLT_ERR_DEVICE, "GROUP BY extraction failed: %s", hipGetErrorString(le));
+    }
+    if (terr[0]) {  // the global table overflowed: discard this result and rerun larger
+      bolt_table_release(tbl);
+      tbl = nullptr;
+      if (attempt >= 3 || cap >= (1ull << 33))
+        return set_error(BOLT_ERR_MEMORY, "GROUP BY hash table overflow at %llu slots", (unsigned long long)cap);
+      cap *= 4;
+      continue;
+    }
+    n_groups = (int64_t)(((uint64_t)eh[3] << 32) | eh[2]);
+    if (terr[1]) {
+      bolt_table_release(tbl);
+      return set_error(BOLT_ERR_OTHER, "GROUP BY extraction look-back timed out");
+    }
+    if (terr[2] & 1u) {
+      bolt_table_release(tbl);
+      return set_error(BOLT_ERR_TYPE, "SUM(integer) overflow: accumulator exceeds i64 range");
+    }
+    if (terr[2] & 2u) {
+      bolt_table_release(tbl);
+      return set_error(BOLT_ERR_TYPE, "SUM(Decimal128) overflow: accumulator exceeds i128 range");
+    }
+    break;
   }
 
   // ---- ordering by packed key (src/exec/groupby.rs:800) ----
   if ((flags & BOLT_GROUPBY_SORTED) && n_groups > 1) {
     DevBuf<uint64_t> kin, kout;
     DevBuf<uint32_t> vin, perm;
-    st = kin.alloc((size_t)n_groups, s);
-    if (st == BOLT_OK) st = kout.alloc((size_t)n_groups, s);
-    if (st == BOLT_OK) st = vin.alloc((size_t)n_groups, s);
-    if (st == BOLT_OK) st = perm.alloc((size_t)n_groups, s);
-    if (st == BOLT_OK) {
+    st = perm.alloc((size_t)n_groups, s);
+    if (st == BOLT_OK && n_groups > kRankSortMax) {
+      st = kin.alloc((size_t)n_groups, s);
+      if (st == BOLT_OK) st = kout.alloc((size_t)n_groups, s);
+      if (st == BOLT_OK) st = vin.alloc((size_t)n_groups, s);
+    }
+    if (st == BOLT_OK && n_groups <= kRankSortMax) {
+      rank_sort_kernel<<<(unsigned)((n_groups + 63) / 64), 256, 0, s>>>(packed.ptr, n_groups, perm.ptr);
+    } else if (st == BOLT_OK) {
       flip_sign_kernel<<<grid_for(n_groups, 256, 4), 256, 0, s>>>(packed.ptr, kin.ptr, vin.ptr, n_groups);
       st = radix_sort_u64_pairs(s, kin.ptr, vin.ptr, n_groups, kout.ptr, perm.ptr, 64);
     }

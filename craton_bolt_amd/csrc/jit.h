@@ -2,6 +2,7 @@
 // with hipRTC (see jit.hip).
 #pragma once
 
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -65,6 +66,8 @@ int jit_get(const std::string& src, hipFunction_t* fn);
 int jit_spec_from(const DevFilter& f, const AggList& aggs, bool probe, int key_dtype, bool key_valid, int carry_dtype,
                   JitSpec* sp, JitLits* lits, JitCols* cols);
 std::string jit_gb_lds_source(const GbJitSpec& sp);
+std::string gb_jit_key(const GbJitSpec& sp);
+int jit_get_keyed(const std::string& key, const std::function<std::string()>& gen, hipFunction_t* fn);
 int jit_row_agg_col(const void* d, const uint8_t* v, int dt, std::vector<JitSpec::Col>* cols, JitCols* jc);
 int jit_launch_row_agg(hipFunction_t fn, unsigned grid, hipStream_t s, const uint4* tbl, uint64_t mask,
                        const uint32_t* flags, int64_t n, const void* key, const uint8_t* kvalid, const JitLits& lits,

@@ -22,6 +22,7 @@
 
 #include <cstdlib>
 #include <fstream>
+#include <functional>
 #include <mutex>
 #include <sstream>
 #include <unordered_map>
@@ -163,6 +164,40 @@ int jit_get(const std::string& src, hipFunction_t* fn) {
   g_cache.emplace(src, e);
   *fn = e.fn;
   return BOLT_OK;
+}
+
+// Kernels looked up by a compact description of their spec, so that a query
+// repeated with the same shape neither regenerates nor hashes the source.
+int jit_get_keyed(const std::string& key, const std::function<std::string()>& gen, hipFunction_t* fn) {
+  static std::mutex mu;
+  static std::unordered_map<std::string, hipFunction_t> by_key;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = by_key.find(key);
+    if (it != by_key.end()) {
+      *fn = it->second;
+      return BOLT_OK;
+    }
+  }
+  BOLT_TRY(jit_get(gen(), fn));
+  std::lock_guard<std::mutex> lk(mu);
+  by_key.emplace(key, *fn);
+  return BOLT_OK;
+}
+
+std::string gb_jit_key(const GbJitSpec& sp) {
+  std::ostringstream k;
+  k << "gb:" << sp.unroll << ':' << sp.n_keys << ':' << sp.t0 << ':' << sp.t1 << ':' << sp.kv0 << sp.kv1 << ':'
+    << sp.words << '|';
+  for (unsigned long long v : sp.ident) k << v << ',';
+  k << '|';
+  for (const JitSpec::Col& c : sp.cols) k << c.dtype << (c.has_valid ? 'v' : 'n') << ',';
+  k << '|';
+  for (const JitSpec::Term& t : sp.terms) k << t.col << '.' << t.op << ',';
+  k << '|';
+  for (const GbJitSpec::Agg& g : sp.aggs)
+    k << g.op << '.' << g.kind << '.' << g.dtype << '.' << g.col << '.' << g.w << '.' << g.nw << ',';
+  return k.str();
 }
 
 // ---- source generation ------------------------------------------------------------

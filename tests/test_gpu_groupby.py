@@ -147,6 +147,18 @@ def test_ragged_sizes_with_spills(gpu, n):
     run_both([(k, N.I64, kv)], [(v, N.I64, None)], [(N.SUM, 0), (N.COUNT_STAR, 0), (N.MIN, 0)], [N.I64])
 
 
+def test_global_table_overflow_reruns_larger(gpu, monkeypatch):
+    """A global table sized far below the group count overflows; the
+    operator detects it with the extraction's read-back and reruns passes B
+    and C with a 4x larger table (three times here: 1024 -> 65536 slots)."""
+    monkeypatch.setenv("BOLT_GB_INITIAL_CAP", "1024")
+    n = 500_000
+    rng = np.random.default_rng(21)
+    k = rng.integers(0, 20_000, n).astype(np.int64) * 31 + 7
+    v = rng.integers(-100, 100, n).astype(np.int64)
+    run_both([(k, N.I64, None)], [(v, N.I64, None)], [(N.SUM, 0), (N.COUNT_STAR, 0), (N.MAX, 0)], [N.I64])
+
+
 @pytest.mark.parametrize("kdt", [N.I32, N.I64, N.F32, N.F64, N.DATE32])
 def test_key_dtypes_nulls_and_all_aggregates(gpu, kdt):
     n = 400_000
