@@ -1,0 +1,373 @@
+// gb_part.hip — the radix-partitioned HBM tier of hash GROUP BY.
+//
+// Replaces the reference's Tier-2 partitioned path (execute_tier2_sum,
+// src/exec/groupby_tier2_orchestrator.rs:214: partition kernels
+// src/jit/partition_kernel.rs:239,374, offsets src/exec/partition_offsets.rs:364,
+// scatter src/jit/scatter_kernel.rs:99, per-partition LDS reduce
+// src/jit/partition_reduce_kernel.rs:139-241), which is capped at
+// 4096 x 1024 groups and one aggregate family per launch.  Here it takes
+// what the LDS tier could not finish — its partial records and its spilled
+// rows — so every aggregate of the query goes through one set of passes and
+// there is no group cap (a partition whose groups overflow its LDS table
+// sends the rest to the global table directly):
+//
+//   compact   the workgroups' spill segments into one dense row list;
+//   histogram every item (partial record or spilled row) gets a partition
+//             id from its key hash; per-workgroup counts in LDS, totals by
+//             one atomic per (workgroup, partition);
+//   scan      partition starts (one workgroup);
+//   scatter   each workgroup reserves its share of every partition with one
+//             atomic each, then writes its items as merge records
+//             (key, tag, accumulator words; a spilled row becomes the words
+//             of a one-row group) ranked by LDS atomics;
+//   reduce    one workgroup per partition folds its records into an LDS
+//             table and writes each finished group into the global table:
+//             the slot is claimed with one CAS, and since no other
+//             workgroup holds the key, its words are plain stores.
+//
+// The global table that results is the one the non-partitioned path builds,
+// so extraction, the VAR / STDDEV second pass and ordering are shared.
+#include <algorithm>
+
+#include "groupby_dev.h"
+
+namespace bolt {
+
+constexpr int kPartBlock = 1024;
+constexpr int kPartMaxBits = 12;  // up to 4096 partitions (48 KiB of LDS for the scatter)
+
+__device__ __forceinline__ uint32_t part_of(uint64_t key, int pbits) {
+  return pbits == 0 ? 0u : (uint32_t)(mix64(key) >> (64 - pbits));
+}
+
+// Accumulator words of a group holding the single row `row` (the update of
+// update_slot applied to the identity).
+__device__ __forceinline__ void row_words(const GPlan& P, int64_t row, unsigned long long* dst, size_t stride) {
+  for (int a = 0; a < P.n; a++) {
+    const GAgg& g = P.a[a];
+    unsigned long long* w = dst + (size_t)g.w * stride;
+    if (g.op == BOLT_AGG_COUNT_STAR) { w[0] = 1; continue; }
+    const bool valid = bit_valid(g.validity, row);
+    if (g.kind == K_COUNT) { w[0] = valid ? 1 : 0; continue; }
+    if (g.nw >= 0) dst[(size_t)g.nw * stride] = valid ? 1 : 0;
+    switch (g.kind) {
+      case K_ISUM: {
+        int64_t v = 0;
+        if (valid) v = g.dtype == BOLT_I64 ? static_cast<const int64_t*>(g.data)[row]
+                                           : (int64_t) static_cast<const int32_t*>(g.data)[row];
+        w[0] = (uint64_t)v;
+        w[stride] = (uint64_t)(v >> 63);
+        break;
+      }
+      case K_DSUM: {
+        uint64_t lo = 0, hi = 0;
+        if (valid) {
+          const uint64_t* p = static_cast<const uint64_t*>(g.data) + 2 * row;
+          lo = p[0];
+          hi = p[1];
+        }
+        w[0] = lo;
+        w[stride] = hi;
+        w[2 * stride] = (uint64_t)((int64_t)hi >> 63);
+        break;
+      }
+      case K_FSUM:
+      case K_AVG:
+      case K_VAR: {
+        double v = 0.0;
+        if (valid)
+          v = g.dtype == BOLT_F64 ? static_cast<const double*>(g.data)[row]
+            : g.dtype == BOLT_F32 ? (double)static_cast<const float*>(g.data)[row]
+            : g.dtype == BOLT_I64 ? (double)static_cast<const int64_t*>(g.data)[row]
+                                  : (double)static_cast<const int32_t*>(g.data)[row];
+        w[0] = as_u64(v);
+        break;
+      }
+      case K_MINMAX: {
+        uint64_t k = g.op == BOLT_AGG_MIN ? ~0ull : 0ull;
+        if (valid) {
+          switch (g.dtype) {
+            case BOLT_F64: k = f64_order_key(static_cast<const double*>(g.data)[row]); break;
+            case BOLT_F32: k = f64_order_key((double)static_cast<const float*>(g.data)[row]); break;
+            case BOLT_I64: k = (uint64_t) static_cast<const int64_t*>(g.data)[row] ^ 0x8000000000000000ull; break;
+            default: k = (uint64_t)(int64_t) static_cast<const int32_t*>(g.data)[row] ^ 0x8000000000000000ull; break;
+          }
+        }
+        w[0] = k;
+        break;
+      }
+    }
+  }
+}
+
+// The items: partial records [0, n_parts), then the spilled rows (PartSrc).
+__device__ __forceinline__ void item_key(const PartSrc& src, const KeySpec& ks, int64_t i, uint64_t* key,
+                                         uint8_t* tag) {
+  if (i < src.n_parts) {
+    *key = src.pkeys[i];
+    *tag = src.ptag[i];
+    return;
+  }
+  uint64_t k = 0;
+  const int kr = row_key(ks, (int64_t)src.rows[i - src.n_parts], &k);
+  // spilled rows carry regular keys; the NULL group is tagged for safety
+  *tag = kr == 2 ? 1 : (k == kEmpty ? 2 : 0);
+  *key = k;
+}
+
+// Spill segments [b * per, b * per + seg[b]) -> dense[prefix(b) ..]; K
+// workgroups share a segment.
+__global__ __launch_bounds__(256) void part_compact_spill_kernel(const uint32_t* __restrict__ spill, int64_t per,
+                                                                 const uint32_t* __restrict__ seg, int K,
+                                                                 uint32_t* __restrict__ dense) {
+  __shared__ uint64_t sh_off;
+  const int b = blockIdx.x / K, k = blockIdx.x % K;
+  if (threadIdx.x == 0) {
+    uint64_t o = 0;
+    for (int j = 0; j < b; j++) o += seg[j];
+    sh_off = o;
+  }
+  __syncthreads();
+  const uint64_t off = sh_off;
+  const uint32_t cnt = seg[b];
+  for (uint32_t i = (uint32_t)k * blockDim.x + threadIdx.x; i < cnt; i += (uint32_t)K * blockDim.x)
+    dense[off + i] = spill[(int64_t)b * per + i];
+}
+
+__global__ __launch_bounds__(kPartBlock) void part_hist_kernel(PartSrc src, KeySpec ks, int pbits, int64_t chunk,
+                                                               uint16_t* __restrict__ pid,
+                                                               uint32_t* __restrict__ hist,
+                                                               uint32_t* __restrict__ total) {
+  __shared__ uint32_t lh[1 << kPartMaxBits];
+  const int np = 1 << pbits;
+  for (int p = threadIdx.x; p < np; p += kPartBlock) lh[p] = 0;
+  __syncthreads();
+  const int64_t i0 = (int64_t)blockIdx.x * chunk;
+  const int64_t i1 = std::min<int64_t>(i0 + chunk, src.n_items);
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kPartBlock) {
+    uint64_t key;
+    uint8_t tag;
+    item_key(src, ks, i, &key, &tag);
+    if (tag) {  // the NULL group / empty-marker key: merged straight into the global table by the scatter
+      pid[i] = 0xFFFFu;
+      continue;
+    }
+    const uint32_t p = part_of(key, pbits);
+    pid[i] = (uint16_t)p;
+    atomicAdd(&lh[p], 1u);
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < np; p += kPartBlock) {
+    const uint32_t c = lh[p];
+    hist[(int64_t)blockIdx.x * np + p] = c;
+    if (c) atomicAdd(&total[p], c);
+  }
+}
+
+// Exclusive scan of the partition totals (one workgroup): start[p], and the
+// scatter cursors initialised to the starts.  start[np] = number of items.
+__global__ __launch_bounds__(kPartBlock) void part_scan_kernel(const uint32_t* __restrict__ total, int np,
+                                                               uint64_t* __restrict__ start,
+                                                               unsigned long long* __restrict__ cursor) {
+  __shared__ uint64_t wsum[kPartBlock / 64];
+  __shared__ uint64_t carry;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < np; base += kPartBlock) {
+    const int p = base + threadIdx.x;
+    const uint64_t v = p < np ? total[p] : 0;
+    uint64_t inc = v;
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint64_t u = __shfl_up(inc, d);
+      if (lane >= d) inc += u;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    uint64_t before = carry;
+    for (int w = 0; w < wave; w++) before += wsum[w];
+    if (p < np) {
+      start[p] = before + inc - v;
+      cursor[p] = before + inc - v;
+    }
+    __syncthreads();
+    if (threadIdx.x == kPartBlock - 1) carry = before + inc;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) start[np] = carry;
+}
+
+// Merge records, one per item: the key then the W accumulator words, `stride`
+// 64-bit words apart (a multiple of two, so a record is whole 16-byte units).
+struct PartRecs {
+  unsigned long long* rec;
+  int64_t cap;
+  int stride;
+};
+
+__global__ __launch_bounds__(kPartBlock) void part_scatter_kernel(PartSrc src, KeySpec ks, GPlan P, int pbits,
+                                                                  int64_t chunk, const uint16_t* __restrict__ pid,
+                                                                  const uint32_t* __restrict__ hist,
+                                                                  unsigned long long* __restrict__ cursor,
+                                                                  PartRecs out, GTable t) {
+  __shared__ unsigned long long lbase[1 << kPartMaxBits];  // reserved start in each partition
+  __shared__ uint32_t lpos[1 << kPartMaxBits];             // items placed so far
+  const int np = 1 << pbits;
+  // reserve this workgroup's share of each partition it has items in (one
+  // atomic each); an item's position is its reservation + an LDS rank
+  for (int p = threadIdx.x; p < np; p += kPartBlock) {
+    const uint32_t c = hist[(int64_t)blockIdx.x * np + p];
+    lbase[p] = c ? atomicAdd(&cursor[p], (unsigned long long)c) : 0ull;
+    lpos[p] = 0;
+  }
+  __syncthreads();
+  const int64_t i0 = (int64_t)blockIdx.x * chunk;
+  const int64_t i1 = std::min<int64_t>(i0 + chunk, src.n_items);
+  const int W = P.words;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kPartBlock) {
+    const uint32_t p = pid[i];
+    if (p == 0xFFFFu) {  // tagged partial record (never a spilled row)
+      const int64_t g = gslot(t, src.ptag[i], src.pkeys[i]);
+      if (g >= 0) merge_slot(P, t.words, t.cap + 2, (uint32_t)g, src.pwords + i, (size_t)src.pcap);
+      continue;
+    }
+    const uint64_t pos = lbase[p] + atomicAdd(&lpos[p], 1u);
+    unsigned long long* r = out.rec + pos * (uint64_t)out.stride;
+    if (i < src.n_parts) {
+      r[0] = src.pkeys[i];
+      for (int w = 0; w < W; w++) r[1 + w] = src.pwords[(size_t)w * src.pcap + i];
+    } else {
+      const int64_t row = (int64_t)src.rows[i - src.n_parts];
+      uint64_t key = 0;
+      (void)row_key(ks, row, &key);
+      r[0] = key;
+      row_words(P, row, r + 1, 1);
+    }
+  }
+}
+
+// Global slot of `key`, claiming an empty one; *fresh says whether this call
+// claimed it.  0xFFFFFFFF when the probe bound is exceeded.
+__device__ __forceinline__ uint32_t global_claim(unsigned long long* keys, uint64_t cap_mask, uint64_t key,
+                                                 bool* fresh) {
+  uint64_t s = mix64(key) & cap_mask;
+  for (uint64_t p = 0; p < (uint64_t)kGlobalProbes && p <= cap_mask; p++) {
+    const unsigned long long cur = keys[s];
+    if (cur == key) { *fresh = false; return (uint32_t)s; }
+    if (cur == kEmpty) {
+      const unsigned long long prev = atomicCAS(&keys[s], kEmpty, (unsigned long long)key);
+      if (prev == kEmpty) { *fresh = true; return (uint32_t)s; }
+      if (prev == key) { *fresh = false; return (uint32_t)s; }
+    }
+    s = (s + 1) & cap_mask;
+  }
+  return 0xFFFFFFFFu;
+}
+
+__global__ __launch_bounds__(kGbBlock) void part_reduce_kernel(GTable t, GPlan P, PartRecs recs,
+                                                               const uint64_t* __restrict__ start, int S) {
+  extern __shared__ unsigned long long sm[];
+  const int SS = S + 2;
+  unsigned long long* skeys = sm;
+  unsigned long long* swords = sm + SS;
+  uint32_t* sused = reinterpret_cast<uint32_t*>(swords + (size_t)P.words * SS);
+  for (int i = threadIdx.x; i < SS; i += kGbBlock) skeys[i] = kEmpty;
+  for (int w = 0; w < P.words; w++) {
+    const unsigned long long id = word_identity(P, w);
+    for (int i = threadIdx.x; i < SS; i += kGbBlock) swords[(size_t)w * SS + i] = id;
+  }
+  if (threadIdx.x < 2) sused[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t r0 = (int64_t)start[blockIdx.x], r1 = (int64_t)start[blockIdx.x + 1];
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += kGbBlock) {
+    const unsigned long long* rec = recs.rec + r * (uint64_t)recs.stride;
+    const unsigned long long key = rec[0];
+    const int slot = gb_lds_slot(skeys, S, key);
+    if (slot >= 0) {
+      merge_slot(P, swords, SS, (uint32_t)slot, rec + 1, 1);
+    } else {  // the partition holds more groups than the LDS table: straight to HBM
+      const int64_t g = gslot(t, 0, key);
+      if (g >= 0) merge_slot(P, t.words, t.cap + 2, (uint32_t)g, rec + 1, 1);
+    }
+  }
+  __syncthreads();
+  const size_t gstride = t.cap + 2;
+  for (int i = threadIdx.x; i < SS; i += kGbBlock) {
+    if (i < S) {
+      const unsigned long long key = skeys[i];
+      if (key == kEmpty) continue;
+      bool fresh = false;
+      const uint32_t g = global_claim(t.keys, t.cap - 1, key, &fresh);
+      if (g == 0xFFFFFFFFu) { atomicOr(&t.err[0], 1u); continue; }
+      if (fresh) {  // no other workgroup holds this key: plain stores
+        for (int w = 0; w < P.words; w++) t.words[(size_t)w * gstride + g] = swords[(size_t)w * SS + i];
+      } else {      // records of this partition that overflowed the LDS table got there first
+        merge_slot(P, t.words, gstride, g, swords + i, (size_t)SS);
+      }
+    }
+  }
+}
+
+__global__ void part_reset_cursor_kernel(const uint64_t* __restrict__ start, unsigned long long* __restrict__ cursor,
+                                         int np) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < np) cursor[p] = start[p];
+}
+
+// ---- host ---------------------------------------------------------------------------
+
+int gb_part_build(hipStream_t s, const KeySpec& ks, const GPlan& P, const unsigned long long* pkeys,
+                  const uint8_t* ptag, const unsigned long long* pwords, int64_t pcap, int64_t n_parts,
+                  const uint32_t* spill, int64_t spill_per, const uint32_t* spill_seg, int nseg, int64_t n_spill,
+                  double est, int S, GbPartitions* out) {
+  const int64_t n_items = n_parts + n_spill;
+  // partitions of about S/2 groups each, so one LDS table holds a partition
+  const double want = std::max(1.0, est / std::max(1.0, 0.5 * S));
+  int pbits = 0;
+  while ((1 << pbits) < want && pbits < kPartMaxBits) pbits++;
+  const int np = 1 << pbits;
+  out->np = np;
+  BOLT_TRY(out->dense.alloc((size_t)std::max<int64_t>(n_spill, 1), s));
+  const int K = 16;  // workgroups per spill segment
+  if (n_spill > 0) part_compact_spill_kernel<<<nseg * K, 256, 0, s>>>(spill, spill_per, spill_seg, K, out->dense.ptr);
+  PartSrc src{pkeys, ptag, pwords, pcap, n_parts, out->dense.ptr, n_items};
+  const int nb = (int)std::min<int64_t>(512, std::max<int64_t>(1, (n_items + 65535) / 65536));
+  const int64_t chunk = (n_items + nb - 1) / nb;
+  DevBuf<uint32_t> total;
+  BOLT_TRY(out->pid.alloc((size_t)std::max<int64_t>(n_items, 1), s));
+  BOLT_TRY(out->hist.alloc((size_t)nb * np, s));
+  BOLT_TRY(total.alloc((size_t)np, s));
+  BOLT_TRY(out->cursor.alloc((size_t)np, s));
+  BOLT_TRY(out->start.alloc((size_t)np + 1, s));
+  BOLT_TRY(fill_u32(total.ptr, 0, (size_t)np, s));
+  part_hist_kernel<<<nb, kPartBlock, 0, s>>>(src, ks, pbits, chunk, out->pid.ptr, out->hist.ptr, total.ptr);
+  part_scan_kernel<<<1, kPartBlock, 0, s>>>(total.ptr, np, out->start.ptr, out->cursor.ptr);
+  out->stride = (1 + P.words + 1) & ~1;  // key + words, rounded up to 16 bytes
+  out->recs_cap = std::max<int64_t>(n_items, 1);
+  BOLT_TRY(out->rec.alloc((size_t)out->recs_cap * out->stride, s));
+  out->src = src;
+  out->nb = nb;
+  out->chunk = chunk;
+  out->pbits = pbits;
+  out->ks = ks;
+  BOLT_HIP(hipGetLastError());
+  return BOLT_OK;
+}
+
+// The scatter needs the global table (tagged records go there directly), so
+// it runs per table attempt, with the reduce.
+int gb_part_reduce(hipStream_t s, const GTable& t, const GPlan& P, GbPartitions& parts, int S) {
+  PartRecs recs{parts.rec.ptr, parts.recs_cap, parts.stride};
+  // cursors back to the partition starts, then scatter
+  part_reset_cursor_kernel<<<(parts.np + 255) / 256, 256, 0, s>>>(parts.start.ptr, parts.cursor.ptr, parts.np);
+  part_scatter_kernel<<<parts.nb, kPartBlock, 0, s>>>(parts.src, parts.ks, P, parts.pbits, parts.chunk,
+                                                      parts.pid.ptr, parts.hist.ptr, parts.cursor.ptr, recs, t);
+  const size_t lds = (size_t)(S + 2) * 8 * (1 + P.words) + 16;
+  BOLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(part_reduce_kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  part_reduce_kernel<<<(unsigned)parts.np, kGbBlock, lds, s>>>(t, P, recs, parts.start.ptr, S);
+  BOLT_HIP(hipGetLastError());
+  return BOLT_OK;
+}
+
+}  // namespace bolt

@@ -707,6 +707,16 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
   }
   bool has_var = false;
   for (int a = 0; a < P.n; a++) has_var |= P.a[a].kind == K_VAR;
+  // many spilled rows over many groups: the radix-partitioned tier instead of
+  // global atomics per spilled row (its records are built once, whatever the
+  // table size)
+  const bool partitioned = use_lds && n > 0 &&
+                           ((flags & BOLT_GROUPBY_FORCE_PARTITIONED) ||
+                            (n_spill > (int64_t)(1 << 20) && est > 0.75 * (double)S));
+  GbPartitions parts;
+  if (partitioned)
+    BOLT_TRY(gb_part_build(s, ks, P, pkeys.ptr, ptag.ptr, pwords.ptr, (int64_t)pkeys.count, n_parts, spill.ptr,
+                           spill_per, spill_seg.ptr, (int)grid, n_spill, est, S, &parts));
   GTable t;
   DevBuf<unsigned long long> tkeys, twords;
   DevBuf<uint32_t> tmisc;
@@ -727,7 +737,9 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
     t.err = tmisc.ptr + 4;
     t.cap = cap;
     gb_init_kernel<<<grid_for((int64_t)cap + 2, 256, 1), 256, 0, s>>>(t, P);
-    if (n_parts > 0) {
+    if (partitioned) {
+      BOLT_TRY(gb_part_reduce(s, t, P, parts, S));
+    } else if (n_parts > 0) {
       if (est < 0.75 * (double)S && n_parts > 4096) {
         // few groups: two-level merge through LDS tables of the same size
         const unsigned mgrid = (unsigned)std::min<int64_t>(64, (n_parts + 4095) / 4096);
@@ -741,7 +753,7 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
                                                                         (int64_t)pkeys.count, n_parts);
       }
     }
-    if (n_spill > 0) {
+    if (n_spill > 0 && !partitioned) {
       if (use_lds) {
         const int K = 16;  // blocks per spill segment
         gb_rows_seg_kernel<<<grid * K, 256, 0, s>>>(t, P, ks, spill.ptr, spill_per, spill_seg.ptr, K);

@@ -281,6 +281,37 @@ __device__ __forceinline__ void finalize_to(const GTable& t, const GAgg& g, cons
 int make_gplan(const bolt_agg_t* aggs, int32_t n_aggs, const bolt_column_t* inputs, int32_t n_inputs, int64_t n,
                GPlan* P, int32_t* out_dtype);
 double hll_estimate(const uint32_t* regs);
+
+// The partitioned HBM tier's state (gb_part.hip): the items' partition
+// ids and per-workgroup counts, the partition starts ([np + 1]) and the
+// record buffer the scatter fills (records of partition p at
+// [start[p], start[p + 1])).
+struct PartSrc {
+  const unsigned long long* pkeys;
+  const uint8_t* ptag;
+  const unsigned long long* pwords;
+  int64_t pcap;
+  int64_t n_parts;
+  const uint32_t* rows;  // dense spilled row ids
+  int64_t n_items;
+};
+struct GbPartitions {
+  int np = 0, pbits = 0, nb = 0, stride = 0;
+  int64_t chunk = 0, recs_cap = 0;
+  PartSrc src{};
+  KeySpec ks{};
+  DevBuf<uint32_t> dense;
+  DevBuf<uint16_t> pid;
+  DevBuf<uint32_t> hist;
+  DevBuf<unsigned long long> cursor;
+  DevBuf<uint64_t> start;
+  DevBuf<unsigned long long> rec;
+};
+int gb_part_build(hipStream_t s, const KeySpec& ks, const GPlan& P, const unsigned long long* pkeys,
+                  const uint8_t* ptag, const unsigned long long* pwords, int64_t pcap, int64_t n_parts,
+                  const uint32_t* spill, int64_t spill_per, const uint32_t* spill_seg, int nseg, int64_t n_spill,
+                  double est, int S, GbPartitions* out);
+int gb_part_reduce(hipStream_t s, const GTable& t, const GPlan& P, GbPartitions& parts, int S);
 uint64_t next_pow2(uint64_t x);
 
 }  // namespace bolt

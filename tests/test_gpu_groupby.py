@@ -122,7 +122,7 @@ def test_plain_groupby_workload_shape(gpu):
 
 
 @pytest.mark.parametrize("card", [1, 16, 1000, 100_000, 2_000_000])
-@pytest.mark.parametrize("flags", [0, N.GROUPBY_FORCE_GLOBAL])
+@pytest.mark.parametrize("flags", [0, N.GROUPBY_FORCE_GLOBAL, N.GROUPBY_FORCE_PARTITIONED])
 def test_cardinality_sweep(gpu, card, flags):
     print(f"[case] groupby card={card} flags={flags}", flush=True)
     n = 3_000_000
@@ -136,7 +136,8 @@ def test_cardinality_sweep(gpu, card, flags):
 
 
 @pytest.mark.parametrize("n", [1, 1023, 8191, 8193, 300_001])
-def test_ragged_sizes_with_spills(gpu, n):
+@pytest.mark.parametrize("flags", [0, N.GROUPBY_FORCE_PARTITIONED])
+def test_ragged_sizes_with_spills(gpu, n, flags):
     """Row counts around the 1024-row and 8192-row workgroup iterations, with
     about as many groups as rows so most rows spill out of the LDS tables."""
     print(f"[case] groupby ragged n={n}", flush=True)
@@ -144,7 +145,8 @@ def test_ragged_sizes_with_spills(gpu, n):
     k = rng.integers(0, max(n, 2), n).astype(np.int64)
     v = rng.integers(-1000, 1000, n).astype(np.int64)
     kv = rng.random(n) > 0.05
-    run_both([(k, N.I64, kv)], [(v, N.I64, None)], [(N.SUM, 0), (N.COUNT_STAR, 0), (N.MIN, 0)], [N.I64])
+    run_both([(k, N.I64, kv)], [(v, N.I64, None)], [(N.SUM, 0), (N.COUNT_STAR, 0), (N.MIN, 0)], [N.I64],
+             flags=flags)
 
 
 def test_global_table_overflow_reruns_larger(gpu, monkeypatch):
@@ -160,7 +162,8 @@ def test_global_table_overflow_reruns_larger(gpu, monkeypatch):
 
 
 @pytest.mark.parametrize("kdt", [N.I32, N.I64, N.F32, N.F64, N.DATE32])
-def test_key_dtypes_nulls_and_all_aggregates(gpu, kdt):
+@pytest.mark.parametrize("flags", [0, N.GROUPBY_FORCE_PARTITIONED])
+def test_key_dtypes_nulls_and_all_aggregates(gpu, kdt, flags):
     n = 400_000
     rng = np.random.default_rng(kdt)
     base = rng.integers(-300, 300, n)
@@ -178,7 +181,7 @@ def test_key_dtypes_nulls_and_all_aggregates(gpu, kdt):
     z = rng.normal(size=n).astype(np.float32)
     aggs = [(N.SUM, 0), (N.COUNT, 0), (N.COUNT_STAR, 0), (N.MIN, 0), (N.MAX, 0), (N.AVG, 1), (N.MIN, 1), (N.MAX, 1),
             (N.VAR_SAMP, 1), (N.STDDEV_POP, 1), (N.SUM, 2), (N.MIN, 2)]
-    run_both([(k, kdt, kv)], [(x, N.I32, xv), (y, N.F64, yv), (z, N.F32, None)], aggs, [kdt])
+    run_both([(k, kdt, kv)], [(x, N.I32, xv), (y, N.F64, yv), (z, N.F32, None)], aggs, [kdt], flags=flags)
 
 
 def test_filtered_groupby(gpu):
@@ -229,13 +232,16 @@ def test_integer_sum_overflow_is_an_error(gpu):
 
 
 @pytest.mark.parametrize("n", [50_000, 1_500_000])
-def test_decimal_sum_by_group(gpu, n):
+@pytest.mark.parametrize("flags", [0, N.GROUPBY_FORCE_PARTITIONED])
+def test_decimal_sum_by_group(gpu, n, flags):
     # 1.5M rows: enough LDS-tier workgroups that the partial records take the
     # two-level (LDS, then global) merge
+    # (partitioned: 20K groups, so spilled Decimal rows become merge records)
     rng = np.random.default_rng(11)
-    k = rng.integers(0, 100, n).astype(np.int32)
+    k = rng.integers(0, 20_000 if flags else 100, n).astype(np.int32)
     vals = [int(x) * 10 ** 18 + int(y) for x, y in zip(rng.integers(-10 ** 12, 10 ** 12, n), rng.integers(0, 10 ** 18, n))]
-    run_both([(k, N.I32, None)], [(dec_array(vals), N.DEC128, None)], [(N.SUM, 0), (N.COUNT_STAR, 0)], [N.I32])
+    run_both([(k, N.I32, None)], [(dec_array(vals), N.DEC128, None)], [(N.SUM, 0), (N.COUNT_STAR, 0)], [N.I32],
+             flags=flags)
 
 
 def test_zipf_high_cardinality_float_sum(gpu):
