@@ -418,7 +418,7 @@ def bench_batch(args) -> dict:
     return line
 
 
-def serve_mix(srv, n: int, seed: int):
+def serve_specs(n: int, seed: int) -> list[dict]:
     """The served query mix (filter -> 1..3 joins -> GROUP BY 10K..100K groups ->
     ORDER BY + LIMIT 100) with seeded parameters; fact columns are
     fk1, fk2, fk3, price, qty, date."""
@@ -443,8 +443,26 @@ def serve_mix(srv, n: int, seed: int):
             spec = dict(terms=[Term(5, N.GE, d0), Term(5, N.LT, d0 + 120)], joins=[(0, 0), (1, 1), (2, 2)],
                         group=[(2, 0), (1, 1)], aggs=[(N.COUNT_STAR, 0), (N.SUM, 4), (N.SUM, 3)], order_agg=2,
                         limit=100, est_groups=25_000)
-        out.append(srv.query(**spec))
+        out.append(spec)
     return out
+
+
+def serve_mix(srv, n: int, seed: int):
+    return [srv.query(**spec) for spec in serve_specs(n, seed)]
+
+
+def _agree(a, b, tol=1e-9):
+    """Same groups and rows; floats within `tol` relative (atomic order)."""
+    if a["status"] != b["status"] or a["n_groups"] != b["n_groups"] or len(a["rows"]) != len(b["rows"]):
+        return False
+    for ra, rb in zip(a["rows"], b["rows"]):
+        for x, y in zip(ra, rb):
+            if isinstance(y, float):
+                if abs(x - y) > tol * max(abs(y), 1.0):
+                    return False
+            elif x != y:
+                return False
+    return True
 
 
 def bench_serve(args) -> dict:
@@ -477,13 +495,29 @@ def bench_serve(args) -> dict:
         res = [srv.wait(tk) for tk in tickets]
         return time.perf_counter() - t, res
 
-    plain = Server(fact, dims, slots=1, chunk_rows=n)
-    closed_loop(plain, serve_mix(plain, 3, seed=1))  # warm-up (module loads, pools)
-    pq = serve_mix(plain, max(6, nq // 4), seed=D.SEED_SERVE)
-    p_t, p_res = closed_loop(plain, pq)
-    plain_qps = len(pq) / p_t
-    print(f"[bench] plain: {plain_qps:.2f} queries/s", file=sys.stderr, flush=True)
-    plain.close()
+    # plain: one query at a time through the operators (the reference's
+    # execution of a star query: compaction, join + take per dimension,
+    # GROUP BY, ORDER BY, LIMIT)
+    from craton_bolt_amd.ops import execute_star_query
+
+    for spec in serve_specs(3, seed=1):  # warm-up (module loads, pools)
+        execute_star_query(fact, dims, **spec)
+    n_plain = max(6, nq // 8)
+    pspecs = serve_specs(n_plain, seed=D.SEED_SERVE)
+    t = time.perf_counter()
+    c_res = [execute_star_query(fact, dims, **spec) for spec in pspecs]
+    c_t = time.perf_counter() - t
+    plain_qps = n_plain / c_t
+    print(f"[bench] plain (operator composition): {plain_qps:.2f} queries/s", file=sys.stderr, flush=True)
+
+    # the fused single-query form: the server with one slot and one chunk
+    fused = Server(fact, dims, slots=1, chunk_rows=n)
+    closed_loop(fused, serve_mix(fused, 3, seed=1))
+    fq = serve_mix(fused, max(6, nq // 4), seed=D.SEED_SERVE)
+    f_t, f_res = closed_loop(fused, fq)
+    fused_qps = len(fq) / f_t
+    print(f"[bench] fused single-slot: {fused_qps:.2f} queries/s", file=sys.stderr, flush=True)
+    fused.close()
 
     srv = Server(fact, dims, slots=args.slots)
     closed_loop(srv, serve_mix(srv, args.slots, seed=2))
@@ -494,21 +528,9 @@ def bench_serve(args) -> dict:
     served_qps = nq / s_t
     bad = sum(1 for r in s_res if r["status"] != 0)
     print(f"[bench] served: {served_qps:.2f} queries/s ({bad} errors)", file=sys.stderr, flush=True)
-    # identical results: the plain queries against their served twins (same
-    # seeds -> same queries); floats within 1e-9 relative (atomic order)
-    def agree(a, b):
-        if a["status"] != b["status"] or a["n_groups"] != b["n_groups"] or len(a["rows"]) != len(b["rows"]):
-            return False
-        for ra, rb in zip(a["rows"], b["rows"]):
-            for x, y in zip(ra, rb):
-                if isinstance(y, float):
-                    if abs(x - y) > 1e-9 * max(abs(y), 1.0):
-                        return False
-                elif x != y:
-                    return False
-        return True
-
-    same = all(agree(a, b) for a, b in zip(p_res, s_res[: len(p_res)]))
+    # identical results: the same seeded queries through all three paths
+    same = all(_agree(a, b) for a, b in zip(c_res, s_res[: len(c_res)])) and \
+        all(_agree(a, b) for a, b in zip(f_res, s_res[: len(f_res)]))
 
     sweep = []
     rng = np.random.default_rng(D.SEED_SERVE + 7)
@@ -545,10 +567,14 @@ def bench_serve(args) -> dict:
     line = _line(served_qps, "queries/s", args, s_t * 1000.0 / nq, "decimal128/int64/f64",
                  {"workload": f"served star queries (filter -> 1..3 joins -> GROUP BY 10K..100K groups -> ORDER BY "
                               f"+ LIMIT 100) over {n} fact rows, {args.slots} slots, continuous batching",
-                  "fact_rows": n, "slots": args.slots, "plain_queries_per_s": plain_qps,
-                  "speedup_vs_plain": served_qps / plain_qps, "saturation_p50_ms": float(np.percentile(lat_closed, 50)),
+                  "fact_rows": n, "slots": args.slots,
+                  "plain": "one query at a time through the operators (count + compaction, hash build + pair "
+                           "probe + take per join, GROUP BY, ORDER BY, LIMIT), as the reference runs a query",
+                  "plain_queries_per_s": plain_qps, "speedup_vs_plain": served_qps / plain_qps,
+                  "fused_single_slot_queries_per_s": fused_qps, "speedup_vs_fused_single_slot": served_qps / fused_qps,
+                  "saturation_p50_ms": float(np.percentile(lat_closed, 50)),
                   "saturation_p99_ms": float(np.percentile(lat_closed, 99)), "poisson_sweep": sweep,
-                  "errors": bad, "plain_and_served_agree": bool(same), "server_stats": st},
+                  "errors": bad, "plain_fused_and_served_agree": bool(same), "server_stats": st},
                  _roof(fact_bytes, s_t * 1000.0 / max(laps, 1e-9),
                        "shared scan step kernel: fact bytes per lap / mean lap time at saturation"))
     return line

@@ -234,6 +234,7 @@ _SIGS = {
     "bolt_filter_compact": (C.c_int, [C.c_void_p, C.POINTER(BoltColumn), C.c_int32, C.POINTER(BoltFilter),
                                       C.c_void_p, C.POINTER(BoltColumn), C.POINTER(C.c_int64)]),
     "bolt_prefix_scan_mask": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(C.c_int64)]),
+    "bolt_take": (C.c_int, [C.c_void_p, C.POINTER(BoltColumn), C.c_void_p, C.c_int64, C.POINTER(BoltColumn)]),
     "bolt_groupby": (C.c_int, [C.c_void_p, C.POINTER(BoltColumn), C.c_int32, C.POINTER(BoltColumn), C.c_int32,
                                C.POINTER(BoltAgg), C.c_int32, C.POINTER(BoltFilter), C.c_uint32,
                                C.POINTER(C.c_void_p)]),

@@ -10,6 +10,8 @@
 // obtains the tile's output offset by decoupled look-back and writes the
 // survivors of every column: each input byte is read once and each output
 // byte written once.
+#include <algorithm>
+
 #include "ops_common.h"
 #include "scan_common.h"
 
@@ -31,6 +33,7 @@ struct GatherCol {
 struct GatherList {
   GatherCol c[kMaxCompactCols];
   int32_t n;
+  int64_t cap;  // rows the outputs hold: survivors beyond it are counted, not written
 };
 
 __device__ __forceinline__ void copy_elem(const GatherCol& g, int64_t from, int64_t to) {
@@ -100,7 +103,7 @@ __global__ __launch_bounds__(kScanBlock) void compact_kernel(DevFilter f, const 
     uint64_t pos = prefix + sh_cnt[j * 4 + wave] + (uint64_t)__popcll(b & below);
     if (MODE == 1) {
       if (row < n) offsets[row] = (uint32_t)pos;
-    } else if (k) {
+    } else if (k && (int64_t)pos < cols.cap) {
       for (int c = 0; c < cols.n; c++) copy_elem(cols.c[c], row, (int64_t)pos);
     }
   }
@@ -117,6 +120,42 @@ static int scan_prepare(ScanScratch& sc, int64_t n_tiles, hipStream_t s) {
   BOLT_TRY(fill_u32(sc.status.ptr, 0, (size_t)n_tiles * 2, s));
   BOLT_TRY(fill_u32(sc.misc.ptr, 0, 4, s));
   return BOLT_OK;
+}
+
+// Gather: dst[i] = src[rows[i]].  The validity bitmap is rebuilt a wave at a
+// time (64 bits -> 8 bytes, written by the wave's first lane), so it needs no
+// clearing; a row id out of range sets *err and gathers zero.
+__global__ __launch_bounds__(256) void take_kernel(const uint8_t* __restrict__ src, const uint8_t* __restrict__ sval,
+                                                   int64_t src_len, int width, const uint32_t* __restrict__ rows,
+                                                   int64_t n, uint8_t* __restrict__ dst, uint8_t* __restrict__ dval,
+                                                   uint32_t* __restrict__ err) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t end = (n + 63) / 64 * 64;  // wave-uniform trip count for the ballot
+  const int64_t nbytes = (n + 7) / 8;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < end; i += stride) {
+    const bool in = i < n;
+    int64_t r = in ? (int64_t)rows[i] : 0;
+    const bool ok = r < src_len;
+    if (in && !ok) atomicOr(err, 1u);
+    if (!ok) r = 0;
+    if (in) {
+      switch (width) {
+        case 1: dst[i] = ok ? src[r] : 0; break;
+        case 4: reinterpret_cast<uint32_t*>(dst)[i] = ok ? reinterpret_cast<const uint32_t*>(src)[r] : 0u; break;
+        case 8: reinterpret_cast<uint64_t*>(dst)[i] = ok ? reinterpret_cast<const uint64_t*>(src)[r] : 0ull; break;
+        case 16:
+          reinterpret_cast<uint4*>(dst)[i] = ok ? reinterpret_cast<const uint4*>(src)[r] : make_uint4(0u, 0u, 0u, 0u);
+          break;
+      }
+    }
+    if (dval) {
+      const uint64_t b = __ballot(in && ok && bit_valid(sval, r));
+      if ((threadIdx.x & 63) == 0) {
+        const int64_t b0 = i / 8;  // i is a multiple of 64 on the wave's first lane
+        for (int k = 0; k < 8 && b0 + k < nbytes; k++) dval[b0 + k] = (uint8_t)(b >> (8 * k));
+      }
+    }
+  }
 }
 
 }  // namespace bolt
@@ -136,6 +175,7 @@ extern "C" int bolt_filter_compact(bolt_stream_t stream, const bolt_column_t* co
   GatherList gl;
   memset(&gl, 0, sizeof gl);
   gl.n = n_cols;
+  gl.cap = out[0].length;
   for (int i = 0; i < n_cols; i++) {
     if (cols[i].length != n)
       return set_error(BOLT_ERR_PLAN, "bolt_filter_compact: column %d has %lld rows, column 0 has %lld", i,
@@ -144,9 +184,9 @@ extern "C" int bolt_filter_compact(bolt_stream_t stream, const bolt_column_t* co
     if (w == 0) return set_error(BOLT_ERR_TYPE, "bolt_filter_compact: column %d has unknown dtype", i);
     if (out[i].dtype != cols[i].dtype)
       return set_error(BOLT_ERR_PLAN, "bolt_filter_compact: output column %d dtype differs from input", i);
-    if (out[i].length < n)
-      return set_error(BOLT_ERR_PLAN, "bolt_filter_compact: output column %d capacity %lld < %lld rows", i,
-                       (long long)out[i].length, (long long)n);
+    if (out[i].length < 0)
+      return set_error(BOLT_ERR_PLAN, "bolt_filter_compact: output column %d has a negative capacity", i);
+    gl.cap = std::min<int64_t>(gl.cap, out[i].length);
     if (cols[i].validity && !out[i].validity)
       return set_error(BOLT_ERR_PLAN, "bolt_filter_compact: input column %d has NULLs, output has no bitmap", i);
     gl.c[i].src = static_cast<const uint8_t*>(cols[i].data);
@@ -161,7 +201,7 @@ extern "C" int bolt_filter_compact(bolt_stream_t stream, const bolt_column_t* co
   BOLT_TRY(make_dev_filter(filter, cols, n_cols, n, &f));
   if (n == 0) return BOLT_OK;
   for (int i = 0; i < n_cols; i++)
-    if (gl.c[i].dst_valid) BOLT_TRY(fill_u32(gl.c[i].dst_valid, 0, (size_t)(n + 31) / 32, s));
+    if (gl.c[i].dst_valid) BOLT_TRY(fill_u32(gl.c[i].dst_valid, 0, (size_t)(std::min(gl.cap, n) + 31) / 32, s));
   int64_t n_tiles = (n + kTileRows - 1) / kTileRows;
   ScanScratch sc;
   BOLT_TRY(scan_prepare(sc, n_tiles, s));
@@ -174,6 +214,9 @@ extern "C" int bolt_filter_compact(bolt_stream_t stream, const bolt_column_t* co
   BOLT_HIP(hipStreamSynchronize(s));
   if (h[1]) return set_error(BOLT_ERR_OTHER, "bolt_filter_compact: look-back timed out");
   *n_out = (int64_t)(((uint64_t)h[3] << 32) | h[2]);
+  if (*n_out > gl.cap)
+    return set_error(BOLT_DECLINE, "bolt_filter_compact: %lld rows survive, the outputs hold %lld",
+                     (long long)*n_out, (long long)gl.cap);
   return BOLT_OK;
 }
 
@@ -202,5 +245,37 @@ extern "C" int bolt_prefix_scan_mask(bolt_stream_t stream, const uint8_t* mask, 
   BOLT_HIP(hipStreamSynchronize(s));
   if (h[1]) return set_error(BOLT_ERR_OTHER, "bolt_prefix_scan_mask: look-back timed out");
   *total_out = (int64_t)(((uint64_t)h[3] << 32) | h[2]);
+  return BOLT_OK;
+}
+
+extern "C" int bolt_take(bolt_stream_t stream, const bolt_column_t* src, const uint32_t* rows, int64_t n,
+                         bolt_column_t* dst) {
+  BOLT_CHECK_DEVICE();
+  clear_error();
+  hipStream_t s = (hipStream_t)stream;
+  if (!src || !dst) return set_error(BOLT_ERR_PLAN, "bolt_take: NULL column");
+  if (n < 0) return set_error(BOLT_ERR_PLAN, "bolt_take: negative length");
+  const int width = dtype_size(src->dtype);
+  if (width == 0) return set_error(BOLT_ERR_TYPE, "bolt_take: unknown dtype %d", src->dtype);
+  if (src->validity && !dst->validity)
+    return set_error(BOLT_ERR_PLAN, "bolt_take: the source has a validity bitmap, the destination none");
+  dst->dtype = src->dtype;
+  dst->scale = src->scale;
+  dst->length = n;
+  if (n == 0) return BOLT_OK;
+  if (!rows || !dst->data) return set_error(BOLT_ERR_PLAN, "bolt_take: NULL rows or destination");
+  DevBuf<uint32_t> err;
+  BOLT_TRY(err.alloc(1, s));
+  BOLT_TRY(fill_u32(err.ptr, 0, 1, s));
+  take_kernel<<<grid_for(n, 256, 4), 256, 0, s>>>(static_cast<const uint8_t*>(src->data), src->validity, src->length,
+                                                  width, rows, n, static_cast<uint8_t*>(const_cast<void*>(dst->data)),
+                                                  src->validity ? const_cast<uint8_t*>(dst->validity) : nullptr,
+                                                  err.ptr);
+  BOLT_HIP(hipGetLastError());
+  uint32_t e = 0;
+  BOLT_HIP(hipMemcpyAsync(&e, err.ptr, 4, hipMemcpyDeviceToHost, s));
+  BOLT_HIP(hipStreamSynchronize(s));
+  if (e) return set_error(BOLT_ERR_PLAN, "bolt_take: a row id is out of range (source has %lld rows)",
+                          (long long)src->length);
   return BOLT_OK;
 }

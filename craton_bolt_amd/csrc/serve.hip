@@ -16,6 +16,7 @@
 // the radix-sort permutation of bolt_sort_indices) and the first LIMIT rows
 // are copied out.  Its slot and memory are then free for the next request.
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -70,6 +71,15 @@ struct SPre {
 struct SQ {
   DevFilter f;
   int8_t term_slot[BOLT_MAX_PRED_TERMS];  // preloaded column of each term, -1 = read it directly
+  // The conjunction as inclusive intervals over the order-preserving 128-bit
+  // image of each preloaded column (pre_image): row passes iff, for every
+  // slot c in cmask, lo[c] <= image_c <= hi[c], and every slot in vmask is
+  // non-NULL.  generic = 1 when a term is not such an interval (NE, IS NULL,
+  // a column outside the preloaded set): the terms are then evaluated one by
+  // one.
+  uint64_t lo_h[kPreCols], lo_l[kPreCols], hi_h[kPreCols], hi_l[kPreCols];
+  uint32_t cmask, vmask;
+  int32_t generic;
   GPlan P;
   SJoin j[BOLT_SERVE_MAX_DIMS];
   SGroupKey g[2];
@@ -85,93 +95,154 @@ __device__ __forceinline__ int64_t fk_value(const void* p, int dt, int64_t row) 
   return dt == BOLT_I64 ? static_cast<const int64_t*>(p)[row] : (int64_t) static_cast<const int32_t*>(p)[row];
 }
 
-constexpr int kServeRows = 4;  // rows per lane in the grid sizing
 
-// Step, part 1: every active query's predicate over one chunk.  Block b
-// owns rows [r0 + b*per, r0 + (b+1)*per) of the chunk and appends the rows
-// that pass query q, in order, to its own segment (q, b) of the row lists,
-// counting in LDS: no global atomics.  The predicate columns of all queries
-// are loaded once per row.  Part 2 then runs the joins and aggregation over
+// Step, part 1: every active query's predicate over one chunk.  Each wave
+// owns rows [r0 + w*per_w, r0 + (w+1)*per_w) of the chunk (w its global
+// wave index) and appends the rows that pass query q, in order, to its own
+// segment (q, w) of the row lists.  The running length of segment (q, w) is
+// lane q of one register (nq <= 64), so the append is a ballot, a popcount
+// and a lane read: no LDS, no barrier, no global atomics.  Each lane takes
+// kFilterRows rows per iteration, loading the predicate columns of all
+// queries once per row, so a query's terms are fetched once per
+// 64 * kFilterRows rows.  Part 2 then runs the joins and aggregation over
 // those dense lists with every lane busy, instead of the few lanes per wave
 // (a few percent of rows) that pass a selective predicate.
 constexpr int kServeGrid = 2048;
 constexpr int kServeWaves = kServeBlock / 64;
+constexpr int64_t kServeAllWaves = (int64_t)kServeGrid * kServeWaves;
+constexpr int kFilterRows = 4;
+
+// rows per wave segment for a chunk of `chunk_rows`
+inline int64_t serve_per_wave(int64_t chunk_rows) {
+  const int64_t step = 64 * kFilterRows;
+  return ((chunk_rows + kServeAllWaves - 1) / kServeAllWaves + step - 1) / step * step;
+}
+
+// Order-preserving 128-bit image (h, l) of a column value, under which every
+// comparison of the column with a literal of its type is an interval:
+// integers and Bool by sign flip; floats by the IEEE total-order map with
+// -0.0 taken as +0.0 (NaN maps to the top, above the image of +inf, so no
+// interval with a finite or infinite bound holds it); Decimal128 as the
+// sign-flipped high word, then the low word.
+__host__ __device__ __forceinline__ void value_image(int dt, int64_t vi, int64_t vhi, double vf, uint64_t* h,
+                                                     uint64_t* l) {
+  if (dt == BOLT_F32 || dt == BOLT_F64) {
+    *h = 0;
+    *l = f64_order_key(vf == 0.0 ? 0.0 : vf);
+  } else if (dt == BOLT_DEC128) {
+    *h = (uint64_t)vhi ^ 0x8000000000000000ull;
+    *l = (uint64_t)vi;
+  } else {
+    *h = 0;
+    *l = (uint64_t)vi ^ 0x8000000000000000ull;
+  }
+}
+
+__device__ __forceinline__ void pre_image(const Raw& r, int dt, uint64_t* h, uint64_t* l) {
+  int64_t vi, vhi;
+  double vf;
+  raw_value(r, dt, vi, vhi, vf);
+  value_image(dt, vi, vhi, vf, h, l);
+}
+
+__device__ __forceinline__ bool in_interval(uint64_t h, uint64_t l, uint64_t lh, uint64_t ll, uint64_t hh,
+                                            uint64_t hl) {
+  const bool ge = h > lh || (h == lh && l >= ll);
+  const bool le = h < hh || (h == hh && l <= hl);
+  return ge && le;
+}
 
 __global__ __launch_bounds__(kServeBlock) void serve_filter_kernel(const SQ* __restrict__ qs, int nq, SPre pre,
-                                                                   int64_t r0, int64_t r1, int64_t per,
+                                                                   int64_t r0, int64_t r1, int64_t per_w,
                                                                    uint32_t* __restrict__ lists,
                                                                    uint32_t* __restrict__ counts) {
-  __shared__ uint32_t wcnt[BOLT_SERVE_MAX_SLOTS][kServeWaves];
-  __shared__ uint32_t running[BOLT_SERVE_MAX_SLOTS];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int64_t gw = (int64_t)blockIdx.x * kServeWaves + (threadIdx.x >> 6);
   const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  for (int q = threadIdx.x; q < nq; q += kServeBlock) running[q] = 0;
-  __syncthreads();
-  const int64_t b0 = r0 + (int64_t)blockIdx.x * per;
-  const int64_t b1 = min(r1, b0 + per);
-  for (int64_t base = b0; base < b1; base += kServeBlock) {  // block-uniform trip count
-    const int64_t row = base + threadIdx.x;
-    const bool in = row < b1;
-    const int64_t rc = in ? row : b1 - 1;  // clamped: every load is in range
-    // named registers (no array, so nothing is indexed at run time)
-    Raw c0{0, 0}, c1{0, 0}, c2{0, 0}, c3{0, 0};
-    bool v0 = true, v1 = true, v2 = true, v3 = true;
-    if (pre.n > 0) { c0 = load_raw(pre.d[0], pre.dt[0], rc); v0 = bit_valid(pre.v[0], rc); }
-    if (pre.n > 1) { c1 = load_raw(pre.d[1], pre.dt[1], rc); v1 = bit_valid(pre.v[1], rc); }
-    if (pre.n > 2) { c2 = load_raw(pre.d[2], pre.dt[2], rc); v2 = bit_valid(pre.v[2], rc); }
-    if (pre.n > 3) { c3 = load_raw(pre.d[3], pre.dt[3], rc); v3 = bit_valid(pre.v[3], rc); }
-    uint64_t passmask = 0;  // bit q: the row passes query q (nq <= 64)
+  const int64_t w0 = r0 + gw * per_w;
+  const int64_t w1 = min(r1, w0 + per_w);
+  uint32_t run = 0;  // lane q: rows appended to segment (q, gw)
+  for (int64_t base = w0; base < w1; base += 64 * kFilterRows) {  // wave-uniform trip count
+    // each preloaded value's image, computed once per row for every query
+    uint64_t h0[kFilterRows], l0[kFilterRows], h1[kFilterRows], l1[kFilterRows];
+    uint64_t h2[kFilterRows], l2[kFilterRows], h3[kFilterRows], l3[kFilterRows];
+    uint32_t vm[kFilterRows];  // bit c: preloaded slot c is non-NULL
+    int64_t rc[kFilterRows];
+    bool in[kFilterRows];
+#pragma unroll
+    for (int r = 0; r < kFilterRows; r++) {
+      const int64_t row = base + r * 64 + lane;
+      in[r] = row < w1;
+      rc[r] = in[r] ? row : w1 - 1;  // clamped: every load is in range
+      h0[r] = l0[r] = h1[r] = l1[r] = h2[r] = l2[r] = h3[r] = l3[r] = 0;
+      vm[r] = 0xFu;
+      if (pre.n > 0) {
+        pre_image(load_raw(pre.d[0], pre.dt[0], rc[r]), pre.dt[0], &h0[r], &l0[r]);
+        if (!bit_valid(pre.v[0], rc[r])) vm[r] &= ~1u;
+      }
+      if (pre.n > 1) {
+        pre_image(load_raw(pre.d[1], pre.dt[1], rc[r]), pre.dt[1], &h1[r], &l1[r]);
+        if (!bit_valid(pre.v[1], rc[r])) vm[r] &= ~2u;
+      }
+      if (pre.n > 2) {
+        pre_image(load_raw(pre.d[2], pre.dt[2], rc[r]), pre.dt[2], &h2[r], &l2[r]);
+        if (!bit_valid(pre.v[2], rc[r])) vm[r] &= ~4u;
+      }
+      if (pre.n > 3) {
+        pre_image(load_raw(pre.d[3], pre.dt[3], rc[r]), pre.dt[3], &h3[r], &l3[r]);
+        if (!bit_valid(pre.v[3], rc[r])) vm[r] &= ~8u;
+      }
+    }
     for (int qi = 0; qi < nq; qi++) {
       const SQ& Q = qs[qi];
-      bool pass = in;
-      for (int t = 0; t < Q.f.n && pass; t++) {
-        const int sl = Q.term_slot[t];
-        if (sl < 0) {
-          pass = eval_term(Q.f.t[t], rc);
+      uint32_t* seg = lists + ((uint64_t)qi * kServeAllWaves + gw) * (uint64_t)per_w;
+      const uint32_t cm = Q.cmask, vq = Q.vmask;
+      const bool generic = Q.generic != 0;
+#pragma unroll
+      for (int r = 0; r < kFilterRows; r++) {
+        bool pass = in[r];
+        if (generic) {
+          for (int t = 0; t < Q.f.n; t++) pass = pass && eval_term(Q.f.t[t], rc[r]);
         } else {
-          const Raw x = sl == 0 ? c0 : sl == 1 ? c1 : sl == 2 ? c2 : c3;
-          const bool xv = sl == 0 ? v0 : sl == 1 ? v1 : sl == 2 ? v2 : v3;
-          pass = eval_term_raw(Q.f.t[t], x, xv);
+          pass = pass && (vm[r] & vq) == vq;
+          if (cm & 1u) pass = pass && in_interval(h0[r], l0[r], Q.lo_h[0], Q.lo_l[0], Q.hi_h[0], Q.hi_l[0]);
+          if (cm & 2u) pass = pass && in_interval(h1[r], l1[r], Q.lo_h[1], Q.lo_l[1], Q.hi_h[1], Q.hi_l[1]);
+          if (cm & 4u) pass = pass && in_interval(h2[r], l2[r], Q.lo_h[2], Q.lo_l[2], Q.hi_h[2], Q.hi_l[2]);
+          if (cm & 8u) pass = pass && in_interval(h3[r], l3[r], Q.lo_h[3], Q.lo_l[3], Q.hi_h[3], Q.hi_l[3]);
+        }
+        const uint64_t m = __ballot(pass);
+        if (m) {
+          const uint32_t off = (uint32_t)__builtin_amdgcn_readlane((int)run, qi);
+          if (pass) seg[off + (uint32_t)__popcll(m & lt)] = (uint32_t)(rc[r] - r0);
+          if (lane == qi) run += (uint32_t)__popcll(m);
         }
       }
-      const uint64_t m = __ballot(pass);
-      if (lane == 0) wcnt[qi][wave] = (uint32_t)__popcll(m);
-      if (pass) passmask |= 1ull << qi;
     }
-    __syncthreads();
-    for (int qi = 0; qi < nq; qi++) {
-      const bool pass = (passmask >> qi) & 1ull;
-      const uint64_t m = __ballot(pass);
-      if (pass) {
-        uint32_t off = running[qi];
-        for (int w = 0; w < wave; w++) off += wcnt[qi][w];
-        off += (uint32_t)__popcll(m & lt);
-        lists[((uint64_t)qi * gridDim.x + blockIdx.x) * (uint64_t)per + off] = (uint32_t)(row - r0);
-      }
-    }
-    __syncthreads();
-    for (int q = threadIdx.x; q < nq; q += kServeBlock) {
-      uint32_t add = 0;
-      for (int w = 0; w < kServeWaves; w++) add += wcnt[q][w];
-      running[q] += add;
-    }
-    __syncthreads();
   }
-  for (int q = threadIdx.x; q < nq; q += kServeBlock) counts[(uint64_t)q * gridDim.x + blockIdx.x] = running[q];
+  if (lane < nq) counts[(uint64_t)lane * kServeAllWaves + gw] = run;
 }
 
 // Step, part 2: block b runs the joins and aggregation for the rows in the
-// segments (q, b) that filter block b wrote.
+// segments (q, w) of its own waves w = 4b .. 4b + 3, taken as one index
+// space so that every lane has a row.
 __global__ __launch_bounds__(kServeBlock) void serve_probe_kernel(const SQ* __restrict__ qs, int nq, SDims dims,
-                                                                  int64_t r0, int64_t per,
+                                                                  int64_t r0, int64_t per_w,
                                                                   const uint32_t* __restrict__ lists,
                                                                   const uint32_t* __restrict__ counts) {
   for (int qi = 0; qi < nq; qi++) {
     const SQ& Q = qs[qi];
-    const uint64_t seg = (uint64_t)qi * gridDim.x + blockIdx.x;
-    const uint32_t cnt = counts[seg];
+    const uint64_t seg0 = (uint64_t)qi * kServeAllWaves + (uint64_t)blockIdx.x * kServeWaves;
+    uint32_t pfx[kServeWaves + 1];
+    pfx[0] = 0;
+#pragma unroll
+    for (int w = 0; w < kServeWaves; w++) pfx[w + 1] = pfx[w] + counts[seg0 + w];
+    const uint32_t cnt = pfx[kServeWaves];
     for (uint32_t i = threadIdx.x; i < cnt; i += kServeBlock) {
-      const int64_t row = r0 + (int64_t)lists[seg * (uint64_t)per + i];
+      int w = 0;
+#pragma unroll
+      for (int k = 1; k < kServeWaves; k++) w += i >= pfx[k] ? 1 : 0;
+      const uint32_t off = i - (w == 0 ? 0u : w == 1 ? pfx[1] : w == 2 ? pfx[2] : pfx[3]);
+      const int64_t row = r0 + (int64_t)lists[(seg0 + w) * (uint64_t)per_w + off];
       uint32_t d0 = 0, d1 = 0, d2 = 0;
       bool ok = true;
       for (int j = 0; j < Q.n_joins && ok; j++) {
@@ -179,9 +250,9 @@ __global__ __launch_bounds__(kServeBlock) void serve_probe_kernel(const SQ* __re
         const int64_t k = fk_value(Q.j[j].fk, Q.j[j].fk_dt, row);
         bool hit = false;
         uint32_t v = 0;
-        probe_chain(D.tbl, D.mask, k, true, [&](uint32_t w) {
+        probe_chain(D.tbl, D.mask, k, true, [&](uint32_t wv) {
           hit = true;
-          v = w;
+          v = wv;
         });
         ok = hit;
         if (j == 0) d0 = v;
@@ -241,6 +312,66 @@ __global__ __launch_bounds__(256) void serve_extract_kernel(GTable t, GPlan P, u
       write_key(o.key[1], pos, packed & 0xFFFFFFFFull);
     }
     for (int a = 0; a < P.n; a++) finalize_to(t, P.a[a], o.agg[a], stride, s, pos);
+  }
+}
+
+// Host: a query's terms on preloaded columns as intervals over value_image
+// (a conjunction of comparisons on one column is one interval).
+static void compile_intervals(SQ& d) {
+  d.cmask = d.vmask = 0;
+  d.generic = 0;
+  for (int c = 0; c < kPreCols; c++) {
+    d.lo_h[c] = d.lo_l[c] = 0;
+    d.hi_h[c] = d.hi_l[c] = ~0ull;
+  }
+  auto less = [](uint64_t ah, uint64_t al, uint64_t bh, uint64_t bl) { return ah < bh || (ah == bh && al < bl); };
+  for (int t = 0; t < d.f.n; t++) {
+    const DevTerm& tm = d.f.t[t];
+    const int sl = d.term_slot[t];
+    if (sl < 0 || tm.op == BOLT_CMP_IS_NULL || tm.op == BOLT_CMP_NE) {
+      d.generic = 1;
+      continue;
+    }
+    d.vmask |= 1u << sl;
+    if (tm.op == BOLT_CMP_IS_NOT_NULL) continue;
+    const bool fl = tm.dtype == BOLT_F32 || tm.dtype == BOLT_F64;
+    if (fl && std::isnan(tm.lit_f64)) {  // every comparison with NaN is false
+      d.generic = 1;
+      continue;
+    }
+    if (!(d.cmask & (1u << sl)) && fl) {  // a constrained float column never holds NaN
+      uint64_t h, l;
+      value_image(tm.dtype, 0, 0, HUGE_VAL, &h, &l);
+      d.hi_h[sl] = h;
+      d.hi_l[sl] = l;
+    }
+    d.cmask |= 1u << sl;
+    uint64_t h, l;
+    value_image(tm.dtype, tm.lit_i64, tm.lit_hi, tm.lit_f64, &h, &l);
+    uint64_t lh = h, ll = l, hh = h, hl = l;  // the term's own interval
+    bool lower = true, upper = true, empty = false;
+    switch (tm.op) {
+      case BOLT_CMP_GE: upper = false; break;
+      case BOLT_CMP_LE: lower = false; break;
+      case BOLT_CMP_GT:
+        upper = false;
+        if (h == ~0ull && l == ~0ull) empty = true;
+        else { ll = l + 1; lh = h + (ll == 0 ? 1 : 0); }
+        break;
+      case BOLT_CMP_LT:
+        lower = false;
+        if (h == 0 && l == 0) empty = true;
+        else { hl = l - 1; hh = h - (l == 0 ? 1 : 0); }
+        break;
+      default: break;  // EQ: both bounds
+    }
+    if (empty) {  // lo > hi: nothing passes
+      d.lo_h[sl] = d.lo_l[sl] = ~0ull;
+      d.hi_h[sl] = d.hi_l[sl] = 0;
+      continue;
+    }
+    if (lower && less(d.lo_h[sl], d.lo_l[sl], lh, ll)) { d.lo_h[sl] = lh; d.lo_l[sl] = ll; }
+    if (upper && less(hh, hl, d.hi_h[sl], d.hi_l[sl])) { d.hi_h[sl] = hh; d.hi_l[sl] = hl; }
   }
 }
 
@@ -657,6 +788,7 @@ struct bolt_server {
               }
               d.term_slot[t] = (int8_t)slot;
             }
+            compile_intervals(d);
             h_desc[i++] = d;
           }
           (void)hipMemcpyAsync(d_desc.ptr, h_desc, sizeof(SQ) * active.size(), hipMemcpyHostToDevice, s);
@@ -665,10 +797,10 @@ struct bolt_server {
         const int64_t r0 = cur_chunk * cfg.chunk_rows;
         const int64_t r1 = std::min(n_rows, r0 + cfg.chunk_rows);
         const auto t_step = SClock::now();
-        const int64_t per = ((cfg.chunk_rows + kServeGrid - 1) / kServeGrid + kServeBlock - 1) / kServeBlock * kServeBlock;
-        serve_filter_kernel<<<kServeGrid, kServeBlock, 0, s>>>(d_desc.ptr, (int)active.size(), pre, r0, r1, per,
+        const int64_t per_w = serve_per_wave(cfg.chunk_rows);
+        serve_filter_kernel<<<kServeGrid, kServeBlock, 0, s>>>(d_desc.ptr, (int)active.size(), pre, r0, r1, per_w,
                                                                d_lists.ptr, d_counts.ptr);
-        serve_probe_kernel<<<kServeGrid, kServeBlock, 0, s>>>(d_desc.ptr, (int)active.size(), sdims, r0, per,
+        serve_probe_kernel<<<kServeGrid, kServeBlock, 0, s>>>(d_desc.ptr, (int)active.size(), sdims, r0, per_w,
                                                               d_lists.ptr, d_counts.ptr);
         (void)hipStreamSynchronize(s);
         step_us += std::chrono::duration_cast<std::chrono::microseconds>(SClock::now() - t_step).count();
@@ -813,10 +945,9 @@ extern "C" int bolt_server_create(bolt_stream_t stream, const bolt_column_t* fac
   }
   int st = srv->d_desc.alloc(srv->cfg.slots, srv->s);
   {
-    const int64_t per =
-        ((srv->cfg.chunk_rows + kServeGrid - 1) / kServeGrid + kServeBlock - 1) / kServeBlock * kServeBlock;
-    if (st == BOLT_OK) st = srv->d_lists.alloc((size_t)srv->cfg.slots * kServeGrid * (size_t)per, srv->s);
-    if (st == BOLT_OK) st = srv->d_counts.alloc((size_t)srv->cfg.slots * kServeGrid, srv->s);
+    const int64_t per_w = serve_per_wave(srv->cfg.chunk_rows);
+    if (st == BOLT_OK) st = srv->d_lists.alloc((size_t)srv->cfg.slots * kServeAllWaves * (size_t)per_w, srv->s);
+    if (st == BOLT_OK) st = srv->d_counts.alloc((size_t)srv->cfg.slots * kServeAllWaves, srv->s);
   }
   if (st == BOLT_OK && hipHostMalloc(reinterpret_cast<void**>(&srv->h_desc), sizeof(SQ) * srv->cfg.slots,
                                      hipHostMallocDefault) != hipSuccess)

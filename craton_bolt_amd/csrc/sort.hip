@@ -617,12 +617,14 @@ __device__ __forceinline__ uint64_t radix_image(const void* d, int dt, int64_t r
       return b ^ m;
     }
     case BOLT_BOOL: return static_cast<const uint8_t*>(d)[row] ? 1 : 0;
+    case BOLT_DEC128: return static_cast<const uint64_t*>(d)[2 * row];  // low word, unsigned (mode 2: high word)
   }
   return 0;
 }
 
 // key[i] = image of column value at perm[i]; mode 0 = value (DESC = NOT),
-// mode 1 = null rank only (0 first / 1 last per nulls_first).
+// mode 1 = null rank only (0 first / 1 last per nulls_first), mode 2 = the
+// signed high word of a Decimal128 (its low word, unsigned, is mode 0).
 __global__ void order_key_kernel(const void* __restrict__ d, const uint8_t* __restrict__ valid, int dt, int bits,
                                  int desc, int nulls_first, int mode, const uint32_t* __restrict__ perm,
                                  uint64_t* __restrict__ key, int64_t n) {
@@ -634,7 +636,9 @@ __global__ void order_key_kernel(const void* __restrict__ d, const uint8_t* __re
       bool isnull = !bit_valid(valid, row);
       key[i] = isnull ? (nulls_first ? 0 : 1) : (nulls_first ? 1 : 0);
     } else {
-      uint64_t k = bit_valid(valid, row) ? radix_image(d, dt, row) : 0;
+      uint64_t k = 0;
+      if (bit_valid(valid, row))
+        k = mode == 2 ? static_cast<const uint64_t*>(d)[2 * row + 1] ^ 0x8000000000000000ull : radix_image(d, dt, row);
       key[i] = (desc ? ~k : k) & mask;
     }
   }
@@ -688,7 +692,8 @@ extern "C" int bolt_sort_indices(bolt_stream_t stream, const bolt_column_t* keys
   for (int k = 0; k < n_keys; k++) {
     if (keys[k].length != n) return set_error(BOLT_ERR_PLAN, "bolt_sort_indices: key columns differ in length");
     int dt = keys[k].dtype;
-    if (!(dt == BOLT_I32 || dt == BOLT_I64 || dt == BOLT_F32 || dt == BOLT_F64 || dt == BOLT_DATE32 || dt == BOLT_BOOL))
+    if (!(dt == BOLT_I32 || dt == BOLT_I64 || dt == BOLT_F32 || dt == BOLT_F64 || dt == BOLT_DATE32 || dt == BOLT_BOOL ||
+          dt == BOLT_DEC128))
       return set_error(BOLT_ERR_UNSUPPORTED, "bolt_sort_indices: sort key dtype %s not supported", dtype_name(dt));
   }
   if (n == 0) return BOLT_OK;
@@ -704,10 +709,15 @@ extern "C" int bolt_sort_indices(bolt_stream_t stream, const bolt_column_t* keys
   // permutation, so earlier (more significant) keys win ties last
   for (int k = n_keys - 1; k >= 0; k--) {
     const bolt_column_t& c = keys[k];
-    const int bits = c.dtype == BOLT_BOOL ? 1 : dtype_size(c.dtype) * 8;
+    const int bits = c.dtype == BOLT_BOOL ? 1 : c.dtype == BOLT_DEC128 ? 64 : dtype_size(c.dtype) * 8;
     const int desc = descending ? descending[k] : 0;
     const int nf = nulls_first ? nulls_first[k] : 1;
-    for (int mode = 0; mode < (c.validity ? 2 : 1); mode++) {
+    // value (a Decimal128 as its unsigned low word, then its signed high
+    // word), then the NULL rank: least significant first
+    std::vector<int> modes{0};
+    if (c.dtype == BOLT_DEC128) modes.push_back(2);
+    if (c.validity) modes.push_back(1);
+    for (int mode : modes) {
       order_key_kernel<<<grid_for(n, 256, 4), 256, 0, s>>>(c.data, c.validity, c.dtype, bits, desc, nf, mode, out_rows,
                                                           k0.ptr, n);
       BOLT_HIP(hipGetLastError());

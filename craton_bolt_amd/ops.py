@@ -319,6 +319,136 @@ class Server:
             pass
 
 
+def take(col: DeviceColumn, rows: DeviceBuffer, n: int, stream: Stream | None = None) -> DeviceColumn:
+    """col[rows[:n]] on the device (bolt_take)."""
+    out = DeviceColumn.empty(n, col.dtype, with_validity=col.validity is not None)
+    out.scale = col.scale
+    oc = out.c()
+    N.check(N.lib().bolt_take(_stream(stream), C.byref(col.c()), rows.ptr, n, C.byref(oc)), "bolt_take")
+    return out
+
+
+def execute_star_query(fact: list[DeviceColumn], dims: list[tuple[DeviceColumn, list[DeviceColumn]]],
+                       terms: list[Term], joins: list[tuple[int, int]], group: list[tuple[int, int]],
+                       aggs: list[tuple[int, int]], order_agg: int = 0, limit: int = 100,
+                       stream: Stream | None = None, est_groups: int = 0) -> dict:
+    """One star query composed from the operators, one at a time, the way the
+    reference's engine runs it (src/exec/engine.rs:2520 sql -> physical plan):
+    WHERE as a count plus filter-compaction of the referenced fact columns,
+    then per join a hash build on the dimension key, a probe emitting
+    (probe row, build row) pairs and a take of every carried column,
+    GROUP BY, ORDER BY the chosen aggregate DESC then the group key, LIMIT.
+    Aggregate-over-join is not fused (src/plan/physical_plan.rs:3691-3712).
+    Same query description and result rows as Server.query / Server.wait."""
+    del est_groups  # sizing hint of the served form; the operators size themselves
+    L = N.lib()
+    s = stream
+    fk_cols = [fk for fk, _ in joins]
+    agg_cols = [c for op, c in aggs if op != N.COUNT_STAR]
+    used = sorted(set(fk_cols + agg_cols + [t.col for t in terms]))
+    pos = {c: i for i, c in enumerate(used)}
+    cols = [fact[c] for c in used]
+    rterms = [Term(pos[t.col], t.op, t.value) for t in terms]
+    # WHERE: survivor count, then one compaction pass of the referenced columns
+    n_pass = execute_aggregate(cols, [(N.COUNT_STAR, 0)], rterms, stream=s)[0]
+    outs = [DeviceColumn.empty(max(n_pass, 1), c.dtype, with_validity=c.validity is not None) for c in cols]
+    ca, oa = columns_array(cols), columns_array(outs)
+    f, keep = _filter(rterms, cols)
+    n_out = C.c_int64()
+    N.check(L.bolt_filter_compact(_stream(s), ca, len(cols), C.byref(f) if f else None, None, oa, C.byref(n_out)),
+            "bolt_filter_compact")
+    n_cur = n_out.value
+    cur = {c: outs[pos[c]] for c in set(fk_cols + agg_cols)}
+    for c in cur.values():
+        c.length = n_cur
+    got = {}  # (join index, attribute) -> gathered dimension attribute
+    for j, (fk, d) in enumerate(joins):
+        dkey, dattrs = dims[d]
+        t = JoinTable(dkey, stream=s)
+        pr, br = DeviceBuffer(max(n_cur, 1) * 4), DeviceBuffer(max(n_cur, 1) * 4)
+        probe = [cur[fk]]
+        n_pairs = C.c_int64()
+        st = L.bolt_join_probe(_stream(s), t.handle, columns_array(probe), 1, 0, None, pr.ptr, br.ptr,
+                               max(n_cur, 1), C.byref(n_pairs))
+        if st == N.BOLT_DECLINE:  # duplicate dimension keys: room for every pair
+            cap = n_pairs.value
+            pr, br = DeviceBuffer(max(cap, 1) * 4), DeviceBuffer(max(cap, 1) * 4)
+            st = L.bolt_join_probe(_stream(s), t.handle, columns_array(probe), 1, 0, None, pr.ptr, br.ptr,
+                                   max(cap, 1), C.byref(n_pairs))
+        N.check(st, "bolt_join_probe")
+        t.release()
+        m = n_pairs.value
+        cur = {c: take(col, pr, m, s) for c, col in cur.items()}
+        got = {k: take(col, pr, m, s) for k, col in got.items()}
+        for jj, a in group:
+            if jj == j:
+                got[(j, a)] = take(dattrs[a], br, m, s)
+        n_cur = m
+    # GROUP BY the dimension attributes
+    keys = [got[(j, a)] for j, a in group]
+    in_pos = {c: i for i, c in enumerate(sorted(set(agg_cols)))}
+    inputs = [cur[c] for c in sorted(set(agg_cols))] or [keys[0]]
+    gaggs = [(op, in_pos.get(c, 0)) for op, c in aggs]
+    tbl = C.c_void_p()
+    N.check(L.bolt_groupby(_stream(s), columns_array(keys), len(keys), columns_array(inputs), len(inputs),
+                           _aggs(gaggs), len(gaggs), None, 0, C.byref(tbl)), "bolt_groupby")
+    try:
+        nr, nc = C.c_int64(), C.c_int32()
+        N.check(L.bolt_table_num_rows(tbl, C.byref(nr)))
+        N.check(L.bolt_table_num_columns(tbl, C.byref(nc)))
+        g = nr.value
+        tcols = []
+        for i in range(nc.value):
+            bc = N.BoltColumn()
+            N.check(L.bolt_table_column(tbl, i, C.byref(bc)))
+            tcols.append(bc)
+        rows = []
+        lim = min(limit, g)
+        if lim > 0:
+            # ORDER BY aggregate DESC NULLS LAST, then the group key ascending
+            oc = len(keys) + order_agg
+            sk = [tcols[oc]] + tcols[: len(keys)]
+            k = len(sk)
+            arr = (N.BoltColumn * k)(*sk)
+            desc = (C.c_int32 * k)(*([1] + [0] * len(keys)))
+            nf = (C.c_int32 * k)(*([0] * k))
+            perm = DeviceBuffer(max(g, 1) * 4)
+            N.check(L.bolt_sort_indices(_stream(s), arr, k, desc, nf, perm.ptr), "bolt_sort_indices")
+            cols_out = []
+            for bc in tcols:
+                src = DeviceColumn(_Borrowed(bc.data), g, bc.dtype, _Borrowed(bc.validity) if bc.validity else None,
+                                   bc.scale)
+                cols_out.append(take(src, perm, lim, s))
+            vals = [c.to_numpy(lim) for c in cols_out]
+            valid = [c.valid_numpy(lim) for c in cols_out]
+            for r in range(lim):
+                row = []
+                for c, bc in enumerate(tcols):
+                    if valid[c] is not None and not valid[c][r]:
+                        row.append(None)
+                    elif bc.dtype == N.DEC128:
+                        row.append((int(vals[c][r][1]) << 64) | (int(vals[c][r][0]) & 0xFFFFFFFFFFFFFFFF))
+                    elif bc.dtype in (N.F32, N.F64):
+                        row.append(float(vals[c][r]))
+                    else:
+                        row.append(int(vals[c][r]))
+                rows.append(tuple(row))
+    finally:
+        L.bolt_table_release(tbl)
+    return {"status": 0, "rows": rows, "n_groups": g}
+
+
+class _Borrowed:
+    """A device pointer owned elsewhere (a result table's column)."""
+
+    def __init__(self, ptr):
+        self.ptr = ptr
+        self.nbytes = 0
+
+    def free(self) -> None:
+        pass
+
+
 # ---- hash join (src/exec/gpu_join.rs:1583, src/exec/join.rs:169) --------------------
 
 class JoinTable:

@@ -142,9 +142,12 @@ int bolt_aggregate(bolt_stream_t stream, const bolt_column_t* inputs, int32_t n_
  * (src/exec/compact.rs:183) + compact_columns_on_gpu
  * (src/exec/gpu_compact.rs:1005).  Keeps the rows passing `filter` AND
  * (if `mask` is non-NULL) with mask[i] != 0, in input order.  `out` holds
- * n_cols caller-allocated device columns of capacity n_rows (their `validity`
- * must be a writable bitmap when the input column has one).  Writes the
- * survivor count to *n_out (host) and synchronises the stream.
+ * n_cols caller-allocated device columns whose `length` is their capacity
+ * (their `validity` must be a writable, 4-byte aligned bitmap when the input
+ * column has one); n_rows always suffices.  Writes the survivor count to
+ * *n_out (host) and synchronises the stream.  If more rows survive than the
+ * smallest capacity, nothing beyond it is written, *n_out is the full count
+ * and BOLT_DECLINE is returned (the capacity contract of bolt_join_probe).
  * Single pass: predicate, decoupled look-back scan and gather in one kernel.
  * ------------------------------------------------------------------------ */
 int bolt_filter_compact(bolt_stream_t stream, const bolt_column_t* cols, int32_t n_cols,
@@ -155,6 +158,16 @@ int bolt_filter_compact(bolt_stream_t stream, const bolt_column_t* cols, int32_t
  * (prefix_scan_mask, src/exec/gpu_compact.rs:415).  n < 2^32. */
 int bolt_prefix_scan_mask(bolt_stream_t stream, const uint8_t* mask, int64_t n, uint32_t* offsets,
                           int64_t* total);
+
+/* Gather by row ids — the take that follows a join (the host arrow-select
+ * take of execute_inner_join's callers, src/exec/join.rs:169, and the device
+ * gather of src/jit/prefix_scan.rs:1227): dst[i] = src[rows[i]] for i < n,
+ * validity bits likewise.  dst is caller-allocated with capacity n (and a
+ * writable bitmap of ceil(n / 8) bytes when src has one); it takes src's
+ * dtype and scale and length n.  A row id >= src->length is an error
+ * (BOLT_ERR_PLAN; that row gathers zero).  Synchronises. */
+int bolt_take(bolt_stream_t stream, const bolt_column_t* src, const uint32_t* rows, int64_t n,
+              bolt_column_t* dst);
 
 /* ------------------------------------------------------------------------
  * Hash GROUP BY — replaces execute_groupby (src/exec/groupby.rs:295) and its
@@ -246,7 +259,9 @@ int bolt_sort_pairs_i64(bolt_stream_t stream, const int64_t* keys_in, const uint
 /* ORDER BY: stable row permutation for a lexicographic multi-column key
  * (execute_sort, src/exec/sort.rs:161; key transforms of
  * src/exec/gpu_sort.rs:1070-1180: signed ints by sign flip, floats by the
- * IEEE-monotonic flip with NaN canonicalised last and -0.0 == +0.0).
+ * IEEE-monotonic flip with NaN canonicalised last and -0.0 == +0.0;
+ * Decimal128 keys as signed 128-bit integers, which the reference's host
+ * lexsort orders and its GPU transform does not).
  * descending / nulls_first are host arrays of n_keys flags. */
 int bolt_sort_indices(bolt_stream_t stream, const bolt_column_t* keys, int32_t n_keys,
                       const int32_t* descending, const int32_t* nulls_first, uint32_t* out_rows);

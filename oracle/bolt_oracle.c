@@ -620,9 +620,20 @@ static int sort_cmp_rows(const sort_ctx_t* S, uint32_t a, uint32_t b) {
       return a_first ? -1 : 1;
     }
     if (!va) continue;
-    uint64_t ia = sort_image(c, a), ib = sort_image(c, b);
-    if (ia == ib) continue;
-    int lt = ia < ib;
+    int lt;
+    if (c->dtype == BO_DEC128) {
+      /* Decimal128 orders as its signed 128-bit value (arrow-ord lexsort,
+         the comparator of src/exec/sort.rs:222) */
+      const uint64_t* p = (const uint64_t*)c->data;
+      i128 xa = (i128)(((unsigned __int128)p[2 * a + 1] << 64) | p[2 * a]);
+      i128 xb = (i128)(((unsigned __int128)p[2 * b + 1] << 64) | p[2 * b]);
+      if (xa == xb) continue;
+      lt = xa < xb;
+    } else {
+      uint64_t ia = sort_image(c, a), ib = sort_image(c, b);
+      if (ia == ib) continue;
+      lt = ia < ib;
+    }
     if (S->desc[k]) lt = !lt;
     return lt ? -1 : 1;
   }

@@ -13,7 +13,7 @@ import oracle as O
 from craton_bolt_amd import _native as N
 from craton_bolt_amd import datagen as D
 from craton_bolt_amd.device import DeviceColumn
-from craton_bolt_amd.ops import Server, Term
+from craton_bolt_amd.ops import Server, Term, execute_star_query
 
 pytestmark = pytest.mark.gpu
 
@@ -133,6 +133,34 @@ def test_serve_queries_match_oracle(star):
     srv.close()
 
 
+def test_serve_predicate_forms(star):
+    """Float, Decimal and Date32 intervals (strict and inclusive bounds,
+    equality, an empty range) and the terms that are not intervals (NE,
+    IS NOT NULL), evaluated per term: all against the oracle, served
+    together in one mix."""
+    d0 = D.STAR_DATE0 + 700
+    base = dict(joins=[(FK3, 2)], group=[(0, 1)], aggs=[(N.SUM, QTY), (N.COUNT_STAR, 0), (N.SUM, PRICE)],
+                order_agg=1, limit=30, est_groups=250)
+    cases = [
+        [Term(QTY, N.GT, 25.5), Term(QTY, N.LE, 40.0)],
+        [Term(QTY, N.GE, 10.0), Term(PRICE, N.LT, 500_000), Term(DATE, N.GE, d0), Term(DATE, N.LE, d0 + 300)],
+        [Term(DATE, N.EQ, d0 + 5)],
+        [Term(QTY, N.GT, 30.0), Term(DATE, N.NE, d0)],
+        [Term(PRICE, N.IS_NOT_NULL), Term(PRICE, N.GT, 900_000)],
+        [Term(QTY, N.GT, 40.0), Term(QTY, N.LT, 20.0)],
+    ]
+    srv = make_server(star, slots=8, chunk_rows=100_000)
+    tickets = [srv.submit(srv.query(terms=t, **base)) for t in cases]
+    got = [srv.wait(tk) for tk in tickets]
+    srv.close()
+    for i, (t, r) in enumerate(zip(cases, got)):
+        print(f"[case] predicate form {i}", flush=True)
+        assert r["status"] == N.BOLT_OK, r["error"]
+        g, want, sabs = reference(star, dict(base, terms=t))
+        assert r["n_groups"] == g
+        same_rows(r["rows"], want, 1, sabs)
+
+
 def test_serve_mix_equals_alone_and_overlaps(star):
     rng = np.random.default_rng(12)
     specs = []
@@ -199,4 +227,21 @@ def test_serve_rejects_bad_queries_and_regrows_small_tables(star):
     g, want, sabs = reference(star, small)
     assert r["n_groups"] == g
     same_rows(r["rows"], want, 1, sabs)
+    srv.close()
+
+
+def test_composed_operator_path_matches_oracle_and_served(star):
+    """The one-query-at-a-time operator composition (count, compaction, hash
+    build + pair probe + take per join, GROUP BY, ORDER BY, LIMIT) returns the
+    oracle's rows and the served form's."""
+    srv = make_server(star, slots=1, chunk_rows=100_000)
+    rng = np.random.default_rng(11)
+    for i, spec in enumerate(templates(rng)):
+        print(f"[case] composed template {i}", flush=True)
+        r = execute_star_query(star["dev"], star["dims_d"], **spec)
+        g, want, sabs = reference(star, spec)
+        assert r["n_groups"] == g
+        same_rows(r["rows"], want, len(spec["group"]), sabs)
+        sv = run_query(srv, spec)
+        same_rows(r["rows"], sv["rows"], len(spec["group"]), sabs)
     srv.close()

@@ -12,7 +12,7 @@ from craton_bolt_amd import datagen as D
 from craton_bolt_amd.device import DeviceBuffer
 from craton_bolt_amd.ops import execute_sort, sort_pairs_i64
 
-from gpu_util import both
+from gpu_util import both, dec_array
 
 pytestmark = pytest.mark.gpu
 
@@ -99,3 +99,20 @@ def test_order_by_float32_single_key(gpu):
     d, h = both(x, N.F32, None)
     assert execute_sort([d], [False]).tolist() == [4, 7, 2, 3, 1, 5, 0, 6]
     assert execute_sort([d], [True]).tolist() == [0, 6, 5, 1, 2, 3, 7, 4]
+
+
+@pytest.mark.parametrize("n", [1000, 70_000])
+def test_order_by_decimal128_keys(gpu, n):
+    """Decimal128 sort keys (signed 128-bit order) with NULLs, DESC and ASC,
+    ahead of an int32 tie-breaker, against the oracle's stable order."""
+    print(f"[case] order by decimal n={n}", flush=True)
+    rng = np.random.default_rng(n + 5)
+    base = [int(x) for x in rng.integers(-2 ** 40, 2 ** 40, n)]
+    vals = [v * (1 << 60) + int(rng.integers(0, 4)) if i % 3 else v for i, v in enumerate(base)]
+    vd = rng.random(n) > 0.05
+    t = rng.integers(0, 50, n).astype(np.int32)
+    cols = [both(dec_array(vals), N.DEC128, vd), both(t, N.I32, None)]
+    for desc, nf in [([True, False], [False, True]), ([False, False], [True, True])]:
+        got = execute_sort([x for x, _ in cols], desc, nf)
+        want = O.sort_indices([h for _, h in cols], desc, nf)
+        assert np.array_equal(got, want)

@@ -358,6 +358,22 @@ def test_sort_float_nan_last_and_signed_zero():
     assert got.tolist() == [0, 6, 5, 1, 2, 3, 7, 4]
 
 
+def test_sort_decimal128_orders_as_signed_128_bit():
+    """Decimal128 keys compare as i128: values straddling the 64-bit word
+    boundary and negatives, checked against Python's integer order (a dropped
+    high word, an unsigned high word or a signed low word each fail it)."""
+    rng = np.random.default_rng(37)
+    vals = [int(x) for x in rng.integers(-2 ** 62, 2 ** 62, 400)]
+    vals += [2 ** 64 - 1, 2 ** 64, 2 ** 64 + 1, -(2 ** 64), -(2 ** 64) - 1, -1, 0, 1, 2 ** 127 - 1, -(2 ** 127)]
+    vals += [v * 3 + (1 << 70) for v in vals[:50]]
+    m = (1 << 64) - 1
+    arr = np.array([[v & m, (v >> 64) & m] for v in vals], dtype=np.uint64).view(np.int64)
+    for desc in (False, True):
+        got = O.sort_indices([O.HostCol(arr, O.DEC128)], [desc], [True])
+        want = sorted(range(len(vals)), key=lambda i: -vals[i] if desc else vals[i])  # stable
+        assert got.tolist() == want
+
+
 def test_sort_nulls_first_last():
     x = np.array([3, 1, 2, 0], np.int64)
     valid = [1, 1, 1, 0]
