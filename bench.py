@@ -307,10 +307,11 @@ def bench_groupby(args, zipf: bool) -> dict:
     L.bolt_profile_events(None, None)
     ms, kern = float(np.mean(op_ms)), float(np.mean(k_ms))
     g = groups["g"]
+    roof = _roof(n * per_row, kern, "GROUP BY LDS-tier pass (keys + inputs read once)")
+    roof["traffic"] = _traffic("groupby-zipf" if zipf else "groupby", kern)
     return _line(n / (ms * 1e-3), "rows/s", args, ms, dtype,
                  {"workload": desc, "n": n, "groups": g, "operator_ms_on_stream": ms, "pass_a_kernel_ms": kern,
-                  "wall_ms_per_call": wall},
-                 _roof(n * per_row, kern, "GROUP BY LDS-tier pass (keys + inputs read once)"))
+                  "wall_ms_per_call": wall}, roof)
 
 
 def bench_calib(args) -> dict:
