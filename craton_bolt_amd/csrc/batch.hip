@@ -241,6 +241,8 @@ __global__ void bg_init_kernel(const BCtl* __restrict__ ctl, GPlan P, unsigned l
   }
 }
 
+constexpr int kBRows = 4;  // rows per lane per iteration of the aggregation pass
+
 __global__ __launch_bounds__(1024) void bg_agg_kernel(const BQuery* __restrict__ qs, int nq, long long items,
                                                       int tile_rows, const BPlan* __restrict__ plan,
                                                       BCtl* __restrict__ ctl, unsigned long long* __restrict__ keys,
@@ -284,34 +286,62 @@ __global__ __launch_bounds__(1024) void bg_agg_kernel(const BQuery* __restrict__
     const int64_t r0 = (it - Q.item0) * (int64_t)tile_rows;
     const int64_t r1 = min(Q.n, r0 + (int64_t)tile_rows);
     const uint32_t smask = (uint32_t)S - 1;
-    for (int64_t row = r0 + threadIdx.x; row < r1; row += nt) {
-      uint64_t key = 0;
-      const int kr = row_key(Q.ks, row, &key);
-      if (kr == 0) continue;
-      int slot = -1;
-      if (kr == 2) {
-        slot = S;
-        sused[0] = 1;
-      } else if (key == kEmpty) {
-        slot = S + 1;
-        sused[1] = 1;
-      } else {
-        uint32_t s = (uint32_t)(mix64(key) >> 32) & smask;
-        for (int p = 0; p < kLdsProbes; p++) {
-          unsigned long long cur = skeys[s];
-          if (cur == key) { slot = (int)s; break; }
-          if (cur == kEmpty) {
-            unsigned long long prev = atomicCAS(&skeys[s], kEmpty, (unsigned long long)key);
-            if (prev == kEmpty || prev == key) { slot = (int)s; break; }
-          }
-          s = (s + 1) & smask;
-        }
+    const int npf = P.n < 4 ? P.n : 4;
+    // kBRows rows per lane, nt apart: keys and the first aggregates' inputs
+    // of all of them are loaded, then their home slots read, before the
+    // first update, so a lane has several rows' HBM reads in flight
+    for (int64_t base = r0 + threadIdx.x; base < r1; base += (int64_t)nt * kBRows) {
+      uint64_t key[kBRows], pw0[kBRows], pw1[kBRows], pw2[kBRows], pw3[kBRows];
+      int kr[kBRows];
+#pragma unroll
+      for (int j = 0; j < kBRows; j++) {
+        const int64_t row = base + (int64_t)j * nt;
+        key[j] = 0;
+        kr[j] = row < r1 ? row_key(Q.ks, row, &key[j]) : 0;
+        const int64_t rc = row < r1 ? row : r0;
+        pw0[j] = npf > 0 ? pf_word(P.a[0], rc) : 0;
+        pw1[j] = npf > 1 ? pf_word(P.a[1], rc) : 0;
+        pw2[j] = npf > 2 ? pf_word(P.a[2], rc) : 0;
+        pw3[j] = npf > 3 ? pf_word(P.a[3], rc) : 0;
       }
-      if (slot >= 0) {
-        update_slot(P, swords, SS, (uint32_t)slot, row);
-      } else {
-        const int64_t g = gslot(t, 0, key);
-        if (g >= 0) update_slot(P, t.words, arena_slots, (uint32_t)g, row);
+      uint32_t home[kBRows];
+      unsigned long long cur[kBRows];
+#pragma unroll
+      for (int j = 0; j < kBRows; j++) {
+        home[j] = (uint32_t)(mix64(key[j]) >> 32) & smask;
+        cur[j] = skeys[home[j]];
+      }
+#pragma unroll
+      for (int j = 0; j < kBRows; j++) {
+        if (kr[j] == 0) continue;
+        const int64_t row = base + (int64_t)j * nt;
+        int slot = -1;
+        if (kr[j] == 2) {
+          slot = S;
+          sused[0] = 1;
+        } else if (key[j] == kEmpty) {
+          slot = S + 1;
+          sused[1] = 1;
+        } else if (cur[j] == key[j]) {
+          slot = (int)home[j];
+        } else {
+          uint32_t s = home[j];
+          for (int p = 0; p < kLdsProbes; p++) {
+            unsigned long long c = skeys[s];
+            if (c == key[j]) { slot = (int)s; break; }
+            if (c == kEmpty) {
+              unsigned long long prev = atomicCAS(&skeys[s], kEmpty, (unsigned long long)key[j]);
+              if (prev == kEmpty || prev == key[j]) { slot = (int)s; break; }
+            }
+            s = (s + 1) & smask;
+          }
+        }
+        if (slot >= 0) {
+          update_slot_pf(P, swords, SS, (uint32_t)slot, row, pw0[j], pw1[j], pw2[j], pw3[j]);
+        } else {
+          const int64_t g = gslot(t, 0, key[j]);
+          if (g >= 0) update_slot_pf(P, t.words, arena_slots, (uint32_t)g, row, pw0[j], pw1[j], pw2[j], pw3[j]);
+        }
       }
     }
     __syncthreads();

@@ -246,6 +246,129 @@ __global__ __launch_bounds__(kPartBlock) void part_scatter_kernel(PartSrc src, K
   }
 }
 
+// ---- LDS-staged scatter -------------------------------------------------------------
+//
+// A pass moves items into the buckets of a 6-bit digit of their partition
+// id: a tile of T items is ranked per digit with LDS atomics, staged in LDS in
+// digit order, and written out as runs (one reservation per digit and tile),
+// so a wave writes contiguous records instead of 32-byte fragments scattered
+// over every partition.  Two passes (the top digit, then the rest) reach up
+// to 4096 partitions; with at most 64 partitions one pass writes the final
+// layout.
+constexpr int kStageBlock = 512;
+constexpr int kStageMaxIpt = 4;  // items per thread per tile
+
+template <int PASS>  // 1: items -> buckets of the top digit; 2: records -> final partitions
+__global__ __launch_bounds__(kStageBlock) void part_stage_kernel(PartSrc src, KeySpec ks, GPlan P, int pbits, int fb,
+                                                                 const uint16_t* __restrict__ pid,
+                                                                 const unsigned long long* __restrict__ in,
+                                                                 unsigned long long* __restrict__ out, int stride, int T,
+                                                                 unsigned long long* __restrict__ cursor,
+                                                                 const uint64_t* __restrict__ start, int K2, GTable t) {
+  extern __shared__ unsigned long long stage[];  // [T][stride]
+  uint8_t* sdig = reinterpret_cast<uint8_t*>(stage + (size_t)T * stride);
+  __shared__ uint32_t cnt[64], pre[64];
+  __shared__ unsigned long long gbase[64];
+  const int ipt = T / kStageBlock;
+  const int W = P.words;
+  const uint32_t dmask = PASS == 1 ? 63u : ((1u << fb) - 1u);
+  // the range of items this workgroup tiles
+  int64_t lo, hi, tile0, tstep;
+  uint32_t c = 0;  // PASS 2: the top digit whose bucket this workgroup reads
+  if (PASS == 1) {
+    lo = 0;
+    hi = src.n_items;
+    tile0 = blockIdx.x;
+    tstep = gridDim.x;
+  } else {
+    c = blockIdx.x / K2;
+    lo = (int64_t)start[(uint64_t)c << fb];
+    hi = (int64_t)start[(uint64_t)(c + 1) << fb];
+    tile0 = blockIdx.x % K2;
+    tstep = K2;
+  }
+  for (int64_t tile = tile0; lo + tile * T < hi; tile += tstep) {
+    const int64_t t0 = lo + tile * T;
+    if (threadIdx.x < 64) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int rank[kStageMaxIpt];
+    uint32_t dig[kStageMaxIpt];
+#pragma unroll
+    for (int u = 0; u < kStageMaxIpt; u++) {
+      rank[u] = -1;
+      dig[u] = 0;
+      if (u >= ipt) continue;
+      const int64_t i = t0 + u * kStageBlock + threadIdx.x;
+      if (i >= hi) continue;
+      uint32_t p;
+      if (PASS == 1) {
+        p = pid[i];
+        if (p == 0xFFFFu) {  // tagged partial record: merged straight into the global table
+          const int64_t g = gslot(t, src.ptag[i], src.pkeys[i]);
+          if (g >= 0) merge_slot(P, t.words, t.cap + 2, (uint32_t)g, src.pwords + i, (size_t)src.pcap);
+          continue;
+        }
+        dig[u] = fb > 0 ? (p >> fb) : p;
+      } else {
+        p = part_of(in[(uint64_t)i * stride], pbits);
+        dig[u] = p & dmask;
+      }
+      rank[u] = (int)atomicAdd(&cnt[dig[u]], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {  // one wave: digit offsets within the tile, one reservation per digit
+      const uint32_t v = cnt[threadIdx.x];
+      uint32_t inc = v;
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(inc, d);
+        if ((int)threadIdx.x >= d) inc += o;
+      }
+      pre[threadIdx.x] = inc - v;
+      const uint64_t dest = PASS == 1 ? threadIdx.x : (((uint64_t)c << fb) | threadIdx.x);
+      gbase[threadIdx.x] = v ? atomicAdd(&cursor[dest], (unsigned long long)v) : 0ull;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kStageMaxIpt; u++) {
+      if (rank[u] < 0) continue;
+      const int64_t i = t0 + u * kStageBlock + threadIdx.x;
+      const uint32_t sl = pre[dig[u]] + (uint32_t)rank[u];
+      sdig[sl] = (uint8_t)dig[u];
+      unsigned long long* r = stage + (size_t)sl * stride;
+      if (PASS == 2) {
+        const unsigned long long* q = in + (uint64_t)i * stride;
+        for (int w = 0; w < stride; w++) r[w] = q[w];
+      } else if (i < src.n_parts) {
+        r[0] = src.pkeys[i];
+        for (int w = 0; w < W; w++) r[1 + w] = src.pwords[(size_t)w * src.pcap + i];
+      } else {
+        const int64_t row = (int64_t)src.rows[i - src.n_parts];
+        uint64_t key = 0;
+        (void)row_key(ks, row, &key);
+        r[0] = key;
+        row_words(P, row, r + 1, 1);
+      }
+    }
+    __syncthreads();
+    const uint32_t total = pre[63] + cnt[63];
+    for (uint32_t j = threadIdx.x; j < total; j += kStageBlock) {
+      const uint32_t d = sdig[j];
+      const uint64_t pos = gbase[d] + (j - pre[d]);
+      const uint4* a = reinterpret_cast<const uint4*>(stage + (size_t)j * stride);
+      uint4* b = reinterpret_cast<uint4*>(out + pos * (uint64_t)stride);
+      for (int w = 0; w < stride / 2; w++) b[w] = a[w];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void part_init_cursors_kernel(const uint64_t* __restrict__ start, unsigned long long* __restrict__ fine,
+                                         unsigned long long* __restrict__ coarse, int np, int fb) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < np) fine[p] = start[p];
+  if (fb > 0 && p < (np >> fb)) coarse[p] = start[(uint64_t)p << fb];
+}
+
 // Global slot of `key`, claiming an empty one; *fresh says whether this call
 // claimed it.  0xFFFFFFFF when the probe bound is exceeded.
 __device__ __forceinline__ uint32_t global_claim(unsigned long long* keys, uint64_t cap_mask, uint64_t key,
@@ -345,6 +468,8 @@ int gb_part_build(hipStream_t s, const KeySpec& ks, const GPlan& P, const unsign
   out->stride = (1 + P.words + 1) & ~1;  // key + words, rounded up to 16 bytes
   out->recs_cap = std::max<int64_t>(n_items, 1);
   BOLT_TRY(out->rec.alloc((size_t)out->recs_cap * out->stride, s));
+  if (pbits > 6) BOLT_TRY(out->coarse.alloc((size_t)out->recs_cap * out->stride, s));
+  BOLT_TRY(out->ccursor.alloc(64, s));
   out->src = src;
   out->nb = nb;
   out->chunk = chunk;
@@ -358,10 +483,36 @@ int gb_part_build(hipStream_t s, const KeySpec& ks, const GPlan& P, const unsign
 // it runs per table attempt, with the reduce.
 int gb_part_reduce(hipStream_t s, const GTable& t, const GPlan& P, GbPartitions& parts, int S) {
   PartRecs recs{parts.rec.ptr, parts.recs_cap, parts.stride};
-  // cursors back to the partition starts, then scatter
-  part_reset_cursor_kernel<<<(parts.np + 255) / 256, 256, 0, s>>>(parts.start.ptr, parts.cursor.ptr, parts.np);
-  part_scatter_kernel<<<parts.nb, kPartBlock, 0, s>>>(parts.src, parts.ks, P, parts.pbits, parts.chunk,
-                                                      parts.pid.ptr, parts.hist.ptr, parts.cursor.ptr, recs, t);
+  const int fb = parts.pbits > 6 ? parts.pbits - 6 : 0;  // bits of the second pass
+  int T = 2048;
+  while (T > kStageBlock && (size_t)T * parts.stride * 8 > 65536) T >>= 1;
+  if ((size_t)T * parts.stride * 8 <= 65536) {
+    // LDS-staged: one pass (<= 64 partitions) or two
+    const size_t lds = (size_t)T * parts.stride * 8 + T;
+    BOLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(part_stage_kernel<1>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    BOLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(part_stage_kernel<2>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    part_init_cursors_kernel<<<(parts.np + 255) / 256, 256, 0, s>>>(parts.start.ptr, parts.cursor.ptr,
+                                                                    parts.ccursor.ptr, parts.np, fb);
+    const int64_t tiles = (parts.src.n_items + T - 1) / T;
+    const unsigned g1 = (unsigned)std::max<int64_t>(1, std::min<int64_t>(1024, tiles));
+    part_stage_kernel<1><<<g1, kStageBlock, lds, s>>>(parts.src, parts.ks, P, parts.pbits, fb, parts.pid.ptr, nullptr,
+                                                      fb > 0 ? parts.coarse.ptr : parts.rec.ptr, parts.stride, T,
+                                                      fb > 0 ? parts.ccursor.ptr : parts.cursor.ptr,
+                                                      parts.start.ptr, 1, t);
+    if (fb > 0) {
+      const int K2 = 8;  // workgroups per top-digit bucket
+      part_stage_kernel<2><<<64 * K2, kStageBlock, lds, s>>>(parts.src, parts.ks, P, parts.pbits, fb, nullptr,
+                                                             parts.coarse.ptr, parts.rec.ptr, parts.stride, T,
+                                                             parts.cursor.ptr, parts.start.ptr, K2, t);
+    }
+  } else {
+    // records too wide to stage: scatter each item to its partition directly
+    part_reset_cursor_kernel<<<(parts.np + 255) / 256, 256, 0, s>>>(parts.start.ptr, parts.cursor.ptr, parts.np);
+    part_scatter_kernel<<<parts.nb, kPartBlock, 0, s>>>(parts.src, parts.ks, P, parts.pbits, parts.chunk,
+                                                        parts.pid.ptr, parts.hist.ptr, parts.cursor.ptr, recs, t);
+  }
   const size_t lds = (size_t)(S + 2) * 8 * (1 + P.words) + 16;
   BOLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(part_reduce_kernel),
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

@@ -124,6 +124,56 @@ __device__ __forceinline__ void update_slot(const GPlan& P, unsigned long long* 
   }
 }
 
+// The 4- or 8-byte input word of aggregate `g` at `row`, for loading ahead of
+// the slot lookup (0 for COUNT(*), Bool and Decimal128, which update_slot_pf
+// reads at update time).
+__device__ __forceinline__ uint64_t pf_word(const GAgg& g, int64_t row) {
+  if (g.op == BOLT_AGG_COUNT_STAR || g.data == nullptr) return 0;
+  switch (g.dtype) {
+    case BOLT_I64: case BOLT_F64: return static_cast<const uint64_t*>(g.data)[row];
+    case BOLT_I32: case BOLT_F32: case BOLT_DATE32: return static_cast<const uint32_t*>(g.data)[row];
+  }
+  return 0;
+}
+
+// update_slot with the input words of the first four aggregates already
+// loaded (pf_word); the same atomics, so results are those of update_slot.
+__device__ __forceinline__ void update_slot_pf(const GPlan& P, unsigned long long* words, size_t stride,
+                                               uint32_t slot, int64_t row, uint64_t w0, uint64_t w1, uint64_t w2,
+                                               uint64_t w3) {
+  for (int a = 0; a < P.n; a++) {
+    const GAgg& g = P.a[a];
+    if (a >= 4 || g.dtype == BOLT_DEC128 || g.dtype == BOLT_BOOL) {
+      GPlan one;
+      one.n = 1;
+      one.words = P.words;
+      one.a[0] = g;
+      update_slot(one, words, stride, slot, row);
+      continue;
+    }
+    const uint64_t rv = a == 0 ? w0 : a == 1 ? w1 : a == 2 ? w2 : w3;
+    unsigned long long* w = words + (size_t)g.w * stride + slot;
+    if (g.op == BOLT_AGG_COUNT_STAR) { atomicAdd(w, 1ull); continue; }
+    if (!bit_valid(g.validity, row)) continue;
+    if (g.kind == K_COUNT) { atomicAdd(w, 1ull); continue; }
+    if (g.nw >= 0) atomicAdd(words + (size_t)g.nw * stride + slot, 1ull);
+    const int64_t iv = g.dtype == BOLT_I64 ? (int64_t)rv : (int64_t)(int32_t)(uint32_t)rv;
+    const double fv = g.dtype == BOLT_F64 ? as_f64(rv)
+                    : g.dtype == BOLT_F32 ? (double)__uint_as_float((uint32_t)rv) : (double)iv;
+    switch (g.kind) {
+      case K_ISUM: atomic_add128(w, w + stride, (uint64_t)iv, (uint64_t)(iv >> 63)); break;
+      case K_FSUM: case K_AVG: case K_VAR: unsafeAtomicAdd(reinterpret_cast<double*>(w), fv); break;
+      case K_MINMAX: {
+        const uint64_t k = (g.dtype == BOLT_F64 || g.dtype == BOLT_F32) ? f64_order_key(fv)
+                                                                          : (uint64_t)iv ^ 0x8000000000000000ull;
+        if (g.op == BOLT_AGG_MIN) atomicMin(w, (unsigned long long)k);
+        else atomicMax(w, (unsigned long long)k);
+        break;
+      }
+    }
+  }
+}
+
 // Merge the words of a partial record (one value per field, `pstride` apart)
 // into a slot of the global table.
 __device__ __forceinline__ void merge_slot(const GPlan& P, unsigned long long* words, size_t stride, uint32_t slot,
@@ -305,7 +355,9 @@ struct GbPartitions {
   DevBuf<uint32_t> hist;
   DevBuf<unsigned long long> cursor;
   DevBuf<uint64_t> start;
-  DevBuf<unsigned long long> rec;
+  DevBuf<unsigned long long> rec;     // final layout: partition p at [start[p], start[p + 1])
+  DevBuf<unsigned long long> coarse;  // first staged pass: buckets of the top 6 bits
+  DevBuf<unsigned long long> ccursor; // [64] scatter cursors of the first pass
 };
 int gb_part_build(hipStream_t s, const KeySpec& ks, const GPlan& P, const unsigned long long* pkeys,
                   const uint8_t* ptag, const unsigned long long* pwords, int64_t pcap, int64_t n_parts,
