@@ -98,23 +98,35 @@ __device__ __forceinline__ u32_t gb_lds_home(u64_t key, int S) {
 }
 
 // Linear probe for `key` (never kEmpty) from slot `s`, claiming an empty slot
-// on the way; -1 if no slot turns up within `probes` slots.
-__device__ __forceinline__ int gb_lds_probe(u64_t* skeys, int S, u64_t key, u32_t s, int probes) {
+// on the way (counted in *occupied); -1 if no slot turns up within `probes`
+// slots.
+__device__ __forceinline__ int gb_lds_probe(u64_t* skeys, int S, u64_t key, u32_t s, int probes, u32_t* occupied) {
   for (int p = 0; p < probes; p++) {
     const u64_t cur = skeys[s];
     if (cur == key) return (int)s;
     if (cur == kEmpty) {
       const u64_t prev = atomicCAS(&skeys[s], kEmpty, key);
-      if (prev == kEmpty || prev == key) return (int)s;
+      if (prev == kEmpty) {
+        atomicAdd(occupied, 1u);
+        return (int)s;
+      }
+      if (prev == key) return (int)s;
     }
     s = (s + 1 == (u32_t)S) ? 0 : s + 1;
   }
   return -1;
 }
 
-// Slot of `key` within kLdsProbes slots of its home, or -1.
-__device__ __forceinline__ int gb_lds_slot(u64_t* skeys, int S, u64_t key) {
-  return gb_lds_probe(skeys, S, key, gb_lds_home(key, S), kLdsProbes);
+// Probe bound: kLdsProbes while the table has room; 2 once 15/16 of the
+// slots are taken, where a key not found near its home almost never finds a
+// slot further on, and a long walk before every spill is what costs.
+__device__ __forceinline__ int gb_lds_probes(int S, const u32_t* occupied) {
+  return *occupied >= (u32_t)(S - (S >> 4)) ? 2 : kLdsProbes;
+}
+
+// Slot of `key` near its home, or -1.
+__device__ __forceinline__ int gb_lds_slot(u64_t* skeys, int S, u64_t key, u32_t* occupied) {
+  return gb_lds_probe(skeys, S, key, gb_lds_home(key, S), gb_lds_probes(S, occupied), occupied);
 }
 
 // Append `row` to this workgroup's spill segment (called by the whole wave;

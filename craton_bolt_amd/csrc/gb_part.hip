@@ -399,13 +399,13 @@ __global__ __launch_bounds__(kGbBlock) void part_reduce_kernel(GTable t, GPlan P
     const unsigned long long id = word_identity(P, w);
     for (int i = threadIdx.x; i < SS; i += kGbBlock) swords[(size_t)w * SS + i] = id;
   }
-  if (threadIdx.x < 2) sused[threadIdx.x] = 0;
+  if (threadIdx.x < 3) sused[threadIdx.x] = 0;  // [2]: occupied slots
   __syncthreads();
   const int64_t r0 = (int64_t)start[blockIdx.x], r1 = (int64_t)start[blockIdx.x + 1];
   for (int64_t r = r0 + threadIdx.x; r < r1; r += kGbBlock) {
     const unsigned long long* rec = recs.rec + r * (uint64_t)recs.stride;
     const unsigned long long key = rec[0];
-    const int slot = gb_lds_slot(skeys, S, key);
+    const int slot = gb_lds_slot(skeys, S, key, &sused[2]);
     if (slot >= 0) {
       merge_slot(P, swords, SS, (uint32_t)slot, rec + 1, 1);
     } else {  // the partition holds more groups than the LDS table: straight to HBM

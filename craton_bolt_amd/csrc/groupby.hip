@@ -61,14 +61,14 @@ __global__ __launch_bounds__(kGbBlock) void gb_lds_kernel(KeySpec ks, DevFilter 
   unsigned long long* skeys = sm;
   unsigned long long* swords = sm + SS;
   uint32_t* shll = reinterpret_cast<uint32_t*>(swords + (size_t)P.words * SS);
-  uint32_t* sused = shll + kHllRegs;  // [2] specials used, [2] spill count
+  uint32_t* sused = shll + kHllRegs;  // [0..1] specials used, [2] spill count, [3] occupied slots
   for (int i = threadIdx.x; i < SS; i += kGbBlock) skeys[i] = kEmpty;
   for (int w = 0; w < P.words; w++) {
     unsigned long long id = word_identity(P, w);
     for (int i = threadIdx.x; i < SS; i += kGbBlock) swords[(size_t)w * SS + i] = id;
   }
   for (int i = threadIdx.x; i < kHllRegs; i += kGbBlock) shll[i] = 0;
-  if (threadIdx.x < 3) sused[threadIdx.x] = 0;
+  if (threadIdx.x < 4) sused[threadIdx.x] = 0;
   __syncthreads();
 
   const int64_t stride = (int64_t)gridDim.x * kGbBlock;
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(kGbBlock) void gb_lds_kernel(KeySpec ks, DevFilter 
           slot = S + 1;
           sused[1] = 1;
         } else {
-          slot = gb_lds_slot(skeys, S, key);
+          slot = gb_lds_slot(skeys, S, key, &sused[3]);
           spill = slot < 0;
         }
       }
@@ -136,13 +136,13 @@ __global__ __launch_bounds__(kGbBlock) void gb_merge_lds_kernel(GTable t, GPlan 
   const int SS = S + 2;
   unsigned long long* skeys = sm;
   unsigned long long* swords = sm + SS;
-  uint32_t* sused = reinterpret_cast<uint32_t*>(swords + (size_t)P.words * SS);
+  uint32_t* sused = reinterpret_cast<uint32_t*>(swords + (size_t)P.words * SS);  // [0..1] specials, [2] occupied
   for (int i = threadIdx.x; i < SS; i += kGbBlock) skeys[i] = kEmpty;
   for (int w = 0; w < P.words; w++) {
     const unsigned long long id = word_identity(P, w);
     for (int i = threadIdx.x; i < SS; i += kGbBlock) swords[(size_t)w * SS + i] = id;
   }
-  if (threadIdx.x < 2) sused[threadIdx.x] = 0;
+  if (threadIdx.x < 3) sused[threadIdx.x] = 0;
   __syncthreads();
   const int64_t stride = (int64_t)gridDim.x * kGbBlock;
   for (int64_t r = (int64_t)blockIdx.x * kGbBlock + threadIdx.x; r < np; r += stride) {
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(kGbBlock) void gb_merge_lds_kernel(GTable t, GPlan 
     int slot;
     if (tag == 1) { slot = S; sused[0] = 1; }
     else if (tag == 2) { slot = S + 1; sused[1] = 1; }
-    else slot = gb_lds_slot(skeys, S, key);
+    else slot = gb_lds_slot(skeys, S, key, &sused[2]);
     if (slot >= 0) {
       merge_slot(P, swords, SS, (uint32_t)slot, pwords + r, (size_t)pcap);
     } else {

@@ -365,7 +365,7 @@ std::string jit_gb_lds_source(const GbJitSpec& sp) {
   for (int w = 0; w < W; w++) o << "    swords[" << w << " * SS + i] = " << sp.ident[w] << "ull;\n";
   o << "  }\n"
        "  for (int i = threadIdx.x; i < kHllRegs; i += 1024) shll[i] = 0;\n"
-       "  if (threadIdx.x < 3) sused[threadIdx.x] = 0;\n"
+       "  if (threadIdx.x < 4) sused[threadIdx.x] = 0;\n"
        "  __syncthreads();\n";
   auto kty = [](int dt) { return dtype_size(dt) == 4 ? "u32_t" : "u64_t"; };
   auto kcanon = [](int dt, const std::string& v) -> std::string {
@@ -439,7 +439,8 @@ std::string jit_gb_lds_source(const GbJitSpec& sp) {
          "        else {\n"
          "          const bool taken = cur" << rs << " != kEmpty;\n"
          "          const u32_t from = taken ? (home" << rs << " + 1 == (u32_t)S ? 0u : home" << rs << " + 1) : home" << rs << ";\n"
-         "          slot = gb_lds_probe(skeys, S, key" << rs << ", from, taken ? kLdsProbes - 1 : kLdsProbes);\n"
+         "          const int lim = gb_lds_probes(S, &sused[3]);\n"
+         "          slot = gb_lds_probe(skeys, S, key" << rs << ", from, taken ? lim - 1 : lim, &sused[3]);\n"
          "          spill = slot < 0;\n"
          "        }\n"
          "      }\n"
