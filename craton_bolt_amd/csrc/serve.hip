@@ -145,6 +145,60 @@ __device__ __forceinline__ void pre_image(const Raw& r, int dt, uint64_t* h, uin
   value_image(dt, vi, vhi, vf, h, l);
 }
 
+// Images of preloaded slot C for the kFilterRows rows rc[] into h[] / l[]
+// (one uniform dtype switch per slot and iteration; branch-free per row).
+// A macro rather than a function so that the arrays stay in registers.
+#define BOLT_PRELOAD_IMAGES(C, h, l)                                                                         \
+  do {                                                                                                       \
+    const void* d_ = pre.d[C];                                                                               \
+    switch (pre.dt[C]) {                                                                                     \
+      case BOLT_I32: case BOLT_DATE32:                                                                       \
+        _Pragma("unroll") for (int r = 0; r < kFilterRows; r++) {                                            \
+          const int32_t v_ = (int32_t)__builtin_nontemporal_load(static_cast<const uint32_t*>(d_) + rc[r]); \
+          h[r] = 0;                                                                                          \
+          l[r] = (uint64_t)(int64_t)v_ ^ 0x8000000000000000ull;                                              \
+        }                                                                                                    \
+        break;                                                                                               \
+      case BOLT_I64:                                                                                         \
+        _Pragma("unroll") for (int r = 0; r < kFilterRows; r++) {                                            \
+          h[r] = 0;                                                                                          \
+          l[r] = __builtin_nontemporal_load(static_cast<const uint64_t*>(d_) + rc[r]) ^ 0x8000000000000000ull; \
+        }                                                                                                    \
+        break;                                                                                               \
+      case BOLT_F32:                                                                                         \
+        _Pragma("unroll") for (int r = 0; r < kFilterRows; r++) {                                            \
+          const double v_ =                                                                                  \
+              (double)__uint_as_float(__builtin_nontemporal_load(static_cast<const uint32_t*>(d_) + rc[r])); \
+          h[r] = 0;                                                                                          \
+          l[r] = f64_order_key(v_ == 0.0 ? 0.0 : v_);                                                        \
+        }                                                                                                    \
+        break;                                                                                               \
+      case BOLT_F64:                                                                                         \
+        _Pragma("unroll") for (int r = 0; r < kFilterRows; r++) {                                            \
+          const double v_ = as_f64(__builtin_nontemporal_load(static_cast<const uint64_t*>(d_) + rc[r]));    \
+          h[r] = 0;                                                                                          \
+          l[r] = f64_order_key(v_ == 0.0 ? 0.0 : v_);                                                        \
+        }                                                                                                    \
+        break;                                                                                               \
+      case BOLT_DEC128:                                                                                      \
+        _Pragma("unroll") for (int r = 0; r < kFilterRows; r++) {                                            \
+          const uint64_t* w_ = static_cast<const uint64_t*>(d_) + 2 * rc[r];                                 \
+          l[r] = __builtin_nontemporal_load(w_);                                                             \
+          h[r] = __builtin_nontemporal_load(w_ + 1) ^ 0x8000000000000000ull;                                 \
+        }                                                                                                    \
+        break;                                                                                               \
+      default:                                                                                               \
+        _Pragma("unroll") for (int r = 0; r < kFilterRows; r++) {                                            \
+          h[r] = 0;                                                                                          \
+          l[r] = (uint64_t)static_cast<const uint8_t*>(d_)[rc[r]] ^ 0x8000000000000000ull;                   \
+        }                                                                                                    \
+        break;                                                                                               \
+    }                                                                                                        \
+    if (pre.v[C]) {                                                                                          \
+      _Pragma("unroll") for (int r = 0; r < kFilterRows; r++) if (!bit_valid(pre.v[C], rc[r])) vm[r] &= ~(1u << C); \
+    }                                                                                                        \
+  } while (0)
+
 __device__ __forceinline__ bool in_interval(uint64_t h, uint64_t l, uint64_t lh, uint64_t ll, uint64_t hh,
                                             uint64_t hl) {
   const bool ge = h > lh || (h == lh && l >= ll);
@@ -163,7 +217,8 @@ __global__ __launch_bounds__(kServeBlock) void serve_filter_kernel(const SQ* __r
   const int64_t w1 = min(r1, w0 + per_w);
   uint32_t run = 0;  // lane q: rows appended to segment (q, gw)
   for (int64_t base = w0; base < w1; base += 64 * kFilterRows) {  // wave-uniform trip count
-    // each preloaded value's image, computed once per row for every query
+    // each preloaded value's image, computed once per row for every query;
+    // the dtype switch is taken once per column and iteration, not per row
     uint64_t h0[kFilterRows], l0[kFilterRows], h1[kFilterRows], l1[kFilterRows];
     uint64_t h2[kFilterRows], l2[kFilterRows], h3[kFilterRows], l3[kFilterRows];
     uint32_t vm[kFilterRows];  // bit c: preloaded slot c is non-NULL
@@ -176,23 +231,11 @@ __global__ __launch_bounds__(kServeBlock) void serve_filter_kernel(const SQ* __r
       rc[r] = in[r] ? row : w1 - 1;  // clamped: every load is in range
       h0[r] = l0[r] = h1[r] = l1[r] = h2[r] = l2[r] = h3[r] = l3[r] = 0;
       vm[r] = 0xFu;
-      if (pre.n > 0) {
-        pre_image(load_raw(pre.d[0], pre.dt[0], rc[r]), pre.dt[0], &h0[r], &l0[r]);
-        if (!bit_valid(pre.v[0], rc[r])) vm[r] &= ~1u;
-      }
-      if (pre.n > 1) {
-        pre_image(load_raw(pre.d[1], pre.dt[1], rc[r]), pre.dt[1], &h1[r], &l1[r]);
-        if (!bit_valid(pre.v[1], rc[r])) vm[r] &= ~2u;
-      }
-      if (pre.n > 2) {
-        pre_image(load_raw(pre.d[2], pre.dt[2], rc[r]), pre.dt[2], &h2[r], &l2[r]);
-        if (!bit_valid(pre.v[2], rc[r])) vm[r] &= ~4u;
-      }
-      if (pre.n > 3) {
-        pre_image(load_raw(pre.d[3], pre.dt[3], rc[r]), pre.dt[3], &h3[r], &l3[r]);
-        if (!bit_valid(pre.v[3], rc[r])) vm[r] &= ~8u;
-      }
     }
+    if (pre.n > 0) BOLT_PRELOAD_IMAGES(0, h0, l0);
+    if (pre.n > 1) BOLT_PRELOAD_IMAGES(1, h1, l1);
+    if (pre.n > 2) BOLT_PRELOAD_IMAGES(2, h2, l2);
+    if (pre.n > 3) BOLT_PRELOAD_IMAGES(3, h3, l3);
     for (int qi = 0; qi < nq; qi++) {
       const SQ& Q = qs[qi];
       uint32_t* seg = lists + ((uint64_t)qi * kServeAllWaves + gw) * (uint64_t)per_w;
