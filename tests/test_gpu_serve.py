@@ -148,6 +148,11 @@ def test_serve_predicate_forms(star):
         [Term(QTY, N.GT, 30.0), Term(DATE, N.NE, d0)],
         [Term(PRICE, N.IS_NOT_NULL), Term(PRICE, N.GT, 900_000)],
         [Term(QTY, N.GT, 40.0), Term(QTY, N.LT, 20.0)],
+        # four preloaded columns (the Int32 image of fk1 too)
+        [Term(FK1, N.LT, 2500), Term(DATE, N.GE, d0), Term(PRICE, N.GE, 200_000), Term(QTY, N.LE, 45.0)],
+        # a fifth predicate column across the mix (the earlier queries preload four):
+        # fk2 is read per term, term-by-term evaluation
+        [Term(FK2, N.GE, 100), Term(DATE, N.LT, d0 + 900), Term(QTY, N.GT, 2.0)],
     ]
     srv = make_server(star, slots=8, chunk_rows=100_000)
     tickets = [srv.submit(srv.query(terms=t, **base)) for t in cases]
