@@ -275,7 +275,7 @@ __global__ __launch_bounds__(1024) void bg_agg_kernel(const BQuery* __restrict__
       const unsigned long long id = word_identity(P, w);
       for (int i = threadIdx.x; i < SS; i += nt) swords[(size_t)w * SS + i] = id;
     }
-    if (threadIdx.x < 2) sused[threadIdx.x] = 0;
+    if (threadIdx.x < 3) sused[threadIdx.x] = 0;  // [0..1] specials used, [2] occupied slots
     __syncthreads();
     GTable t;
     t.keys = keys + pl.off;
@@ -325,16 +325,9 @@ __global__ __launch_bounds__(1024) void bg_agg_kernel(const BQuery* __restrict__
         } else if (cur[j] == key[j]) {
           slot = (int)home[j];
         } else {
-          uint32_t s = home[j];
-          for (int p = 0; p < kLdsProbes; p++) {
-            unsigned long long c = skeys[s];
-            if (c == key[j]) { slot = (int)s; break; }
-            if (c == kEmpty) {
-              unsigned long long prev = atomicCAS(&skeys[s], kEmpty, (unsigned long long)key[j]);
-              if (prev == kEmpty || prev == key[j]) { slot = (int)s; break; }
-            }
-            s = (s + 1) & smask;
-          }
+          // the shared probe (power-of-two S here): kLdsProbes slots while the
+          // table has room, 2 once 15/16 of it is taken
+          slot = gb_lds_probe(skeys, S, key[j], home[j], gb_lds_probes(S, &sused[2]), &sused[2]);
         }
         if (slot >= 0) {
           update_slot_pf(P, swords, SS, (uint32_t)slot, row, pw0[j], pw1[j], pw2[j], pw3[j]);
