@@ -20,6 +20,7 @@
 // 16M x 256M probe, load 0.25 (1 GiB) costs 0.5 ms more per probe than the
 // 0.16 ms it saves in the build.
 #include <cstdlib>
+#include <cstring>
 
 #include "jit.h"
 #include "join_dev.h"
@@ -147,16 +148,25 @@ __global__ __launch_bounds__(kProbeBlock) void join_probe_agg_kernel(
 // pairs come out in probe-row order.  With unique build keys one walk finds each row's
 // match and keeps it in a register; with duplicates the chain is walked
 // twice (count, then emit).
+//
+// kPreserve (LEFT / RIGHT / FULL: the probe side is the preserved side) emits
+// one (probe_row, kNullRow) pair for a probe row that finds no match, in its
+// place in probe-row order, as the reference's host outer join does
+// (src/exec/join.rs:396-437).  A non-null `unmatched` (FULL) has one byte per
+// build row, 1 on entry; every build row that matches is cleared to 0 with a
+// plain byte store (all writers store the same value).
 constexpr int kEmitItems = 8;
 constexpr int kEmitTile = kProbeBlock * kEmitItems;  // 2048 rows
+constexpr uint32_t kNullRow = 0xFFFFFFFFu;
 
-template <typename KeyT>
+template <typename KeyT, bool kPreserve>
 __global__ __launch_bounds__(kProbeBlock) void join_probe_emit_kernel(
     const uint4* __restrict__ tbl, uint64_t mask, const uint32_t* __restrict__ nonunique,
     const KeyT* __restrict__ keys, const uint8_t* __restrict__ kvalid, int64_t n, DevFilter f,
     uint64_t* __restrict__ status, uint32_t* __restrict__ ticket, uint32_t* __restrict__ err,
     unsigned long long* __restrict__ total_out, int64_t capacity, uint32_t* __restrict__ out_probe,
-    uint32_t* __restrict__ out_build, const uint4* __restrict__ bloom, uint64_t bmask) {
+    uint32_t* __restrict__ out_build, const uint4* __restrict__ bloom, uint64_t bmask,
+    uint8_t* __restrict__ unmatched) {
   __shared__ uint32_t sh_tile;
   __shared__ uint32_t wsum[kProbeBlock / 64];
   __shared__ unsigned long long sh_base;
@@ -167,19 +177,26 @@ __global__ __launch_bounds__(kProbeBlock) void join_probe_emit_kernel(
   const int64_t tbase = tile * kEmitTile;
   const bool first_only = (*nonunique == 0u);
   uint32_t m[kEmitItems], bv[kEmitItems];
-  uint32_t mine = 0;
+  uint32_t mine = 0, pad = 0;
 #pragma unroll
   for (int j = 0; j < kEmitItems; j++) {
     const int64_t i = tbase + ((int64_t)threadIdx.x * kEmitItems + j);
     m[j] = 0;
     bv[j] = 0;
-    if (i < n && bit_valid(kvalid, i) && eval_filter(f, i)) {
+    const bool live = i < n && eval_filter(f, i);
+    if (live && bit_valid(kvalid, i)) {
       const int64_t k = (int64_t)keys[i];
       if (bloom_maybe(bloom, bmask, k))
         probe_chain(tbl, mask, k, first_only, [&](uint32_t w) {
           m[j]++;
           bv[j] = w;
+          if (unmatched) unmatched[w] = 0;
         });
+    }
+    if (kPreserve && live && m[j] == 0) {
+      m[j] = 1;
+      bv[j] = kNullRow;
+      pad |= 1u << j;
     }
     mine += m[j];
   }
@@ -208,7 +225,7 @@ __global__ __launch_bounds__(kProbeBlock) void join_probe_emit_kernel(
   for (int j = 0; j < kEmitItems; j++) {
     if (!m[j]) continue;
     const int64_t i = tbase + ((int64_t)threadIdx.x * kEmitItems + j);
-    if (first_only) {
+    if (first_only || (pad >> j) & 1u) {
       if ((int64_t)pos < capacity) {
         out_probe[pos] = (uint32_t)i;
         out_build[pos] = bv[j];
@@ -222,6 +239,85 @@ __global__ __launch_bounds__(kProbeBlock) void join_probe_emit_kernel(
         }
         pos++;
       });
+    }
+  }
+}
+
+// ---- join keys that are not a plain Int32 / Int64 column -----------------------
+//
+// One int64 per row, equal exactly when the SQL keys are equal
+// (encode_keys_for_shape, src/exec/gpu_join.rs:719): two 32-bit columns pack
+// first-high / second-low (TwoI32, :788); floats compare by their canonical
+// bits, -0.0 as +0.0 and every NaN as one NaN (canonical_f32_bits /
+// canonical_f64_bits, :682-705); Bool as 0 / 1.  A row with a NULL in any key
+// column is NULL (src/exec/join.rs:1045).  The validity bitmap is written a
+// 64-row word at a time from a ballot (`vout` is 8-byte aligned; NULL when no
+// key column has one).
+struct KeyCols {
+  const void* d[2];
+  const uint8_t* v[2];
+  int dt[2];
+  int n_cols;
+};
+
+__device__ __forceinline__ uint64_t key_word(const void* d, int dt, int64_t i) {
+  switch (dt) {
+    case BOLT_I64: return (uint64_t) static_cast<const int64_t*>(d)[i];
+    case BOLT_F64: {
+      const uint64_t b = static_cast<const uint64_t*>(d)[i];
+      if ((b & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull) return 0x7FF8000000000000ull;
+      return (b & 0x7FFFFFFFFFFFFFFFull) == 0 ? 0ull : b;
+    }
+    case BOLT_F32: {
+      const uint32_t b = static_cast<const uint32_t*>(d)[i];
+      if ((b & 0x7FFFFFFFu) > 0x7F800000u) return 0x7FC00000ull;
+      return (b & 0x7FFFFFFFu) == 0 ? 0ull : (uint64_t)b;
+    }
+    case BOLT_BOOL: return static_cast<const uint8_t*>(d)[i] != 0 ? 1ull : 0ull;
+    default: return (uint64_t)(int64_t) static_cast<const int32_t*>(d)[i];  // Int32 / Date32
+  }
+}
+
+__global__ __launch_bounds__(256) void join_key_kernel(KeyCols kc, int64_t n, int64_t* __restrict__ out,
+                                                       uint64_t* __restrict__ vout) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t end = (n + 63) / 64 * 64;  // wave-uniform trip count for the ballot
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < end; i += stride) {
+    const bool in = i < n;
+    bool valid = in;
+    uint64_t k = 0;
+    if (in) {
+      if (kc.n_cols == 1) {
+        k = key_word(kc.d[0], kc.dt[0], i);
+        valid = bit_valid(kc.v[0], i);
+      } else {
+        k = ((uint64_t) static_cast<const uint32_t*>(kc.d[0])[i] << 32) | static_cast<const uint32_t*>(kc.d[1])[i];
+        valid = bit_valid(kc.v[0], i) && bit_valid(kc.v[1], i);
+      }
+      out[i] = (int64_t)k;
+    }
+    if (vout) {
+      const uint64_t b = __ballot(in && valid);
+      if ((threadIdx.x & 63) == 0) vout[i >> 6] = b;
+    }
+  }
+}
+
+// FULL OUTER's second part: build rows no probe row matched, ascending, each
+// as (kNullRow, build_row) at base + its rank among them
+// (src/exec/join.rs:440-449).
+__global__ __launch_bounds__(256) void join_unmatched_kernel(const uint8_t* __restrict__ unmatched,
+                                                             const uint32_t* __restrict__ rank, int64_t n,
+                                                             int64_t base, int64_t capacity,
+                                                             uint32_t* __restrict__ out_probe,
+                                                             uint32_t* __restrict__ out_build) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    if (!unmatched[i]) continue;
+    const int64_t pos = base + (int64_t)rank[i];
+    if (pos < capacity) {
+      out_probe[pos] = kNullRow;
+      out_build[pos] = (uint32_t)i;
     }
   }
 }
@@ -480,6 +576,49 @@ extern "C" int bolt_join_probe_aggregate(bolt_stream_t stream, const bolt_join_t
   return BOLT_OK;
 }
 
+// Pair-emitting probe of `key` (rows passing `f`) into (out_probe, out_build);
+// *total receives the full pair count, also past `capacity`.  `preserve` and
+// `unmatched` as for join_probe_emit_kernel.  Synchronises.
+static int probe_emit(hipStream_t s, const bolt_join_table_t* t, const bolt_column_t& key, const DevFilter& f,
+                      bool preserve, uint8_t* unmatched, uint32_t* out_probe, uint32_t* out_build,
+                      int64_t capacity, int64_t* total_out) {
+  DevBuf<unsigned long long> counter;
+  DevBuf<uint64_t> status;
+  DevBuf<uint32_t> misc;  // [0] ticket, [1] look-back error
+  BOLT_TRY(counter.alloc(1, s));
+  BOLT_TRY(fill_u32(counter.ptr, 0, 2, s));
+  BOLT_TRY(misc.alloc(4, s));
+  BOLT_TRY(fill_u32(misc.ptr, 0, 4, s));
+  if (key.length > 0) {
+    const int64_t tiles = (key.length + kEmitTile - 1) / kEmitTile;
+    BOLT_TRY(status.alloc((size_t)tiles, s));
+    BOLT_TRY(fill_u32(status.ptr, 0, (size_t)tiles * 2, s));
+#define EMIT_LAUNCH(KT, PRESERVE)                                                                                \
+  join_probe_emit_kernel<KT, PRESERVE><<<(unsigned)tiles, kProbeBlock, 0, s>>>(                                  \
+      t->tbl, t->mask, t->flags, static_cast<const KT*>(key.data), key.validity, key.length, f, status.ptr,      \
+      misc.ptr, misc.ptr + 1, counter.ptr, capacity, out_probe, out_build, t->bloom, t->bloom_mask, unmatched)
+    if (key.dtype == BOLT_I64) {
+      if (preserve) EMIT_LAUNCH(int64_t, true);
+      else EMIT_LAUNCH(int64_t, false);
+    } else {
+      if (preserve) EMIT_LAUNCH(int32_t, true);
+      else EMIT_LAUNCH(int32_t, false);
+    }
+#undef EMIT_LAUNCH
+    BOLT_HIP(hipGetLastError());
+  }
+  unsigned long long total = 0;
+  uint32_t fl[2], hm[2];
+  BOLT_HIP(hipMemcpyAsync(&total, counter.ptr, 8, hipMemcpyDeviceToHost, s));
+  BOLT_HIP(hipMemcpyAsync(fl, t->flags, 8, hipMemcpyDeviceToHost, s));
+  BOLT_HIP(hipMemcpyAsync(hm, misc.ptr, 8, hipMemcpyDeviceToHost, s));
+  BOLT_HIP(hipStreamSynchronize(s));
+  if (fl[1] != 0) return set_error(BOLT_ERR_OTHER, "hash join build lost %u rows (table full)", fl[1]);
+  if (hm[1] != 0) return set_error(BOLT_ERR_OTHER, "hash join probe: output offset look-back timed out");
+  *total_out = (int64_t)total;
+  return BOLT_OK;
+}
+
 extern "C" int bolt_join_probe(bolt_stream_t stream, const bolt_join_table_t* t, const bolt_column_t* probe_cols,
                                int32_t n_probe_cols, int32_t key_col, const bolt_filter_t* filter,
                                uint32_t* out_probe_rows, uint32_t* out_build_rows, int64_t capacity,
@@ -494,38 +633,11 @@ extern "C" int bolt_join_probe(bolt_stream_t stream, const bolt_join_table_t* t,
   const bolt_column_t& key = probe_cols[key_col];
   DevFilter f;
   BOLT_TRY(make_dev_filter(filter, probe_cols, n_probe_cols, key.length, &f));
-  DevBuf<unsigned long long> counter;
-  DevBuf<uint64_t> status;
-  DevBuf<uint32_t> misc;  // [0] ticket, [1] look-back error
-  BOLT_TRY(counter.alloc(1, s));
-  BOLT_TRY(fill_u32(counter.ptr, 0, 2, s));
-  BOLT_TRY(misc.alloc(4, s));
-  BOLT_TRY(fill_u32(misc.ptr, 0, 4, s));
-  if (key.length > 0) {
-    const int64_t tiles = (key.length + kEmitTile - 1) / kEmitTile;
-    BOLT_TRY(status.alloc((size_t)tiles, s));
-    BOLT_TRY(fill_u32(status.ptr, 0, (size_t)tiles * 2, s));
-    if (key.dtype == BOLT_I64)
-      join_probe_emit_kernel<int64_t><<<(unsigned)tiles, kProbeBlock, 0, s>>>(
-          t->tbl, t->mask, t->flags, static_cast<const int64_t*>(key.data), key.validity, key.length, f, status.ptr,
-          misc.ptr, misc.ptr + 1, counter.ptr, capacity, out_probe_rows, out_build_rows, t->bloom, t->bloom_mask);
-    else
-      join_probe_emit_kernel<int32_t><<<(unsigned)tiles, kProbeBlock, 0, s>>>(
-          t->tbl, t->mask, t->flags, static_cast<const int32_t*>(key.data), key.validity, key.length, f, status.ptr,
-          misc.ptr, misc.ptr + 1, counter.ptr, capacity, out_probe_rows, out_build_rows, t->bloom, t->bloom_mask);
-    BOLT_HIP(hipGetLastError());
-  }
-  unsigned long long total = 0;
-  uint32_t fl[2], hm[2];
-  BOLT_HIP(hipMemcpyAsync(&total, counter.ptr, 8, hipMemcpyDeviceToHost, s));
-  BOLT_HIP(hipMemcpyAsync(fl, t->flags, 8, hipMemcpyDeviceToHost, s));
-  BOLT_HIP(hipMemcpyAsync(hm, misc.ptr, 8, hipMemcpyDeviceToHost, s));
-  BOLT_HIP(hipStreamSynchronize(s));
-  if (fl[1] != 0) return set_error(BOLT_ERR_OTHER, "hash join build lost %u rows (table full)", fl[1]);
-  if (hm[1] != 0) return set_error(BOLT_ERR_OTHER, "hash join probe: output offset look-back timed out");
-  *n_out = (int64_t)total;
-  if ((int64_t)total > capacity)
-    return set_error(BOLT_DECLINE, "hash join: %llu matches exceed the output capacity %lld", total,
+  int64_t total = 0;
+  BOLT_TRY(probe_emit(s, t, key, f, false, nullptr, out_probe_rows, out_build_rows, capacity, &total));
+  *n_out = total;
+  if (total > capacity)
+    return set_error(BOLT_DECLINE, "hash join: %lld matches exceed the output capacity %lld", (long long)total,
                      (long long)capacity);
   return BOLT_OK;
 }
@@ -548,4 +660,145 @@ extern "C" int bolt_inner_join(bolt_stream_t stream, const bolt_column_t* left_k
                          : bolt_join_probe(stream, t, p, 1, 0, nullptr, out_left_rows, out_right_rows, capacity, n_out);
   bolt_join_table_release(t);
   return st;
+}
+
+// Key columns of one side as a probe / build key: a plain Int32 / Int64 /
+// Date32 column is used in place; anything else is encoded into `enc` /
+// `enc_valid` by join_key_kernel.
+static int side_key(hipStream_t s, const bolt_column_t* cols, int32_t n_keys, DevBuf<int64_t>& enc,
+                    DevBuf<uint64_t>& enc_valid, bolt_column_t* key) {
+  const int64_t n = cols[0].length;
+  const int dt = cols[0].dtype;
+  if (n_keys == 1 && (dt == BOLT_I32 || dt == BOLT_I64 || dt == BOLT_DATE32)) {
+    *key = cols[0];
+    return BOLT_OK;
+  }
+  KeyCols kc;
+  memset(&kc, 0, sizeof kc);
+  kc.n_cols = n_keys;
+  bool any_nulls = false;
+  for (int c = 0; c < n_keys; c++) {
+    kc.d[c] = cols[c].data;
+    kc.v[c] = cols[c].validity;
+    kc.dt[c] = cols[c].dtype;
+    any_nulls |= cols[c].validity != nullptr;
+  }
+  BOLT_TRY(enc.alloc((size_t)(n > 0 ? n : 1), s));
+  if (any_nulls) BOLT_TRY(enc_valid.alloc((size_t)((n + 63) / 64 + 1), s));
+  if (n > 0) {
+    join_key_kernel<<<grid_for(n, 256, 4), 256, 0, s>>>(kc, n, enc.ptr, any_nulls ? enc_valid.ptr : nullptr);
+    BOLT_HIP(hipGetLastError());
+  }
+  memset(key, 0, sizeof *key);
+  key->data = enc.ptr;
+  key->validity = any_nulls ? reinterpret_cast<const uint8_t*>(enc_valid.ptr) : nullptr;
+  key->length = n;
+  key->dtype = BOLT_I64;
+  return BOLT_OK;
+}
+
+static int check_join_side(const bolt_column_t* cols, int32_t n_keys, const char* side) {
+  if (cols == nullptr) return set_error(BOLT_ERR_PLAN, "bolt_join: %s key columns are NULL", side);
+  for (int c = 0; c < n_keys; c++) {
+    const bolt_column_t& k = cols[c];
+    if (k.length != cols[0].length)
+      return set_error(BOLT_ERR_PLAN, "bolt_join: %s key column %d has %lld rows, column 0 has %lld", side, c,
+                       (long long)k.length, (long long)cols[0].length);
+    if (k.length < 0 || k.length >= (int64_t)kNullRow)
+      return set_error(BOLT_ERR_PLAN, "bolt_join: %s side of %lld rows exceeds the u32 row-id limit", side,
+                       (long long)k.length);
+    if (k.length > 0 && k.data == nullptr) return set_error(BOLT_ERR_PLAN, "bolt_join: %s key column %d has no data", side, c);
+  }
+  return BOLT_OK;
+}
+
+extern "C" int bolt_join(bolt_stream_t stream, const bolt_column_t* left_keys, const bolt_column_t* right_keys,
+                         int32_t n_keys, int32_t join_type, uint32_t* out_left_rows, uint32_t* out_right_rows,
+                         int64_t capacity, int64_t* n_out) {
+  BOLT_CHECK_DEVICE();
+  clear_error();
+  hipStream_t s = (hipStream_t)stream;
+  *n_out = 0;
+  if (join_type < BOLT_JOIN_INNER || join_type > BOLT_JOIN_FULL)
+    return set_error(BOLT_ERR_PLAN, "bolt_join: unknown join type %d", join_type);
+  if (n_keys < 1)
+    return set_error(BOLT_ERR_PLAN, "bolt_join: at least one equi-join key is required (src/exec/join.rs:290)");
+  BOLT_TRY(check_join_side(left_keys, n_keys, "left"));
+  BOLT_TRY(check_join_side(right_keys, n_keys, "right"));
+  for (int c = 0; c < n_keys; c++) {
+    const int dl = left_keys[c].dtype, dr = right_keys[c].dtype;
+    if (dl != dr)
+      return set_error(BOLT_ERR_TYPE, "bolt_join: key %d is %s on the left and %s on the right", c, dtype_name(dl),
+                       dtype_name(dr));
+    const bool ok1 = dl == BOLT_I32 || dl == BOLT_I64 || dl == BOLT_DATE32 || dl == BOLT_F32 || dl == BOLT_F64 ||
+                     dl == BOLT_BOOL;
+    const bool ok2 = dl == BOLT_I32 || dl == BOLT_DATE32;
+    if (n_keys == 1 && !ok1)
+      return set_error(BOLT_ERR_TYPE, "bolt_join: join key dtype %s not supported", dtype_name(dl));
+    if (n_keys > 1 && !ok2)
+      return set_error(BOLT_ERR_UNSUPPORTED,
+                       "bolt_join: multi-column keys pack two 32-bit integer columns (key %d is %s)", c,
+                       dtype_name(dl));
+  }
+  if (n_keys > 2)
+    return set_error(BOLT_ERR_UNSUPPORTED, "bolt_join: %d key columns (two 32-bit columns pack into one key)",
+                     n_keys);
+  const int64_t nl = left_keys[0].length, nr = right_keys[0].length;
+  // Build side: the smaller for INNER (src/exec/join.rs:192); the side that is
+  // not preserved for LEFT / RIGHT, the right for FULL (join.rs:305).
+  bool build_is_left;
+  switch (join_type) {
+    case BOLT_JOIN_INNER: build_is_left = nl <= nr; break;
+    case BOLT_JOIN_RIGHT: build_is_left = true; break;
+    default: build_is_left = false; break;
+  }
+  const bool preserve = join_type != BOLT_JOIN_INNER;
+  const bool full = join_type == BOLT_JOIN_FULL;
+  if (join_type == BOLT_JOIN_INNER && (nl == 0 || nr == 0)) return BOLT_OK;
+  const bolt_column_t* bcols = build_is_left ? left_keys : right_keys;
+  const bolt_column_t* pcols = build_is_left ? right_keys : left_keys;
+  uint32_t* out_build = build_is_left ? out_left_rows : out_right_rows;
+  uint32_t* out_probe = build_is_left ? out_right_rows : out_left_rows;
+  if ((out_left_rows == nullptr || out_right_rows == nullptr) && capacity > 0)
+    return set_error(BOLT_ERR_PLAN, "bolt_join: NULL output arrays");
+
+  DevBuf<int64_t> benc, penc;
+  DevBuf<uint64_t> bval, pval;
+  bolt_column_t bkey, pkey;
+  BOLT_TRY(side_key(s, bcols, n_keys, benc, bval, &bkey));
+  BOLT_TRY(side_key(s, pcols, n_keys, penc, pval, &pkey));
+  bolt_join_table_t* t = nullptr;
+  BOLT_TRY(bolt_join_build(stream, &bkey, nullptr, &t));
+  const int64_t n_build = bkey.length;
+  DevBuf<uint8_t> unmatched;
+  int st = BOLT_OK;
+  if (full && n_build > 0) {
+    st = unmatched.alloc((size_t)((n_build + 3) / 4 * 4), s);
+    if (st == BOLT_OK) st = fill_u32(unmatched.ptr, 0x01010101u, (size_t)((n_build + 3) / 4), s);
+  }
+  DevFilter f;
+  memset(&f, 0, sizeof f);
+  int64_t total = 0;
+  if (st == BOLT_OK)
+    st = probe_emit(s, t, pkey, f, preserve, full ? unmatched.ptr : nullptr, out_probe, out_build, capacity, &total);
+  bolt_join_table_release(t);
+  BOLT_TRY(st);
+  if (full && n_build > 0) {
+    DevBuf<uint32_t> rank;
+    BOLT_TRY(rank.alloc((size_t)n_build, s));
+    int64_t n_unmatched = 0;
+    BOLT_TRY(bolt_prefix_scan_mask(stream, unmatched.ptr, n_build, rank.ptr, &n_unmatched));
+    if (n_unmatched > 0 && total < capacity) {
+      join_unmatched_kernel<<<grid_for(n_build, 256, 4), 256, 0, s>>>(unmatched.ptr, rank.ptr, n_build, total,
+                                                                      capacity, out_probe, out_build);
+      BOLT_HIP(hipGetLastError());
+      BOLT_HIP(hipStreamSynchronize(s));
+    }
+    total += n_unmatched;
+  }
+  *n_out = total;
+  if (total > capacity)
+    return set_error(BOLT_DECLINE, "bolt_join: %lld output rows exceed the capacity %lld", (long long)total,
+                     (long long)capacity);
+  return BOLT_OK;
 }

@@ -135,8 +135,9 @@ __global__ __launch_bounds__(256) void take_kernel(const uint8_t* __restrict__ s
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < end; i += stride) {
     const bool in = i < n;
     int64_t r = in ? (int64_t)rows[i] : 0;
-    const bool ok = r < src_len;
-    if (in && !ok) atomicOr(err, 1u);
+    const bool null_row = r == 0xFFFFFFFFll;  // the NULL index of an outer join
+    const bool ok = r < src_len && !null_row;
+    if (in && !ok) atomicOr(err, null_row ? (dval ? 0u : 2u) : 1u);
     if (!ok) r = 0;
     if (in) {
       switch (width) {
@@ -269,13 +270,14 @@ extern "C" int bolt_take(bolt_stream_t stream, const bolt_column_t* src, const u
   BOLT_TRY(fill_u32(err.ptr, 0, 1, s));
   take_kernel<<<grid_for(n, 256, 4), 256, 0, s>>>(static_cast<const uint8_t*>(src->data), src->validity, src->length,
                                                   width, rows, n, static_cast<uint8_t*>(const_cast<void*>(dst->data)),
-                                                  src->validity ? const_cast<uint8_t*>(dst->validity) : nullptr,
+                                                  const_cast<uint8_t*>(dst->validity),
                                                   err.ptr);
   BOLT_HIP(hipGetLastError());
   uint32_t e = 0;
   BOLT_HIP(hipMemcpyAsync(&e, err.ptr, 4, hipMemcpyDeviceToHost, s));
   BOLT_HIP(hipStreamSynchronize(s));
-  if (e) return set_error(BOLT_ERR_PLAN, "bolt_take: a row id is out of range (source has %lld rows)",
-                          (long long)src->length);
+  if (e & 1u) return set_error(BOLT_ERR_PLAN, "bolt_take: a row id is out of range (source has %lld rows)",
+                               (long long)src->length);
+  if (e & 2u) return set_error(BOLT_ERR_PLAN, "bolt_take: a NULL row id needs a destination validity bitmap");
   return BOLT_OK;
 }

@@ -319,9 +319,10 @@ class Server:
             pass
 
 
-def take(col: DeviceColumn, rows: DeviceBuffer, n: int, stream: Stream | None = None) -> DeviceColumn:
-    """col[rows[:n]] on the device (bolt_take)."""
-    out = DeviceColumn.empty(n, col.dtype, with_validity=col.validity is not None)
+def take(col: DeviceColumn, rows: DeviceBuffer, n: int, stream: Stream | None = None,
+         nullable: bool = False) -> DeviceColumn:
+    """col[rows[:n]] on the device (bolt_take); a NULL_ROW id gathers a NULL (give `nullable=True`)."""
+    out = DeviceColumn.empty(n, col.dtype, with_validity=col.validity is not None or nullable)
     out.scale = col.scale
     oc = out.c()
     N.check(N.lib().bolt_take(_stream(stream), C.byref(col.c()), rows.ptr, n, C.byref(oc)), "bolt_take")
@@ -522,6 +523,34 @@ def execute_inner_join(left_key: DeviceColumn, right_key: DeviceColumn, capacity
             cap = n_out.value
             continue
         N.check(st, "bolt_inner_join")
+        k = n_out.value
+        return ol.to_numpy(np.uint32, k), orr.to_numpy(np.uint32, k)
+
+
+JOIN_TYPES = {"inner": 0, "left": 1, "right": 2, "full": 3}
+NULL_ROW = 0xFFFFFFFF
+
+
+def execute_join(left_keys: list[DeviceColumn], right_keys: list[DeviceColumn], how: str = "inner",
+                 capacity: int | None = None, stream: Stream | None = None) -> tuple[np.ndarray, np.ndarray]:
+    """(left_row, right_row) pairs of an equi-join (execute_join, src/exec/join.rs:132) of kind `how`
+    (inner / left / right / full); NULL_ROW marks the padded side of an outer-join pair."""
+    if how not in JOIN_TYPES:
+        raise ValueError(f"unknown join type {how!r}")
+    if len(left_keys) != len(right_keys):
+        raise ValueError("left and right need the same number of key columns")
+    nl, nr = left_keys[0].length, right_keys[0].length
+    cap = capacity if capacity is not None else max(nl + nr, 1)
+    la, ra = columns_array(left_keys), columns_array(right_keys)
+    while True:
+        ol, orr = DeviceBuffer(max(cap, 1) * 4), DeviceBuffer(max(cap, 1) * 4)
+        n_out = C.c_int64()
+        st = N.lib().bolt_join(_stream(stream), la, ra, len(left_keys), JOIN_TYPES[how], ol.ptr, orr.ptr, cap,
+                               C.byref(n_out))
+        if st == N.BOLT_DECLINE and capacity is None and n_out.value > cap:
+            cap = n_out.value
+            continue
+        N.check(st, "bolt_join")
         k = n_out.value
         return ol.to_numpy(np.uint32, k), orr.to_numpy(np.uint32, k)
 

@@ -163,8 +163,11 @@ int bolt_prefix_scan_mask(bolt_stream_t stream, const uint8_t* mask, int64_t n, 
  * take of execute_inner_join's callers, src/exec/join.rs:169, and the device
  * gather of src/jit/prefix_scan.rs:1227): dst[i] = src[rows[i]] for i < n,
  * validity bits likewise.  dst is caller-allocated with capacity n (and a
- * writable bitmap of ceil(n / 8) bytes when src has one); it takes src's
- * dtype and scale and length n.  A row id >= src->length is an error
+ * writable bitmap of ceil(n / 8) bytes when src has one or a row id is
+ * BOLT_NULL_ROW); it takes src's
+ * dtype and scale and length n.  A row id of BOLT_NULL_ROW gathers a NULL
+ * (the NULL index of an outer join's take, src/exec/join.rs:754; dst then
+ * needs a bitmap).  Any other row id >= src->length is an error
  * (BOLT_ERR_PLAN; that row gathers zero).  Synchronises. */
 int bolt_take(bolt_stream_t stream, const bolt_column_t* src, const uint32_t* rows, int64_t n,
               bolt_column_t* dst);
@@ -242,6 +245,36 @@ int bolt_join_probe_aggregate(bolt_stream_t stream, const bolt_join_table_t* t,
 int bolt_inner_join(bolt_stream_t stream, const bolt_column_t* left_key, const bolt_column_t* right_key,
                     uint32_t* out_left_rows, uint32_t* out_right_rows, int64_t capacity,
                     int64_t* n_out);
+
+/* Equi-join of any kind — replaces execute_join's equi-join paths
+ * (src/exec/join.rs:132: execute_inner_join :169, execute_outer_join :283)
+ * and their GPU form execute_outer_join_indices_on_gpu
+ * (src/exec/gpu_join.rs:1907) with the key encodings of
+ * encode_keys_for_shape (:719).  n_keys key columns per side, of equal dtypes
+ * left and right: one column of Int32 / Int64 / Date32 / Float32 / Float64 /
+ * Bool (floats compare -0.0 equal to +0.0 and every NaN equal to every NaN,
+ * canonical_f64_bits :682), or two Int32 / Date32 columns (TwoI32, :788).
+ * Other multi-column shapes return BOLT_ERR_UNSUPPORTED.  A row with a NULL
+ * in any key column matches nothing (src/exec/join.rs:1045); on a preserved
+ * side it is still emitted.
+ * Output: (left_row, right_row) pairs, BOLT_NULL_ROW on the padded side.
+ * Build side: the smaller for INNER (join.rs:192), else the side that is not
+ * preserved (the right for LEFT and FULL, the left for RIGHT, join.rs:305).
+ * Order: the probe side's rows ascending, each with its matches (build rows
+ * in hash-chain order) or, when preserved and unmatched, one NULL-padded
+ * pair in its place; for FULL then the unmatched build rows ascending
+ * (join.rs:396-449: the reference's host order up to the order of one probe
+ * row's matches).  Capacity contract of bolt_join_probe.  Synchronises. */
+enum bolt_join_type {
+  BOLT_JOIN_INNER = 0,
+  BOLT_JOIN_LEFT = 1,  /* LEFT [OUTER] */
+  BOLT_JOIN_RIGHT = 2, /* RIGHT [OUTER] */
+  BOLT_JOIN_FULL = 3   /* FULL [OUTER] */
+};
+#define BOLT_NULL_ROW 0xFFFFFFFFu /* the NULL-padded side of an outer-join pair */
+int bolt_join(bolt_stream_t stream, const bolt_column_t* left_keys, const bolt_column_t* right_keys,
+              int32_t n_keys, int32_t join_type, uint32_t* out_left_rows, uint32_t* out_right_rows,
+              int64_t capacity, int64_t* n_out);
 
 /* ------------------------------------------------------------------------
  * Radix sort — replaces sort_indices_on_gpu_radix{,_multi}

@@ -72,6 +72,9 @@ def lib() -> C.CDLL:
         L.bo_inner_join.restype = C.c_int64
         L.bo_inner_join.argtypes = [C.POINTER(Col), C.POINTER(Col), C.c_void_p, C.c_void_p, C.c_int64,
                                     C.POINTER(C.c_int64)]
+        L.bo_join.restype = C.c_int64
+        L.bo_join.argtypes = [C.POINTER(Col), C.POINTER(Col), C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                              C.c_int64, C.POINTER(C.c_int64)]
         L.bo_sort_indices.argtypes = [C.POINTER(Col), C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                       C.c_int64, C.c_void_p]
         L.bo_filter_join_agg.restype = C.c_int32
@@ -199,6 +202,21 @@ def inner_join(left: HostCol, right: HostCol) -> tuple[np.ndarray, np.ndarray]:
     k = need.value
     ol, orr = np.empty(max(k, 1), np.uint32), np.empty(max(k, 1), np.uint32)
     lib().bo_inner_join(C.byref(left.col), C.byref(right.col), ol.ctypes.data, orr.ctypes.data, k, C.byref(need))
+    return ol[:k], orr[:k]
+
+
+JOIN_TYPES = {"inner": 0, "left": 1, "right": 2, "full": 3}
+
+
+def join(left: list[HostCol], right: list[HostCol], how: str = "inner") -> tuple[np.ndarray, np.ndarray]:
+    """src/exec/join.rs:132 (INNER :169, OUTER :283): (left_row, right_row) pairs in the reference's
+    host order, 0xFFFFFFFF on the NULL-padded side."""
+    lc, rc = _cols(left), _cols(right)
+    need = C.c_int64()
+    lib().bo_join(lc, rc, len(left), JOIN_TYPES[how], None, None, 0, C.byref(need))
+    k = need.value
+    ol, orr = np.empty(max(k, 1), np.uint32), np.empty(max(k, 1), np.uint32)
+    lib().bo_join(lc, rc, len(left), JOIN_TYPES[how], ol.ctypes.data, orr.ctypes.data, k, C.byref(need))
     return ol[:k], orr[:k]
 
 

@@ -580,6 +580,105 @@ int64_t bo_inner_join(const bo_col* left_key, const bo_col* right_key, uint32_t*
   return k <= cap ? k : -1;
 }
 
+/* one join key as an i64 that is equal exactly when the keys are (SQL
+ * equality of extract_key, src/exec/join.rs:1045; floats by canonical bits,
+ * gpu_join.rs:682-705; two 32-bit columns packed first-high, :788).
+ * Returns 0 for a NULL key. */
+static int join_key(const bo_col* cols, int32_t n_keys, int64_t i, int64_t* key) {
+  for (int c = 0; c < n_keys; c++)
+    if (!is_valid(&cols[c], i)) return 0;
+  if (n_keys == 2) {
+    uint32_t hi = (uint32_t)((const int32_t*)cols[0].data)[i];
+    uint32_t lo = (uint32_t)((const int32_t*)cols[1].data)[i];
+    *key = (int64_t)(((uint64_t)hi << 32) | lo);
+    return 1;
+  }
+  const bo_col* c = &cols[0];
+  switch (c->dtype) {
+    case BO_BOOL: *key = ((const uint8_t*)c->data)[i] != 0; return 1;
+    case BO_F32: {
+      float f = ((const float*)c->data)[i];
+      uint32_t b;
+      if (isnan(f)) b = 0x7FC00000u;      /* f32::NAN.to_bits() */
+      else if (f == 0.0f) b = 0;          /* -0.0 == +0.0 */
+      else memcpy(&b, &f, 4);
+      *key = (int64_t)(uint64_t)b;
+      return 1;
+    }
+    case BO_F64: {
+      double d = ((const double*)c->data)[i];
+      uint64_t b;
+      if (isnan(d)) b = 0x7FF8000000000000ull;
+      else if (d == 0.0) b = 0;
+      else memcpy(&b, &d, 8);
+      *key = (int64_t)b;
+      return 1;
+    }
+    default: *key = get_i64(c, i); return 1;
+  }
+}
+
+int64_t bo_join(const bo_col* left_keys, const bo_col* right_keys, int32_t n_keys, int32_t join_type,
+                uint32_t* out_left, uint32_t* out_right, int64_t cap, int64_t* needed) {
+  const uint32_t NUL = 0xFFFFFFFFu;
+  int64_t nl = left_keys[0].n, nr = right_keys[0].n;
+  *needed = 0;
+  /* build side (join.rs:192 for INNER, :305 for OUTER) */
+  int build_is_left = join_type == 0 ? nl <= nr : join_type == 2;
+  int preserve_probe = join_type != 0, preserve_build = join_type == 3;
+  if (join_type == 0 && (nl == 0 || nr == 0)) return 0;
+  const bo_col* b = build_is_left ? left_keys : right_keys;
+  const bo_col* p = build_is_left ? right_keys : left_keys;
+  int64_t nb = b[0].n, np = p[0].n;
+  hmap_t m;
+  hmap_init(&m, nb > 0 ? nb : 1);
+  int64_t* next = (int64_t*)malloc((size_t)(nb > 0 ? nb : 1) * 8);
+  int64_t* tail = (int64_t*)malloc((size_t)(nb > 0 ? nb : 1) * 8);
+  uint8_t* matched = (uint8_t*)calloc((size_t)(nb > 0 ? nb : 1), 1);
+  for (int64_t i = 0; i < nb; i++) { /* build_hash_map (join.rs:844): NULL keys skipped */
+    next[i] = -1;
+    int64_t k;
+    if (!join_key(b, n_keys, i, &k)) continue;
+    int found;
+    uint64_t s = hmap_slot(&m, k, &found);
+    if (!found) { m.used[s] = 1; m.keys[s] = k; m.vals[s] = i; tail[i] = i; }
+    else { int64_t h = m.vals[s]; next[tail[h]] = i; tail[h] = i; }
+  }
+  int64_t n = 0;
+#define EMIT(BROW, PROW)                                                               \
+  do {                                                                                 \
+    if (n < cap) {                                                                     \
+      if (build_is_left) { out_left[n] = (BROW); out_right[n] = (PROW); }              \
+      else { out_left[n] = (PROW); out_right[n] = (BROW); }                            \
+    }                                                                                  \
+    n++;                                                                               \
+  } while (0)
+  for (int64_t j = 0; j < np; j++) { /* join.rs:396-437 */
+    int64_t k, head = -1;
+    if (join_key(p, n_keys, j, &k)) {
+      int found;
+      uint64_t s = hmap_slot(&m, k, &found);
+      if (found) head = m.vals[s];
+    }
+    if (head >= 0) {
+      for (int64_t r = head; r >= 0; r = next[r]) {
+        EMIT((uint32_t)r, (uint32_t)j);
+        matched[r] = 1;
+      }
+    } else if (preserve_probe) {
+      EMIT(NUL, (uint32_t)j);
+    }
+  }
+  if (preserve_build) /* join.rs:440-449 */
+    for (int64_t r = 0; r < nb; r++)
+      if (!matched[r]) EMIT((uint32_t)r, NUL);
+#undef EMIT
+  free(next); free(tail); free(matched);
+  hmap_free(&m);
+  *needed = n;
+  return n <= cap ? n : -1;
+}
+
 /* ------------------------------------------------------------------------ */
 /* ORDER BY — src/exec/sort.rs:161-230, gpu_sort.rs:1070-1180               */
 /* ------------------------------------------------------------------------ */

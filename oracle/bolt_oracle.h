@@ -106,6 +106,20 @@ int32_t bo_groupby(const bo_col* keys, int32_t n_keys, const bo_col* inputs, con
 int64_t bo_inner_join(const bo_col* left_key, const bo_col* right_key, uint32_t* out_left,
                       uint32_t* out_right, int64_t cap, int64_t* needed);
 
+/* ---- Equi-join of any kind (execute_join, src/exec/join.rs:132: INNER
+ *      :169-270, LEFT / RIGHT / FULL OUTER :283-460; key values of
+ *      extract_key :1045-1130 with the float / multi-key encodings of
+ *      src/exec/gpu_join.rs:682-790) ----
+ * join_type 0 INNER, 1 LEFT, 2 RIGHT, 3 FULL.  n_keys 1 (any of Bool / Int32
+ * / Int64 / Date32 / Float32 / Float64) or 2 (Int32 / Date32).  A NULL in any
+ * key column matches nothing.  Pairs are (left_row, right_row) with
+ * 0xFFFFFFFF on the NULL-padded side, in the reference's host order: probe
+ * rows ascending, each with its build matches in build-row order or one
+ * padded pair, then (FULL) the unmatched build rows ascending.  Returns the
+ * pair count, or -1 if `cap` is too small (*needed holds the count). */
+int64_t bo_join(const bo_col* left_keys, const bo_col* right_keys, int32_t n_keys, int32_t join_type,
+                uint32_t* out_left, uint32_t* out_right, int64_t cap, int64_t* needed);
+
 /* ---- ORDER BY (src/exec/sort.rs:161-230 lexsort + take; key transforms of
  *      src/exec/gpu_sort.rs:1070-1180) ----
  * Stable: equal keys keep input order (the canonical order used to compare

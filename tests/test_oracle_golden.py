@@ -338,6 +338,141 @@ def test_join_matches_bruteforce_with_duplicates_and_nulls():
     assert r.tolist() == sorted(r.tolist())
 
 
+# ---- joins of every kind (src/exec/join.rs:132) ---------------------------------------------
+
+NUL = 0xFFFFFFFF
+
+
+def _outer_fixture_sides():
+    # tests/gpu_join_e2e.rs:378-487: left keys 0..4096, right keys 2048..6144
+    return (O.HostCol(np.arange(4096, dtype=np.int32), O.I32),
+            O.HostCol(np.arange(2048, 2048 + 4096, dtype=np.int32), O.I32))
+
+
+def test_join_left_outer_fixture():
+    # tests/gpu_join_e2e.rs:378-417: every left row once, 2048 with the right side NULL
+    lt, rt = _outer_fixture_sides()
+    l, r = O.join([lt], [rt], "left")
+    assert len(l) == 4096 and (r == NUL).sum() == 2048 and (l == NUL).sum() == 0
+    assert sorted(l.tolist()) == list(range(4096))
+
+
+def test_join_right_outer_fixture():
+    # tests/gpu_join_e2e.rs:421-450: every right row once, 2048 with the left side NULL
+    lt, rt = _outer_fixture_sides()
+    l, r = O.join([lt], [rt], "right")
+    assert len(l) == 4096 and (l == NUL).sum() == 2048 and (r == NUL).sum() == 0
+    assert sorted(r.tolist()) == list(range(4096))
+
+
+def test_join_full_outer_fixture():
+    # tests/gpu_join_e2e.rs:454-487: 2048 matched + 2048 left-only + 2048 right-only
+    lt, rt = _outer_fixture_sides()
+    l, r = O.join([lt], [rt], "full")
+    assert len(l) == 6144 and (l == NUL).sum() == 2048 and (r == NUL).sum() == 2048
+
+
+def test_join_duplicate_build_keys_fixture():
+    # tests/gpu_join_e2e.rs:491-524: 4 build x 8 probe rows per key, 1024 keys
+    bk = (np.arange(4096) % 1024).astype(np.int32)
+    pk = (np.arange(8192) % 1024).astype(np.int32)
+    l, r = O.join([O.HostCol(bk, O.I32)], [O.HostCol(pk, O.I32)], "inner")
+    assert len(l) == 32_768
+
+
+def test_join_float_nan_fixture():
+    # tests/gpu_join_e2e.rs:528-597: every NaN probe (a non-canonical NaN
+    # pattern) meets the one NaN build row
+    bk = np.arange(1, 4097, dtype=np.float64)
+    bk[-1] = np.nan
+    weird = np.array([np.nan]).view(np.uint64) ^ np.uint64(1)
+    pk = np.where(np.arange(8192) < 4095, np.arange(1, 8193, dtype=np.float64), weird.view(np.float64)[0])
+    assert np.isnan(pk[-1]) and pk[-1:].view(np.uint64)[0] != np.array([np.nan]).view(np.uint64)[0]
+    l, r = O.join([O.HostCol(bk, O.F64)], [O.HostCol(pk, O.F64)], "inner")
+    assert len(l) == (4096 - 1) + (8192 - (4096 - 1))
+
+
+def test_join_two_key_fixture():
+    # tests/gpu_join_e2e.rs:320-374: (i / 64, i % 64) build tuples, half the probe hits
+    i = np.arange(4096, dtype=np.int32)
+    j = np.arange(8192, dtype=np.int32)
+    pa = np.where(j < 4096, j // 64, 1000 + j // 64).astype(np.int32)
+    l, r = O.join([O.HostCol(i // 64, O.I32), O.HostCol(i % 64, O.I32)],
+                  [O.HostCol(pa, O.I32), O.HostCol(j % 64, O.I32)], "inner")
+    assert len(l) == 4096
+    assert np.array_equal(np.sort(r), np.arange(4096))
+
+
+def _brute_join(lvals, lvalid, rvals, rvalid, how):
+    """Nested loops over SQL key equality (no hashing), in the reference's host order."""
+    nl, nr = len(lvals), len(rvals)
+    if how == "inner" and (nl == 0 or nr == 0):
+        return []
+    build_is_left = (nl <= nr) if how == "inner" else how == "right"
+    bv, bok, pv, pok = (lvals, lvalid, rvals, rvalid) if build_is_left else (rvals, rvalid, lvals, lvalid)
+    out, hit = [], [False] * len(bv)
+    for j in range(len(pv)):
+        ms = [i for i in range(len(bv)) if pok[j] and bok[i] and bv[i] == pv[j]]
+        for i in ms:
+            out.append((i, j))
+            hit[i] = True
+        if not ms and how != "inner":
+            out.append((NUL, j))
+    if how == "full":
+        out += [(i, NUL) for i in range(len(bv)) if not hit[i]]
+    return [(b, p) if build_is_left else (p, b) for b, p in out]
+
+
+def _sql_key(v):
+    if isinstance(v, float):
+        return "nan" if math.isnan(v) else v + 0.0  # -0.0 == +0.0; NaN == NaN
+    return v
+
+
+@pytest.mark.parametrize("how", ["inner", "left", "right", "full"])
+@pytest.mark.parametrize("kind", ["i64", "f64", "f32", "bool", "two_i32", "date32"])
+def test_join_kinds_match_bruteforce(how, kind):
+    rng = np.random.default_rng(hash((how, kind)) & 0xFFFF)
+    nl, nr = 70, 90
+    valid_l, valid_r = rng.random(nl) > 0.15, rng.random(nr) > 0.15
+    def col(n):
+        if kind == "i64":
+            return [rng.integers(-3, 9, n).astype(np.int64)], [O.I64]
+        if kind == "date32":
+            return [rng.integers(18000, 18012, n).astype(np.int32)], [O.DATE32]
+        if kind == "f64":
+            v = rng.choice(np.array([0.0, -0.0, 1.5, -2.25, np.nan, 7.0]), n)
+            v.view(np.uint64)[rng.random(n) < 0.1] = np.uint64(0x7FF8000000000003)  # NaN, non-canonical bits
+            return [v], [O.F64]
+        if kind == "f32":
+            return [rng.choice(np.array([0.0, -0.0, 1.5, np.nan, 3.0], np.float32), n)], [O.F32]
+        if kind == "bool":
+            return [rng.integers(0, 2, n).astype(np.uint8)], [O.BOOL]
+        return [rng.integers(-2, 3, n).astype(np.int32), rng.integers(0, 4, n).astype(np.int32)], [O.I32, O.I32]
+    lv, lt = col(nl)
+    rv, rt = col(nr)
+    lcols = [O.HostCol(v, t, valid_l if c == 0 else None) for c, (v, t) in enumerate(zip(lv, lt))]
+    rcols = [O.HostCol(v, t, valid_r if c == len(rv) - 1 else None) for c, (v, t) in enumerate(zip(rv, rt))]
+    lnull = valid_l.copy()
+    rnull = valid_r.copy()
+    lkey = [tuple(_sql_key(v[i].item()) for v in lv) for i in range(nl)]
+    rkey = [tuple(_sql_key(v[i].item()) for v in rv) for i in range(nr)]
+    want = _brute_join(lkey, lnull, rkey, rnull, how)
+    l, r = O.join(lcols, rcols, how)
+    assert list(zip(l.tolist(), r.tolist())) == want
+
+
+def test_join_outer_empty_sides():
+    # src/exec/join.rs:320-360: one side empty
+    e = O.HostCol(np.zeros(0, np.int64), O.I64)
+    x = O.HostCol(np.array([5, 6, 5], np.int64), O.I64)
+    assert O.join([x], [e], "left")[0].tolist() == [0, 1, 2]
+    assert O.join([x], [e], "full")[1].tolist() == [NUL] * 3
+    assert len(O.join([x], [e], "right")[0]) == 0
+    assert O.join([e], [x], "full")[1].tolist() == [0, 1, 2]
+    assert len(O.join([e], [x], "inner")[0]) == 0
+
+
 # ---- ORDER BY ----------------------------------------------------------------------------
 
 def test_sort_matches_numpy_stable_lexsort():
