@@ -138,6 +138,48 @@ __global__ __launch_bounds__(kProbeBlock) void join_probe_agg_kernel(
   block_reduce_store<NA>(acc, aggs, parts + (int64_t)blockIdx.x * NA);
 }
 
+// Key columns of one side of a join: up to kMaxJoinKeys columns.  `mode` 0:
+// one column (key_word); 1: two 32-bit columns packed first-high / second-low
+// (TwoI32, src/exec/gpu_join.rs:788); 2: a 64-bit hash of the columns' values,
+// equal for equal keys but not only for them, so every hash match is checked
+// against the columns themselves (the exact form of the reference's
+// candidate + host-verify path for TwoI64 / MultiI32, :840 and
+// verify_pairs_on_host).
+constexpr int kMaxJoinKeys = 4;
+struct KeyCols {
+  const void* d[kMaxJoinKeys];
+  const uint8_t* v[kMaxJoinKeys];
+  int dt[kMaxJoinKeys];
+  int n_cols;
+  int mode;
+  uint64_t hash_mask;  // all ones; fewer bits force hash collisions (tests of the check)
+};
+
+__device__ __forceinline__ uint64_t key_word(const void* d, int dt, int64_t i) {
+  switch (dt) {
+    case BOLT_I64: return (uint64_t) static_cast<const int64_t*>(d)[i];
+    case BOLT_F64: {
+      const uint64_t b = static_cast<const uint64_t*>(d)[i];
+      if ((b & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull) return 0x7FF8000000000000ull;
+      return (b & 0x7FFFFFFFFFFFFFFFull) == 0 ? 0ull : b;
+    }
+    case BOLT_F32: {
+      const uint32_t b = static_cast<const uint32_t*>(d)[i];
+      if ((b & 0x7FFFFFFFu) > 0x7F800000u) return 0x7FC00000ull;
+      return (b & 0x7FFFFFFFu) == 0 ? 0ull : (uint64_t)b;
+    }
+    case BOLT_BOOL: return static_cast<const uint8_t*>(d)[i] != 0 ? 1ull : 0ull;
+    default: return (uint64_t)(int64_t) static_cast<const int32_t*>(d)[i];  // Int32 / Date32
+  }
+}
+
+// Probe row `i` and build row `b` hold equal (non-NULL) keys.
+__device__ __forceinline__ bool same_key(const KeyCols& p, int64_t i, const KeyCols& b, uint32_t r) {
+  for (int c = 0; c < p.n_cols; c++)
+    if (key_word(p.d[c], p.dt[c], i) != key_word(b.d[c], b.dt[c], r)) return false;
+  return true;
+}
+
 // ---- probe emitting (probe_row, build_row) pairs -----------------------------
 //
 // Tiled with a decoupled look-back, like the compaction: a tile of
@@ -159,14 +201,14 @@ constexpr int kEmitItems = 8;
 constexpr int kEmitTile = kProbeBlock * kEmitItems;  // 2048 rows
 constexpr uint32_t kNullRow = 0xFFFFFFFFu;
 
-template <typename KeyT, bool kPreserve>
+template <typename KeyT, bool kPreserve, bool kVerify>
 __global__ __launch_bounds__(kProbeBlock) void join_probe_emit_kernel(
     const uint4* __restrict__ tbl, uint64_t mask, const uint32_t* __restrict__ nonunique,
     const KeyT* __restrict__ keys, const uint8_t* __restrict__ kvalid, int64_t n, DevFilter f,
     uint64_t* __restrict__ status, uint32_t* __restrict__ ticket, uint32_t* __restrict__ err,
     unsigned long long* __restrict__ total_out, int64_t capacity, uint32_t* __restrict__ out_probe,
     uint32_t* __restrict__ out_build, const uint4* __restrict__ bloom, uint64_t bmask,
-    uint8_t* __restrict__ unmatched) {
+    uint8_t* __restrict__ unmatched, KeyCols pk, KeyCols bk) {
   __shared__ uint32_t sh_tile;
   __shared__ uint32_t wsum[kProbeBlock / 64];
   __shared__ unsigned long long sh_base;
@@ -188,6 +230,7 @@ __global__ __launch_bounds__(kProbeBlock) void join_probe_emit_kernel(
       const int64_t k = (int64_t)keys[i];
       if (bloom_maybe(bloom, bmask, k))
         probe_chain(tbl, mask, k, first_only, [&](uint32_t w) {
+          if (kVerify && !same_key(pk, i, bk, w)) return;
           m[j]++;
           bv[j] = w;
           if (unmatched) unmatched[w] = 0;
@@ -233,6 +276,7 @@ __global__ __launch_bounds__(kProbeBlock) void join_probe_emit_kernel(
       pos++;
     } else {
       probe_chain(tbl, mask, (int64_t)keys[i], false, [&](uint32_t w) {
+        if (kVerify && !same_key(pk, i, bk, w)) return;
         if ((int64_t)pos < capacity) {
           out_probe[pos] = (uint32_t)i;
           out_build[pos] = w;
@@ -245,39 +289,14 @@ __global__ __launch_bounds__(kProbeBlock) void join_probe_emit_kernel(
 
 // ---- join keys that are not a plain Int32 / Int64 column -----------------------
 //
-// One int64 per row, equal exactly when the SQL keys are equal
-// (encode_keys_for_shape, src/exec/gpu_join.rs:719): two 32-bit columns pack
-// first-high / second-low (TwoI32, :788); floats compare by their canonical
-// bits, -0.0 as +0.0 and every NaN as one NaN (canonical_f32_bits /
-// canonical_f64_bits, :682-705); Bool as 0 / 1.  A row with a NULL in any key
-// column is NULL (src/exec/join.rs:1045).  The validity bitmap is written a
+// One int64 per row (encode_keys_for_shape, src/exec/gpu_join.rs:719),
+// equal exactly when the SQL keys are equal in modes 0 and 1: floats compare
+// by their canonical bits, -0.0 as +0.0 and every NaN as one NaN
+// (canonical_f32_bits / canonical_f64_bits, :682-705), Bool as 0 / 1, two
+// 32-bit columns packed.  Mode 2 hashes the columns (see KeyCols).  A row
+// with a NULL in any key column is NULL (src/exec/join.rs:1045).  The validity bitmap is written a
 // 64-row word at a time from a ballot (`vout` is 8-byte aligned; NULL when no
 // key column has one).
-struct KeyCols {
-  const void* d[2];
-  const uint8_t* v[2];
-  int dt[2];
-  int n_cols;
-};
-
-__device__ __forceinline__ uint64_t key_word(const void* d, int dt, int64_t i) {
-  switch (dt) {
-    case BOLT_I64: return (uint64_t) static_cast<const int64_t*>(d)[i];
-    case BOLT_F64: {
-      const uint64_t b = static_cast<const uint64_t*>(d)[i];
-      if ((b & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull) return 0x7FF8000000000000ull;
-      return (b & 0x7FFFFFFFFFFFFFFFull) == 0 ? 0ull : b;
-    }
-    case BOLT_F32: {
-      const uint32_t b = static_cast<const uint32_t*>(d)[i];
-      if ((b & 0x7FFFFFFFu) > 0x7F800000u) return 0x7FC00000ull;
-      return (b & 0x7FFFFFFFu) == 0 ? 0ull : (uint64_t)b;
-    }
-    case BOLT_BOOL: return static_cast<const uint8_t*>(d)[i] != 0 ? 1ull : 0ull;
-    default: return (uint64_t)(int64_t) static_cast<const int32_t*>(d)[i];  // Int32 / Date32
-  }
-}
-
 __global__ __launch_bounds__(256) void join_key_kernel(KeyCols kc, int64_t n, int64_t* __restrict__ out,
                                                        uint64_t* __restrict__ vout) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -287,13 +306,15 @@ __global__ __launch_bounds__(256) void join_key_kernel(KeyCols kc, int64_t n, in
     bool valid = in;
     uint64_t k = 0;
     if (in) {
-      if (kc.n_cols == 1) {
+      if (kc.mode == 0) {
         k = key_word(kc.d[0], kc.dt[0], i);
-        valid = bit_valid(kc.v[0], i);
-      } else {
+      } else if (kc.mode == 1) {
         k = ((uint64_t) static_cast<const uint32_t*>(kc.d[0])[i] << 32) | static_cast<const uint32_t*>(kc.d[1])[i];
-        valid = bit_valid(kc.v[0], i) && bit_valid(kc.v[1], i);
+      } else {
+        for (int c = 0; c < kc.n_cols; c++) k = fmix64((k * 0x9E3779B97F4A7C15ull) ^ key_word(kc.d[c], kc.dt[c], i));
+        k &= kc.hash_mask;
       }
+      for (int c = 0; c < kc.n_cols; c++) valid = valid && bit_valid(kc.v[c], i);
       out[i] = (int64_t)k;
     }
     if (vout) {
@@ -581,7 +602,16 @@ extern "C" int bolt_join_probe_aggregate(bolt_stream_t stream, const bolt_join_t
 // `unmatched` as for join_probe_emit_kernel.  Synchronises.
 static int probe_emit(hipStream_t s, const bolt_join_table_t* t, const bolt_column_t& key, const DevFilter& f,
                       bool preserve, uint8_t* unmatched, uint32_t* out_probe, uint32_t* out_build,
-                      int64_t capacity, int64_t* total_out) {
+                      int64_t capacity, int64_t* total_out, const KeyCols* verify_probe = nullptr,
+                      const KeyCols* verify_build = nullptr) {
+  KeyCols pk, bk;
+  memset(&pk, 0, sizeof pk);
+  memset(&bk, 0, sizeof bk);
+  const bool verify = verify_probe != nullptr;
+  if (verify) {
+    pk = *verify_probe;
+    bk = *verify_build;
+  }
   DevBuf<unsigned long long> counter;
   DevBuf<uint64_t> status;
   DevBuf<uint32_t> misc;  // [0] ticket, [1] look-back error
@@ -593,16 +623,21 @@ static int probe_emit(hipStream_t s, const bolt_join_table_t* t, const bolt_colu
     const int64_t tiles = (key.length + kEmitTile - 1) / kEmitTile;
     BOLT_TRY(status.alloc((size_t)tiles, s));
     BOLT_TRY(fill_u32(status.ptr, 0, (size_t)tiles * 2, s));
-#define EMIT_LAUNCH(KT, PRESERVE)                                                                                \
-  join_probe_emit_kernel<KT, PRESERVE><<<(unsigned)tiles, kProbeBlock, 0, s>>>(                                  \
+#define EMIT_LAUNCH(KT, PRESERVE, VERIFY)                                                                        \
+  join_probe_emit_kernel<KT, PRESERVE, VERIFY><<<(unsigned)tiles, kProbeBlock, 0, s>>>(                          \
       t->tbl, t->mask, t->flags, static_cast<const KT*>(key.data), key.validity, key.length, f, status.ptr,      \
-      misc.ptr, misc.ptr + 1, counter.ptr, capacity, out_probe, out_build, t->bloom, t->bloom_mask, unmatched)
-    if (key.dtype == BOLT_I64) {
-      if (preserve) EMIT_LAUNCH(int64_t, true);
-      else EMIT_LAUNCH(int64_t, false);
+      misc.ptr, misc.ptr + 1, counter.ptr, capacity, out_probe, out_build, t->bloom, t->bloom_mask, unmatched,  \
+      pk, bk)
+    if (verify && key.dtype != BOLT_I64) return set_error(BOLT_ERR_OTHER, "hash join: hashed keys must be Int64");
+    if (verify) {
+      if (preserve) EMIT_LAUNCH(int64_t, true, true);
+      else EMIT_LAUNCH(int64_t, false, true);
+    } else if (key.dtype == BOLT_I64) {
+      if (preserve) EMIT_LAUNCH(int64_t, true, false);
+      else EMIT_LAUNCH(int64_t, false, false);
     } else {
-      if (preserve) EMIT_LAUNCH(int32_t, true);
-      else EMIT_LAUNCH(int32_t, false);
+      if (preserve) EMIT_LAUNCH(int32_t, true, false);
+      else EMIT_LAUNCH(int32_t, false, false);
     }
 #undef EMIT_LAUNCH
     BOLT_HIP(hipGetLastError());
@@ -665,6 +700,31 @@ extern "C" int bolt_inner_join(bolt_stream_t stream, const bolt_column_t* left_k
 // Key columns of one side as a probe / build key: a plain Int32 / Int64 /
 // Date32 column is used in place; anything else is encoded into `enc` /
 // `enc_valid` by join_key_kernel.
+static bool is_int32_key(int dt) { return dt == BOLT_I32 || dt == BOLT_DATE32; }
+
+// Encoding mode of a key shape (KeyCols::mode).
+static int key_mode(const bolt_column_t* cols, int32_t n_keys) {
+  if (n_keys == 1) return 0;
+  if (n_keys == 2 && is_int32_key(cols[0].dtype) && is_int32_key(cols[1].dtype)) return 1;
+  return 2;
+}
+
+static void fill_key_cols(const bolt_column_t* cols, int32_t n_keys, KeyCols* kc) {
+  memset(kc, 0, sizeof *kc);
+  kc->n_cols = n_keys;
+  kc->mode = key_mode(cols, n_keys);
+  kc->hash_mask = ~0ull;
+  if (const char* env = getenv("BOLT_JOIN_HASH_BITS")) {  // tests: force collisions of hashed keys
+    const int bits = atoi(env);
+    if (bits > 0 && bits < 64) kc->hash_mask = (1ull << bits) - 1;
+  }
+  for (int c = 0; c < n_keys; c++) {
+    kc->d[c] = cols[c].data;
+    kc->v[c] = cols[c].validity;
+    kc->dt[c] = cols[c].dtype;
+  }
+}
+
 static int side_key(hipStream_t s, const bolt_column_t* cols, int32_t n_keys, DevBuf<int64_t>& enc,
                     DevBuf<uint64_t>& enc_valid, bolt_column_t* key) {
   const int64_t n = cols[0].length;
@@ -674,15 +734,9 @@ static int side_key(hipStream_t s, const bolt_column_t* cols, int32_t n_keys, De
     return BOLT_OK;
   }
   KeyCols kc;
-  memset(&kc, 0, sizeof kc);
-  kc.n_cols = n_keys;
+  fill_key_cols(cols, n_keys, &kc);
   bool any_nulls = false;
-  for (int c = 0; c < n_keys; c++) {
-    kc.d[c] = cols[c].data;
-    kc.v[c] = cols[c].validity;
-    kc.dt[c] = cols[c].dtype;
-    any_nulls |= cols[c].validity != nullptr;
-  }
+  for (int c = 0; c < n_keys; c++) any_nulls |= cols[c].validity != nullptr;
   BOLT_TRY(enc.alloc((size_t)(n > 0 ? n : 1), s));
   if (any_nulls) BOLT_TRY(enc_valid.alloc((size_t)((n + 63) / 64 + 1), s));
   if (n > 0) {
@@ -732,17 +786,16 @@ extern "C" int bolt_join(bolt_stream_t stream, const bolt_column_t* left_keys, c
                        dtype_name(dr));
     const bool ok1 = dl == BOLT_I32 || dl == BOLT_I64 || dl == BOLT_DATE32 || dl == BOLT_F32 || dl == BOLT_F64 ||
                      dl == BOLT_BOOL;
-    const bool ok2 = dl == BOLT_I32 || dl == BOLT_DATE32;
+    const bool ok_multi = dl == BOLT_I32 || dl == BOLT_I64 || dl == BOLT_DATE32;
     if (n_keys == 1 && !ok1)
       return set_error(BOLT_ERR_TYPE, "bolt_join: join key dtype %s not supported", dtype_name(dl));
-    if (n_keys > 1 && !ok2)
+    if (n_keys > 1 && !ok_multi)
       return set_error(BOLT_ERR_UNSUPPORTED,
-                       "bolt_join: multi-column keys pack two 32-bit integer columns (key %d is %s)", c,
+                       "bolt_join: multi-column keys are Int32 / Int64 / Date32 columns (key %d is %s)", c,
                        dtype_name(dl));
   }
-  if (n_keys > 2)
-    return set_error(BOLT_ERR_UNSUPPORTED, "bolt_join: %d key columns (two 32-bit columns pack into one key)",
-                     n_keys);
+  if (n_keys > kMaxJoinKeys)
+    return set_error(BOLT_ERR_UNSUPPORTED, "bolt_join: %d key columns (at most %d)", n_keys, kMaxJoinKeys);
   const int64_t nl = left_keys[0].length, nr = right_keys[0].length;
   // Build side: the smaller for INNER (src/exec/join.rs:192); the side that is
   // not preserved for LEFT / RIGHT, the right for FULL (join.rs:305).
@@ -779,8 +832,15 @@ extern "C" int bolt_join(bolt_stream_t stream, const bolt_column_t* left_keys, c
   DevFilter f;
   memset(&f, 0, sizeof f);
   int64_t total = 0;
+  KeyCols pk, bk;
+  const bool verify = key_mode(bcols, n_keys) == 2;
+  if (verify) {
+    fill_key_cols(pcols, n_keys, &pk);
+    fill_key_cols(bcols, n_keys, &bk);
+  }
   if (st == BOLT_OK)
-    st = probe_emit(s, t, pkey, f, preserve, full ? unmatched.ptr : nullptr, out_probe, out_build, capacity, &total);
+    st = probe_emit(s, t, pkey, f, preserve, full ? unmatched.ptr : nullptr, out_probe, out_build, capacity, &total,
+                    verify ? &pk : nullptr, verify ? &bk : nullptr);
   bolt_join_table_release(t);
   BOLT_TRY(st);
   if (full && n_build > 0) {

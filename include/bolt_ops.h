@@ -253,8 +253,11 @@ int bolt_inner_join(bolt_stream_t stream, const bolt_column_t* left_key, const b
  * encode_keys_for_shape (:719).  n_keys key columns per side, of equal dtypes
  * left and right: one column of Int32 / Int64 / Date32 / Float32 / Float64 /
  * Bool (floats compare -0.0 equal to +0.0 and every NaN equal to every NaN,
- * canonical_f64_bits :682), or two Int32 / Date32 columns (TwoI32, :788).
- * Other multi-column shapes return BOLT_ERR_UNSUPPORTED.  A row with a NULL
+ * canonical_f64_bits :682), or 2 to 4 Int32 / Int64 / Date32 columns: two
+ * 32-bit columns pack into one key (TwoI32, :788); other shapes (TwoI64,
+ * MultiI32) are hashed and every hash match is checked against the key
+ * columns on the device, so the result is exact without the reference's host
+ * verify (verify_pairs_on_host).  A row with a NULL
  * in any key column matches nothing (src/exec/join.rs:1045); on a preserved
  * side it is still emitted.
  * Output: (left_row, right_row) pairs, BOLT_NULL_ROW on the padded side.

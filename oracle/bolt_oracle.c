@@ -587,6 +587,15 @@ int64_t bo_inner_join(const bo_col* left_key, const bo_col* right_key, uint32_t*
 static int join_key(const bo_col* cols, int32_t n_keys, int64_t i, int64_t* key) {
   for (int c = 0; c < n_keys; c++)
     if (!is_valid(&cols[c], i)) return 0;
+  if (n_keys > 2 || (n_keys == 2 && (cols[0].dtype == BO_I64 || cols[1].dtype == BO_I64))) {
+    /* the wrapping sum of the tuple: collision-prone on purpose ((1, 2) and
+     * (2, 1) share it), so join_tuple_eq, which separates such tuples, is
+     * exercised by every test that has multi-column keys */
+    uint64_t h = 0;
+    for (int c = 0; c < n_keys; c++) h += (uint64_t)get_i64(&cols[c], i);
+    *key = (int64_t)h;
+    return 1;
+  }
   if (n_keys == 2) {
     uint32_t hi = (uint32_t)((const int32_t*)cols[0].data)[i];
     uint32_t lo = (uint32_t)((const int32_t*)cols[1].data)[i];
@@ -616,6 +625,14 @@ static int join_key(const bo_col* cols, int32_t n_keys, int64_t i, int64_t* key)
     }
     default: *key = get_i64(c, i); return 1;
   }
+}
+
+/* exact SQL equality of two non-NULL key tuples (JoinKey::Many, join.rs:1062) */
+static int join_tuple_eq(const bo_col* a, int64_t i, const bo_col* b, int64_t j, int32_t n_keys) {
+  if (n_keys == 1) return 1; /* one column: the key itself is exact */
+  for (int c = 0; c < n_keys; c++)
+    if (get_i64(&a[c], i) != get_i64(&b[c], j)) return 0;
+  return 1;
 }
 
 int64_t bo_join(const bo_col* left_keys, const bo_col* right_keys, int32_t n_keys, int32_t join_type,
@@ -660,14 +677,14 @@ int64_t bo_join(const bo_col* left_keys, const bo_col* right_keys, int32_t n_key
       uint64_t s = hmap_slot(&m, k, &found);
       if (found) head = m.vals[s];
     }
-    if (head >= 0) {
-      for (int64_t r = head; r >= 0; r = next[r]) {
-        EMIT((uint32_t)r, (uint32_t)j);
-        matched[r] = 1;
-      }
-    } else if (preserve_probe) {
-      EMIT(NUL, (uint32_t)j);
+    int hit = 0;
+    for (int64_t r = head; r >= 0; r = next[r]) {
+      if (!join_tuple_eq(b, r, p, j, n_keys)) continue;
+      EMIT((uint32_t)r, (uint32_t)j);
+      matched[r] = 1;
+      hit = 1;
     }
+    if (!hit && preserve_probe) EMIT(NUL, (uint32_t)j);
   }
   if (preserve_build) /* join.rs:440-449 */
     for (int64_t r = 0; r < nb; r++)

@@ -105,28 +105,40 @@ def _key_cols(rng, kind, n, null_frac):
         return [both(rng.integers(-(n // 4) - 1, n // 4 + 1, n).astype(np.int32), N.I32, valid)]
     if kind == "date32":
         return [both(rng.integers(18000, 18000 + max(n // 2, 1), n).astype(np.int32), N.DATE32, valid)]
+    # a few rows of each special float value (every zero meets every zero and
+    # every NaN every NaN, so a fixed fraction of them would square the output)
+    def few():
+        return rng.integers(0, n, min(n, 12))
     if kind == "f64":
         v = rng.integers(0, max(n // 3, 1), n).astype(np.float64) - 0.5
-        v[rng.random(n) < 0.05] = -0.0
-        v[rng.random(n) < 0.05] = 0.0
-        v[rng.random(n) < 0.02] = np.nan
-        v.view(np.uint64)[rng.random(n) < 0.02] = np.uint64(0xFFF8000000000007)
+        v[few()] = -0.0
+        v[few()] = 0.0
+        v[few()] = np.nan
+        v.view(np.uint64)[few()] = np.uint64(0xFFF8000000000007)
         return [both(v, N.F64, valid)]
     if kind == "f32":
         v = rng.integers(0, max(n // 3, 1), n).astype(np.float32) * np.float32(0.25)
-        v[rng.random(n) < 0.05] = np.float32(-0.0)
-        v.view(np.uint32)[rng.random(n) < 0.02] = np.uint32(0x7FC00005)
+        v[few()] = np.float32(-0.0)
+        v[few()] = np.float32(0.0)
+        v.view(np.uint32)[few()] = np.uint32(0x7FC00005)
         return [both(v, N.F32, valid)]
     if kind == "bool":
         return [both(rng.integers(0, 2, n).astype(np.uint8), N.BOOL, valid)]
-    # two Int32 columns (the second with its own NULLs)
     v2 = rng.random(n) >= null_frac if null_frac else None
+    if kind == "two_i64":  # hashed and checked on the device (TwoI64)
+        return [both(rng.integers(-3, 4, n).astype(np.int64) << 36, N.I64, valid),
+                both(rng.integers(0, max(n // 10, 1), n).astype(np.int64) - (1 << 40), N.I64, v2)]
+    if kind == "three_mixed":  # MultiI32-like: three columns, hashed
+        return [both(rng.integers(0, 3, n).astype(np.int32), N.I32, valid),
+                both(rng.integers(0, max(n // 20, 1), n).astype(np.int64), N.I64, v2),
+                both(rng.integers(18000, 18004, n).astype(np.int32), N.DATE32)]
+    # two Int32 columns (the second with its own NULLs)
     return [both(rng.integers(-3, 4, n).astype(np.int32), N.I32, valid),
             both(rng.integers(0, max(n // 10, 1), n).astype(np.int32), N.I32, v2)]
 
 
 @pytest.mark.parametrize("how", HOWS)
-@pytest.mark.parametrize("kind", ["i64", "i32", "date32", "f64", "f32", "two_i32"])
+@pytest.mark.parametrize("kind", ["i64", "i32", "date32", "f64", "f32", "two_i32", "two_i64", "three_mixed"])
 @pytest.mark.parametrize("nl,nr,null_frac", [(1, 1, 0.0), (700, 300, 0.1), (40_000, 90_000, 0.05),
                                              (300_000, 120_000, 0.0)])
 def test_join_kinds_match_oracle(gpu, how, kind, nl, nr, null_frac):
@@ -154,6 +166,38 @@ def test_join_empty_sides(gpu, how):
         _run([a], [b], how, [ha], [hb])
 
 
+@pytest.mark.parametrize("how", HOWS)
+@pytest.mark.parametrize("kind", ["two_i64", "three_mixed"])
+def test_join_hashed_keys_checked_under_collisions(gpu, monkeypatch, how, kind):
+    """With the key hash cut to 3 bits nearly every hash match is a different
+    tuple: the result stays exact only if every match is checked against the
+    key columns."""
+    monkeypatch.setenv("BOLT_JOIN_HASH_BITS", "3")
+    print(f"[case] hashed keys, 3-bit hash, {how} {kind}", flush=True)
+    rng = np.random.default_rng(77)
+    lc = _key_cols(rng, kind, 1500, 0.05)
+    rc = _key_cols(rng, kind, 2500, 0.05)
+    _run([d for d, _ in lc], [d for d, _ in rc], how, [h for _, h in lc], [h for _, h in rc])
+
+
+def test_join_lossy_two_i64_fixtures(gpu):
+    # tests/gpu_join_e2e.rs:762-832 (INNER) and :1063-1162 (LEFT) over two Int64 keys
+    i = np.arange(4096, dtype=np.int64)
+    j = np.arange(8192, dtype=np.int64)
+    pa = np.where(j < 4096, j // 2, 99_999 + j)
+    pb = np.where(j < 4096, j * 3 + 1, 88_888 + j)
+    b = [both(i // 2, N.I64), both(i * 3 + 1, N.I64)]
+    p = [both(pa, N.I64), both(pb, N.I64)]
+    l, _ = _run([d for d, _ in b], [d for d, _ in p], "inner", [h for _, h in b], [h for _, h in p])
+    assert len(l) == 4096
+    n = np.arange(4096, dtype=np.int64)
+    m = np.arange(2048, dtype=np.int64)
+    lt = [both(n, N.I64), both(1_000_000 + n, N.I64)]
+    rt = [both(m, N.I64), both(1_000_000 + m, N.I64)]
+    l, r = _run([d for d, _ in lt], [d for d, _ in rt], "left", [h for _, h in lt], [h for _, h in rt])
+    assert len(l) == 4096 and (r != NULL_ROW).sum() == 2048
+
+
 def test_join_capacity_declines(gpu):
     db, _ = both((np.arange(1000) % 10).astype(np.int64), N.I64)
     dp, _ = both((np.arange(3000) % 10).astype(np.int64), N.I64)
@@ -170,9 +214,13 @@ def test_join_type_errors(gpu):
     with pytest.raises(N.BoltError) as e:
         execute_join([a], [b], "inner")
     assert e.value.code == N.BOLT_ERR_TYPE
-    c, _ = both(np.arange(10, dtype=np.int64), N.I64)
-    with pytest.raises(N.BoltError) as e:
+    c, _ = both(np.arange(10, dtype=np.float64), N.F64)
+    with pytest.raises(N.BoltError) as e:  # multi-column keys are integer columns
         execute_join([c, c], [c, c], "inner")
+    assert e.value.code == N.BOLT_ERR_UNSUPPORTED
+    d, _ = both(np.arange(10, dtype=np.int64), N.I64)
+    with pytest.raises(N.BoltError) as e:
+        execute_join([d] * 5, [d] * 5, "inner")
     assert e.value.code == N.BOLT_ERR_UNSUPPORTED
 
 

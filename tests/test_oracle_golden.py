@@ -430,7 +430,7 @@ def _sql_key(v):
 
 
 @pytest.mark.parametrize("how", ["inner", "left", "right", "full"])
-@pytest.mark.parametrize("kind", ["i64", "f64", "f32", "bool", "two_i32", "date32"])
+@pytest.mark.parametrize("kind", ["i64", "f64", "f32", "bool", "two_i32", "date32", "two_i64", "three_mixed"])
 def test_join_kinds_match_bruteforce(how, kind):
     rng = np.random.default_rng(hash((how, kind)) & 0xFFFF)
     nl, nr = 70, 90
@@ -448,6 +448,12 @@ def test_join_kinds_match_bruteforce(how, kind):
             return [rng.choice(np.array([0.0, -0.0, 1.5, np.nan, 3.0], np.float32), n)], [O.F32]
         if kind == "bool":
             return [rng.integers(0, 2, n).astype(np.uint8)], [O.BOOL]
+        if kind == "two_i64":
+            return [rng.integers(-2, 3, n).astype(np.int64) << 40, rng.integers(0, 4, n).astype(np.int64)], \
+                [O.I64, O.I64]
+        if kind == "three_mixed":
+            return [rng.integers(0, 2, n).astype(np.int32), rng.integers(-1, 2, n).astype(np.int64) * (1 << 33),
+                    rng.integers(18000, 18003, n).astype(np.int32)], [O.I32, O.I64, O.DATE32]
         return [rng.integers(-2, 3, n).astype(np.int32), rng.integers(0, 4, n).astype(np.int32)], [O.I32, O.I32]
     lv, lt = col(nl)
     rv, rt = col(nr)
@@ -460,6 +466,23 @@ def test_join_kinds_match_bruteforce(how, kind):
     want = _brute_join(lkey, lnull, rkey, rnull, how)
     l, r = O.join(lcols, rcols, how)
     assert list(zip(l.tolist(), r.tolist())) == want
+
+
+def test_join_lossy_two_i64_fixtures():
+    # tests/gpu_join_e2e.rs:762-832: unique (i / 2, 3i + 1) build tuples, half the probe hits
+    i = np.arange(4096, dtype=np.int64)
+    j = np.arange(8192, dtype=np.int64)
+    pa = np.where(j < 4096, j // 2, 99_999 + j)
+    pb = np.where(j < 4096, j * 3 + 1, 88_888 + j)
+    l, r = O.join([O.HostCol(i // 2, O.I64), O.HostCol(i * 3 + 1, O.I64)],
+                  [O.HostCol(pa, O.I64), O.HostCol(pb, O.I64)], "inner")
+    assert len(l) == 4096
+    # tests/gpu_join_e2e.rs:1063-1162: LEFT over two Int64 keys, 2048 of 4096 matched
+    n = np.arange(4096, dtype=np.int64)
+    m = np.arange(2048, dtype=np.int64)
+    l, r = O.join([O.HostCol(n, O.I64), O.HostCol(1_000_000 + n, O.I64)],
+                  [O.HostCol(m, O.I64), O.HostCol(1_000_000 + m, O.I64)], "left")
+    assert len(l) == 4096 and (r != NUL).sum() == 2048 and (r == NUL).sum() == 2048
 
 
 def test_join_outer_empty_sides():
