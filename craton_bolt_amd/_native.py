@@ -256,6 +256,8 @@ _SIGS = {
                             C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
     "bolt_sort_pairs_i64": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p,
                                       C.c_void_p]),
+    "bolt_sort_limit": (C.c_int, [C.c_void_p, C.POINTER(BoltColumn), C.c_int32, C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_int32), C.c_int64, C.c_int64, C.c_void_p, C.POINTER(C.c_int64)]),
     "bolt_sort_indices": (C.c_int, [C.c_void_p, C.POINTER(BoltColumn), C.c_int32, C.POINTER(C.c_int32),
                                     C.POINTER(C.c_int32), C.c_void_p]),
     # batched GROUP BY (bolt_batch.h)

@@ -29,6 +29,7 @@
 // as the last one stores.
 #include <algorithm>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "bolt_internal.h"
@@ -644,6 +645,85 @@ __global__ void order_key_kernel(const void* __restrict__ d, const uint8_t* __re
   }
 }
 
+// ---- ORDER BY + LIMIT: radix select on the leading key ---------------------------------
+//
+// `coarse_image` maps the leading key of a row to a u64 that never decreases
+// along the full sort order (NULL rank, direction and value in one word; a
+// Decimal128 by its signed high word).  The K-th smallest image is T; every
+// row among the first K of the order has an image <= T, so sorting only
+// those rows by all keys (and by row id, for the stability of the full sort)
+// gives the same first K rows as sorting everything.
+struct TopKey {
+  const void* d;
+  const uint8_t* v;
+  int dt, bits, desc, nulls_first;
+};
+
+__device__ __forceinline__ uint64_t coarse_image(const TopKey& k, int64_t row) {
+  const bool valid = bit_valid(k.v, row);
+  uint64_t x = 0;
+  if (valid) {
+    x = k.dt == BOLT_DEC128 ? static_cast<const uint64_t*>(k.d)[2 * row + 1] ^ 0x8000000000000000ull
+                            : radix_image(k.d, k.dt, row);
+    if (k.desc) x = ~x & (k.bits >= 64 ? ~0ull : ((1ull << k.bits) - 1));
+  }
+  if (k.v == nullptr) return x;
+  if (!valid) return k.nulls_first ? 0ull : ~0ull;
+  return k.nulls_first ? (x >> 1) + 1 : x >> 1;
+}
+
+constexpr int kSelBits = 12;
+constexpr int kSelBins = 1 << kSelBits;
+
+// Histogram of digit (image >> shift) & (2^width - 1) over the rows whose
+// image >> (shift + width) equals `prefix` (every row when shift + width = 64).
+__global__ __launch_bounds__(256) void topk_hist_kernel(TopKey k, int64_t n, int shift, int width, uint64_t prefix,
+                                                        uint32_t* __restrict__ hist) {
+  __shared__ uint32_t sh[kSelBins];
+  for (int b = threadIdx.x; b < kSelBins; b += 256) sh[b] = 0;
+  __syncthreads();
+  const int hi = shift + width;
+  const uint64_t dmask = (1ull << width) - 1;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+    const uint64_t x = coarse_image(k, i);
+    if (hi < 64 && (x >> hi) != prefix) continue;
+    atomicAdd(&sh[(x >> shift) & dmask], 1u);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < kSelBins; b += 256)
+    if (sh[b]) atomicAdd(&hist[b], sh[b]);
+}
+
+// Appends (unordered) the rows whose image >> shift <= bound.
+__global__ __launch_bounds__(256) void topk_collect_kernel(TopKey k, int64_t n, int shift, uint64_t bound,
+                                                           uint32_t* __restrict__ out,
+                                                           unsigned long long* __restrict__ count, int64_t cap) {
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  const int64_t end = (n + 255) / 256 * 256;  // block-uniform trip count for the ballot
+  const int lane = threadIdx.x & 63;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < end; i += stride) {
+    const bool take = i < n && (shift >= 64 ? 0ull : coarse_image(k, i) >> shift) <= bound;
+    const uint64_t m = __ballot(take);
+    if (m == 0) continue;
+    const int leader = __builtin_ctzll(m);
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader);
+    if (take) {
+      const int64_t pos = (int64_t)base + __popcll(m & ((1ull << lane) - 1));
+      if (pos < cap) out[pos] = (uint32_t)i;
+    }
+  }
+}
+
+// out[j] = rows[perm[first + j]]
+__global__ void topk_emit_kernel(const uint32_t* __restrict__ rows, const uint32_t* __restrict__ perm, int64_t first,
+                                 int64_t m, uint32_t* __restrict__ out) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < m) out[j] = rows[perm[first + j]];
+}
+
 __global__ void iota_kernel(uint32_t* __restrict__ out, int64_t n) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = (uint32_t)i;
@@ -726,5 +806,130 @@ extern "C" int bolt_sort_indices(bolt_stream_t stream, const bolt_column_t* keys
     }
   }
   BOLT_HIP(hipStreamSynchronize(s));
+  return BOLT_OK;
+}
+
+extern "C" int bolt_sort_limit(bolt_stream_t stream, const bolt_column_t* keys, int32_t n_keys,
+                               const int32_t* descending, const int32_t* nulls_first, int64_t offset, int64_t limit,
+                               uint32_t* out_rows, int64_t* n_out) {
+  BOLT_CHECK_DEVICE();
+  clear_error();
+  hipStream_t s = (hipStream_t)stream;
+  *n_out = 0;
+  if (n_keys < 1 || keys == nullptr) return set_error(BOLT_ERR_PLAN, "bolt_sort_limit: no sort keys");
+  if (offset < 0 || limit < 0) return set_error(BOLT_ERR_PLAN, "bolt_sort_limit: negative offset or limit");
+  const int64_t n = keys[0].length;
+  if (n >= (1ll << 31)) return set_error(BOLT_ERR_PLAN, "bolt_sort_limit: %lld rows exceeds 2^31", (long long)n);
+  // execute_limit (src/exec/limit.rs:18): min(limit, n - offset) rows, none past the end
+  const int64_t m = offset >= n ? 0 : (limit < n - offset ? limit : n - offset);
+  if (m == 0) {
+    for (int k = 0; k < n_keys; k++)
+      if (keys[k].length != n) return set_error(BOLT_ERR_PLAN, "bolt_sort_limit: key columns differ in length");
+    return BOLT_OK;
+  }
+  const int64_t K = offset + m;  // rows of the order that must be exact
+  std::vector<int32_t> desc(n_keys + 1, 0), nf(n_keys + 1, 1);
+  for (int k = 0; k < n_keys; k++) {
+    desc[k] = descending ? descending[k] : 0;
+    nf[k] = nulls_first ? nulls_first[k] : 1;
+  }
+  // The full sort, then the slice.
+  auto full_sort = [&]() -> int {
+    DevBuf<uint32_t> perm;
+    BOLT_TRY(perm.alloc((size_t)n, s));
+    BOLT_TRY(bolt_sort_indices(stream, keys, n_keys, desc.data(), nf.data(), perm.ptr));
+    BOLT_HIP(hipMemcpyAsync(out_rows, perm.ptr + offset, (size_t)m * 4, hipMemcpyDeviceToDevice, s));
+    BOLT_HIP(hipStreamSynchronize(s));
+    *n_out = m;
+    return BOLT_OK;
+  };
+  // small inputs, or a limit that reaches most rows
+  if (n <= (1 << 16) || K * 4 >= n) return full_sort();
+  for (int k = 0; k < n_keys; k++) {
+    if (keys[k].length != n) return set_error(BOLT_ERR_PLAN, "bolt_sort_limit: key columns differ in length");
+    const int dt = keys[k].dtype;
+    if (!(dt == BOLT_I32 || dt == BOLT_I64 || dt == BOLT_F32 || dt == BOLT_F64 || dt == BOLT_DATE32 || dt == BOLT_BOOL ||
+          dt == BOLT_DEC128))
+      return set_error(BOLT_ERR_UNSUPPORTED, "bolt_sort_limit: sort key dtype %s not supported", dtype_name(dt));
+  }
+  const bolt_column_t& c0 = keys[0];
+  TopKey tk;
+  tk.d = c0.data;
+  tk.v = c0.validity;
+  tk.dt = c0.dtype;
+  tk.bits = c0.dtype == BOLT_BOOL ? 1 : (c0.dtype == BOLT_DEC128 ? 64 : dtype_size(c0.dtype) * 8);
+  tk.desc = desc[0];
+  tk.nulls_first = nf[0];
+  // Radix select, most significant digit first: after each pass `prefix`
+  // holds the top 64 - hi bits of T, `below` the rows with a smaller prefix
+  // and `eq` those with the same one; stop once the tie bucket is small.
+  DevBuf<uint32_t> hist;
+  BOLT_TRY(hist.alloc(kSelBins, s));
+  std::vector<uint32_t> h(kSelBins);
+  uint64_t prefix = 0;
+  int hi = 64;
+  int64_t below = 0, need = K, eq = n;
+  const unsigned grid = grid_for(n, 256, 4, 2048);
+  while (hi > 0) {
+    const int width = hi >= kSelBits ? kSelBits : hi;
+    const int shift = hi - width;
+    BOLT_TRY(fill_u32(hist.ptr, 0, kSelBins, s));
+    topk_hist_kernel<<<grid, 256, 0, s>>>(tk, n, shift, width, prefix, hist.ptr);
+    BOLT_HIP(hipGetLastError());
+    BOLT_HIP(hipMemcpyAsync(h.data(), hist.ptr, kSelBins * 4, hipMemcpyDeviceToHost, s));
+    BOLT_HIP(hipStreamSynchronize(s));
+    int64_t cum = 0;
+    int d = 0;
+    for (; d < (1 << width); d++) {
+      if (cum + (int64_t)h[d] >= need) break;
+      cum += h[d];
+    }
+    if (d == (1 << width)) return set_error(BOLT_ERR_OTHER, "bolt_sort_limit: radix select lost rows");
+    below += cum;
+    need -= cum;
+    eq = h[d];
+    prefix = (prefix << width) | (uint64_t)d;
+    hi = shift;
+    if (eq <= (1 << 16)) break;
+  }
+  const int64_t C = below + eq;  // rows whose image >> hi <= prefix
+  if (C * 2 > n) return full_sort();  // the leading key ties too often to narrow the rows
+  DevBuf<uint32_t> cand;
+  DevBuf<unsigned long long> cnt;
+  BOLT_TRY(cand.alloc((size_t)C, s));
+  BOLT_TRY(cnt.alloc(1, s));
+  BOLT_TRY(fill_u32(cnt.ptr, 0, 2, s));
+  topk_collect_kernel<<<grid, 256, 0, s>>>(tk, n, hi, prefix, cand.ptr, cnt.ptr, C);
+  BOLT_HIP(hipGetLastError());
+  unsigned long long got = 0;
+  BOLT_HIP(hipMemcpyAsync(&got, cnt.ptr, 8, hipMemcpyDeviceToHost, s));
+  BOLT_HIP(hipStreamSynchronize(s));
+  if ((int64_t)got != C)
+    return set_error(BOLT_ERR_OTHER, "bolt_sort_limit: collected %llu candidates, expected %lld", got, (long long)C);
+  // Sort the candidates by every key, then by row id (the full sort is stable).
+  std::vector<DevBuf<uint8_t>> vals(n_keys), valid(n_keys);
+  std::vector<bolt_column_t> ck(n_keys + 1);
+  for (int k = 0; k < n_keys; k++) {
+    const int w = dtype_size(keys[k].dtype);
+    BOLT_TRY(vals[k].alloc((size_t)C * w, s));
+    memset(&ck[k], 0, sizeof ck[k]);
+    ck[k].data = vals[k].ptr;
+    if (keys[k].validity) {
+      BOLT_TRY(valid[k].alloc((size_t)((C + 7) / 8 + 8), s));
+      ck[k].validity = valid[k].ptr;
+    }
+    BOLT_TRY(bolt_take(stream, &keys[k], cand.ptr, C, &ck[k]));
+  }
+  memset(&ck[n_keys], 0, sizeof ck[n_keys]);
+  ck[n_keys].data = cand.ptr;  // row ids < 2^31: ascending as Int32
+  ck[n_keys].length = C;
+  ck[n_keys].dtype = BOLT_I32;
+  DevBuf<uint32_t> perm;
+  BOLT_TRY(perm.alloc((size_t)C, s));
+  BOLT_TRY(bolt_sort_indices(stream, ck.data(), n_keys + 1, desc.data(), nf.data(), perm.ptr));
+  topk_emit_kernel<<<(unsigned)((m + 255) / 256), 256, 0, s>>>(cand.ptr, perm.ptr, offset, m, out_rows);
+  BOLT_HIP(hipGetLastError());
+  BOLT_HIP(hipStreamSynchronize(s));
+  *n_out = m;
   return BOLT_OK;
 }

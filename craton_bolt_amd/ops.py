@@ -576,3 +576,18 @@ def execute_sort(keys: list[DeviceColumn], descending: list[bool] | None = None,
     st = N.lib().bolt_sort_indices(_stream(stream), columns_array(keys), k, desc, nf, out.ptr)
     N.check(st, "bolt_sort_indices")
     return out.to_numpy(np.uint32, n)
+
+
+def execute_sort_limit(keys: list[DeviceColumn], limit: int, offset: int = 0, descending: list[bool] | None = None,
+                       nulls_first: list[bool] | None = None, stream: Stream | None = None) -> np.ndarray:
+    """Row ids of ORDER BY keys LIMIT limit OFFSET offset (execute_sort + execute_limit,
+    src/exec/sort.rs:161, src/exec/limit.rs:18): the slice of the stable permutation."""
+    k = len(keys)
+    desc = (C.c_int32 * k)(*[int(bool(x)) for x in (descending or [False] * k)])
+    nf = (C.c_int32 * k)(*[int(bool(x)) for x in (nulls_first or [True] * k)])
+    out = DeviceBuffer(max(limit, 1) * 4)
+    n_out = C.c_int64()
+    st = N.lib().bolt_sort_limit(_stream(stream), columns_array(keys), k, desc, nf, offset, limit, out.ptr,
+                                 C.byref(n_out))
+    N.check(st, "bolt_sort_limit")
+    return out.to_numpy(np.uint32, n_out.value)

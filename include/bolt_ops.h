@@ -302,6 +302,18 @@ int bolt_sort_pairs_i64(bolt_stream_t stream, const int64_t* keys_in, const uint
 int bolt_sort_indices(bolt_stream_t stream, const bolt_column_t* keys, int32_t n_keys,
                       const int32_t* descending, const int32_t* nulls_first, uint32_t* out_rows);
 
+/* ORDER BY ... LIMIT limit OFFSET offset (execute_sort + execute_limit,
+ * src/exec/sort.rs:161 and src/exec/limit.rs:18, which sort every row on
+ * the host and then slice): writes the row ids at positions
+ * [offset, offset + limit) of bolt_sort_indices's permutation for the same
+ * keys, *n_out = min(limit, max(n - offset, 0)) of them, to out_rows
+ * (capacity limit).  A radix select on the leading key picks the rows that
+ * can reach those positions, and only they are sorted by every key, then by
+ * row id, so the result equals the slice of the stable full sort.
+ * n < 2^31.  Synchronises. */
+int bolt_sort_limit(bolt_stream_t stream, const bolt_column_t* keys, int32_t n_keys, const int32_t* descending,
+                    const int32_t* nulls_first, int64_t offset, int64_t limit, uint32_t* out_rows, int64_t* n_out);
+
 /* ------------------------------------------------------------------------
  * Library information
  * ------------------------------------------------------------------------ */

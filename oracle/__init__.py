@@ -231,6 +231,21 @@ def sort_indices(keys: list[HostCol], descending: list[bool], nulls_first: list[
     return out[:n]
 
 
+def limit_window(n: int, limit: int, offset: int) -> tuple[int, int]:
+    """execute_limit (src/exec/limit.rs:18): the [start, end) window of LIMIT/OFFSET over n rows."""
+    if offset >= n:
+        return n, n
+    return offset, offset + min(limit, n - offset)
+
+
+def sort_limit(keys: list[HostCol], descending: list[bool], nulls_first: list[bool], limit: int,
+               offset: int = 0) -> np.ndarray:
+    """ORDER BY + LIMIT as the reference runs it: the full stable sort (sort.rs:161), then the slice."""
+    perm = sort_indices(keys, descending, nulls_first)
+    a, b = limit_window(len(perm), limit, offset)
+    return perm[a:b]
+
+
 def filter_join_agg(build_key, build_payload, probe_key, probe_amount, amount_gt: float):
     """filter (compact.rs:146) -> host join (join.rs:169) -> SUM/SUM/COUNT (aggregate.rs:154)."""
     bk = np.ascontiguousarray(build_key, np.int64)

@@ -1,0 +1,110 @@
+"""GPU parity: ORDER BY + LIMIT / OFFSET (bolt_sort_limit).  The row ids are
+bit-exact against the slice of the oracle's stable sort (execute_sort +
+execute_limit, src/exec/sort.rs:161, src/exec/limit.rs:18) and, at sizes the
+oracle is slow on, of numpy's stable argsort."""
+import numpy as np
+import pytest
+
+import oracle as O
+from craton_bolt_amd import _native as N
+from craton_bolt_amd.device import DeviceColumn
+from craton_bolt_amd.ops import execute_sort, execute_sort_limit
+
+from gpu_util import both, dec_array
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(cols, desc, nf, limit, offset):
+    dev = [d for d, _ in cols]
+    host = [h for _, h in cols]
+    got = execute_sort_limit(dev, limit, offset, desc, nf)
+    want = O.sort_limit(host, desc, nf, limit, offset)
+    assert np.array_equal(got, want), (limit, offset, got[:8], want[:8])
+
+
+@pytest.mark.parametrize("n", [70_000, 1_000_000])
+@pytest.mark.parametrize("limit,offset", [(1, 0), (10, 0), (100, 37), (5000, 1000), (0, 0), (10, 10 ** 9)])
+@pytest.mark.parametrize("desc", [False, True])
+def test_sort_limit_int64_matches_oracle(gpu, n, limit, offset, desc):
+    print(f"[case] sort_limit i64 n={n} limit={limit} offset={offset} desc={desc}", flush=True)
+    rng = np.random.default_rng(n + limit)
+    cols = [both(rng.integers(-(1 << 62), 1 << 62, n).astype(np.int64), N.I64)]
+    _check(cols, [desc], [True], limit, offset)
+
+
+@pytest.mark.parametrize("dtype", ["i32", "f64", "f32", "date32", "dec128"])
+@pytest.mark.parametrize("nulls_first", [True, False])
+@pytest.mark.parametrize("desc", [False, True])
+def test_sort_limit_dtypes_with_nulls(gpu, dtype, nulls_first, desc):
+    print(f"[case] sort_limit {dtype} nulls_first={nulls_first} desc={desc}", flush=True)
+    n = 300_000
+    rng = np.random.default_rng(7)
+    valid = rng.random(n) > 0.001  # about 300 NULLs: a limit of 1000 reaches past them when they come first
+    if dtype == "i32":
+        cols = [both(rng.integers(-10 ** 9, 10 ** 9, n).astype(np.int32), N.I32, valid)]
+    elif dtype == "date32":
+        cols = [both(rng.integers(0, 40_000, n).astype(np.int32), N.DATE32, valid)]
+    elif dtype == "f64":
+        v = rng.standard_normal(n) * 1e6
+        v[rng.integers(0, n, 50)] = np.nan
+        v[rng.integers(0, n, 50)] = -0.0
+        v[rng.integers(0, n, 50)] = 0.0
+        v[rng.integers(0, n, 20)] = np.inf
+        v[rng.integers(0, n, 20)] = -np.inf
+        cols = [both(v, N.F64, valid)]
+    elif dtype == "f32":
+        v = (rng.standard_normal(n) * 1e3).astype(np.float32)
+        v[rng.integers(0, n, 50)] = np.nan
+        v[rng.integers(0, n, 50)] = -0.0
+        cols = [both(v, N.F32, valid)]
+    else:
+        vals = [int(x) * (1 << 70) + int(y) for x, y in zip(rng.integers(-50, 50, n), rng.integers(0, 1 << 62, n))]
+        cols = [both(dec_array(vals), N.DEC128, valid, scale=2)]
+    for limit, offset in ((1, 0), (1000, 0), (200, 250), (50, n - 20)):
+        _check(cols, [desc], [nulls_first], limit, offset)
+
+
+def test_sort_limit_multi_key_ties(gpu):
+    """A leading key with few distinct values leaves large tie buckets; the
+    second key and then the row id order them (the full sort is stable)."""
+    n = 2_000_000
+    rng = np.random.default_rng(11)
+    k0 = rng.integers(0, 1000, n).astype(np.int32)
+    k1 = rng.integers(0, 5, n).astype(np.int64)
+    v1 = rng.random(n) > 0.1
+    cols = [both(k0, N.I32), both(k1, N.I64, v1)]
+    for limit, offset in ((10, 0), (3000, 0), (777, 4000)):
+        for desc in ([False, True], [True, False]):
+            print(f"[case] multi-key ties limit={limit} offset={offset} desc={desc}", flush=True)
+            _check(cols, desc, [True, False], limit, offset)
+
+
+def test_sort_limit_leading_key_constant_falls_back(gpu):
+    """Every row ties on the leading key: the select cannot narrow the rows
+    and the full sort runs; the result is the same slice."""
+    n = 200_000
+    rng = np.random.default_rng(3)
+    cols = [both(np.ones(n, np.uint8), N.BOOL), both(rng.integers(0, 100, n).astype(np.int32), N.I32)]
+    _check(cols, [False, True], [True, True], 100, 5)
+
+
+def test_sort_limit_large_against_numpy(gpu):
+    n = 32 << 20
+    rng = np.random.default_rng(5)
+    key = rng.integers(-(1 << 40), 1 << 40, n).astype(np.int64)
+    dev = DeviceColumn.from_numpy(key, N.I64)
+    for limit, offset, desc in ((100, 0, False), (10_000, 123, True)):
+        print(f"[case] sort_limit 32M limit={limit} offset={offset} desc={desc}", flush=True)
+        got = execute_sort_limit([dev], limit, offset, [desc])
+        want = np.argsort(-key if desc else key, kind="stable")[offset:offset + limit]
+        assert np.array_equal(got, want.astype(np.uint32))
+
+
+def test_sort_limit_equals_full_sort_slice(gpu):
+    """bolt_sort_limit equals the slice of bolt_sort_indices on the same keys."""
+    n = 500_000
+    rng = np.random.default_rng(9)
+    dev = DeviceColumn.from_numpy(rng.integers(0, 50_000, n).astype(np.int64), N.I64)
+    full = execute_sort([dev])
+    assert np.array_equal(execute_sort_limit([dev], 2500, 100), full[100:2600])

@@ -532,6 +532,18 @@ def test_sort_decimal128_orders_as_signed_128_bit():
         assert got.tolist() == want
 
 
+def test_limit_window_fixtures():
+    # src/exec/limit.rs:54-131 unit tests (n, limit, offset) -> rows
+    cases = [((10, 3, 0), [0, 1, 2]), ((10, 3, 2), [2, 3, 4]), ((5, 100, 0), [0, 1, 2, 3, 4]),
+             ((10, 5, 7), [7, 8, 9]), ((4, 10, 4), []), ((4, 10, 999), []), ((10, 0, 0), []), ((0, 10, 0), [])]
+    for (n, limit, offset), want in cases:
+        a, b = O.limit_window(n, limit, offset)
+        assert list(range(n))[a:b] == want
+    v = np.arange(10, dtype=np.int32)
+    got = O.sort_limit([O.HostCol(v, O.I32)], [False], [True], 3, 2)
+    assert got.tolist() == [2, 3, 4]
+
+
 def test_sort_nulls_first_last():
     x = np.array([3, 1, 2, 0], np.int64)
     valid = [1, 1, 1, 0]
