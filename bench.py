@@ -29,7 +29,9 @@ def _parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="join", choices=["join", "sort", "groupby", "groupby-zipf", "calib", "batch", "serve"])
+    p.add_argument("--workload", default="join", choices=["join", "sort", "topk", "groupby", "groupby-zipf", "calib", "batch",
+                                                             "serve"])
+    p.add_argument("--limit", type=int, default=100, help="topk workload: LIMIT")
     p.add_argument("--fact-rows", type=int, default=200_000_000, help="serve workload: fact table rows")
     p.add_argument("--serve-queries", type=int, default=64, help="serve workload: queries per measurement")
     p.add_argument("--slots", type=int, default=8, help="serve workload: concurrent queries")
@@ -246,6 +248,30 @@ def bench_sort(args) -> dict:
                   "n": n, "compulsory_bytes": n * 24, "model_bytes": model,
                   "sort_ms_on_stream": ms, "wall_ms_per_call": wall},
                  _roof(model, ms, "hybrid radix sort (histogram, 2 onesweep passes, LDS bucket sorts), 200 B/key model"))
+
+
+def bench_topk(args) -> dict:
+    """ORDER BY key LIMIT k over 256M int64 keys: one bolt_sort_limit call per
+    step (it synchronises), against one full bolt_sort_indices + slice."""
+    from craton_bolt_amd import _native as N
+    from craton_bolt_amd import datagen as D
+    from craton_bolt_amd.device import DeviceColumn, Stream
+    from craton_bolt_amd.ops import execute_sort, execute_sort_limit
+
+    n, k = args.n, args.limit
+    key, _ = D.sort_pairs(n)
+    col = DeviceColumn.from_numpy(key, N.I64)
+    s = Stream()
+    ms, wall = _timed(s, lambda: execute_sort_limit([col], k, 0, stream=s), args.steps, args.warmup)
+    ms = wall  # the call synchronises the host: its wall time is the step
+    t0 = time.perf_counter()
+    execute_sort([col], stream=s)
+    full_ms = (time.perf_counter() - t0) * 1000.0
+    return _line(n / (ms * 1e-3), "rows/s", args, ms, "int64",
+                 {"workload": f"ORDER BY int64 key LIMIT {k}: {n} rows (uniform over the full range)",
+                  "n": n, "limit": k, "full_sort_then_slice_ms": full_ms},
+                 _roof(n * 8, ms, "radix select (12-bit histogram passes) + candidate collect + candidate sort; "
+                                  "8 B/row (one read of the key) is the compulsory traffic"))
 
 
 def bench_groupby(args, zipf: bool) -> dict:
@@ -591,6 +617,8 @@ def main():
         line = bench_join(args)
     elif args.workload == "sort":
         line = bench_sort(args)
+    elif args.workload == "topk":
+        line = bench_topk(args)
     elif args.workload == "groupby":
         line = bench_groupby(args, zipf=False)
     elif args.workload == "groupby-zipf":

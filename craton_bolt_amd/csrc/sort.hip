@@ -647,7 +647,7 @@ __global__ void order_key_kernel(const void* __restrict__ d, const uint8_t* __re
 
 // ---- ORDER BY + LIMIT: radix select on the leading key ---------------------------------
 //
-// `coarse_image` maps the leading key of a row to a u64 that never decreases
+// `coarse_of` maps the leading key of a row to a u64 that never decreases
 // along the full sort order (NULL rank, direction and value in one word; a
 // Decimal128 by its signed high word).  The K-th smallest image is T; every
 // row among the first K of the order has an image <= T, so sorting only
@@ -659,12 +659,40 @@ struct TopKey {
   int dt, bits, desc, nulls_first;
 };
 
-__device__ __forceinline__ uint64_t coarse_image(const TopKey& k, int64_t row) {
-  const bool valid = bit_valid(k.v, row);
+// Raw bits of row i's key value: W = byte width (16 = Decimal128, whose
+// signed high word is the part of the value the select looks at).
+template <int W>
+__device__ __forceinline__ uint64_t load_raw(const void* d, int64_t i) {
+  if constexpr (W == 1) return static_cast<const uint8_t*>(d)[i];
+  else if constexpr (W == 4) return static_cast<const uint32_t*>(d)[i];
+  else if constexpr (W == 8) return static_cast<const uint64_t*>(d)[i];
+  else return static_cast<const uint64_t*>(d)[2 * i + 1];
+}
+
+// Ascending image of raw key bits (the transforms of radix_image).
+__device__ __forceinline__ uint64_t raw_image(uint64_t b, int dt) {
+  switch (dt) {
+    case BOLT_I32: case BOLT_DATE32: return (uint64_t)((uint32_t)b ^ 0x80000000u);
+    case BOLT_F32: {
+      uint32_t x = (uint32_t)b;
+      if ((x & 0x7FFFFFFFu) > 0x7F800000u) x = 0x7FC00000u;
+      else if ((x & 0x7FFFFFFFu) == 0) x = 0;
+      return (uint64_t)(x ^ ((uint32_t)((int32_t)x >> 31) | 0x80000000u));
+    }
+    case BOLT_F64: {
+      if ((b & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull) b = 0x7FF8000000000000ull;
+      else if ((b & 0x7FFFFFFFFFFFFFFFull) == 0) b = 0;
+      return b ^ ((uint64_t)((int64_t)b >> 63) | 0x8000000000000000ull);
+    }
+    case BOLT_BOOL: return b ? 1 : 0;
+    default: return b ^ 0x8000000000000000ull;  // Int64; Decimal128 high word
+  }
+}
+
+__device__ __forceinline__ uint64_t coarse_of(const TopKey& k, uint64_t raw, bool valid) {
   uint64_t x = 0;
   if (valid) {
-    x = k.dt == BOLT_DEC128 ? static_cast<const uint64_t*>(k.d)[2 * row + 1] ^ 0x8000000000000000ull
-                            : radix_image(k.d, k.dt, row);
+    x = raw_image(raw, k.dt);
     if (k.desc) x = ~x & (k.bits >= 64 ? ~0ull : ((1ull << k.bits) - 1));
   }
   if (k.v == nullptr) return x;
@@ -672,47 +700,92 @@ __device__ __forceinline__ uint64_t coarse_image(const TopKey& k, int64_t row) {
   return k.nulls_first ? (x >> 1) + 1 : x >> 1;
 }
 
-constexpr int kSelBits = 12;
-constexpr int kSelBins = 1 << kSelBits;
+constexpr int kSelBits = 14;
+constexpr int kSelBins = 1 << kSelBits;  // 64 KB of LDS counters
+constexpr int kSelBlock = 1024;
+constexpr int kSelUnroll = 4;  // rows in flight per lane
 
-// Histogram of digit (image >> shift) & (2^width - 1) over the rows whose
-// image >> (shift + width) equals `prefix` (every row when shift + width = 64).
-__global__ __launch_bounds__(256) void topk_hist_kernel(TopKey k, int64_t n, int shift, int width, uint64_t prefix,
-                                                        uint32_t* __restrict__ hist) {
+// Per-workgroup histogram of digit (image >> shift) & (2^width - 1) over the
+// rows whose image >> (shift + width) equals `prefix` (every row when
+// shift + width = 64), written to part[block][bin] with plain stores (a
+// global atomic per bin per workgroup would cost more than the row pass).
+template <int W>
+__global__ __launch_bounds__(kSelBlock) void topk_hist_kernel(TopKey k, int64_t n, int shift, int width,
+                                                              uint64_t prefix, uint32_t* __restrict__ part) {
   __shared__ uint32_t sh[kSelBins];
-  for (int b = threadIdx.x; b < kSelBins; b += 256) sh[b] = 0;
+  const int nb = 1 << width;
+  for (int b = threadIdx.x; b < nb; b += kSelBlock) sh[b] = 0;
   __syncthreads();
   const int hi = shift + width;
-  const uint64_t dmask = (1ull << width) - 1;
-  const int64_t stride = (int64_t)gridDim.x * 256;
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
-    const uint64_t x = coarse_image(k, i);
-    if (hi < 64 && (x >> hi) != prefix) continue;
-    atomicAdd(&sh[(x >> shift) & dmask], 1u);
+  const uint64_t dmask = (uint64_t)nb - 1;
+  const int64_t stride = (int64_t)gridDim.x * kSelBlock;
+  for (int64_t i0 = (int64_t)blockIdx.x * kSelBlock + threadIdx.x; i0 < n; i0 += stride * kSelUnroll) {
+    uint64_t x[kSelUnroll];
+#pragma unroll
+    for (int u = 0; u < kSelUnroll; u++) {
+      const int64_t i = i0 + u * stride;
+      x[u] = i < n ? load_raw<W>(k.d, i) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kSelUnroll; u++) {
+      const int64_t i = i0 + u * stride;
+      if (i >= n) continue;
+      const uint64_t c = coarse_of(k, x[u], bit_valid(k.v, i));
+      if (hi >= 64 || (c >> hi) == prefix) atomicAdd(&sh[(c >> shift) & dmask], 1u);
+    }
   }
   __syncthreads();
-  for (int b = threadIdx.x; b < kSelBins; b += 256)
-    if (sh[b]) atomicAdd(&hist[b], sh[b]);
+  uint32_t* out = part + (int64_t)blockIdx.x * nb;
+  for (int b = threadIdx.x; b < nb; b += kSelBlock) out[b] = sh[b];
+}
+
+// hist[b] += sum of part[w][b] over this launch row's share of workgroups
+// (hist starts at zero; gridDim.y shares split the workgroups)
+__global__ __launch_bounds__(256) void topk_hist_sum_kernel(const uint32_t* __restrict__ part, int blocks, int nb,
+                                                            uint32_t* __restrict__ hist) {
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= nb) return;
+  const int per = (blocks + (int)gridDim.y - 1) / (int)gridDim.y;
+  const int w0 = (int)blockIdx.y * per;
+  const int w1 = w0 + per < blocks ? w0 + per : blocks;
+  uint32_t t = 0;
+  for (int w = w0; w < w1; w++) t += part[(int64_t)w * nb + b];
+  if (t) atomicAdd(&hist[b], t);
 }
 
 // Appends (unordered) the rows whose image >> shift <= bound.
+template <int W>
 __global__ __launch_bounds__(256) void topk_collect_kernel(TopKey k, int64_t n, int shift, uint64_t bound,
                                                            uint32_t* __restrict__ out,
                                                            unsigned long long* __restrict__ count, int64_t cap) {
   const int64_t stride = (int64_t)gridDim.x * 256;
   const int64_t end = (n + 255) / 256 * 256;  // block-uniform trip count for the ballot
   const int lane = threadIdx.x & 63;
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < end; i += stride) {
-    const bool take = i < n && (shift >= 64 ? 0ull : coarse_image(k, i) >> shift) <= bound;
-    const uint64_t m = __ballot(take);
-    if (m == 0) continue;
-    const int leader = __builtin_ctzll(m);
-    unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(m));
-    base = __shfl(base, leader);
-    if (take) {
-      const int64_t pos = (int64_t)base + __popcll(m & ((1ull << lane) - 1));
-      if (pos < cap) out[pos] = (uint32_t)i;
+  for (int64_t i0 = (int64_t)blockIdx.x * 256 + threadIdx.x; i0 < end; i0 += stride * kSelUnroll) {
+    uint64_t x[kSelUnroll];
+#pragma unroll
+    for (int u = 0; u < kSelUnroll; u++) {
+      const int64_t i = i0 + u * stride;
+      x[u] = i < n ? load_raw<W>(k.d, i) : 0;
+    }
+    bool take[kSelUnroll];
+#pragma unroll
+    for (int u = 0; u < kSelUnroll; u++) {
+      const int64_t i = i0 + u * stride;
+      take[u] = i < n && (shift >= 64 ? 0ull : coarse_of(k, x[u], bit_valid(k.v, i)) >> shift) <= bound;
+    }
+#pragma unroll
+    for (int u = 0; u < kSelUnroll; u++) {
+      const uint64_t m = __ballot(take[u]);
+      if (m == 0) continue;
+      const int leader = __builtin_ctzll(m);
+      unsigned long long base = 0;
+      if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(m));
+      base = __shfl(base, leader);
+      if (take[u]) {
+        const int64_t pos = (int64_t)base + __popcll(m & ((1ull << lane) - 1));
+        if (pos < cap) out[pos] = (uint32_t)(i0 + u * stride);
+      }
     }
   }
 }
@@ -860,23 +933,33 @@ extern "C" int bolt_sort_limit(bolt_stream_t stream, const bolt_column_t* keys, 
   tk.bits = c0.dtype == BOLT_BOOL ? 1 : (c0.dtype == BOLT_DEC128 ? 64 : dtype_size(c0.dtype) * 8);
   tk.desc = desc[0];
   tk.nulls_first = nf[0];
+  const int kw = c0.dtype == BOLT_BOOL ? 1 : dtype_size(c0.dtype);  // 16 for Decimal128
   // Radix select, most significant digit first: after each pass `prefix`
   // holds the top 64 - hi bits of T, `below` the rows with a smaller prefix
   // and `eq` those with the same one; stop once the tie bucket is small.
-  DevBuf<uint32_t> hist;
+  const unsigned hgrid = grid_for(n, kSelBlock, kSelUnroll, 512);
+  DevBuf<uint32_t> hist, part;
   BOLT_TRY(hist.alloc(kSelBins, s));
+  BOLT_TRY(part.alloc((size_t)hgrid * kSelBins, s));
   std::vector<uint32_t> h(kSelBins);
   uint64_t prefix = 0;
   int hi = 64;
   int64_t below = 0, need = K, eq = n;
-  const unsigned grid = grid_for(n, 256, 4, 2048);
+  const unsigned grid = grid_for(n, 256, kSelUnroll, 4096);
   while (hi > 0) {
     const int width = hi >= kSelBits ? kSelBits : hi;
     const int shift = hi - width;
-    BOLT_TRY(fill_u32(hist.ptr, 0, kSelBins, s));
-    topk_hist_kernel<<<grid, 256, 0, s>>>(tk, n, shift, width, prefix, hist.ptr);
+    switch (kw) {
+      case 1: topk_hist_kernel<1><<<hgrid, kSelBlock, 0, s>>>(tk, n, shift, width, prefix, part.ptr); break;
+      case 4: topk_hist_kernel<4><<<hgrid, kSelBlock, 0, s>>>(tk, n, shift, width, prefix, part.ptr); break;
+      case 8: topk_hist_kernel<8><<<hgrid, kSelBlock, 0, s>>>(tk, n, shift, width, prefix, part.ptr); break;
+      default: topk_hist_kernel<16><<<hgrid, kSelBlock, 0, s>>>(tk, n, shift, width, prefix, part.ptr); break;
+    }
+    BOLT_TRY(fill_u32(hist.ptr, 0, (size_t)1 << width, s));
+    topk_hist_sum_kernel<<<dim3((unsigned)(((1 << width) + 255) / 256), 32), 256, 0, s>>>(part.ptr, (int)hgrid,
+                                                                                        1 << width, hist.ptr);
     BOLT_HIP(hipGetLastError());
-    BOLT_HIP(hipMemcpyAsync(h.data(), hist.ptr, kSelBins * 4, hipMemcpyDeviceToHost, s));
+    BOLT_HIP(hipMemcpyAsync(h.data(), hist.ptr, (size_t)(1 << width) * 4, hipMemcpyDeviceToHost, s));
     BOLT_HIP(hipStreamSynchronize(s));
     int64_t cum = 0;
     int d = 0;
@@ -899,7 +982,12 @@ extern "C" int bolt_sort_limit(bolt_stream_t stream, const bolt_column_t* keys, 
   BOLT_TRY(cand.alloc((size_t)C, s));
   BOLT_TRY(cnt.alloc(1, s));
   BOLT_TRY(fill_u32(cnt.ptr, 0, 2, s));
-  topk_collect_kernel<<<grid, 256, 0, s>>>(tk, n, hi, prefix, cand.ptr, cnt.ptr, C);
+  switch (kw) {
+    case 1: topk_collect_kernel<1><<<grid, 256, 0, s>>>(tk, n, hi, prefix, cand.ptr, cnt.ptr, C); break;
+    case 4: topk_collect_kernel<4><<<grid, 256, 0, s>>>(tk, n, hi, prefix, cand.ptr, cnt.ptr, C); break;
+    case 8: topk_collect_kernel<8><<<grid, 256, 0, s>>>(tk, n, hi, prefix, cand.ptr, cnt.ptr, C); break;
+    default: topk_collect_kernel<16><<<grid, 256, 0, s>>>(tk, n, hi, prefix, cand.ptr, cnt.ptr, C); break;
+  }
   BOLT_HIP(hipGetLastError());
   unsigned long long got = 0;
   BOLT_HIP(hipMemcpyAsync(&got, cnt.ptr, 8, hipMemcpyDeviceToHost, s));
