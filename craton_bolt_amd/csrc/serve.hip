@@ -13,7 +13,7 @@
 // fits the budget next to the queries already running; a query starts at the
 // current chunk, wraps around, and completes after one full lap, when its
 // groups are extracted, ordered (ORDER BY aggregate DESC, group key ASC, via
-// the radix-sort permutation of bolt_sort_indices) and the first LIMIT rows
+// bolt_sort_limit: the first LIMIT rows of the radix-sort order) and those rows
 // are copied out.  Its slot and memory are then free for the next request.
 #include <algorithm>
 #include <cmath>
@@ -418,16 +418,6 @@ static void compile_intervals(SQ& d) {
   }
 }
 
-// Decimal128 -> two signed 64-bit sort keys with the same order: (hi, lo ^ sign bit)
-__global__ void dec_split_kernel(const uint64_t* __restrict__ dec, int64_t* __restrict__ hi,
-                                 int64_t* __restrict__ lo, int64_t n) {
-  const int64_t st = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += st) {
-    lo[i] = (int64_t)(dec[2 * i] ^ 0x8000000000000000ull);
-    hi[i] = (int64_t)dec[2 * i + 1];
-  }
-}
-
 __global__ void serve_gather_kernel(const uint8_t* __restrict__ src, int width, const uint32_t* __restrict__ perm,
                                     int64_t n, uint8_t* __restrict__ dst) {
   const int64_t st = (int64_t)gridDim.x * blockDim.x;
@@ -658,32 +648,24 @@ struct bolt_server {
       // ORDER BY aggs[order_agg] DESC NULLS LAST, then the group key ascending
       std::vector<bolt_column_t> sk;
       std::vector<int32_t> desc, nf;
-      DevBuf<int64_t> hi, lo;
       const int oc = q.n_group + q.order_agg;
       const uint8_t* ov = reinterpret_cast<const uint8_t*>(vcols[oc].ptr);
-      if (dts[oc] == BOLT_DEC128) {
-        BOLT_TRY(hi.alloc(n_groups, s));
-        BOLT_TRY(lo.alloc(n_groups, s));
-        dec_split_kernel<<<grid_for(n_groups, 256, 4), 256, 0, s>>>(reinterpret_cast<const uint64_t*>(cols[oc].ptr),
-                                                                    hi.ptr, lo.ptr, n_groups);
-        BOLT_HIP(hipGetLastError());
-        sk.push_back(bolt_column_t{hi.ptr, ov, n_groups, BOLT_I64, 0});
-        sk.push_back(bolt_column_t{lo.ptr, ov, n_groups, BOLT_I64, 0});
-        desc.insert(desc.end(), {1, 1});
-        nf.insert(nf.end(), {0, 0});
-      } else {
-        sk.push_back(bolt_column_t{cols[oc].ptr, ov, n_groups, dts[oc], 0});
-        desc.push_back(1);
-        nf.push_back(0);
-      }
+      sk.push_back(bolt_column_t{cols[oc].ptr, ov, n_groups, dts[oc], 0});
+      desc.push_back(1);
+      nf.push_back(0);
       for (int g = 0; g < q.n_group; g++) {
         sk.push_back(bolt_column_t{cols[g].ptr, nullptr, n_groups, dts[g], 0});
         desc.push_back(0);
         nf.push_back(0);
       }
+      // the first L rows of the order only (a radix select above 65536 groups)
       DevBuf<uint32_t> perm;
-      BOLT_TRY(perm.alloc(n_groups, s));
-      BOLT_TRY(bolt_sort_indices((bolt_stream_t)s, sk.data(), (int32_t)sk.size(), desc.data(), nf.data(), perm.ptr));
+      int64_t got = 0;
+      BOLT_TRY(perm.alloc(L, s));
+      BOLT_TRY(bolt_sort_limit((bolt_stream_t)s, sk.data(), (int32_t)sk.size(), desc.data(), nf.data(), 0, L,
+                               perm.ptr, &got));
+      if (got != L) return set_error(BOLT_ERR_OTHER, "serve: ORDER BY + LIMIT returned %lld of %lld rows",
+                                     (long long)got, (long long)L);
       // gather the first L rows of every column, then one copy each
       size_t row_bytes = 0;
       for (int c = 0; c < nc; c++) row_bytes += dtype_size(dts[c]);

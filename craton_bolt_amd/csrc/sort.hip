@@ -916,8 +916,9 @@ extern "C" int bolt_sort_limit(bolt_stream_t stream, const bolt_column_t* keys, 
     *n_out = m;
     return BOLT_OK;
   };
-  // small inputs, or a limit that reaches most rows
-  if (n <= (1 << 16) || K * 4 >= n) return full_sort();
+  // inputs of up to 2^20 rows (where the sort is launch-bound and a select
+  // saves nothing), or a limit that reaches a quarter of the rows
+  if (n <= (1 << 20) || K * 4 >= n) return full_sort();
   for (int k = 0; k < n_keys; k++) {
     if (keys[k].length != n) return set_error(BOLT_ERR_PLAN, "bolt_sort_limit: key columns differ in length");
     const int dt = keys[k].dtype;

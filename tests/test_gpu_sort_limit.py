@@ -23,7 +23,7 @@ def _check(cols, desc, nf, limit, offset):
     assert np.array_equal(got, want), (limit, offset, got[:8], want[:8])
 
 
-@pytest.mark.parametrize("n", [70_000, 1_000_000])
+@pytest.mark.parametrize("n", [70_000, 1_500_000])
 @pytest.mark.parametrize("limit,offset", [(1, 0), (10, 0), (100, 37), (5000, 1000), (0, 0), (10, 10 ** 9)])
 @pytest.mark.parametrize("desc", [False, True])
 def test_sort_limit_int64_matches_oracle(gpu, n, limit, offset, desc):
@@ -38,9 +38,9 @@ def test_sort_limit_int64_matches_oracle(gpu, n, limit, offset, desc):
 @pytest.mark.parametrize("desc", [False, True])
 def test_sort_limit_dtypes_with_nulls(gpu, dtype, nulls_first, desc):
     print(f"[case] sort_limit {dtype} nulls_first={nulls_first} desc={desc}", flush=True)
-    n = 300_000
+    n = 1_200_000
     rng = np.random.default_rng(7)
-    valid = rng.random(n) > 0.001  # about 300 NULLs: a limit of 1000 reaches past them when they come first
+    valid = rng.random(n) > 0.00025  # about 300 NULLs: a limit of 1000 reaches past them when they come first
     if dtype == "i32":
         cols = [both(rng.integers(-10 ** 9, 10 ** 9, n).astype(np.int32), N.I32, valid)]
     elif dtype == "date32":
@@ -83,7 +83,7 @@ def test_sort_limit_multi_key_ties(gpu):
 def test_sort_limit_leading_key_constant_falls_back(gpu):
     """Every row ties on the leading key: the select cannot narrow the rows
     and the full sort runs; the result is the same slice."""
-    n = 200_000
+    n = 1_200_000
     rng = np.random.default_rng(3)
     cols = [both(np.ones(n, np.uint8), N.BOOL), both(rng.integers(0, 100, n).astype(np.int32), N.I32)]
     _check(cols, [False, True], [True, True], 100, 5)
@@ -103,7 +103,7 @@ def test_sort_limit_large_against_numpy(gpu):
 
 def test_sort_limit_equals_full_sort_slice(gpu):
     """bolt_sort_limit equals the slice of bolt_sort_indices on the same keys."""
-    n = 500_000
+    n = 3_000_000
     rng = np.random.default_rng(9)
     dev = DeviceColumn.from_numpy(rng.integers(0, 50_000, n).astype(np.int64), N.I64)
     full = execute_sort([dev])
