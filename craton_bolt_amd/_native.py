@@ -64,6 +64,15 @@ class BoltScalar(C.Structure):
     ]
 
 
+class BoltExprOp(C.Structure):
+    """bolt_expr_op_t: one op of a per-row expression program."""
+    _fields_ = [
+        ("kind", C.c_int32), ("dst", C.c_int32), ("a", C.c_int32), ("b", C.c_int32), ("c", C.c_int32),
+        ("col", C.c_int32), ("op", C.c_int32), ("dtype", C.c_int32),
+        ("lit_i64", C.c_int64), ("lit_f64", C.c_double), ("lit_hi", C.c_int64),
+    ]
+
+
 class BoltPredTerm(C.Structure):
     _fields_ = [
         ("col", C.c_int32),
@@ -234,6 +243,8 @@ _SIGS = {
     "bolt_filter_compact": (C.c_int, [C.c_void_p, C.POINTER(BoltColumn), C.c_int32, C.POINTER(BoltFilter),
                                       C.c_void_p, C.POINTER(BoltColumn), C.POINTER(C.c_int64)]),
     "bolt_prefix_scan_mask": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(C.c_int64)]),
+    "bolt_project": (C.c_int, [C.c_void_p, C.POINTER(BoltColumn), C.c_int32, C.POINTER(BoltExprOp), C.c_int32,
+                               C.POINTER(BoltColumn), C.c_int32]),
     "bolt_take": (C.c_int, [C.c_void_p, C.POINTER(BoltColumn), C.c_void_p, C.c_int64, C.POINTER(BoltColumn)]),
     "bolt_groupby": (C.c_int, [C.c_void_p, C.POINTER(BoltColumn), C.c_int32, C.POINTER(BoltColumn), C.c_int32,
                                C.POINTER(BoltAgg), C.c_int32, C.POINTER(BoltFilter), C.c_uint32,

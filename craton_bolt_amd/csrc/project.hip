@@ -1,0 +1,387 @@
+// project.hip — per-row expression programs (bolt_project).
+//
+// Replaces the projection kernel of Engine::execute_projection
+// (src/exec/engine.rs:4267), which the reference lowers from a KernelSpec's
+// Op list (src/plan/physical_plan.rs:81) to PTX (src/jit/ptx_gen.rs).  Here
+// the program is type-checked on the host, printed as one HIP kernel (one C
+// statement per op, registers as typed locals, Decimal128 as __int128), and
+// compiled with hipRTC; the code object is cached by the program's text, so a
+// repeated query shape compiles once.  The kernel is a plain grid-stride pass:
+// every input column is read once and every output written once, so it runs at
+// the HBM bound of its columns.
+#include <cstdint>
+#include <cstring>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "bolt_internal.h"
+#include "jit.h"
+
+namespace bolt {
+namespace {
+
+struct RegInfo {
+  int dtype = 0;       // 0 = not yet written
+  uint32_t deps = 0;   // input columns whose values reach the register (bit per input)
+};
+
+const char* ctype(int dt) {
+  switch (dt) {
+    case BOLT_BOOL: case BOLT_I32: case BOLT_DATE32: return "int";
+    case BOLT_I64: return "long long";
+    case BOLT_F32: return "float";
+    case BOLT_F64: return "double";
+    case BOLT_DEC128: return "__int128";
+  }
+  return "?";
+}
+
+// Storage type of a column element.
+const char* stype(int dt) {
+  switch (dt) {
+    case BOLT_BOOL: return "unsigned char";
+    case BOLT_I32: case BOLT_DATE32: return "int";
+    case BOLT_I64: return "long long";
+    case BOLT_F32: return "float";
+    case BOLT_F64: return "double";
+  }
+  return "?";
+}
+
+bool is_int(int dt) { return dt == BOLT_I32 || dt == BOLT_I64 || dt == BOLT_DATE32; }
+bool is_float(int dt) { return dt == BOLT_F32 || dt == BOLT_F64; }
+bool supported(int dt) {
+  return dt == BOLT_BOOL || dt == BOLT_I32 || dt == BOLT_I64 || dt == BOLT_F32 || dt == BOLT_F64 ||
+         dt == BOLT_DATE32 || dt == BOLT_DEC128;
+}
+bool is_cmp(int op) { return op >= BOLT_BIN_EQ && op <= BOLT_BIN_GE; }
+
+// Functions the generated kernels call (guarded integer division and
+// remainder, saturating float -> int conversion, ordered comparisons).
+const char* kProjectPrelude = R"BOLTP(
+#pragma clang fp contract(off)
+typedef unsigned long long u64;
+__device__ __forceinline__ int bdiv32(int a, int b) { return b == 0 ? 0 : (a == (-2147483647 - 1) && b == -1) ? a : a / b; }
+__device__ __forceinline__ long long bdiv64(long long a, long long b) {
+  return b == 0 ? 0 : (a == (-9223372036854775807LL - 1) && b == -1) ? a : a / b;
+}
+__device__ __forceinline__ int bmod32(int a, int b) { return (b == 0 || b == -1) ? 0 : a % b; }
+__device__ __forceinline__ long long bmod64(long long a, long long b) { return (b == 0 || b == -1) ? 0 : a % b; }
+__device__ __forceinline__ int f2i32(double x) {
+  if (!(x == x)) return 0;
+  if (x >= 2147483647.0) return 2147483647;
+  if (x <= -2147483648.0) return -2147483647 - 1;
+  return (int)x;
+}
+__device__ __forceinline__ long long f2i64(double x) {
+  if (!(x == x)) return 0;
+  if (x >= 9223372036854775807.0) return 9223372036854775807LL;
+  if (x <= -9223372036854775808.0) return -9223372036854775807LL - 1;
+  return (long long)x;
+}
+__device__ __forceinline__ int fne(double a, double b) { return a < b || a > b; }
+__device__ __forceinline__ int bitv(const unsigned char* v, long long i) { return (v[i >> 3] >> (i & 7)) & 1; }
+struct PArgs {
+  const void* in[16];
+  const unsigned char* iv[16];
+  void* out[16];
+  unsigned char* ov[16];
+  long long n;
+};
+)BOLTP";
+
+std::string reg(int r) { return "r" + std::to_string(r); }
+
+std::string literal(const bolt_expr_op_t& o) {
+  char buf[96];
+  switch (o.dtype) {
+    case BOLT_BOOL: return o.lit_i64 ? "1" : "0";
+    case BOLT_I32: case BOLT_DATE32:
+      snprintf(buf, sizeof buf, "(int)%lldLL", (long long)(int32_t)o.lit_i64);
+      return buf;
+    case BOLT_I64:
+      snprintf(buf, sizeof buf, "(long long)0x%016llxULL", (unsigned long long)o.lit_i64);
+      return buf;
+    case BOLT_F32: {
+      float f = (float)o.lit_f64;
+      uint32_t b;
+      memcpy(&b, &f, 4);
+      snprintf(buf, sizeof buf, "__builtin_bit_cast(float, 0x%08xu)", b);
+      return buf;
+    }
+    case BOLT_F64: {
+      uint64_t b;
+      memcpy(&b, &o.lit_f64, 8);
+      snprintf(buf, sizeof buf, "__builtin_bit_cast(double, 0x%016llxULL)", (unsigned long long)b);
+      return buf;
+    }
+    case BOLT_DEC128:
+      snprintf(buf, sizeof buf, "(__int128)(((unsigned __int128)0x%016llxULL << 64) | 0x%016llxULL)",
+               (unsigned long long)o.lit_hi, (unsigned long long)o.lit_i64);
+      return buf;
+  }
+  return "0";
+}
+
+// C expression for `(to) src` under ptx_gen.rs's cast rules (:1890-1990).
+int cast_expr(int from, int to, const std::string& src, std::string* out) {
+  if (from == to) { *out = src; return BOLT_OK; }
+  if (from == BOLT_DEC128 || to == BOLT_DEC128 || from == BOLT_DATE32 || to == BOLT_DATE32)
+    return set_error(BOLT_ERR_TYPE, "bolt_project: no cast from %s to %s", dtype_name(from), dtype_name(to));
+  if (to == BOLT_BOOL) {
+    *out = is_float(from) ? "fne((double)" + src + ", 0.0)" : "(" + src + " != 0 ? 1 : 0)";
+    return BOLT_OK;
+  }
+  if (is_float(from) && (to == BOLT_I32 || to == BOLT_I64)) {
+    *out = std::string(to == BOLT_I32 ? "f2i32" : "f2i64") + "((double)" + src + ")";
+    return BOLT_OK;
+  }
+  *out = std::string("(") + ctype(to) + ")" + src;  // Int / Bool -> Int / Float, float widening / narrowing
+  return BOLT_OK;
+}
+
+int binary_expr(int op, int dt, const std::string& a, const std::string& b, std::string* out, int* rdt) {
+  const bool i = is_int(dt), f = is_float(dt), d = dt == BOLT_DEC128, bl = dt == BOLT_BOOL;
+  const char* u = dt == BOLT_I64 ? "u64" : d ? "unsigned __int128" : "unsigned";
+  *rdt = dt;
+  auto wrap = [&](const char* sym) {
+    return std::string("(") + ctype(dt) + ")((" + u + ")" + a + " " + sym + " (" + u + ")" + b + ")";
+  };
+  switch (op) {
+    case BOLT_BIN_ADD: case BOLT_BIN_SUB: case BOLT_BIN_MUL: {
+      const char* sym = op == BOLT_BIN_ADD ? "+" : op == BOLT_BIN_SUB ? "-" : "*";
+      if (dt == BOLT_DATE32) {
+        if (op != BOLT_BIN_SUB) break;  // Date32 - Date32 -> Int32 days only
+        *out = wrap(sym);
+        *rdt = BOLT_I32;
+        return BOLT_OK;
+      }
+      if (f) *out = "(" + a + " " + sym + " " + b + ")";
+      else if (i || d) *out = wrap(sym);
+      else break;
+      return BOLT_OK;
+    }
+    case BOLT_BIN_DIV: case BOLT_BIN_MOD:
+      if (f && op == BOLT_BIN_DIV) { *out = "(" + a + " / " + b + ")"; return BOLT_OK; }
+      if (dt == BOLT_I32 || dt == BOLT_I64) {
+        *out = std::string(op == BOLT_BIN_DIV ? "bdiv" : "bmod") + (dt == BOLT_I32 ? "32(" : "64(") + a + ", " + b + ")";
+        return BOLT_OK;
+      }
+      break;
+    case BOLT_BIN_AND: case BOLT_BIN_OR:
+      if (!bl) break;
+      *out = "(" + a + (op == BOLT_BIN_AND ? " & " : " | ") + b + ")";
+      return BOLT_OK;
+    case BOLT_BIN_BITAND: case BOLT_BIN_BITOR:
+      if (!(dt == BOLT_I32 || dt == BOLT_I64)) break;
+      *out = "(" + a + (op == BOLT_BIN_BITAND ? " & " : " | ") + b + ")";
+      return BOLT_OK;
+    default:
+      if (!is_cmp(op)) return set_error(BOLT_ERR_PLAN, "bolt_project: unknown binary op %d", op);
+      *rdt = BOLT_BOOL;
+      if (f && op == BOLT_BIN_NE) { *out = "fne(" + a + ", " + b + ")"; return BOLT_OK; }
+      {
+        static const char* syms[] = {"==", "!=", "<", "<=", ">", ">="};
+        *out = "(" + a + " " + syms[op - BOLT_BIN_EQ] + " " + b + " ? 1 : 0)";
+      }
+      return BOLT_OK;
+  }
+  return set_error(BOLT_ERR_TYPE, "bolt_project: binary op %d not defined on %s", op, dtype_name(dt));
+}
+
+// Type-checks the program and prints the kernel.
+int gen_source(const bolt_column_t* in, int n_in, const bolt_expr_op_t* prog, int n_ops, const bolt_column_t* out,
+               int n_out, std::string* src) {
+  std::vector<RegInfo> regs(BOLT_MAX_REGS);
+  std::vector<int> stored(n_out, 0);
+  std::ostringstream body, stores, valid;
+  auto use = [&](int r, const char* what) -> int {
+    if (r < 0 || r >= BOLT_MAX_REGS || regs[r].dtype == 0)
+      return set_error(BOLT_ERR_PLAN, "bolt_project: %s reads register %d before it is written", what, r);
+    return BOLT_OK;
+  };
+  for (int k = 0; k < n_ops; k++) {
+    const bolt_expr_op_t& o = prog[k];
+    if (o.kind != BOLT_X_STORE && (o.dst < 0 || o.dst >= BOLT_MAX_REGS || regs[o.dst].dtype != 0))
+      return set_error(BOLT_ERR_PLAN, "bolt_project: op %d writes register %d twice or out of range", k, o.dst);
+    RegInfo& d = regs[o.kind == BOLT_X_STORE ? 0 : o.dst];
+    std::string e;
+    switch (o.kind) {
+      case BOLT_X_LOAD: {
+        if (o.col < 0 || o.col >= n_in) return set_error(BOLT_ERR_PLAN, "bolt_project: op %d loads column %d", k, o.col);
+        const int dt = in[o.col].dtype;
+        if (!supported(dt) || dt != o.dtype)
+          return set_error(BOLT_ERR_TYPE, "bolt_project: op %d loads column %d (%s) as %s", k, o.col, dtype_name(dt),
+                           dtype_name(o.dtype));
+        if (dt == BOLT_DEC128)
+          e = "(__int128)(((unsigned __int128)((const u64*)p.in[" + std::to_string(o.col) + "])[2 * r + 1] << 64) | ((const u64*)p.in[" +
+              std::to_string(o.col) + "])[2 * r])";
+        else if (dt == BOLT_BOOL)
+          e = "(((const unsigned char*)p.in[" + std::to_string(o.col) + "])[r] != 0 ? 1 : 0)";
+        else
+          e = std::string("((const ") + stype(dt) + "*)p.in[" + std::to_string(o.col) + "])[r]";
+        d.dtype = dt;
+        d.deps = 1u << o.col;
+        break;
+      }
+      case BOLT_X_CONST:
+        if (!supported(o.dtype)) return set_error(BOLT_ERR_TYPE, "bolt_project: op %d constant of dtype %d", k, o.dtype);
+        e = literal(o);
+        d.dtype = o.dtype;
+        break;
+      case BOLT_X_CAST: {
+        BOLT_TRY(use(o.a, "CAST"));
+        if (!supported(o.dtype)) return set_error(BOLT_ERR_TYPE, "bolt_project: op %d casts to dtype %d", k, o.dtype);
+        BOLT_TRY(cast_expr(regs[o.a].dtype, o.dtype, reg(o.a), &e));
+        d.dtype = o.dtype;
+        d.deps = regs[o.a].deps;
+        break;
+      }
+      case BOLT_X_BINARY: {
+        BOLT_TRY(use(o.a, "BINARY"));
+        BOLT_TRY(use(o.b, "BINARY"));
+        if (regs[o.a].dtype != o.dtype || regs[o.b].dtype != o.dtype)
+          return set_error(BOLT_ERR_TYPE, "bolt_project: op %d operands are %s and %s, op says %s (cast first)", k,
+                           dtype_name(regs[o.a].dtype), dtype_name(regs[o.b].dtype), dtype_name(o.dtype));
+        int rdt = 0;
+        BOLT_TRY(binary_expr(o.op, o.dtype, reg(o.a), reg(o.b), &e, &rdt));
+        d.dtype = rdt;
+        d.deps = regs[o.a].deps | regs[o.b].deps;
+        break;
+      }
+      case BOLT_X_SELECT:
+        BOLT_TRY(use(o.a, "SELECT"));
+        BOLT_TRY(use(o.b, "SELECT"));
+        BOLT_TRY(use(o.c, "SELECT"));
+        if (regs[o.a].dtype != BOLT_BOOL || regs[o.b].dtype != regs[o.c].dtype)
+          return set_error(BOLT_ERR_TYPE, "bolt_project: op %d SELECT needs a Bool condition and equal value dtypes", k);
+        e = "(" + reg(o.a) + " ? " + reg(o.b) + " : " + reg(o.c) + ")";
+        d.dtype = regs[o.b].dtype;
+        d.deps = regs[o.a].deps | regs[o.b].deps | regs[o.c].deps;
+        break;
+      case BOLT_X_NOT:
+        BOLT_TRY(use(o.a, "NOT"));
+        if (regs[o.a].dtype != BOLT_BOOL) return set_error(BOLT_ERR_TYPE, "bolt_project: op %d NOT of a non-Bool", k);
+        e = "(" + reg(o.a) + " ? 0 : 1)";
+        d.dtype = BOLT_BOOL;
+        d.deps = regs[o.a].deps;
+        break;
+      case BOLT_X_IS_NULL:
+        if (o.col < 0 || o.col >= n_in) return set_error(BOLT_ERR_PLAN, "bolt_project: op %d tests column %d", k, o.col);
+        if (in[o.col].validity)
+          e = std::string("(bitv(p.iv[") + std::to_string(o.col) + "], r) ? " + (o.op ? "0 : 1)" : "1 : 0)");
+        else
+          e = o.op ? "0" : "1";  // a column without a bitmap has no NULLs
+        d.dtype = BOLT_BOOL;
+        break;
+      case BOLT_X_STORE: {
+        BOLT_TRY(use(o.a, "STORE"));
+        if (o.col < 0 || o.col >= n_out) return set_error(BOLT_ERR_PLAN, "bolt_project: op %d stores column %d", k, o.col);
+        if (stored[o.col]) return set_error(BOLT_ERR_PLAN, "bolt_project: output %d is stored twice", o.col);
+        const int dt = out[o.col].dtype;
+        if (dt != regs[o.a].dtype)
+          return set_error(BOLT_ERR_TYPE, "bolt_project: output %d is %s, its value %s", o.col, dtype_name(dt),
+                           dtype_name(regs[o.a].dtype));
+        stored[o.col] = 1;
+        const std::string oc = std::to_string(o.col);
+        if (dt == BOLT_DEC128)
+          stores << "      ((u64*)p.out[" << oc << "])[2 * i] = (u64)" << reg(o.a) << "; ((u64*)p.out[" << oc
+                 << "])[2 * i + 1] = (u64)((unsigned __int128)" << reg(o.a) << " >> 64);\n";
+        else
+          stores << "      ((" << stype(dt) << "*)p.out[" << oc << "])[i] = (" << stype(dt) << ")" << reg(o.a) << ";\n";
+        if (out[o.col].validity) {
+          valid << "    { int ok = in;";
+          for (int c = 0; c < n_in; c++)
+            if ((regs[o.a].deps >> c) & 1u && in[c].validity) valid << " ok = ok && bitv(p.iv[" << c << "], r);";
+          valid << "\n      const u64 m = __ballot(ok);\n"
+                << "      if ((threadIdx.x & 63) == 0)\n"
+                << "        for (int k = 0; k < 8 && (i >> 3) + k < nbytes; k++) p.ov[" << oc
+                << "][(i >> 3) + k] = (unsigned char)(m >> (8 * k)); }\n";
+        }
+        continue;
+      }
+      default: return set_error(BOLT_ERR_PLAN, "bolt_project: op %d has unknown kind %d", k, o.kind);
+    }
+    body << "    const " << ctype(d.dtype) << " " << reg(o.dst) << " = " << e << ";\n";
+  }
+  for (int c = 0; c < n_out; c++)
+    if (!stored[c]) return set_error(BOLT_ERR_PLAN, "bolt_project: output %d is never stored", c);
+  std::ostringstream s;
+  s << kProjectPrelude
+    << "extern \"C\" __global__ void __launch_bounds__(256) bolt_jit_kernel(PArgs p) {\n"
+    << "  const long long stride = (long long)gridDim.x * 256;\n"
+    << "  const long long end = (p.n + 63) / 64 * 64;  // wave-uniform trip count for the validity ballots\n"
+    << "  const long long nbytes = (p.n + 7) / 8;\n"
+    << "  (void)nbytes;\n"
+    << "  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < end; i += stride) {\n"
+    << "    const int in = i < p.n;\n"
+    << "    const long long r = in ? i : 0;  // rows past the end load row 0 and store nothing\n"
+    << body.str() << "    if (in) {\n"
+    << stores.str() << "    }\n"
+    << valid.str() << "  }\n}\n";
+  *src = s.str();
+  return BOLT_OK;
+}
+
+struct PArgs {
+  const void* in[16];
+  const unsigned char* iv[16];
+  void* out[16];
+  unsigned char* ov[16];
+  long long n;
+};
+
+}  // namespace
+}  // namespace bolt
+
+using namespace bolt;
+
+extern "C" int bolt_project(bolt_stream_t stream, const bolt_column_t* inputs, int32_t n_inputs,
+                            const bolt_expr_op_t* prog, int32_t n_ops, bolt_column_t* outputs, int32_t n_outputs) {
+  BOLT_CHECK_DEVICE();
+  clear_error();
+  hipStream_t s = (hipStream_t)stream;
+  if (n_outputs < 1 || n_outputs > BOLT_MAX_PROJECT_OUTPUTS || outputs == nullptr)
+    return set_error(BOLT_ERR_PLAN, "bolt_project: 1 to %d outputs", BOLT_MAX_PROJECT_OUTPUTS);
+  if (n_inputs < 0 || n_inputs > BOLT_MAX_PROJECT_INPUTS || (n_inputs > 0 && inputs == nullptr))
+    return set_error(BOLT_ERR_PLAN, "bolt_project: 0 to %d inputs", BOLT_MAX_PROJECT_INPUTS);
+  if (n_ops < 1 || prog == nullptr) return set_error(BOLT_ERR_PLAN, "bolt_project: empty program");
+  const int64_t n = outputs[0].length;
+  if (n < 0) return set_error(BOLT_ERR_PLAN, "bolt_project: negative length");
+  for (int c = 0; c < n_inputs; c++) {
+    if (inputs[c].length != n)
+      return set_error(BOLT_ERR_PLAN, "bolt_project: input %d has %lld rows, output 0 has %lld", c,
+                       (long long)inputs[c].length, (long long)n);
+    if (n > 0 && inputs[c].data == nullptr) return set_error(BOLT_ERR_PLAN, "bolt_project: input %d has no data", c);
+  }
+  for (int c = 0; c < n_outputs; c++) {
+    if (outputs[c].length != n)
+      return set_error(BOLT_ERR_PLAN, "bolt_project: output %d has %lld rows, output 0 has %lld", c,
+                       (long long)outputs[c].length, (long long)n);
+    if (!supported(outputs[c].dtype))
+      return set_error(BOLT_ERR_TYPE, "bolt_project: output %d dtype %d", c, outputs[c].dtype);
+    if (n > 0 && outputs[c].data == nullptr) return set_error(BOLT_ERR_PLAN, "bolt_project: output %d has no data", c);
+  }
+  std::string src;
+  BOLT_TRY(gen_source(inputs, n_inputs, prog, n_ops, outputs, n_outputs, &src));
+  if (n == 0) return BOLT_OK;
+  hipFunction_t fn;
+  BOLT_TRY(jit_get(src, &fn));
+  PArgs a;
+  memset(&a, 0, sizeof a);
+  for (int c = 0; c < n_inputs; c++) {
+    a.in[c] = inputs[c].data;
+    a.iv[c] = inputs[c].validity;
+  }
+  for (int c = 0; c < n_outputs; c++) {
+    a.out[c] = outputs[c].data;
+    a.ov[c] = const_cast<uint8_t*>(outputs[c].validity);
+  }
+  a.n = n;
+  void* args[] = {&a};
+  const unsigned grid = grid_for(n, 256, 4, 256 * 32);
+  BOLT_HIP(hipModuleLaunchKernel(fn, grid, 1, 1, 256, 1, 1, 0, s, args, nullptr));
+  BOLT_HIP(hipStreamSynchronize(s));
+  return BOLT_OK;
+}

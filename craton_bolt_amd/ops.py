@@ -527,6 +527,83 @@ def execute_inner_join(left_key: DeviceColumn, right_key: DeviceColumn, capacity
         return ol.to_numpy(np.uint32, k), orr.to_numpy(np.uint32, k)
 
 
+# ---- per-row expression programs (src/plan/physical_plan.rs:81 Op) ---------------------------
+
+X_LOAD, X_CONST, X_CAST, X_BINARY, X_SELECT, X_NOT, X_IS_NULL, X_STORE = range(1, 9)
+BIN_OPS = {"+": 1, "-": 2, "*": 3, "/": 4, "%": 5, "==": 6, "!=": 7, "<": 8, "<=": 9, ">": 10, ">=": 11,
+           "and": 12, "or": 13, "&": 14, "|": 15}
+
+
+class Program:
+    """Builder of a per-row expression program for bolt_project: each method
+    appends one op and returns the register it writes (registers in order)."""
+
+    def __init__(self):
+        self.ops: list[dict] = []
+        self._next = 0
+
+    def _emit(self, kind: int, **kw) -> int:
+        op = dict(kind=kind, dst=0, a=0, b=0, c=0, col=0, op=0, dtype=0, lit_i64=0, lit_f64=0.0, lit_hi=0)
+        if kind != X_STORE:
+            op["dst"] = self._next
+            self._next += 1
+        op.update(kw)
+        self.ops.append(op)
+        return op["dst"]
+
+    def load(self, col: int, dtype: int) -> int:
+        return self._emit(X_LOAD, col=col, dtype=dtype)
+
+    def const(self, value, dtype: int) -> int:
+        if dtype in (N.F32, N.F64):
+            return self._emit(X_CONST, dtype=dtype, lit_f64=float(value))
+        v = int(value)
+        if dtype == N.DEC128:
+            m = (1 << 64) - 1
+            lo, hi = v & m, (v >> 64) & m
+            return self._emit(X_CONST, dtype=dtype, lit_i64=lo - (1 << 64) if lo >> 63 else lo,
+                              lit_hi=hi - (1 << 64) if hi >> 63 else hi)
+        return self._emit(X_CONST, dtype=dtype, lit_i64=v)
+
+    def cast(self, a: int, dtype: int) -> int:
+        return self._emit(X_CAST, a=a, dtype=dtype)
+
+    def binary(self, op: str, a: int, b: int, dtype: int) -> int:
+        return self._emit(X_BINARY, a=a, b=b, op=BIN_OPS[op], dtype=dtype)
+
+    def select(self, cond: int, then: int, other: int) -> int:
+        return self._emit(X_SELECT, a=cond, b=then, c=other)
+
+    def not_(self, a: int) -> int:
+        return self._emit(X_NOT, a=a)
+
+    def is_null(self, col: int, want_null: bool = True) -> int:
+        return self._emit(X_IS_NULL, col=col, op=int(want_null))
+
+    def store(self, a: int, col: int) -> None:
+        self._emit(X_STORE, a=a, col=col)
+
+    def array(self, struct):
+        arr = (struct * len(self.ops))()
+        for i, op in enumerate(self.ops):
+            for k, v in op.items():
+                setattr(arr[i], k, v)
+        return arr
+
+
+def execute_projection(inputs: list[DeviceColumn], program: Program, out_dtypes: list[int],
+                       out_validity: list[bool] | None = None, stream: Stream | None = None) -> list[DeviceColumn]:
+    """Runs `program` over every row of `inputs` (bolt_project); returns the output columns."""
+    n = inputs[0].length if inputs else 0
+    outs = [DeviceColumn.empty(n, dt, with_validity=bool(out_validity and out_validity[i]))
+            for i, dt in enumerate(out_dtypes)]
+    oc = columns_array(outs)
+    st = N.lib().bolt_project(_stream(stream), columns_array(inputs), len(inputs), program.array(N.BoltExprOp),
+                              len(program.ops), oc, len(outs))
+    N.check(st, "bolt_project")
+    return outs
+
+
 JOIN_TYPES = {"inner": 0, "left": 1, "right": 2, "full": 3}
 NULL_ROW = 0xFFFFFFFF
 

@@ -315,6 +315,62 @@ int bolt_sort_limit(bolt_stream_t stream, const bolt_column_t* keys, int32_t n_k
                     const int32_t* nulls_first, int64_t offset, int64_t limit, uint32_t* out_rows, int64_t* n_out);
 
 /* ------------------------------------------------------------------------
+ * Per-row expression programs — replaces the projection kernel of
+ * Engine::execute_projection (src/exec/engine.rs:4267) generated from a
+ * KernelSpec's Op list (src/plan/physical_plan.rs:81, lowered by
+ * src/jit/ptx_gen.rs).  A program is a list of ops over registers (each
+ * register written once, before use).  Semantics of ptx_gen.rs:
+ *   integer Add / Sub / Mul wrap (two's complement); integer Div and Mod by
+ *   zero give 0, INT_MIN / -1 gives INT_MIN and INT_MIN % -1 gives 0
+ *   (emit_int_div_guarded / emit_int_rem_guarded, :2231,2332); float ops are
+ *   IEEE, one rounding per op (no contraction); float comparisons are ordered
+ *   (NaN compares false, != included); Float -> Int casts truncate and
+ *   saturate (NaN -> 0); numeric -> Bool is "!= 0"; Date32 - Date32 is Int32
+ *   days; Decimal128 Add / Sub / Mul act on the unscaled 128-bit integers
+ *   (wrapping) and Decimal128 values compare as signed 128-bit integers.
+ * Each output with a validity bitmap receives the AND of the validity of
+ * every input column its value loads (the AND-of-inputs fold of
+ * ptx_gen.rs:474); values of NULL rows are computed from the stored bits.
+ * ------------------------------------------------------------------------ */
+enum bolt_expr_kind {
+  BOLT_X_LOAD = 1,    /* dst = inputs[col][row]                      (LoadColumn) */
+  BOLT_X_CONST = 2,   /* dst = literal of `dtype`                      (Const) */
+  BOLT_X_CAST = 3,    /* dst = (dtype) a                               (Cast) */
+  BOLT_X_BINARY = 4,  /* dst = a op b, operands of `dtype`             (Binary) */
+  BOLT_X_SELECT = 5,  /* dst = a ? b : c, a Bool                       (Select) */
+  BOLT_X_NOT = 6,     /* dst = !a, a Bool                              (Not) */
+  BOLT_X_IS_NULL = 7, /* dst = inputs[col] is NULL (op 1) / not (op 0) (IsNullCheck) */
+  BOLT_X_STORE = 8    /* outputs[col][row] = a                         (Store) */
+};
+enum bolt_bin_op {
+  BOLT_BIN_ADD = 1, BOLT_BIN_SUB = 2, BOLT_BIN_MUL = 3, BOLT_BIN_DIV = 4, BOLT_BIN_MOD = 5,
+  BOLT_BIN_EQ = 6, BOLT_BIN_NE = 7, BOLT_BIN_LT = 8, BOLT_BIN_LE = 9, BOLT_BIN_GT = 10, BOLT_BIN_GE = 11,
+  BOLT_BIN_AND = 12, BOLT_BIN_OR = 13, BOLT_BIN_BITAND = 14, BOLT_BIN_BITOR = 15
+};
+typedef struct bolt_expr_op {
+  int32_t kind;    /* bolt_expr_kind */
+  int32_t dst;     /* destination register, 0 .. BOLT_MAX_REGS - 1 */
+  int32_t a, b, c; /* operand registers */
+  int32_t col;     /* LOAD / IS_NULL: input column; STORE: output column */
+  int32_t op;      /* BINARY: bolt_bin_op; IS_NULL: 1 = IS NULL, 0 = IS NOT NULL */
+  int32_t dtype;   /* LOAD / CONST: value dtype; CAST: target; BINARY: operand dtype */
+  int64_t lit_i64; /* CONST: integer / date / Bool value, Decimal128 low word */
+  double lit_f64;  /* CONST: float value */
+  int64_t lit_hi;  /* CONST: Decimal128 high word */
+} bolt_expr_op_t;
+#define BOLT_MAX_REGS 256
+#define BOLT_MAX_PROJECT_INPUTS 16
+#define BOLT_MAX_PROJECT_OUTPUTS 16
+
+/* Runs `prog` for every row: inputs and outputs (caller-allocated, dtypes
+ * set, validity bitmaps optional) all have outputs[0].length rows.  The
+ * program is compiled once per distinct program and column shape (hipRTC)
+ * and cached.  A type error in the program returns BOLT_ERR_TYPE, a
+ * malformed program BOLT_ERR_PLAN.  Synchronises. */
+int bolt_project(bolt_stream_t stream, const bolt_column_t* inputs, int32_t n_inputs, const bolt_expr_op_t* prog,
+                 int32_t n_ops, bolt_column_t* outputs, int32_t n_outputs);
+
+/* ------------------------------------------------------------------------
  * Library information
  * ------------------------------------------------------------------------ */
 /* Build identifier string (target arch, version). */

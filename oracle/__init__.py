@@ -72,6 +72,9 @@ def lib() -> C.CDLL:
         L.bo_inner_join.restype = C.c_int64
         L.bo_inner_join.argtypes = [C.POINTER(Col), C.POINTER(Col), C.c_void_p, C.c_void_p, C.c_int64,
                                     C.POINTER(C.c_int64)]
+        L.bo_project.restype = C.c_int32
+        L.bo_project.argtypes = [C.POINTER(Col), C.c_int32, C.POINTER(ExprOp), C.c_int32, C.POINTER(Col), C.c_int32,
+                                 C.c_int64]
         L.bo_join.restype = C.c_int64
         L.bo_join.argtypes = [C.POINTER(Col), C.POINTER(Col), C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
                               C.c_int64, C.POINTER(C.c_int64)]
@@ -86,6 +89,15 @@ def lib() -> C.CDLL:
 
 
 # ---- host column helper ---------------------------------------------------------
+
+class ExprOp(C.Structure):
+    """bo_expr_op (same layout as the product's bolt_expr_op_t)."""
+    _fields_ = [
+        ("kind", C.c_int32), ("dst", C.c_int32), ("a", C.c_int32), ("b", C.c_int32), ("c", C.c_int32),
+        ("col", C.c_int32), ("op", C.c_int32), ("dtype", C.c_int32),
+        ("lit_i64", C.c_int64), ("lit_f64", C.c_double), ("lit_hi", C.c_int64),
+    ]
+
 
 class HostCol:
     """Keeps numpy buffers alive behind a `Col` descriptor."""
@@ -229,6 +241,30 @@ def sort_indices(keys: list[HostCol], descending: list[bool], nulls_first: list[
     nf = (C.c_int32 * k)(*[int(x) for x in nulls_first])
     lib().bo_sort_indices(_cols(keys), k, d, nf, n, out.ctypes.data)
     return out[:n]
+
+
+_NP = {BOOL: np.uint8, I32: np.int32, I64: np.int64, F32: np.float32, F64: np.float64, DATE32: np.int32}
+
+
+def project(inputs: list[HostCol], ops: list[dict], out_dtypes: list[int], n: int,
+            out_validity: list[bool] | None = None):
+    """Expression program (src/plan/physical_plan.rs:81, semantics of src/jit/ptx_gen.rs) over n rows.
+    Returns (status, [values], [validity or None]); Decimal128 values as (n, 2) int64 [lo, hi]."""
+    arr = (ExprOp * max(len(ops), 1))()
+    for i, op in enumerate(ops):
+        for k, v in op.items():
+            setattr(arr[i], k, v)
+    vals, bms, outs = [], [], []
+    for i, dt in enumerate(out_dtypes):
+        v = np.zeros((n, 2), np.int64) if dt == DEC128 else np.zeros(n, _NP[dt])
+        bm = np.zeros((n + 7) // 8 + 1, np.uint8) if out_validity and out_validity[i] else None
+        vals.append(v)
+        bms.append(bm)
+        outs.append(Col(v.ctypes.data, bm.ctypes.data if bm is not None else None, n, dt, 0))
+    oc = (Col * max(len(outs), 1))(*outs)
+    st = lib().bo_project(_cols(inputs), len(inputs), arr, len(ops), oc, len(outs), n)
+    valid = [np.unpackbits(b, bitorder="little")[:n].astype(bool) if b is not None else None for b in bms]
+    return st, vals, valid
 
 
 def limit_window(n: int, limit: int, offset: int) -> tuple[int, int]:

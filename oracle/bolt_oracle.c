@@ -820,3 +820,231 @@ int32_t bo_filter_join_agg(const int64_t* build_key, const int32_t* build_payloa
   *count = c;
   return status;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Per-row expression programs — src/plan/physical_plan.rs:81, ptx_gen.rs    */
+/* ------------------------------------------------------------------------ */
+
+enum { X_LOAD = 1, X_CONST, X_CAST, X_BINARY, X_SELECT, X_NOT, X_IS_NULL, X_STORE };
+enum { B_ADD = 1, B_SUB, B_MUL, B_DIV, B_MOD, B_EQ, B_NE, B_LT, B_LE, B_GT, B_GE, B_AND, B_OR, B_BAND, B_BOR };
+
+typedef struct {
+  int32_t dt;
+  int64_t i;    /* Bool / Int32 / Int64 / Date32 */
+  double f;     /* Float64; Float32 values are held rounded to float */
+  __int128 d;   /* Decimal128 */
+} xval;
+
+#define XREGS 256
+
+static int x_int(int dt) { return dt == BO_I32 || dt == BO_I64 || dt == BO_DATE32; }
+static int x_float(int dt) { return dt == BO_F32 || dt == BO_F64; }
+
+/* two's-complement wrap of an integer result to the width of dt */
+static int64_t x_wrap(int dt, uint64_t v) {
+  if (dt == BO_I64) return (int64_t)v;
+  return (int64_t)(int32_t)(uint32_t)v;
+}
+
+/* cvt.rzi.s32 / .s64: truncate, saturate, NaN -> 0 */
+static int64_t x_f2i(double x, int to) {
+  double lo = to == BO_I32 ? -2147483648.0 : -9223372036854775808.0;
+  double hi = to == BO_I32 ? 2147483647.0 : 9223372036854775807.0;
+  if (isnan(x)) return 0;
+  if (x >= hi) return to == BO_I32 ? 2147483647LL : INT64_MAX;
+  if (x <= lo) return to == BO_I32 ? -2147483648LL : INT64_MIN;
+  return (int64_t)x;
+}
+
+static int x_cast(xval* r, const xval* a, int to) {
+  int from = a->dt;
+  xval v = {to, 0, 0.0, 0};
+  if (from == to) { *r = *a; return BO_OK; }
+  if (from == BO_DEC128 || to == BO_DEC128 || from == BO_DATE32 || to == BO_DATE32) return BO_ERR_TYPE;
+  if (to == BO_BOOL) {
+    if (x_float(from)) v.i = (a->f < 0.0 || a->f > 0.0); /* ordered != 0: NaN -> 0 */
+    else v.i = a->i != 0;
+  } else if (to == BO_I32 || to == BO_I64) {
+    if (x_float(from)) v.i = x_f2i(a->f, to);
+    else v.i = x_wrap(to, (uint64_t)a->i);
+  } else if (to == BO_F32) {
+    v.f = x_float(from) ? (double)(float)a->f : (double)(float)a->i;
+  } else {
+    v.f = x_float(from) ? a->f : (double)a->i;
+  }
+  *r = v;
+  return BO_OK;
+}
+
+static int x_binary(xval* r, const xval* a, const xval* b, int op, int dt) {
+  xval v = {dt, 0, 0.0, 0};
+  if (a->dt != dt || b->dt != dt) return BO_ERR_TYPE;
+  if (op >= B_EQ && op <= B_GE) {
+    int c;
+    v.dt = BO_BOOL;
+    if (x_float(dt)) {
+      double x = a->f, y = b->f;
+      switch (op) {
+        case B_EQ: c = x == y; break;
+        case B_NE: c = x < y || x > y; break; /* ordered: NaN -> false */
+        case B_LT: c = x < y; break;
+        case B_LE: c = x <= y; break;
+        case B_GT: c = x > y; break;
+        default: c = x >= y; break;
+      }
+    } else if (dt == BO_DEC128) {
+      __int128 x = a->d, y = b->d;
+      c = op == B_EQ ? x == y : op == B_NE ? x != y : op == B_LT ? x < y : op == B_LE ? x <= y : op == B_GT ? x > y : x >= y;
+    } else {
+      int64_t x = a->i, y = b->i;
+      c = op == B_EQ ? x == y : op == B_NE ? x != y : op == B_LT ? x < y : op == B_LE ? x <= y : op == B_GT ? x > y : x >= y;
+    }
+    v.i = c;
+    *r = v;
+    return BO_OK;
+  }
+  switch (op) {
+    case B_ADD: case B_SUB: case B_MUL:
+      if (dt == BO_DATE32) {
+        if (op != B_SUB) return BO_ERR_TYPE;
+        v.dt = BO_I32;
+        v.i = x_wrap(BO_I32, (uint64_t)a->i - (uint64_t)b->i);
+      } else if (x_float(dt)) {
+        double x = a->f, y = b->f, z = op == B_ADD ? x + y : op == B_SUB ? x - y : x * y;
+        if (dt == BO_F32) { float xf = (float)x, yf = (float)y; z = op == B_ADD ? xf + yf : op == B_SUB ? xf - yf : xf * yf; z = (float)z; }
+        v.f = z;
+      } else if (dt == BO_DEC128) {
+        unsigned __int128 x = (unsigned __int128)a->d, y = (unsigned __int128)b->d;
+        v.d = (__int128)(op == B_ADD ? x + y : op == B_SUB ? x - y : x * y);
+      } else if (x_int(dt)) {
+        uint64_t x = (uint64_t)a->i, y = (uint64_t)b->i;
+        v.i = x_wrap(dt, op == B_ADD ? x + y : op == B_SUB ? x - y : x * y);
+      } else return BO_ERR_TYPE;
+      break;
+    case B_DIV: case B_MOD:
+      if (x_float(dt) && op == B_DIV) {
+        v.f = dt == BO_F32 ? (double)((float)a->f / (float)b->f) : a->f / b->f;
+      } else if (dt == BO_I32 || dt == BO_I64) {
+        int64_t x = a->i, y = b->i, mn = dt == BO_I32 ? -2147483648LL : INT64_MIN;
+        if (y == 0) v.i = 0;                            /* x / 0 and x % 0 -> 0 */
+        else if (x == mn && y == -1) v.i = op == B_DIV ? mn : 0; /* overflow corner */
+        else v.i = op == B_DIV ? x / y : x % y;
+      } else return BO_ERR_TYPE;
+      break;
+    case B_AND: case B_OR:
+      if (dt != BO_BOOL) return BO_ERR_TYPE;
+      v.i = op == B_AND ? (a->i & b->i) : (a->i | b->i);
+      break;
+    case B_BAND: case B_BOR:
+      if (dt != BO_I32 && dt != BO_I64) return BO_ERR_TYPE;
+      v.i = op == B_BAND ? (a->i & b->i) : (a->i | b->i);
+      break;
+    default: return BO_ERR_PLAN;
+  }
+  *r = v;
+  return BO_OK;
+}
+
+static xval x_load(const bo_col* c, int64_t i) {
+  xval v = {c->dtype, 0, 0.0, 0};
+  switch (c->dtype) {
+    case BO_BOOL: v.i = ((const uint8_t*)c->data)[i] != 0; break;
+    case BO_I32: case BO_DATE32: v.i = ((const int32_t*)c->data)[i]; break;
+    case BO_I64: v.i = ((const int64_t*)c->data)[i]; break;
+    case BO_F32: v.f = ((const float*)c->data)[i]; break;
+    case BO_F64: v.f = ((const double*)c->data)[i]; break;
+    case BO_DEC128: {
+      const uint64_t* w = (const uint64_t*)c->data;
+      v.d = (__int128)(((unsigned __int128)w[2 * i + 1] << 64) | w[2 * i]);
+      break;
+    }
+  }
+  return v;
+}
+
+static void x_store(bo_col* c, int64_t i, const xval* v) {
+  switch (c->dtype) {
+    case BO_BOOL: ((uint8_t*)c->data)[i] = (uint8_t)v->i; break;
+    case BO_I32: case BO_DATE32: ((int32_t*)c->data)[i] = (int32_t)v->i; break;
+    case BO_I64: ((int64_t*)c->data)[i] = v->i; break;
+    case BO_F32: ((float*)c->data)[i] = (float)v->f; break;
+    case BO_F64: ((double*)c->data)[i] = v->f; break;
+    case BO_DEC128: {
+      uint64_t* w = (uint64_t*)c->data;
+      w[2 * i] = (uint64_t)(unsigned __int128)v->d;
+      w[2 * i + 1] = (uint64_t)((unsigned __int128)v->d >> 64);
+      break;
+    }
+  }
+}
+
+int32_t bo_project(const bo_col* inputs, int32_t n_inputs, const bo_expr_op* prog, int32_t n_ops, bo_col* outputs,
+                   int32_t n_outputs, int64_t n_rows) {
+  xval* r = (xval*)calloc(XREGS, sizeof(xval));
+  uint32_t* deps = (uint32_t*)calloc(XREGS, sizeof(uint32_t));
+  int32_t st = BO_OK;
+  for (int64_t i = 0; i < n_rows && st == BO_OK; i++) {
+    for (int k = 0; k < n_ops && st == BO_OK; k++) {
+      const bo_expr_op* o = &prog[k];
+      if (o->kind != X_STORE && (o->dst < 0 || o->dst >= XREGS)) { st = BO_ERR_PLAN; break; }
+      xval* d = &r[o->kind == X_STORE ? 0 : o->dst];
+      switch (o->kind) {
+        case X_LOAD:
+          if (o->col < 0 || o->col >= n_inputs || inputs[o->col].dtype != o->dtype) { st = BO_ERR_TYPE; break; }
+          *d = x_load(&inputs[o->col], i);
+          deps[o->dst] = 1u << o->col;
+          break;
+        case X_CONST: {
+          xval v = {o->dtype, 0, 0.0, 0};
+          if (o->dtype == BO_F32) v.f = (float)o->lit_f64;
+          else if (o->dtype == BO_F64) v.f = o->lit_f64;
+          else if (o->dtype == BO_DEC128) v.d = (__int128)(((unsigned __int128)(uint64_t)o->lit_hi << 64) | (uint64_t)o->lit_i64);
+          else if (o->dtype == BO_BOOL) v.i = o->lit_i64 != 0;
+          else v.i = x_wrap(o->dtype == BO_I64 ? BO_I64 : BO_I32, (uint64_t)o->lit_i64);
+          *d = v;
+          deps[o->dst] = 0;
+          break;
+        }
+        case X_CAST: st = x_cast(d, &r[o->a], o->dtype); deps[o->dst] = deps[o->a]; break;
+        case X_BINARY: st = x_binary(d, &r[o->a], &r[o->b], o->op, o->dtype); deps[o->dst] = deps[o->a] | deps[o->b]; break;
+        case X_SELECT:
+          if (r[o->a].dt != BO_BOOL || r[o->b].dt != r[o->c].dt) { st = BO_ERR_TYPE; break; }
+          *d = r[o->a].i ? r[o->b] : r[o->c];
+          deps[o->dst] = deps[o->a] | deps[o->b] | deps[o->c];
+          break;
+        case X_NOT:
+          if (r[o->a].dt != BO_BOOL) { st = BO_ERR_TYPE; break; }
+          d->dt = BO_BOOL;
+          d->i = !r[o->a].i;
+          deps[o->dst] = deps[o->a];
+          break;
+        case X_IS_NULL: {
+          int isnull = !is_valid(&inputs[o->col], i);
+          d->dt = BO_BOOL;
+          d->i = o->op ? isnull : !isnull;
+          deps[o->dst] = 0;
+          break;
+        }
+        case X_STORE: {
+          bo_col* c = &outputs[o->col];
+          if (c->dtype != r[o->a].dt) { st = BO_ERR_TYPE; break; }
+          x_store(c, i, &r[o->a]);
+          if (c->validity) {
+            int ok = 1;
+            for (int q = 0; q < n_inputs; q++)
+              if ((deps[o->a] >> q) & 1u) ok = ok && is_valid(&inputs[q], i);
+            uint8_t* vb = (uint8_t*)c->validity;
+            if (ok) vb[i >> 3] |= (uint8_t)(1u << (i & 7));
+            else vb[i >> 3] &= (uint8_t)~(1u << (i & 7));
+          }
+          break;
+        }
+        default: st = BO_ERR_PLAN;
+      }
+    }
+  }
+  (void)n_outputs;
+  free(r);
+  free(deps);
+  return st;
+}

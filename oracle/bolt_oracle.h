@@ -137,6 +137,27 @@ int32_t bo_filter_join_agg(const int64_t* build_key, const int32_t* build_payloa
                            double amount_gt, int64_t* sum_payload, double* sum_amount_naive,
                            double* sum_amount_neumaier, int64_t* count);
 
+/* ---- Per-row expression programs (the projection kernel of
+ *      Engine::execute_projection, src/exec/engine.rs:4267, generated from
+ *      the Op list of src/plan/physical_plan.rs:81 by src/jit/ptx_gen.rs) ----
+ * Same op layout and kinds as the product ABI's bolt_expr_op_t.  Interprets
+ * the program row by row with ptx_gen.rs's semantics: wrapping integer
+ * arithmetic, guarded Div / Mod (emit_int_div_guarded :2231,
+ * emit_int_rem_guarded :2332), IEEE float ops, ordered float comparisons,
+ * saturating truncating Float -> Int casts (cvt.rzi), numeric -> Bool as
+ * != 0, Date32 - Date32 as Int32, Decimal128 Add / Sub / Mul on the
+ * unscaled integers.  Outputs with a validity bitmap get the AND of the
+ * validity of the input columns their value loads (:474).  Returns BO_OK,
+ * BO_ERR_TYPE or BO_ERR_PLAN. */
+typedef struct bo_expr_op {
+  int32_t kind, dst, a, b, c, col, op, dtype;
+  int64_t lit_i64;
+  double lit_f64;
+  int64_t lit_hi;
+} bo_expr_op;
+int32_t bo_project(const bo_col* inputs, int32_t n_inputs, const bo_expr_op* prog, int32_t n_ops, bo_col* outputs,
+                   int32_t n_outputs, int64_t n_rows);
+
 #ifdef __cplusplus
 }
 #endif

@@ -568,3 +568,149 @@ def test_filter_join_agg_matches_bruteforce():
     assert r["count"] == int(m.sum())
     assert r["sum_payload"] == int(bp[idx[m]].astype(np.int64).sum())
     assert r["sum_amount"] == math.fsum(pa[m])
+
+
+# ---- per-row expression programs (src/plan/physical_plan.rs:81, src/jit/ptx_gen.rs) --------------
+
+from craton_bolt_amd.ops import Program  # noqa: E402  (the op builder only; nothing runs on a device)
+
+I32_MIN, I32_MAX = -(1 << 31), (1 << 31) - 1
+
+
+def _run_prog(cols, prog, out_dtypes, out_validity=None):
+    n = cols[0].col.n
+    st, vals, valid = O.project(cols, prog.ops, out_dtypes, n, out_validity)
+    assert st == O.OK
+    return vals, valid
+
+
+def test_project_int_arithmetic_wraps_and_guards():
+    a = np.array([7, -7, I32_MIN, I32_MAX, 100, 0, I32_MIN, 5], np.int32)
+    b = np.array([2, 2, -1, 2, 0, 3, 1, -1], np.int32)
+    p = Program()
+    ra, rb = p.load(0, O.I32), p.load(1, O.I32)
+    p.store(p.binary("+", ra, rb, O.I32), 0)
+    p.store(p.binary("*", ra, rb, O.I32), 1)
+    p.store(p.binary("/", ra, rb, O.I32), 2)
+    p.store(p.binary("%", ra, rb, O.I32), 3)
+    p.store(p.binary("-", ra, rb, O.I32), 4)
+    vals, _ = _run_prog([O.HostCol(a, O.I32), O.HostCol(b, O.I32)], p, [O.I32] * 5)
+    with np.errstate(over="ignore"):
+        assert np.array_equal(vals[0], a + b) and np.array_equal(vals[1], a * b) and np.array_equal(vals[4], a - b)
+    # ptx_gen.rs:2231-2400: x / 0 = 0, x % 0 = 0, MIN / -1 = MIN, MIN % -1 = 0; C truncating division
+    want_div = [3, -3, I32_MIN, I32_MAX // 2, 0, 0, I32_MIN, -5]
+    want_mod = [1, -1, 0, 1, 0, 0, 0, 0]
+    assert vals[2].tolist() == want_div and vals[3].tolist() == want_mod
+
+
+def test_project_int64_guards_match_python():
+    rng = np.random.default_rng(4)
+    a = rng.integers(-(1 << 62), 1 << 62, 2000).astype(np.int64)
+    b = rng.integers(-50, 50, 2000).astype(np.int64)
+    a[:3] = [-(1 << 63), -(1 << 63), 12]
+    b[:3] = [-1, 0, 0]
+    p = Program()
+    ra, rb = p.load(0, O.I64), p.load(1, O.I64)
+    p.store(p.binary("/", ra, rb, O.I64), 0)
+    p.store(p.binary("%", ra, rb, O.I64), 1)
+    vals, _ = _run_prog([O.HostCol(a, O.I64), O.HostCol(b, O.I64)], p, [O.I64, O.I64])
+    for x, y, q, r in zip(a.tolist(), b.tolist(), vals[0].tolist(), vals[1].tolist()):
+        if y == 0:
+            assert (q, r) == (0, 0)
+        elif x == -(1 << 63) and y == -1:
+            assert (q, r) == (x, 0)
+        else:
+            tq = abs(x) // abs(y) * (1 if (x >= 0) == (y >= 0) else -1)
+            assert (q, r) == (tq, x - tq * y)
+
+
+def test_project_float_ops_compare_and_cast():
+    x = np.array([1.5, -2.5, np.nan, np.inf, -0.0, 3e10, -3e10, 2.9999999], np.float64)
+    y = np.array([1.5, 2.0, np.nan, 1.0, 0.0, 1.0, np.nan, 3.0], np.float64)
+    p = Program()
+    rx, ry = p.load(0, O.F64), p.load(1, O.F64)
+    p.store(p.binary("!=", rx, ry, O.F64), 0)
+    p.store(p.binary("==", rx, ry, O.F64), 1)
+    p.store(p.cast(rx, O.I32), 2)
+    p.store(p.cast(rx, O.BOOL), 3)
+    p.store(p.binary("/", rx, ry, O.F64), 4)
+    p.store(p.cast(rx, O.F32), 5)
+    vals, _ = _run_prog([O.HostCol(x, O.F64), O.HostCol(y, O.F64)], p, [O.BOOL, O.BOOL, O.I32, O.BOOL, O.F64, O.F32])
+    ordered_ne = (x < y) | (x > y)  # NaN compares false, != included (setp.ne is ordered)
+    assert np.array_equal(vals[0].astype(bool), ordered_ne) and np.array_equal(vals[1].astype(bool), x == y)
+    sat = np.where(np.isnan(x), 0, np.clip(np.trunc(np.nan_to_num(x, posinf=1e300, neginf=-1e300)), I32_MIN, I32_MAX))
+    assert vals[2].tolist() == sat.astype(np.int64).tolist()  # cvt.rzi: truncate, saturate, NaN -> 0
+    assert vals[3].astype(bool).tolist() == [True, True, False, True, False, True, True, True]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        assert np.array_equal(vals[4], x / y, equal_nan=True)
+    assert np.array_equal(vals[5], x.astype(np.float32), equal_nan=True)
+
+
+def test_project_float32_rounds_once_per_op():
+    rng = np.random.default_rng(8)
+    a = rng.standard_normal(5000).astype(np.float32)
+    b = rng.standard_normal(5000).astype(np.float32)
+    c = rng.standard_normal(5000).astype(np.float32)
+    p = Program()
+    ra, rb, rc = p.load(0, O.F32), p.load(1, O.F32), p.load(2, O.F32)
+    p.store(p.binary("+", p.binary("*", ra, rb, O.F32), rc, O.F32), 0)  # no fused multiply-add
+    vals, _ = _run_prog([O.HostCol(a, O.F32), O.HostCol(b, O.F32), O.HostCol(c, O.F32)], p, [O.F32])
+    assert np.array_equal(vals[0], (a * b) + c)
+
+
+def test_project_select_not_is_null_and_validity_fold():
+    v = np.array([5, 6, 7, 8], np.int64)
+    w = np.array([1, 2, 3, 4], np.int64)
+    vv = np.array([True, False, True, True])
+    wv = np.array([True, True, False, True])
+    p = Program()
+    rv, rw = p.load(0, O.I64), p.load(1, O.I64)
+    isn = p.is_null(0)
+    p.store(p.select(isn, p.const(-1, O.I64), rv), 0)  # CASE WHEN v IS NULL THEN -1 ELSE v END
+    p.store(p.binary("+", rv, rw, O.I64), 1)           # depends on v and w
+    p.store(p.not_(p.binary(">", rw, p.const(2, O.I64), O.I64)), 2)
+    vals, valid = _run_prog([O.HostCol(v, O.I64, vv), O.HostCol(w, O.I64, wv)], p, [O.I64, O.I64, O.BOOL],
+                            [True, True, True])
+    assert vals[0].tolist() == [5, -1, 7, 8]
+    assert vals[1].tolist() == [6, 8, 10, 12]  # values of NULL rows come from the stored bits
+    assert vals[2].tolist() == [1, 1, 0, 0]
+    # AND of the validity of the loaded inputs (ptx_gen.rs:474); IS NULL reads no value
+    assert valid[0].tolist() == vv.tolist()
+    assert valid[1].tolist() == (vv & wv).tolist()
+    assert valid[2].tolist() == wv.tolist()
+
+
+def test_project_decimal_and_date_arithmetic():
+    m = (1 << 64) - 1
+    xs = [10 ** 20 + 7, -(10 ** 19), 5, -(1 << 126)]
+    ys = [3, 10 ** 18, -5, 4]
+    def dec(vals):
+        return np.array([[v & m, (v >> 64) & m] for v in vals], dtype=np.uint64).view(np.int64)
+    p = Program()
+    rx, ry = p.load(0, O.DEC128), p.load(1, O.DEC128)
+    p.store(p.binary("+", rx, ry, O.DEC128), 0)
+    p.store(p.binary("*", rx, ry, O.DEC128), 1)
+    p.store(p.binary("<", rx, ry, O.DEC128), 2)
+    vals, _ = _run_prog([O.HostCol(dec(xs), O.DEC128), O.HostCol(dec(ys), O.DEC128)], p, [O.DEC128, O.DEC128, O.BOOL])
+    def back(a):
+        u = [(int(lo) & m) | ((int(hi) & m) << 64) for lo, hi in a]
+        return [x - (1 << 128) if x >> 127 else x for x in u]
+    wrap = lambda z: ((z + (1 << 127)) % (1 << 128)) - (1 << 127)  # noqa: E731
+    assert back(vals[0]) == [wrap(x + y) for x, y in zip(xs, ys)]
+    assert back(vals[1]) == [wrap(x * y) for x, y in zip(xs, ys)]  # -(2^126) * 4 wraps to 0
+    assert vals[2].tolist() == [int(x < y) for x, y in zip(xs, ys)]
+    d1 = np.array([19000, 0, -5], np.int32)
+    d2 = np.array([18000, 1, 5], np.int32)
+    q = Program()
+    q.store(q.binary("-", q.load(0, O.DATE32), q.load(1, O.DATE32), O.DATE32), 0)  # Date32 - Date32 -> Int32
+    vals, _ = _run_prog([O.HostCol(d1, O.DATE32), O.HostCol(d2, O.DATE32)], q, [O.I32])
+    assert vals[0].tolist() == [1000, -1, -10]
+
+
+def test_project_type_errors():
+    a = O.HostCol(np.arange(4, dtype=np.int32), O.I32)
+    p = Program()
+    ra = p.load(0, O.I32)
+    p.store(p.binary("+", ra, p.const(1.0, O.F64), O.I32), 0)  # operands of different dtypes: cast first
+    st, _, _ = O.project([a], p.ops, [O.I32], 4)
+    assert st == O.ERR_TYPE
