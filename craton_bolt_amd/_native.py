@@ -245,6 +245,8 @@ _SIGS = {
     "bolt_prefix_scan_mask": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(C.c_int64)]),
     "bolt_project": (C.c_int, [C.c_void_p, C.POINTER(BoltColumn), C.c_int32, C.POINTER(BoltExprOp), C.c_int32,
                                C.POINTER(BoltColumn), C.c_int32]),
+    "bolt_aggregate_expr": (C.c_int, [C.c_void_p, C.POINTER(BoltColumn), C.c_int32, C.POINTER(BoltExprOp), C.c_int32,
+                                      C.c_int32, C.c_int32, C.POINTER(BoltAgg), C.c_int32, C.POINTER(BoltScalar)]),
     "bolt_take": (C.c_int, [C.c_void_p, C.POINTER(BoltColumn), C.c_void_p, C.c_int64, C.POINTER(BoltColumn)]),
     "bolt_groupby": (C.c_int, [C.c_void_p, C.POINTER(BoltColumn), C.c_int32, C.POINTER(BoltColumn), C.c_int32,
                                C.POINTER(BoltAgg), C.c_int32, C.POINTER(BoltFilter), C.c_uint32,

@@ -61,6 +61,8 @@ struct GbJitSpec {
 };
 
 bool jit_enabled();
+// Device helpers every generated kernel starts with (device_acc.h, gb_lds.h).
+const char* jit_prelude_text();
 std::string jit_row_agg_source(const JitSpec& sp);
 int jit_get(const std::string& src, hipFunction_t* fn);
 int jit_spec_from(const DevFilter& f, const AggList& aggs, bool probe, int key_dtype, bool key_valid, int carry_dtype,

@@ -115,6 +115,8 @@ void emit_col_ptrs(std::ostringstream& o, const std::vector<JitSpec::Col>& cols)
 
 }  // namespace
 
+const char* jit_prelude_text() { return kJitPrelude; }
+
 int jit_get(const std::string& src, hipFunction_t* fn) {
   std::lock_guard<std::mutex> lk(g_mu);
   auto it = g_cache.find(src);

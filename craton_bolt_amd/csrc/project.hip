@@ -17,6 +17,7 @@
 
 #include "bolt_internal.h"
 #include "jit.h"
+#include "reduce_common.h"
 
 namespace bolt {
 namespace {
@@ -190,12 +191,23 @@ int binary_expr(int op, int dt, const std::string& a, const std::string& b, std:
   return set_error(BOLT_ERR_TYPE, "bolt_project: binary op %d not defined on %s", op, dtype_name(dt));
 }
 
-// Type-checks the program and prints the kernel.
-int gen_source(const bolt_column_t* in, int n_in, const bolt_expr_op_t* prog, int n_ops, const bolt_column_t* out,
-               int n_out, std::string* src) {
+// A type-checked program: its statements (one per op, for row `r`) and, per
+// STORE, the register, dtype and input dependencies of the stored value.
+struct Compiled {
+  std::string body;
+  std::vector<int> reg, dtype;
+  std::vector<uint32_t> deps;
+};
+
+// `n_out` STOREs are expected (outputs 0 .. n_out - 1, each once); with
+// `out` non-null each must match its output column's dtype.
+int compile_program(const bolt_column_t* in, int n_in, const bolt_expr_op_t* prog, int n_ops,
+                    const bolt_column_t* out, int n_out, Compiled* cp) {
   std::vector<RegInfo> regs(BOLT_MAX_REGS);
-  std::vector<int> stored(n_out, 0);
-  std::ostringstream body, stores, valid;
+  cp->reg.assign(n_out, -1);
+  cp->dtype.assign(n_out, 0);
+  cp->deps.assign(n_out, 0);
+  std::ostringstream body;
   auto use = [&](int r, const char* what) -> int {
     if (r < 0 || r >= BOLT_MAX_REGS || regs[r].dtype == 0)
       return set_error(BOLT_ERR_PLAN, "bolt_project: %s reads register %d before it is written", what, r);
@@ -214,13 +226,14 @@ int gen_source(const bolt_column_t* in, int n_in, const bolt_expr_op_t* prog, in
         if (!supported(dt) || dt != o.dtype)
           return set_error(BOLT_ERR_TYPE, "bolt_project: op %d loads column %d (%s) as %s", k, o.col, dtype_name(dt),
                            dtype_name(o.dtype));
+        const std::string c = std::to_string(o.col);
         if (dt == BOLT_DEC128)
-          e = "(__int128)(((unsigned __int128)((const u64*)p.in[" + std::to_string(o.col) + "])[2 * r + 1] << 64) | ((const u64*)p.in[" +
-              std::to_string(o.col) + "])[2 * r])";
+          e = "(__int128)(((unsigned __int128)((const u64*)p.in[" + c + "])[2 * r + 1] << 64) | ((const u64*)p.in[" + c +
+              "])[2 * r])";
         else if (dt == BOLT_BOOL)
-          e = "(((const unsigned char*)p.in[" + std::to_string(o.col) + "])[r] != 0 ? 1 : 0)";
+          e = "(((const unsigned char*)p.in[" + c + "])[r] != 0 ? 1 : 0)";
         else
-          e = std::string("((const ") + stype(dt) + "*)p.in[" + std::to_string(o.col) + "])[r]";
+          e = std::string("((const ") + stype(dt) + "*)p.in[" + c + "])[r]";
         d.dtype = dt;
         d.deps = 1u << o.col;
         break;
@@ -278,27 +291,13 @@ int gen_source(const bolt_column_t* in, int n_in, const bolt_expr_op_t* prog, in
       case BOLT_X_STORE: {
         BOLT_TRY(use(o.a, "STORE"));
         if (o.col < 0 || o.col >= n_out) return set_error(BOLT_ERR_PLAN, "bolt_project: op %d stores column %d", k, o.col);
-        if (stored[o.col]) return set_error(BOLT_ERR_PLAN, "bolt_project: output %d is stored twice", o.col);
-        const int dt = out[o.col].dtype;
-        if (dt != regs[o.a].dtype)
-          return set_error(BOLT_ERR_TYPE, "bolt_project: output %d is %s, its value %s", o.col, dtype_name(dt),
-                           dtype_name(regs[o.a].dtype));
-        stored[o.col] = 1;
-        const std::string oc = std::to_string(o.col);
-        if (dt == BOLT_DEC128)
-          stores << "      ((u64*)p.out[" << oc << "])[2 * i] = (u64)" << reg(o.a) << "; ((u64*)p.out[" << oc
-                 << "])[2 * i + 1] = (u64)((unsigned __int128)" << reg(o.a) << " >> 64);\n";
-        else
-          stores << "      ((" << stype(dt) << "*)p.out[" << oc << "])[i] = (" << stype(dt) << ")" << reg(o.a) << ";\n";
-        if (out[o.col].validity) {
-          valid << "    { int ok = in;";
-          for (int c = 0; c < n_in; c++)
-            if ((regs[o.a].deps >> c) & 1u && in[c].validity) valid << " ok = ok && bitv(p.iv[" << c << "], r);";
-          valid << "\n      const u64 m = __ballot(ok);\n"
-                << "      if ((threadIdx.x & 63) == 0)\n"
-                << "        for (int k = 0; k < 8 && (i >> 3) + k < nbytes; k++) p.ov[" << oc
-                << "][(i >> 3) + k] = (unsigned char)(m >> (8 * k)); }\n";
-        }
+        if (cp->reg[o.col] >= 0) return set_error(BOLT_ERR_PLAN, "bolt_project: output %d is stored twice", o.col);
+        if (out && out[o.col].dtype != regs[o.a].dtype)
+          return set_error(BOLT_ERR_TYPE, "bolt_project: output %d is %s, its value %s", o.col,
+                           dtype_name(out[o.col].dtype), dtype_name(regs[o.a].dtype));
+        cp->reg[o.col] = o.a;
+        cp->dtype[o.col] = regs[o.a].dtype;
+        cp->deps[o.col] = regs[o.a].deps;
         continue;
       }
       default: return set_error(BOLT_ERR_PLAN, "bolt_project: op %d has unknown kind %d", k, o.kind);
@@ -306,7 +305,39 @@ int gen_source(const bolt_column_t* in, int n_in, const bolt_expr_op_t* prog, in
     body << "    const " << ctype(d.dtype) << " " << reg(o.dst) << " = " << e << ";\n";
   }
   for (int c = 0; c < n_out; c++)
-    if (!stored[c]) return set_error(BOLT_ERR_PLAN, "bolt_project: output %d is never stored", c);
+    if (cp->reg[c] < 0) return set_error(BOLT_ERR_PLAN, "bolt_project: output %d is never stored", c);
+  cp->body = body.str();
+  return BOLT_OK;
+}
+
+// C condition "every input column that output `c` loads is non-NULL at row r".
+std::string valid_expr(const bolt_column_t* in, int n_in, uint32_t deps) {
+  std::string v = "1";
+  for (int c = 0; c < n_in; c++)
+    if ((deps >> c) & 1u && in[c].validity) v += " && bitv(p.iv[" + std::to_string(c) + "], r)";
+  return v;
+}
+
+// The projection kernel.
+int gen_source(const bolt_column_t* in, int n_in, const bolt_expr_op_t* prog, int n_ops, const bolt_column_t* out,
+               int n_out, std::string* src) {
+  Compiled cp;
+  BOLT_TRY(compile_program(in, n_in, prog, n_ops, out, n_out, &cp));
+  std::ostringstream stores, valid;
+  for (int c = 0; c < n_out; c++) {
+    const int dt = cp.dtype[c];
+    const std::string oc = std::to_string(c), rg = reg(cp.reg[c]);
+    if (dt == BOLT_DEC128)
+      stores << "      ((u64*)p.out[" << oc << "])[2 * i] = (u64)" << rg << "; ((u64*)p.out[" << oc
+             << "])[2 * i + 1] = (u64)((unsigned __int128)" << rg << " >> 64);\n";
+    else
+      stores << "      ((" << stype(dt) << "*)p.out[" << oc << "])[i] = (" << stype(dt) << ")" << rg << ";\n";
+    if (out[c].validity)
+      valid << "    { const u64 m = __ballot(in && (" << valid_expr(in, n_in, cp.deps[c]) << "));\n"
+            << "      if ((threadIdx.x & 63) == 0)\n"
+            << "        for (int k = 0; k < 8 && (i >> 3) + k < nbytes; k++) p.ov[" << oc
+            << "][(i >> 3) + k] = (unsigned char)(m >> (8 * k)); }\n";
+  }
   std::ostringstream s;
   s << kProjectPrelude
     << "extern \"C\" __global__ void __launch_bounds__(256) bolt_jit_kernel(PArgs p) {\n"
@@ -317,9 +348,57 @@ int gen_source(const bolt_column_t* in, int n_in, const bolt_expr_op_t* prog, in
     << "  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < end; i += stride) {\n"
     << "    const int in = i < p.n;\n"
     << "    const long long r = in ? i : 0;  // rows past the end load row 0 and store nothing\n"
-    << body.str() << "    if (in) {\n"
+    << cp.body << "    if (in) {\n"
     << stores.str() << "    }\n"
     << valid.str() << "  }\n}\n";
+  *src = s.str();
+  return BOLT_OK;
+}
+
+// The fused WHERE + aggregate kernel: the program per row, the predicate
+// (a Bool output, true and non-NULL), then each aggregate over its output
+// (skipped where that output is NULL); per-workgroup partials as the
+// precompiled scalar aggregate writes them.
+int gen_agg_source(const bolt_column_t* in, int n_in, const Compiled& cp, int filter_col, const AggList& aggs,
+                   const bolt_agg_t* spec, std::string* src) {
+  std::ostringstream s;
+  s << jit_prelude_text() << "\n" << kProjectPrelude
+    << "extern \"C\" __global__ void __launch_bounds__(256) bolt_jit_kernel(PArgs p, bolt::Acc* __restrict__ parts) {\n"
+    << "  using namespace bolt;\n";
+  for (int a = 0; a < aggs.n; a++) s << "  Acc acc" << a << "; acc_init(acc" << a << ", " << aggs.a[a].op << ");\n";
+  s << "  const long long stride = (long long)gridDim.x * 256;\n"
+    << "  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < p.n; i += stride) {\n"
+    << "    const long long r = i;\n"
+    << cp.body;
+  if (filter_col >= 0)
+    s << "    if (!(" << reg(cp.reg[filter_col]) << " && (" << valid_expr(in, n_in, cp.deps[filter_col])
+      << "))) continue;\n";
+  for (int a = 0; a < aggs.n; a++) {
+    const std::string A = "acc" + std::to_string(a);
+    if (aggs.a[a].op == BOLT_AGG_COUNT_STAR) {
+      s << "    " << A << ".n++;\n";
+      continue;
+    }
+    const int c = spec[a].col, dt = cp.dtype[c];
+    const std::string rg = reg(cp.reg[c]);
+    std::string vi = "0", vhi = "0", vf = "0.0";
+    if (dt == BOLT_DEC128) {
+      vi = "(long long)(u64)" + rg;
+      vhi = "(long long)(u64)((unsigned __int128)" + rg + " >> 64)";
+    } else if (is_float(dt)) {
+      vf = "(double)" + rg;
+    } else {
+      vi = "(long long)" + rg;
+      vf = "(double)(long long)" + rg;
+    }
+    s << "    if (" << valid_expr(in, n_in, cp.deps[c]) << ") acc_update(" << A << ", " << aggs.a[a].op << ", " << dt
+      << ", " << vi << ", " << vhi << ", " << vf << ");\n";
+  }
+  s << "  }\n  __shared__ Acc sh[4];\n";
+  for (int a = 0; a < aggs.n; a++)
+    s << "  block_reduce_acc(acc" << a << ", " << aggs.a[a].op << ", " << aggs.a[a].dtype
+      << ", sh, 4, parts + (long long)blockIdx.x * " << aggs.n << " + " << a << ");\n";
+  s << "}\n";
   *src = s.str();
   return BOLT_OK;
 }
@@ -383,5 +462,72 @@ extern "C" int bolt_project(bolt_stream_t stream, const bolt_column_t* inputs, i
   const unsigned grid = grid_for(n, 256, 4, 256 * 32);
   BOLT_HIP(hipModuleLaunchKernel(fn, grid, 1, 1, 256, 1, 1, 0, s, args, nullptr));
   BOLT_HIP(hipStreamSynchronize(s));
+  return BOLT_OK;
+}
+
+extern "C" int bolt_aggregate_expr(bolt_stream_t stream, const bolt_column_t* inputs, int32_t n_inputs,
+                                   const bolt_expr_op_t* prog, int32_t n_ops, int32_t n_values, int32_t filter_col,
+                                   const bolt_agg_t* aggs, int32_t n_aggs, bolt_scalar_t* results) {
+  BOLT_CHECK_DEVICE();
+  clear_error();
+  hipStream_t s = (hipStream_t)stream;
+  if (n_inputs < 1 || n_inputs > BOLT_MAX_PROJECT_INPUTS || inputs == nullptr)
+    return set_error(BOLT_ERR_PLAN, "bolt_aggregate_expr: 1 to %d inputs", BOLT_MAX_PROJECT_INPUTS);
+  if (n_values < 1 || n_values > BOLT_MAX_PROJECT_OUTPUTS)
+    return set_error(BOLT_ERR_PLAN, "bolt_aggregate_expr: 1 to %d program values", BOLT_MAX_PROJECT_OUTPUTS);
+  if (n_ops < 1 || prog == nullptr) return set_error(BOLT_ERR_PLAN, "bolt_aggregate_expr: empty program");
+  const int64_t n = inputs[0].length;
+  for (int c = 0; c < n_inputs; c++) {
+    if (inputs[c].length != n)
+      return set_error(BOLT_ERR_PLAN, "bolt_aggregate_expr: input %d has %lld rows, input 0 has %lld", c,
+                       (long long)inputs[c].length, (long long)n);
+    if (n > 0 && inputs[c].data == nullptr) return set_error(BOLT_ERR_PLAN, "bolt_aggregate_expr: input %d has no data", c);
+  }
+  Compiled cp;
+  BOLT_TRY(compile_program(inputs, n_inputs, prog, n_ops, nullptr, n_values, &cp));
+  if (filter_col >= n_values || (filter_col >= 0 && cp.dtype[filter_col] != BOLT_BOOL))
+    return set_error(BOLT_ERR_PLAN, "bolt_aggregate_expr: filter value %d is not a Bool value of the program", filter_col);
+  // the program's values as (dataless) columns, for the aggregate checks and output dtypes
+  std::vector<bolt_column_t> vcols(n_values);
+  static const uint8_t kSomeBitmap = 0;
+  for (int v = 0; v < n_values; v++) {
+    memset(&vcols[v], 0, sizeof vcols[v]);
+    vcols[v].dtype = cp.dtype[v];
+    vcols[v].length = n;
+    bool nullable = false;
+    for (int c = 0; c < n_inputs; c++) nullable |= ((cp.deps[v] >> c) & 1u) && inputs[c].validity;
+    vcols[v].validity = nullable ? &kSomeBitmap : nullptr;
+  }
+  AggList all;
+  BOLT_TRY(make_agg_list(aggs, n_aggs, vcols.data(), n_values, 0, &all));
+  if (n_aggs == 0) return BOLT_OK;
+  std::string src;
+  BOLT_TRY(gen_agg_source(inputs, n_inputs, cp, filter_col, all, aggs, &src));
+  hipFunction_t fn;
+  BOLT_TRY(jit_get(src, &fn));
+  PArgs a;
+  memset(&a, 0, sizeof a);
+  for (int c = 0; c < n_inputs; c++) {
+    a.in[c] = inputs[c].data;
+    a.iv[c] = inputs[c].validity;
+  }
+  a.n = n;
+  const unsigned grid = grid_for(n, 256, 4, 256 * 32);
+  DevBuf<Acc> parts, outs;
+  BOLT_TRY(parts.alloc((size_t)grid * n_aggs, s));
+  BOLT_TRY(outs.alloc((size_t)n_aggs, s));
+  Acc* pp = parts.ptr;
+  void* args[] = {&a, &pp};
+  prof_begin(s);
+  BOLT_HIP(hipModuleLaunchKernel(fn, grid, 1, 1, 256, 1, 1, 0, s, args, nullptr));
+  prof_end(s);
+  for (int first = 0; first < n_aggs; first += 4) {
+    const int na = n_aggs - first < 4 ? n_aggs - first : 4;
+    BOLT_TRY(launch_finish(na, parts.ptr, (int)grid, n_aggs, first, agg_chunk(all, first, na), outs.ptr + first, s));
+  }
+  Acc host[kMaxAggs];
+  BOLT_HIP(hipMemcpyAsync(host, outs.ptr, sizeof(Acc) * n_aggs, hipMemcpyDeviceToHost, s));
+  BOLT_HIP(hipStreamSynchronize(s));
+  for (int i = 0; i < n_aggs; i++) BOLT_TRY(acc_finalize(host[i], all.a[i].op, all.a[i].dtype, &results[i]));
   return BOLT_OK;
 }

@@ -604,6 +604,17 @@ def execute_projection(inputs: list[DeviceColumn], program: Program, out_dtypes:
     return outs
 
 
+def execute_aggregate_expr(inputs: list[DeviceColumn], program: Program, n_values: int, aggs: list[tuple[int, int]],
+                           filter_value: int = -1, stream: Stream | None = None) -> list:
+    """Aggregates over the values of `program` for the rows where value `filter_value` is true
+    (bolt_aggregate_expr: one fused kernel, nothing materialised)."""
+    res = (N.BoltScalar * max(len(aggs), 1))()
+    st = N.lib().bolt_aggregate_expr(_stream(stream), columns_array(inputs), len(inputs), program.array(N.BoltExprOp),
+                                     len(program.ops), n_values, filter_value, _aggs(aggs), len(aggs), res)
+    N.check(st, "bolt_aggregate_expr")
+    return [scalar_value(res[i]) for i in range(len(aggs))]
+
+
 JOIN_TYPES = {"inner": 0, "left": 1, "right": 2, "full": 3}
 NULL_ROW = 0xFFFFFFFF
 

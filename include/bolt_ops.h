@@ -370,6 +370,22 @@ typedef struct bolt_expr_op {
 int bolt_project(bolt_stream_t stream, const bolt_column_t* inputs, int32_t n_inputs, const bolt_expr_op_t* prog,
                  int32_t n_ops, bolt_column_t* outputs, int32_t n_outputs);
 
+/* Fused WHERE + aggregates over an expression program — replaces
+ * agg_with_pre::try_execute_resident (src/exec/agg_with_pre.rs:157), whose
+ * pre kernel materialises the expression inputs (and expr_agg.rs's host
+ * evaluator) before the aggregate runs.  Here one generated kernel runs the
+ * program, the predicate and the aggregates per row and materialises
+ * nothing.  The program's STOREs define n_values values (STORE col =
+ * value index, each once); `filter_col` (or -1) names a Bool value as the
+ * WHERE predicate (a row is kept where it is true and not NULL); each
+ * aggregate's `col` indexes a value (ignored for COUNT(*)).  A value is NULL
+ * where an input column it loads is NULL, and NULL values are skipped as in
+ * bolt_aggregate, whose result rules (output dtypes, SUM overflow ->
+ * BOLT_ERR_TYPE) apply.  Synchronises. */
+int bolt_aggregate_expr(bolt_stream_t stream, const bolt_column_t* inputs, int32_t n_inputs,
+                        const bolt_expr_op_t* prog, int32_t n_ops, int32_t n_values, int32_t filter_col,
+                        const bolt_agg_t* aggs, int32_t n_aggs, bolt_scalar_t* results);
+
 /* ------------------------------------------------------------------------
  * Library information
  * ------------------------------------------------------------------------ */

@@ -267,6 +267,25 @@ def project(inputs: list[HostCol], ops: list[dict], out_dtypes: list[int], n: in
     return st, vals, valid
 
 
+def aggregate_expr(inputs: list[HostCol], ops: list[dict], value_dtypes: list[int], filter_value: int,
+                   aggs: list[tuple[int, int]], n: int) -> list:
+    """agg_with_pre::try_execute_resident (src/exec/agg_with_pre.rs:157): the pre kernel materialises the
+    program's values (project above), the WHERE value selects rows, then each aggregate runs over its value
+    (src/exec/aggregate.rs:154).  Returns one Scalar per aggregate (COUNT(*) as an int)."""
+    st, vals, valid = project(inputs, ops, value_dtypes, n, [True] * len(value_dtypes))
+    if st != OK:
+        raise ValueError(f"program status {st}")
+    keep = np.ones(n, bool) if filter_value < 0 else (vals[filter_value] != 0) & valid[filter_value]
+    out = []
+    for op, col in aggs:
+        if op == 3:  # COUNT(*)
+            out.append(int(keep.sum()))
+            continue
+        v = vals[col][keep]
+        out.append(scalar_agg(HostCol(v, value_dtypes[col], valid[col][keep]), op))
+    return out
+
+
 def limit_window(n: int, limit: int, offset: int) -> tuple[int, int]:
     """execute_limit (src/exec/limit.rs:18): the [start, end) window of LIMIT/OFFSET over n rows."""
     if offset >= n:

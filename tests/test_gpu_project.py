@@ -177,3 +177,88 @@ def test_project_errors(gpu):
     with pytest.raises(N.BoltError) as e:
         execute_projection([d], r, [N.I32])
     assert e.value.code == N.BOLT_ERR_PLAN
+
+
+# ---- fused WHERE + aggregates over expression programs (bolt_aggregate_expr) ----------------------
+
+from craton_bolt_amd.ops import execute_aggregate_expr  # noqa: E402
+
+SUM, COUNT, COUNT_STAR, MIN, MAX, AVG, VAR_POP, STDDEV_SAMP = 1, 2, 3, 4, 5, 6, 7, 10
+REL_TOL = 1e-9
+
+
+def _agg_check(cols, prog, value_dtypes, filter_value, aggs):
+    dev = [d for d, _ in cols]
+    host = [h for _, h in cols]
+    got = execute_aggregate_expr(dev, prog, len(value_dtypes), aggs, filter_value)
+    want = O.aggregate_expr(host, prog.ops, value_dtypes, filter_value, aggs, dev[0].length)
+    for (op, col), g, w in zip(aggs, got, want):
+        if op == COUNT_STAR:
+            assert g == w
+            continue
+        assert w.status == O.OK
+        if w.is_null:
+            assert g is None
+        elif w.dtype in (O.F32, O.F64):
+            assert abs(g - w.f64) <= REL_TOL * max(abs(w.f64), 1e-300), (op, g, w.f64)
+        elif w.dtype == O.DEC128:
+            assert g == (int(w.hi) << 64) | (int(w.i64) & ((1 << 64) - 1))
+        else:
+            assert g == w.i64, (op, g, w.i64)
+
+
+@pytest.mark.parametrize("n", [0, 1, 10_000, 4_000_037])
+def test_aggregate_expr_q6_shape(gpu, n):
+    """SUM(price * disc), COUNT(*), MIN/MAX/AVG(price * disc) WHERE qty < 24 AND disc BETWEEN 0.05 AND 0.07
+    AND price + qty > 100 (a TPC-H Q6 shape with an expression predicate)."""
+    print(f"[case] aggregate_expr q6 n={n}", flush=True)
+    rng = np.random.default_rng(n + 1)
+    price = rng.uniform(1, 1000, n)
+    disc = np.round(rng.uniform(0, 0.1, n), 2)
+    qty = rng.integers(1, 50, n).astype(np.int32)
+    cols = [both(price, N.F64, rng.random(n) > 0.02), both(disc, N.F64), both(qty, N.I32)]
+    p = Program()
+    pr, d, q = p.load(0, N.F64), p.load(1, N.F64), p.load(2, N.I32)
+    p.store(p.binary("*", pr, d, N.F64), 0)
+    c1 = p.binary("<", q, p.const(24, N.I32), N.I32)
+    c2 = p.binary(">=", d, p.const(0.05, N.F64), N.F64)
+    c3 = p.binary("<=", d, p.const(0.07, N.F64), N.F64)
+    c4 = p.binary(">", p.binary("+", pr, p.cast(q, N.F64), N.F64), p.const(100.0, N.F64), N.F64)
+    p.store(p.binary("and", p.binary("and", c1, c2, N.BOOL), p.binary("and", c3, c4, N.BOOL), N.BOOL), 1)
+    _agg_check(cols, p, [N.F64, N.BOOL], 1, [(SUM, 0), (COUNT_STAR, 0), (MIN, 0), (MAX, 0), (AVG, 0), (COUNT, 0)])
+
+
+def test_aggregate_expr_integer_and_decimal(gpu):
+    n = 1_500_000
+    rng = np.random.default_rng(21)
+    a = rng.integers(-(1 << 30), 1 << 30, n).astype(np.int64)
+    b = rng.integers(-1000, 1000, n).astype(np.int32)
+    prices = [int(x) for x in rng.integers(100, 10 ** 6, n)]  # Decimal128(12,2)
+    cols = [both(a, N.I64, rng.random(n) > 0.1), both(b, N.I32), both(dec_array(prices), N.DEC128, None, 2)]
+    p = Program()
+    ra, rb, rp = p.load(0, N.I64), p.load(1, N.I32), p.load(2, N.DEC128)
+    p.store(p.binary("*", ra, p.cast(rb, N.I64), N.I64), 0)                      # a * b
+    p.store(p.binary("%", p.binary("-", ra, p.cast(rb, N.I64), N.I64), p.const(97, N.I64), N.I64), 1)
+    p.store(p.binary("*", rp, p.const(3, N.DEC128), N.DEC128), 2)                 # price * 3
+    p.store(p.binary("!=", rb, p.const(0, N.I32), N.I32), 3)                      # WHERE b <> 0
+    _agg_check(cols, p, [N.I64, N.I64, N.DEC128, N.BOOL], 3,
+               [(SUM, 0), (MIN, 0), (MAX, 0), (SUM, 1), (VAR_POP, 1), (STDDEV_SAMP, 0), (SUM, 2), (COUNT, 0)])
+
+
+def test_aggregate_expr_no_filter_and_errors(gpu):
+    n = 100_000
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(n).astype(np.float32)
+    cols = [both(x, N.F32, rng.random(n) > 0.5)]
+    p = Program()
+    rx = p.load(0, N.F32)
+    p.store(p.binary("*", rx, rx, N.F32), 0)
+    _agg_check(cols, p, [N.F32], -1, [(SUM, 0), (COUNT, 0), (COUNT_STAR, 0), (AVG, 0)])
+    with pytest.raises(N.BoltError) as e:  # the filter value must be Bool
+        execute_aggregate_expr([cols[0][0]], p, 1, [(SUM, 0)], 0)
+    assert e.value.code == N.BOLT_ERR_PLAN
+    q = Program()
+    q.store(q.binary(">", q.load(0, N.F32), q.const(0.0, N.F32), N.F32), 0)
+    with pytest.raises(N.BoltError) as e:  # SUM over a Bool value
+        execute_aggregate_expr([cols[0][0]], q, 1, [(SUM, 0)])
+    assert e.value.code == N.BOLT_ERR_TYPE
