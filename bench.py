@@ -29,8 +29,8 @@ def _parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="join", choices=["join", "sort", "topk", "groupby", "groupby-zipf", "calib", "batch",
-                                                             "serve"])
+    p.add_argument("--workload", default="join", choices=["join", "sort", "topk", "expr", "groupby", "groupby-zipf", "calib",
+                                                             "batch", "serve"])
     p.add_argument("--limit", type=int, default=100, help="topk workload: LIMIT")
     p.add_argument("--fact-rows", type=int, default=200_000_000, help="serve workload: fact table rows")
     p.add_argument("--serve-queries", type=int, default=64, help="serve workload: queries per measurement")
@@ -272,6 +272,58 @@ def bench_topk(args) -> dict:
                   "n": n, "limit": k, "full_sort_then_slice_ms": full_ms},
                  _roof(n * 8, ms, "radix select (12-bit histogram passes) + candidate collect + candidate sort; "
                                   "8 B/row (one read of the key) is the compulsory traffic"))
+
+
+def bench_expr(args) -> dict:
+    """A TPC-H Q6 shape over 256M rows: SUM(price * disc) WHERE qty < 24 AND disc BETWEEN 0.05 AND 0.07
+    AND price + qty > 100, one bolt_aggregate_expr call (one fused generated kernel) per step; the
+    composed form (bolt_project of the two values, then bolt_aggregate on them) timed beside it."""
+    from craton_bolt_amd import _native as N
+    from craton_bolt_amd.device import DeviceColumn, Event, Stream
+    from craton_bolt_amd.ops import Program, Term, execute_aggregate, execute_aggregate_expr, execute_projection
+
+    n = args.n
+    rng = np.random.default_rng(6)
+    price = DeviceColumn.from_numpy(rng.uniform(1, 1000, n), N.F64)
+    disc = DeviceColumn.from_numpy(np.round(rng.uniform(0, 0.1, n), 2), N.F64)
+    qty = DeviceColumn.from_numpy(rng.integers(1, 50, n).astype(np.int32), N.I32)
+    p = Program()
+    pr, d, q = p.load(0, N.F64), p.load(1, N.F64), p.load(2, N.I32)
+    p.store(p.binary("*", pr, d, N.F64), 0)
+    c1 = p.binary("<", q, p.const(24, N.I32), N.I32)
+    c2 = p.binary(">=", d, p.const(0.05, N.F64), N.F64)
+    c3 = p.binary("<=", d, p.const(0.07, N.F64), N.F64)
+    c4 = p.binary(">", p.binary("+", pr, p.cast(q, N.F64), N.F64), p.const(100.0, N.F64), N.F64)
+    p.store(p.binary("and", p.binary("and", c1, c2, N.BOOL), p.binary("and", c3, c4, N.BOOL), N.BOOL), 1)
+    cols = [price, disc, qty]
+    aggs = [(1, 0), (3, 0)]
+    s = Stream()
+    kev = [Event(), Event()]
+    kms = []
+
+    def one():
+        execute_aggregate_expr(cols, p, 2, aggs, 1, stream=s)
+        kms.append(kev[0].elapsed_ms(kev[1]))
+
+    N.lib().bolt_profile_events(kev[0].handle, kev[1].handle)
+    _, wall = _timed(s, one, args.steps, args.warmup)
+    N.lib().bolt_profile_events(None, None)
+    kern = float(np.mean(kms[args.warmup:]))
+    res = execute_aggregate_expr(cols, p, 2, aggs, 1, stream=s)
+    for rep in range(2):  # the second call is timed (the first compiles the projection kernel)
+        t0 = time.perf_counter()
+        vals = execute_projection(cols, p, [N.F64, N.BOOL], stream=s)
+        comp = execute_aggregate(vals, aggs, [Term(1, 5, 1)], stream=s)
+        composed_ms = (time.perf_counter() - t0) * 1000.0
+        del vals
+    per_row = 8 + 8 + 4
+    return _line(n / (wall * 1e-3), "rows/s", args, wall, "f64",
+                 {"workload": f"TPC-H Q6 shape: {n} rows (price f64, disc f64, qty i32); SUM(price * disc), COUNT(*) "
+                              "WHERE qty < 24 AND disc BETWEEN 0.05 AND 0.07 AND price + qty > 100",
+                  "n": n, "fused_kernel_ms": kern, "wall_ms_per_call": wall,
+                  "composed_project_then_aggregate_ms": composed_ms, "result": res,
+                  "composed_agrees": comp[1] == res[1] and abs(comp[0] - res[0]) <= 1e-9 * abs(res[0])},
+                 _roof(n * per_row, kern, "fused program + predicate + aggregate kernel (20 B/row read once)"))
 
 
 def bench_groupby(args, zipf: bool) -> dict:
@@ -619,6 +671,8 @@ def main():
         line = bench_sort(args)
     elif args.workload == "topk":
         line = bench_topk(args)
+    elif args.workload == "expr":
+        line = bench_expr(args)
     elif args.workload == "groupby":
         line = bench_groupby(args, zipf=False)
     elif args.workload == "groupby-zipf":

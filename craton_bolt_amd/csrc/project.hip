@@ -202,7 +202,9 @@ struct Compiled {
 // `n_out` STOREs are expected (outputs 0 .. n_out - 1, each once); with
 // `out` non-null each must match its output column's dtype.
 int compile_program(const bolt_column_t* in, int n_in, const bolt_expr_op_t* prog, int n_ops,
-                    const bolt_column_t* out, int n_out, Compiled* cp) {
+                    const bolt_column_t* out, int n_out, Compiled* cp, const std::string& sfx = "",
+                    const std::string& row = "r") {
+  auto reg = [&](int r) { return "r" + std::to_string(r) + sfx; };
   std::vector<RegInfo> regs(BOLT_MAX_REGS);
   cp->reg.assign(n_out, -1);
   cp->dtype.assign(n_out, 0);
@@ -228,12 +230,12 @@ int compile_program(const bolt_column_t* in, int n_in, const bolt_expr_op_t* pro
                            dtype_name(o.dtype));
         const std::string c = std::to_string(o.col);
         if (dt == BOLT_DEC128)
-          e = "(__int128)(((unsigned __int128)((const u64*)p.in[" + c + "])[2 * r + 1] << 64) | ((const u64*)p.in[" + c +
-              "])[2 * r])";
+          e = "(__int128)(((unsigned __int128)((const u64*)p.in[" + c + "])[2 * " + row + " + 1] << 64) | ((const u64*)p.in[" +
+              c + "])[2 * " + row + "])";
         else if (dt == BOLT_BOOL)
-          e = "(((const unsigned char*)p.in[" + c + "])[r] != 0 ? 1 : 0)";
+          e = "(((const unsigned char*)p.in[" + c + "])[" + row + "] != 0 ? 1 : 0)";
         else
-          e = std::string("((const ") + stype(dt) + "*)p.in[" + c + "])[r]";
+          e = std::string("((const ") + stype(dt) + "*)p.in[" + c + "])[" + row + "]";
         d.dtype = dt;
         d.deps = 1u << o.col;
         break;
@@ -283,7 +285,7 @@ int compile_program(const bolt_column_t* in, int n_in, const bolt_expr_op_t* pro
       case BOLT_X_IS_NULL:
         if (o.col < 0 || o.col >= n_in) return set_error(BOLT_ERR_PLAN, "bolt_project: op %d tests column %d", k, o.col);
         if (in[o.col].validity)
-          e = std::string("(bitv(p.iv[") + std::to_string(o.col) + "], r) ? " + (o.op ? "0 : 1)" : "1 : 0)");
+          e = std::string("(bitv(p.iv[") + std::to_string(o.col) + "], " + row + ") ? " + (o.op ? "0 : 1)" : "1 : 0)");
         else
           e = o.op ? "0" : "1";  // a column without a bitmap has no NULLs
         d.dtype = BOLT_BOOL;
@@ -310,11 +312,11 @@ int compile_program(const bolt_column_t* in, int n_in, const bolt_expr_op_t* pro
   return BOLT_OK;
 }
 
-// C condition "every input column that output `c` loads is non-NULL at row r".
-std::string valid_expr(const bolt_column_t* in, int n_in, uint32_t deps) {
+// C condition "every input column in `deps` is non-NULL at row `row`".
+std::string valid_expr(const bolt_column_t* in, int n_in, uint32_t deps, const std::string& row = "r") {
   std::string v = "1";
   for (int c = 0; c < n_in; c++)
-    if ((deps >> c) & 1u && in[c].validity) v += " && bitv(p.iv[" + std::to_string(c) + "], r)";
+    if ((deps >> c) & 1u && in[c].validity) v += " && bitv(p.iv[" + std::to_string(c) + "], " + row + ")";
   return v;
 }
 
@@ -358,41 +360,56 @@ int gen_source(const bolt_column_t* in, int n_in, const bolt_expr_op_t* prog, in
 // The fused WHERE + aggregate kernel: the program per row, the predicate
 // (a Bool output, true and non-NULL), then each aggregate over its output
 // (skipped where that output is NULL); per-workgroup partials as the
-// precompiled scalar aggregate writes them.
-int gen_agg_source(const bolt_column_t* in, int n_in, const Compiled& cp, int filter_col, const AggList& aggs,
-                   const bolt_agg_t* spec, std::string* src) {
+// precompiled scalar aggregate writes them.  Each lane runs kAggUnroll rows
+// per iteration, 256 apart, whose loads the compiler issues together.
+constexpr int kAggUnroll = 4;
+
+int gen_agg_source(const bolt_column_t* in, int n_in, const bolt_expr_op_t* prog, int n_ops, int n_values,
+                   int filter_col, const AggList& aggs, const bolt_agg_t* spec, std::string* src) {
   std::ostringstream s;
   s << jit_prelude_text() << "\n" << kProjectPrelude
     << "extern \"C\" __global__ void __launch_bounds__(256) bolt_jit_kernel(PArgs p, bolt::Acc* __restrict__ parts) {\n"
     << "  using namespace bolt;\n";
   for (int a = 0; a < aggs.n; a++) s << "  Acc acc" << a << "; acc_init(acc" << a << ", " << aggs.a[a].op << ");\n";
-  s << "  const long long stride = (long long)gridDim.x * 256;\n"
-    << "  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < p.n; i += stride) {\n"
-    << "    const long long r = i;\n"
-    << cp.body;
-  if (filter_col >= 0)
-    s << "    if (!(" << reg(cp.reg[filter_col]) << " && (" << valid_expr(in, n_in, cp.deps[filter_col])
-      << "))) continue;\n";
-  for (int a = 0; a < aggs.n; a++) {
-    const std::string A = "acc" + std::to_string(a);
-    if (aggs.a[a].op == BOLT_AGG_COUNT_STAR) {
-      s << "    " << A << ".n++;\n";
-      continue;
+  s << "  const long long tile = 256LL * " << kAggUnroll << ";\n"
+    << "  for (long long base = (long long)blockIdx.x * tile + threadIdx.x; base < p.n; base += (long long)gridDim.x * tile) {\n";
+  std::vector<Compiled> cps(kAggUnroll);
+  for (int u = 0; u < kAggUnroll; u++) {
+    const std::string sfx = "_" + std::to_string(u), row = "r" + sfx;
+    s << "    const bool in" << sfx << " = base + " << 256 * u << " < p.n;\n"
+      << "    const long long " << row << " = in" << sfx << " ? base + " << 256 * u << " : 0;\n";
+    BOLT_TRY(compile_program(in, n_in, prog, n_ops, nullptr, n_values, &cps[u], sfx, row));
+    s << cps[u].body;
+  }
+  for (int u = 0; u < kAggUnroll; u++) {
+    const Compiled& cp = cps[u];
+    const std::string sfx = "_" + std::to_string(u), row = "r" + sfx;
+    auto rg = [&](int c) { return "r" + std::to_string(cp.reg[c]) + sfx; };
+    s << "    if (in" << sfx;
+    if (filter_col >= 0) s << " && " << rg(filter_col) << " && (" << valid_expr(in, n_in, cp.deps[filter_col], row) << ")";
+    s << ") {\n";
+    for (int a = 0; a < aggs.n; a++) {
+      const std::string A = "acc" + std::to_string(a);
+      if (aggs.a[a].op == BOLT_AGG_COUNT_STAR) {
+        s << "      " << A << ".n++;\n";
+        continue;
+      }
+      const int c = spec[a].col, dt = cp.dtype[c];
+      const std::string v = rg(c);
+      std::string vi = "0", vhi = "0", vf = "0.0";
+      if (dt == BOLT_DEC128) {
+        vi = "(long long)(u64)" + v;
+        vhi = "(long long)(u64)((unsigned __int128)" + v + " >> 64)";
+      } else if (is_float(dt)) {
+        vf = "(double)" + v;
+      } else {
+        vi = "(long long)" + v;
+        vf = "(double)(long long)" + v;
+      }
+      s << "      if (" << valid_expr(in, n_in, cp.deps[c], row) << ") acc_update(" << A << ", " << aggs.a[a].op << ", "
+        << dt << ", " << vi << ", " << vhi << ", " << vf << ");\n";
     }
-    const int c = spec[a].col, dt = cp.dtype[c];
-    const std::string rg = reg(cp.reg[c]);
-    std::string vi = "0", vhi = "0", vf = "0.0";
-    if (dt == BOLT_DEC128) {
-      vi = "(long long)(u64)" + rg;
-      vhi = "(long long)(u64)((unsigned __int128)" + rg + " >> 64)";
-    } else if (is_float(dt)) {
-      vf = "(double)" + rg;
-    } else {
-      vi = "(long long)" + rg;
-      vf = "(double)(long long)" + rg;
-    }
-    s << "    if (" << valid_expr(in, n_in, cp.deps[c]) << ") acc_update(" << A << ", " << aggs.a[a].op << ", " << dt
-      << ", " << vi << ", " << vhi << ", " << vf << ");\n";
+    s << "    }\n";
   }
   s << "  }\n  __shared__ Acc sh[4];\n";
   for (int a = 0; a < aggs.n; a++)
@@ -502,7 +519,7 @@ extern "C" int bolt_aggregate_expr(bolt_stream_t stream, const bolt_column_t* in
   BOLT_TRY(make_agg_list(aggs, n_aggs, vcols.data(), n_values, 0, &all));
   if (n_aggs == 0) return BOLT_OK;
   std::string src;
-  BOLT_TRY(gen_agg_source(inputs, n_inputs, cp, filter_col, all, aggs, &src));
+  BOLT_TRY(gen_agg_source(inputs, n_inputs, prog, n_ops, n_values, filter_col, all, aggs, &src));
   hipFunction_t fn;
   BOLT_TRY(jit_get(src, &fn));
   PArgs a;
@@ -512,7 +529,7 @@ extern "C" int bolt_aggregate_expr(bolt_stream_t stream, const bolt_column_t* in
     a.iv[c] = inputs[c].validity;
   }
   a.n = n;
-  const unsigned grid = grid_for(n, 256, 4, 256 * 32);
+  const unsigned grid = grid_for(n, 256, kAggUnroll, 256 * 32);
   DevBuf<Acc> parts, outs;
   BOLT_TRY(parts.alloc((size_t)grid * n_aggs, s));
   BOLT_TRY(outs.alloc((size_t)n_aggs, s));
