@@ -29,8 +29,8 @@ def _parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="join", choices=["join", "sort", "topk", "expr", "groupby", "groupby-zipf", "calib",
-                                                             "batch", "serve"])
+    p.add_argument("--workload", default="join", choices=["join", "sort", "topk", "expr", "h2o", "groupby", "groupby-zipf",
+                                                             "calib", "batch", "serve"])
     p.add_argument("--limit", type=int, default=100, help="topk workload: LIMIT")
     p.add_argument("--fact-rows", type=int, default=200_000_000, help="serve workload: fact table rows")
     p.add_argument("--serve-queries", type=int, default=64, help="serve workload: queries per measurement")
@@ -324,6 +324,67 @@ def bench_expr(args) -> dict:
                   "composed_project_then_aggregate_ms": composed_ms, "result": res,
                   "composed_agrees": comp[1] == res[1] and abs(comp[0] - res[0]) <= 1e-9 * abs(res[0])},
                  _roof(n * per_row, kern, "fused program + predicate + aggregate kernel (20 B/row read once)"))
+
+
+# h2o.ai groupby q1..q5 (benches/olap_benchmarks.rs:77-90) and the reference's published times for them
+# (docs/BENCHMARKS.md:28-32: end-to-end Engine::sql on an RTX 2060, host -> device upload included)
+H2O_QUERIES = {
+    "q1": (["id1"], ["v1"], [(1, 0)], 51.4),
+    "q2": (["id2"], ["v1", "v2"], [(1, 0), (1, 1)], 384.0),
+    "q3": (["id1", "id2"], ["v1"], [(1, 0)], 219.0),
+    "q4": (["id1"], ["v1", "v2", "v3"], [(6, 0), (6, 1), (6, 2)], 70.5),
+    "q5": (["id3"], ["v1"], [(1, 0)], 237.0),
+}
+
+
+def bench_h2o(args) -> dict:
+    """The reference's published benchmark: h2o groupby q1..q5 over 10M rows.  Per query: the GROUP BY
+    with its columns resident in HBM, and end to end as the reference measures it (upload of the query's
+    columns from pageable host memory, the GROUP BY in key order, the result copied back)."""
+    from craton_bolt_amd import _native as N
+    from craton_bolt_amd import datagen as D
+    from craton_bolt_amd.device import DeviceColumn, Stream
+    from craton_bolt_amd.ops import execute_groupby
+
+    n = 10_000_000
+    host = {c: D.h2o_column(c, n) for c in ("id1", "id2", "id3", "v1", "v2", "v3")}
+    dt = {c: (N.I32 if c.startswith("id") else N.F64) for c in host}
+    dev = {c: DeviceColumn.from_numpy(host[c], dt[c]) for c in host}
+    s = Stream()
+    per = {}
+    for q, (keys, vals, aggs, ref_ms) in H2O_QUERIES.items():
+        def resident():
+            execute_groupby([dev[k] for k in keys], [dev[v] for v in vals], aggs, stream=s)
+
+        def end_to_end():
+            cols = {c: DeviceColumn.from_numpy(host[c], dt[c]) for c in keys + vals}
+            r = execute_groupby([cols[k] for k in keys], [cols[v] for v in vals], aggs, stream=s)
+            return r["n_groups"]
+
+        for _ in range(args.warmup):
+            resident()
+            end_to_end()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            resident()
+        res_ms = (time.perf_counter() - t0) * 1000.0 / args.steps
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            g = end_to_end()
+        e2e_ms = (time.perf_counter() - t0) * 1000.0 / args.steps
+        per[q] = {"groups": g, "resident_ms": res_ms, "end_to_end_ms": e2e_ms, "reference_ms": ref_ms,
+                  "speedup_end_to_end": ref_ms / e2e_ms}
+    tot_res = sum(v["resident_ms"] for v in per.values())
+    tot_e2e = sum(v["end_to_end_ms"] for v in per.values())
+    tot_ref = sum(v["reference_ms"] for v in per.values())
+    line = _line(5 * n / (tot_res * 1e-3), "rows/s", args, tot_res / 5, "f64",
+                 {"workload": "h2o.ai groupby q1..q5 over 10M rows (benches/olap_benchmarks.rs generator), "
+                              "output in key order; value = 5 x 10M rows / sum of resident query times",
+                  "n": n, "queries": per, "sum_end_to_end_ms": tot_e2e, "sum_reference_ms": tot_ref,
+                  "reference_setup": "end-to-end Engine::sql, RTX 2060 (docs/BENCHMARKS.md:12-32)"},
+                 None)
+    line["vs_baseline"] = tot_ref / tot_e2e  # published end-to-end times over ours, same protocol
+    return line
 
 
 def bench_groupby(args, zipf: bool) -> dict:
@@ -673,6 +734,8 @@ def main():
         line = bench_topk(args)
     elif args.workload == "expr":
         line = bench_expr(args)
+    elif args.workload == "h2o":
+        line = bench_h2o(args)
     elif args.workload == "groupby":
         line = bench_groupby(args, zipf=False)
     elif args.workload == "groupby-zipf":

@@ -147,3 +147,29 @@ def star_dim(d: int, n: int) -> tuple[np.ndarray, list[np.ndarray]]:
 
 def scramble64(x: int) -> int:
     return int(lib().bolt_scramble64(x))
+
+
+# ---- h2o.ai groupby "small" columns (benches/olap_benchmarks.rs:63-123) -----------------------------
+#
+# Row i's values are fixed functions of i (no seed): a 64-bit wrapping product,
+# truncated to its low 32 bits as a signed int, then a Euclidean remainder.
+
+H2O_CARD = {"id1": 100, "id2": 10_000, "id3": 1_000_000}
+_H2O_MUL = {"id1": 2_654_435_761, "id2": 40_503, "id3": 11_400_714_819_323_198_485, "v1": 7, "v2": 13, "v3": 17}
+
+
+def _h2o_i32(i: np.ndarray, mult: int, modulus: int) -> np.ndarray:
+    low = (i * np.uint64(mult)).astype(np.uint32).view(np.int32).astype(np.int64)
+    return low % modulus  # non-negative for a positive modulus (rem_euclid)
+
+
+def h2o_column(name: str, n: int) -> np.ndarray:
+    """Column `name` (id1..id3 Int32, v1..v3 Float64) of the h2o groupby table of n rows."""
+    i = np.arange(n, dtype=np.uint64)
+    if name in H2O_CARD:
+        return _h2o_i32(i, _H2O_MUL[name], H2O_CARD[name]).astype(np.int32)
+    if name == "v1":
+        return (_h2o_i32(i, 7, 5) + 1).astype(np.float64)
+    if name == "v2":
+        return (_h2o_i32(i, 13, 15) + 1).astype(np.float64)
+    return _h2o_i32(i, 17, 10_000).astype(np.float64) / 100.0

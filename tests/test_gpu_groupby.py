@@ -253,3 +253,27 @@ def test_zipf_high_cardinality_float_sum(gpu):
     got = res["aggs"][0][0]
     rel = np.abs(got - want["f64"][0]) / np.maximum(np.abs(want["f64"][0]), 1e-300)
     print(f"[zipf] groups={res['n_groups']} max_rel_err={rel.max():.3e} median={np.median(rel):.3e}", flush=True)
+
+
+@pytest.mark.parametrize("query", ["q1", "q2", "q3", "q4", "q5"])
+def test_gpu_h2o_queries_match_oracle(gpu, query):
+    """h2o.ai groupby q1..q5 (benches/olap_benchmarks.rs:77-90) on the reference's generator, 2M rows."""
+    from craton_bolt_amd import datagen as D
+    from craton_bolt_amd.ops import execute_groupby
+    keys, vals, aggs = {"q1": (["id1"], ["v1"], [(1, 0)]), "q2": (["id2"], ["v1", "v2"], [(1, 0), (1, 1)]),
+                        "q3": (["id1", "id2"], ["v1"], [(1, 0)]),
+                        "q4": (["id1"], ["v1", "v2", "v3"], [(6, 0), (6, 1), (6, 2)]),
+                        "q5": (["id3"], ["v1"], [(1, 0)])}[query]
+    print(f"[case] h2o {query}", flush=True)
+    n = 2_000_000
+    kd = [both(D.h2o_column(k, n), N.I32) for k in keys]
+    vd = [both(D.h2o_column(v, n), N.F64) for v in vals]
+    got = execute_groupby([d for d, _ in kd], [d for d, _ in vd], aggs)
+    want = O.groupby([h for _, h in kd], [h for _, h in vd], aggs)
+    assert got["n_groups"] == want["n_groups"]
+    if len(keys) == 1:
+        assert np.array_equal(got["keys"][0][0].astype(np.int64), want["keys"])
+    for a in range(len(aggs)):
+        g = got["aggs"][a][0]
+        w = want["f64"][a]
+        assert np.allclose(g, w, rtol=1e-12, atol=0), query
