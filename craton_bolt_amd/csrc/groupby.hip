@@ -709,10 +709,14 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
   for (int a = 0; a < P.n; a++) has_var |= P.a[a].kind == K_VAR;
   // many spilled rows over many groups: the radix-partitioned tier instead of
   // global atomics per spilled row (its records are built once, whatever the
-  // table size)
+  // table size).  Its passes cost a fixed ~2 ms more than the atomics save
+  // below a few tens of millions of spilled rows (h2o q2 / q3 / q5 at 10M
+  // rows: 5.0 / 3.8 / 5.6 ms partitioned, 3.0 / 3.0 / 4.4 ms with the global
+  // table; 256M Zipf rows: 10.4 ms partitioned, 14.7 ms global), so it takes
+  // over from 2^25 spilled rows.
   const bool partitioned = use_lds && n > 0 &&
                            ((flags & BOLT_GROUPBY_FORCE_PARTITIONED) ||
-                            (n_spill > (int64_t)(1 << 20) && est > 0.75 * (double)S));
+                            (n_spill > (int64_t)(1 << 25) && est > 0.75 * (double)S));
   GbPartitions parts;
   if (partitioned)
     BOLT_TRY(gb_part_build(s, ks, P, pkeys.ptr, ptag.ptr, pwords.ptr, (int64_t)pkeys.count, n_parts, spill.ptr,
