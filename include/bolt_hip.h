@@ -75,7 +75,11 @@ int bolt_hip_host_free(void* ptr);
 /* cuMemGetInfo_v2 (cuda_sys.rs:188). */
 int bolt_hip_mem_get_info(size_t* free_bytes, size_t* total_bytes);
 
-/* cuMemcpy{HtoD,DtoH,DtoD}_v2 and the Async forms (cuda_sys.rs:130-148). */
+/* cuMemcpy{HtoD,DtoH,DtoD}_v2 and the Async forms (cuda_sys.rs:130-148).
+ * bolt_hip_memcpy_htod is synchronous and ordered after the device's earlier
+ * work; from pageable host memory (256 KB and up) it stages the copy through
+ * pinned buffers with up to four host threads, so the host-side copies and
+ * the DMA overlap (pinned sources are copied directly). */
 int bolt_hip_memcpy_htod(void* dst, const void* src, size_t bytes);
 int bolt_hip_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 int bolt_hip_memcpy_dtod(void* dst, const void* src, size_t bytes);
