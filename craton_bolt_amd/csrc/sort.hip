@@ -790,6 +790,90 @@ __global__ __launch_bounds__(256) void topk_collect_kernel(TopKey k, int64_t n, 
   }
 }
 
+// ---- small final sets: narrowed over the candidates, sorted in one workgroup ----------
+constexpr int kSmallSort = 4096;  // (image, row) pairs sorted in LDS by one workgroup
+
+// hist[digit] over the candidates whose image >> (shift + width) == prefix
+template <int W>
+__global__ __launch_bounds__(256) void topk_cand_hist_kernel(TopKey k, const uint32_t* __restrict__ cand, int64_t c,
+                                                             int shift, int width, uint64_t prefix,
+                                                             uint32_t* __restrict__ hist) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= c) return;
+  const uint64_t x = coarse_of(k, load_raw<W>(k.d, cand[j]), true);
+  const int hi = shift + width;
+  if (hi < 64 && (x >> hi) != prefix) return;
+  atomicAdd(&hist[(x >> shift) & ((1ull << width) - 1)], 1u);
+}
+
+// appends the candidates whose image >> shift <= bound
+template <int W>
+__global__ __launch_bounds__(256) void topk_cand_collect_kernel(TopKey k, const uint32_t* __restrict__ cand,
+                                                                int64_t c, int shift, uint64_t bound,
+                                                                uint32_t* __restrict__ out,
+                                                                unsigned long long* __restrict__ count, int64_t cap) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  bool take = false;
+  uint32_t r = 0;
+  if (j < c) {
+    r = cand[j];
+    take = (shift >= 64 ? 0ull : coarse_of(k, load_raw<W>(k.d, r), true) >> shift) <= bound;
+  }
+  const uint64_t m = __ballot(take);
+  if (m == 0) return;
+  const int leader = __builtin_ctzll(m);
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(m));
+  base = __shfl(base, leader);
+  if (take) {
+    const int64_t pos = (int64_t)base + __popcll(m & ((1ull << lane) - 1));
+    if (pos < cap) out[pos] = r;
+  }
+}
+
+// Sorts c <= kSmallSort rows by (image, row id) with a bitonic network in
+// LDS (padded to a power of two with pairs that sort last) and writes rows
+// [first, first + m) of the order.
+template <int W>
+__global__ __launch_bounds__(1024) void topk_small_sort_kernel(TopKey k, const uint32_t* __restrict__ rows, int c,
+                                                               int npow, int first, int m,
+                                                               uint32_t* __restrict__ out) {
+  __shared__ uint64_t key[kSmallSort];
+  __shared__ uint32_t row[kSmallSort];
+  for (int i = threadIdx.x; i < npow; i += 1024) {
+    if (i < c) {
+      const uint32_t r = rows[i];
+      key[i] = coarse_of(k, load_raw<W>(k.d, r), true);
+      row[i] = r;
+    } else {
+      key[i] = ~0ull;
+      row[i] = 0xFFFFFFFFu;
+    }
+  }
+  __syncthreads();
+  for (int size = 2; size <= npow; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = threadIdx.x; i < npow; i += 1024) {
+        const int j = i ^ stride;
+        if (j > i) {
+          const bool up = (i & size) == 0;
+          const bool gt = key[i] > key[j] || (key[i] == key[j] && row[i] > row[j]);
+          if (gt == up) {
+            const uint64_t tk = key[i];
+            key[i] = key[j];
+            key[j] = tk;
+            const uint32_t tr = row[i];
+            row[i] = row[j];
+            row[j] = tr;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  for (int j = threadIdx.x; j < m; j += 1024) out[j] = row[first + j];
+}
+
 // out[j] = rows[perm[first + j]]
 __global__ void topk_emit_kernel(const uint32_t* __restrict__ rows, const uint32_t* __restrict__ perm, int64_t first,
                                  int64_t m, uint32_t* __restrict__ out) {
@@ -995,6 +1079,69 @@ extern "C" int bolt_sort_limit(bolt_stream_t stream, const bolt_column_t* keys, 
   BOLT_HIP(hipStreamSynchronize(s));
   if ((int64_t)got != C)
     return set_error(BOLT_ERR_OTHER, "bolt_sort_limit: collected %llu candidates, expected %lld", got, (long long)C);
+  // One key without NULLs (its image is then exact): narrow the candidates
+  // further and sort the few that remain in one workgroup.
+  if (n_keys == 1 && !c0.validity && kw != 16 && c0.dtype != BOLT_BOOL && K <= kSmallSort / 2) {
+    // continue the select inside the bucket of the first one: `below` rows
+    // precede it, the K-th row is the need-th of its `eq`
+    uint64_t p2 = prefix;
+    int hi2 = hi;
+    int64_t below2 = below, need2 = need, eq2 = eq;
+    const unsigned cgrid = (unsigned)((C + 255) / 256);
+    while (hi2 > 0 && below2 + eq2 > kSmallSort) {
+      const int width = hi2 >= kSelBits ? kSelBits : hi2;
+      const int shift = hi2 - width;
+      BOLT_TRY(fill_u32(hist.ptr, 0, (size_t)1 << width, s));
+      switch (kw) {
+        case 1: topk_cand_hist_kernel<1><<<cgrid, 256, 0, s>>>(tk, cand.ptr, C, shift, width, p2, hist.ptr); break;
+        case 4: topk_cand_hist_kernel<4><<<cgrid, 256, 0, s>>>(tk, cand.ptr, C, shift, width, p2, hist.ptr); break;
+        default: topk_cand_hist_kernel<8><<<cgrid, 256, 0, s>>>(tk, cand.ptr, C, shift, width, p2, hist.ptr); break;
+      }
+      BOLT_HIP(hipGetLastError());
+      BOLT_HIP(hipMemcpyAsync(h.data(), hist.ptr, (size_t)(1 << width) * 4, hipMemcpyDeviceToHost, s));
+      BOLT_HIP(hipStreamSynchronize(s));
+      int64_t cum = 0;
+      int d = 0;
+      for (; d < (1 << width); d++) {
+        if (cum + (int64_t)h[d] >= need2) break;
+        cum += h[d];
+      }
+      if (d == (1 << width)) return set_error(BOLT_ERR_OTHER, "bolt_sort_limit: candidate select lost rows");
+      below2 += cum;
+      need2 -= cum;
+      eq2 = h[d];
+      p2 = (p2 << width) | (uint64_t)d;
+      hi2 = shift;
+    }
+    const int64_t C2 = below2 + eq2;
+    if (C2 <= kSmallSort) {
+      DevBuf<uint32_t> cand2;
+      BOLT_TRY(cand2.alloc((size_t)C2, s));
+      BOLT_TRY(fill_u32(cnt.ptr, 0, 2, s));
+      switch (kw) {
+        case 1: topk_cand_collect_kernel<1><<<cgrid, 256, 0, s>>>(tk, cand.ptr, C, hi2, p2, cand2.ptr, cnt.ptr, C2); break;
+        case 4: topk_cand_collect_kernel<4><<<cgrid, 256, 0, s>>>(tk, cand.ptr, C, hi2, p2, cand2.ptr, cnt.ptr, C2); break;
+        default: topk_cand_collect_kernel<8><<<cgrid, 256, 0, s>>>(tk, cand.ptr, C, hi2, p2, cand2.ptr, cnt.ptr, C2); break;
+      }
+      BOLT_HIP(hipGetLastError());
+      int npow = 1;
+      while (npow < C2) npow <<= 1;
+      switch (kw) {
+        case 1: topk_small_sort_kernel<1><<<1, 1024, 0, s>>>(tk, cand2.ptr, (int)C2, npow, (int)offset, (int)m, out_rows); break;
+        case 4: topk_small_sort_kernel<4><<<1, 1024, 0, s>>>(tk, cand2.ptr, (int)C2, npow, (int)offset, (int)m, out_rows); break;
+        default: topk_small_sort_kernel<8><<<1, 1024, 0, s>>>(tk, cand2.ptr, (int)C2, npow, (int)offset, (int)m, out_rows); break;
+      }
+      BOLT_HIP(hipGetLastError());
+      unsigned long long got2 = 0;
+      BOLT_HIP(hipMemcpyAsync(&got2, cnt.ptr, 8, hipMemcpyDeviceToHost, s));
+      BOLT_HIP(hipStreamSynchronize(s));
+      if ((int64_t)got2 != C2)
+        return set_error(BOLT_ERR_OTHER, "bolt_sort_limit: narrowed to %llu candidates, expected %lld", got2,
+                         (long long)C2);
+      *n_out = m;
+      return BOLT_OK;
+    }
+  }
   // Sort the candidates by every key, then by row id (the full sort is stable).
   std::vector<DevBuf<uint8_t>> vals(n_keys), valid(n_keys);
   std::vector<bolt_column_t> ck(n_keys + 1);

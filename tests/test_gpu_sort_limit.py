@@ -108,3 +108,28 @@ def test_sort_limit_equals_full_sort_slice(gpu):
     dev = DeviceColumn.from_numpy(rng.integers(0, 50_000, n).astype(np.int64), N.I64)
     full = execute_sort([dev])
     assert np.array_equal(execute_sort_limit([dev], 2500, 100), full[100:2600])
+
+
+@pytest.mark.parametrize("dtype", ["i64", "i32", "f64", "f32", "date32"])
+@pytest.mark.parametrize("limit,offset,desc", [(1, 0, False), (100, 0, True), (2000, 48, False), (30, 7, True)])
+def test_sort_limit_small_final_set(gpu, dtype, limit, offset, desc):
+    """One key without NULLs: the candidates are narrowed and the last few
+    sorted in one workgroup; heavy duplicates put ties across the window's
+    edge, which the row id must order."""
+    print(f"[case] small final set {dtype} limit={limit} offset={offset} desc={desc}", flush=True)
+    n = 3_000_000
+    rng = np.random.default_rng(len(dtype) + limit)
+    if dtype == "i64":
+        cols = [both(rng.integers(-(1 << 62), 1 << 62, n).astype(np.int64), N.I64)]
+    elif dtype == "i32":
+        cols = [both(rng.integers(0, 5000, n).astype(np.int32), N.I32)]  # about 600 copies of each value
+    elif dtype == "date32":
+        cols = [both(rng.integers(0, 40_000, n).astype(np.int32), N.DATE32)]
+    elif dtype == "f64":
+        v = np.round(rng.standard_normal(n), 3)
+        v[rng.integers(0, n, 40)] = np.nan
+        v[rng.integers(0, n, 40)] = -0.0
+        cols = [both(v, N.F64)]
+    else:
+        cols = [both(rng.standard_normal(n).astype(np.float32), N.F32)]
+    _check(cols, [desc], [True], limit, offset)
