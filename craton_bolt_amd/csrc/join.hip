@@ -82,18 +82,26 @@ __global__ __launch_bounds__(kBuildBlock) void bloom_build_kernel(uint4* __restr
   }
 }
 
-// Marks the table non-unique if any build key is stored more than once.
-__global__ __launch_bounds__(kBuildBlock) void join_dup_kernel(const uint4* __restrict__ tbl, uint64_t mask,
-                                                               const void* __restrict__ key, int kdt,
-                                                               const uint8_t* __restrict__ kvalid, int64_t n,
-                                                               uint32_t* __restrict__ nonunique) {
-  const int64_t stride = (int64_t)gridDim.x * kBuildBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBuildBlock + threadIdx.x; i < n; i += stride) {
-    if (!bit_valid(kvalid, i)) continue;
-    const int64_t k = load_key(key, kdt, i);
-    int c = 0;
-    probe_chain(tbl, mask, k, false, [&](uint32_t) { c++; });
-    if (c > 1) *nonunique = 1u;
+// Marks the table non-unique if any key is stored twice, by one sequential
+// pass over the finished table: an occupied slot s holding k was claimed after
+// its insert walked every slot from k's home to s, all occupied, so an earlier
+// copy of k can only sit in [home(k), s).  That range is usually empty or in
+// s's own cache line, so the pass costs about one read of the table instead of
+// a random chain walk per build row.
+__global__ __launch_bounds__(kBuildBlock) void join_dup_scan_kernel(const uint4* __restrict__ tbl, uint64_t mask,
+                                                                    uint32_t* __restrict__ nonunique) {
+  const uint64_t stride = (uint64_t)gridDim.x * kBuildBlock;
+  for (uint64_t s = (uint64_t)blockIdx.x * kBuildBlock + threadIdx.x; s <= mask; s += stride) {
+    const uint4 e = tbl[s];
+    if (e.w == 0u) continue;
+    const int64_t k = slot_key(e);
+    for (uint64_t t = fmix64((uint64_t)k) & mask; t != s; t = (t + 1) & mask) {
+      const uint4 f = tbl[t];
+      if (slot_key(f) == k) {
+        *nonunique = 1u;
+        break;
+      }
+    }
   }
 }
 
@@ -444,8 +452,8 @@ extern "C" int bolt_join_build(bolt_stream_t stream, const bolt_column_t* build_
                                                    build_key->validity,
                                                    carry ? static_cast<const uint32_t*>(carry->data) : nullptr, n,
                                                    t->flags + 1);
-    join_dup_kernel<<<grid, kBuildBlock, 0, s>>>(t->tbl, t->mask, build_key->data, t->key_dtype, build_key->validity,
-                                                 n, t->flags);
+    join_dup_scan_kernel<<<grid_for((int64_t)t->mask + 1, kBuildBlock, 4, 8192), kBuildBlock, 0, s>>>(t->tbl, t->mask,
+                                                                                                   t->flags);
     if (t->bloom)
       bloom_build_kernel<<<grid, kBuildBlock, 0, s>>>(t->bloom, t->bloom_mask, build_key->data, t->key_dtype,
                                                       build_key->validity, n);
