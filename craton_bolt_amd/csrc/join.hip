@@ -82,6 +82,202 @@ __global__ __launch_bounds__(kBuildBlock) void bloom_build_kernel(uint4* __restr
   }
 }
 
+// ---- region build ---------------------------------------------------------------
+//
+// Large tables are built region by region instead of by one global claim
+// atomic per row: the rows are bucketed by the 4096-slot region of their home
+// slot (an LDS histogram per workgroup, offsets, an LDS-cursor scatter), then
+// one workgroup per region inserts its rows into an LDS copy of the region,
+// checks it for duplicate keys, and writes the whole region out, empty slots
+// included, so the table needs no separate fill.  Rows whose probe runs past
+// the end of their region (about one in ten thousand at load 0.15) are
+// appended to an overflow list and inserted afterwards with global claims.
+constexpr int kRegBits = 12;
+constexpr int kRegSlots = 1 << kRegBits;
+constexpr int kMaxRegions = 32768;  // u32 counters of every region in one workgroup's LDS
+constexpr int kRegGrid = 256;       // workgroups of the histogram and scatter passes
+constexpr int kRegBlock = 1024;
+
+// counts[b][r] = valid rows of workgroup b's chunk whose home lies in region r
+__global__ __launch_bounds__(kRegBlock) void join_region_hist_kernel(const void* __restrict__ key, int kdt,
+                                                                     const uint8_t* __restrict__ kvalid, int64_t n,
+                                                                     int64_t per, uint64_t mask, int n_regions,
+                                                                     uint32_t* __restrict__ counts) {
+  extern __shared__ uint32_t sc[];
+  for (int r = threadIdx.x; r < n_regions; r += kRegBlock) sc[r] = 0;
+  __syncthreads();
+  const int64_t lo = (int64_t)blockIdx.x * per;
+  const int64_t hi = lo + per < n ? lo + per : n;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kRegBlock) {
+    if (!bit_valid(kvalid, i)) continue;
+    const uint64_t h = fmix64((uint64_t)load_key(key, kdt, i)) & mask;
+    atomicAdd(&sc[h >> kRegBits], 1u);
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < n_regions; r += kRegBlock) counts[(int64_t)blockIdx.x * n_regions + r] = sc[r];
+}
+
+// counts[b][r] becomes the rows of region r in chunks before b; totals[r] the region's rows
+__global__ __launch_bounds__(256) void join_region_offsets_kernel(uint32_t* __restrict__ counts, int blocks,
+                                                                  int n_regions, uint32_t* __restrict__ totals) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= n_regions) return;
+  uint32_t run = 0;
+  for (int b0 = 0; b0 < blocks; b0 += 16) {  // 16 loads in flight per lane
+    uint32_t c[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) c[j] = b0 + j < blocks ? counts[(int64_t)(b0 + j) * n_regions + r] : 0u;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      if (b0 + j < blocks) counts[(int64_t)(b0 + j) * n_regions + r] = run;
+      run += c[j];
+    }
+  }
+  totals[r] = run;
+}
+
+// base[r] = rows of regions before r (base[n_regions] = all); one workgroup
+__global__ __launch_bounds__(kRegBlock) void join_region_scan_kernel(const uint32_t* __restrict__ totals,
+                                                                     int n_regions, uint32_t* __restrict__ base) {
+  __shared__ uint32_t part[kRegBlock];
+  const int per = (n_regions + kRegBlock - 1) / kRegBlock;
+  const int lo = threadIdx.x * per;
+  uint32_t sum = 0;
+  for (int r = lo; r < lo + per && r < n_regions; r++) sum += totals[r];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  // inclusive scan of the 1024 partial sums (Hillis-Steele in LDS)
+  for (int d = 1; d < kRegBlock; d <<= 1) {
+    const uint32_t v = threadIdx.x >= (unsigned)d ? part[threadIdx.x - d] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  if (threadIdx.x == kRegBlock - 1) base[n_regions] = part[threadIdx.x];
+  uint32_t run = part[threadIdx.x] - sum;  // exclusive
+  for (int r = lo; r < lo + per && r < n_regions; r++) {
+    base[r] = run;
+    run += totals[r];
+  }
+}
+
+// Scatters the valid rows of workgroup b's chunk into their regions' buckets
+// as slot-shaped 16-byte records.
+__global__ __launch_bounds__(kRegBlock) void join_region_scatter_kernel(
+    const void* __restrict__ key, int kdt, const uint8_t* __restrict__ kvalid, const uint32_t* __restrict__ carry,
+    int64_t n, int64_t per, uint64_t mask, int n_regions, const uint32_t* __restrict__ counts,
+    const uint32_t* __restrict__ base, uint4* __restrict__ brec) {
+  extern __shared__ uint32_t cur[];
+  for (int r = threadIdx.x; r < n_regions; r += kRegBlock)
+    cur[r] = base[r] + counts[(int64_t)blockIdx.x * n_regions + r];
+  __syncthreads();
+  const int64_t lo = (int64_t)blockIdx.x * per;
+  const int64_t hi = lo + per < n ? lo + per : n;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kRegBlock) {
+    if (!bit_valid(kvalid, i)) continue;
+    const int64_t k = load_key(key, kdt, i);
+    const uint64_t h = fmix64((uint64_t)k) & mask;
+    const uint32_t pos = atomicAdd(&cur[h >> kRegBits], 1u);
+    // one 16-byte record (a slot's layout, state 1) per row: one scattered write
+    brec[pos] = make_uint4((uint32_t)k, (uint32_t)((uint64_t)k >> 32), carry ? carry[i] : (uint32_t)i, 1u);
+  }
+}
+
+// One workgroup per region: linear probing in LDS from each row's home slot,
+// no wrap (a row that reaches the region's end overflows), then the
+// duplicate check of the join_dup_scan_kernel on the final LDS copy, then
+// the region written out whole.
+constexpr int kRegBuildBlock = 1024;
+
+__global__ __launch_bounds__(kRegBuildBlock) void join_region_build_kernel(const uint4* __restrict__ brec,
+                                                                const uint32_t* __restrict__ base, uint64_t mask,
+                                                                uint4* __restrict__ tbl, int64_t* __restrict__ okey,
+                                                                uint32_t* __restrict__ oval,
+                                                                unsigned long long* __restrict__ ocount,
+                                                                uint32_t* __restrict__ nonunique) {
+  __shared__ uint4 sl[kRegSlots];
+  const int r = blockIdx.x;
+  for (int t = threadIdx.x; t < kRegSlots; t += kRegBuildBlock) sl[t] = make_uint4(0u, 0u, 0u, 0u);
+  __syncthreads();
+  const uint32_t lo = base[r], hi = base[r + 1];
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += kRegBuildBlock) {
+    const uint4 rec = brec[i];
+    const int64_t k = slot_key(rec);
+    const uint32_t v = rec.z;
+    bool placed = false;
+    for (int t = (int)((fmix64((uint64_t)k) & mask) & (kRegSlots - 1)); t < kRegSlots; t++) {
+      if (sl[t].w == 0u && atomicCAS(&sl[t].w, 0u, 1u) == 0u) {
+        sl[t].x = (uint32_t)k;
+        sl[t].y = (uint32_t)((uint64_t)k >> 32);
+        sl[t].z = v;
+        placed = true;
+        break;
+      }
+    }
+    if (!placed) {
+      const unsigned long long p = atomicAdd(ocount, 1ull);  // the list holds every row: p < rows
+      okey[p] = k;
+      oval[p] = v;
+    }
+  }
+  __syncthreads();
+  for (int s = threadIdx.x; s < kRegSlots; s += kRegBuildBlock) {
+    const uint4 e = sl[s];
+    if (e.w == 0u) continue;
+    const int64_t k = slot_key(e);
+    for (int t = (int)((fmix64((uint64_t)k) & mask) & (kRegSlots - 1)); t < s; t++)
+      if (slot_key(sl[t]) == k) {
+        *nonunique = 1u;
+        break;
+      }
+  }
+  uint4* out = tbl + (uint64_t)r * kRegSlots;
+  for (int t = threadIdx.x; t < kRegSlots; t += kRegBuildBlock) out[t] = sl[t];
+}
+
+// Global claims for the overflow rows (after every region is written).
+__global__ __launch_bounds__(kBuildBlock) void join_overflow_insert_kernel(uint4* __restrict__ tbl, uint64_t mask,
+                                                                           const int64_t* __restrict__ okey,
+                                                                           const uint32_t* __restrict__ oval,
+                                                                           const unsigned long long* __restrict__ ocount,
+                                                                           uint32_t* __restrict__ lost) {
+  const int64_t n = (int64_t)*ocount;
+  const int64_t stride = (int64_t)gridDim.x * kBuildBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBuildBlock + threadIdx.x; i < n; i += stride) {
+    const int64_t k = okey[i];
+    uint64_t s = fmix64((uint64_t)k) & mask;
+    bool placed = false;
+    for (uint64_t step = 0; step <= mask; step++) {
+      uint32_t* st = &tbl[s].w;
+      if (*st == 0u && atomicCAS(st, 0u, 1u) == 0u) {
+        tbl[s].x = (uint32_t)k;
+        tbl[s].y = (uint32_t)((uint64_t)k >> 32);
+        tbl[s].z = oval[i];
+        placed = true;
+        break;
+      }
+      s = (s + 1) & mask;
+    }
+    if (!placed) atomicAdd(lost, 1u);
+  }
+}
+
+// An overflow row's key stored more than once anywhere on its chain (after
+// every insert) marks the table non-unique; copies that stayed in one region
+// were checked there.
+__global__ __launch_bounds__(kBuildBlock) void join_overflow_dup_kernel(const uint4* __restrict__ tbl, uint64_t mask,
+                                                                        const int64_t* __restrict__ okey,
+                                                                        const unsigned long long* __restrict__ ocount,
+                                                                        uint32_t* __restrict__ nonunique) {
+  const int64_t n = (int64_t)*ocount;
+  const int64_t stride = (int64_t)gridDim.x * kBuildBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBuildBlock + threadIdx.x; i < n; i += stride) {
+    int c = 0;
+    probe_chain(tbl, mask, okey[i], false, [&](uint32_t) { c++; });
+    if (c > 1) *nonunique = 1u;
+  }
+}
+
 // Marks the table non-unique if any key is stored twice, by one sequential
 // pass over the finished table: an occupied slot s holding k was claimed after
 // its insert walked every slot from k's home to s, all occupied, so an earlier
@@ -441,19 +637,69 @@ extern "C" int bolt_join_build(bolt_stream_t stream, const bolt_column_t* build_
     bolt_join_table_release(t);
     return st;
   }
-  // state words must start at 0 (the fill writes the whole table)
-  if ((st = fill_u32(t->tbl, 0, bytes / 4, s)) != BOLT_OK || (st = fill_u32(t->flags, 0, 16, s)) != BOLT_OK) {
+  const uint32_t* cv = carry ? static_cast<const uint32_t*>(carry->data) : nullptr;
+  // region build for tables of 2^20 .. kMaxRegions * 4096 slots (BOLT_JOIN_BUILD=atomic: global claims)
+  const char* how = getenv("BOLT_JOIN_BUILD");
+  const bool region = n > 0 && cap >= (1ull << 20) && (cap >> kRegBits) <= (uint64_t)kMaxRegions &&
+                      !(how && strcmp(how, "atomic") == 0);
+  // state words must start at 0: the region build writes every slot, the other path fills the table first
+  if ((!region && (st = fill_u32(t->tbl, 0, bytes / 4, s)) != BOLT_OK) || (st = fill_u32(t->flags, 0, 16, s)) != BOLT_OK) {
     bolt_join_table_release(t);
     return st;
   }
+  if (region) {
+    const int n_regions = (int)(cap >> kRegBits);
+    const int64_t per = (n + kRegGrid - 1) / kRegGrid;
+    DevBuf<uint32_t> counts, totals, rbase, oval;
+    DevBuf<uint4> brec;
+    DevBuf<int64_t> okey;
+    DevBuf<unsigned long long> ocount;
+    st = counts.alloc((size_t)kRegGrid * n_regions, s);
+    if (st == BOLT_OK) st = totals.alloc((size_t)n_regions, s);
+    if (st == BOLT_OK) st = rbase.alloc((size_t)n_regions + 1, s);
+    if (st == BOLT_OK) st = brec.alloc((size_t)n, s);
+    if (st == BOLT_OK) st = okey.alloc((size_t)n, s);
+    if (st == BOLT_OK) st = oval.alloc((size_t)n, s);
+    if (st == BOLT_OK) st = ocount.alloc(1, s);
+    if (st == BOLT_OK) st = fill_u32(ocount.ptr, 0, 2, s);
+    if (st != BOLT_OK) {
+      bolt_join_table_release(t);
+      return st;
+    }
+    const size_t lds = (size_t)n_regions * 4;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(join_region_hist_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(join_region_scatter_kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e == hipSuccess) {
+      join_region_hist_kernel<<<kRegGrid, kRegBlock, lds, s>>>(build_key->data, t->key_dtype, build_key->validity, n, per,
+                                                               t->mask, n_regions, counts.ptr);
+      join_region_offsets_kernel<<<(n_regions + 255) / 256, 256, 0, s>>>(counts.ptr, kRegGrid, n_regions, totals.ptr);
+      join_region_scan_kernel<<<1, kRegBlock, 0, s>>>(totals.ptr, n_regions, rbase.ptr);
+      join_region_scatter_kernel<<<kRegGrid, kRegBlock, lds, s>>>(build_key->data, t->key_dtype, build_key->validity,
+                                                                  cv, n, per, t->mask, n_regions, counts.ptr, rbase.ptr,
+                                                                  brec.ptr);
+      join_region_build_kernel<<<(unsigned)n_regions, kRegBuildBlock, 0, s>>>(brec.ptr, rbase.ptr, t->mask, t->tbl,
+                                                                              okey.ptr, oval.ptr, ocount.ptr, t->flags);
+      join_overflow_insert_kernel<<<64, kBuildBlock, 0, s>>>(t->tbl, t->mask, okey.ptr, oval.ptr, ocount.ptr,
+                                                             t->flags + 1);
+      join_overflow_dup_kernel<<<64, kBuildBlock, 0, s>>>(t->tbl, t->mask, okey.ptr, ocount.ptr, t->flags);
+      e = hipGetLastError();
+    }
+    if (e != hipSuccess) {
+      bolt_join_table_release(t);
+      return set_error(BOLT_ERR_DEVICE, "hash join region build failed: %s", hipGetErrorString(e));
+    }
+  }
   if (n > 0) {
     unsigned grid = grid_for(n, kBuildBlock, 4, 8192);
-    join_build_kernel<<<grid, kBuildBlock, 0, s>>>(t->tbl, t->mask, build_key->data, t->key_dtype,
-                                                   build_key->validity,
-                                                   carry ? static_cast<const uint32_t*>(carry->data) : nullptr, n,
-                                                   t->flags + 1);
-    join_dup_scan_kernel<<<grid_for((int64_t)t->mask + 1, kBuildBlock, 4, 8192), kBuildBlock, 0, s>>>(t->tbl, t->mask,
-                                                                                                   t->flags);
+    if (!region) {
+      join_build_kernel<<<grid, kBuildBlock, 0, s>>>(t->tbl, t->mask, build_key->data, t->key_dtype,
+                                                     build_key->validity, cv, n, t->flags + 1);
+      join_dup_scan_kernel<<<grid_for((int64_t)t->mask + 1, kBuildBlock, 4, 8192), kBuildBlock, 0, s>>>(
+          t->tbl, t->mask, t->flags);
+    }
     if (t->bloom)
       bloom_build_kernel<<<grid, kBuildBlock, 0, s>>>(t->bloom, t->bloom_mask, build_key->data, t->key_dtype,
                                                       build_key->validity, n);

@@ -152,3 +152,36 @@ def test_join_aggregate_with_duplicate_build_keys(gpu):
     assert got[1] == len(ol)
     assert got[2] == int(bp[ol].min())
     assert rel_close(got[3], math.fsum(amt[orr]), REL_TOL)
+
+
+@pytest.mark.parametrize("load", ["0.15", "0.9"])
+def test_region_build_matches_atomic_build(gpu, monkeypatch, load, kernel_path):
+    """The region build (tables of 2^20 slots and more) and the global-claim
+    build give the same join: duplicates, NULL keys, and at load 0.9 many rows
+    that overflow their region."""
+    if kernel_path != "jit":
+        pytest.skip("the build does not depend on the probe kernel")
+    print(f"[case] region build load={load}", flush=True)
+    monkeypatch.setenv("BOLT_JOIN_LOAD_FACTOR", load)
+    rng = np.random.default_rng(99)
+    nb, npr = 1_500_000, 3_000_000
+    bk = rng.integers(0, 1_200_000, nb).astype(np.int64) * 7 - 3
+    pk = rng.integers(0, 1_400_000, npr).astype(np.int64) * 7 - 3
+    bv = rng.random(nb) > 0.02
+    db, _ = both(bk, N.I64, bv)
+    dp, _ = both(pk, N.I64)
+    pairs = {}
+    for how in ("region", "atomic"):
+        monkeypatch.setenv("BOLT_JOIN_BUILD", how)
+        t = JoinTable(db)
+        assert not t.unique()
+        pr, br = t.probe([dp], 0, capacity=4 * npr)
+        pairs[how] = _canon(pr, br)
+        t.release()
+    assert np.array_equal(pairs["region"], pairs["atomic"])
+    # unique keys: the region build's flag says unique
+    monkeypatch.setenv("BOLT_JOIN_BUILD", "region")
+    du, _ = both(rng.permutation(nb).astype(np.int64), N.I64)
+    t = JoinTable(du)
+    assert t.unique()
+    t.release()
