@@ -374,7 +374,9 @@ int bolt_hip_memcpy_htod(void* dst, const void* src, size_t bytes) {
     RT_HIP(up_lane(g_up[0], static_cast<char*>(dst), static_cast<const char*>(src), bytes));
     return 0;
   }
-  const size_t share = (bytes / lanes + 4095) / 4096 * 4096;
+  // ceil(bytes / lanes) rounded up to 4 KB, so that lanes * share >= bytes and
+  // the last lane takes what remains (possibly nothing)
+  const size_t share = ((bytes + lanes - 1) / lanes + 4095) / 4096 * 4096;
   hipError_t err[kUpLanes];
   std::thread th[kUpLanes];
   for (int i = 0; i < lanes; i++) {
