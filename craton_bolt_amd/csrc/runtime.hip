@@ -325,13 +325,27 @@ bool is_pageable(const void* p) {
   return a.type == hipMemoryTypeUnregistered;  // pinned / registered / device memory is copied as it is
 }
 
+// Creates what is still missing (a failed attempt resumes where it stopped;
+// the staged path is skipped until everything exists).
 int up_init() {
   if (g_up_ready) return 0;
   for (UpLane& l : g_up) {
-    if (hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking) != hipSuccess) return -1;
+    if (!l.stream && hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking) != hipSuccess) {
+      l.stream = nullptr;
+      (void)hipGetLastError();
+      return -1;
+    }
     for (int b = 0; b < 2; b++) {
-      if (hipHostMalloc(reinterpret_cast<void**>(&l.buf[b]), kUpChunk, hipHostMallocDefault) != hipSuccess) return -1;
-      if (hipEventCreateWithFlags(&l.done[b], hipEventDisableTiming) != hipSuccess) return -1;
+      if (!l.buf[b] && hipHostMalloc(reinterpret_cast<void**>(&l.buf[b]), kUpChunk, hipHostMallocDefault) != hipSuccess) {
+        l.buf[b] = nullptr;
+        (void)hipGetLastError();
+        return -1;
+      }
+      if (!l.done[b] && hipEventCreateWithFlags(&l.done[b], hipEventDisableTiming) != hipSuccess) {
+        l.done[b] = nullptr;
+        (void)hipGetLastError();
+        return -1;
+      }
     }
   }
   g_up_ready = true;
