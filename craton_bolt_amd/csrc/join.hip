@@ -423,24 +423,41 @@ __global__ __launch_bounds__(kProbeBlock) void join_probe_emit_kernel(
   const int64_t tbase = tile * kEmitTile;
   const bool first_only = (*nonunique == 0u);
   uint32_t m[kEmitItems], bv[kEmitItems];
-  uint32_t mine = 0, pad = 0;
+  uint32_t mine = 0, pad = 0, live = 0;
+  int64_t kk[kEmitItems];
+  uint64_t ss[kEmitItems];
+  uint4 ee[kEmitItems];
+  // keys and the home slots of the lane's rows first, all in flight together,
+  // then each row's chain from its already-loaded home slot
+#pragma unroll
+  for (int j = 0; j < kEmitItems; j++) {
+    const int64_t i = tbase + ((int64_t)threadIdx.x * kEmitItems + j);
+    const bool lv = i < n && eval_filter(f, i);
+    live |= (uint32_t)lv << j;
+    const bool ok = lv && bit_valid(kvalid, i);
+    kk[j] = ok ? (int64_t)keys[i] : 0;
+    ss[j] = ok && bloom_maybe(bloom, bmask, kk[j]) ? fmix64((uint64_t)kk[j]) & mask : ~0ull;
+  }
+#pragma unroll
+  for (int j = 0; j < kEmitItems; j++) ee[j] = ss[j] != ~0ull ? tbl[ss[j]] : make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
   for (int j = 0; j < kEmitItems; j++) {
     const int64_t i = tbase + ((int64_t)threadIdx.x * kEmitItems + j);
     m[j] = 0;
     bv[j] = 0;
-    const bool live = i < n && eval_filter(f, i);
-    if (live && bit_valid(kvalid, i)) {
-      const int64_t k = (int64_t)keys[i];
-      if (bloom_maybe(bloom, bmask, k))
-        probe_chain(tbl, mask, k, first_only, [&](uint32_t w) {
-          if (kVerify && !same_key(pk, i, bk, w)) return;
-          m[j]++;
-          bv[j] = w;
-          if (unmatched) unmatched[w] = 0;
-        });
+    uint4 e = ee[j];
+    uint64_t sl = ss[j];
+    for (uint64_t step = 0; e.w != 0u && step <= mask; step++) {
+      if (slot_key(e) == kk[j] && (!kVerify || same_key(pk, i, bk, e.z))) {
+        m[j]++;
+        bv[j] = e.z;
+        if (unmatched) unmatched[e.z] = 0;
+        if (first_only) break;
+      }
+      sl = (sl + 1) & mask;
+      e = tbl[sl];
     }
-    if (kPreserve && live && m[j] == 0) {
+    if (kPreserve && ((live >> j) & 1u) && m[j] == 0) {
       m[j] = 1;
       bv[j] = kNullRow;
       pad |= 1u << j;
