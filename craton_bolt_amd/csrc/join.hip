@@ -232,7 +232,12 @@ __global__ __launch_bounds__(kRegBuildBlock) void join_region_build_kernel(const
       }
   }
   uint4* out = tbl + (uint64_t)r * kRegSlots;
-  for (int t = threadIdx.x; t < kRegSlots; t += kRegBuildBlock) out[t] = sl[t];
+  typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+  for (int t = threadIdx.x; t < kRegSlots; t += kRegBuildBlock) {  // streaming stores: the table is not read back soon
+    const uint4 e = sl[t];
+    const v4u v = {e.x, e.y, e.z, e.w};
+    __builtin_nontemporal_store(v, reinterpret_cast<v4u*>(out + t));
+  }
 }
 
 // Global claims for the overflow rows (after every region is written).
