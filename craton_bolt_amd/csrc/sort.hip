@@ -709,9 +709,16 @@ constexpr int kSelUnroll = 4;  // rows in flight per lane
 // rows whose image >> (shift + width) equals `prefix` (every row when
 // shift + width = 64), written to part[block][bin] with plain stores (a
 // global atomic per bin per workgroup would cost more than the row pass).
+// With `spec` non-null (the first pass) the rows whose digit is at most
+// spec_bound are also appended to spec (unordered, up to spec_cap): a guess
+// at the candidates that saves the collect pass when the K-th row's digit
+// turns out to be within the bound.
 template <int W>
 __global__ __launch_bounds__(kSelBlock) void topk_hist_kernel(TopKey k, int64_t n, int shift, int width,
-                                                              uint64_t prefix, uint32_t* __restrict__ part) {
+                                                              uint64_t prefix, uint32_t* __restrict__ part,
+                                                              uint64_t spec_bound, uint32_t* __restrict__ spec,
+                                                              unsigned long long* __restrict__ spec_count,
+                                                              int64_t spec_cap) {
   __shared__ uint32_t sh[kSelBins];
   const int nb = 1 << width;
   for (int b = threadIdx.x; b < nb; b += kSelBlock) sh[b] = 0;
@@ -729,9 +736,24 @@ __global__ __launch_bounds__(kSelBlock) void topk_hist_kernel(TopKey k, int64_t 
 #pragma unroll
     for (int u = 0; u < kSelUnroll; u++) {
       const int64_t i = i0 + u * stride;
-      if (i >= n) continue;
-      const uint64_t c = coarse_of(k, x[u], bit_valid(k.v, i));
-      if (hi >= 64 || (c >> hi) == prefix) atomicAdd(&sh[(c >> shift) & dmask], 1u);
+      bool take = false;
+      if (i < n) {
+        const uint64_t c = coarse_of(k, x[u], bit_valid(k.v, i));
+        if (hi >= 64 || (c >> hi) == prefix) atomicAdd(&sh[(c >> shift) & dmask], 1u);
+        take = spec != nullptr && (c >> shift) <= spec_bound;
+      }
+      const uint64_t m = __ballot(take);  // lanes still in the loop
+      if (m) {
+        const int lane = threadIdx.x & 63;
+        const int leader = __builtin_ctzll(m);
+        unsigned long long b = 0;
+        if (lane == leader) b = atomicAdd(spec_count, (unsigned long long)__popcll(m));
+        b = __shfl(b, leader);
+        if (take) {
+          const int64_t pos = (int64_t)b + __popcll(m & ((1ull << lane) - 1));
+          if (pos < spec_cap) spec[pos] = (uint32_t)i;
+        }
+      }
     }
   }
   __syncthreads();
@@ -1031,20 +1053,39 @@ extern "C" int bolt_sort_limit(bolt_stream_t stream, const bolt_column_t* keys, 
   int hi = 64;
   int64_t below = 0, need = K, eq = n;
   const unsigned grid = grid_for(n, 256, kSelUnroll, 4096);
+  // Candidates guessed in the first pass: the rows whose first digit is at
+  // most spec_bound, where the K-th row's digit lies when the keys are spread
+  // (about K * 2^14 / n); capacity for twice the expected count.
+  const uint64_t spec_bound = (uint64_t)((double)K * (double)kSelBins / (double)n) + 1;
+  const int64_t spec_cap = (int64_t)std::min<double>(
+      (double)n, 2.0 * (double)(spec_bound + 1) * (double)n / (double)kSelBins + 65536.0);
+  DevBuf<uint32_t> spec;
+  DevBuf<unsigned long long> spec_cnt;
+  BOLT_TRY(spec.alloc((size_t)spec_cap, s));
+  BOLT_TRY(spec_cnt.alloc(1, s));
+  BOLT_TRY(fill_u32(spec_cnt.ptr, 0, 2, s));
+  unsigned long long spec_got = 0;
+  int first_digit = -1;
   while (hi > 0) {
     const int width = hi >= kSelBits ? kSelBits : hi;
     const int shift = hi - width;
+    uint32_t* sp = first_digit < 0 ? spec.ptr : nullptr;
+#define HIST_LAUNCH(WW)                                                                                         \
+  topk_hist_kernel<WW><<<hgrid, kSelBlock, 0, s>>>(tk, n, shift, width, prefix, part.ptr, spec_bound, sp,       \
+                                                   spec_cnt.ptr, spec_cap)
     switch (kw) {
-      case 1: topk_hist_kernel<1><<<hgrid, kSelBlock, 0, s>>>(tk, n, shift, width, prefix, part.ptr); break;
-      case 4: topk_hist_kernel<4><<<hgrid, kSelBlock, 0, s>>>(tk, n, shift, width, prefix, part.ptr); break;
-      case 8: topk_hist_kernel<8><<<hgrid, kSelBlock, 0, s>>>(tk, n, shift, width, prefix, part.ptr); break;
-      default: topk_hist_kernel<16><<<hgrid, kSelBlock, 0, s>>>(tk, n, shift, width, prefix, part.ptr); break;
+      case 1: HIST_LAUNCH(1); break;
+      case 4: HIST_LAUNCH(4); break;
+      case 8: HIST_LAUNCH(8); break;
+      default: HIST_LAUNCH(16); break;
     }
+#undef HIST_LAUNCH
     BOLT_TRY(fill_u32(hist.ptr, 0, (size_t)1 << width, s));
     topk_hist_sum_kernel<<<dim3((unsigned)(((1 << width) + 255) / 256), 32), 256, 0, s>>>(part.ptr, (int)hgrid,
                                                                                         1 << width, hist.ptr);
     BOLT_HIP(hipGetLastError());
     BOLT_HIP(hipMemcpyAsync(h.data(), hist.ptr, (size_t)(1 << width) * 4, hipMemcpyDeviceToHost, s));
+    if (first_digit < 0) BOLT_HIP(hipMemcpyAsync(&spec_got, spec_cnt.ptr, 8, hipMemcpyDeviceToHost, s));
     BOLT_HIP(hipStreamSynchronize(s));
     int64_t cum = 0;
     int d = 0;
@@ -1053,6 +1094,7 @@ extern "C" int bolt_sort_limit(bolt_stream_t stream, const bolt_column_t* keys, 
       cum += h[d];
     }
     if (d == (1 << width)) return set_error(BOLT_ERR_OTHER, "bolt_sort_limit: radix select lost rows");
+    if (first_digit < 0) first_digit = d;
     below += cum;
     need -= cum;
     eq = h[d];
@@ -1060,25 +1102,37 @@ extern "C" int bolt_sort_limit(bolt_stream_t stream, const bolt_column_t* keys, 
     hi = shift;
     if (eq <= (1 << 16)) break;
   }
-  const int64_t C = below + eq;  // rows whose image >> hi <= prefix
-  if (C * 2 > n) return full_sort();  // the leading key ties too often to narrow the rows
+  const int64_t Cx = below + eq;  // rows whose image >> hi <= prefix
+  if (Cx * 2 > n) return full_sort();  // the leading key ties too often to narrow the rows
   DevBuf<uint32_t> cand;
   DevBuf<unsigned long long> cnt;
-  BOLT_TRY(cand.alloc((size_t)C, s));
   BOLT_TRY(cnt.alloc(1, s));
   BOLT_TRY(fill_u32(cnt.ptr, 0, 2, s));
-  switch (kw) {
-    case 1: topk_collect_kernel<1><<<grid, 256, 0, s>>>(tk, n, hi, prefix, cand.ptr, cnt.ptr, C); break;
-    case 4: topk_collect_kernel<4><<<grid, 256, 0, s>>>(tk, n, hi, prefix, cand.ptr, cnt.ptr, C); break;
-    case 8: topk_collect_kernel<8><<<grid, 256, 0, s>>>(tk, n, hi, prefix, cand.ptr, cnt.ptr, C); break;
-    default: topk_collect_kernel<16><<<grid, 256, 0, s>>>(tk, n, hi, prefix, cand.ptr, cnt.ptr, C); break;
+  // The candidates: the first pass's guess when it holds every row of the
+  // K-th row's first digit and below (a superset of the Cx rows, which the
+  // steps below accept), else a collect pass for exactly the Cx rows.
+  const uint32_t* cptr;
+  int64_t C;
+  if (first_digit >= 0 && (uint64_t)first_digit <= spec_bound && (int64_t)spec_got <= spec_cap) {
+    cptr = spec.ptr;
+    C = (int64_t)spec_got;
+  } else {
+    BOLT_TRY(cand.alloc((size_t)Cx, s));
+    switch (kw) {
+      case 1: topk_collect_kernel<1><<<grid, 256, 0, s>>>(tk, n, hi, prefix, cand.ptr, cnt.ptr, Cx); break;
+      case 4: topk_collect_kernel<4><<<grid, 256, 0, s>>>(tk, n, hi, prefix, cand.ptr, cnt.ptr, Cx); break;
+      case 8: topk_collect_kernel<8><<<grid, 256, 0, s>>>(tk, n, hi, prefix, cand.ptr, cnt.ptr, Cx); break;
+      default: topk_collect_kernel<16><<<grid, 256, 0, s>>>(tk, n, hi, prefix, cand.ptr, cnt.ptr, Cx); break;
+    }
+    BOLT_HIP(hipGetLastError());
+    unsigned long long got = 0;
+    BOLT_HIP(hipMemcpyAsync(&got, cnt.ptr, 8, hipMemcpyDeviceToHost, s));
+    BOLT_HIP(hipStreamSynchronize(s));
+    if ((int64_t)got != Cx)
+      return set_error(BOLT_ERR_OTHER, "bolt_sort_limit: collected %llu candidates, expected %lld", got, (long long)Cx);
+    cptr = cand.ptr;
+    C = Cx;
   }
-  BOLT_HIP(hipGetLastError());
-  unsigned long long got = 0;
-  BOLT_HIP(hipMemcpyAsync(&got, cnt.ptr, 8, hipMemcpyDeviceToHost, s));
-  BOLT_HIP(hipStreamSynchronize(s));
-  if ((int64_t)got != C)
-    return set_error(BOLT_ERR_OTHER, "bolt_sort_limit: collected %llu candidates, expected %lld", got, (long long)C);
   // One key without NULLs (its image is then exact): narrow the candidates
   // further and sort the few that remain in one workgroup.
   if (n_keys == 1 && !c0.validity && kw != 16 && c0.dtype != BOLT_BOOL && K <= kSmallSort / 2) {
@@ -1093,9 +1147,9 @@ extern "C" int bolt_sort_limit(bolt_stream_t stream, const bolt_column_t* keys, 
       const int shift = hi2 - width;
       BOLT_TRY(fill_u32(hist.ptr, 0, (size_t)1 << width, s));
       switch (kw) {
-        case 1: topk_cand_hist_kernel<1><<<cgrid, 256, 0, s>>>(tk, cand.ptr, C, shift, width, p2, hist.ptr); break;
-        case 4: topk_cand_hist_kernel<4><<<cgrid, 256, 0, s>>>(tk, cand.ptr, C, shift, width, p2, hist.ptr); break;
-        default: topk_cand_hist_kernel<8><<<cgrid, 256, 0, s>>>(tk, cand.ptr, C, shift, width, p2, hist.ptr); break;
+        case 1: topk_cand_hist_kernel<1><<<cgrid, 256, 0, s>>>(tk, cptr, C, shift, width, p2, hist.ptr); break;
+        case 4: topk_cand_hist_kernel<4><<<cgrid, 256, 0, s>>>(tk, cptr, C, shift, width, p2, hist.ptr); break;
+        default: topk_cand_hist_kernel<8><<<cgrid, 256, 0, s>>>(tk, cptr, C, shift, width, p2, hist.ptr); break;
       }
       BOLT_HIP(hipGetLastError());
       BOLT_HIP(hipMemcpyAsync(h.data(), hist.ptr, (size_t)(1 << width) * 4, hipMemcpyDeviceToHost, s));
@@ -1119,9 +1173,9 @@ extern "C" int bolt_sort_limit(bolt_stream_t stream, const bolt_column_t* keys, 
       BOLT_TRY(cand2.alloc((size_t)C2, s));
       BOLT_TRY(fill_u32(cnt.ptr, 0, 2, s));
       switch (kw) {
-        case 1: topk_cand_collect_kernel<1><<<cgrid, 256, 0, s>>>(tk, cand.ptr, C, hi2, p2, cand2.ptr, cnt.ptr, C2); break;
-        case 4: topk_cand_collect_kernel<4><<<cgrid, 256, 0, s>>>(tk, cand.ptr, C, hi2, p2, cand2.ptr, cnt.ptr, C2); break;
-        default: topk_cand_collect_kernel<8><<<cgrid, 256, 0, s>>>(tk, cand.ptr, C, hi2, p2, cand2.ptr, cnt.ptr, C2); break;
+        case 1: topk_cand_collect_kernel<1><<<cgrid, 256, 0, s>>>(tk, cptr, C, hi2, p2, cand2.ptr, cnt.ptr, C2); break;
+        case 4: topk_cand_collect_kernel<4><<<cgrid, 256, 0, s>>>(tk, cptr, C, hi2, p2, cand2.ptr, cnt.ptr, C2); break;
+        default: topk_cand_collect_kernel<8><<<cgrid, 256, 0, s>>>(tk, cptr, C, hi2, p2, cand2.ptr, cnt.ptr, C2); break;
       }
       BOLT_HIP(hipGetLastError());
       int npow = 1;
@@ -1154,16 +1208,16 @@ extern "C" int bolt_sort_limit(bolt_stream_t stream, const bolt_column_t* keys, 
       BOLT_TRY(valid[k].alloc((size_t)((C + 7) / 8 + 8), s));
       ck[k].validity = valid[k].ptr;
     }
-    BOLT_TRY(bolt_take(stream, &keys[k], cand.ptr, C, &ck[k]));
+    BOLT_TRY(bolt_take(stream, &keys[k], cptr, C, &ck[k]));
   }
   memset(&ck[n_keys], 0, sizeof ck[n_keys]);
-  ck[n_keys].data = cand.ptr;  // row ids < 2^31: ascending as Int32
+  ck[n_keys].data = const_cast<uint32_t*>(cptr);  // row ids < 2^31: ascending as Int32
   ck[n_keys].length = C;
   ck[n_keys].dtype = BOLT_I32;
   DevBuf<uint32_t> perm;
   BOLT_TRY(perm.alloc((size_t)C, s));
   BOLT_TRY(bolt_sort_indices(stream, ck.data(), n_keys + 1, desc.data(), nf.data(), perm.ptr));
-  topk_emit_kernel<<<(unsigned)((m + 255) / 256), 256, 0, s>>>(cand.ptr, perm.ptr, offset, m, out_rows);
+  topk_emit_kernel<<<(unsigned)((m + 255) / 256), 256, 0, s>>>(cptr, perm.ptr, offset, m, out_rows);
   BOLT_HIP(hipGetLastError());
   BOLT_HIP(hipStreamSynchronize(s));
   *n_out = m;
