@@ -48,3 +48,32 @@ def test_sort_pairs_and_zipf_shape():
     h = np.sum(np.arange(1, 10_001) ** -1.1)
     assert abs(top[0] / 200_000 - 1 / h) < 0.01
     assert abs(v.mean()) < 0.02 and abs(v.std() - 1) < 0.02
+
+
+def test_h2o_columns_follow_the_reference_generator():
+    """benches/olap_benchmarks.rs:106-123, restated with Python integers: a
+    64-bit wrapping product, its low 32 bits as a signed int, then rem_euclid."""
+    import numpy as np
+
+    from craton_bolt_amd import datagen as D
+
+    def low32(v):
+        v &= (1 << 32) - 1
+        return v - (1 << 32) if v >> 31 else v
+
+    def rem_euclid(a, m):
+        return a % m  # Python's % is Euclidean for a positive modulus
+
+    rows = [0, 1, 2, 3, 4095, 65_537, 1_000_003, 9_999_999]
+    n = rows[-1] + 1
+    cols = {c: D.h2o_column(c, n) for c in ("id1", "id2", "id3", "v1", "v2", "v3")}
+    m64 = (1 << 64) - 1
+    for i in rows:
+        assert cols["id1"][i] == rem_euclid(low32((i * 2_654_435_761) & m64), 100)
+        assert cols["id2"][i] == rem_euclid(low32((i * 40_503) & m64), 10_000)
+        assert cols["id3"][i] == rem_euclid(low32((i * 11_400_714_819_323_198_485) & m64), 1_000_000)
+        assert cols["v1"][i] == float(rem_euclid(low32(i * 7), 5) + 1)
+        assert cols["v2"][i] == float(rem_euclid(low32(i * 13), 15) + 1)
+        assert cols["v3"][i] == rem_euclid(low32(i * 17), 10_000) / 100.0
+    assert cols["id1"].dtype == np.int32 and cols["v3"].dtype == np.float64
+    assert cols["id3"].min() >= 0 and cols["id3"].max() < 1_000_000
