@@ -133,3 +133,26 @@ def test_sort_limit_small_final_set(gpu, dtype, limit, offset, desc):
     else:
         cols = [both(rng.standard_normal(n).astype(np.float32), N.F32)]
     _check(cols, [desc], [True], limit, offset)
+
+
+@pytest.mark.parametrize("limit,offset,desc", [(100, 0, False), (100, 0, True), (1, 0, False), (300, 200, True),
+                                               (2000, 0, False)])
+def test_sort_limit_first_pass_candidates(gpu, limit, offset, desc):
+    """Int64 keys over the whole range: the K-th row's top digit is within the
+    first pass's guess, so the candidates come from that pass (no collect pass)."""
+    print(f"[case] first-pass candidates limit={limit} offset={offset} desc={desc}", flush=True)
+    n = 4_000_000
+    rng = np.random.default_rng(limit + offset)
+    keys = rng.integers(np.iinfo(np.int64).min, np.iinfo(np.int64).max, n, dtype=np.int64)
+    keys[rng.integers(0, n, 20)] = keys[0]  # a few ties
+    _check([both(keys, N.I64)], [desc], [True], limit, offset)
+
+
+def test_sort_limit_first_pass_candidates_two_keys(gpu):
+    """The same guess with a second key: the general candidate sort over the guessed superset."""
+    n = 3_000_000
+    rng = np.random.default_rng(4)
+    k0 = rng.integers(np.iinfo(np.int64).min, np.iinfo(np.int64).max, n, dtype=np.int64)
+    k0[rng.integers(0, n, 50)] = np.iinfo(np.int64).min  # ties at the very front
+    k1 = rng.integers(0, 3, n).astype(np.int32)
+    _check([both(k0, N.I64), both(k1, N.I32, rng.random(n) > 0.1)], [False, True], [True, False], 120, 10)
