@@ -189,8 +189,16 @@ _lib: C.CDLL | None = None
 _SIGS = {
     "bolt_last_error": (C.c_char_p, []),
     "bolt_build_info": (C.c_char_p, []),
+    "bolt_jit_cache_stats": (C.c_int, [C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "bolt_profile_events": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "bolt_hip_get_error_name": (C.c_int, [C.c_int, C.POINTER(C.c_char_p)]),
+    "bolt_hip_get_error_string": (C.c_int, [C.c_int, C.POINTER(C.c_char_p)]),
     "bolt_hip_init": (C.c_int, [C.c_uint]),
+    "bolt_hip_device_get": (C.c_int, [C.POINTER(C.c_int), C.c_int]),
+    "bolt_hip_ctx_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint, C.c_int]),
+    "bolt_hip_ctx_destroy": (C.c_int, [C.c_void_p]),
+    "bolt_hip_ctx_set_current": (C.c_int, [C.c_void_p]),
+    "bolt_hip_ctx_get_current": (C.c_int, [C.POINTER(C.c_void_p)]),
     "bolt_hip_device_get_count": (C.c_int, [C.POINTER(C.c_int)]),
     "bolt_hip_device_get_name": (C.c_int, [C.c_char_p, C.c_int, C.c_int]),
     "bolt_hip_device_get_attribute": (C.c_int, [C.POINTER(C.c_int), C.c_int, C.c_int]),
@@ -214,6 +222,8 @@ _SIGS = {
     "bolt_hip_memset_d8": (C.c_int, [C.c_void_p, C.c_ubyte, C.c_size_t]),
     "bolt_hip_memset_d8_async": (C.c_int, [C.c_void_p, C.c_ubyte, C.c_size_t, C.c_void_p]),
     "bolt_hip_module_load_data": (C.c_int, [C.POINTER(C.c_void_p), C.c_void_p]),
+    "bolt_hip_module_load_data_ex": (C.c_int, [C.POINTER(C.c_void_p), C.c_void_p, C.c_uint, C.POINTER(C.c_int),
+                                               C.POINTER(C.c_void_p)]),
     "bolt_hip_module_unload": (C.c_int, [C.c_void_p]),
     "bolt_hip_module_get_function": (C.c_int, [C.POINTER(C.c_void_p), C.c_void_p, C.c_char_p]),
     "bolt_hip_launch_kernel": (C.c_int, [C.c_void_p, C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.c_uint,
@@ -232,7 +242,8 @@ _SIGS = {
     "bolt_hip_event_destroy": (C.c_int, [C.c_void_p]),
     "bolt_hip_stream_begin_capture": (C.c_int, [C.c_void_p, C.c_uint]),
     "bolt_hip_stream_end_capture": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
-    "bolt_hip_graph_instantiate": (C.c_int, [C.POINTER(C.c_void_p), C.c_void_p]),
+    "bolt_hip_graph_instantiate": (C.c_int, [C.POINTER(C.c_void_p), C.c_void_p, C.POINTER(C.c_void_p), C.c_char_p,
+                                             C.c_size_t]),
     "bolt_hip_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
     "bolt_hip_graph_exec_destroy": (C.c_int, [C.c_void_p]),
     "bolt_hip_graph_destroy": (C.c_int, [C.c_void_p]),

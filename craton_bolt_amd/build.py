@@ -80,6 +80,20 @@ def build_hip(quiet: bool = True, jobs: int | None = None) -> Path:
     return LIB_HIP
 
 
+TEST_KERNEL = ROOT / "tests" / "kernels" / "abi_fill.hip"
+
+
+def build_test_kernels(quiet: bool = True) -> Path:
+    """The stand-alone code object the runtime-ABI tests load through
+    bolt_hip_module_load_data_ex."""
+    out = TEST_KERNEL.with_suffix(".hsaco")
+    tmp = out.with_suffix(".hsaco.tmp")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-O3", "--genco", "-Xclang", "-target-feature", "-Xclang",
+          "-packed-fp32-ops", "-o", str(tmp), str(TEST_KERNEL)], quiet)
+    os.replace(tmp, out)
+    return out
+
+
 def build_datagen(quiet: bool = True) -> Path:
     cc = shutil.which("gcc") or "cc"
     tmp = LIB_GEN.with_suffix(".so.tmp")
@@ -92,6 +106,7 @@ def build_datagen(quiet: bool = True) -> Path:
 def build_all(quiet: bool = True) -> None:
     build_datagen(quiet)
     build_hip(quiet)
+    build_test_kernels(quiet)
 
 
 if __name__ == "__main__":

@@ -8,7 +8,10 @@
 // partials on the host; here one pass evaluates the filter and up to four
 // aggregates per launch, and a second one-block kernel folds the partials in
 // a fixed order on the device.
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <vector>
 
 #include "jit.h"
 #include "reduce_common.h"
@@ -105,7 +108,7 @@ int acc_finalize(const Acc& a, int op, int dt, bolt_scalar_t* r) {
       r->dtype = dt == BOLT_I32 ? BOLT_I64 : dt;
       if (n == 0) { r->is_null = 1; return BOLT_OK; }
       if (dt == BOLT_DEC128) {
-        if (ovf) return set_error(BOLT_ERR_TYPE, "SUM(Decimal128) overflow: accumulator exceeds i128 range");
+        if (ovf) return set_error(BOLT_ERR_TYPE, "SUM(Decimal128) precision overflow: accumulator exceeds i128 range");
         r->i64 = (int64_t)a.a;
         r->hi = (int64_t)a.b;
       } else if (dtype_is_int(dt)) {
@@ -231,6 +234,92 @@ static int launch_scalar_chunk(int na, const DevFilter& f, const AggList& chunk,
   }
 }
 
+// ---- row-order overflow of SUM(Int64) ----------------------------------------
+//
+// The reference folds an integer SUM with checked_add in row order and fails
+// as soon as a running total leaves i64 (src/exec/aggregate.rs:2120-2140), so
+// [i64::MAX, 1, -1] is an error although its total fits.  The fold's state
+// over a run of rows is (sum, highest running total, lowest running total),
+// and runs combine in order:
+//   (s1, hi1, lo1) + (s2, hi2, lo2) = (s1 + s2, max(hi1, s1 + hi2), min(lo1, s1 + lo2)).
+// Each workgroup folds one contiguous range: 256 rows per step, combined in
+// lane order by a shuffle tree, then across the four waves in order; the host
+// folds the workgroups' triples in order.  Sums stay below 2^31 * 2^63, so
+// 128-bit arithmetic is exact, and a run with no row has hi = -kPfxNone,
+// lo = +kPfxNone, far outside any real total.
+struct Pfx {
+  __int128 s, hi, lo;
+};
+constexpr __int128 kPfxNone = (__int128)1 << 120;
+
+__host__ __device__ __forceinline__ Pfx pfx_cat(const Pfx& a, const Pfx& b) {
+  Pfx r;
+  r.s = a.s + b.s;
+  const __int128 h = a.s + b.hi, l = a.s + b.lo;
+  r.hi = a.hi > h ? a.hi : h;
+  r.lo = a.lo < l ? a.lo : l;
+  return r;
+}
+
+__device__ __forceinline__ __int128 shfl_down_i128(__int128 v, int off) {
+  const uint64_t lo = (uint64_t)v, hi = (uint64_t)(v >> 64);
+  const uint64_t l2 = __shfl_down(lo, off), h2 = __shfl_down(hi, off);
+  return (__int128)(((unsigned __int128)h2 << 64) | l2);
+}
+
+__global__ __launch_bounds__(kReduceBlock) void sum_prefix_kernel(DevFilter f, const int64_t* __restrict__ x,
+                                                                 const uint8_t* __restrict__ valid, int64_t n,
+                                                                 int64_t per, Pfx* __restrict__ out) {
+  __shared__ Pfx sh[kWaves];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  Pfx run{0, -kPfxNone, kPfxNone};
+  const int64_t r0 = (int64_t)blockIdx.x * per;
+  const int64_t r1 = r0 + per < n ? r0 + per : n;
+  for (int64_t base = r0; base < r1; base += kReduceBlock) {
+    const int64_t i = base + threadIdx.x;
+    Pfx m{0, -kPfxNone, kPfxNone};
+    if (i < r1 && bit_valid(valid, i) && eval_filter_nt<-1>(f, i)) {
+      const __int128 v = x[i];
+      m = Pfx{v, v, v};
+    }
+    // in-order tree over the wave: lane L ends holding lanes [L, L + 2 * off)
+    for (int off = 1; off < 64; off <<= 1) {
+      Pfx o;
+      o.s = shfl_down_i128(m.s, off);
+      o.hi = shfl_down_i128(m.hi, off);
+      o.lo = shfl_down_i128(m.lo, off);
+      if ((lane & (2 * off - 1)) == 0 && lane + off < 64) m = pfx_cat(m, o);
+    }
+    if (lane == 0) sh[wave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int w = 0; w < kWaves; w++) run = pfx_cat(run, sh[w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = run;
+}
+
+// BOLT_OK when no running total of the filtered, non-NULL rows of `col`
+// (Int64), in row order, leaves i64; the reference's overflow error otherwise.
+static int check_sum_prefix(hipStream_t s, const bolt_column_t& col, const DevFilter& f) {
+  const int64_t n = col.length;
+  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 65535) / 65536));
+  const int64_t per = (n + grid - 1) / grid;
+  DevBuf<Pfx> parts;
+  BOLT_TRY(parts.alloc(grid, s));
+  sum_prefix_kernel<<<grid, kReduceBlock, 0, s>>>(f, static_cast<const int64_t*>(col.data), col.validity, n, per,
+                                                  parts.ptr);
+  BOLT_HIP(hipGetLastError());
+  std::vector<Pfx> h(grid);
+  BOLT_HIP(hipMemcpyAsync(h.data(), parts.ptr, sizeof(Pfx) * grid, hipMemcpyDeviceToHost, s));
+  BOLT_HIP(hipStreamSynchronize(s));
+  Pfx run{0, -kPfxNone, kPfxNone};
+  for (const Pfx& p : h) run = pfx_cat(run, p);
+  if (run.hi > (__int128)INT64_MAX || run.lo < (__int128)INT64_MIN)
+    return set_error(BOLT_ERR_TYPE, "SUM(integer) overflow: accumulator exceeds i64 range");
+  return BOLT_OK;
+}
+
 }  // namespace bolt
 
 using namespace bolt;
@@ -250,9 +339,32 @@ extern "C" int bolt_aggregate(bolt_stream_t stream, const bolt_column_t* inputs,
                        (long long)inputs[i].length, (long long)n);
   DevFilter f;
   BOLT_TRY(make_dev_filter(filter, inputs, n_inputs, n, &f));
+  if (n_aggs == 0) {
+    AggList none;
+    return make_agg_list(aggs, n_aggs, inputs, n_inputs, 0, &none);
+  }
+  // Each SUM over an Int64 column also gets a hidden MIN and MAX of the
+  // column: when count * max(|MIN|, |MAX|) fits i64, no running total can
+  // leave i64, and the row-order check (check_sum_prefix) is skipped.
+  const int32_t n_user = n_aggs;
+  std::vector<bolt_agg_t> ext(aggs, aggs + n_user);
+  std::vector<std::pair<int, int>> guards;  // (SUM index, index of its hidden MIN or -1)
+  if (aggs)
+    for (int i = 0; i < n_user; i++)
+      if (aggs[i].op == BOLT_AGG_SUM && aggs[i].col >= 0 && aggs[i].col < n_inputs &&
+          inputs[aggs[i].col].dtype == BOLT_I64 && n >= 2) {
+        if ((int)ext.size() + 2 <= kMaxAggs) {
+          guards.emplace_back(i, (int)ext.size());
+          ext.push_back(bolt_agg_t{BOLT_AGG_MIN, aggs[i].col});
+          ext.push_back(bolt_agg_t{BOLT_AGG_MAX, aggs[i].col});
+        } else {
+          guards.emplace_back(i, -1);
+        }
+      }
+  aggs = ext.data();
+  n_aggs = (int32_t)ext.size();
   AggList all;
   BOLT_TRY(make_agg_list(aggs, n_aggs, inputs, n_inputs, 0, &all));
-  if (n_aggs == 0) return BOLT_OK;
 
   unsigned grid = grid_for(n, kReduceBlock, kAggUnroll * 4, 256 * 8);
   DevBuf<Acc> parts, outs;
@@ -291,6 +403,18 @@ extern "C" int bolt_aggregate(bolt_stream_t stream, const bolt_column_t* inputs,
   Acc host[kMaxAggs];
   BOLT_HIP(hipMemcpyAsync(host, outs.ptr, sizeof(Acc) * n_aggs, hipMemcpyDeviceToHost, s));
   BOLT_HIP(hipStreamSynchronize(s));
-  for (int i = 0; i < n_aggs; i++) BOLT_TRY(acc_finalize(host[i], all.a[i].op, all.a[i].dtype, &results[i]));
+  bolt_scalar_t res[kMaxAggs];
+  for (int i = 0; i < n_aggs; i++) BOLT_TRY(acc_finalize(host[i], all.a[i].op, all.a[i].dtype, &res[i]));
+  for (const auto& g : guards) {
+    if (res[g.first].is_null || acc_count(host[g.first]) < 2) continue;
+    if (g.second >= 0) {
+      const __int128 lo = res[g.second].i64, hi = res[g.second + 1].i64;
+      const unsigned __int128 a = (unsigned __int128)(lo < 0 ? -lo : lo);
+      const unsigned __int128 b = (unsigned __int128)(hi < 0 ? -hi : hi);
+      if ((a > b ? a : b) * (unsigned __int128)acc_count(host[g.first]) <= (unsigned __int128)INT64_MAX) continue;
+    }
+    BOLT_TRY(check_sum_prefix(s, inputs[aggs[g.first].col], f));
+  }
+  for (int i = 0; i < n_user; i++) results[i] = res[i];
   return BOLT_OK;
 }

@@ -648,7 +648,7 @@ struct bolt_gb_batch {
       }
       if (h_ctl->err[2] & 1u) return set_error(BOLT_ERR_TYPE, "SUM(integer) overflow: accumulator exceeds i64 range");
       if (h_ctl->err[2] & 2u)
-        return set_error(BOLT_ERR_TYPE, "SUM(Decimal128) overflow: accumulator exceeds i128 range");
+        return set_error(BOLT_ERR_TYPE, "SUM(Decimal128) precision overflow: accumulator exceeds i128 range");
       have_result = true;
       return BOLT_OK;
     }

@@ -312,6 +312,45 @@ __global__ __launch_bounds__(kExBlock) void gb_extract_kernel(GTable t, GPlan P,
   }
 }
 
+// The reference synthesises the NULL group of a single nullable key column
+// under the packed key i64::MAX; when a genuine key equal to i64::MAX exists
+// it runs its valid-flag executor instead, which drops the NULL-key rows
+// (src/exec/groupby.rs:553-583).  res[0] = output row of the NULL group
+// (preset -1), res[1] = 1 when a genuine i64::MAX key is among the groups.
+__global__ void gb_null_clash_kernel(const unsigned long long* __restrict__ packed, const uint32_t* __restrict__ slot_of,
+                                     int64_t n, uint64_t cap, int64_t* __restrict__ res) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    if (slot_of[i] == (uint32_t)cap) res[0] = i;
+    else if (packed[i] == 0x7FFFFFFFFFFFFFFFull) res[1] = 1;
+  }
+}
+
+struct MoveCols {
+  uint8_t* d[kMaxAggs + 2];
+  uint32_t* v[kMaxAggs + 2];
+  int32_t w[kMaxAggs + 2];
+  int32_t n;
+};
+
+// Output row `src` (the last) moves to row `dst`, in every column, its
+// validity bit and the packed key / slot arrays the ordering reads.
+__global__ void gb_move_row_kernel(MoveCols mc, unsigned long long* __restrict__ packed, uint32_t* __restrict__ slot_of,
+                                   int64_t dst, int64_t src) {
+  const int c = threadIdx.x;
+  if (c == 0) {
+    packed[dst] = packed[src];
+    slot_of[dst] = slot_of[src];
+  }
+  if (c >= mc.n) return;
+  for (int b = 0; b < mc.w[c]; b++) mc.d[c][dst * mc.w[c] + b] = mc.d[c][src * mc.w[c] + b];
+  if (uint32_t* v = mc.v[c]) {
+    const uint32_t bit = 1u << (dst & 31);
+    if ((v[src >> 5] >> (src & 31)) & 1u) atomicOr(&v[dst >> 5], bit);
+    else atomicAnd(&v[dst >> 5], ~bit);
+  }
+}
+
 // Gather for the sorted output order: out[i] = in[perm[i]].
 __global__ void gather_kernel(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, int width,
                               const uint32_t* __restrict__ perm, int64_t n, const uint32_t* __restrict__ vsrc,
@@ -840,9 +879,42 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
     }
     if (terr[2] & 2u) {
       bolt_table_release(tbl);
-      return set_error(BOLT_ERR_TYPE, "SUM(Decimal128) overflow: accumulator exceeds i128 range");
+      return set_error(BOLT_ERR_TYPE, "SUM(Decimal128) precision overflow: accumulator exceeds i128 range");
     }
     break;
+  }
+
+  if (ks.null_group && keys[0].dtype == BOLT_I64 && n_groups >= 2) {
+    DevBuf<int64_t> clash;
+    int64_t hc[2] = {-1, 0};
+    st = clash.alloc(2, s);
+    if (st == BOLT_OK) st = fill_u32(clash.ptr, 0xFFFFFFFFu, 2, s);
+    if (st == BOLT_OK) st = fill_u32(reinterpret_cast<uint32_t*>(clash.ptr + 1), 0, 2, s);
+    if (st == BOLT_OK) {
+      gb_null_clash_kernel<<<grid_for(n_groups, 256, 4), 256, 0, s>>>(packed.ptr, slot_of.ptr, n_groups, cap,
+                                                                     clash.ptr);
+      if (hipGetLastError() != hipSuccess || hipMemcpyAsync(hc, clash.ptr, 16, hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess)
+        st = set_error(BOLT_ERR_DEVICE, "GROUP BY NULL-group check failed");
+    }
+    if (st == BOLT_OK && hc[0] >= 0 && hc[1]) {
+      MoveCols mc;
+      memset(&mc, 0, sizeof mc);
+      mc.n = (int32_t)cols.size();
+      for (int c = 0; c < mc.n; c++) {
+        mc.d[c] = static_cast<uint8_t*>(const_cast<void*>(cols[c].data));
+        mc.v[c] = reinterpret_cast<uint32_t*>(const_cast<uint8_t*>(cols[c].validity));
+        mc.w[c] = dtype_size(cols[c].dtype);
+      }
+      if (hc[0] != n_groups - 1)
+        gb_move_row_kernel<<<1, 64, 0, s>>>(mc, packed.ptr, slot_of.ptr, hc[0], n_groups - 1);
+      if (hipGetLastError() != hipSuccess) st = set_error(BOLT_ERR_DEVICE, "GROUP BY NULL-group removal failed");
+      n_groups--;
+    }
+    if (st != BOLT_OK) {
+      bolt_table_release(tbl);
+      return st;
+    }
   }
 
   // ---- ordering by packed key (src/exec/groupby.rs:800) ----

@@ -18,7 +18,15 @@
 // same text the precompiled kernels use (device_acc.h), so results are the
 // same whichever path runs.  If hipRTC fails, callers fall back to the
 // precompiled generic kernels; BOLT_JIT=0 forces that fallback.
+#include <dirent.h>
 #include <hip/hiprtc.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <thread>
 
 #include <cstdlib>
 #include <fstream>
@@ -47,15 +55,6 @@ struct JitEntry {
 };
 std::mutex g_mu;
 std::unordered_map<std::string, JitEntry> g_cache;
-
-uint64_t fnv1a(const std::string& s) {
-  uint64_t h = 1469598103934665603ull;
-  for (unsigned char c : s) {
-    h ^= c;
-    h *= 1099511628211ull;
-  }
-  return h;
-}
 
 const char* ctype(int dt) {
   switch (dt) {
@@ -117,6 +116,147 @@ void emit_col_ptrs(std::ostringstream& o, const std::vector<JitSpec::Col>& cols)
 
 const char* jit_prelude_text() { return kJitPrelude; }
 
+// ---- persistent code-object cache (BOLT_JIT_CACHE_DIR) ----------------------
+//
+// The role of src/jit/disk_cache.rs for this build: generated kernels survive
+// the process as gfx950 code objects, one file per kernel.
+//   * Opt-in: BOLT_JIT_CACHE_DIR names the directory; unset or empty = off.
+//   * Trust (disk_cache.rs:70-90): the directory must be owned by the
+//     effective uid and not be group- or world-writable, else the disk layer
+//     is off (a code object read back is launched).
+//   * Key: FNV-1a 64 of a salt (library version, target, hipRTC version, the
+//     compile options) followed by the source.  The file starts with a header
+//     holding a magic, that key, an independent digest and the length of the
+//     source, and the code length; a file whose header does not match the
+//     kernel being looked up, or whose body is short, is ignored and
+//     recompiled (no collision or stale file is ever loaded).
+//   * Atomicity: written to a per-process, per-thread temporary name in the
+//     same directory, then renamed into place.
+//   * Bounds: at most BOLT_JIT_CACHE_MAX_ENTRIES files (default 4096) and
+//     BOLT_JIT_CACHE_MAX_BYTES bytes (default 256 MiB); the oldest files by
+//     modification time go first.
+namespace {
+
+const char* const kJitOpts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+                                "-packed-fp32-ops"};
+constexpr int kJitNOpts = (int)(sizeof kJitOpts / sizeof kJitOpts[0]);
+constexpr char kCoMagic[8] = {'B', 'O', 'L', 'T', 'C', 'O', '1', '\0'};
+
+struct CoHeader {
+  char magic[8];
+  uint64_t key;
+  uint64_t src_digest;
+  uint64_t src_len;
+  uint64_t code_len;
+};
+
+std::atomic<int64_t> g_compiled{0}, g_disk_hits{0}, g_disk_stores{0};
+
+uint64_t fnv1a_from(uint64_t h, const std::string& s) {
+  for (unsigned char c : s) {
+    h ^= c;
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+const std::string& cache_salt() {
+  static const std::string salt = [] {
+    int maj = 0, min = 0;
+    hiprtcVersion(&maj, &min);
+    std::ostringstream o;
+    o << "craton_bolt_amd 0.1|gfx950|hiprtc " << maj << '.' << min << '|';
+    for (int i = 0; i < kJitNOpts; i++) o << kJitOpts[i] << ' ';
+    o << '\n';
+    return o.str();
+  }();
+  return salt;
+}
+
+// The cache directory, or "" when the disk layer is off.
+std::string cache_dir() {
+  const char* dir = getenv("BOLT_JIT_CACHE_DIR");
+  if (!dir || !*dir) return "";
+  struct stat st;
+  if (stat(dir, &st) != 0 || !S_ISDIR(st.st_mode) || st.st_uid != geteuid() || (st.st_mode & 022) != 0) return "";
+  return dir;
+}
+
+int64_t env_i64(const char* name, int64_t dflt) {
+  const char* e = getenv(name);
+  return e && *e ? strtoll(e, nullptr, 10) : dflt;
+}
+
+// Removes the oldest cache files until both bounds hold.
+void cache_evict(const std::string& dir) {
+  const int64_t max_n = env_i64("BOLT_JIT_CACHE_MAX_ENTRIES", 4096);
+  const int64_t max_b = env_i64("BOLT_JIT_CACHE_MAX_BYTES", (int64_t)256 << 20);
+  DIR* d = opendir(dir.c_str());
+  if (!d) return;
+  struct Ent {
+    std::string path;
+    int64_t mtime, size;
+  };
+  std::vector<Ent> ents;
+  int64_t total = 0;
+  while (struct dirent* e = readdir(d)) {
+    const std::string name = e->d_name;
+    if (name.size() < 9 || name.compare(0, 5, "bolt_") != 0 || name.compare(name.size() - 3, 3, ".co") != 0) continue;
+    struct stat st;
+    const std::string path = dir + "/" + name;
+    if (stat(path.c_str(), &st) != 0) continue;
+    ents.push_back({path, (int64_t)st.st_mtime, (int64_t)st.st_size});
+    total += st.st_size;
+  }
+  closedir(d);
+  if ((max_n <= 0 || (int64_t)ents.size() <= max_n) && (max_b <= 0 || total <= max_b)) return;
+  std::sort(ents.begin(), ents.end(), [](const Ent& a, const Ent& b) { return a.mtime < b.mtime; });
+  int64_t n = (int64_t)ents.size();
+  for (const Ent& e : ents) {
+    if ((max_n <= 0 || n <= max_n) && (max_b <= 0 || total <= max_b)) break;
+    if (unlink(e.path.c_str()) == 0) {
+      n--;
+      total -= e.size;
+    }
+  }
+}
+
+bool cache_read(const std::string& path, uint64_t key, uint64_t digest, uint64_t src_len, std::vector<char>* code) {
+  std::ifstream in(path, std::ios::binary);
+  if (!in) return false;
+  CoHeader h;
+  if (!in.read(reinterpret_cast<char*>(&h), sizeof h)) return false;
+  if (memcmp(h.magic, kCoMagic, 8) != 0 || h.key != key || h.src_digest != digest || h.src_len != src_len ||
+      h.code_len == 0 || h.code_len > ((uint64_t)1 << 30))
+    return false;
+  code->resize(h.code_len);
+  if (!in.read(code->data(), (std::streamsize)h.code_len)) return false;
+  return in.peek() == std::char_traits<char>::eof();
+}
+
+void cache_write(const std::string& dir, const std::string& path, const CoHeader& h, const std::vector<char>& code) {
+  std::ostringstream tmp;
+  tmp << path << ".tmp." << getpid() << '.' << std::hash<std::thread::id>{}(std::this_thread::get_id());
+  {
+    std::ofstream out(tmp.str(), std::ios::binary);
+    out.write(reinterpret_cast<const char*>(&h), sizeof h);
+    out.write(code.data(), (std::streamsize)code.size());
+    if (!out) {
+      out.close();
+      unlink(tmp.str().c_str());
+      return;
+    }
+  }
+  if (std::rename(tmp.str().c_str(), path.c_str()) != 0) {
+    unlink(tmp.str().c_str());
+    return;
+  }
+  g_disk_stores++;
+  cache_evict(dir);
+}
+
+}  // namespace
+
 int jit_get(const std::string& src, hipFunction_t* fn) {
   std::lock_guard<std::mutex> lk(g_mu);
   auto it = g_cache.find(src);
@@ -125,21 +265,26 @@ int jit_get(const std::string& src, hipFunction_t* fn) {
     return BOLT_OK;
   }
   std::vector<char> code;
-  std::string cache_path;
-  if (const char* dir = getenv("BOLT_JIT_CACHE_DIR")) {
+  std::string path;
+  const std::string dir = cache_dir();
+  CoHeader hdr;
+  memset(&hdr, 0, sizeof hdr);
+  if (!dir.empty()) {
+    memcpy(hdr.magic, kCoMagic, 8);
+    hdr.key = fnv1a_from(fnv1a_from(1469598103934665603ull, cache_salt()), src);
+    hdr.src_digest = fnv1a_from(0x6a09e667f3bcc908ull, src);  // a second basis: an independent digest
+    hdr.src_len = src.size();
     char name[64];
-    snprintf(name, sizeof name, "/bolt_%016llx.co", (unsigned long long)fnv1a(src));
-    cache_path = std::string(dir) + name;
-    std::ifstream in(cache_path, std::ios::binary);
-    if (in) code.assign(std::istreambuf_iterator<char>(in), std::istreambuf_iterator<char>());
+    snprintf(name, sizeof name, "/bolt_%016llx.co", (unsigned long long)hdr.key);
+    path = dir + name;
+    if (cache_read(path, hdr.key, hdr.src_digest, hdr.src_len, &code)) g_disk_hits++;
+    else code.clear();
   }
   if (code.empty()) {
     hiprtcProgram prog;
     if (hiprtcCreateProgram(&prog, src.c_str(), "bolt_jit.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
       return set_error(BOLT_ERR_OTHER, "hipRTC: program creation failed");
-    const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
-                          "-packed-fp32-ops"};
-    hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof opts / sizeof opts[0]), opts);
+    hiprtcResult r = hiprtcCompileProgram(prog, kJitNOpts, kJitOpts);
     if (r != HIPRTC_SUCCESS) {
       size_t ls = 0;
       hiprtcGetProgramLogSize(prog, &ls);
@@ -153,11 +298,10 @@ int jit_get(const std::string& src, hipFunction_t* fn) {
     code.resize(cs);
     hiprtcGetCode(prog, code.data());
     hiprtcDestroyProgram(&prog);
-    if (!cache_path.empty()) {
-      std::ofstream out(cache_path + ".tmp", std::ios::binary);
-      out.write(code.data(), (std::streamsize)code.size());
-      out.close();
-      std::rename((cache_path + ".tmp").c_str(), cache_path.c_str());
+    g_compiled++;
+    if (!path.empty()) {
+      hdr.code_len = code.size();
+      cache_write(dir, path, hdr, code);
     }
   }
   JitEntry e;
@@ -573,3 +717,10 @@ int jit_launch_row_agg(hipFunction_t fn, unsigned grid, hipStream_t s, const uin
 }
 
 }  // namespace bolt
+
+extern "C" int bolt_jit_cache_stats(int64_t* compiled, int64_t* disk_hits, int64_t* disk_stores) {
+  if (compiled) *compiled = bolt::g_compiled.load();
+  if (disk_hits) *disk_hits = bolt::g_disk_hits.load();
+  if (disk_stores) *disk_stores = bolt::g_disk_stores.load();
+  return BOLT_OK;
+}

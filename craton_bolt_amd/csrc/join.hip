@@ -731,7 +731,19 @@ extern "C" int bolt_join_build(bolt_stream_t stream, const bolt_column_t* build_
       return set_error(BOLT_ERR_DEVICE, "hash join build launch failed: %s", hipGetErrorString(e));
     }
   }
+  if (hipEventCreateWithFlags(&t->built, hipEventDisableTiming) != hipSuccess ||
+      hipEventRecord(t->built, s) != hipSuccess) {
+    bolt_join_table_release(t);
+    return set_error(BOLT_ERR_DEVICE, "hash join build: completion event");
+  }
   *out = t;
+  return BOLT_OK;
+}
+
+// A probe on a stream other than the build's waits for the build to finish
+// (on the device: the host does not block).
+static int wait_built(const bolt_join_table_t* t, hipStream_t s) {
+  if (t->built && s != t->stream) BOLT_HIP(hipStreamWaitEvent(s, t->built, 0));
   return BOLT_OK;
 }
 
@@ -750,6 +762,7 @@ extern "C" void bolt_join_table_release(bolt_join_table_t* t) {
   if (t->tbl) dev_free(t->tbl, t->stream);
   if (t->flags) dev_free(t->flags, t->stream);
   if (t->bloom) dev_free(t->bloom, t->stream);
+  if (t->built) (void)hipEventDestroy(t->built);
   delete t;
 }
 
@@ -802,6 +815,13 @@ static int launch_probe_agg(int na, const bolt_join_table_t* t, const bolt_colum
   }
 }
 
+// The radix-partitioned probe + aggregate (join_part.hip).
+namespace bolt {
+bool join_partitioned_wanted(const bolt_join_table_t* t, int64_t n_probe, const AggList& aggs);
+int join_probe_aggregate_partitioned(hipStream_t s, const bolt_join_table_t* t, const bolt_column_t& key,
+                                     const DevFilter& f, const AggList& all, Acc* outs);
+}  // namespace bolt
+
 // Grid for the probe kernels; BOLT_JOIN_PROBE_GRID overrides (tuning).
 static unsigned probe_grid(int64_t n) {
   const char* env = getenv("BOLT_JOIN_PROBE_GRID");
@@ -817,6 +837,7 @@ extern "C" int bolt_join_probe_aggregate(bolt_stream_t stream, const bolt_join_t
   clear_error();
   hipStream_t s = (hipStream_t)stream;
   BOLT_TRY(check_probe_args(t, probe_cols, n_probe_cols, key_col, "bolt_join_probe_aggregate"));
+  BOLT_TRY(wait_built(t, s));
   const bolt_column_t& key = probe_cols[key_col];
   DevFilter f;
   BOLT_TRY(make_dev_filter(filter, probe_cols, n_probe_cols, key.length, &f));
@@ -827,7 +848,11 @@ extern "C" int bolt_join_probe_aggregate(bolt_stream_t stream, const bolt_join_t
   DevBuf<Acc> parts, outs;
   BOLT_TRY(outs.alloc((size_t)n_aggs, s));
   bool done = false;
-  if (jit_enabled() && key.length > 0) {
+  if (join_partitioned_wanted(t, key.length, all)) {
+    BOLT_TRY(join_probe_aggregate_partitioned(s, t, key, f, all, outs.ptr));
+    done = true;
+  }
+  if (!done && jit_enabled() && key.length > 0) {
     JitSpec sp;
     JitLits lits;
     JitCols cols;
@@ -939,6 +964,7 @@ extern "C" int bolt_join_probe(bolt_stream_t stream, const bolt_join_table_t* t,
   hipStream_t s = (hipStream_t)stream;
   *n_out = 0;
   BOLT_TRY(check_probe_args(t, probe_cols, n_probe_cols, key_col, "bolt_join_probe"));
+  BOLT_TRY(wait_built(t, s));
   if (t->carry_dtype != 0)
     return set_error(BOLT_ERR_PLAN, "bolt_join_probe: table stores a carried column, not build row ids");
   const bolt_column_t& key = probe_cols[key_col];

@@ -13,7 +13,8 @@ struct bolt_join_table {
   int32_t key_dtype = 0;
   int32_t carry_dtype = 0;  // 0 = values are build row ids
   uint32_t* flags = nullptr; // [0] nonunique, [1] lost rows
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;  // the build's stream (its kernels may still run when build returns)
+  hipEvent_t built = nullptr;    // recorded after the build's last kernel; probes on other streams wait on it
   uint4* bloom = nullptr;     // split-block Bloom filter of the build keys (nullptr = none)
   uint64_t bloom_mask = 0;    // blocks - 1 (power of two)
 };

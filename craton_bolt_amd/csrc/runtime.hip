@@ -189,6 +189,77 @@ int bolt_profile_events(bolt_event_t start, bolt_event_t end) {
   return 0;
 }
 
+static const char* status_name(int e) {
+  if (e == BOLT_HIP_ERROR_NO_DEVICE) return "BOLT_HIP_ERROR_NO_DEVICE";
+  const char* n = hipGetErrorName((hipError_t)e);
+  return (n && strcmp(n, "hipErrorUnknown") != 0) || e == (int)hipErrorUnknown ? n : nullptr;
+}
+
+int bolt_hip_get_error_name(int error, const char** str) {
+  *str = status_name(error);
+  return *str ? 0 : (int)hipErrorInvalidValue;
+}
+
+int bolt_hip_get_error_string(int error, const char** str) {
+  if (error == BOLT_HIP_ERROR_NO_DEVICE) {
+    *str = "no usable HIP device (the library was loaded on a host without a GPU)";
+    return 0;
+  }
+  if (!status_name(error)) {
+    *str = nullptr;
+    return (int)hipErrorInvalidValue;
+  }
+  *str = hipGetErrorString((hipError_t)error);
+  return 0;
+}
+
+// A context handle is device + 1, so that NULL stays "no context".
+static thread_local bolt_ctx_t t_current_ctx = nullptr;
+static bolt_ctx_t ctx_of(int device) { return reinterpret_cast<bolt_ctx_t>((uintptr_t)device + 1); }
+static int device_of(bolt_ctx_t ctx) { return (int)(reinterpret_cast<uintptr_t>(ctx) - 1); }
+
+int bolt_hip_device_get(int* device, int ordinal) {
+  RT_CHECK();
+  int n = 0;
+  RT_HIP(hipGetDeviceCount(&n));
+  if (ordinal < 0 || ordinal >= n)
+    return set_error((int)hipErrorInvalidDevice, "bolt_hip_device_get: ordinal %d of %d devices", ordinal, n);
+  *device = ordinal;
+  return 0;
+}
+
+int bolt_hip_ctx_create(bolt_ctx_t* ctx, unsigned int flags, int device) {
+  (void)flags;
+  *ctx = nullptr;
+  RT_CHECK();
+  hipCtx_t primary;
+  RT_HIP(hipDevicePrimaryCtxRetain(&primary, device));
+  RT_HIP(hipSetDevice(device));
+  *ctx = t_current_ctx = ctx_of(device);
+  return 0;
+}
+
+int bolt_hip_ctx_destroy(bolt_ctx_t ctx) {
+  RT_CHECK();
+  if (!ctx) return set_error((int)hipErrorInvalidContext, "bolt_hip_ctx_destroy: NULL context");
+  RT_HIP(hipDevicePrimaryCtxRelease(device_of(ctx)));
+  if (t_current_ctx == ctx) t_current_ctx = nullptr;
+  return 0;
+}
+
+int bolt_hip_ctx_set_current(bolt_ctx_t ctx) {
+  RT_CHECK();
+  if (ctx) RT_HIP(hipSetDevice(device_of(ctx)));
+  t_current_ctx = ctx;
+  return 0;
+}
+
+int bolt_hip_ctx_get_current(bolt_ctx_t* ctx) {
+  RT_CHECK();
+  *ctx = t_current_ctx;
+  return 0;
+}
+
 int bolt_hip_init(unsigned int flags) {
   if (flags != 0) return set_error(hipErrorInvalidValue, "bolt_hip_init: flags must be 0");
   RT_CHECK();
@@ -451,6 +522,13 @@ int bolt_hip_module_load_data(bolt_module_t* module, const void* image) {
   *module = (bolt_module_t)m;
   return 0;
 }
+int bolt_hip_module_load_data_ex(bolt_module_t* module, const void* image, unsigned int num_options, int* options,
+                                 void** option_values) {
+  (void)num_options;
+  (void)options;
+  (void)option_values;
+  return bolt_hip_module_load_data(module, image);
+}
 int bolt_hip_module_unload(bolt_module_t module) {
   RT_CHECK();
   RT_HIP(hipModuleUnload((hipModule_t)module));
@@ -561,10 +639,14 @@ int bolt_hip_stream_end_capture(bolt_stream_t stream, bolt_graph_t* graph) {
   *graph = (bolt_graph_t)g;
   return 0;
 }
-int bolt_hip_graph_instantiate(bolt_graph_exec_t* exec, bolt_graph_t graph) {
+int bolt_hip_graph_instantiate(bolt_graph_exec_t* exec, bolt_graph_t graph, void** error_node, char* log_buffer,
+                               size_t buffer_size) {
   RT_CHECK();
   hipGraphExec_t x;
-  RT_HIP(hipGraphInstantiate(&x, (hipGraph_t)graph, nullptr, nullptr, 0));
+  hipGraphNode_t bad = nullptr;
+  if (log_buffer && buffer_size) log_buffer[0] = '\0';
+  RT_HIP(hipGraphInstantiate(&x, (hipGraph_t)graph, &bad, log_buffer, log_buffer ? buffer_size : 0));
+  if (error_node) *error_node = bad;
   *exec = (bolt_graph_exec_t)x;
   return 0;
 }

@@ -636,7 +636,7 @@ struct bolt_server {
     BOLT_HIP(hipMemcpyAsync(err, A.misc.ptr + 2, 16, hipMemcpyDeviceToHost, s));
     BOLT_HIP(hipStreamSynchronize(s));
     if (err[2] & 1u) return set_error(BOLT_ERR_TYPE, "SUM(integer) overflow: accumulator exceeds i64 range");
-    if (err[2] & 2u) return set_error(BOLT_ERR_TYPE, "SUM(Decimal128) overflow: accumulator exceeds i128 range");
+    if (err[2] & 2u) return set_error(BOLT_ERR_TYPE, "SUM(Decimal128) precision overflow: accumulator exceeds i128 range");
     const int64_t n_groups = (int64_t)ng;
     const int64_t L = std::min<int64_t>(q.limit, n_groups);
     const int nc = q.n_group + q.n_aggs;
@@ -869,13 +869,19 @@ struct bolt_server {
       int st = finish(A, d, fs);
       bool retry = false;
       SReq r = A.r;
+      const int64_t admitted = A.r.need;  // what admission added to bytes_in_use
       if (st == 1 && A.r.attempt < 3) {
-        // a group table filled up: run the query again with 4x the table
+        // a group table filled up: run the query again with 4x the table,
+        // unless the larger table no longer fits the whole budget (then it
+        // could never be admitted and would block the queue head)
         r.attempt++;
         r.min_cap = A.r.cap * 4;
         const int pst = plan_request(r);
-        if (pst == BOLT_OK) retry = true;
-        else st = pst;
+        if (pst != BOLT_OK) st = pst;
+        else if (r.need > cfg.hbm_budget)
+          st = set_error(BOLT_DECLINE, "serve: the re-planned group table (%lld bytes) exceeds the HBM budget",
+                         (long long)r.need);
+        else retry = true;
       } else if (st == 1) {
         st = set_error(BOLT_ERR_MEMORY, "serve: group table overflow (est_groups far too low)");
       }
@@ -884,7 +890,7 @@ struct bolt_server {
       finish_us += std::chrono::duration_cast<std::chrono::microseconds>(SClock::now() - t0f).count();
       {
         std::lock_guard<std::mutex> lk(mu);
-        bytes_in_use -= r.need;
+        bytes_in_use -= admitted;  // the retry is charged again when it is re-admitted
         if (retry) queue.push_front(r);
         fin_pending--;
       }

@@ -696,7 +696,8 @@ __device__ __forceinline__ uint64_t coarse_of(const TopKey& k, uint64_t raw, boo
     if (k.desc) x = ~x & (k.bits >= 64 ? ~0ull : ((1ull << k.bits) - 1));
   }
   if (k.v == nullptr) return x;
-  if (!valid) return k.nulls_first ? 0ull : ~0ull;
+  // every image stays below 2^bits, so the select can start at bit `bits`
+  if (!valid) return k.nulls_first ? 0ull : (k.bits >= 64 ? ~0ull : (1ull << k.bits) - 1);
   return k.nulls_first ? (x >> 1) + 1 : x >> 1;
 }
 
@@ -720,8 +721,10 @@ __global__ __launch_bounds__(kSelBlock) void topk_hist_kernel(TopKey k, int64_t 
                                                               unsigned long long* __restrict__ spec_count,
                                                               int64_t spec_cap) {
   __shared__ uint32_t sh[kSelBins];
+  __shared__ int spec_full;  // set once this workgroup has seen the guess buffer full
   const int nb = 1 << width;
   for (int b = threadIdx.x; b < nb; b += kSelBlock) sh[b] = 0;
+  if (threadIdx.x == 0) spec_full = 0;
   __syncthreads();
   const int hi = shift + width;
   const uint64_t dmask = (uint64_t)nb - 1;
@@ -740,14 +743,19 @@ __global__ __launch_bounds__(kSelBlock) void topk_hist_kernel(TopKey k, int64_t 
       if (i < n) {
         const uint64_t c = coarse_of(k, x[u], bit_valid(k.v, i));
         if (hi >= 64 || (c >> hi) == prefix) atomicAdd(&sh[(c >> shift) & dmask], 1u);
-        take = spec != nullptr && (c >> shift) <= spec_bound;
+        // a guess that has already overflowed is abandoned: the collect pass
+        // runs instead, so further appends (one atomic per wave) are waste
+        take = spec != nullptr && (c >> shift) <= spec_bound && !*(volatile int*)&spec_full;
       }
       const uint64_t m = __ballot(take);  // lanes still in the loop
       if (m) {
         const int lane = threadIdx.x & 63;
         const int leader = __builtin_ctzll(m);
         unsigned long long b = 0;
-        if (lane == leader) b = atomicAdd(spec_count, (unsigned long long)__popcll(m));
+        if (lane == leader) {
+          b = atomicAdd(spec_count, (unsigned long long)__popcll(m));
+          if ((int64_t)b + __popcll(m) > spec_cap) *(volatile int*)&spec_full = 1;
+        }
         b = __shfl(b, leader);
         if (take) {
           const int64_t pos = (int64_t)b + __popcll(m & ((1ull << lane) - 1));
@@ -1050,7 +1058,7 @@ extern "C" int bolt_sort_limit(bolt_stream_t stream, const bolt_column_t* keys, 
   BOLT_TRY(part.alloc((size_t)hgrid * kSelBins, s));
   std::vector<uint32_t> h(kSelBins);
   uint64_t prefix = 0;
-  int hi = 64;
+  int hi = tk.bits;  // every image is below 2^bits (coarse_of)
   int64_t below = 0, need = K, eq = n;
   const unsigned grid = grid_for(n, 256, kSelUnroll, 4096);
   // Candidates guessed in the first pass: the rows whose first digit is at

@@ -131,7 +131,10 @@ typedef struct bolt_filter {
  * and agg_with_pre::try_execute_resident (src/exec/agg_with_pre.rs:157).
  * Computes n_aggs aggregates over the rows of `inputs` that pass `filter`
  * (NULL filter = all rows) in one pass, and writes `results` (host array of
- * n_aggs).  Synchronises the stream.  Integer SUM overflow -> BOLT_ERR_TYPE.
+ * n_aggs).  Synchronises the stream.  Integer SUM overflow -> BOLT_ERR_TYPE,
+ * decided as the reference's checked_add fold decides it: as soon as a
+ * running total of the passing non-NULL values, in row order, leaves i64
+ * (src/exec/aggregate.rs:2120-2140), even when the total fits.
  * ------------------------------------------------------------------------ */
 int bolt_aggregate(bolt_stream_t stream, const bolt_column_t* inputs, int32_t n_inputs,
                    const bolt_agg_t* aggs, int32_t n_aggs, const bolt_filter_t* filter,
@@ -207,6 +210,11 @@ void bolt_table_release(bolt_table_t* t);
  * Int32 / Float32 / Date32 column of the build side) is stored beside each
  * key so probe-side aggregates read it without a gather.
  * A build side beyond the table cap returns BOLT_DECLINE (gpu_join.rs:378-393).
+ * Streams: bolt_join_build returns with its kernels still queued on `stream`
+ * and records an event after them; a probe on any other stream waits for
+ * that event on the device before reading the table.  Release frees the
+ * table in `stream` order: the probes below synchronise their own stream
+ * before returning, so a released table is never read.
  * ------------------------------------------------------------------------ */
 typedef struct bolt_join_table bolt_join_table_t;
 
@@ -391,6 +399,13 @@ int bolt_aggregate_expr(bolt_stream_t stream, const bolt_column_t* inputs, int32
  * ------------------------------------------------------------------------ */
 /* Build identifier string (target arch, version). */
 const char* bolt_build_info(void);
+
+/* Generated-kernel cache counters of this process (src/jit/disk_cache.rs
+ * keeps its persistent layer behind BOLT_PTX_CACHE_DIR; here
+ * BOLT_JIT_CACHE_DIR holds gfx950 code objects): kernels compiled with
+ * hipRTC, kernels loaded from the directory instead, and files written to it.
+ * Any pointer may be NULL. */
+int bolt_jit_cache_stats(int64_t* compiled, int64_t* disk_hits, int64_t* disk_stores);
 
 /* Measurement hook (no reference counterpart; for benchmarks).  While set on
  * the calling thread, the operators record `start` immediately before and

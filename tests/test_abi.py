@@ -43,3 +43,32 @@ def test_no_device_contract():
         N.BOLT_ERR_NO_DEVICE
     with pytest.raises(N.NoDeviceError):
         N.check(L.bolt_aggregate(None, C.byref(col), 1, aggs, 1, None, res))
+
+
+def test_error_names_and_strings():
+    """cuGetErrorName / cuGetErrorString (src/cuda/cuda_sys.rs:151-152):
+    static text for every status of this ABI, and an error for unknown codes."""
+    L = N.lib()
+    for code, name in [(0, b"hipSuccess"), (1, b"hipErrorInvalidValue"), (600, b"hipErrorNotReady"),
+                       (100000, b"BOLT_HIP_ERROR_NO_DEVICE")]:
+        p, q = C.c_char_p(), C.c_char_p()
+        assert L.bolt_hip_get_error_name(code, C.byref(p)) == 0
+        assert p.value == name
+        assert L.bolt_hip_get_error_string(code, C.byref(q)) == 0
+        assert q.value
+    p = C.c_char_p(b"x")
+    assert L.bolt_hip_get_error_name(123456, C.byref(p)) == 1 and p.value is None
+    assert L.bolt_hip_get_error_string(123456, C.byref(p)) == 1 and p.value is None
+
+
+@pytest.mark.skipif(device_count() > 0, reason="stub contract is only observable without a GPU")
+def test_no_device_contract_for_contexts_and_modules():
+    L = N.lib()
+    d, ctx, mod = C.c_int(-1), C.c_void_p(), C.c_void_p()
+    assert L.bolt_hip_device_get(C.byref(d), 0) == N.BOLT_ERR_NO_DEVICE
+    assert L.bolt_hip_ctx_create(C.byref(ctx), 0, 0) == N.BOLT_ERR_NO_DEVICE
+    assert ctx.value is None
+    assert L.bolt_hip_ctx_get_current(C.byref(ctx)) == N.BOLT_ERR_NO_DEVICE
+    assert L.bolt_hip_module_load_data_ex(C.byref(mod), b"\0" * 16, 0, None, None) == N.BOLT_ERR_NO_DEVICE
+    ex = C.c_void_p()
+    assert L.bolt_hip_graph_instantiate(C.byref(ex), None, None, None, 0) == N.BOLT_ERR_NO_DEVICE

@@ -95,6 +95,54 @@ def test_integer_sum_overflow_is_an_error(gpu):
     assert execute_aggregate([dev], [(N.SUM, 0)]) == [O.scalar_value(O.scalar_agg(host, O.SUM))]
 
 
+def _row_order_overflows(vals):
+    s = 0
+    for v in vals:
+        s += int(v)
+        if not -2 ** 63 <= s < 2 ** 63:
+            return True
+    return False
+
+
+@pytest.mark.parametrize("case", ["max_then_one", "climb_then_fall", "fall_then_climb", "late", "filtered", "nulls"])
+def test_integer_sum_overflows_in_row_order(gpu, case):
+    """SUM(Int64) fails as soon as a running total in row order leaves i64
+    (checked_add, src/exec/aggregate.rs:2120-2140), also when the total fits."""
+    big = 2 ** 63 - 1
+    rng = np.random.default_rng(5)
+    terms, valid = None, None
+    if case == "max_then_one":
+        x = np.array([big, 1, -1], np.int64)
+    elif case == "climb_then_fall":
+        x = np.array([2 ** 62, 2 ** 62, -2 ** 62, -2 ** 62], np.int64)
+    elif case == "fall_then_climb":
+        x = np.array([-2 ** 62, -2 ** 62, -2 ** 62, 2 ** 62, 2 ** 62, 2 ** 62], np.int64)
+    elif case == "late":
+        # 2M rows of small values, then the excursion near the end (many workgroups)
+        x = rng.integers(-1000, 1000, 2_000_000).astype(np.int64)
+        x[-3:] = [big, big, -big]
+    elif case == "filtered":
+        # the row that would overflow is filtered out: no error
+        x = np.array([big, 1, -1, 5], np.int64)
+        terms = [Term(0, N.LT, big)]
+    else:
+        # the row that would overflow is NULL: no error
+        x = np.array([big, 1, -1, -5], np.int64)
+        valid = [1, 0, 1, 1]
+    dev, host = both(x, N.I64, valid)
+    kept = x if terms is None else x[x < big]
+    if valid is not None:
+        kept = x[np.array(valid, bool)]
+    want = O.scalar_agg(host, O.SUM) if terms is None else None
+    if _row_order_overflows(kept):
+        assert want is None or want.status != O.OK
+        with pytest.raises(N.BoltTypeError, match="SUM\\(integer\\) overflow"):
+            execute_aggregate([dev], [(N.SUM, 0)], terms)
+    else:
+        assert execute_aggregate([dev], [(N.SUM, 0), (N.COUNT, 0)], terms) == [int(sum(int(v) for v in kept)),
+                                                                            len(kept)]
+
+
 def test_decimal_sum_fixtures(gpu):
     # tests/decimal_type_test.rs:324-431
     dev, _ = both(dec_array([123, -456, 789, 1, 10_000]), N.DEC128, scale=2)
@@ -103,7 +151,7 @@ def test_decimal_sum_fixtures(gpu):
     assert execute_aggregate([dev], [(N.SUM, 0)]) == [150]
     big = 10 ** 38 - 1
     dev, _ = both(dec_array([big, big]), N.DEC128)
-    with pytest.raises(N.BoltTypeError, match="Decimal128"):
+    with pytest.raises(N.BoltTypeError, match="SUM\\(Decimal128\\) precision overflow"):
         execute_aggregate([dev], [(N.SUM, 0)])
     rng = np.random.default_rng(4)
     vals = [int(v) * 10 ** 20 + int(w) for v, w in zip(rng.integers(-10 ** 15, 10 ** 15, 100_000),

@@ -231,6 +231,28 @@ def test_integer_sum_overflow_is_an_error(gpu):
         execute_groupby([both(k, N.I32, None)[0]], [both(v, N.I64, None)[0]], [(N.SUM, 0)])
 
 
+@pytest.mark.parametrize("flags", [0, N.GROUPBY_FORCE_GLOBAL, N.GROUPBY_FORCE_PARTITIONED])
+def test_null_group_dropped_next_to_a_genuine_max_key(gpu, flags):
+    """A single nullable Int64 key column holding a genuine i64::MAX: the
+    reference cannot tell that key from its NULL-group sentinel and runs the
+    valid-flag executor, which drops the NULL-key rows (groupby.rs:553-583).
+    Without such a key, the NULL rows form one NULL group."""
+    rng = np.random.default_rng(21)
+    n = 300_000
+    k = rng.integers(-50, 50, n).astype(np.int64)
+    k[::997] = 2 ** 63 - 1
+    valid = rng.random(n) > 0.1
+    v = rng.integers(-1000, 1000, n).astype(np.int64)
+    aggs = [(N.SUM, 0), (N.COUNT_STAR, 0), (N.MAX, 0)]
+    res, want = run_both([(k, N.I64, valid)], [(v, N.I64, None)], aggs, [N.I64], flags=flags)
+    kv = res["keys"][0][1]
+    assert kv is None or kv.all(), "NULL group present next to a genuine i64::MAX key"
+    assert (res["keys"][0][0] == 2 ** 63 - 1).sum() == 1
+    k2 = np.where(k == 2 ** 63 - 1, 7, k)
+    res2, _ = run_both([(k2, N.I64, valid)], [(v, N.I64, None)], aggs, [N.I64], flags=flags)
+    assert res2["keys"][0][1] is not None and (~res2["keys"][0][1]).sum() == 1
+
+
 @pytest.mark.parametrize("n", [50_000, 1_500_000])
 @pytest.mark.parametrize("flags", [0, N.GROUPBY_FORCE_PARTITIONED])
 def test_decimal_sum_by_group(gpu, n, flags):

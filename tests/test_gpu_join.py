@@ -20,13 +20,16 @@ REL_TOL = 1e-9
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(autouse=True, params=["jit", "generic", "jit+bloom"])
+@pytest.fixture(autouse=True, params=["jit", "generic", "jit+bloom", "partitioned"])
 def kernel_path(request, monkeypatch):
     """Every case runs through the per-query hipRTC kernel, the precompiled
-    generic kernel (BOLT_JIT=0), and the hipRTC kernel behind the build
-    side's Bloom filter (BOLT_JOIN_BLOOM=1)."""
+    generic kernel (BOLT_JIT=0), the hipRTC kernel behind the build side's
+    Bloom filter (BOLT_JOIN_BLOOM=1), and with the fused probe + aggregate
+    forced onto the radix-partitioned form (BOLT_JOIN_PARTITIONED=1; the
+    random form otherwise, chosen only for large tables)."""
     monkeypatch.setenv("BOLT_JIT", "0" if request.param == "generic" else "1")
     monkeypatch.setenv("BOLT_JOIN_BLOOM", "1" if request.param == "jit+bloom" else "0")
+    monkeypatch.setenv("BOLT_JOIN_PARTITIONED", "1" if request.param == "partitioned" else "0")
     return request.param
 
 
@@ -185,3 +188,40 @@ def test_region_build_matches_atomic_build(gpu, monkeypatch, load, kernel_path):
     t = JoinTable(du)
     assert t.unique()
     t.release()
+
+
+@pytest.mark.parametrize("key_dt", [N.I64, N.I32])
+@pytest.mark.parametrize("part_bits", ["3", "6", "10"])
+def test_partitioned_probe_aggregate_matches_oracle(gpu, kernel_path, monkeypatch, key_dt, part_bits):
+    """The radix-partitioned probe + aggregate against the oracle's host join:
+    duplicate build keys, NULL probe keys, three filter terms over two
+    columns, two probe columns carried with the key (Float64, Int32), the
+    carried build column, and five aggregates (two probe launches)."""
+    if kernel_path != "partitioned":
+        pytest.skip("a test of the partitioned form")
+    monkeypatch.setenv("BOLT_JOIN_PART_BITS", part_bits)
+    print(f"[case] partitioned key={key_dt} bits={part_bits}", flush=True)
+    rng = np.random.default_rng(int(part_bits) * 10 + key_dt)
+    nb, npr = 400_000, 2_500_000
+    np_dt = np.int64 if key_dt == N.I64 else np.int32
+    bk = (rng.integers(0, 300_000, nb) * 13 - 1000).astype(np_dt)
+    bp = rng.integers(-50, 1000, nb).astype(np.int32)
+    pk = (rng.integers(0, 450_000, npr) * 13 - 1000).astype(np_dt)
+    pv = rng.random(npr) > 0.03
+    amt = rng.uniform(0, 100, npr)
+    q = rng.integers(-20, 20, npr).astype(np.int32)
+    t = JoinTable(DeviceColumn.from_numpy(bk), carry=DeviceColumn.from_numpy(bp))
+    assert not t.unique()
+    cols = [DeviceColumn.from_numpy(pk, valid=pv), DeviceColumn.from_numpy(amt), DeviceColumn.from_numpy(q)]
+    aggs = [(N.SUM, N.JOIN_CARRY), (N.SUM, 1), (N.COUNT_STAR, 0), (N.MAX, 2), (N.SUM, 2)]
+    terms = [Term(1, N.GT, 10.0), Term(1, N.LE, 95.0), Term(2, N.NE, 0)]
+    got = t.probe_aggregate(cols, 0, aggs, terms)
+    okd = O.I64 if key_dt == N.I64 else O.I32
+    ol, orr = O.inner_join(O.HostCol(bk, okd), O.HostCol(pk, okd, pv))
+    m = (amt[orr] > 10.0) & (amt[orr] <= 95.0) & (q[orr] != 0)
+    ol, orr = ol[m], orr[m]
+    assert got[0] == int(bp[ol].astype(np.int64).sum())
+    assert rel_close(got[1], math.fsum(amt[orr]), REL_TOL)
+    assert got[2] == len(ol)
+    assert got[3] == int(q[orr].max())
+    assert got[4] == int(q[orr].astype(np.int64).sum())

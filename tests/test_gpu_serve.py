@@ -250,3 +250,29 @@ def test_composed_operator_path_matches_oracle_and_served(star):
         sv = run_query(srv, spec)
         same_rows(r["rows"], sv["rows"], len(spec["group"]), sabs)
     srv.close()
+
+
+def test_serve_declines_a_regrown_table_over_the_budget(star):
+    """A query whose table overflows is re-planned 4x larger; when that no
+    longer fits the whole budget it completes with BOLT_DECLINE instead of
+    waiting at the queue head for ever, the memory its first attempt held is
+    returned exactly, and later queries still run within the budget."""
+    fits = dict(terms=[], joins=[(FK3, 2)], group=[(0, 1)], aggs=[(N.SUM, QTY), (N.COUNT_STAR, 0)], order_agg=1,
+                limit=20, est_groups=1)  # <= 100 groups: no overflow at the smallest table
+    grows = dict(terms=[], joins=[(FK1, 0)], group=[(0, 0)], aggs=[(N.SUM, QTY), (N.COUNT_STAR, 0)], order_agg=1,
+                 limit=20, est_groups=1)  # thousands of groups: overflows the smallest table
+    probe = make_server(star, slots=1, chunk_rows=100_000)
+    need = run_query(probe, fits)["hbm_bytes"]
+    probe.close()
+    srv = make_server(star, slots=2, chunk_rows=100_000, hbm_budget=need)
+    tickets = [srv.submit(srv.query(**grows))] + [srv.submit(srv.query(**fits)) for _ in range(3)]
+    res = [srv.wait(t) for t in tickets]
+    st = srv.stats()
+    srv.close()
+    assert res[0]["status"] == N.BOLT_DECLINE, res[0]
+    assert all(r["status"] == N.BOLT_OK for r in res[1:])
+    assert all(r["rows"] == res[1]["rows"] for r in res[1:])
+    assert st["peak_bytes"] <= need
+    g, want, sabs = reference(star, fits)
+    assert res[1]["n_groups"] == g
+    same_rows(res[1]["rows"], want, 1, sabs)
