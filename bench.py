@@ -8,6 +8,18 @@ Default workload (one "step" = one query over device-resident tables):
     query  : WHERE amount > 10, INNER JOIN, SUM(payload), SUM(amount), COUNT(*)
   metric: probe rows/s; the dominant kernel is the fused probe.
 
+The default run ("all") prints that line with a `sub_records` object beside
+it: the other configurations of the benchmark, each with its rate, roofline
+(operator-level and dominant kernel), a full-size correctness check against
+the oracle, float error against the FP64 compensated sum where floats are
+summed, and a single-threaded CPU baseline (median of 5 runs after one
+warm-up, on a bounded sample, host recorded):
+  groupby_1M / groupby_256M : plain GROUP BY (int32 key over 1000 values, int64 value)
+  sort_256M                 : ORDER BY radix sort of (int64 key, u32 row id) pairs
+  groupby_zipf_256M         : GROUP BY int64 Zipf(1.1) key over 10M ranks, SUM(float64), COUNT(*)
+  batch_512                 : 512 batched GROUP BY queries vs 512 plain ones
+  serve_200M                : served star queries over a 200M-row fact table
+
 Prints one JSON line.  Inputs are generated on the host (seeded), uploaded
 once, and resident in HBM before the timed region starts.
 """
@@ -29,8 +41,8 @@ def _parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="join", choices=["join", "sort", "topk", "expr", "h2o", "groupby", "groupby-zipf",
-                                                             "calib", "batch", "serve"])
+    p.add_argument("--workload", default="all", choices=["all", "join", "sort", "topk", "expr", "h2o", "groupby",
+                                                            "groupby-zipf", "calib", "batch", "serve"])
     p.add_argument("--limit", type=int, default=100, help="topk workload: LIMIT")
     p.add_argument("--fact-rows", type=int, default=200_000_000, help="serve workload: fact table rows")
     p.add_argument("--serve-queries", type=int, default=64, help="serve workload: queries per measurement")
@@ -41,8 +53,38 @@ def _parse():
     p.add_argument("--n-build", type=int, default=16 << 20)
     p.add_argument("--n-probe", type=int, default=256 << 20)
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-sample", type=int, default=32 << 20, help="probe rows timed on the CPU")
+    p.add_argument("--cpu-sample", type=int, default=8 << 20, help="probe rows timed on the CPU")
+    p.add_argument("--no-check", action="store_true", help="skip the full-size checks against the oracle")
     return p.parse_args()
+
+
+def _host_info() -> dict:
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
+def _cpu_median(fn, reps: int = 5) -> float:
+    """Median wall time (s) of `reps` runs of fn after one warm-up run."""
+    fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def _cpu_line(value: float, unit: str, sample: str, median_s: float) -> dict:
+    return {"value": value, "unit": unit, "cores": 1, "kind": "port", "sample": sample, "median_s": median_s,
+            "runs": "median of 5 after 1 warm-up", "host": _host_info()}
 
 
 def cpu_baseline_join(bk, bp, fk, amt, n_probe_full: int, sample: int) -> dict:
@@ -53,22 +95,14 @@ def cpu_baseline_join(bk, bp, fk, amt, n_probe_full: int, sample: int) -> dict:
     import oracle as O
 
     O.lib()
-    t0 = time.perf_counter()
-    O.filter_join_agg(bk, bp, fk[:0], amt[:0], 10.0)
-    t_build = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    O.filter_join_agg(bk, bp, fk[:sample], amt[:sample], 10.0)
-    t_both = time.perf_counter() - t0
+    t_build = _cpu_median(lambda: O.filter_join_agg(bk, bp, fk[:0], amt[:0], 10.0))
+    t_both = _cpu_median(lambda: O.filter_join_agg(bk, bp, fk[:sample], amt[:sample], 10.0))
     t_probe = max(t_both - t_build, 1e-9)
     t_full = t_build + t_probe * (n_probe_full / sample)
-    return {
-        "value": n_probe_full / t_full,
-        "unit": "probe rows/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": f"C restatement of join.rs:169 host join + aggregate.rs:154; full {len(bk)}-row build "
-                  f"({t_build:.2f}s) + {sample} probe rows ({t_probe:.2f}s), extrapolated to {n_probe_full} rows",
-    }
+    return _cpu_line(n_probe_full / t_full, "probe rows/s",
+                     f"C restatement of join.rs:169 host join + aggregate.rs:154; full {len(bk)}-row build "
+                     f"({t_build:.2f}s) + {sample} probe rows ({t_probe:.2f}s), extrapolated to {n_probe_full} rows",
+                     t_both)
 
 
 def bench_join(args) -> dict:
@@ -167,6 +201,19 @@ def bench_join(args) -> dict:
             "bytes_per_launch": probe_bytes,
         },
     }
+    if not args.no_check:
+        import oracle as O
+
+        t0 = time.perf_counter()
+        want = O.filter_join_agg(bk, bp, fk, amt, 10.0)
+        rel = abs(res[1] - want["sum_amount"]) / max(abs(want["sum_amount"]), 1e-300)
+        line["check"] = {"ok": bool(want["status"] == 0 and res[0] == want["sum_payload"] and res[2] == want["count"]
+                                    and rel <= 1e-12),
+                         "oracle": "filter (compact.rs:146) -> host hash join (join.rs:169) -> SUM / SUM / COUNT "
+                                   "(aggregate.rs:154), full size",
+                         "sum_payload_exact": res[0] == want["sum_payload"], "count_exact": res[2] == want["count"],
+                         "oracle_s": time.perf_counter() - t0}
+        line["float_error"] = {"sum_amount_rel_err_vs_fp64_neumaier": rel, "tolerance": 1e-12}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline_join(bk, bp, fk, amt, npb, min(args.cpu_sample, npb))
     return line
@@ -473,7 +520,7 @@ def bench_calib(args) -> dict:
                  _roof(n * 8, ms_sum, "scalar_agg_kernel"))
 
 
-def bench_batch(args) -> dict:
+def bench_batch(args, keep: bool = False) -> dict:
     """512 independent GROUP BY k, SUM(v_i64), SUM(v_f32), COUNT(*) queries
     (rows log-uniform 1e4..4e6, Zipf(1.1) int32 keys over a log-uniform
     16..1e6 cardinality): one graph-captured persistent batch versus the same
@@ -554,7 +601,10 @@ def bench_batch(args) -> dict:
                   "speedup_vs_plain": plain_ms / wall_ms, "group_count_mismatches": mism,
                   "create_ms": create_ms, "tune": tune, "arena_bytes": arena},
                  _roof(algo, dev_ms, "whole batched pipeline (sketch, plan, aggregate, extract)"))
-    b.release()
+    if keep:
+        line["_batch"], line["_queries"] = b, queries
+    else:
+        b.release()
     return line
 
 
@@ -605,7 +655,7 @@ def _agree(a, b, tol=1e-9):
     return True
 
 
-def bench_serve(args) -> dict:
+def bench_serve(args, sweep_fracs=(0.25, 0.5, 0.75, 0.9), check: bool = False) -> dict:
     """Served form over a device-resident star schema: plain capacity (one
     query at a time, one launch per query) versus continuous batching with
     `slots` overlapping queries, plus a Poisson-arrival sweep with p50 / p99."""
@@ -621,10 +671,12 @@ def bench_serve(args) -> dict:
     f = D.star_fact(n)
     dts = [N.I32, N.I32, N.I32, N.DEC128, N.F64, N.DATE32]
     fact = [DeviceColumn.from_numpy(f[k], dt) for k, dt in zip(["fk1", "fk2", "fk3", "price", "qty", "date"], dts)]
+    host = [f[k] for k in ["fk1", "fk2", "fk3", "price", "qty", "date"]] if check else None
     del f
-    dims = []
+    dims, dims_h = [], []
     for d, nd in enumerate(D.STAR_DIMS):
         k, attrs = D.star_dim(d, nd)
+        dims_h.append((k, attrs))
         dims.append((DeviceColumn.from_numpy(k), [DeviceColumn.from_numpy(a) for a in attrs]))
     print(f"[bench] star schema {n} fact rows ready in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     nq = args.serve_queries
@@ -674,7 +726,7 @@ def bench_serve(args) -> dict:
 
     sweep = []
     rng = np.random.default_rng(D.SEED_SERVE + 7)
-    for frac in (0.25, 0.5, 0.75, 0.9):
+    for frac in sweep_fracs:
         qps = served_qps * frac
         queries = serve_mix(srv, nq, seed=D.SEED_SERVE + int(frac * 100))
         gaps = rng.exponential(1.0 / qps, size=nq)
@@ -717,16 +769,402 @@ def bench_serve(args) -> dict:
                   "errors": bad, "plain_fused_and_served_agree": bool(same), "server_stats": st},
                  _roof(fact_bytes, s_t * 1000.0 / max(laps, 1e-9),
                        "shared scan step kernel: fact bytes per lap / mean lap time at saturation"))
+    if check:
+        import oracle as O
+
+        specs = serve_specs(nq, seed=D.SEED_SERVE)[:3]  # one of each query template, as served above
+        bad, worst = [], 0.0
+        for i, spec in enumerate(specs):
+            g, want, sabs = star_reference(host, dims_h, spec)
+            got = s_res[i]
+            ok = got["status"] == 0 and got["n_groups"] == g and len(got["rows"]) == len(want)
+            nk = len(spec["group"])
+            for a, b in zip(got["rows"], want) if ok else []:
+                ok = ok and a[:nk] == b[:nk]
+                for x, y in zip(a[nk:], b[nk:]):
+                    if isinstance(y, float):
+                        worst = max(worst, abs(x - y) / max(abs(y), 1e-300))
+                        ok = ok and abs(x - y) <= 1e-12 * sabs + 1e-9 * abs(y)
+                    else:
+                        ok = ok and x == y
+            if not ok:
+                bad.append(i)
+        line["check"] = {"ok": not bad and bool(same), "queries_checked": len(specs), "mismatches": bad,
+                         "plain_fused_and_served_agree": bool(same),
+                         "oracle": "one query of each template at the full 200M-row fact table and 1M / 100K / 2.5K "
+                                   "dimensions against the oracle's filter, host joins, GROUP BY, ORDER BY, LIMIT "
+                                   "(keys, counts, Decimal128 sums exact; float sums within 1e-12 * sum(|x|))"}
+        line["float_error"] = {"sum_qty_max_rel_err_vs_fp64_neumaier": worst}
+        m = min(n, 10_000_000)
+        hs = [a[:m] for a in host]
+        t = _cpu_median(lambda: [star_reference(hs, dims_h, sp) for sp in specs])
+        t_q = t / len(specs) * (n / m)
+        line["cpu_baseline"] = _cpu_line(1.0 / t_q, "queries/s",
+                                         f"the oracle's composition (C filter, host joins, C GROUP BY fold, sort) of "
+                                         f"one query per template over the first {m} fact rows, scaled by rows to "
+                                         f"{n}", t)
+    return line
+
+
+# ---- sub-records of the default run ---------------------------------------------------------------
+
+def _sub(value, unit, ms, steps, dtype, config, roofline, check, float_error=None, cpu=None) -> dict:
+    return {"value": value, "unit": unit, "ms_per_step": ms, "steps": steps, "dtype": dtype, "config": config,
+            "roofline": roofline, "check": check, "float_error": float_error, "cpu_baseline": cpu}
+
+
+def _op_roof(bytes_per_call, op_ms, what, kernel_bytes=None, kernel_ms=None, kernel=None) -> dict:
+    """Operator-level roofline (every kernel of the call, on its stream), plus
+    the dominant kernel's where the library times it separately."""
+    r = _roof(bytes_per_call, op_ms, what)
+    r["level"] = "operator"
+    if kernel_ms:
+        k = _roof(kernel_bytes, kernel_ms, kernel)
+        r["dominant_kernel"] = {"kernel": kernel, "ms": kernel_ms, "achieved": k["achieved"], "frac": k["frac"],
+                                "bytes_per_launch": kernel_bytes}
+    return r
+
+
+def _time_groupby(dk, dv, aggs, flags, steps, warmup):
+    """(operator ms on the stream, dominant-kernel ms (LDS-tier pass), groups)."""
+    import ctypes as C
+
+    from craton_bolt_amd import _native as N
+    from craton_bolt_amd.device import Event, Stream, columns_array
+    from craton_bolt_amd.ops import _aggs
+
+    s = Stream()
+    ka, ia, ag = columns_array([dk]), columns_array([dv]), _aggs(aggs)
+    L = N.lib()
+    ev, kev = [Event(), Event()], [Event(), Event()]
+    groups = {}
+
+    def one(record):
+        tbl = C.c_void_p()
+        if record:
+            ev[0].record(s)
+        N.check(L.bolt_groupby(s.handle, ka, 1, ia, 1, ag, len(aggs), None, flags, C.byref(tbl)), "bolt_groupby")
+        if record:
+            ev[1].record(s)
+        nr = C.c_int64()
+        L.bolt_table_num_rows(tbl, C.byref(nr))
+        groups["g"] = nr.value
+        L.bolt_table_release(tbl)
+
+    for _ in range(warmup):
+        one(False)
+    s.synchronize()
+    op, km = [], []
+    L.bolt_profile_events(kev[0].handle, kev[1].handle)
+    for _ in range(steps):
+        one(True)
+        ev[1].synchronize()
+        op.append(ev[0].elapsed_ms(ev[1]))
+        km.append(kev[0].elapsed_ms(kev[1]))
+    L.bolt_profile_events(None, None)
+    return float(np.mean(op)), float(np.mean(km)), groups["g"]
+
+
+def sub_groupby_small(args, n: int) -> dict:
+    """Config 1: SELECT k, SUM(v), COUNT(*), MIN(v), MAX(v) GROUP BY k over n rows, output in key order."""
+    import oracle as O
+    from craton_bolt_amd import _native as N
+    from craton_bolt_amd import datagen as D
+    from craton_bolt_amd.device import DeviceColumn
+    from craton_bolt_amd.ops import execute_groupby
+
+    k, v = D.groupby_small(n)
+    dk, dv = DeviceColumn.from_numpy(k), DeviceColumn.from_numpy(v)
+    aggs = [(N.SUM, 0), (N.COUNT_STAR, 0), (N.MIN, 0), (N.MAX, 0)]
+    op_ms, k_ms, g = _time_groupby(dk, dv, aggs, N.GROUPBY_SORTED, args.steps, args.warmup)
+    check = None
+    if not args.no_check:
+        res = execute_groupby([dk], [dv], aggs)
+        want = O.groupby([O.HostCol(k, O.I32)], [O.HostCol(v, O.I64)], aggs, max_groups=4096)
+        keys = res["keys"][0][0].astype(np.uint32).astype(np.int64)
+        ok = want["status"] == O.OK and res["n_groups"] == want["n_groups"] and np.array_equal(keys, want["keys"])
+        for a in range(4):
+            ok = ok and np.array_equal(res["aggs"][a][0].astype(np.int64), want["i64"][a])
+        check = {"ok": bool(ok), "groups": int(res["n_groups"]),
+                 "oracle": "host GROUP BY fold (groupby.rs:295 classic semantics), full size, every group and aggregate"}
+    sample = min(n, 1 << 24)
+    ks, vs = [O.HostCol(k[:sample], O.I32)], [O.HostCol(v[:sample], O.I64)]
+    t = _cpu_median(lambda: O.groupby(ks, vs, aggs, max_groups=4096))
+    cpu = _cpu_line(sample / t, "rows/s", f"C restatement of the host GROUP BY fold (groupby_wide.rs:85 HashMap per "
+                                          f"group, checked i64 SUM), first {sample} rows", t)
+    byts = n * 12 + g * (4 + 8 + 8 + 8 + 8)
+    return _sub(n / (op_ms * 1e-3), "rows/s", op_ms, args.steps, "int64",
+                {"workload": f"plain GROUP BY: {n} rows, int32 key over 1000 values, int64 value; SUM, COUNT(*), "
+                             f"MIN, MAX; output in key order", "n": n, "groups": g},
+                _op_roof(byts, op_ms, "whole GROUP BY call (LDS tier, merge, extraction, key-order sort)",
+                         n * 12, k_ms, "generated LDS-tier pass (keys + values read once)"), check, None, cpu)
+
+
+def sub_sort(args) -> dict:
+    """Config 3, sort: 256M (int64 key, u32 row id) pairs."""
+    import oracle as O
+    from craton_bolt_amd import _native as N
+    from craton_bolt_amd import datagen as D
+    from craton_bolt_amd.device import DeviceBuffer, Event, Stream
+    from craton_bolt_amd.ops import sort_pairs_i64
+
+    n = args.n
+    key, rid = D.sort_pairs(n)
+    dk, dv = DeviceBuffer.from_numpy(key), DeviceBuffer.from_numpy(rid)
+    s = Stream()
+    kev = [Event(), Event()]
+    kms = []
+
+    def one():
+        sort_pairs_i64(dk, dv, n, stream=s)
+        kev[1].synchronize()
+        kms.append(kev[0].elapsed_ms(kev[1]))
+
+    N.lib().bolt_profile_events(kev[0].handle, kev[1].handle)
+    _timed(s, one, args.steps, args.warmup)
+    N.lib().bolt_profile_events(None, None)
+    ms = float(np.mean(kms[args.warmup:]))
+    check = None
+    if not args.no_check:
+        ko, vo = sort_pairs_i64(dk, dv, n, stream=s)
+        s.synchronize()
+        k2, r2 = ko.to_numpy(np.int64, n), vo.to_numpy(np.uint32, n)
+        del ko, vo
+        inc = bool(np.all((k2[1:] > k2[:-1]) | ((k2[1:] == k2[:-1]) & (r2[1:] > r2[:-1]))))
+        inr = bool(r2.max() < n) if n else True
+        same = bool(inr and np.array_equal(key[r2], k2))
+        check = {"ok": inc and same, "pairs_strictly_increasing": inc, "key_in_of_rid_equals_key_out": same,
+                 "oracle": "property at full size: (key, row id) strictly increasing (sorted and stable, as "
+                           "tests/sort_e2e.rs:57-93 checks order and permutation) and key_in[row id] == key_out"}
+        del k2, r2
+    sample = min(n, 1 << 22)
+    hc = [O.HostCol(key[:sample], O.I64)]
+    t = _cpu_median(lambda: O.sort_indices(hc, [False], [True]))
+    t_full = t * (n * np.log2(max(n, 2))) / (sample * np.log2(max(sample, 2)))
+    cpu = _cpu_line(n / t_full, "keys/s", f"C restatement of the host lexsort + take (sort.rs:222, a stable comparison "
+                                          f"sort of row ids): {sample} keys in {t:.3f} s, extrapolated by n log n "
+                                          f"to {n} keys", t)
+    return _sub(n / (ms * 1e-3), "keys/s", ms, args.steps, "int64",
+                {"workload": f"ORDER BY radix sort: {n} (int64 key uniform over the full range, u32 row id)",
+                 "n": n, "compulsory_bytes": n * 24, "model_bytes": n * 200},
+                _op_roof(n * 200, ms, "hybrid radix sort (histogram, 2 onesweep passes, LDS bucket sorts), "
+                                      "200 B/key pass model"), check, None, cpu)
+
+
+def sub_zipf(args) -> dict:
+    """Config 3, GROUP BY: 256M rows, int64 key Zipf(1.1) over 10M ranks, SUM(float64), COUNT(*)."""
+    import oracle as O
+    from craton_bolt_amd import _native as N
+    from craton_bolt_amd import datagen as D
+    from craton_bolt_amd.device import DeviceColumn
+    from craton_bolt_amd.ops import execute_groupby
+
+    n = args.n
+    k, v = D.zipf_groupby(n, n_ranks=10_000_000)
+    dk, dv = DeviceColumn.from_numpy(k), DeviceColumn.from_numpy(v)
+    aggs = [(N.SUM, 0), (N.COUNT_STAR, 0)]
+    op_ms, k_ms, g = _time_groupby(dk, dv, aggs, 0, args.steps, args.warmup)
+    check, ferr = None, None
+    if not args.no_check:
+        res = execute_groupby([dk], [dv], aggs, sorted_output=False)
+        gk, gs, gc = res["keys"][0][0], res["aggs"][0][0], res["aggs"][1][0]
+        # every group whose key has its low 6 bits clear (keys are scrambled ranks, so about 1/64 of the
+        # groups and rows), against the oracle run on exactly those rows
+        rows = (k & 63) == 0
+        ks, vs = k[rows], v[rows]
+        want = O.groupby([O.HostCol(ks, O.I64)], [O.HostCol(vs, O.F64)], aggs)
+        wabs = O.groupby([O.HostCol(ks, O.I64)], [O.HostCol(np.abs(vs), O.F64)], [(N.SUM, 0)])
+        sel = (gk & 63) == 0
+        order = np.argsort(gk[sel], kind="stable")
+        sk, ss, sc = gk[sel][order], gs[sel][order], gc[sel][order]
+        same_keys = want["status"] == O.OK and np.array_equal(sk, want["keys"])
+        cnt_ok = same_keys and np.array_equal(sc, want["i64"][1])
+        err = np.abs(ss - want["f64"][0]) if same_keys else np.array([np.inf])
+        scale = wabs["f64"][0] if same_keys else np.array([1.0])
+        within = bool(np.all(err <= 1e-12 * scale + 1e-300))
+        big = np.abs(want["f64"][0]) > 1e-6 * scale if same_keys else np.array([False])
+        total_gpu = float(np.sum(gs.astype(np.float64)))
+        total_want = O.neumaier_sum(v)
+        check = {"ok": bool(same_keys and cnt_ok and within and int(gc.sum()) == n), "groups": int(g),
+                 "checked_groups": int(sel.sum()), "checked_rows": int(rows.sum()),
+                 "count_sum_equals_rows": int(gc.sum()) == n, "keys_and_counts_exact": bool(cnt_ok),
+                 "oracle": "every group with key & 63 == 0 (about 1/64 of groups and rows) against the host GROUP "
+                           "BY fold of exactly those rows, full-size run; all counts sum to n"}
+        ferr = {"sum_max_abs_err_over_sum_abs": float(np.max(err / scale)) if len(err) else 0.0,
+                "sum_max_rel_err": float(np.max(err[big] / np.abs(want["f64"][0][big]))) if np.any(big) else 0.0,
+                "tolerance": "1e-12 * sum(|x|) per group",
+                "total_rel_err_vs_fp64_neumaier": abs(total_gpu - total_want) / max(abs(total_want), 1e-300)}
+    sample = min(n, 1 << 23)
+    kc, vc = [O.HostCol(k[:sample], O.I64)], [O.HostCol(v[:sample], O.F64)]
+    t = _cpu_median(lambda: O.groupby(kc, vc, aggs, max_groups=sample))
+    cpu = _cpu_line(sample / t, "rows/s", f"C restatement of the host GROUP BY fold (HashMap per group, sequential "
+                                          f"f64 SUM): first {sample} rows (a smaller group table than the full run)",
+                    t)
+    byts = n * 16 + g * (8 + 8 + 8)
+    return _sub(n / (op_ms * 1e-3), "rows/s", op_ms, args.steps, "f64",
+                {"workload": f"high-cardinality GROUP BY: {n} rows, int64 key Zipf(1.1) over 10M ranks, "
+                             f"SUM(float64), COUNT(*)", "n": n, "groups": g},
+                _op_roof(byts, op_ms, "whole GROUP BY call (LDS tier, partitioned tier, extraction)",
+                         n * 16, k_ms, "generated LDS-tier pass"), check, ferr, cpu)
+
+
+def sub_batch(args) -> dict:
+    """Config 4: the batched form, every query's values against the plain path, a sample against the oracle."""
+    import oracle as O
+    from craton_bolt_amd import _native as N
+    from craton_bolt_amd import datagen as D
+    from craton_bolt_amd.ops import execute_groupby
+
+    line = bench_batch(args, keep=True)
+    b, queries = line.pop("_batch"), line.pop("_queries")
+    aggs = [(N.SUM, 0), (N.SUM, 1), (N.COUNT_STAR, 0)]
+    check, ferr = None, None
+    if not args.no_check:
+        bad, worst = [], 0.0
+        for q, (kc, ic) in enumerate(queries):
+            rb = b.result(q)
+            rp = execute_groupby(kc, ic, aggs)
+            # both in ascending int32 key order (the plain path's packed-key order puts negative keys last)
+            o = np.argsort(rb["keys"][0][0], kind="stable")
+            op = np.argsort(rp["keys"][0][0], kind="stable")
+            ok = rb["n_groups"] == rp["n_groups"] and np.array_equal(rb["keys"][0][0][o], rp["keys"][0][0][op])
+            ok = ok and np.array_equal(rb["aggs"][0][0][o], rp["aggs"][0][0][op]) and \
+                np.array_equal(rb["aggs"][2][0][o], rp["aggs"][2][0][op])
+            if ok:
+                fb, fp = rb["aggs"][1][0][o].astype(np.float64), rp["aggs"][1][0][op].astype(np.float64)
+                d = np.abs(fb - fp) / np.maximum(np.abs(fp), 1e-30)
+                worst = max(worst, float(d.max()) if len(d) else 0.0)
+            if not ok:
+                bad.append(q)
+        # a sample of queries against the oracle (keys, integer sums, counts exact; the f32 sums' error)
+        shapes = D.batch_shapes(args.queries)
+        sample_q = list(range(0, args.queries, max(1, args.queries // 8)))[:8]
+        o_bad, o_err, rows_s, t_s = [], 0.0, 0, 0.0
+        for q in sample_q:
+            nq, card = shapes[q]
+            k, vi, vf = D.zipf_small(nq, card, seed=D.SEED_BATCH + q)
+            t0 = time.perf_counter()
+            want = O.groupby([O.HostCol(k, O.I32)], [O.HostCol(vi, O.I64), O.HostCol(vf, O.F32)], aggs,
+                             max_groups=card + 1)
+            t_s += time.perf_counter() - t0
+            rows_s += nq
+            rb = b.result(q)
+            o = np.argsort(rb["keys"][0][0], kind="stable")
+            wk = want["keys"].astype(np.uint32).view(np.int32)
+            wo = np.argsort(wk, kind="stable")
+            ok = np.array_equal(rb["keys"][0][0][o], wk[wo]) and np.array_equal(rb["aggs"][0][0][o], want["i64"][0][wo]) \
+                and np.array_equal(rb["aggs"][2][0][o], want["i64"][2][wo])
+            if ok and len(wo):
+                wf = want["f64"][1][wo]
+                o_err = max(o_err, float(np.max(np.abs(rb["aggs"][1][0][o].astype(np.float64) - wf) /
+                                                np.maximum(np.abs(wf), 1e-30))))
+            if not ok:
+                o_bad.append(q)
+        check = {"ok": not bad and not o_bad, "queries_equal_to_plain": args.queries - len(bad),
+                 "mismatched_queries": bad[:10], "oracle_sample_queries": sample_q, "oracle_mismatches": o_bad,
+                 "oracle": "all queries: groups, keys, SUM(int64), COUNT exact against the plain per-query "
+                           "operator; the sampled queries against the host GROUP BY fold"}
+        ferr = {"sum_f32_max_rel_diff_batch_vs_plain": worst,
+                "sum_f32_max_rel_err_vs_fp64_neumaier_sample": o_err,
+                "note": "SUM(Float32) is accumulated in f64 and rounded to f32 once"}
+        # the CPU baseline: the sampled queries' host GROUP BYs, scaled by rows to the whole batch
+        total_rows = line["config"]["total_rows"]
+        t_all = t_s * total_rows / max(rows_s, 1)
+        line["cpu_baseline"] = _cpu_line(args.queries / t_all, "queries/s",
+                                         f"C restatement of the host GROUP BY fold, one query after another: "
+                                         f"{len(sample_q)} sampled queries ({rows_s} rows) in {t_s:.2f} s (one run), "
+                                         f"scaled by rows to the {args.queries} queries", t_s)
+    b.release()
+    line["check"], line["float_error"] = check, ferr
+    return _sub(line["value"], line["unit"], line["ms_per_step"], args.steps, line["dtype"], line["config"],
+                line["roofline"], check, ferr, line.get("cpu_baseline"))
+
+
+def star_reference(host, dims_h, spec):
+    """The oracle's composition of a star query: filter, host joins against the dimension keys, GROUP BY,
+    then ORDER BY the chosen aggregate DESC, key ASC, LIMIT."""
+    import oracle as O
+    from craton_bolt_amd import _native as N
+
+    dts = [N.I32, N.I32, N.I32, N.DEC128, N.F64, N.DATE32]
+    odt = [{N.I32: O.I32, N.DEC128: O.DEC128, N.F64: O.F64, N.DATE32: O.DATE32}[dt] for dt in dts]
+    n = len(host[0])
+    hc = [O.HostCol(a, o) for a, o in zip(host, odt)]
+    rows = O.filter_rows(hc, [(t.col, t.op, t.value) for t in spec["terms"]], n) if spec["terms"] else np.arange(n)
+    drow = []
+    for fk, d in spec["joins"]:
+        key = dims_h[d][0]
+        order = np.argsort(key)
+        pos = np.minimum(np.searchsorted(key[order], host[fk][rows]), len(key) - 1)
+        hit = key[order][pos] == host[fk][rows]
+        rows = rows[hit]
+        drow = [r[hit] for r in drow] + [order[pos[hit]]]
+    gk = [dims_h[spec["joins"][j][1]][1][a][drow[j]] for j, a in spec["group"]]
+    inputs = [O.HostCol(np.ascontiguousarray(host[c][rows]), odt[c]) for c in range(len(host))]
+    want = O.groupby([O.HostCol(np.ascontiguousarray(x), O.I32) for x in gk], inputs, spec["aggs"],
+                     max_groups=len(rows) + 1)
+    keys = want["keys"]
+    kcols = [keys.astype(np.uint32).view(np.int32).astype(np.int64)] if len(gk) == 1 else \
+        [(keys >> 32).astype(np.int64), (keys & 0xFFFFFFFF).astype(np.int64)]
+    vals = []
+    for a, (op, col) in enumerate(spec["aggs"]):
+        if op in (N.COUNT_STAR, N.COUNT):
+            vals.append([int(x) for x in want["i64"][a]])
+        elif dts[col] == N.DEC128:
+            vals.append([(int(h) << 64) | (int(l) & 0xFFFFFFFFFFFFFFFF) for l, h in zip(want["i64"][a], want["hi"][a])])
+        else:
+            vals.append([float(x) for x in want["f64"][a]])
+    out = [tuple(int(k[i]) for k in kcols) + tuple(v[i] for v in vals) for i in range(want["n_groups"])]
+    oa = len(kcols) + spec["order_agg"]
+    out.sort(key=lambda r: tuple(r[:len(kcols)]))
+    out.sort(key=lambda r: r[oa], reverse=True)
+    return want["n_groups"], out[: spec["limit"]], float(np.abs(host[4][rows]).sum()) if len(rows) else 0.0
+
+
+def sub_serve(args) -> dict:
+    """Config 5, trimmed to fit the default run: 32 queries at saturation, a two-point Poisson sweep."""
+    line = bench_serve(args, sweep_fracs=(0.5, 0.9), check=not args.no_check)
+    return _sub(line["value"], line["unit"], line["ms_per_step"], args.serve_queries, line["dtype"], line["config"],
+                line["roofline"], line.get("check"), line.get("float_error"), line.get("cpu_baseline"))
+
+
+def bench_all(args) -> dict:
+    """The headline line (config 2) with every other configuration as a sub-record."""
+    line = bench_join(args)
+    subs = {}
+    steps, warmup = args.steps, args.warmup
+    plan = [("groupby_1M", lambda: sub_groupby_small(args, 1 << 20)),
+            ("groupby_256M", lambda: sub_groupby_small(args, args.n)),
+            ("sort_256M", lambda: sub_sort(args)),
+            ("groupby_zipf_256M", lambda: sub_zipf(args)),
+            ("batch_512", lambda: sub_batch(args)),
+            ("serve_200M", lambda: sub_serve(args))]
+    for name, fn in plan:
+        t0 = time.perf_counter()
+        print(f"[bench] sub-record {name} ...", file=sys.stderr, flush=True)
+        args.steps, args.warmup = min(steps, 10), min(warmup, 2)
+        try:
+            subs[name] = fn()
+        except Exception as e:  # a failing configuration is reported, not hidden
+            subs[name] = {"error": f"{type(e).__name__}: {e}"}
+        args.steps, args.warmup = steps, warmup
+        subs[name]["bench_s"] = time.perf_counter() - t0
+        print(f"[bench] sub-record {name} done in {subs[name]['bench_s']:.1f}s", file=sys.stderr, flush=True)
+    line["sub_records"] = subs
     return line
 
 
 def main():
     args = _parse()
+    if args.workload == "all" and args.serve_queries == 64:
+        args.serve_queries = 32
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each box has one MI355X); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    if args.workload == "join":
+    if args.workload == "all":
+        line = bench_all(args)
+    elif args.workload == "join":
         line = bench_join(args)
     elif args.workload == "sort":
         line = bench_sort(args)

@@ -174,6 +174,7 @@ class GroupByBatch:
         self.n_inputs = len(queries[0][1])
         self.n_aggs = len(aggs)
         self._cols = queries  # keep the device columns alive
+        self._stream = stream  # and the stream: the batch runs on it and frees its buffers on it
         ka = (N.BoltColumn * (self.n_queries * self.n_keys))()
         ia = (N.BoltColumn * max(1, self.n_queries * self.n_inputs))()
         for q, (ks, ins) in enumerate(queries):

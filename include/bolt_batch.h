@@ -52,7 +52,9 @@ typedef struct bolt_gb_batch bolt_gb_batch_t;
 /* Query q (0 <= q < n_queries) groups keys[q * n_keys .. + n_keys) and
  * aggregates inputs[q * n_inputs .. + n_inputs) with the shared aggregate list
  * `aggs` (column indices are per query).  Every query must use the same key /
- * input dtypes and nullability.  `tune` may be NULL. */
+ * input dtypes and nullability.  `tune` may be NULL.  A non-NULL `stream`
+ * must outlive the batch: every run is queued on it and release frees the
+ * batch's device buffers in its order (the NULL stream gets a private one). */
 int bolt_gb_batch_create(bolt_stream_t stream, const bolt_column_t* keys, int32_t n_keys,
                          const bolt_column_t* inputs, int32_t n_inputs, int32_t n_queries,
                          const bolt_agg_t* aggs, int32_t n_aggs, const bolt_gb_tune_t* tune,

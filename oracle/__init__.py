@@ -18,7 +18,7 @@ LIB_PATH = HERE / "build" / "libbolt_oracle.so"
 BOOL, I32, I64, F32, F64, DEC128, DATE32 = 1, 2, 3, 4, 5, 6, 7
 SUM, COUNT, COUNT_STAR, MIN, MAX, AVG, VAR_POP, VAR_SAMP, STDDEV_POP, STDDEV_SAMP = range(1, 11)
 LT, LE, GT, GE, EQ, NE, IS_NULL, IS_NOT_NULL = range(1, 9)
-OK, ERR_PLAN, ERR_TYPE = 0, 1, 2
+OK, ERR_PLAN, ERR_TYPE, ERR_CAP = 0, 1, 2, 3
 
 
 class Col(C.Structure):
@@ -68,7 +68,7 @@ def lib() -> C.CDLL:
         L.bo_groupby.argtypes = [C.POINTER(Col), C.c_int32, C.POINTER(Col), C.POINTER(C.c_int32),
                                  C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int64), C.c_void_p, C.c_void_p,
                                  C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
-                                 C.POINTER(C.c_void_p)]
+                                 C.POINTER(C.c_void_p), C.c_int64]
         L.bo_inner_join.restype = C.c_int64
         L.bo_inner_join.argtypes = [C.POINTER(Col), C.POINTER(Col), C.c_void_p, C.c_void_p, C.c_int64,
                                     C.POINTER(C.c_int64)]
@@ -176,10 +176,12 @@ def filter_rows(cols: list[HostCol], terms: list[tuple], n: int) -> np.ndarray:
     return out[:k]
 
 
-def groupby(keys: list[HostCol], inputs: list[HostCol], aggs: list[tuple[int, int]]):
-    """src/exec/groupby.rs:295 classic semantics. Returns dict or raises on error status."""
+def groupby(keys: list[HostCol], inputs: list[HostCol], aggs: list[tuple[int, int]], max_groups: int | None = None):
+    """src/exec/groupby.rs:295 classic semantics. Returns dict or raises on error status.
+    Outputs are sized for `max_groups` groups (default: one per row); more
+    groups than that costs one more fold with outputs of the exact size."""
     n = keys[0].col.n
-    cap = max(n, 1)
+    cap = max(min(n, max_groups) if max_groups else n, 1)
     na = len(aggs)
     ops = (C.c_int32 * max(na, 1))(*[a[0] for a in aggs])
     colsi = (C.c_int32 * max(na, 1))(*[a[1] for a in aggs])
@@ -198,7 +200,9 @@ def groupby(keys: list[HostCol], inputs: list[HostCol], aggs: list[tuple[int, in
 
     ng = C.c_int64()
     st = lib().bo_groupby(_cols(keys), len(keys), _cols(inputs), ops, colsi, na, C.byref(ng), out_keys.ctypes.data,
-                          out_kn.ctypes.data, ptrs(oi), ptrs(of), ptrs(oh), ptrs(on))
+                          out_kn.ctypes.data, ptrs(oi), ptrs(of), ptrs(oh), ptrs(on), cap)
+    if st == ERR_CAP:
+        return groupby(keys, inputs, aggs, max_groups=ng.value)
     if st != OK:
         return {"status": st}
     g = ng.value

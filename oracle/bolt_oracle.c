@@ -342,6 +342,26 @@ static void hmap_init(hmap_t* m, int64_t n) {
   m->mask = cap - 1;
 }
 static void hmap_free(hmap_t* m) { free(m->keys); free(m->vals); free(m->used); }
+/* doubles the table and re-inserts every key (the groups' map grows with
+ * the groups seen, not with the rows) */
+static void hmap_grow(hmap_t* m) {
+  hmap_t g;
+  uint64_t cap = (m->mask + 1) * 2;
+  g.keys = (int64_t*)malloc(cap * 8);
+  g.vals = (int64_t*)malloc(cap * 8);
+  g.used = (uint8_t*)calloc(cap, 1);
+  g.mask = cap - 1;
+  for (uint64_t s = 0; s <= m->mask; s++) {
+    if (!m->used[s]) continue;
+    uint64_t t = mix64((uint64_t)m->keys[s]) & g.mask;
+    while (g.used[t]) t = (t + 1) & g.mask;
+    g.used[t] = 1;
+    g.keys[t] = m->keys[s];
+    g.vals[t] = m->vals[s];
+  }
+  hmap_free(m);
+  *m = g;
+}
 /* returns the slot for key; *found set when already present */
 static uint64_t hmap_slot(hmap_t* m, int64_t key, int* found) {
   uint64_t s = mix64((uint64_t)key) & m->mask;
@@ -400,7 +420,7 @@ static int cmp_i64(const void* a, const void* b) {
 int32_t bo_groupby(const bo_col* keys, int32_t n_keys, const bo_col* inputs, const int32_t* agg_ops,
                    const int32_t* agg_cols, int32_t n_aggs, int64_t* n_groups, int64_t* out_keys,
                    uint8_t* out_key_null, int64_t** out_i64, double** out_f64, int64_t** out_hi,
-                   uint8_t** out_null) {
+                   uint8_t** out_null, int64_t out_cap) {
   if (n_keys < 1 || n_keys > 2) return BO_ERR_PLAN;
   int64_t n = keys[0].n;
   if (n_keys == 2) {
@@ -428,10 +448,14 @@ int32_t bo_groupby(const bo_col* keys, int32_t n_keys, const bo_col* inputs, con
     for (int64_t i = 0; i < n; i++)
       if (keep[i] && packed[i] == NULL_GROUP_KEY) { synth_null = 0; break; }
   }
+  /* accumulators, keys and the map are sized by the groups seen so far
+   * (doubling), so memory follows the group count, not the row count */
+  const size_t na = (size_t)(n_aggs > 0 ? n_aggs : 1);
+  int64_t gcap = 1024;
   hmap_t m;
-  hmap_init(&m, n > 0 ? n : 1);
-  gacc_t* acc = (gacc_t*)calloc((size_t)(n > 0 ? n : 1) * (size_t)(n_aggs > 0 ? n_aggs : 1), sizeof(gacc_t));
-  int64_t* gkey = (int64_t*)malloc((size_t)(n > 0 ? n : 1) * 8);
+  hmap_init(&m, gcap);
+  gacc_t* acc = (gacc_t*)calloc((size_t)gcap * na, sizeof(gacc_t));
+  int64_t* gkey = (int64_t*)malloc((size_t)gcap * 8);
   int64_t ng = 0;
   int32_t status = BO_OK;
   for (int64_t i = 0; i < n && status == BO_OK; i++) {
@@ -443,7 +467,19 @@ int32_t bo_groupby(const bo_col* keys, int32_t n_keys, const bo_col* inputs, con
     uint64_t s = hmap_slot(&m, key, &found);
     int64_t g;
     if (found) g = m.vals[s];
-    else { m.used[s] = 1; m.keys[s] = key; m.vals[s] = ng; gkey[ng] = key; g = ng++; }
+    else {
+      if (ng == gcap) {
+        acc = (gacc_t*)realloc(acc, (size_t)gcap * 2 * na * sizeof(gacc_t));
+        memset(acc + (size_t)gcap * na, 0, (size_t)gcap * na * sizeof(gacc_t));
+        gkey = (int64_t*)realloc(gkey, (size_t)gcap * 2 * 8);
+        gcap *= 2;
+      }
+      if ((uint64_t)(ng + 1) * 2 > m.mask + 1) {
+        hmap_grow(&m);
+        s = hmap_slot(&m, key, &found);
+      }
+      m.used[s] = 1; m.keys[s] = key; m.vals[s] = ng; gkey[ng] = key; g = ng++;
+    }
     for (int32_t a = 0; a < n_aggs; a++) {
       gacc_t* A = &acc[g * n_aggs + a];
       int32_t op = agg_ops[a];
@@ -486,6 +522,11 @@ int32_t bo_groupby(const bo_col* keys, int32_t n_keys, const bo_col* inputs, con
     }
   }
   if (status != BO_OK) goto done;
+  if (ng > out_cap) {
+    *n_groups = ng;
+    status = BO_ERR_CAP;
+    goto done;
+  }
   /* sort groups by packed key (groupby.rs:800) */
   int64_t* order = (int64_t*)malloc((size_t)(ng > 0 ? ng : 1) * 16);
   for (int64_t g = 0; g < ng; g++) { order[2 * g] = gkey[g]; order[2 * g + 1] = g; }

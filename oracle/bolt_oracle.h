@@ -30,7 +30,7 @@ enum {
 /* comparison codes for predicate terms (same values as bolt_cmp_op) */
 enum { BO_LT = 1, BO_LE = 2, BO_GT = 3, BO_GE = 4, BO_EQ = 5, BO_NE = 6, BO_IS_NULL = 7, BO_IS_NOT_NULL = 8 };
 /* status codes (same values as the product ABI) */
-enum { BO_OK = 0, BO_ERR_PLAN = 1, BO_ERR_TYPE = 2 };
+enum { BO_OK = 0, BO_ERR_PLAN = 1, BO_ERR_TYPE = 2, BO_ERR_CAP = 3 };
 
 /* A host column: Arrow-style values buffer plus optional LSB-first validity
  * bitmap (NULL = all valid).  Decimal128 values are 16-byte little-endian. */
@@ -91,12 +91,14 @@ int64_t bo_filter_rows(const bo_col* cols, int32_t n_cols, const bo_term* terms,
  *   out_keys[g]       packed i64 key, out_key_null[g] 1 for the NULL group;
  *   out_i64[a][g] / out_f64[a][g] / out_hi[a][g] per aggregate a (integer,
  *   float and decimal-high results), out_null[a][g] 1 for an SQL NULL.
- * Capacity of every output array must be >= number of rows.
- * Returns BO_OK and sets *n_groups, or BO_ERR_TYPE on integer SUM overflow. */
+ * Every output array holds out_cap groups.
+ * Returns BO_OK and sets *n_groups, BO_ERR_TYPE on integer SUM overflow, or
+ * BO_ERR_CAP (nothing written, *n_groups = the group count) when the groups
+ * exceed out_cap. */
 int32_t bo_groupby(const bo_col* keys, int32_t n_keys, const bo_col* inputs, const int32_t* agg_ops,
                    const int32_t* agg_cols, int32_t n_aggs, int64_t* n_groups, int64_t* out_keys,
                    uint8_t* out_key_null, int64_t** out_i64, double** out_f64, int64_t** out_hi,
-                   uint8_t** out_null);
+                   uint8_t** out_null, int64_t out_cap);
 
 /* ---- INNER hash join (src/exec/join.rs:169-270, build_hash_map :844) ----
  * Smaller side builds (left when n_left <= n_right); NULL keys never match.

@@ -264,6 +264,30 @@ def test_groupby_matches_pandas_bruteforce():
     assert np.allclose(r["f64"][9], g.f.var(ddof=1).reindex(ks).to_numpy(), rtol=1e-9)
 
 
+def test_groupby_growth_and_output_capacity():
+    """Many groups (the per-group tables grow many times) against pandas, and
+    the same result whether the outputs are sized for every row, for too few
+    groups (one more fold at the exact size) or exactly."""
+    rng = np.random.default_rng(23)
+    n = 300_000
+    k = rng.integers(-2 ** 40, 2 ** 40, size=n).astype(np.int64) // 7
+    k[::3] = k[1::3][: len(k[::3])]  # some repeats
+    v = rng.integers(-1000, 1000, size=n).astype(np.int64)
+    aggs = [(O.SUM, 0), (O.COUNT_STAR, 0), (O.MAX, 0)]
+    full = O.groupby([O.HostCol(k, O.I64)], [O.HostCol(v, O.I64)], aggs)
+    df = pd.DataFrame({"k": k, "v": v}).groupby("k").v
+    ks = np.sort(np.unique(k))
+    assert np.array_equal(full["keys"], ks)
+    assert np.array_equal(full["i64"][0], df.sum().reindex(ks).to_numpy())
+    assert np.array_equal(full["i64"][1], df.size().reindex(ks).to_numpy())
+    assert np.array_equal(full["i64"][2], df.max().reindex(ks).to_numpy())
+    for cap in (10, len(ks)):
+        r = O.groupby([O.HostCol(k, O.I64)], [O.HostCol(v, O.I64)], aggs, max_groups=cap)
+        assert r["n_groups"] == len(ks)
+        assert np.array_equal(r["keys"], full["keys"])
+        assert all(np.array_equal(a, b) for a, b in zip(r["i64"], full["i64"]))
+
+
 def test_groupby_null_group_single_column():
     # groupby.rs:528-606: all NULL-key rows form one group, key NULL, sorted last
     k = np.array([1, 2, 0, 1, 0], np.int64)
