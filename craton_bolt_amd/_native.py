@@ -262,6 +262,8 @@ _SIGS = {
     "bolt_groupby": (C.c_int, [C.c_void_p, C.POINTER(BoltColumn), C.c_int32, C.POINTER(BoltColumn), C.c_int32,
                                C.POINTER(BoltAgg), C.c_int32, C.POINTER(BoltFilter), C.c_uint32,
                                C.POINTER(C.c_void_p)]),
+    "bolt_groupby_last_plan": (C.c_int, [C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                         C.POINTER(C.c_int32)]),
     "bolt_table_num_rows": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "bolt_table_num_columns": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "bolt_table_column": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(BoltColumn)]),

@@ -312,6 +312,15 @@ __global__ __launch_bounds__(kExBlock) void gb_extract_kernel(GTable t, GPlan P,
   }
 }
 
+// err[0] = 1 when a valid Decimal128 key does not fit i64 (its high word is
+// not the sign extension of its low word).
+__global__ void dec_key_fit_kernel(const uint64_t* __restrict__ d, const uint8_t* __restrict__ valid, int64_t n,
+                                   uint32_t* __restrict__ err) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    if (bit_valid(valid, i) && d[2 * i + 1] != (uint64_t)((int64_t)d[2 * i] >> 63)) *err = 1u;
+}
+
 // The reference synthesises the NULL group of a single nullable key column
 // under the packed key i64::MAX; when a genuine key equal to i64::MAX exists
 // it runs its valid-flag executor instead, which drops the NULL-key rows
@@ -350,6 +359,14 @@ __global__ void gb_move_row_kernel(MoveCols mc, unsigned long long* __restrict__
     else atomicAnd(&v[dst >> 5], ~bit);
   }
 }
+
+// The calling thread's last bolt_groupby plan (bolt_groupby_last_plan).
+struct GbLastPlan {
+  int32_t tier;  // 0 LDS tier only, 1 global table, 2 partitioned tier
+  int64_t spilled, partials;
+  int32_t partitions;
+};
+static thread_local GbLastPlan t_last_plan{0, 0, 0, 0};
 
 // Gather for the sorted output order: out[i] = in[perm[i]].
 __global__ void gather_kernel(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, int width,
@@ -643,8 +660,11 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
   ks.n_keys = n_keys;
   for (int k = 0; k < n_keys; k++) {
     int dt = keys[k].dtype;
-    if (!(dt == BOLT_I32 || dt == BOLT_I64 || dt == BOLT_F32 || dt == BOLT_F64 || dt == BOLT_DATE32))
-      // src/exec/groupby_common.rs:117 key_bit_width
+    // src/exec/groupby_common.rs:117 key_bit_width accepts Int32 / Int64 /
+    // floats; Date32 packs like Int32, and a single Decimal128 key whose
+    // values fit i64 packs like Int64 (wider values decline)
+    if (!(dt == BOLT_I32 || dt == BOLT_I64 || dt == BOLT_F32 || dt == BOLT_F64 || dt == BOLT_DATE32 ||
+          (dt == BOLT_DEC128 && n_keys == 1)))
       return set_error(BOLT_ERR_TYPE, "GROUP BY key dtype %s not supported", dtype_name(dt));
     if (keys[k].length != n) return set_error(BOLT_ERR_PLAN, "bolt_groupby: key columns differ in length");
     if (n_keys == 2 && dtype_size(dt) != 4)
@@ -665,6 +685,19 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
   BOLT_TRY(make_gplan(aggs, n_aggs, inputs, n_inputs, n, &P, out_dt));
   DevFilter f;
   BOLT_TRY(make_dev_filter(filter, inputs, n_inputs, n, &f));
+  if (keys[0].dtype == BOLT_DEC128 && n > 0) {
+    DevBuf<uint32_t> err;
+    uint32_t h = 0;
+    BOLT_TRY(err.alloc(1, s));
+    BOLT_TRY(fill_u32(err.ptr, 0, 1, s));
+    dec_key_fit_kernel<<<grid_for(n, 256, 4), 256, 0, s>>>(static_cast<const uint64_t*>(keys[0].data),
+                                                          keys[0].validity, n, err.ptr);
+    BOLT_HIP(hipGetLastError());
+    BOLT_HIP(hipMemcpyAsync(&h, err.ptr, 4, hipMemcpyDeviceToHost, s));
+    BOLT_HIP(hipStreamSynchronize(s));
+    if (h)
+      return set_error(BOLT_DECLINE, "GROUP BY on a Decimal128 key with values beyond 64 bits (the host path takes it)");
+  }
 
   // ---- pass A ----
   const int W = P.words;
@@ -760,6 +793,8 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
   if (partitioned)
     BOLT_TRY(gb_part_build(s, ks, P, pkeys.ptr, ptag.ptr, pwords.ptr, (int64_t)pkeys.count, n_parts, spill.ptr,
                            spill_per, spill_seg.ptr, (int)grid, n_spill, est, S, &parts));
+  t_last_plan = GbLastPlan{!use_lds ? 1 : (partitioned ? 2 : (n_spill > 0 ? 1 : 0)), n_spill, n_parts,
+                           partitioned ? parts.np : 0};
   GTable t;
   DevBuf<unsigned long long> tkeys, twords;
   DevBuf<uint32_t> tmisc;
@@ -817,6 +852,7 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
     memset(&o, 0, sizeof o);
     for (int k = 0; k < n_keys && st == BOLT_OK; k++) {
       st = table_alloc_col(tbl, keys[k].dtype, out_cap, k == 0 && ks.null_group, s, &cols[k]);
+      cols[k].scale = keys[k].scale;
       o.key[k] = OutCol{static_cast<uint8_t*>(cols[k].data), (uint32_t*)cols[k].validity, keys[k].dtype, 0};
     }
     for (int a = 0; a < n_aggs && st == BOLT_OK; a++) {
@@ -884,7 +920,7 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
     break;
   }
 
-  if (ks.null_group && keys[0].dtype == BOLT_I64 && n_groups >= 2) {
+  if (ks.null_group && (keys[0].dtype == BOLT_I64 || keys[0].dtype == BOLT_DEC128) && n_groups >= 2) {
     DevBuf<int64_t> clash;
     int64_t hc[2] = {-1, 0};
     st = clash.alloc(2, s);
@@ -954,5 +990,14 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
   tbl->cols = cols;
   tbl->n_rows = n_groups;
   *out = tbl;
+  return BOLT_OK;
+}
+
+extern "C" int bolt_groupby_last_plan(int32_t* tier, int64_t* spilled_rows, int64_t* partial_records,
+                                      int32_t* partitions) {
+  if (tier) *tier = bolt::t_last_plan.tier;
+  if (spilled_rows) *spilled_rows = bolt::t_last_plan.spilled;
+  if (partial_records) *partial_records = bolt::t_last_plan.partials;
+  if (partitions) *partitions = bolt::t_last_plan.partitions;
   return BOLT_OK;
 }

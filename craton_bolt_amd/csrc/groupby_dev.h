@@ -46,6 +46,9 @@ __device__ __forceinline__ uint64_t key_bits(const void* d, int dt, int64_t row)
   switch (dt) {
     case BOLT_I32: case BOLT_DATE32: return (uint64_t)static_cast<const uint32_t*>(d)[row];
     case BOLT_I64: return static_cast<const uint64_t*>(d)[row];
+    // a Decimal128 key whose values all fit i64 (checked before the operator
+    // runs) packs as its low word, like an Int64 key
+    case BOLT_DEC128: return static_cast<const uint64_t*>(d)[2 * row];
     case BOLT_F32: return canon_f32_bits(static_cast<const uint32_t*>(d)[row]);
     case BOLT_F64: return canon_f64_bits(static_cast<const uint64_t*>(d)[row]);
   }
@@ -264,8 +267,14 @@ struct OutSpec {
 __device__ __forceinline__ void set_valid(uint32_t* bm, uint64_t pos) { atomicOr(&bm[pos >> 5], 1u << (pos & 31)); }
 
 __device__ __forceinline__ void write_key(const OutCol& c, uint64_t pos, uint64_t bits) {
-  if (c.dtype == BOLT_I64 || c.dtype == BOLT_F64) reinterpret_cast<uint64_t*>(c.data)[pos] = bits;
-  else reinterpret_cast<uint32_t*>(c.data)[pos] = (uint32_t)bits;
+  if (c.dtype == BOLT_DEC128) {  // the low word, sign-extended
+    reinterpret_cast<uint64_t*>(c.data)[2 * pos] = bits;
+    reinterpret_cast<uint64_t*>(c.data)[2 * pos + 1] = (uint64_t)((int64_t)bits >> 63);
+  } else if (c.dtype == BOLT_I64 || c.dtype == BOLT_F64) {
+    reinterpret_cast<uint64_t*>(c.data)[pos] = bits;
+  } else {
+    reinterpret_cast<uint32_t*>(c.data)[pos] = (uint32_t)bits;
+  }
 }
 
 __device__ __forceinline__ void finalize_to(const GTable& t, const GAgg& g, const OutCol& c, size_t total, uint64_t s,

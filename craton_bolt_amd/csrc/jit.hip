@@ -530,7 +530,8 @@ std::string jit_gb_lds_source(const GbJitSpec& sp) {
     o << "    const long long " << i << " = base + " << 1024 * r << ";\n"
       << "    const bool in" << rs << " = " << i << " < n;\n"
       << "    const long long " << j << " = in" << rs << " ? " << i << " : n - 1;  // clamped: loads stay branch-free\n"
-      << "    const " << kty(sp.t0) << " ka" << rs << " = __builtin_nontemporal_load(k0 + " << j << ");\n";
+      << "    const " << kty(sp.t0) << " ka" << rs << " = __builtin_nontemporal_load(k0 + "
+      << (sp.t0 == BOLT_DEC128 ? "2 * " + j : j) << ");  // a Decimal128 key: its low word\n";
     if (sp.n_keys == 2) o << "    const " << kty(sp.t1) << " kb" << rs << " = __builtin_nontemporal_load(k1 + " << j << ");\n";
     for (int c = 0; c < (int)sp.cols.size(); c++) {
       const std::string x = "x" + std::to_string(c) + "_" + rs;

@@ -179,7 +179,10 @@ int bolt_take(bolt_stream_t stream, const bolt_column_t* src, const uint32_t* ro
  * Hash GROUP BY — replaces execute_groupby (src/exec/groupby.rs:295) and its
  * Tier-1 / Tier-2 / classic cascade.  One or two key columns (Int32 / Int64 /
  * Float32 / Float64 / Date32; two only when both are 32-bit, the packing of
- * src/exec/groupby_common.rs:352), any number of aggregates.  A single-column
+ * src/exec/groupby_common.rs:352), or one Decimal128 key, grouped by its exact
+ * value (key_bit_width rejects it, src/exec/groupby_common.rs:117; here values
+ * that fit 64 bits group on the GPU and a wider value returns BOLT_DECLINE),
+ * any number of aggregates.  A single-column
  * key with NULLs forms one NULL group (src/exec/groupby.rs:528-606).
  * Rows that fail `filter` are skipped.  The result is a library-owned table:
  * n_keys key columns then one column per aggregate, groups in ascending
@@ -195,6 +198,13 @@ typedef struct bolt_table bolt_table_t;
 int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int32_t n_keys,
                  const bolt_column_t* inputs, int32_t n_inputs, const bolt_agg_t* aggs,
                  int32_t n_aggs, const bolt_filter_t* filter, uint32_t flags, bolt_table_t** out);
+/* How the calling thread's last bolt_groupby ran (no reference counterpart;
+ * the reference logs its tier choice, src/exec/groupby.rs:355-454): tier 0 =
+ * the LDS tier's partial records merged into the table, 1 = rows also spilled
+ * to the global table with atomics, 2 = the radix-partitioned tier; the rows
+ * the LDS tier spilled, its partial records, and the partitions (tier 2).
+ * Any pointer may be NULL. */
+int bolt_groupby_last_plan(int32_t* tier, int64_t* spilled_rows, int64_t* partial_records, int32_t* partitions);
 int bolt_table_num_rows(const bolt_table_t* t, int64_t* n_rows);
 int bolt_table_num_columns(const bolt_table_t* t, int32_t* n_cols);
 /* Column i of a result table (device pointers owned by the table). */

@@ -386,6 +386,8 @@ static uint64_t key_bits(const bo_col* c, int64_t i) {
                                     are packed like Int32 */
       return (uint64_t)(uint32_t)((const int32_t*)c->data)[i];
     case BO_I64: return (uint64_t)((const int64_t*)c->data)[i];
+    case BO_DEC128: /* a Decimal128 key that fits i64 (checked by the caller) packs as its low word */
+      return (uint64_t)((const int64_t*)c->data)[2 * i];
     case BO_F32: {
       float f = ((const float*)c->data)[i];
       uint32_t b;
@@ -423,6 +425,17 @@ int32_t bo_groupby(const bo_col* keys, int32_t n_keys, const bo_col* inputs, con
                    uint8_t** out_null, int64_t out_cap) {
   if (n_keys < 1 || n_keys > 2) return BO_ERR_PLAN;
   int64_t n = keys[0].n;
+  /* Decimal128 keys: one column, every valid value within i64 (the GPU
+   * operator declines wider ones; the reference rejects Decimal keys,
+   * groupby_common.rs:117) */
+  if (keys[0].dtype == BO_DEC128) {
+    if (n_keys != 1) return BO_ERR_PLAN;
+    for (int64_t i = 0; i < n; i++)
+      if (is_valid(&keys[0], i)) {
+        const int64_t* w = (const int64_t*)keys[0].data + 2 * i;
+        if (w[1] != (w[0] >> 63)) return BO_ERR_PLAN;
+      }
+  }
   if (n_keys == 2) {
     int w0 = (keys[0].dtype == BO_I32 || keys[0].dtype == BO_F32);
     int w1 = (keys[1].dtype == BO_I32 || keys[1].dtype == BO_F32);

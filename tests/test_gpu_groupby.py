@@ -14,6 +14,7 @@ import pytest
 import oracle as O
 from craton_bolt_amd import _native as N
 from craton_bolt_amd import datagen as D
+from craton_bolt_amd.device import DeviceColumn
 from craton_bolt_amd.ops import Term, execute_groupby
 
 from gpu_util import both, dec_array
@@ -299,3 +300,40 @@ def test_gpu_h2o_queries_match_oracle(gpu, query):
         g = got["aggs"][a][0]
         w = want["f64"][a]
         assert np.allclose(g, w, rtol=1e-12, atol=0), query
+
+
+@pytest.mark.parametrize("n", [40_000, 2_000_000])
+@pytest.mark.parametrize("flags", [0, N.GROUPBY_FORCE_GLOBAL, N.GROUPBY_FORCE_PARTITIONED])
+def test_decimal128_keys(gpu, n, flags):
+    """GROUP BY a Decimal128(12, 2) key (the served fact table's price type):
+    grouped by its exact value, in value order, NULL group last, output key
+    column Decimal128 with the input's scale; integer / decimal sums and
+    counts exact.  A value beyond 64 bits declines (the host path takes it)."""
+    print(f"[case] decimal keys n={n} flags={flags}", flush=True)
+    rng = np.random.default_rng(n + flags)
+    vals = [int(x) for x in rng.integers(-5000, 5000, n) * 100 + rng.integers(0, 3, n)]
+    valid = rng.random(n) > 0.05
+    v = rng.integers(-1000, 1000, n).astype(np.int64)
+    price = dec_array([int(x) * 10 ** 9 for x in rng.integers(-10 ** 6, 10 ** 6, n)])
+    dk = DeviceColumn.from_numpy(dec_array(vals), N.DEC128, valid=valid, scale=2)
+    aggs = [(N.SUM, 0), (N.COUNT_STAR, 0), (N.SUM, 1), (N.MAX, 0)]
+    got = execute_groupby([dk], [DeviceColumn.from_numpy(v), DeviceColumn.from_numpy(price, N.DEC128)], aggs,
+                          flags=flags)
+    want = O.groupby([O.HostCol(dec_array(vals), O.DEC128, valid)], [O.HostCol(v, O.I64), O.HostCol(price, O.DEC128)],
+                     aggs)
+    assert want["status"] == O.OK
+    assert got["n_groups"] == want["n_groups"]
+    kv, kvalid = got["keys"][0]
+    assert kv.shape == (want["n_groups"], 2)
+    assert np.array_equal(~kvalid, want["key_null"])
+    live = ~want["key_null"]
+    assert np.array_equal(kv[live, 0], want["keys"][live])
+    assert np.array_equal(kv[live, 1], want["keys"][live] >> 63)  # sign-extended high word
+    assert np.array_equal(got["aggs"][0][0], want["i64"][0])
+    assert np.array_equal(got["aggs"][1][0], want["i64"][1])
+    dsum = got["aggs"][2][0]
+    assert np.array_equal(dsum[:, 0], want["i64"][2]) and np.array_equal(dsum[:, 1], want["hi"][2])
+    assert np.array_equal(got["aggs"][3][0], want["i64"][3])
+    wide = dec_array([1, 2 ** 70, 3])
+    with pytest.raises(N.GpuCapacity):
+        execute_groupby([DeviceColumn.from_numpy(wide, N.DEC128)], [DeviceColumn.from_numpy(v[:3])], [(N.SUM, 0)])

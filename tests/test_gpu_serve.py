@@ -271,8 +271,9 @@ def test_serve_declines_a_regrown_table_over_the_budget(star):
     srv.close()
     assert res[0]["status"] == N.BOLT_DECLINE, res[0]
     assert all(r["status"] == N.BOLT_OK for r in res[1:])
-    assert all(r["rows"] == res[1]["rows"] for r in res[1:])
     assert st["peak_bytes"] <= need
     g, want, sabs = reference(star, fits)
+    for r in res[2:]:  # SUM(qty) is a float sum: equal within the tolerance, not bit for bit
+        same_rows(r["rows"], res[1]["rows"], 1, sabs)
     assert res[1]["n_groups"] == g
     same_rows(res[1]["rows"], want, 1, sabs)

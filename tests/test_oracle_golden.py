@@ -288,6 +288,40 @@ def test_groupby_growth_and_output_capacity():
         assert all(np.array_equal(a, b) for a, b in zip(r["i64"], full["i64"]))
 
 
+def _dec(vals):
+    """Decimal128 values as (n, 2) little-endian two's complement words."""
+    m = (1 << 64) - 1
+    return np.array([[v & m, (v >> 64) & m] for v in vals], dtype=np.uint64).view(np.int64)
+
+
+def test_groupby_decimal128_keys_bruteforce():
+    """Decimal128 GROUP BY keys (the reference rejects them, groupby_common.rs:117; here a
+    single key column whose values fit i64 groups by its exact value, in value order, with
+    its NULL group last) against a Python fold; a value beyond 64 bits is refused."""
+    rng = np.random.default_rng(31)
+    n = 5000
+    vals = [int(x) for x in rng.integers(-40, 40, n) * 10 ** 15 + rng.integers(-3, 3, n)]
+    valid = rng.random(n) > 0.1
+    v = rng.integers(-1000, 1000, n).astype(np.int64)
+    r = O.groupby([O.HostCol(_dec(vals), O.DEC128, valid)], [O.HostCol(v, O.I64)], [(O.SUM, 0), (O.COUNT_STAR, 0)])
+    assert r["status"] == O.OK
+    want = {}
+    for x, ok, y in zip(vals, valid, v):
+        k = x if ok else None
+        s, c = want.get(k, (0, 0))
+        want[k] = (s + int(y), c + 1)
+    keys = sorted(k for k in want if k is not None) + [None]
+    assert r["n_groups"] == len(keys)
+    assert r["key_null"].tolist() == [k is None for k in keys]
+    assert [int(x) for x in r["keys"][:-1]] == keys[:-1]
+    assert r["i64"][0].tolist() == [want[k][0] for k in keys]
+    assert r["i64"][1].tolist() == [want[k][1] for k in keys]
+    wide = _dec([1, 2 ** 63, 3])
+    assert O.groupby([O.HostCol(wide, O.DEC128)], [O.HostCol(v[:3], O.I64)], [(O.SUM, 0)])["status"] != O.OK
+    # a wide value in a NULL row does not count
+    assert O.groupby([O.HostCol(wide, O.DEC128, [1, 0, 1])], [O.HostCol(v[:3], O.I64)], [(O.SUM, 0)])["status"] == O.OK
+
+
 def test_groupby_null_group_single_column():
     # groupby.rs:528-606: all NULL-key rows form one group, key NULL, sorted last
     k = np.array([1, 2, 0, 1, 0], np.int64)
