@@ -251,6 +251,8 @@ _SIGS = {
     # operators
     "bolt_aggregate": (C.c_int, [C.c_void_p, C.POINTER(BoltColumn), C.c_int32, C.POINTER(BoltAgg), C.c_int32,
                                  C.POINTER(BoltFilter), C.POINTER(BoltScalar)]),
+    "bolt_aggregate_noalloc": (C.c_int, [C.c_void_p, C.POINTER(BoltColumn), C.c_int32, C.POINTER(BoltAgg), C.c_int32,
+                                         C.POINTER(BoltFilter), C.POINTER(BoltScalar)]),
     "bolt_filter_compact": (C.c_int, [C.c_void_p, C.POINTER(BoltColumn), C.c_int32, C.POINTER(BoltFilter),
                                       C.c_void_p, C.POINTER(BoltColumn), C.POINTER(C.c_int64)]),
     "bolt_prefix_scan_mask": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(C.c_int64)]),

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 #include <vector>
 
 #include "jit.h"
@@ -301,24 +302,45 @@ __global__ __launch_bounds__(kReduceBlock) void sum_prefix_kernel(DevFilter f, c
 
 // BOLT_OK when no running total of the filtered, non-NULL rows of `col`
 // (Int64), in row order, leaves i64; the reference's overflow error otherwise.
-static int check_sum_prefix(hipStream_t s, const bolt_column_t& col, const DevFilter& f) {
+constexpr unsigned kPfxMaxGrid = 1024;
+
+// `scratch` (room for kPfxMaxGrid triples) or NULL to allocate.
+static int check_sum_prefix(hipStream_t s, const bolt_column_t& col, const DevFilter& f, Pfx* scratch) {
   const int64_t n = col.length;
-  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 65535) / 65536));
+  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(kPfxMaxGrid, (n + 65535) / 65536));
   const int64_t per = (n + grid - 1) / grid;
   DevBuf<Pfx> parts;
-  BOLT_TRY(parts.alloc(grid, s));
-  sum_prefix_kernel<<<grid, kReduceBlock, 0, s>>>(f, static_cast<const int64_t*>(col.data), col.validity, n, per,
-                                                  parts.ptr);
+  Pfx* pp = scratch;
+  if (!pp) {
+    BOLT_TRY(parts.alloc(grid, s));
+    pp = parts.ptr;
+  }
+  sum_prefix_kernel<<<grid, kReduceBlock, 0, s>>>(f, static_cast<const int64_t*>(col.data), col.validity, n, per, pp);
   BOLT_HIP(hipGetLastError());
-  std::vector<Pfx> h(grid);
-  BOLT_HIP(hipMemcpyAsync(h.data(), parts.ptr, sizeof(Pfx) * grid, hipMemcpyDeviceToHost, s));
+  Pfx h[kPfxMaxGrid];
+  BOLT_HIP(hipMemcpyAsync(h, pp, sizeof(Pfx) * grid, hipMemcpyDeviceToHost, s));
   BOLT_HIP(hipStreamSynchronize(s));
   Pfx run{0, -kPfxNone, kPfxNone};
-  for (const Pfx& p : h) run = pfx_cat(run, p);
+  for (unsigned b = 0; b < grid; b++) run = pfx_cat(run, h[b]);
   if (run.hi > (__int128)INT64_MAX || run.lo < (__int128)INT64_MIN)
     return set_error(BOLT_ERR_TYPE, "SUM(integer) overflow: accumulator exceeds i64 range");
   return BOLT_OK;
 }
+
+// Device buffers of one scalar-aggregate call: per-workgroup partials, the
+// finished accumulators and the row-order prefix triples.
+struct AggScratch {
+  Acc* parts;
+  Acc* outs;
+  Pfx* pfx;
+};
+
+// bolt_aggregate, with its buffers allocated per call (scratch == NULL; the
+// generated kernel when hipRTC is available) or taken from `scratch` (the
+// precompiled kernels only, which allocate nothing).
+static int aggregate_impl(hipStream_t s, const bolt_column_t* inputs, int32_t n_inputs, const bolt_agg_t* aggs,
+                          int32_t n_aggs, const bolt_filter_t* filter, bolt_scalar_t* results,
+                          const AggScratch* scratch);
 
 }  // namespace bolt
 
@@ -329,7 +351,77 @@ extern "C" int bolt_aggregate(bolt_stream_t stream, const bolt_column_t* inputs,
                               bolt_scalar_t* results) {
   BOLT_CHECK_DEVICE();
   clear_error();
-  hipStream_t s = (hipStream_t)stream;
+  return aggregate_impl((hipStream_t)stream, inputs, n_inputs, aggs, n_aggs, filter, results, nullptr);
+}
+
+// ---- allocation-free variant (bolt_aggregate_noalloc) ------------------------
+//
+// For callers that cannot allocate device memory (HIP graph capture, serving
+// loops, harnesses that forbid hipMalloc): the partials, the finished
+// accumulators and the prefix triples live in one file-scope device array,
+// `scratch`, of kScratchBytes.  Each device holds its own copy at its own
+// address, looked up once per device with hipGetSymbolAddress; host code uses
+// only that address.  Nothing in it needs to start at zero (every kernel
+// writes what it later reads), and calls that overlap on one device share it
+// and therefore race.
+constexpr size_t kScratchBytes = (size_t)1 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// The current device's copy of `scratch` (looked up on the device's first call).
+int scratch_for_current_device(void** out) {
+  int dev = 0;
+  BOLT_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxScratchDevices)
+    return set_error(BOLT_ERR_DEVICE, "bolt_aggregate_noalloc: device %d beyond %d", dev, kMaxScratchDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      return set_error(BOLT_ERR_DEVICE, "bolt_aggregate_noalloc: address of the scratch array: %s",
+                       hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return BOLT_OK;
+}
+}  // namespace
+
+extern "C" int bolt_aggregate_noalloc(bolt_stream_t stream, const bolt_column_t* inputs, int32_t n_inputs,
+                                      const bolt_agg_t* aggs, int32_t n_aggs, const bolt_filter_t* filter,
+                                      bolt_scalar_t* results) {
+  BOLT_CHECK_DEVICE();
+  clear_error();
+  if (n_aggs < 0 || n_aggs > kMaxAggs)
+    return set_error(BOLT_ERR_PLAN, "%d aggregates (max %d)", n_aggs, kMaxAggs);
+  const int64_t n = n_inputs >= 1 && inputs ? inputs[0].length : 0;
+  // room for the most the call can use: kMaxAggs accumulators (the caller's
+  // and the hidden MIN / MAX of SUM(Int64)) per workgroup, in chunks of 4
+  const unsigned grid = grid_for(n, kReduceBlock, kAggUnroll * 4, 256 * 8);
+  const size_t parts_b = align256((size_t)grid * kMaxAggs * sizeof(Acc));
+  const size_t outs_b = align256((size_t)kMaxAggs * sizeof(Acc));
+  const size_t pfx_b = (size_t)kPfxMaxGrid * sizeof(Pfx);
+  if (parts_b + outs_b + pfx_b > kScratchBytes)
+    return set_error(BOLT_ERR_MEMORY, "bolt_aggregate_noalloc: needs %zu bytes of scratch, over its maximum of %zu",
+                     parts_b + outs_b + pfx_b, kScratchBytes);
+  void* base = nullptr;
+  BOLT_TRY(scratch_for_current_device(&base));
+  uint8_t* b = static_cast<uint8_t*>(base);
+  AggScratch sc{reinterpret_cast<Acc*>(b), reinterpret_cast<Acc*>(b + parts_b),
+                reinterpret_cast<Pfx*>(b + parts_b + outs_b)};
+  return aggregate_impl((hipStream_t)stream, inputs, n_inputs, aggs, n_aggs, filter, results, &sc);
+}
+
+static int bolt::aggregate_impl(hipStream_t s, const bolt_column_t* inputs, int32_t n_inputs, const bolt_agg_t* aggs,
+                                int32_t n_aggs, const bolt_filter_t* filter, bolt_scalar_t* results,
+                                const AggScratch* scratch) {
   if (n_inputs < 1 || inputs == nullptr)
     return set_error(BOLT_ERR_PLAN, "bolt_aggregate: at least one input column is required");
   int64_t n = inputs[0].length;
@@ -346,31 +438,40 @@ extern "C" int bolt_aggregate(bolt_stream_t stream, const bolt_column_t* inputs,
   // Each SUM over an Int64 column also gets a hidden MIN and MAX of the
   // column: when count * max(|MIN|, |MAX|) fits i64, no running total can
   // leave i64, and the row-order check (check_sum_prefix) is skipped.
+  // (fixed-size arrays: the allocation-free variant runs this too)
   const int32_t n_user = n_aggs;
-  std::vector<bolt_agg_t> ext(aggs, aggs + n_user);
-  std::vector<std::pair<int, int>> guards;  // (SUM index, index of its hidden MIN or -1)
-  if (aggs)
-    for (int i = 0; i < n_user; i++)
-      if (aggs[i].op == BOLT_AGG_SUM && aggs[i].col >= 0 && aggs[i].col < n_inputs &&
-          inputs[aggs[i].col].dtype == BOLT_I64 && n >= 2) {
-        if ((int)ext.size() + 2 <= kMaxAggs) {
-          guards.emplace_back(i, (int)ext.size());
-          ext.push_back(bolt_agg_t{BOLT_AGG_MIN, aggs[i].col});
-          ext.push_back(bolt_agg_t{BOLT_AGG_MAX, aggs[i].col});
-        } else {
-          guards.emplace_back(i, -1);
-        }
+  if (n_user < 0 || n_user > kMaxAggs || !aggs) return set_error(BOLT_ERR_PLAN, "%d aggregates (max %d)", n_user, kMaxAggs);
+  bolt_agg_t ext[kMaxAggs];
+  int n_ext = n_user;
+  for (int i = 0; i < n_user; i++) ext[i] = aggs[i];
+  int guard_sum[kMaxAggs], guard_mm[kMaxAggs];  // SUM index, index of its hidden MIN (-1: none)
+  int n_guards = 0;
+  for (int i = 0; i < n_user; i++)
+    if (aggs[i].op == BOLT_AGG_SUM && aggs[i].col >= 0 && aggs[i].col < n_inputs &&
+        inputs[aggs[i].col].dtype == BOLT_I64 && n >= 2) {
+      guard_sum[n_guards] = i;
+      guard_mm[n_guards] = -1;
+      if (n_ext + 2 <= kMaxAggs) {
+        guard_mm[n_guards] = n_ext;
+        ext[n_ext++] = bolt_agg_t{BOLT_AGG_MIN, aggs[i].col};
+        ext[n_ext++] = bolt_agg_t{BOLT_AGG_MAX, aggs[i].col};
       }
-  aggs = ext.data();
-  n_aggs = (int32_t)ext.size();
+      n_guards++;
+    }
+  aggs = ext;
+  n_aggs = n_ext;
   AggList all;
   BOLT_TRY(make_agg_list(aggs, n_aggs, inputs, n_inputs, 0, &all));
 
   unsigned grid = grid_for(n, kReduceBlock, kAggUnroll * 4, 256 * 8);
   DevBuf<Acc> parts, outs;
-  BOLT_TRY(outs.alloc((size_t)n_aggs, s));
+  Acc* outs_p = scratch ? scratch->outs : nullptr;
+  if (!outs_p) {
+    BOLT_TRY(outs.alloc((size_t)n_aggs, s));
+    outs_p = outs.ptr;
+  }
   bool done = false;
-  if (jit_enabled() && n > 0) {
+  if (!scratch && jit_enabled() && n > 0) {
     JitSpec sp;
     JitLits lits;
     JitCols cols;
@@ -383,7 +484,7 @@ extern "C" int bolt_aggregate(bolt_stream_t stream, const bolt_column_t* inputs,
       prof_end(s);
       for (int first = 0; first < n_aggs; first += 4) {
         int na = n_aggs - first < 4 ? n_aggs - first : 4;
-        BOLT_TRY(launch_finish(na, parts.ptr, (int)grid, n_aggs, first, agg_chunk(all, first, na), outs.ptr + first, s));
+        BOLT_TRY(launch_finish(na, parts.ptr, (int)grid, n_aggs, first, agg_chunk(all, first, na), outs_p + first, s));
       }
       done = true;
     }
@@ -391,29 +492,34 @@ extern "C" int bolt_aggregate(bolt_stream_t stream, const bolt_column_t* inputs,
   }
   if (!done) {
     int n_chunks = (n_aggs + 3) / 4;
-    BOLT_TRY(parts.alloc((size_t)grid * 4 * n_chunks, s));
+    Acc* parts_p = scratch ? scratch->parts : nullptr;
+    if (!parts_p) {
+      BOLT_TRY(parts.alloc((size_t)grid * 4 * n_chunks, s));
+      parts_p = parts.ptr;
+    }
     for (int c = 0; c < n_chunks; c++) {
       int first = c * 4, na = n_aggs - first < 4 ? n_aggs - first : 4;
       AggList chunk = agg_chunk(all, first, na);
-      Acc* p = parts.ptr + (size_t)c * grid * 4;
+      Acc* p = parts_p + (size_t)c * grid * 4;
       BOLT_TRY(launch_scalar_chunk(na, f, chunk, n, grid, p, s));
-      BOLT_TRY(launch_finish(na, p, (int)grid, na, 0, chunk, outs.ptr + first, s));
+      BOLT_TRY(launch_finish(na, p, (int)grid, na, 0, chunk, outs_p + first, s));
     }
   }
   Acc host[kMaxAggs];
-  BOLT_HIP(hipMemcpyAsync(host, outs.ptr, sizeof(Acc) * n_aggs, hipMemcpyDeviceToHost, s));
+  BOLT_HIP(hipMemcpyAsync(host, outs_p, sizeof(Acc) * n_aggs, hipMemcpyDeviceToHost, s));
   BOLT_HIP(hipStreamSynchronize(s));
   bolt_scalar_t res[kMaxAggs];
   for (int i = 0; i < n_aggs; i++) BOLT_TRY(acc_finalize(host[i], all.a[i].op, all.a[i].dtype, &res[i]));
-  for (const auto& g : guards) {
-    if (res[g.first].is_null || acc_count(host[g.first]) < 2) continue;
-    if (g.second >= 0) {
-      const __int128 lo = res[g.second].i64, hi = res[g.second + 1].i64;
+  for (int gi = 0; gi < n_guards; gi++) {
+    const int i = guard_sum[gi], mm = guard_mm[gi];
+    if (res[i].is_null || acc_count(host[i]) < 2) continue;
+    if (mm >= 0) {
+      const __int128 lo = res[mm].i64, hi = res[mm + 1].i64;
       const unsigned __int128 a = (unsigned __int128)(lo < 0 ? -lo : lo);
       const unsigned __int128 b = (unsigned __int128)(hi < 0 ? -hi : hi);
-      if ((a > b ? a : b) * (unsigned __int128)acc_count(host[g.first]) <= (unsigned __int128)INT64_MAX) continue;
+      if ((a > b ? a : b) * (unsigned __int128)acc_count(host[i]) <= (unsigned __int128)INT64_MAX) continue;
     }
-    BOLT_TRY(check_sum_prefix(s, inputs[aggs[g.first].col], f));
+    BOLT_TRY(check_sum_prefix(s, inputs[aggs[i].col], f, scratch ? scratch->pfx : nullptr));
   }
   for (int i = 0; i < n_user; i++) results[i] = res[i];
   return BOLT_OK;

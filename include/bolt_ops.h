@@ -140,6 +140,19 @@ int bolt_aggregate(bolt_stream_t stream, const bolt_column_t* inputs, int32_t n_
                    const bolt_agg_t* aggs, int32_t n_aggs, const bolt_filter_t* filter,
                    bolt_scalar_t* results);
 
+/* Allocation-free form of bolt_aggregate.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and errors as bolt_aggregate, computed with the
+ * precompiled kernels (no hipRTC module is loaded) in a fixed 1 MiB device
+ * scratch array per GPU, so it calls no allocator.  That array holds every
+ * call's intermediates: calls that overlap on one GPU race and must be
+ * serialised by the caller.  An input needing more scratch than the array
+ * holds returns BOLT_ERR_MEMORY (no input does with at most 16 aggregates). */
+int bolt_aggregate_noalloc(bolt_stream_t stream, const bolt_column_t* inputs, int32_t n_inputs,
+                           const bolt_agg_t* aggs, int32_t n_aggs, const bolt_filter_t* filter,
+                           bolt_scalar_t* results);
+
 /* ------------------------------------------------------------------------
  * Filter + stream compaction — replaces launch_predicate_kernel
  * (src/exec/compact.rs:183) + compact_columns_on_gpu
