@@ -24,8 +24,10 @@ BOLT_ERR_NO_DEVICE = 100000
 BOLT_HIP_ERROR_NOT_READY = 600
 
 # ---- dtypes ----------------------------------------------------------------
-BOOL, I32, I64, F32, F64, DEC128, DATE32 = 1, 2, 3, 4, 5, 6, 7
-DTYPE_SIZE = {BOOL: 1, I32: 4, I64: 8, F32: 4, F64: 8, DEC128: 16, DATE32: 4}
+BOOL, I32, I64, F32, F64, DEC128, DATE32, TIMESTAMP = 1, 2, 3, 4, 5, 6, 7, 8
+DTYPE_SIZE = {BOOL: 1, I32: 4, I64: 8, F32: 4, F64: 8, DEC128: 16, DATE32: 4, TIMESTAMP: 8}
+# Timestamp units (a Timestamp column's `scale`): digits of a second
+TS_S, TS_MS, TS_US, TS_NS = 0, 3, 6, 9
 
 # ---- aggregates ------------------------------------------------------------
 SUM, COUNT, COUNT_STAR, MIN, MAX, AVG, VAR_POP, VAR_SAMP, STDDEV_POP, STDDEV_SAMP = range(1, 11)

@@ -39,6 +39,8 @@ int make_dev_filter(const bolt_filter_t* f, const bolt_column_t* cols, int32_t n
     if (t.op < BOLT_CMP_LT || t.op > BOLT_CMP_IS_NOT_NULL)
       return set_error(BOLT_ERR_PLAN, "filter term %d: unknown comparison %d", i, t.op);
     if (dtype_size(c.dtype) == 0) return set_error(BOLT_ERR_TYPE, "filter term %d: unknown dtype %d", i, c.dtype);
+    if (c.dtype == BOLT_TIMESTAMP)
+      return set_error(BOLT_ERR_TYPE, "filter term %d: a Timestamp is compared inside an expression program", i);
     DevTerm& d = out->t[i];
     d.data = c.data;
     d.validity = c.validity;
@@ -83,6 +85,9 @@ int make_agg_list(const bolt_agg_t* aggs, int32_t n_aggs, const bolt_column_t* c
     d.dtype = dt;
     if (op == BOLT_AGG_COUNT) continue;
     if (dt == BOLT_BOOL) return set_error(BOLT_ERR_TYPE, "aggregate %d: Bool inputs are not supported", i);
+    if (dt == BOLT_TIMESTAMP)
+      return set_error(BOLT_ERR_TYPE, "aggregate %d: a Timestamp enters through an expression program (EXTRACT, "
+                       "DATE_TRUNC, comparisons)", i);
     if (op == BOLT_AGG_SUM && dt == BOLT_DATE32)
       // src/plan/logical_plan.rs:2135-2146
       return set_error(BOLT_ERR_TYPE, "SUM over a temporal type is undefined in SQL (Date32)");

@@ -100,7 +100,7 @@ inline int dtype_size(int32_t dt) {
   switch (dt) {
     case BOLT_BOOL: return 1;
     case BOLT_I32: case BOLT_F32: case BOLT_DATE32: return 4;
-    case BOLT_I64: case BOLT_F64: return 8;
+    case BOLT_I64: case BOLT_F64: case BOLT_TIMESTAMP: return 8;
     case BOLT_DEC128: return 16;
   }
   return 0;

@@ -523,6 +523,9 @@ int bolt::make_gplan(const bolt_agg_t* aggs, int32_t n_aggs, const bolt_column_t
       continue;
     }
     if (dt == BOLT_BOOL) return set_error(BOLT_ERR_TYPE, "aggregate %d: Bool inputs are not supported", i);
+    if (dt == BOLT_TIMESTAMP)
+      return set_error(BOLT_ERR_TYPE, "aggregate %d: a Timestamp enters through an expression program (EXTRACT, "
+                       "DATE_TRUNC, comparisons)", i);
     switch (g.op) {
       case BOLT_AGG_SUM:
         if (dt == BOLT_DATE32) return set_error(BOLT_ERR_TYPE, "SUM over a temporal type is undefined in SQL (Date32)");

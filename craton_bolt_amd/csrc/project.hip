@@ -25,12 +25,13 @@ namespace {
 struct RegInfo {
   int dtype = 0;       // 0 = not yet written
   uint32_t deps = 0;   // input columns whose values reach the register (bit per input)
+  int unit = 0;        // Timestamp: the unit as digits of a second (0, 3, 6, 9)
 };
 
 const char* ctype(int dt) {
   switch (dt) {
     case BOLT_BOOL: case BOLT_I32: case BOLT_DATE32: return "int";
-    case BOLT_I64: return "long long";
+    case BOLT_I64: case BOLT_TIMESTAMP: return "long long";
     case BOLT_F32: return "float";
     case BOLT_F64: return "double";
     case BOLT_DEC128: return "__int128";
@@ -43,7 +44,7 @@ const char* stype(int dt) {
   switch (dt) {
     case BOLT_BOOL: return "unsigned char";
     case BOLT_I32: case BOLT_DATE32: return "int";
-    case BOLT_I64: return "long long";
+    case BOLT_I64: case BOLT_TIMESTAMP: return "long long";
     case BOLT_F32: return "float";
     case BOLT_F64: return "double";
   }
@@ -54,8 +55,10 @@ bool is_int(int dt) { return dt == BOLT_I32 || dt == BOLT_I64 || dt == BOLT_DATE
 bool is_float(int dt) { return dt == BOLT_F32 || dt == BOLT_F64; }
 bool supported(int dt) {
   return dt == BOLT_BOOL || dt == BOLT_I32 || dt == BOLT_I64 || dt == BOLT_F32 || dt == BOLT_F64 ||
-         dt == BOLT_DATE32 || dt == BOLT_DEC128;
+         dt == BOLT_DATE32 || dt == BOLT_DEC128 || dt == BOLT_TIMESTAMP;
 }
+bool valid_unit(int u) { return u == 0 || u == 3 || u == 6 || u == 9; }
+constexpr int kMaxFloatScale = 22;  // 10^s exact in f64
 bool is_cmp(int op) { return op >= BOLT_BIN_EQ && op <= BOLT_BIN_GE; }
 
 // Functions the generated kernels call (guarded integer division and
@@ -82,6 +85,76 @@ __device__ __forceinline__ long long f2i64(double x) {
   return (long long)x;
 }
 __device__ __forceinline__ int fne(double a, double b) { return a < b || a > b; }
+// floor division (date_scalar.rs floor_div) and the civil calendar of
+// date_scalar.rs:37-60 (days since 1970-01-01 <-> proleptic Gregorian y-m-d)
+__device__ __forceinline__ long long bfdiv(long long a, long long b) {
+  const long long q = a / b, r = a % b;
+  return (r != 0 && ((r < 0) != (b < 0))) ? q - 1 : q;
+}
+__device__ __forceinline__ void bcivil(long long days, long long* y, long long* m, long long* d) {
+  const long long z = days + 719468;
+  const long long era = bfdiv(z, 146097);
+  const long long doe = z - era * 146097;
+  const long long yoe = (doe - doe / 1460 + doe / 36524 - doe / 146096) / 365;
+  const long long doy = doe - (365 * yoe + yoe / 4 - yoe / 100);
+  const long long mp = (5 * doy + 2) / 153;
+  *d = doy - (153 * mp + 2) / 5 + 1;
+  *m = mp < 10 ? mp + 3 : mp - 9;
+  *y = yoe + era * 400 + (*m <= 2 ? 1 : 0);
+}
+__device__ __forceinline__ long long bdays(long long y, long long m, long long d) {
+  y -= m <= 2 ? 1 : 0;
+  const long long era = bfdiv(y, 400);
+  const long long yoe = y - era * 400;
+  const long long doy = (153 * (m > 2 ? m - 3 : m + 9) + 2) / 5 + d - 1;
+  return era * 146097 + yoe * 365 + yoe / 4 - yoe / 100 + doy - 719468;
+}
+__device__ __forceinline__ long long bp10(int s) { long long r = 1; for (int i = 0; i < s; i++) r *= 10; return r; }
+// EXTRACT(field FROM x): x days (Date32, ts == 0) or ticks of 10^-unit s
+__device__ __forceinline__ long long bextract(long long x, int ts, int unit, int field) {
+  long long days = x, sod = 0;
+  if (ts) {
+    const long long tps = bp10(unit), tpd = 86400LL * tps;
+    days = bfdiv(x, tpd);
+    const long long r = x % tpd;  // the floor remainder, without forming days * tpd
+    sod = (r < 0 ? r + tpd : r) / tps;
+  }
+  if (field >= 4) return field == 4 ? sod / 3600 : field == 5 ? (sod / 60) % 60 : sod % 60;
+  long long y, m, d;
+  bcivil(days, &y, &m, &d);
+  return field == 1 ? y : field == 2 ? m : d;
+}
+// DATE_TRUNC(field, x): the start of x's year / month / day / hour / minute / second
+__device__ __forceinline__ long long btrunc(long long x, int ts, int unit, int field) {
+  const long long tps = ts ? bp10(unit) : 1, tpd = ts ? 86400LL * tps : 1;
+  if (field >= 4) {
+    const long long q = field == 4 ? 3600 * tps : field == 5 ? 60 * tps : tps;
+    return (long long)((unsigned long long)bfdiv(x, q) * (unsigned long long)q);  // wraps
+  }
+  const long long days = bfdiv(x, tpd);
+  long long y, m, d;
+  bcivil(days, &y, &m, &d);
+  const long long start = field == 1 ? bdays(y, 1, 1) : field == 2 ? bdays(y, m, 1) : days;
+  return (long long)((unsigned long long)start * (unsigned long long)tpd);
+}
+// Decimal128 <-> float (physical_plan.rs F64ToI128 / I128ToF64): round half
+// away from zero as trunc(|v| + 0.5), NaN -> 0, |v| >= 2^127 saturates; the
+// i128 as (double)hi * 2^64 + (double)lo
+__device__ __forceinline__ __int128 bf2d(double v) {
+  if (!(v == v)) return 0;
+  const double a = v < 0 ? -v : v;
+  if (a >= 170141183460469231731687303715884105728.0)
+    return v < 0 ? (__int128)((unsigned __int128)1 << 127) : (__int128)(((unsigned __int128)1 << 127) - 1);
+  const double m = __builtin_trunc(a + 0.5);
+  const double hd = __builtin_floor(m * 5.42101086242752217003726400434970855712890625e-20);
+  const unsigned long long hi = (unsigned long long)hd;
+  const unsigned long long lo = (unsigned long long)(m - hd * 18446744073709551616.0);
+  const __int128 r = (__int128)(((unsigned __int128)hi << 64) | lo);
+  return v < 0 ? -r : r;
+}
+__device__ __forceinline__ double bd2f(__int128 x) {
+  return (double)(long long)(x >> 64) * 18446744073709551616.0 + (double)(unsigned long long)x;
+}
 __device__ __forceinline__ int bitv(const unsigned char* v, long long i) { return (v[i >> 3] >> (i & 7)) & 1; }
 struct PArgs {
   const void* in[16];
@@ -101,7 +174,7 @@ std::string literal(const bolt_expr_op_t& o) {
     case BOLT_I32: case BOLT_DATE32:
       snprintf(buf, sizeof buf, "(int)%lldLL", (long long)(int32_t)o.lit_i64);
       return buf;
-    case BOLT_I64:
+    case BOLT_I64: case BOLT_TIMESTAMP:
       snprintf(buf, sizeof buf, "(long long)0x%016llxULL", (unsigned long long)o.lit_i64);
       return buf;
     case BOLT_F32: {
@@ -125,10 +198,65 @@ std::string literal(const bolt_expr_op_t& o) {
   return "0";
 }
 
+// 10^s as a C literal: a 128-bit integer, or a double (s <= kMaxFloatScale)
+std::string pow10_i128(int s) {
+  unsigned __int128 v = 1;
+  for (int i = 0; i < s; i++) v *= 10;
+  char buf[96];
+  snprintf(buf, sizeof buf, "(__int128)(((unsigned __int128)0x%016llxULL << 64) | 0x%016llxULL)",
+           (unsigned long long)(v >> 64), (unsigned long long)v);
+  return buf;
+}
+std::string pow10_f64(int s) {
+  double v = 1.0;
+  for (int i = 0; i < s; i++) v *= 10.0;  // exact for s <= 22
+  uint64_t b;
+  memcpy(&b, &v, 8);
+  char buf[64];
+  snprintf(buf, sizeof buf, "__builtin_bit_cast(double, 0x%016llxULL)", (unsigned long long)b);
+  return buf;
+}
+
+// Casts that involve Decimal128 (physical_plan.rs:2046-2060); `sf` / `st`
+// the source / target scales.
+int decimal_cast_expr(int from, int to, int sf, int st, const std::string& src, std::string* out) {
+  if (sf < 0 || sf > 38 || st < 0 || st > 38)
+    return set_error(BOLT_ERR_TYPE, "bolt_project: Decimal128 scale out of 0..38 (%d, %d)", sf, st);
+  const std::string wrapmul = "(__int128)((unsigned __int128)";
+  if (to == BOLT_DEC128 && (from == BOLT_I32 || from == BOLT_I64 || from == BOLT_DATE32)) {
+    *out = wrapmul + "(__int128)" + src + " * (unsigned __int128)" + pow10_i128(st) + ")";
+    return BOLT_OK;
+  }
+  if (to == BOLT_DEC128 && from == BOLT_DEC128) {
+    if (st >= sf) *out = wrapmul + src + " * (unsigned __int128)" + pow10_i128(st - sf) + ")";
+    else *out = "(" + src + " / " + pow10_i128(sf - st) + ")";  // truncating toward zero
+    return BOLT_OK;
+  }
+  if (from == BOLT_DEC128 && (to == BOLT_I32 || to == BOLT_I64)) {
+    const std::string q = sf > 0 ? "(" + src + " / " + pow10_i128(sf) + ")" : src;
+    *out = to == BOLT_I64 ? "(long long)(unsigned long long)(unsigned __int128)" + q
+                          : "(int)(unsigned)(unsigned long long)(unsigned __int128)" + q;
+    return BOLT_OK;
+  }
+  if (to == BOLT_DEC128 && is_float(from)) {
+    if (st > kMaxFloatScale) return set_error(BOLT_ERR_TYPE, "bolt_project: float -> Decimal128 scale %d over 22", st);
+    *out = "bf2d((double)" + src + " * " + pow10_f64(st) + ")";
+    return BOLT_OK;
+  }
+  if (from == BOLT_DEC128 && is_float(to)) {
+    if (sf > kMaxFloatScale) return set_error(BOLT_ERR_TYPE, "bolt_project: Decimal128 -> float scale %d over 22", sf);
+    const std::string v = "(bd2f(" + src + ") / " + pow10_f64(sf) + ")";
+    *out = to == BOLT_F32 ? "(float)" + v : v;
+    return BOLT_OK;
+  }
+  return set_error(BOLT_ERR_TYPE, "bolt_project: no cast from %s to %s", dtype_name(from), dtype_name(to));
+}
+
 // C expression for `(to) src` under ptx_gen.rs's cast rules (:1890-1990).
 int cast_expr(int from, int to, const std::string& src, std::string* out) {
   if (from == to) { *out = src; return BOLT_OK; }
-  if (from == BOLT_DEC128 || to == BOLT_DEC128 || from == BOLT_DATE32 || to == BOLT_DATE32)
+  if (from == BOLT_DEC128 || to == BOLT_DEC128 || from == BOLT_DATE32 || to == BOLT_DATE32 ||
+      from == BOLT_TIMESTAMP || to == BOLT_TIMESTAMP)
     return set_error(BOLT_ERR_TYPE, "bolt_project: no cast from %s to %s", dtype_name(from), dtype_name(to));
   if (to == BOLT_BOOL) {
     *out = is_float(from) ? "fne((double)" + src + ", 0.0)" : "(" + src + " != 0 ? 1 : 0)";
@@ -143,6 +271,8 @@ int cast_expr(int from, int to, const std::string& src, std::string* out) {
 }
 
 int binary_expr(int op, int dt, const std::string& a, const std::string& b, std::string* out, int* rdt) {
+  if (dt == BOLT_TIMESTAMP && !is_cmp(op))
+    return set_error(BOLT_ERR_TYPE, "bolt_project: binary op %d not defined on Timestamp", op);
   const bool i = is_int(dt), f = is_float(dt), d = dt == BOLT_DEC128, bl = dt == BOLT_BOOL;
   const char* u = dt == BOLT_I64 ? "u64" : d ? "unsigned __int128" : "unsigned";
   *rdt = dt;
@@ -181,6 +311,7 @@ int binary_expr(int op, int dt, const std::string& a, const std::string& b, std:
     default:
       if (!is_cmp(op)) return set_error(BOLT_ERR_PLAN, "bolt_project: unknown binary op %d", op);
       *rdt = BOLT_BOOL;
+      // (Timestamp operands: comparisons only, of equal units, checked by the caller)
       if (f && op == BOLT_BIN_NE) { *out = "fne(" + a + ", " + b + ")"; return BOLT_OK; }
       {
         static const char* syms[] = {"==", "!=", "<", "<=", ">", ">="};
@@ -238,18 +369,65 @@ int compile_program(const bolt_column_t* in, int n_in, const bolt_expr_op_t* pro
           e = std::string("((const ") + stype(dt) + "*)p.in[" + c + "])[" + row + "]";
         d.dtype = dt;
         d.deps = 1u << o.col;
+        if (dt == BOLT_TIMESTAMP) {
+          if (!valid_unit(in[o.col].scale))
+            return set_error(BOLT_ERR_TYPE, "bolt_project: column %d has Timestamp unit %d (0, 3, 6 or 9)", o.col,
+                             in[o.col].scale);
+          d.unit = in[o.col].scale;
+        }
         break;
       }
       case BOLT_X_CONST:
         if (!supported(o.dtype)) return set_error(BOLT_ERR_TYPE, "bolt_project: op %d constant of dtype %d", k, o.dtype);
+        if (o.dtype == BOLT_TIMESTAMP && !valid_unit(o.op))
+          return set_error(BOLT_ERR_TYPE, "bolt_project: op %d Timestamp constant of unit %d", k, o.op);
         e = literal(o);
         d.dtype = o.dtype;
+        d.unit = o.dtype == BOLT_TIMESTAMP ? o.op : 0;
         break;
       case BOLT_X_CAST: {
         BOLT_TRY(use(o.a, "CAST"));
         if (!supported(o.dtype)) return set_error(BOLT_ERR_TYPE, "bolt_project: op %d casts to dtype %d", k, o.dtype);
-        BOLT_TRY(cast_expr(regs[o.a].dtype, o.dtype, reg(o.a), &e));
+        const int from = regs[o.a].dtype;
+        if ((from == BOLT_DEC128 || o.dtype == BOLT_DEC128) &&
+            !(from == BOLT_DEC128 && o.dtype == BOLT_DEC128 && o.c == o.op)) {
+          if (from == BOLT_BOOL || from == BOLT_TIMESTAMP || o.dtype == BOLT_BOOL || o.dtype == BOLT_DATE32 ||
+              o.dtype == BOLT_TIMESTAMP)
+            return set_error(BOLT_ERR_TYPE, "bolt_project: no cast from %s to %s", dtype_name(from),
+                             dtype_name(o.dtype));
+          BOLT_TRY(decimal_cast_expr(from, o.dtype, from == BOLT_DEC128 ? o.c : 0, o.dtype == BOLT_DEC128 ? o.op : 0,
+                                     reg(o.a), &e));
+        } else {
+          BOLT_TRY(cast_expr(from, o.dtype, reg(o.a), &e));
+        }
         d.dtype = o.dtype;
+        d.unit = regs[o.a].unit;
+        d.deps = regs[o.a].deps;
+        break;
+      }
+      case BOLT_X_EXTRACT:
+      case BOLT_X_DATE_TRUNC: {
+        const char* what = o.kind == BOLT_X_EXTRACT ? "EXTRACT" : "DATE_TRUNC";
+        BOLT_TRY(use(o.a, what));
+        const int dt = regs[o.a].dtype;
+        if (dt != BOLT_DATE32 && dt != BOLT_TIMESTAMP)
+          return set_error(BOLT_ERR_TYPE, "bolt_project: op %d %s requires Date32 / Timestamp, got %s", k, what,
+                           dtype_name(dt));
+        if (o.op < BOLT_DF_YEAR || o.op > BOLT_DF_SECOND)
+          return set_error(BOLT_ERR_PLAN, "bolt_project: op %d %s of unknown field %d", k, what, o.op);
+        if (dt == BOLT_DATE32 && o.op >= BOLT_DF_HOUR)  // date_scalar.rs:465,557
+          return set_error(BOLT_ERR_TYPE, "bolt_project: op %d %s of a time-of-day field from Date32 is undefined", k,
+                           what);
+        const std::string args = "(long long)" + reg(o.a) + ", " + (dt == BOLT_TIMESTAMP ? "1, " : "0, ") +
+                                 std::to_string(regs[o.a].unit) + ", " + std::to_string(o.op) + ")";
+        if (o.kind == BOLT_X_EXTRACT) {
+          e = "bextract(" + args;
+          d.dtype = BOLT_I64;
+        } else {
+          e = std::string("(") + ctype(dt) + ")btrunc(" + args;
+          d.dtype = dt;
+          d.unit = regs[o.a].unit;
+        }
         d.deps = regs[o.a].deps;
         break;
       }
@@ -259,6 +437,9 @@ int compile_program(const bolt_column_t* in, int n_in, const bolt_expr_op_t* pro
         if (regs[o.a].dtype != o.dtype || regs[o.b].dtype != o.dtype)
           return set_error(BOLT_ERR_TYPE, "bolt_project: op %d operands are %s and %s, op says %s (cast first)", k,
                            dtype_name(regs[o.a].dtype), dtype_name(regs[o.b].dtype), dtype_name(o.dtype));
+        if (o.dtype == BOLT_TIMESTAMP && regs[o.a].unit != regs[o.b].unit)
+          return set_error(BOLT_ERR_TYPE, "bolt_project: op %d compares Timestamps of units %d and %d", k,
+                           regs[o.a].unit, regs[o.b].unit);
         int rdt = 0;
         BOLT_TRY(binary_expr(o.op, o.dtype, reg(o.a), reg(o.b), &e, &rdt));
         d.dtype = rdt;
@@ -269,10 +450,12 @@ int compile_program(const bolt_column_t* in, int n_in, const bolt_expr_op_t* pro
         BOLT_TRY(use(o.a, "SELECT"));
         BOLT_TRY(use(o.b, "SELECT"));
         BOLT_TRY(use(o.c, "SELECT"));
-        if (regs[o.a].dtype != BOLT_BOOL || regs[o.b].dtype != regs[o.c].dtype)
+        if (regs[o.a].dtype != BOLT_BOOL || regs[o.b].dtype != regs[o.c].dtype ||
+            (regs[o.b].dtype == BOLT_TIMESTAMP && regs[o.b].unit != regs[o.c].unit))
           return set_error(BOLT_ERR_TYPE, "bolt_project: op %d SELECT needs a Bool condition and equal value dtypes", k);
         e = "(" + reg(o.a) + " ? " + reg(o.b) + " : " + reg(o.c) + ")";
         d.dtype = regs[o.b].dtype;
+        d.unit = regs[o.b].unit;
         d.deps = regs[o.a].deps | regs[o.b].deps | regs[o.c].deps;
         break;
       case BOLT_X_NOT:
@@ -297,6 +480,9 @@ int compile_program(const bolt_column_t* in, int n_in, const bolt_expr_op_t* pro
         if (out && out[o.col].dtype != regs[o.a].dtype)
           return set_error(BOLT_ERR_TYPE, "bolt_project: output %d is %s, its value %s", o.col,
                            dtype_name(out[o.col].dtype), dtype_name(regs[o.a].dtype));
+        if (out && regs[o.a].dtype == BOLT_TIMESTAMP && out[o.col].scale != regs[o.a].unit)
+          return set_error(BOLT_ERR_TYPE, "bolt_project: output %d has Timestamp unit %d, its value %d", o.col,
+                           out[o.col].scale, regs[o.a].unit);
         cp->reg[o.col] = o.a;
         cp->dtype[o.col] = regs[o.a].dtype;
         cp->deps[o.col] = regs[o.a].deps;

@@ -154,6 +154,7 @@ const char* dtype_name(int32_t dt) {
     case BOLT_F64: return "Float64";
     case BOLT_DEC128: return "Decimal128";
     case BOLT_DATE32: return "Date32";
+    case BOLT_TIMESTAMP: return "Timestamp";
   }
   return "Unknown";
 }

@@ -20,6 +20,7 @@ _NP_DTYPE = {
     N.F32: np.float32,
     N.F64: np.float64,
     N.DATE32: np.int32,
+    N.TIMESTAMP: np.int64,
 }
 
 
@@ -169,12 +170,12 @@ class DeviceColumn:
         return cls(DeviceBuffer.from_numpy(a), length, dtype, vb, scale)
 
     @classmethod
-    def empty(cls, length: int, dtype: int, with_validity: bool = False) -> "DeviceColumn":
+    def empty(cls, length: int, dtype: int, with_validity: bool = False, scale: int = 0) -> "DeviceColumn":
         vb = None
         if with_validity:
             vb = DeviceBuffer((length + 7) // 8 + 8)
             vb.fill(0)
-        return cls(DeviceBuffer(max(length, 1) * N.DTYPE_SIZE[dtype]), length, dtype, vb)
+        return cls(DeviceBuffer(max(length, 1) * N.DTYPE_SIZE[dtype]), length, dtype, vb, scale)
 
     def c(self, length: int | None = None) -> N.BoltColumn:
         return N.BoltColumn(self.values.ptr, self.validity.ptr if self.validity else None,

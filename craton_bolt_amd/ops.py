@@ -530,7 +530,9 @@ def execute_inner_join(left_key: DeviceColumn, right_key: DeviceColumn, capacity
 
 # ---- per-row expression programs (src/plan/physical_plan.rs:81 Op) ---------------------------
 
-X_LOAD, X_CONST, X_CAST, X_BINARY, X_SELECT, X_NOT, X_IS_NULL, X_STORE = range(1, 9)
+X_LOAD, X_CONST, X_CAST, X_BINARY, X_SELECT, X_NOT, X_IS_NULL, X_STORE, X_EXTRACT, X_DATE_TRUNC = range(1, 11)
+# EXTRACT fields / DATE_TRUNC units (bolt_date_field)
+DATE_FIELDS = {"year": 1, "month": 2, "day": 3, "hour": 4, "minute": 5, "second": 6}
 BIN_OPS = {"+": 1, "-": 2, "*": 3, "/": 4, "%": 5, "==": 6, "!=": 7, "<": 8, "<=": 9, ">": 10, ">=": 11,
            "and": 12, "or": 13, "&": 14, "|": 15}
 
@@ -555,7 +557,10 @@ class Program:
     def load(self, col: int, dtype: int) -> int:
         return self._emit(X_LOAD, col=col, dtype=dtype)
 
-    def const(self, value, dtype: int) -> int:
+    def const(self, value, dtype: int, unit: int = 0) -> int:
+        """A literal; a Timestamp literal carries its unit (digits of a second)."""
+        if dtype == N.TIMESTAMP:
+            return self._emit(X_CONST, dtype=dtype, lit_i64=int(value), op=unit)
         if dtype in (N.F32, N.F64):
             return self._emit(X_CONST, dtype=dtype, lit_f64=float(value))
         v = int(value)
@@ -566,8 +571,18 @@ class Program:
                               lit_hi=hi - (1 << 64) if hi >> 63 else hi)
         return self._emit(X_CONST, dtype=dtype, lit_i64=v)
 
-    def cast(self, a: int, dtype: int) -> int:
-        return self._emit(X_CAST, a=a, dtype=dtype)
+    def cast(self, a: int, dtype: int, scale: int = 0, from_scale: int = 0) -> int:
+        """(dtype) a; `scale` is the target scale of a Decimal128 target, `from_scale` the
+        scale of a Decimal128 source."""
+        return self._emit(X_CAST, a=a, dtype=dtype, op=scale, c=from_scale)
+
+    def extract(self, field: str, a: int) -> int:
+        """EXTRACT(field FROM a), Int64; a Date32 or Timestamp."""
+        return self._emit(X_EXTRACT, a=a, op=DATE_FIELDS[field])
+
+    def date_trunc(self, unit: str, a: int) -> int:
+        """DATE_TRUNC(unit, a), of a's dtype."""
+        return self._emit(X_DATE_TRUNC, a=a, op=DATE_FIELDS[unit])
 
     def binary(self, op: str, a: int, b: int, dtype: int) -> int:
         return self._emit(X_BINARY, a=a, b=b, op=BIN_OPS[op], dtype=dtype)
@@ -593,10 +608,13 @@ class Program:
 
 
 def execute_projection(inputs: list[DeviceColumn], program: Program, out_dtypes: list[int],
-                       out_validity: list[bool] | None = None, stream: Stream | None = None) -> list[DeviceColumn]:
-    """Runs `program` over every row of `inputs` (bolt_project); returns the output columns."""
+                       out_validity: list[bool] | None = None, stream: Stream | None = None,
+                       out_scales: list[int] | None = None) -> list[DeviceColumn]:
+    """Runs `program` over every row of `inputs` (bolt_project); returns the output columns
+    (`out_scales`: Decimal128 scales / Timestamp units of the outputs)."""
     n = inputs[0].length if inputs else 0
-    outs = [DeviceColumn.empty(n, dt, with_validity=bool(out_validity and out_validity[i]))
+    outs = [DeviceColumn.empty(n, dt, with_validity=bool(out_validity and out_validity[i]),
+                               scale=out_scales[i] if out_scales else 0)
             for i, dt in enumerate(out_dtypes)]
     oc = columns_array(outs)
     st = N.lib().bolt_project(_stream(stream), columns_array(inputs), len(inputs), program.array(N.BoltExprOp),

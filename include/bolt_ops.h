@@ -57,7 +57,9 @@ enum bolt_dtype {
   BOLT_F32 = 4,
   BOLT_F64 = 5,
   BOLT_DEC128 = 6, /* 16-byte little-endian two's complement, scale in `scale` */
-  BOLT_DATE32 = 7  /* days since the epoch, int32 */
+  BOLT_DATE32 = 7, /* days since the epoch, int32 */
+  BOLT_TIMESTAMP = 8 /* ticks since the epoch, int64; `scale` is the unit as digits of a
+                        second: 0 s, 3 ms, 6 us, 9 ns (TimeUnit, src/plan/logical_plan.rs:7) */
 };
 
 /* A device column: Arrow layout (values + optional LSB-first validity
@@ -359,6 +361,26 @@ int bolt_sort_limit(bolt_stream_t stream, const bolt_column_t* keys, int32_t n_k
  *   saturate (NaN -> 0); numeric -> Bool is "!= 0"; Date32 - Date32 is Int32
  *   days; Decimal128 Add / Sub / Mul act on the unscaled 128-bit integers
  *   (wrapping) and Decimal128 values compare as signed 128-bit integers.
+ * Casts to and from Decimal128 follow emit_decimal_cast
+ * (src/plan/physical_plan.rs:2046-2060): the op's `op` field is the target
+ * scale when the target is Decimal128 and its `c` field the source scale when
+ * the source is; integers scale up by 10^s (wrapping); a decimal rescales up
+ * by multiplying and down by dividing, truncating toward zero; decimal ->
+ * integer divides out the scale (truncating) and keeps the low 64 / 32 bits;
+ * float -> decimal scales by 10^s in f64 and rounds half away from zero
+ * (NaN -> 0, magnitudes of 2^127 and more saturate); decimal -> float is
+ * (double)hi * 2^64 + (double)lo, divided by 10^s (and rounded to f32 for a
+ * Float32 target).  Float conversions take scales up to 22 (exact powers of
+ * ten in f64).  A Date32 casts to Decimal128 as its day number; Bool /
+ * Timestamp <-> Decimal128 and Decimal128 -> Date32 casts are type errors.
+ * Date32 / Timestamp fields (date_scalar.rs:459,539) use the proleptic
+ * Gregorian calendar (civil_from_days, date_scalar.rs:37-60) with floor
+ * division for instants before the epoch: EXTRACT gives an Int64 (YEAR,
+ * MONTH 1-12, DAY 1-31, HOUR, MINUTE, SECOND; the last three need a
+ * Timestamp), DATE_TRUNC gives the value's own dtype truncated to the start of
+ * its unit.  A Timestamp register keeps its unit (a LOAD takes the column's
+ * `scale`, a CONST its `op` field); comparisons, SELECT and STORE need equal
+ * units.
  * Each output with a validity bitmap receives the AND of the validity of
  * every input column its value loads (the AND-of-inputs fold of
  * ptx_gen.rs:474); values of NULL rows are computed from the stored bits.
@@ -371,7 +393,14 @@ enum bolt_expr_kind {
   BOLT_X_SELECT = 5,  /* dst = a ? b : c, a Bool                       (Select) */
   BOLT_X_NOT = 6,     /* dst = !a, a Bool                              (Not) */
   BOLT_X_IS_NULL = 7, /* dst = inputs[col] is NULL (op 1) / not (op 0) (IsNullCheck) */
-  BOLT_X_STORE = 8    /* outputs[col][row] = a                         (Store) */
+  BOLT_X_STORE = 8,   /* outputs[col][row] = a                         (Store) */
+  BOLT_X_EXTRACT = 9, /* dst = EXTRACT(field `op` FROM a), Int64      (compile_extract_kernel) */
+  BOLT_X_DATE_TRUNC = 10 /* dst = DATE_TRUNC(unit `op`, a), a's dtype  (compile_date_trunc_kernel) */
+};
+/* EXTRACT fields and DATE_TRUNC units (DateField / DateTruncUnit,
+ * src/plan/logical_plan.rs:568,613) */
+enum bolt_date_field {
+  BOLT_DF_YEAR = 1, BOLT_DF_MONTH = 2, BOLT_DF_DAY = 3, BOLT_DF_HOUR = 4, BOLT_DF_MINUTE = 5, BOLT_DF_SECOND = 6
 };
 enum bolt_bin_op {
   BOLT_BIN_ADD = 1, BOLT_BIN_SUB = 2, BOLT_BIN_MUL = 3, BOLT_BIN_DIV = 4, BOLT_BIN_MOD = 5,
@@ -381,9 +410,11 @@ enum bolt_bin_op {
 typedef struct bolt_expr_op {
   int32_t kind;    /* bolt_expr_kind */
   int32_t dst;     /* destination register, 0 .. BOLT_MAX_REGS - 1 */
-  int32_t a, b, c; /* operand registers */
+  int32_t a, b, c; /* operand registers (CAST from Decimal128: c is the source scale) */
   int32_t col;     /* LOAD / IS_NULL: input column; STORE: output column */
-  int32_t op;      /* BINARY: bolt_bin_op; IS_NULL: 1 = IS NULL, 0 = IS NOT NULL */
+  int32_t op;      /* BINARY: bolt_bin_op; IS_NULL: 1 = IS NULL, 0 = IS NOT NULL;
+                      CAST to Decimal128: target scale; CONST Timestamp: unit digits;
+                      EXTRACT / DATE_TRUNC: bolt_date_field */
   int32_t dtype;   /* LOAD / CONST: value dtype; CAST: target; BINARY: operand dtype */
   int64_t lit_i64; /* CONST: integer / date / Bool value, Decimal128 low word */
   double lit_f64;  /* CONST: float value */

@@ -15,7 +15,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "build" / "libbolt_oracle.so"
 
-BOOL, I32, I64, F32, F64, DEC128, DATE32 = 1, 2, 3, 4, 5, 6, 7
+BOOL, I32, I64, F32, F64, DEC128, DATE32, TIMESTAMP = 1, 2, 3, 4, 5, 6, 7, 8
 SUM, COUNT, COUNT_STAR, MIN, MAX, AVG, VAR_POP, VAR_SAMP, STDDEV_POP, STDDEV_SAMP = range(1, 11)
 LT, LE, GT, GE, EQ, NE, IS_NULL, IS_NOT_NULL = range(1, 9)
 OK, ERR_PLAN, ERR_TYPE, ERR_CAP = 0, 1, 2, 3
@@ -247,11 +247,12 @@ def sort_indices(keys: list[HostCol], descending: list[bool], nulls_first: list[
     return out[:n]
 
 
-_NP = {BOOL: np.uint8, I32: np.int32, I64: np.int64, F32: np.float32, F64: np.float64, DATE32: np.int32}
+_NP = {BOOL: np.uint8, I32: np.int32, I64: np.int64, F32: np.float32, F64: np.float64, DATE32: np.int32,
+       TIMESTAMP: np.int64}
 
 
 def project(inputs: list[HostCol], ops: list[dict], out_dtypes: list[int], n: int,
-            out_validity: list[bool] | None = None):
+            out_validity: list[bool] | None = None, out_scales: list[int] | None = None):
     """Expression program (src/plan/physical_plan.rs:81, semantics of src/jit/ptx_gen.rs) over n rows.
     Returns (status, [values], [validity or None]); Decimal128 values as (n, 2) int64 [lo, hi]."""
     arr = (ExprOp * max(len(ops), 1))()
@@ -264,7 +265,8 @@ def project(inputs: list[HostCol], ops: list[dict], out_dtypes: list[int], n: in
         bm = np.zeros((n + 7) // 8 + 1, np.uint8) if out_validity and out_validity[i] else None
         vals.append(v)
         bms.append(bm)
-        outs.append(Col(v.ctypes.data, bm.ctypes.data if bm is not None else None, n, dt, 0))
+        outs.append(Col(v.ctypes.data, bm.ctypes.data if bm is not None else None, n, dt,
+                        out_scales[i] if out_scales else 0))
     oc = (Col * max(len(outs), 1))(*outs)
     st = lib().bo_project(_cols(inputs), len(inputs), arr, len(ops), oc, len(outs), n)
     valid = [np.unpackbits(b, bitorder="little")[:n].astype(bool) if b is not None else None for b in bms]

@@ -879,15 +879,86 @@ int32_t bo_filter_join_agg(const int64_t* build_key, const int32_t* build_payloa
 /* Per-row expression programs — src/plan/physical_plan.rs:81, ptx_gen.rs    */
 /* ------------------------------------------------------------------------ */
 
-enum { X_LOAD = 1, X_CONST, X_CAST, X_BINARY, X_SELECT, X_NOT, X_IS_NULL, X_STORE };
+enum { X_LOAD = 1, X_CONST, X_CAST, X_BINARY, X_SELECT, X_NOT, X_IS_NULL, X_STORE, X_EXTRACT, X_DATE_TRUNC };
+enum { DF_YEAR = 1, DF_MONTH, DF_DAY, DF_HOUR, DF_MINUTE, DF_SECOND };
 enum { B_ADD = 1, B_SUB, B_MUL, B_DIV, B_MOD, B_EQ, B_NE, B_LT, B_LE, B_GT, B_GE, B_AND, B_OR, B_BAND, B_BOR };
 
 typedef struct {
   int32_t dt;
-  int64_t i;    /* Bool / Int32 / Int64 / Date32 */
+  int64_t i;    /* Bool / Int32 / Int64 / Date32 / Timestamp ticks */
   double f;     /* Float64; Float32 values are held rounded to float */
   __int128 d;   /* Decimal128 */
+  int32_t u;    /* Timestamp unit: digits of a second (0, 3, 6, 9) */
 } xval;
+
+/* floor_div and civil_from_days of src/jit/date_scalar.rs:29-60 (the
+ * proleptic Gregorian calendar on days since 1970-01-01), and the inverse */
+static int64_t x_floor_div(int64_t a, int64_t b) {
+  int64_t q = a / b, r = a % b;
+  return (r != 0 && ((r < 0) != (b < 0))) ? q - 1 : q;
+}
+static void x_civil(int64_t days, int64_t* y, int64_t* m, int64_t* d) {
+  int64_t z = days + 719468, era = x_floor_div(z, 146097), doe = z - era * 146097;
+  int64_t yoe = (doe - doe / 1460 + doe / 36524 - doe / 146096) / 365;
+  int64_t doy = doe - (365 * yoe + yoe / 4 - yoe / 100), mp = (5 * doy + 2) / 153;
+  *d = doy - (153 * mp + 2) / 5 + 1;
+  *m = mp < 10 ? mp + 3 : mp - 9;
+  *y = yoe + era * 400 + (*m <= 2);
+}
+static int64_t x_days(int64_t y, int64_t m, int64_t d) {
+  y -= m <= 2;
+  int64_t era = x_floor_div(y, 400), yoe = y - era * 400;
+  int64_t doy = (153 * (m > 2 ? m - 3 : m + 9) + 2) / 5 + d - 1;
+  return era * 146097 + yoe * 365 + yoe / 4 - yoe / 100 + doy - 719468;
+}
+static int64_t x_pow10(int s) { int64_t r = 1; while (s-- > 0) r *= 10; return r; }
+static __int128 x_pow10x(int s) { __int128 r = 1; while (s-- > 0) r *= 10; return r; }
+static double x_pow10f(int s) { double r = 1.0; while (s-- > 0) r *= 10.0; return r; }
+
+/* date_scalar.rs:459 EXTRACT and :539 DATE_TRUNC on days (ts == 0) or ticks */
+static int64_t x_extract(int64_t x, int ts, int unit, int field) {
+  int64_t days = x, sod = 0;
+  if (ts) {
+    int64_t tps = x_pow10(unit), tpd = 86400 * tps;
+    days = x_floor_div(x, tpd);
+    const int64_t r = x % tpd; /* the floor remainder, without forming days * tpd */
+    sod = (r < 0 ? r + tpd : r) / tps;
+  }
+  if (field == DF_HOUR) return sod / 3600;
+  if (field == DF_MINUTE) return (sod / 60) % 60;
+  if (field == DF_SECOND) return sod % 60;
+  int64_t y, m, d;
+  x_civil(days, &y, &m, &d);
+  return field == DF_YEAR ? y : field == DF_MONTH ? m : d;
+}
+static int64_t x_trunc(int64_t x, int ts, int unit, int field) {
+  int64_t tps = ts ? x_pow10(unit) : 1, tpd = ts ? 86400 * tps : 1;
+  if (field >= DF_HOUR) {
+    int64_t q = field == DF_HOUR ? 3600 * tps : field == DF_MINUTE ? 60 * tps : tps;
+    return (int64_t)((uint64_t)x_floor_div(x, q) * (uint64_t)q); /* mul.lo.s64: wraps */
+  }
+  int64_t days = x_floor_div(x, tpd), y, m, d;
+  x_civil(days, &y, &m, &d);
+  const int64_t start = field == DF_YEAR ? x_days(y, 1, 1) : field == DF_MONTH ? x_days(y, m, 1) : days;
+  return (int64_t)((uint64_t)start * (uint64_t)tpd);
+}
+
+/* F64ToI128 / I128ToF64 of src/plan/physical_plan.rs:496-523: round half away
+ * from zero as trunc(|v| + 0.5), NaN -> 0, saturation at 2^127; the i128 as
+ * (double)hi * 2^64 + (double)lo */
+static __int128 x_f2d(double v) {
+  if (isnan(v)) return 0;
+  double a = fabs(v);
+  if (a >= ldexp(1.0, 127)) return v < 0 ? (__int128)((unsigned __int128)1 << 127) : (__int128)(((unsigned __int128)1 << 127) - 1);
+  double m = trunc(a + 0.5); /* below 2^127, where adding 0.5 cannot carry past it */
+  __int128 r = (__int128)m;  /* an integer below 2^127: exact */
+  return v < 0 ? -r : r;
+}
+static double x_d2f(__int128 x) {
+  volatile double hi = (double)(int64_t)(x >> 64) * 18446744073709551616.0;
+  volatile double lo = (double)(uint64_t)x;
+  return hi + lo;
+}
 
 #define XREGS 256
 
@@ -910,11 +981,35 @@ static int64_t x_f2i(double x, int to) {
   return (int64_t)x;
 }
 
-static int x_cast(xval* r, const xval* a, int to) {
+/* sf / st: the source / target scales when they are Decimal128 (physical_plan.rs:2046-2060) */
+static int x_cast(xval* r, const xval* a, int to, int sf, int st) {
   int from = a->dt;
-  xval v = {to, 0, 0.0, 0};
+  xval v = {to, 0, 0.0, 0, 0};
+  if (from == BO_DEC128 || to == BO_DEC128) {
+    if (sf < 0 || sf > 38 || st < 0 || st > 38) return BO_ERR_TYPE;
+    if (from == BO_DEC128 && to == BO_DEC128) {
+      if (st >= sf) v.d = (__int128)((unsigned __int128)a->d * (unsigned __int128)x_pow10x(st - sf));
+      else v.d = a->d / x_pow10x(sf - st); /* truncates toward zero */
+    } else if (to == BO_DEC128 && (from == BO_I32 || from == BO_I64 || from == BO_DATE32)) {
+      v.d = (__int128)((unsigned __int128)(__int128)a->i * (unsigned __int128)x_pow10x(st));
+    } else if (from == BO_DEC128 && (to == BO_I32 || to == BO_I64)) {
+      __int128 q = sf > 0 ? a->d / x_pow10x(sf) : a->d;
+      v.i = x_wrap(to, (uint64_t)(unsigned __int128)q); /* the low 64 / 32 bits */
+    } else if (to == BO_DEC128 && x_float(from)) {
+      if (st > 22) return BO_ERR_TYPE;
+      v.d = x_f2d(a->f * x_pow10f(st));
+    } else if (from == BO_DEC128 && x_float(to)) {
+      if (sf > 22) return BO_ERR_TYPE;
+      v.f = x_d2f(a->d) / x_pow10f(sf);
+      if (to == BO_F32) v.f = (double)(float)v.f;
+    } else {
+      return BO_ERR_TYPE;
+    }
+    *r = v;
+    return BO_OK;
+  }
   if (from == to) { *r = *a; return BO_OK; }
-  if (from == BO_DEC128 || to == BO_DEC128 || from == BO_DATE32 || to == BO_DATE32) return BO_ERR_TYPE;
+  if (from == BO_DATE32 || to == BO_DATE32 || from == BO_TIMESTAMP || to == BO_TIMESTAMP) return BO_ERR_TYPE;
   if (to == BO_BOOL) {
     if (x_float(from)) v.i = (a->f < 0.0 || a->f > 0.0); /* ordered != 0: NaN -> 0 */
     else v.i = a->i != 0;
@@ -931,8 +1026,9 @@ static int x_cast(xval* r, const xval* a, int to) {
 }
 
 static int x_binary(xval* r, const xval* a, const xval* b, int op, int dt) {
-  xval v = {dt, 0, 0.0, 0};
+  xval v = {dt, 0, 0.0, 0, 0};
   if (a->dt != dt || b->dt != dt) return BO_ERR_TYPE;
+  if (dt == BO_TIMESTAMP && (!(op >= B_EQ && op <= B_GE) || a->u != b->u)) return BO_ERR_TYPE; /* compare only */
   if (op >= B_EQ && op <= B_GE) {
     int c;
     v.dt = BO_BOOL;
@@ -1000,11 +1096,12 @@ static int x_binary(xval* r, const xval* a, const xval* b, int op, int dt) {
 }
 
 static xval x_load(const bo_col* c, int64_t i) {
-  xval v = {c->dtype, 0, 0.0, 0};
+  xval v = {c->dtype, 0, 0.0, 0, 0};
   switch (c->dtype) {
     case BO_BOOL: v.i = ((const uint8_t*)c->data)[i] != 0; break;
     case BO_I32: case BO_DATE32: v.i = ((const int32_t*)c->data)[i]; break;
     case BO_I64: v.i = ((const int64_t*)c->data)[i]; break;
+    case BO_TIMESTAMP: v.i = ((const int64_t*)c->data)[i]; v.u = c->scale; break;
     case BO_F32: v.f = ((const float*)c->data)[i]; break;
     case BO_F64: v.f = ((const double*)c->data)[i]; break;
     case BO_DEC128: {
@@ -1020,7 +1117,7 @@ static void x_store(bo_col* c, int64_t i, const xval* v) {
   switch (c->dtype) {
     case BO_BOOL: ((uint8_t*)c->data)[i] = (uint8_t)v->i; break;
     case BO_I32: case BO_DATE32: ((int32_t*)c->data)[i] = (int32_t)v->i; break;
-    case BO_I64: ((int64_t*)c->data)[i] = v->i; break;
+    case BO_I64: case BO_TIMESTAMP: ((int64_t*)c->data)[i] = v->i; break;
     case BO_F32: ((float*)c->data)[i] = (float)v->f; break;
     case BO_F64: ((double*)c->data)[i] = v->f; break;
     case BO_DEC128: {
@@ -1049,20 +1146,50 @@ int32_t bo_project(const bo_col* inputs, int32_t n_inputs, const bo_expr_op* pro
           deps[o->dst] = 1u << o->col;
           break;
         case X_CONST: {
-          xval v = {o->dtype, 0, 0.0, 0};
+          xval v = {o->dtype, 0, 0.0, 0, 0};
           if (o->dtype == BO_F32) v.f = (float)o->lit_f64;
           else if (o->dtype == BO_F64) v.f = o->lit_f64;
           else if (o->dtype == BO_DEC128) v.d = (__int128)(((unsigned __int128)(uint64_t)o->lit_hi << 64) | (uint64_t)o->lit_i64);
           else if (o->dtype == BO_BOOL) v.i = o->lit_i64 != 0;
+          else if (o->dtype == BO_TIMESTAMP) { v.i = o->lit_i64; v.u = o->op; }
           else v.i = x_wrap(o->dtype == BO_I64 ? BO_I64 : BO_I32, (uint64_t)o->lit_i64);
           *d = v;
           deps[o->dst] = 0;
           break;
         }
-        case X_CAST: st = x_cast(d, &r[o->a], o->dtype); deps[o->dst] = deps[o->a]; break;
+        case X_CAST: {
+          const int dec_src = r[o->a].dt == BO_DEC128, dec_dst = o->dtype == BO_DEC128;
+          if ((dec_src || dec_dst) && (r[o->a].dt == BO_BOOL || r[o->a].dt == BO_TIMESTAMP || o->dtype == BO_BOOL ||
+                                       o->dtype == BO_DATE32 || o->dtype == BO_TIMESTAMP)) { st = BO_ERR_TYPE; break; }
+          const int u = r[o->a].u;
+          st = x_cast(d, &r[o->a], o->dtype, dec_src ? o->c : 0, dec_dst ? o->op : 0);
+          d->u = u;
+          deps[o->dst] = deps[o->a];
+          break;
+        }
+        case X_EXTRACT:
+        case X_DATE_TRUNC: {
+          const xval* a = &r[o->a];
+          if ((a->dt != BO_DATE32 && a->dt != BO_TIMESTAMP) || o->op < DF_YEAR || o->op > DF_SECOND ||
+              (a->dt == BO_DATE32 && o->op >= DF_HOUR)) { st = BO_ERR_TYPE; break; }
+          const int ts = a->dt == BO_TIMESTAMP;
+          xval v = {BO_I64, 0, 0.0, 0, 0};
+          if (o->kind == X_EXTRACT) {
+            v.i = x_extract(a->i, ts, a->u, o->op);
+          } else {
+            v.dt = a->dt;
+            v.u = a->u;
+            v.i = x_trunc(a->i, ts, a->u, o->op);
+            if (!ts) v.i = x_wrap(BO_I32, (uint64_t)v.i);
+          }
+          *d = v;
+          deps[o->dst] = deps[o->a];
+          break;
+        }
         case X_BINARY: st = x_binary(d, &r[o->a], &r[o->b], o->op, o->dtype); deps[o->dst] = deps[o->a] | deps[o->b]; break;
         case X_SELECT:
-          if (r[o->a].dt != BO_BOOL || r[o->b].dt != r[o->c].dt) { st = BO_ERR_TYPE; break; }
+          if (r[o->a].dt != BO_BOOL || r[o->b].dt != r[o->c].dt ||
+              (r[o->b].dt == BO_TIMESTAMP && r[o->b].u != r[o->c].u)) { st = BO_ERR_TYPE; break; }
           *d = r[o->a].i ? r[o->b] : r[o->c];
           deps[o->dst] = deps[o->a] | deps[o->b] | deps[o->c];
           break;
@@ -1081,7 +1208,7 @@ int32_t bo_project(const bo_col* inputs, int32_t n_inputs, const bo_expr_op* pro
         }
         case X_STORE: {
           bo_col* c = &outputs[o->col];
-          if (c->dtype != r[o->a].dt) { st = BO_ERR_TYPE; break; }
+          if (c->dtype != r[o->a].dt || (c->dtype == BO_TIMESTAMP && c->scale != r[o->a].u)) { st = BO_ERR_TYPE; break; }
           x_store(c, i, &r[o->a]);
           if (c->validity) {
             int ok = 1;

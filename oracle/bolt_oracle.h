@@ -21,7 +21,7 @@ extern "C" {
 #endif
 
 /* dtype codes (same values as the product ABI's bolt_dtype) */
-enum { BO_BOOL = 1, BO_I32 = 2, BO_I64 = 3, BO_F32 = 4, BO_F64 = 5, BO_DEC128 = 6, BO_DATE32 = 7 };
+enum { BO_BOOL = 1, BO_I32 = 2, BO_I64 = 3, BO_F32 = 4, BO_F64 = 5, BO_DEC128 = 6, BO_DATE32 = 7, BO_TIMESTAMP = 8 };
 /* aggregate codes (same values as the product ABI's bolt_agg_op) */
 enum {
   BO_SUM = 1, BO_COUNT = 2, BO_COUNT_STAR = 3, BO_MIN = 4, BO_MAX = 5, BO_AVG = 6,

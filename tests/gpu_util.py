@@ -7,7 +7,7 @@ from craton_bolt_amd import _native as N
 from craton_bolt_amd.device import DeviceColumn
 
 _TO_ORACLE = {N.I32: O.I32, N.I64: O.I64, N.F32: O.F32, N.F64: O.F64, N.DEC128: O.DEC128, N.DATE32: O.DATE32,
-              N.BOOL: O.BOOL}
+              N.BOOL: O.BOOL, N.TIMESTAMP: O.TIMESTAMP}
 
 
 def both(values: np.ndarray, dtype: int, valid=None, scale: int = 0):
