@@ -262,12 +262,30 @@ _SIGS = {
                                C.POINTER(BoltColumn), C.c_int32]),
     "bolt_aggregate_expr": (C.c_int, [C.c_void_p, C.POINTER(BoltColumn), C.c_int32, C.POINTER(BoltExprOp), C.c_int32,
                                       C.c_int32, C.c_int32, C.POINTER(BoltAgg), C.c_int32, C.POINTER(BoltScalar)]),
+    "bolt_groupby_expr": (C.c_int, [C.c_void_p, C.POINTER(BoltColumn), C.c_int32, C.POINTER(BoltExprOp), C.c_int32,
+                                    C.c_int32, C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32), C.c_int32,
+                                    C.POINTER(BoltAgg), C.c_int32, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "bolt_join_probe_aggregate_expr": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BoltColumn), C.c_int32,
+                                                 C.POINTER(BoltExprOp), C.c_int32, C.c_int32, C.POINTER(C.c_int32),
+                                                 C.c_int32, C.c_int32, C.POINTER(BoltAgg), C.c_int32,
+                                                 C.POINTER(BoltScalar)]),
     "bolt_take": (C.c_int, [C.c_void_p, C.POINTER(BoltColumn), C.c_void_p, C.c_int64, C.POINTER(BoltColumn)]),
     "bolt_groupby": (C.c_int, [C.c_void_p, C.POINTER(BoltColumn), C.c_int32, C.POINTER(BoltColumn), C.c_int32,
                                C.POINTER(BoltAgg), C.c_int32, C.POINTER(BoltFilter), C.c_uint32,
                                C.POINTER(C.c_void_p)]),
     "bolt_groupby_last_plan": (C.c_int, [C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                          C.POINTER(C.c_int32)]),
+    # Arrow C Data Interface ingestion (bolt_arrow.h)
+    "bolt_arrow_format_dtype": (C.c_int, [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "bolt_gpu_table_from_arrow": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "bolt_gpu_table_append_arrow": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "bolt_gpu_table_reserve": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p]),
+    "bolt_gpu_table_num_rows": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "bolt_gpu_table_num_columns": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    "bolt_gpu_table_column": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(BoltColumn)]),
+    "bolt_gpu_table_column_name": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_char_p), C.POINTER(C.c_char_p)]),
+    "bolt_gpu_table_release": (C.c_int, [C.c_void_p]),
+    "bolt_column_to_arrow": (C.c_int, [C.c_void_p, C.POINTER(BoltColumn), C.c_char_p, C.c_void_p, C.c_void_p]),
     "bolt_table_num_rows": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "bolt_table_num_columns": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "bolt_table_column": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(BoltColumn)]),

@@ -326,7 +326,7 @@ int binary_expr(int op, int dt, const std::string& a, const std::string& b, std:
 // STORE, the register, dtype and input dependencies of the stored value.
 struct Compiled {
   std::string body;
-  std::vector<int> reg, dtype;
+  std::vector<int> reg, dtype, unit;
   std::vector<uint32_t> deps;
 };
 
@@ -339,6 +339,7 @@ int compile_program(const bolt_column_t* in, int n_in, const bolt_expr_op_t* pro
   std::vector<RegInfo> regs(BOLT_MAX_REGS);
   cp->reg.assign(n_out, -1);
   cp->dtype.assign(n_out, 0);
+  cp->unit.assign(n_out, 0);
   cp->deps.assign(n_out, 0);
   std::ostringstream body;
   auto use = [&](int r, const char* what) -> int {
@@ -485,6 +486,7 @@ int compile_program(const bolt_column_t* in, int n_in, const bolt_expr_op_t* pro
                            out[o.col].scale, regs[o.a].unit);
         cp->reg[o.col] = o.a;
         cp->dtype[o.col] = regs[o.a].dtype;
+        cp->unit[o.col] = regs[o.a].unit;
         cp->deps[o.col] = regs[o.a].deps;
         continue;
       }
@@ -733,4 +735,100 @@ extern "C" int bolt_aggregate_expr(bolt_stream_t stream, const bolt_column_t* in
   BOLT_HIP(hipStreamSynchronize(s));
   for (int i = 0; i < n_aggs; i++) BOLT_TRY(acc_finalize(host[i], all.a[i].op, all.a[i].dtype, &results[i]));
   return BOLT_OK;
+}
+
+// ---- programs feeding GROUP BY and the join probe ----------------------------
+
+namespace {
+
+// The pre stage of groupby_with_pre.rs / agg_with_pre.rs: the program's
+// n_values values materialised into pooled device columns (a value gets a
+// validity bitmap when an input it loads has one), `filter_col` checked to
+// be a Bool value.  Timestamp values keep the program's unit; Decimal128
+// values take value_scales[v] (0 without it).
+struct PreValues {
+  std::vector<bolt_column_t> cols;
+  std::vector<DevBuf<uint8_t>> bufs;
+};
+
+int run_pre(hipStream_t s, const char* who, const bolt_column_t* inputs, int32_t n_inputs,
+            const bolt_expr_op_t* prog, int32_t n_ops, int32_t n_values, const int32_t* value_scales,
+            int32_t filter_col, PreValues* pv) {
+  if (n_inputs < 1 || n_inputs > BOLT_MAX_PROJECT_INPUTS || inputs == nullptr)
+    return set_error(BOLT_ERR_PLAN, "%s: 1 to %d inputs", who, BOLT_MAX_PROJECT_INPUTS);
+  if (n_values < 1 || n_values > BOLT_MAX_PROJECT_OUTPUTS)
+    return set_error(BOLT_ERR_PLAN, "%s: 1 to %d program values", who, BOLT_MAX_PROJECT_OUTPUTS);
+  if (n_ops < 1 || prog == nullptr) return set_error(BOLT_ERR_PLAN, "%s: empty program", who);
+  const int64_t n = inputs[0].length;
+  for (int c = 0; c < n_inputs; c++)
+    if (inputs[c].length != n)
+      return set_error(BOLT_ERR_PLAN, "%s: input %d has %lld rows, input 0 has %lld", who, c,
+                       (long long)inputs[c].length, (long long)n);
+  Compiled cp;
+  BOLT_TRY(compile_program(inputs, n_inputs, prog, n_ops, nullptr, n_values, &cp));
+  if (filter_col >= n_values || (filter_col >= 0 && cp.dtype[filter_col] != BOLT_BOOL))
+    return set_error(BOLT_ERR_PLAN, "%s: filter value %d is not a Bool value of the program", who, filter_col);
+  pv->cols.assign(n_values, bolt_column_t{});
+  pv->bufs.resize(2 * (size_t)n_values);
+  for (int v = 0; v < n_values; v++) {
+    bolt_column_t& c = pv->cols[v];
+    c.dtype = cp.dtype[v];
+    c.length = n;
+    c.scale = c.dtype == BOLT_TIMESTAMP ? cp.unit[v] : c.dtype == BOLT_DEC128 && value_scales ? value_scales[v] : 0;
+    BOLT_TRY(pv->bufs[2 * v].alloc((size_t)(n ? n : 1) * dtype_size(c.dtype), s));
+    c.data = pv->bufs[2 * v].ptr;
+    bool nullable = false;
+    for (int i = 0; i < n_inputs; i++) nullable |= ((cp.deps[v] >> i) & 1u) && inputs[i].validity;
+    if (nullable) {
+      BOLT_TRY(pv->bufs[2 * v + 1].alloc((size_t)(n + 7) / 8 + 8, s));
+      c.validity = pv->bufs[2 * v + 1].ptr;
+    }
+  }
+  return bolt_project((bolt_stream_t)s, inputs, n_inputs, prog, n_ops, pv->cols.data(), n_values);
+}
+
+}  // namespace
+
+extern "C" int bolt_groupby_expr(bolt_stream_t stream, const bolt_column_t* inputs, int32_t n_inputs,
+                                 const bolt_expr_op_t* prog, int32_t n_ops, int32_t n_values,
+                                 const int32_t* value_scales, int32_t filter_col, const int32_t* key_values,
+                                 int32_t n_keys, const bolt_agg_t* aggs, int32_t n_aggs, uint32_t flags,
+                                 bolt_table_t** out) {
+  BOLT_CHECK_DEVICE();
+  clear_error();
+  if (!out) return set_error(BOLT_ERR_PLAN, "bolt_groupby_expr: null output");
+  *out = nullptr;
+  if (n_keys < 1 || n_keys > 2 || key_values == nullptr)
+    return set_error(BOLT_ERR_PLAN, "bolt_groupby_expr: 1 or 2 key values");
+  for (int k = 0; k < n_keys; k++)
+    if (key_values[k] < 0 || key_values[k] >= n_values)
+      return set_error(BOLT_ERR_PLAN, "bolt_groupby_expr: key value %d of %d", key_values[k], n_values);
+  hipStream_t s = (hipStream_t)stream;
+  PreValues pv;
+  BOLT_TRY(run_pre(s, "bolt_groupby_expr", inputs, n_inputs, prog, n_ops, n_values, value_scales, filter_col, &pv));
+  bolt_column_t keys[2];
+  for (int k = 0; k < n_keys; k++) keys[k] = pv.cols[key_values[k]];
+  const bolt_pred_term_t keep = {filter_col, BOLT_CMP_EQ, 1, 0.0, 0};  // true and not NULL
+  const bolt_filter_t f = {&keep, 1};
+  return bolt_groupby(stream, keys, n_keys, pv.cols.data(), n_values, aggs, n_aggs, filter_col >= 0 ? &f : nullptr,
+                      flags, out);
+}
+
+extern "C" int bolt_join_probe_aggregate_expr(bolt_stream_t stream, const bolt_join_table_t* t,
+                                              const bolt_column_t* inputs, int32_t n_inputs,
+                                              const bolt_expr_op_t* prog, int32_t n_ops, int32_t n_values,
+                                              const int32_t* value_scales, int32_t key_value, int32_t filter_col,
+                                              const bolt_agg_t* aggs, int32_t n_aggs, bolt_scalar_t* results) {
+  BOLT_CHECK_DEVICE();
+  clear_error();
+  if (key_value < 0 || key_value >= n_values)
+    return set_error(BOLT_ERR_PLAN, "bolt_join_probe_aggregate_expr: key value %d of %d", key_value, n_values);
+  hipStream_t s = (hipStream_t)stream;
+  PreValues pv;
+  BOLT_TRY(run_pre(s, "bolt_join_probe_aggregate_expr", inputs, n_inputs, prog, n_ops, n_values, value_scales,
+                   filter_col, &pv));
+  const bolt_pred_term_t keep = {filter_col, BOLT_CMP_EQ, 1, 0.0, 0};
+  const bolt_filter_t f = {&keep, 1};
+  return bolt_join_probe_aggregate(stream, t, pv.cols.data(), n_values, key_value, filter_col >= 0 ? &f : nullptr,
+                                   aggs, n_aggs, results);
 }

@@ -127,6 +127,11 @@ def execute_groupby(keys: list[DeviceColumn], inputs: list[DeviceColumn], aggs: 
     st = N.lib().bolt_groupby(_stream(stream), ka, len(keys), ia, len(inputs), _aggs(aggs), len(aggs),
                               C.byref(f) if f else None, fl, C.byref(tbl))
     N.check(st, "bolt_groupby")
+    return _table_result(tbl, len(keys))
+
+
+def _table_result(tbl: C.c_void_p, n_keys: int) -> dict:
+    """Downloads and releases a GROUP BY result table."""
     try:
         nr = C.c_int64()
         nc = C.c_int32()
@@ -139,7 +144,7 @@ def execute_groupby(keys: list[DeviceColumn], inputs: list[DeviceColumn], aggs: 
             cols.append(_download(bc, nr.value))
     finally:
         N.lib().bolt_table_release(tbl)
-    return {"n_groups": nr.value, "keys": cols[: len(keys)], "aggs": cols[len(keys):]}
+    return {"n_groups": nr.value, "keys": cols[:n_keys], "aggs": cols[n_keys:]}
 
 
 def _download(bc: N.BoltColumn, n: int):
@@ -498,6 +503,19 @@ class JoinTable:
         N.check(st, "bolt_join_probe_aggregate")
         return [scalar_value(res[i]) for i in range(len(aggs))]
 
+    def probe_aggregate_expr(self, inputs: list[DeviceColumn], program: "Program", n_values: int, key_value: int,
+                             aggs: list[tuple[int, int]], filter_value: int = -1,
+                             value_scales: list[int] | None = None) -> list:
+        """probe_aggregate with the probe key, predicate and aggregate inputs computed by `program` over
+        `inputs` (bolt_join_probe_aggregate_expr)."""
+        res = (N.BoltScalar * max(len(aggs), 1))()
+        st = N.lib().bolt_join_probe_aggregate_expr(_stream(self.stream), self._h, columns_array(inputs), len(inputs),
+                                                    program.array(N.BoltExprOp), len(program.ops), n_values,
+                                                    _i32_array(value_scales), key_value, filter_value, _aggs(aggs),
+                                                    len(aggs), res)
+        N.check(st, "bolt_join_probe_aggregate_expr")
+        return [scalar_value(res[i]) for i in range(len(aggs))]
+
     def release(self) -> None:
         if self._h:
             N.lib().bolt_join_table_release(self._h)
@@ -632,6 +650,24 @@ def execute_aggregate_expr(inputs: list[DeviceColumn], program: Program, n_value
                                      len(program.ops), n_values, filter_value, _aggs(aggs), len(aggs), res)
     N.check(st, "bolt_aggregate_expr")
     return [scalar_value(res[i]) for i in range(len(aggs))]
+
+
+def _i32_array(xs):
+    return (C.c_int32 * len(xs))(*xs) if xs else None
+
+
+def execute_groupby_expr(inputs: list[DeviceColumn], program: Program, n_values: int, key_values: list[int],
+                         aggs: list[tuple[int, int]], filter_value: int = -1, value_scales: list[int] | None = None,
+                         sorted_output: bool = True, flags: int = 0, stream: Stream | None = None) -> dict:
+    """GROUP BY the program values `key_values`, aggregating values (each aggregate's col is a value index),
+    over the rows where value `filter_value` is true (bolt_groupby_expr).  Same result as execute_groupby."""
+    tbl = C.c_void_p()
+    fl = flags | (N.GROUPBY_SORTED if sorted_output else 0)
+    st = N.lib().bolt_groupby_expr(_stream(stream), columns_array(inputs), len(inputs), program.array(N.BoltExprOp),
+                                   len(program.ops), n_values, _i32_array(value_scales), filter_value,
+                                   _i32_array(key_values), len(key_values), _aggs(aggs), len(aggs), fl, C.byref(tbl))
+    N.check(st, "bolt_groupby_expr")
+    return _table_result(tbl, len(key_values))
 
 
 JOIN_TYPES = {"inner": 0, "left": 1, "right": 2, "full": 3}

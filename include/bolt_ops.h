@@ -448,6 +448,37 @@ int bolt_aggregate_expr(bolt_stream_t stream, const bolt_column_t* inputs, int32
                         const bolt_expr_op_t* prog, int32_t n_ops, int32_t n_values, int32_t filter_col,
                         const bolt_agg_t* aggs, int32_t n_aggs, bolt_scalar_t* results);
 
+/* GROUP BY over an expression program — replaces execute_groupby_with_pre
+ * (src/exec/groupby_with_pre.rs:222), whose pre kernel materialises the
+ * program's outputs, compacts them on the host by the predicate and then
+ * groups bare columns.  Here the program's n_values values (its STOREs, as
+ * in bolt_aggregate_expr) are materialised once into pooled device columns
+ * (a value is NULL where an input column it loads is NULL), the Bool value
+ * `filter_col` (or -1) becomes the GROUP BY's row filter (a row is kept
+ * where it is true and not NULL: nothing is compacted), and bolt_groupby
+ * runs with key_values (1 or 2 value indexes) as its keys and the values as
+ * its inputs (each aggregate's `col` is a value index).  value_scales
+ * (optional, n_values entries) gives each Decimal128 value's scale for the
+ * result; Timestamp values keep the program's unit.  Result, flags, DECLINE
+ * and errors as bolt_groupby. */
+int bolt_groupby_expr(bolt_stream_t stream, const bolt_column_t* inputs, int32_t n_inputs,
+                      const bolt_expr_op_t* prog, int32_t n_ops, int32_t n_values, const int32_t* value_scales,
+                      int32_t filter_col, const int32_t* key_values, int32_t n_keys, const bolt_agg_t* aggs,
+                      int32_t n_aggs, uint32_t flags, bolt_table_t** out);
+
+/* The fused join probe over an expression program: the probe side's key,
+ * predicate and aggregate inputs are values of `prog` over the probe
+ * columns `inputs` (the pre stage of agg_with_pre.rs:157 feeding the
+ * aggregate-over-join composition, src/plan/physical_plan.rs:3691-3712).
+ * The values are materialised once as in bolt_groupby_expr; key_value is the
+ * probe key value, filter_col (or -1) a Bool value, and each aggregate's
+ * `col` a value index or BOLT_JOIN_CARRY.  Results and errors as
+ * bolt_join_probe_aggregate. */
+int bolt_join_probe_aggregate_expr(bolt_stream_t stream, const bolt_join_table_t* t, const bolt_column_t* inputs,
+                                   int32_t n_inputs, const bolt_expr_op_t* prog, int32_t n_ops, int32_t n_values,
+                                   const int32_t* value_scales, int32_t key_value, int32_t filter_col,
+                                   const bolt_agg_t* aggs, int32_t n_aggs, bolt_scalar_t* results);
+
 /* ------------------------------------------------------------------------
  * Library information
  * ------------------------------------------------------------------------ */
