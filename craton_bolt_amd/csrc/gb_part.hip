@@ -198,7 +198,7 @@ __global__ __launch_bounds__(kPartBlock) void part_scan_kernel(const uint32_t* _
 }
 
 // Merge records, one per item: the key then the W accumulator words, `stride`
-// 64-bit words apart (a multiple of two, so a record is whole 16-byte units).
+// = 1 + W 64-bit words apart.
 struct PartRecs {
   unsigned long long* rec;
   int64_t cap;
@@ -350,13 +350,16 @@ __global__ __launch_bounds__(kStageBlock) void part_stage_kernel(PartSrc src, Ke
       }
     }
     __syncthreads();
+    // write the staged records out word by word, consecutive lanes on
+    // consecutive words, so each digit's run is one contiguous stream of
+    // stores whatever the record width
     const uint32_t total = pre[63] + cnt[63];
-    for (uint32_t j = threadIdx.x; j < total; j += kStageBlock) {
+    const uint32_t nwords = total * (uint32_t)stride;
+    for (uint32_t u = threadIdx.x; u < nwords; u += kStageBlock) {
+      const uint32_t j = u / (uint32_t)stride, w = u - j * (uint32_t)stride;
       const uint32_t d = sdig[j];
       const uint64_t pos = gbase[d] + (j - pre[d]);
-      const uint4* a = reinterpret_cast<const uint4*>(stage + (size_t)j * stride);
-      uint4* b = reinterpret_cast<uint4*>(out + pos * (uint64_t)stride);
-      for (int w = 0; w < stride / 2; w++) b[w] = a[w];
+      out[pos * (uint64_t)stride + w] = stage[u];
     }
     __syncthreads();
   }
@@ -465,7 +468,7 @@ int gb_part_build(hipStream_t s, const KeySpec& ks, const GPlan& P, const unsign
   BOLT_TRY(fill_u32(total.ptr, 0, (size_t)np, s));
   part_hist_kernel<<<nb, kPartBlock, 0, s>>>(src, ks, pbits, chunk, out->pid.ptr, out->hist.ptr, total.ptr);
   part_scan_kernel<<<1, kPartBlock, 0, s>>>(total.ptr, np, out->start.ptr, out->cursor.ptr);
-  out->stride = (1 + P.words + 1) & ~1;  // key + words, rounded up to 16 bytes
+  out->stride = 1 + P.words;  // key + words, unpadded (the staged writes are word streams)
   out->recs_cap = std::max<int64_t>(n_items, 1);
   BOLT_TRY(out->rec.alloc((size_t)out->recs_cap * out->stride, s));
   if (pbits > 6) BOLT_TRY(out->coarse.alloc((size_t)out->recs_cap * out->stride, s));
