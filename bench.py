@@ -957,13 +957,16 @@ def sub_zipf(args) -> dict:
     from craton_bolt_amd import _native as N
     from craton_bolt_amd import datagen as D
     from craton_bolt_amd.device import DeviceColumn
-    from craton_bolt_amd.ops import execute_groupby
+    from craton_bolt_amd.ops import autotune_groupby_partitions, execute_groupby, groupby_last_plan
 
     n = args.n
     k, v = D.zipf_groupby(n, n_ranks=10_000_000)
     dk, dv = DeviceColumn.from_numpy(k), DeviceColumn.from_numpy(v)
     aggs = [(N.SUM, 0), (N.COUNT_STAR, 0)]
+    # the partitioned tier's fan-out, tuned on this input before the timed calls (which then use it)
+    tuned_parts, tuned_ms = autotune_groupby_partitions([dk], [dv], aggs, reps=1)
     op_ms, k_ms, g = _time_groupby(dk, dv, aggs, 0, args.steps, args.warmup)
+    plan = groupby_last_plan()
     check, ferr = None, None
     if not args.no_check:
         res = execute_groupby([dk], [dv], aggs, sorted_output=False)
@@ -1003,7 +1006,10 @@ def sub_zipf(args) -> dict:
     byts = n * 16 + g * (8 + 8 + 8)
     return _sub(n / (op_ms * 1e-3), "rows/s", op_ms, args.steps, "f64",
                 {"workload": f"high-cardinality GROUP BY: {n} rows, int64 key Zipf(1.1) over 10M ranks, "
-                             f"SUM(float64), COUNT(*)", "n": n, "groups": g},
+                             f"SUM(float64), COUNT(*)", "n": n, "groups": g, "tier": plan["tier"],
+                 "partitions": plan["partitions"], "spilled_rows": plan["spilled_rows"],
+                 "partial_records": plan["partial_records"],
+                 "autotuned_partitions": {"chosen": tuned_parts, "ms_per_call_while_tuning": tuned_ms}},
                 _op_roof(byts, op_ms, "whole GROUP BY call (LDS tier, partitioned tier, extraction)",
                          n * 16, k_ms, "generated LDS-tier pass"), check, ferr, cpu)
 

@@ -275,6 +275,9 @@ _SIGS = {
                                C.POINTER(C.c_void_p)]),
     "bolt_groupby_last_plan": (C.c_int, [C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                          C.POINTER(C.c_int32)]),
+    "bolt_groupby_autotune_partitions": (C.c_int, [C.c_void_p, C.POINTER(BoltColumn), C.c_int32,
+                                                   C.POINTER(BoltColumn), C.c_int32, C.POINTER(BoltAgg), C.c_int32,
+                                                   C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_float)]),
     # Arrow C Data Interface ingestion (bolt_arrow.h)
     "bolt_arrow_format_dtype": (C.c_int, [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "bolt_gpu_table_from_arrow": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),

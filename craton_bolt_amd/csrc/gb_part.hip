@@ -442,15 +442,21 @@ __global__ void part_reset_cursor_kernel(const uint64_t* __restrict__ start, uns
 
 // ---- host ---------------------------------------------------------------------------
 
-int gb_part_build(hipStream_t s, const KeySpec& ks, const GPlan& P, const unsigned long long* pkeys,
-                  const uint8_t* ptag, const unsigned long long* pwords, int64_t pcap, int64_t n_parts,
-                  const uint32_t* spill, int64_t spill_per, const uint32_t* spill_seg, int nseg, int64_t n_spill,
-                  double est, int S, GbPartitions* out) {
-  const int64_t n_items = n_parts + n_spill;
-  // partitions of about S/2 groups each, so one LDS table holds a partition
+// The fan-out: partitions of about S/2 groups each, so one LDS table holds a
+// partition, moved by `bits_delta` (the autotuner's choice), in 0 .. 12 bits.
+int gb_part_bits(double est, int S, int bits_delta) {
   const double want = std::max(1.0, est / std::max(1.0, 0.5 * S));
   int pbits = 0;
   while ((1 << pbits) < want && pbits < kPartMaxBits) pbits++;
+  return std::min(kPartMaxBits, std::max(0, pbits + bits_delta));
+}
+
+int gb_part_build(hipStream_t s, const KeySpec& ks, const GPlan& P, const unsigned long long* pkeys,
+                  const uint8_t* ptag, const unsigned long long* pwords, int64_t pcap, int64_t n_parts,
+                  const uint32_t* spill, int64_t spill_per, const uint32_t* spill_seg, int nseg, int64_t n_spill,
+                  double est, int S, int bits_delta, GbPartitions* out) {
+  const int64_t n_items = n_parts + n_spill;
+  int pbits = gb_part_bits(est, S, bits_delta);
   const int np = 1 << pbits;
   out->np = np;
   BOLT_TRY(out->dense.alloc((size_t)std::max<int64_t>(n_spill, 1), s));

@@ -38,8 +38,11 @@
 // restriction: a key equal to the empty marker and the NULL group live in two
 // dedicated slots after the table.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "groupby_dev.h"
@@ -365,8 +368,30 @@ struct GbLastPlan {
   int32_t tier;  // 0 LDS tier only, 1 global table, 2 partitioned tier
   int64_t spilled, partials;
   int32_t partitions;
+  int32_t est_bucket, words;  // the fan-out tuning key
 };
-static thread_local GbLastPlan t_last_plan{0, 0, 0, 0};
+static thread_local GbLastPlan t_last_plan{0, 0, 0, 0, 0, 0};
+
+// ---- the partitioned tier's fan-out ---------------------------------------------
+// bolt_groupby_autotune_partitions times the partition counts around the
+// default (gb_part_bits) and keeps the fastest per (log2 of the estimated
+// group count, accumulator words) for the rest of the process;
+// BOLT_GB_PART_BITS_DELTA overrides.  While it times a candidate,
+// t_force_delta holds it.
+std::mutex g_part_mu;
+std::map<std::pair<int, int>, int> g_part_delta;
+static thread_local int t_force_delta = 0;
+static thread_local bool t_forcing = false;
+
+int est_bucket(double est) { return est < 1 ? 0 : (int)std::floor(std::log2(est)); }
+
+int part_bits_delta(double est, int words) {
+  if (t_forcing) return t_force_delta;
+  if (const char* e = getenv("BOLT_GB_PART_BITS_DELTA")) return std::max(-12, std::min(12, atoi(e)));
+  std::lock_guard<std::mutex> lk(g_part_mu);
+  const auto it = g_part_delta.find({est_bucket(est), words});
+  return it == g_part_delta.end() ? 0 : it->second;
+}
 
 // Gather for the sorted output order: out[i] = in[perm[i]].
 __global__ void gather_kernel(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, int width,
@@ -795,9 +820,9 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
   GbPartitions parts;
   if (partitioned)
     BOLT_TRY(gb_part_build(s, ks, P, pkeys.ptr, ptag.ptr, pwords.ptr, (int64_t)pkeys.count, n_parts, spill.ptr,
-                           spill_per, spill_seg.ptr, (int)grid, n_spill, est, S, &parts));
+                           spill_per, spill_seg.ptr, (int)grid, n_spill, est, S, part_bits_delta(est, W), &parts));
   t_last_plan = GbLastPlan{!use_lds ? 1 : (partitioned ? 2 : (n_spill > 0 ? 1 : 0)), n_spill, n_parts,
-                           partitioned ? parts.np : 0};
+                           partitioned ? parts.np : 0, est_bucket(est), W};
   GTable t;
   DevBuf<unsigned long long> tkeys, twords;
   DevBuf<uint32_t> tmisc;
@@ -1002,5 +1027,53 @@ extern "C" int bolt_groupby_last_plan(int32_t* tier, int64_t* spilled_rows, int6
   if (spilled_rows) *spilled_rows = bolt::t_last_plan.spilled;
   if (partial_records) *partial_records = bolt::t_last_plan.partials;
   if (partitions) *partitions = bolt::t_last_plan.partitions;
+  return BOLT_OK;
+}
+
+extern "C" int bolt_groupby_autotune_partitions(bolt_stream_t stream, const bolt_column_t* keys, int32_t n_keys,
+                                                const bolt_column_t* inputs, int32_t n_inputs, const bolt_agg_t* aggs,
+                                                int32_t n_aggs, int32_t reps, int32_t* best_partitions,
+                                                float* best_ms) {
+  using namespace bolt;
+  if (reps < 1) reps = 1;
+  double best = 1e300;
+  int best_delta = 0, best_np = 0, key_bucket = 0, key_words = 0;
+  int seen_np[5] = {-1, -1, -1, -1, -1};
+  int st = BOLT_OK;
+  for (int d = -2; d <= 2 && st == BOLT_OK; d++) {
+    t_forcing = true;
+    t_force_delta = d;
+    double total = 0;
+    for (int r = 0; r <= reps && st == BOLT_OK; r++) {  // one untimed call first (pool, code objects)
+      bolt_table_t* t = nullptr;
+      const auto t0 = std::chrono::steady_clock::now();
+      st = bolt_groupby(stream, keys, n_keys, inputs, n_inputs, aggs, n_aggs, nullptr,
+                        BOLT_GROUPBY_FORCE_PARTITIONED, &t);
+      const auto t1 = std::chrono::steady_clock::now();
+      if (t) bolt_table_release(t);
+      if (r > 0) total += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    }
+    t_forcing = false;
+    if (st != BOLT_OK) break;
+    const int np = t_last_plan.partitions;
+    bool dup = false;  // a delta that clamps to a partition count already timed
+    for (int k = 0; k < 5; k++) dup |= seen_np[k] == np;
+    seen_np[d + 2] = np;
+    key_bucket = t_last_plan.est_bucket;
+    key_words = t_last_plan.words;
+    if (!dup && total / reps < best) {
+      best = total / reps;
+      best_delta = d;
+      best_np = np;
+    }
+  }
+  t_forcing = false;
+  if (st != BOLT_OK) return st;
+  {
+    std::lock_guard<std::mutex> lk(g_part_mu);
+    g_part_delta[{key_bucket, key_words}] = best_delta;
+  }
+  if (best_partitions) *best_partitions = best_np;
+  if (best_ms) *best_ms = (float)best;
   return BOLT_OK;
 }

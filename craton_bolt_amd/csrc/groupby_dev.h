@@ -371,7 +371,8 @@ struct GbPartitions {
 int gb_part_build(hipStream_t s, const KeySpec& ks, const GPlan& P, const unsigned long long* pkeys,
                   const uint8_t* ptag, const unsigned long long* pwords, int64_t pcap, int64_t n_parts,
                   const uint32_t* spill, int64_t spill_per, const uint32_t* spill_seg, int nseg, int64_t n_spill,
-                  double est, int S, GbPartitions* out);
+                  double est, int S, int bits_delta, GbPartitions* out);
+int gb_part_bits(double est, int S, int bits_delta);
 int gb_part_reduce(hipStream_t s, const GTable& t, const GPlan& P, GbPartitions& parts, int S);
 uint64_t next_pow2(uint64_t x);
 

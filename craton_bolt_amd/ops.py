@@ -130,6 +130,26 @@ def execute_groupby(keys: list[DeviceColumn], inputs: list[DeviceColumn], aggs: 
     return _table_result(tbl, len(keys))
 
 
+def groupby_last_plan() -> dict:
+    """How the calling thread's last GROUP BY ran (bolt_groupby_last_plan)."""
+    tier, spilled, partials, parts = C.c_int32(), C.c_int64(), C.c_int64(), C.c_int32()
+    N.check(N.lib().bolt_groupby_last_plan(C.byref(tier), C.byref(spilled), C.byref(partials), C.byref(parts)))
+    return {"tier": tier.value, "spilled_rows": spilled.value, "partial_records": partials.value,
+            "partitions": parts.value}
+
+
+def autotune_groupby_partitions(keys: list[DeviceColumn], inputs: list[DeviceColumn], aggs: list[tuple[int, int]],
+                                reps: int = 2, stream: Stream | None = None) -> tuple[int, float]:
+    """Times the partitioned tier's fan-out candidates on these inputs and keeps the fastest for inputs of
+    this shape (bolt_groupby_autotune_partitions); returns (partitions, ms per call)."""
+    parts, ms = C.c_int32(), C.c_float()
+    st = N.lib().bolt_groupby_autotune_partitions(_stream(stream), columns_array(keys), len(keys), columns_array(inputs),
+                                                  len(inputs), _aggs(aggs), len(aggs), reps, C.byref(parts),
+                                                  C.byref(ms))
+    N.check(st, "bolt_groupby_autotune_partitions")
+    return parts.value, float(ms.value)
+
+
 def _table_result(tbl: C.c_void_p, n_keys: int) -> dict:
     """Downloads and releases a GROUP BY result table."""
     try:

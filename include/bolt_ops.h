@@ -220,6 +220,20 @@ int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int32_t n_keys
  * the LDS tier spilled, its partial records, and the partitions (tier 2).
  * Any pointer may be NULL. */
 int bolt_groupby_last_plan(int32_t* tier, int64_t* spilled_rows, int64_t* partial_records, int32_t* partitions);
+/* Autotuner of the radix-partitioned tier's fan-out, the partition count
+ * the reference fixes at 4096 (TIER2 partitions,
+ * src/exec/groupby_tier2_orchestrator.rs:214).  Runs bolt_groupby on these
+ * inputs with the partitioned tier forced, once untimed and `reps` times
+ * timed, for each partition count from a quarter to four times the default
+ * (partitions of about half an LDS table of groups, at most 4096), and keeps
+ * the fastest for the rest of the process for inputs of this shape (the
+ * log2 of the estimated group count and the accumulator words), so later
+ * bolt_groupby calls use it.  Returns the chosen partition count and its
+ * mean time per call (milliseconds, host clock around the synchronising
+ * call).  Errors as bolt_groupby. */
+int bolt_groupby_autotune_partitions(bolt_stream_t stream, const bolt_column_t* keys, int32_t n_keys,
+                                     const bolt_column_t* inputs, int32_t n_inputs, const bolt_agg_t* aggs,
+                                     int32_t n_aggs, int32_t reps, int32_t* best_partitions, float* best_ms);
 int bolt_table_num_rows(const bolt_table_t* t, int64_t* n_rows);
 int bolt_table_num_columns(const bolt_table_t* t, int32_t* n_cols);
 /* Column i of a result table (device pointers owned by the table). */
