@@ -28,6 +28,10 @@ struct PartOut {
   u32_t* spill_seg;     // [grid] rows in each workgroup's segment
   i64_t per;            // segment capacity
   u32_t* hll;           // [kHllRegs]
+  // Optional, for large inputs: the spilled rows' keys beside their row ids,
+  // so the partitioned tier reads them in order instead of gathering them by
+  // row id (the key is in a register when the row spills).
+  u64_t* spill_key;     // [grid][per] or null
 };
 
 // Float keys group -0.0 with +0.0 and every NaN together
@@ -142,7 +146,9 @@ __device__ __forceinline__ void gb_spill(bool spill, i64_t row, u64_t key, u32_t
     base = __shfl(base, leader);
     if (spill) {
       const u64_t below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
-      out.spill[(i64_t)blockIdx.x * out.per + base + (u32_t)__popcll(below)] = (u32_t)row;
+      const i64_t at = (i64_t)blockIdx.x * out.per + base + (u32_t)__popcll(below);
+      out.spill[at] = (u32_t)row;
+      if (out.spill_key) out.spill_key[at] = key;
       hll_add(shll, key);
     }
   }

@@ -109,29 +109,36 @@ __device__ __forceinline__ void item_key(const PartSrc& src, const KeySpec& ks, 
     return;
   }
   uint64_t k = 0;
-  const int kr = row_key(ks, (int64_t)src.rows[i - src.n_parts], &k);
+  int kr = 1;
+  if (src.keys) k = src.keys[i - src.n_parts];
+  else kr = row_key(ks, (int64_t)src.rows[i - src.n_parts], &k);
   // spilled rows carry regular keys; the NULL group is tagged for safety
   *tag = kr == 2 ? 1 : (k == kEmpty ? 2 : 0);
   *key = k;
 }
 
-// Spill segments [b * per, b * per + seg[b]) -> dense[prefix(b) ..]; K
-// workgroups share a segment.
-__global__ __launch_bounds__(256) void part_compact_spill_kernel(const uint32_t* __restrict__ spill, int64_t per,
-                                                                 const uint32_t* __restrict__ seg, int K,
-                                                                 uint32_t* __restrict__ dense) {
-  __shared__ uint64_t sh_off;
+// Spill segments [b * per, b * per + seg[b]) -> dense[prefix(b) ..] (and the
+// captured keys alongside, when there are any); K workgroups share a segment.
+// The segment's offset is a block sum of the counts before it.
+__global__ __launch_bounds__(256) void part_compact_spill_kernel(const uint32_t* __restrict__ spill,
+                                                                 const unsigned long long* __restrict__ skey,
+                                                                 int64_t per, const uint32_t* __restrict__ seg, int K,
+                                                                 uint32_t* __restrict__ dense,
+                                                                 unsigned long long* __restrict__ dkey) {
+  __shared__ uint64_t wsum[4];
   const int b = blockIdx.x / K, k = blockIdx.x % K;
-  if (threadIdx.x == 0) {
-    uint64_t o = 0;
-    for (int j = 0; j < b; j++) o += seg[j];
-    sh_off = o;
-  }
+  uint64_t part = 0;
+  for (int j = threadIdx.x; j < b; j += 256) part += seg[j];
+  for (int d = 32; d >= 1; d >>= 1) part += __shfl_xor(part, d);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = part;
   __syncthreads();
-  const uint64_t off = sh_off;
+  const uint64_t off = wsum[0] + wsum[1] + wsum[2] + wsum[3];
   const uint32_t cnt = seg[b];
-  for (uint32_t i = (uint32_t)k * blockDim.x + threadIdx.x; i < cnt; i += (uint32_t)K * blockDim.x)
-    dense[off + i] = spill[(int64_t)b * per + i];
+  for (uint32_t i = (uint32_t)k * blockDim.x + threadIdx.x; i < cnt; i += (uint32_t)K * blockDim.x) {
+    const int64_t from = (int64_t)b * per + i;
+    dense[off + i] = spill[from];
+    if (skey) dkey[off + i] = skey[from];
+  }
 }
 
 __global__ __launch_bounds__(kPartBlock) void part_hist_kernel(PartSrc src, KeySpec ks, int pbits, int64_t chunk,
@@ -197,6 +204,17 @@ __global__ __launch_bounds__(kPartBlock) void part_scan_kernel(const uint32_t* _
   if (threadIdx.x == 0) start[np] = carry;
 }
 
+// The merge record of spilled row j (key, then the words of a one-row
+// group), the key from the captured keys when there are any.
+__device__ __forceinline__ void spilled_record(const PartSrc& src, const KeySpec& ks, const GPlan& P, int64_t j,
+                                               unsigned long long* r) {
+  uint64_t key = 0;
+  if (src.keys) key = src.keys[j];
+  else (void)row_key(ks, (int64_t)src.rows[j], &key);
+  r[0] = key;
+  row_words(P, (int64_t)src.rows[j], r + 1, 1);
+}
+
 // Merge records, one per item: the key then the W accumulator words, `stride`
 // = 1 + W 64-bit words apart.
 struct PartRecs {
@@ -237,11 +255,7 @@ __global__ __launch_bounds__(kPartBlock) void part_scatter_kernel(PartSrc src, K
       r[0] = src.pkeys[i];
       for (int w = 0; w < W; w++) r[1 + w] = src.pwords[(size_t)w * src.pcap + i];
     } else {
-      const int64_t row = (int64_t)src.rows[i - src.n_parts];
-      uint64_t key = 0;
-      (void)row_key(ks, row, &key);
-      r[0] = key;
-      row_words(P, row, r + 1, 1);
+      spilled_record(src, ks, P, i - src.n_parts, r);
     }
   }
 }
@@ -342,11 +356,7 @@ __global__ __launch_bounds__(kStageBlock) void part_stage_kernel(PartSrc src, Ke
         r[0] = src.pkeys[i];
         for (int w = 0; w < W; w++) r[1 + w] = src.pwords[(size_t)w * src.pcap + i];
       } else {
-        const int64_t row = (int64_t)src.rows[i - src.n_parts];
-        uint64_t key = 0;
-        (void)row_key(ks, row, &key);
-        r[0] = key;
-        row_words(P, row, r + 1, 1);
+        spilled_record(src, ks, P, i - src.n_parts, r);
       }
     }
     __syncthreads();
@@ -453,16 +463,20 @@ int gb_part_bits(double est, int S, int bits_delta) {
 
 int gb_part_build(hipStream_t s, const KeySpec& ks, const GPlan& P, const unsigned long long* pkeys,
                   const uint8_t* ptag, const unsigned long long* pwords, int64_t pcap, int64_t n_parts,
-                  const uint32_t* spill, int64_t spill_per, const uint32_t* spill_seg, int nseg, int64_t n_spill,
-                  double est, int S, int bits_delta, GbPartitions* out) {
+                  const uint32_t* spill, const unsigned long long* spill_key, int64_t spill_per,
+                  const uint32_t* spill_seg, int nseg, int64_t n_spill, double est, int S, int bits_delta,
+                  GbPartitions* out) {
   const int64_t n_items = n_parts + n_spill;
   int pbits = gb_part_bits(est, S, bits_delta);
   const int np = 1 << pbits;
   out->np = np;
   BOLT_TRY(out->dense.alloc((size_t)std::max<int64_t>(n_spill, 1), s));
+  if (spill_key) BOLT_TRY(out->dense_key.alloc((size_t)std::max<int64_t>(n_spill, 1), s));
   const int K = 16;  // workgroups per spill segment
-  if (n_spill > 0) part_compact_spill_kernel<<<nseg * K, 256, 0, s>>>(spill, spill_per, spill_seg, K, out->dense.ptr);
-  PartSrc src{pkeys, ptag, pwords, pcap, n_parts, out->dense.ptr, n_items};
+  if (n_spill > 0)
+    part_compact_spill_kernel<<<nseg * K, 256, 0, s>>>(spill, spill_key, spill_per, spill_seg, K, out->dense.ptr,
+                                                        out->dense_key.ptr);
+  PartSrc src{pkeys, ptag, pwords, pcap, n_parts, out->dense.ptr, n_items, spill_key ? out->dense_key.ptr : nullptr};
   const int nb = (int)std::min<int64_t>(512, std::max<int64_t>(1, (n_items + 65535) / 65536));
   const int64_t chunk = (n_items + nb - 1) / nb;
   DevBuf<uint32_t> total;

@@ -738,6 +738,7 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
   DevBuf<unsigned long long> pkeys, pwords;
   DevBuf<uint8_t> ptag;
   DevBuf<uint32_t> spill, hll, spill_seg;
+  DevBuf<unsigned long long> spill_key;
   unsigned grid = 0;
   int64_t spill_per = 0;
   if (use_lds) {
@@ -765,12 +766,17 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
     BOLT_TRY(pwords.alloc((size_t)pcap * (W > 0 ? W : 1), s));
     BOLT_TRY(spill.alloc((size_t)grid * spill_per, s));
     BOLT_TRY(spill_seg.alloc(grid, s));
+    // inputs large enough for the partitioned tier: the spilled rows' keys are
+    // captured beside the row ids (8 bytes more per spilled row, from a
+    // register), so that tier's histogram and first scatter read them in
+    // order instead of gathering them by row id
+    if (n > ((int64_t)1 << 25)) BOLT_TRY(spill_key.alloc((size_t)grid * spill_per, s));
     // sketch registers and the two counters in one buffer: one read-back
     BOLT_TRY(hll.alloc(kHllRegs + 4, s));
     BOLT_TRY(fill_u32(hll.ptr, 0, kHllRegs + 4, s));
     unsigned long long* counters = reinterpret_cast<unsigned long long*>(hll.ptr + kHllRegs);
     PartOut po{pkeys.ptr, ptag.ptr, pwords.ptr, pcap, counters, spill.ptr, counters + 1, spill_seg.ptr,
-               spill_per, hll.ptr};
+               spill_per, hll.ptr, spill_key.ptr};
     prof_begin(s);
     if (gfn) {
       long long nn = n;
@@ -820,7 +826,8 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
   GbPartitions parts;
   if (partitioned)
     BOLT_TRY(gb_part_build(s, ks, P, pkeys.ptr, ptag.ptr, pwords.ptr, (int64_t)pkeys.count, n_parts, spill.ptr,
-                           spill_per, spill_seg.ptr, (int)grid, n_spill, est, S, part_bits_delta(est, W), &parts));
+                           spill_key.ptr, spill_per, spill_seg.ptr, (int)grid, n_spill, est, S,
+                           part_bits_delta(est, W), &parts));
   t_last_plan = GbLastPlan{!use_lds ? 1 : (partitioned ? 2 : (n_spill > 0 ? 1 : 0)), n_spill, n_parts,
                            partitioned ? parts.np : 0, est_bucket(est), W};
   GTable t;

@@ -353,6 +353,8 @@ struct PartSrc {
   int64_t n_parts;
   const uint32_t* rows;  // dense spilled row ids
   int64_t n_items;
+  // dense spilled keys (PartOut::spill_key), or null: then they are read by row id
+  const unsigned long long* keys;
 };
 struct GbPartitions {
   int np = 0, pbits = 0, nb = 0, stride = 0;
@@ -360,6 +362,7 @@ struct GbPartitions {
   PartSrc src{};
   KeySpec ks{};
   DevBuf<uint32_t> dense;
+  DevBuf<unsigned long long> dense_key;
   DevBuf<uint16_t> pid;
   DevBuf<uint32_t> hist;
   DevBuf<unsigned long long> cursor;
@@ -370,8 +373,9 @@ struct GbPartitions {
 };
 int gb_part_build(hipStream_t s, const KeySpec& ks, const GPlan& P, const unsigned long long* pkeys,
                   const uint8_t* ptag, const unsigned long long* pwords, int64_t pcap, int64_t n_parts,
-                  const uint32_t* spill, int64_t spill_per, const uint32_t* spill_seg, int nseg, int64_t n_spill,
-                  double est, int S, int bits_delta, GbPartitions* out);
+                  const uint32_t* spill, const unsigned long long* spill_key, int64_t spill_per,
+                  const uint32_t* spill_seg, int nseg, int64_t n_spill, double est, int S, int bits_delta,
+                  GbPartitions* out);
 int gb_part_bits(double est, int S, int bits_delta);
 int gb_part_reduce(hipStream_t s, const GTable& t, const GPlan& P, GbPartitions& parts, int S);
 uint64_t next_pow2(uint64_t x);

@@ -16,8 +16,9 @@
 //   4. jp_scatter_kernel  filter, then key + the probe columns the aggregates
 //                         read, staged by partition in LDS and written as runs
 //   5. jp_probe_kernel    persistent: workgroup b serves XCD b % 8 (workgroups
-//                         are dispatched to the XCDs round-robin), whose
-//                         partitions it takes in order from a per-XCD queue,
+//                         are dispatched to the XCDs round-robin), whose work
+//                         items (a few chunk runs of one partition each,
+//                         partitions in order) it takes from a per-XCD queue,
 //                         so at any time an XCD's workgroups probe one or two
 //                         partitions: a table region of a few MB that stays in
 //                         its L2
@@ -42,6 +43,7 @@ constexpr int kJpMaxParts = 1024;
 constexpr int kJpMaxPay = 2;       // probe columns carried with the key
 constexpr int kJpProbeBlock = 256;
 constexpr int kJpProbeUnroll = 4;  // rows in flight per lane in the probe
+constexpr int kJpProbeRuns = 4;    // chunk runs per probe work item (kJpGrid is a multiple)
 
 struct JpPay {
   const void* src[kJpMaxPay];
@@ -230,42 +232,60 @@ __global__ __launch_bounds__(kJpBlock) void jp_scatter_kernel(const KeyT* __rest
     actual[(int64_t)g * P + p] = cur[p] - (base[p] + counts[(int64_t)g * P + p]);
 }
 
-// Persistent probe.  Items of XCD x are (partition x + 8 * (i / G), chunk
-// i % G) for i = 0, 1, ...; its workgroups take them in order.  Aggregates
-// read the carried build column from the table (src 1) or the partitioned
-// probe columns (AggDesc data points at them).
-template <int NA>
+// Probe.  Workgroup b serves XCD x = b % 8 (workgroups are dispatched to the
+// XCDs round-robin).  The work of XCD x is its partitions x, x + 8, ... in
+// order, each cut into items of CG consecutive chunk runs, which the XCD's
+// workgroups take from a per-XCD queue, so at any time they cover one or two
+// partitions, whose table region stays in the XCD's L2.  An item's CG run
+// lengths are read by every lane (broadcast loads) and its rows are flattened
+// across the runs, so a lane's four rows in flight may come from different
+// runs.  Aggregates read the carried build column from the table (src 1) or
+// the partitioned probe columns (AggDesc data points at them).
+template <int NA, int CG>
 __global__ __launch_bounds__(kJpProbeBlock) void jp_probe_kernel(
     const uint4* __restrict__ tbl, uint64_t mask, const uint32_t* __restrict__ nonunique,
     const int64_t* __restrict__ pkeys, int P, int G, const uint32_t* __restrict__ counts,
     const uint32_t* __restrict__ base, const uint32_t* __restrict__ actual, uint32_t* __restrict__ queue,
     AggList aggs, int carry_dtype, Acc* __restrict__ parts) {
-  __shared__ uint32_t sh_item;
   Acc acc[NA];
 #pragma unroll
   for (int a = 0; a < NA; a++) acc_init(acc[a], aggs.a[a].op);
+  __shared__ uint32_t sh_item;
   const bool first_only = (*nonunique == 0u);
   const int x = blockIdx.x & 7;
-  const uint32_t items = (uint32_t)(P / 8) * (uint32_t)G;
+  const uint32_t per_part = (uint32_t)(G / CG);
+  const uint32_t items = (uint32_t)(P / 8) * per_part;
   for (;;) {
     if (threadIdx.x == 0) sh_item = atomicAdd(&queue[x], 1u);
     __syncthreads();
     const uint32_t item = sh_item;
     __syncthreads();
     if (item >= items) break;
-    const int p = x + 8 * (int)(item / (uint32_t)G);
-    const int g = (int)(item % (uint32_t)G);
-    const int64_t start = (int64_t)base[p] + counts[(int64_t)g * P + p];
-    const int64_t len = actual[(int64_t)g * P + p];
-    for (int64_t r0 = 0; r0 < len; r0 += kJpProbeBlock * kJpProbeUnroll) {
+    const int p = x + 8 * (int)(item / per_part);
+    const int g0 = (int)(item % per_part) * CG;
+    int64_t st[CG];
+    uint32_t pre[CG + 1];
+    pre[0] = 0;
+#pragma unroll
+    for (int c = 0; c < CG; c++) {
+      const int64_t cell = (int64_t)(g0 + c) * P + p;
+      st[c] = (int64_t)base[p] + counts[cell];
+      pre[c + 1] = pre[c] + actual[cell];
+    }
+    const uint32_t len = pre[CG];
+    for (uint32_t r0 = 0; r0 < len; r0 += kJpProbeBlock * kJpProbeUnroll) {
       int64_t k[kJpProbeUnroll];
       uint64_t s[kJpProbeUnroll];
       uint4 e[kJpProbeUnroll];
       Raw ra[kJpProbeUnroll][NA];
 #pragma unroll
       for (int u = 0; u < kJpProbeUnroll; u++) {
-        const int64_t r = r0 + u * kJpProbeBlock + threadIdx.x;
-        const int64_t i = start + (r < len ? r : len - 1);
+        uint32_t v = r0 + u * kJpProbeBlock + threadIdx.x;
+        v = v < len ? v : len - 1;
+        int64_t i = st[0] + v;
+#pragma unroll
+        for (int c = 1; c < CG; c++)
+          if (v >= pre[c]) i = st[c] + (v - pre[c]);
         k[u] = __builtin_nontemporal_load(pkeys + i);
 #pragma unroll
         for (int a = 0; a < NA; a++)
@@ -306,7 +326,7 @@ __global__ __launch_bounds__(kJpProbeBlock) void jp_probe_kernel(
 
 // Whether the partitioned form runs for this table and probe side.  It is
 // opt-in (BOLT_JOIN_PARTITIONED=1, where its limits allow): measured on the
-// 16M x 256M fused probe it takes 8.4 ms (hist 0.35, scatter 4.1, probe 4.7)
+// 16M x 256M fused probe it takes 8.3 ms (hist 0.35, scatter 3.36, probe 4.58)
 // against 5.3 ms for the random probe, so the random probe stays the default.
 bool join_partitioned_wanted(const bolt_join_table_t* t, int64_t n_probe, const AggList& aggs) {
   const char* e = getenv("BOLT_JOIN_PARTITIONED");
@@ -374,8 +394,12 @@ int join_probe_aggregate_partitioned(hipStream_t s, const bolt_join_table_t* t, 
   const int64_t n = key.length;
   int L = 0;
   while ((1ull << L) < t->mask + 1) L++;
-  // partitions of about 2^17 slots (2 MB), 8 to 1024 of them
-  int pbits = L - 17;
+  // partitions of about 2^19 slots (8 MB), 8 to 1024 of them: at 2 MB (2^17
+  // slots, half an XCD's L2) the scatter's runs are too short (1024 regions
+  // for the 16M x 256M join: 4.1 ms scatter), at 8 MB they are four times
+  // longer (3.36 ms) while the probe, whose regions then sit in the MALL
+  // rather than the L2, is as fast (4.58 against 4.48 ms)
+  int pbits = L - 19;
   if (const char* e = getenv("BOLT_JOIN_PART_BITS")) pbits = atoi(e);
   pbits = pbits < 3 ? 3 : (pbits > 10 ? 10 : pbits);
   if (pbits > L) pbits = L;
@@ -443,9 +467,10 @@ int join_probe_aggregate_partitioned(hipStream_t s, const bolt_join_table_t* t, 
     AggList chunk = agg_chunk(pa, first, na);
     if (first > 0) BOLT_TRY(fill_u32(queue.ptr, 0, 8, s));
 #define JP_PROBE(NA)                                                                                                 \
-  jp_probe_kernel<NA><<<pgrid, kJpProbeBlock, 0, s>>>(t->tbl, t->mask, t->flags, okeys.ptr, P, kJpGrid, counts.ptr, \
-                                                      base.ptr, actual.ptr, queue.ptr, chunk, t->carry_dtype,       \
-                                                      parts.ptr + (size_t)first * pgrid)
+  jp_probe_kernel<NA, kJpProbeRuns><<<pgrid, kJpProbeBlock, 0, s>>>(t->tbl, t->mask, t->flags, okeys.ptr, P, kJpGrid, \
+                                                                    counts.ptr, base.ptr, actual.ptr, queue.ptr,      \
+                                                                    chunk, t->carry_dtype,                            \
+                                                                    parts.ptr + (size_t)first * pgrid)
     switch (na) {
       case 1: JP_PROBE(1); break;
       case 2: JP_PROBE(2); break;
