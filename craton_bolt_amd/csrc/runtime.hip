@@ -87,14 +87,128 @@ static void account_free(void* p) {
   }
 }
 
+// ---- large-block cache ------------------------------------------------------
+// ROCm's stream-ordered allocator maps and unmaps large blocks on every
+// hipMallocAsync / hipFreeAsync even with the pool's release threshold at its
+// maximum (about 0.5 ms of host time per GiB on MI355X), which was a quarter
+// of a 256M-row GROUP BY call.  Blocks of 32 MiB and more freed outside graph
+// capture are kept per device, each with an event recorded on the freeing
+// stream, and handed to a later allocation of at most their size and more
+// than half of it, whose stream first waits for that event (so the reuse is
+// ordered after every use before the free, whatever the streams).  At most
+// 32 GiB is kept; an allocation that fails empties the cache and is retried.
+
+struct CachedBlock {
+  void* p;
+  size_t bytes;
+  hipEvent_t freed;
+  int dev;
+};
+static std::mutex g_cache_mu;
+static std::vector<CachedBlock> g_cache;
+static std::unordered_map<void*, size_t> g_block_bytes;  // size of every block from hipMallocAsync
+static size_t g_cached_bytes = 0;
+constexpr size_t kCacheMinBlock = (size_t)32 << 20;
+constexpr size_t kCacheMaxBytes = (size_t)32 << 30;
+
+static bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return cs != hipStreamCaptureStatusNone;
+}
+
+static void* cache_take(size_t bytes, hipStream_t s) {
+  if (bytes < kCacheMinBlock) return nullptr;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  CachedBlock got{};
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    size_t best = g_cache.size();
+    for (size_t i = 0; i < g_cache.size(); i++) {
+      const CachedBlock& b = g_cache[i];
+      if (b.dev == dev && b.bytes >= bytes && b.bytes / 2 < bytes &&
+          (best == g_cache.size() || b.bytes < g_cache[best].bytes))
+        best = i;
+    }
+    if (best == g_cache.size()) return nullptr;
+    got = g_cache[best];
+    g_cached_bytes -= got.bytes;
+    g_cache.erase(g_cache.begin() + (std::ptrdiff_t)best);
+  }
+  if (hipStreamWaitEvent(s, got.freed, 0) != hipSuccess) {  // keep the order, or do not reuse
+    (void)hipGetLastError();
+    (void)hipEventSynchronize(got.freed);
+  }
+  (void)hipEventDestroy(got.freed);
+  return got.p;
+}
+
+static bool cache_put(void* p, hipStream_t s) {
+  size_t bytes = 0;
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    const auto it = g_block_bytes.find(p);
+    if (it == g_block_bytes.end() || it->second < kCacheMinBlock || g_cached_bytes + it->second > kCacheMaxBytes)
+      return false;
+    bytes = it->second;
+  }
+  int dev = 0;
+  hipEvent_t ev = nullptr;
+  if (hipGetDevice(&dev) != hipSuccess || hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  if (hipEventRecord(ev, s) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipEventDestroy(ev);
+    return false;
+  }
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  g_cache.push_back(CachedBlock{p, bytes, ev, dev});
+  g_cached_bytes += bytes;
+  return true;
+}
+
+static void cache_flush() {
+  std::vector<CachedBlock> all;
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    all.swap(g_cache);
+    g_cached_bytes = 0;
+    for (const CachedBlock& b : all) g_block_bytes.erase(b.p);
+  }
+  for (const CachedBlock& b : all) {
+    (void)hipEventSynchronize(b.freed);
+    (void)hipEventDestroy(b.freed);
+    (void)hipFree(b.p);
+  }
+}
+
 int dev_alloc(void** p, size_t bytes, hipStream_t s) {
   BOLT_CHECK_DEVICE();
   *p = nullptr;
-  hipError_t e = hipMallocAsync(p, bytes ? bytes : 1, s);
-  if (e != hipSuccess) {
-    (void)hipGetLastError();
-    return set_error(BOLT_ERR_MEMORY, "device allocation of %zu bytes failed: %s", bytes,
-                     hipGetErrorString(e));
+  const bool cap = capturing(s);
+  if (!cap) *p = cache_take(bytes, s);
+  if (!*p) {
+    hipError_t e = hipMallocAsync(p, bytes ? bytes : 1, s);
+    if (e != hipSuccess && !cap) {
+      (void)hipGetLastError();
+      cache_flush();
+      e = hipMallocAsync(p, bytes ? bytes : 1, s);
+    }
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return set_error(BOLT_ERR_MEMORY, "device allocation of %zu bytes failed: %s", bytes,
+                       hipGetErrorString(e));
+    }
+    if (!cap && bytes >= kCacheMinBlock) {
+      std::lock_guard<std::mutex> lk(g_cache_mu);
+      g_block_bytes[*p] = bytes;
+    }
   }
   account_alloc(*p, bytes);
   return BOLT_OK;
@@ -103,6 +217,11 @@ int dev_alloc(void** p, size_t bytes, hipStream_t s) {
 int dev_free(void* p, hipStream_t s) {
   if (!p) return BOLT_OK;
   account_free(p);
+  if (!capturing(s) && cache_put(p, s)) return BOLT_OK;
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    g_block_bytes.erase(p);
+  }
   BOLT_HIP(hipFreeAsync(p, s));
   return BOLT_OK;
 }
