@@ -44,6 +44,8 @@ def _parse():
     p.add_argument("--workload", default="all", choices=["all", "join", "sort", "topk", "expr", "h2o", "groupby",
                                                             "groupby-zipf", "calib", "batch", "serve"])
     p.add_argument("--limit", type=int, default=100, help="topk workload: LIMIT")
+    p.add_argument("--key-dtype", default="int64", choices=["int64", "int32", "date32"],
+                   help="topk workload: the ORDER BY key's type")
     p.add_argument("--fact-rows", type=int, default=200_000_000, help="serve workload: fact table rows")
     p.add_argument("--serve-queries", type=int, default=64, help="serve workload: queries per measurement")
     p.add_argument("--slots", type=int, default=8, help="serve workload: concurrent queries")
@@ -298,8 +300,9 @@ def bench_sort(args) -> dict:
 
 
 def bench_topk(args) -> dict:
-    """ORDER BY key LIMIT k over 256M int64 keys: one bolt_sort_limit call per
-    step (it synchronises), against one full bolt_sort_indices + slice."""
+    """ORDER BY key LIMIT k over 256M keys (int64 over the full range by default; int32 uniform, or
+    Date32 over 20000 days): one bolt_sort_limit call per step (it synchronises), against one full
+    bolt_sort_indices + slice."""
     from craton_bolt_amd import _native as N
     from craton_bolt_amd import datagen as D
     from craton_bolt_amd.device import DeviceColumn, Stream
@@ -307,18 +310,27 @@ def bench_topk(args) -> dict:
 
     n, k = args.n, args.limit
     key, _ = D.sort_pairs(n)
-    col = DeviceColumn.from_numpy(key, N.I64)
+    kd = args.key_dtype
+    if kd == "int32":
+        key = (key >> 32).astype(np.int32)
+    elif kd == "date32":
+        key = (key.view(np.uint64) % np.uint64(20_000)).astype(np.int32)
+    dt = {"int64": N.I64, "int32": N.I32, "date32": N.DATE32}[kd]
+    width = 8 if kd == "int64" else 4
+    col = DeviceColumn.from_numpy(key, dt)
     s = Stream()
     ms, wall = _timed(s, lambda: execute_sort_limit([col], k, 0, stream=s), args.steps, args.warmup)
     ms = wall  # the call synchronises the host: its wall time is the step
     t0 = time.perf_counter()
     execute_sort([col], stream=s)
     full_ms = (time.perf_counter() - t0) * 1000.0
-    return _line(n / (ms * 1e-3), "rows/s", args, ms, "int64",
-                 {"workload": f"ORDER BY int64 key LIMIT {k}: {n} rows (uniform over the full range)",
-                  "n": n, "limit": k, "full_sort_then_slice_ms": full_ms},
-                 _roof(n * 8, ms, "radix select (12-bit histogram passes) + candidate collect + candidate sort; "
-                                  "8 B/row (one read of the key) is the compulsory traffic"))
+    what = {"int64": "int64 key (uniform over the full range)", "int32": "int32 key (uniform)",
+            "date32": "Date32 key (20000 distinct days)"}[kd]
+    return _line(n / (ms * 1e-3), "rows/s", args, ms, kd,
+                 {"workload": f"ORDER BY {what} LIMIT {k}: {n} rows", "n": n, "limit": k,
+                  "full_sort_then_slice_ms": full_ms},
+                 _roof(n * width, ms, f"radix select (12-bit histogram passes) + candidate collect + candidate "
+                                      f"sort; {width} B/row (one read of the key) is the compulsory traffic"))
 
 
 def bench_expr(args) -> dict:
