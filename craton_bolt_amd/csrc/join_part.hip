@@ -274,10 +274,9 @@ __global__ __launch_bounds__(kJpProbeBlock) void jp_probe_kernel(
     }
     const uint32_t len = pre[CG];
     for (uint32_t r0 = 0; r0 < len; r0 += kJpProbeBlock * kJpProbeUnroll) {
-      int64_t k[kJpProbeUnroll];
+      int64_t k[kJpProbeUnroll], idx[kJpProbeUnroll];
       uint64_t s[kJpProbeUnroll];
       uint4 e[kJpProbeUnroll];
-      Raw ra[kJpProbeUnroll][NA];
 #pragma unroll
       for (int u = 0; u < kJpProbeUnroll; u++) {
         uint32_t v = r0 + u * kJpProbeBlock + threadIdx.x;
@@ -286,11 +285,8 @@ __global__ __launch_bounds__(kJpProbeBlock) void jp_probe_kernel(
 #pragma unroll
         for (int c = 1; c < CG; c++)
           if (v >= pre[c]) i = st[c] + (v - pre[c]);
+        idx[u] = i;
         k[u] = __builtin_nontemporal_load(pkeys + i);
-#pragma unroll
-        for (int a = 0; a < NA; a++)
-          if (aggs.a[a].src == 0 && aggs.a[a].op != BOLT_AGG_COUNT_STAR)
-            ra[u][a] = load_raw(aggs.a[a].data, aggs.a[a].dtype, i);
         s[u] = fmix64((uint64_t)k[u]) & mask;
       }
 #pragma unroll
@@ -309,8 +305,10 @@ __global__ __launch_bounds__(kJpProbeBlock) void jp_probe_kernel(
               }
               int64_t vi, vhi = 0;
               double vf;
+              // a probe column's value is read only for a matching row: the
+              // runs are in probe order, so matching lanes read neighbours
               if (d.src == 1) carry_value(e[u].z, carry_dtype, vi, vf);
-              else raw_value(ra[u][a], d.dtype, vi, vhi, vf);
+              else raw_value(load_raw(d.data, d.dtype, idx[u]), d.dtype, vi, vhi, vf);
               acc_update(acc[a], d.op, d.dtype, vi, vhi, vf);
             }
             if (first_only) break;
