@@ -836,19 +836,39 @@ __global__ __launch_bounds__(256) void topk_cand_hist_kernel(TopKey k, const uin
   atomicAdd(&hist[(x >> shift) & ((1ull << width) - 1)], 1u);
 }
 
-// appends the candidates whose image >> shift <= bound
+// Candidates tied on the whole image `img`: histogram of their row ids'
+// bits [shift, shift + width) among those whose higher row-id bits are
+// `rprefix` (the order among equal keys is by row id).
+template <int W>
+__global__ __launch_bounds__(256) void topk_tie_hist_kernel(TopKey k, const uint32_t* __restrict__ cand, int64_t c,
+                                                            uint64_t img, int shift, int width, uint32_t rprefix,
+                                                            uint32_t* __restrict__ hist) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= c) return;
+  const uint32_t r = cand[j];
+  if (coarse_of(k, load_raw<W>(k.d, r), true) != img) return;
+  const int hi = shift + width;
+  if (hi < 32 && (r >> hi) != rprefix) return;
+  atomicAdd(&hist[(r >> shift) & ((1u << width) - 1)], 1u);
+}
+
+// appends the candidates whose image >> shift <= bound; with shift 0 and
+// rmax < 0xFFFFFFFF, of the rows whose image equals `bound` only those with
+// row id <= rmax
 template <int W>
 __global__ __launch_bounds__(256) void topk_cand_collect_kernel(TopKey k, const uint32_t* __restrict__ cand,
                                                                 int64_t c, int shift, uint64_t bound,
                                                                 uint32_t* __restrict__ out,
-                                                                unsigned long long* __restrict__ count, int64_t cap) {
+                                                                unsigned long long* __restrict__ count, int64_t cap,
+                                                                uint32_t rmax = 0xFFFFFFFFu) {
   const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int lane = threadIdx.x & 63;
   bool take = false;
   uint32_t r = 0;
   if (j < c) {
     r = cand[j];
-    take = (shift >= 64 ? 0ull : coarse_of(k, load_raw<W>(k.d, r), true) >> shift) <= bound;
+    const uint64_t x = shift >= 64 ? 0ull : coarse_of(k, load_raw<W>(k.d, r), true) >> shift;
+    take = x < bound || (x == bound && r <= rmax);
   }
   const uint64_t m = __ballot(take);
   if (m == 0) return;
@@ -1175,15 +1195,49 @@ extern "C" int bolt_sort_limit(bolt_stream_t stream, const bolt_column_t* keys, 
       p2 = (p2 << width) | (uint64_t)d;
       hi2 = shift;
     }
-    const int64_t C2 = below2 + eq2;
+    int64_t C2 = below2 + eq2;
+    // More rows tie on the K-th row's whole key than one workgroup sorts:
+    // the order among them is by row id, so select the need2-th smallest
+    // row id of the tie (14, 14 and 4 bits) and keep the tied rows up to it.
+    uint32_t rmax = 0xFFFFFFFFu;
+    if (C2 > kSmallSort && hi2 == 0 && below2 + need2 <= kSmallSort) {
+      uint32_t rpre = 0;
+      int rhi = 32;
+      int64_t need3 = need2;
+      while (rhi > 0) {
+        const int width = rhi >= kSelBits ? kSelBits : rhi;
+        const int shift = rhi - width;
+        BOLT_TRY(fill_u32(hist.ptr, 0, (size_t)1 << width, s));
+        switch (kw) {
+          case 1: topk_tie_hist_kernel<1><<<cgrid, 256, 0, s>>>(tk, cptr, C, p2, shift, width, rpre, hist.ptr); break;
+          case 4: topk_tie_hist_kernel<4><<<cgrid, 256, 0, s>>>(tk, cptr, C, p2, shift, width, rpre, hist.ptr); break;
+          default: topk_tie_hist_kernel<8><<<cgrid, 256, 0, s>>>(tk, cptr, C, p2, shift, width, rpre, hist.ptr); break;
+        }
+        BOLT_HIP(hipGetLastError());
+        BOLT_HIP(hipMemcpyAsync(h.data(), hist.ptr, (size_t)(1 << width) * 4, hipMemcpyDeviceToHost, s));
+        BOLT_HIP(hipStreamSynchronize(s));
+        int64_t cum = 0;
+        int d = 0;
+        for (; d < (1 << width); d++) {
+          if (cum + (int64_t)h[d] >= need3) break;
+          cum += h[d];
+        }
+        if (d == (1 << width)) return set_error(BOLT_ERR_OTHER, "bolt_sort_limit: tie select lost rows");
+        need3 -= cum;
+        rpre = (rpre << width) | (uint32_t)d;
+        rhi = shift;
+      }
+      rmax = rpre;  // row ids are distinct: exactly need2 tied rows are <= rmax
+      C2 = below2 + need2;
+    }
     if (C2 <= kSmallSort) {
       DevBuf<uint32_t> cand2;
       BOLT_TRY(cand2.alloc((size_t)C2, s));
       BOLT_TRY(fill_u32(cnt.ptr, 0, 2, s));
       switch (kw) {
-        case 1: topk_cand_collect_kernel<1><<<cgrid, 256, 0, s>>>(tk, cptr, C, hi2, p2, cand2.ptr, cnt.ptr, C2); break;
-        case 4: topk_cand_collect_kernel<4><<<cgrid, 256, 0, s>>>(tk, cptr, C, hi2, p2, cand2.ptr, cnt.ptr, C2); break;
-        default: topk_cand_collect_kernel<8><<<cgrid, 256, 0, s>>>(tk, cptr, C, hi2, p2, cand2.ptr, cnt.ptr, C2); break;
+        case 1: topk_cand_collect_kernel<1><<<cgrid, 256, 0, s>>>(tk, cptr, C, hi2, p2, cand2.ptr, cnt.ptr, C2, rmax); break;
+        case 4: topk_cand_collect_kernel<4><<<cgrid, 256, 0, s>>>(tk, cptr, C, hi2, p2, cand2.ptr, cnt.ptr, C2, rmax); break;
+        default: topk_cand_collect_kernel<8><<<cgrid, 256, 0, s>>>(tk, cptr, C, hi2, p2, cand2.ptr, cnt.ptr, C2, rmax); break;
       }
       BOLT_HIP(hipGetLastError());
       int npow = 1;

@@ -156,3 +156,23 @@ def test_sort_limit_first_pass_candidates_two_keys(gpu):
     k0[rng.integers(0, n, 50)] = np.iinfo(np.int64).min  # ties at the very front
     k1 = rng.integers(0, 3, n).astype(np.int32)
     _check([both(k0, N.I64), both(k1, N.I32, rng.random(n) > 0.1)], [False, True], [True, False], 120, 10)
+
+
+@pytest.mark.parametrize("dtype", ["i64", "i32", "date32"])
+@pytest.mark.parametrize("limit,offset,desc", [(100, 0, False), (100, 0, True), (1, 0, False), (1500, 300, True),
+                                               (2047, 0, False)])
+def test_sort_limit_large_tie_on_the_kth_key(gpu, dtype, limit, offset, desc):
+    """More rows tie on the K-th row's key than one workgroup sorts (about
+    20,000 copies of each value): the tied rows are narrowed by row id, and the
+    window must still be the stable order's."""
+    print(f"[case] large tie {dtype} limit={limit} offset={offset} desc={desc}", flush=True)
+    n = 2_000_003
+    rng = np.random.default_rng(7 * limit + offset + len(dtype))
+    v = rng.integers(-50, 50, n)
+    if dtype == "i64":
+        cols = [both((v * 1_000_000_007).astype(np.int64), N.I64)]
+    elif dtype == "i32":
+        cols = [both(v.astype(np.int32), N.I32)]
+    else:
+        cols = [both((v + 19_000).astype(np.int32), N.DATE32)]
+    _check(cols, [desc], [True], limit, offset)
