@@ -415,15 +415,29 @@ __global__ __launch_bounds__(kGbBlock) void part_reduce_kernel(GTable t, GPlan P
   if (threadIdx.x < 3) sused[threadIdx.x] = 0;  // [2]: occupied slots
   __syncthreads();
   const int64_t r0 = (int64_t)start[blockIdx.x], r1 = (int64_t)start[blockIdx.x + 1];
-  for (int64_t r = r0 + threadIdx.x; r < r1; r += kGbBlock) {
-    const unsigned long long* rec = recs.rec + r * (uint64_t)recs.stride;
-    const unsigned long long key = rec[0];
-    const int slot = gb_lds_slot(skeys, S, key, &sused[2]);
-    if (slot >= 0) {
-      merge_slot(P, swords, SS, (uint32_t)slot, rec + 1, 1);
-    } else {  // the partition holds more groups than the LDS table: straight to HBM
-      const int64_t g = gslot(t, 0, key);
-      if (g >= 0) merge_slot(P, t.words, t.cap + 2, (uint32_t)g, rec + 1, 1);
+  // four records per lane per iteration: their keys are loaded together, so
+  // the lane waits on one HBM round trip per four records (the words then sit
+  // in the lines the keys brought in)
+  constexpr int kU = 4;
+  for (int64_t base = r0 + threadIdx.x; base < r1; base += (int64_t)kU * kGbBlock) {
+    unsigned long long keys[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const int64_t r = base + (int64_t)u * kGbBlock;
+      keys[u] = r < r1 ? recs.rec[r * (uint64_t)recs.stride] : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const int64_t r = base + (int64_t)u * kGbBlock;
+      if (r >= r1) continue;
+      const unsigned long long* rec = recs.rec + r * (uint64_t)recs.stride;
+      const int slot = gb_lds_slot(skeys, S, keys[u], &sused[2]);
+      if (slot >= 0) {
+        merge_slot(P, swords, SS, (uint32_t)slot, rec + 1, 1);
+      } else {  // the partition holds more groups than the LDS table: straight to HBM
+        const int64_t g = gslot(t, 0, keys[u]);
+        if (g >= 0) merge_slot(P, t.words, t.cap + 2, (uint32_t)g, rec + 1, 1);
+      }
     }
   }
   __syncthreads();
