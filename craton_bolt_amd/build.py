@@ -51,9 +51,11 @@ def _run(cmd: list[str], quiet: bool) -> None:
 
 
 def _write_jit_prelude() -> None:
-    """Embed device_acc.h and gb_lds.h as a string literal: they are prepended
-    to the kernels the library generates per query and compiles with hipRTC."""
-    text = "".join((CSRC / h).read_text().replace("#pragma once\n", "") for h in ("device_acc.h", "gb_lds.h"))
+    """Embed device_acc.h, gb_lds.h and jp_tile.h as a string literal: they are
+    prepended to the kernels the library generates per query and compiles with
+    hipRTC."""
+    text = "".join((CSRC / h).read_text().replace("#pragma once\n", "")
+                   for h in ("device_acc.h", "gb_lds.h", "jp_tile.h"))
     out = OBJ / "jit_prelude.inc"
     body = 'static const char kJitPrelude[] = R"BOLTJIT(\n' + text + '\n)BOLTJIT";\n'
     if not out.exists() or out.read_text() != body:

@@ -30,6 +30,7 @@ struct JitSpec {
   bool probe = false;
   bool bloom = false;  // probe through the build side's Bloom filter first
   int unroll = 4;  // rows per lane per block iteration (1..8)
+  int probe_slots = 1;  // slots of each probe sequence read before the first compare (1..8)
   int key_dtype = 0;
   bool key_valid = false;
   int carry_dtype = 0;
@@ -60,6 +61,17 @@ struct GbJitSpec {
   std::vector<Agg> aggs;
 };
 
+// The shape of a generated partitioned-join scatter: the probe key, the
+// filter, and which columns are carried into the partitioned copies.
+struct JpJitSpec {
+  int items = 8;  // rows per lane per tile (tile = 1024 * items rows)
+  int key_dtype = 0;
+  bool key_valid = false;
+  std::vector<JitSpec::Col> cols;
+  std::vector<JitSpec::Term> terms;
+  std::vector<int> pay;  // column index carried into partitioned copy j
+};
+
 bool jit_enabled();
 // Device helpers every generated kernel starts with (device_acc.h, gb_lds.h).
 const char* jit_prelude_text();
@@ -68,6 +80,11 @@ int jit_get(const std::string& src, hipFunction_t* fn);
 int jit_spec_from(const DevFilter& f, const AggList& aggs, bool probe, int key_dtype, bool key_valid, int carry_dtype,
                   JitSpec* sp, JitLits* lits, JitCols* cols);
 std::string jit_gb_lds_source(const GbJitSpec& sp);
+// Partitioned join: the filtered scatter of the probe side into per-partition
+// runs, and the region-ordered probe + aggregate over those runs.
+std::string jit_jp_scatter_source(const JpJitSpec& sp);
+std::string jp_scatter_jit_key(const JpJitSpec& sp);
+std::string jit_jp_probe_source(const JitSpec& sp, int runs);
 std::string gb_jit_key(const GbJitSpec& sp);
 int jit_get_keyed(const std::string& key, const std::function<std::string()>& gen, hipFunction_t* fn);
 int jit_row_agg_col(const void* d, const uint8_t* v, int dt, std::vector<JitSpec::Col>* cols, JitCols* jc);

@@ -305,8 +305,17 @@ int jit_get(const std::string& src, hipFunction_t* fn) {
     }
   }
   JitEntry e;
-  BOLT_HIP(hipModuleLoadData(&e.mod, code.data()));
-  BOLT_HIP(hipModuleGetFunction(&e.fn, e.mod, "bolt_jit_kernel"));
+  hipError_t he = hipModuleLoadData(&e.mod, code.data());
+  if (he == hipSuccess) {
+    he = hipModuleGetFunction(&e.fn, e.mod, "bolt_jit_kernel");
+    if (he != hipSuccess) (void)hipModuleUnload(e.mod);
+  }
+  if (he != hipSuccess) {
+    // callers fall back to precompiled kernels: leave no HIP error pending for
+    // their launch checks to report
+    (void)hipGetLastError();
+    return set_error(BOLT_ERR_DEVICE, "hipRTC module load: %s (%d)", hipGetErrorString(he), (int)he);
+  }
   g_cache.emplace(src, e);
   *fn = e.fn;
   return BOLT_OK;
@@ -347,6 +356,112 @@ std::string gb_jit_key(const GbJitSpec& sp) {
 }
 
 // ---- source generation ------------------------------------------------------------
+
+namespace {
+
+// Accumulator updates of row r (values x<col>_<r>, row index i<r>, join carry w),
+// one statement per aggregate, guarded by the input's validity bit.
+std::string agg_updates(const JitSpec& sp, int r, const char* ind) {
+  const int na = (int)sp.aggs.size();
+  std::ostringstream up;
+  const std::string i = "i" + std::to_string(r);
+  for (int a = 0; a < na; a++) {
+    const JitSpec::Agg& g = sp.aggs[a];
+    const std::string A = "acc" + std::to_string(a);
+    if (g.op == BOLT_AGG_COUNT_STAR) {
+      up << ind << A << ".n++;\n";
+      continue;
+    }
+    std::string V;
+    int dt = g.dtype;
+    std::string guard;
+    if (g.src == 1) {
+      V = dt == BOLT_F32 ? "(double)__uint_as_float(w)" : "(long long)(int)w";
+    } else {
+      V = "x" + std::to_string(g.col) + "_" + std::to_string(r);
+      if (sp.cols[g.col].has_valid) guard = "vbit(v" + std::to_string(g.col) + ", " + i + ")";
+    }
+    std::ostringstream s;
+    switch (g.op) {
+      case BOLT_AGG_COUNT: s << A << ".n++;"; break;
+      case BOLT_AGG_SUM:
+        if (dt == BOLT_DEC128) s << "add128(" << A << ", " << V << "_lo, (long long)" << V << "_hi); " << A << ".n++;";
+        else if (dt == BOLT_F32 || dt == BOLT_F64) s << "neum_add(" << A << ", (double)" << V << "); " << A << ".n++;";
+        else s << "{ const long long v_ = (long long)" << V << "; add128(" << A << ", (unsigned long long)v_, v_ >> 63); "
+               << A << ".n++; }";
+        break;
+      case BOLT_AGG_AVG: s << "neum_add(" << A << ", (double)" << V << "); " << A << ".n++;"; break;
+      case BOLT_AGG_MIN:
+      case BOLT_AGG_MAX: {
+        const char* is_min = g.op == BOLT_AGG_MIN ? "true" : "false";
+        if (dt == BOLT_F32 || dt == BOLT_F64)
+          s << "minmax_push(" << A << ", " << is_min << ", f64_order_key((double)" << V << "));";
+        else
+          s << "minmax_push(" << A << ", " << is_min << ", (unsigned long long)(long long)" << V
+            << " ^ 0x8000000000000000ull);";
+        break;
+      }
+      default: s << "welford_push(" << A << ", (double)" << V << ");"; break;
+    }
+    if (!guard.empty()) up << ind << "if (" << guard << ") { " << s.str() << " }\n";
+    else up << ind << s.str() << "\n";
+  }
+  return up.str();
+}
+
+// Probe sequences of R rows whose keys are k<r> and home slots s<r>; row r
+// probes when `act(r)` holds.  The first K slots of every row's sequence are
+// read together (8 slots share a 128-byte line, so mostly one line), rows
+// that do not probe reading slots from 0, which stay cached; each row then
+// resolves its chain from registers, and only a chain longer than K slots
+// goes on with dependent reads.  The walk is that of probe_chain (join_dev.h):
+// stop at the first empty slot, or after the first match when first_only.
+std::string probe_chains(const JitSpec& sp, int R, const std::function<std::string(int)>& act, const char* ind) {
+  const int K = sp.probe_slots;
+  std::ostringstream o;
+  for (int r = 0; r < R; r++) {
+    const std::string rs = std::to_string(r);
+    o << ind << "const unsigned long long b" << rs << " = " << act(r) << " ? s" << rs << " : 0ull;\n";
+    for (int j = 0; j < K; j++)
+      o << ind << "const uint4 e" << rs << "_" << j << " = tbl[(b" << rs << " + " << j << "ull) & mask];\n";
+  }
+  auto match = [&](const std::string& e, int r) {
+    return "(long long)(((unsigned long long)" + e + ".y << 32) | " + e + ".x) == k" + std::to_string(r);
+  };
+  const std::string in2 = std::string(ind) + "  ", in4 = std::string(ind) + "    ", in6 = std::string(ind) + "      ";
+  for (int r = 0; r < R; r++) {
+    const std::string rs = std::to_string(r);
+    o << ind << "if (" << act(r) << ") {\n" << in2 << "bool live = true;\n";
+    for (int j = 0; j < K; j++) {
+      const std::string e = "e" + rs + "_" + std::to_string(j);
+      o << in2 << "if (live && " << j << "ull <= mask) {\n"
+        << in4 << "if (" << e << ".w == 0u) live = false;\n"
+        << in4 << "else if (" << match(e, r) << ") {\n"
+        << in6 << "const unsigned int w = " << e << ".z;\n"
+        << agg_updates(sp, r, in6.c_str())
+        << in6 << "if (first_only) live = false;\n"
+        << in4 << "}\n"
+        << in2 << "}\n";
+    }
+    o << in2 << "if (live && " << K << "ull <= mask) {\n"
+      << in4 << "unsigned long long sv = (s" << rs << " + " << K << "ull) & mask;\n"
+      << in4 << "uint4 e = tbl[sv];\n"
+      << in4 << "for (unsigned long long step = " << K << "; e.w != 0u && step <= mask; step++) {\n"
+      << in6 << "if (" << match("e", r) << ") {\n"
+      << in6 << "  const unsigned int w = e.z;\n"
+      << agg_updates(sp, r, (in6 + "  ").c_str())
+      << in6 << "  if (first_only) break;\n"
+      << in6 << "}\n"
+      << in6 << "sv = (sv + 1) & mask;\n"
+      << in6 << "e = tbl[sv];\n"
+      << in4 << "}\n"
+      << in2 << "}\n"
+      << ind << "}\n";
+  }
+  return o.str();
+}
+
+}  // namespace
 
 std::string jit_row_agg_source(const JitSpec& sp) {
   std::ostringstream o;
@@ -396,53 +511,7 @@ std::string jit_row_agg_source(const JitSpec& sp) {
     for (int t = 0; t < (int)sp.terms.size(); t++)
       o << "    " << keep << " = " << keep << " && " << term_expr(sp.cols, sp.terms[t], t, r, i) << ";\n";
   }
-  // aggregate updates of row r
-  auto updates = [&](int r, const char* ind) {
-    std::ostringstream up;
-    const std::string i = "i" + std::to_string(r);
-    for (int a = 0; a < na; a++) {
-      const JitSpec::Agg& g = sp.aggs[a];
-      const std::string A = "acc" + std::to_string(a);
-      if (g.op == BOLT_AGG_COUNT_STAR) {
-        up << ind << A << ".n++;\n";
-        continue;
-      }
-      std::string V;
-      int dt = g.dtype;
-      std::string guard;
-      if (g.src == 1) {
-        V = dt == BOLT_F32 ? "(double)__uint_as_float(w)" : "(long long)(int)w";
-      } else {
-        V = "x" + std::to_string(g.col) + "_" + std::to_string(r);
-        if (sp.cols[g.col].has_valid) guard = "vbit(v" + std::to_string(g.col) + ", " + i + ")";
-      }
-      std::ostringstream s;
-      switch (g.op) {
-        case BOLT_AGG_COUNT: s << A << ".n++;"; break;
-        case BOLT_AGG_SUM:
-          if (dt == BOLT_DEC128) s << "add128(" << A << ", " << V << "_lo, (long long)" << V << "_hi); " << A << ".n++;";
-          else if (dt == BOLT_F32 || dt == BOLT_F64) s << "neum_add(" << A << ", (double)" << V << "); " << A << ".n++;";
-          else s << "{ const long long v_ = (long long)" << V << "; add128(" << A << ", (unsigned long long)v_, v_ >> 63); "
-                 << A << ".n++; }";
-          break;
-        case BOLT_AGG_AVG: s << "neum_add(" << A << ", (double)" << V << "); " << A << ".n++;"; break;
-        case BOLT_AGG_MIN:
-        case BOLT_AGG_MAX: {
-          const char* is_min = g.op == BOLT_AGG_MIN ? "true" : "false";
-          if (dt == BOLT_F32 || dt == BOLT_F64)
-            s << "minmax_push(" << A << ", " << is_min << ", f64_order_key((double)" << V << "));";
-          else
-            s << "minmax_push(" << A << ", " << is_min << ", (unsigned long long)(long long)" << V
-              << " ^ 0x8000000000000000ull);";
-          break;
-        }
-        default: s << "welford_push(" << A << ", (double)" << V << ");"; break;
-      }
-      if (!guard.empty()) up << ind << "if (" << guard << ") { " << s.str() << " }\n";
-      else up << ind << s.str() << "\n";
-    }
-    return up.str();
-  };
+  auto updates = [&](int r, const char* ind) { return agg_updates(sp, r, ind); };
   if (sp.probe) {
     // first reads of all R rows, then each row's chain
     for (int r = 0; r < R; r++)
@@ -456,23 +525,7 @@ std::string jit_row_agg_source(const JitSpec& sp) {
       for (int r = 0; r < R; r++)
         o << "    keep" << r << " = keep" << r << " && bloom_hit(b" << r << ", h" << r << ");\n";
     }
-    for (int r = 0; r < R; r++)
-      o << "    uint4 e" << r << " = make_uint4(0u, 0u, 0u, 0u);\n"
-        << "    if (keep" << r << ") e" << r << " = tbl[s" << r << "];\n";
-    for (int r = 0; r < R; r++) {
-      const std::string e = "e" + std::to_string(r), sv = "s" + std::to_string(r);
-      o << "    if (keep" << r << ") {\n"
-        << "      for (unsigned long long step = 0; " << e << ".w != 0u && step <= mask; step++) {\n"
-        << "        if ((long long)(((unsigned long long)" << e << ".y << 32) | " << e << ".x) == k" << r << ") {\n"
-        << "          const unsigned int w = " << e << ".z;\n"
-        << updates(r, "          ")
-        << "          if (first_only) break;\n"
-        << "        }\n"
-        << "        " << sv << " = (" << sv << " + 1) & mask;\n"
-        << "        " << e << " = tbl[" << sv << "];\n"
-        << "      }\n"
-        << "    }\n";
-    }
+    o << probe_chains(sp, R, [](int r) { return "keep" + std::to_string(r); }, "    ");
   } else {
     for (int r = 0; r < R; r++) o << "    if (keep" << r << ") {\n" << updates(r, "      ") << "    }\n";
   }
@@ -480,6 +533,188 @@ std::string jit_row_agg_source(const JitSpec& sp) {
   for (int a = 0; a < na; a++)
     o << "  block_reduce_acc(acc" << a << ", " << sp.aggs[a].op << ", " << sp.aggs[a].dtype << ", sh, 4, parts + (long long)blockIdx.x * "
       << na << " + " << a << ");\n";
+  o << "}\n";
+  return o.str();
+}
+
+// Partitioned join, scatter (the generated form of jp_scatter_kernel in
+// join_part.hip): workgroup g owns chunk g of the probe side and walks it in
+// tiles of 1024 * R rows.  A lane's R rows' key and column loads are issued
+// together, and the next tile's are issued as soon as the current tile sits in
+// LDS, so they are in flight while the tile is written out.  Filter terms and
+// column types are baked in; ranking, staging and the write-out are the
+// shared jp_tile.h steps.
+std::string jit_jp_scatter_source(const JpJitSpec& sp) {
+  std::ostringstream o;
+  const int R = sp.items;
+  const int np = (int)sp.pay.size();
+  const int nc = (int)sp.cols.size();
+  o << kJitPrelude << "\n";
+  o << "extern \"C\" __global__ void __launch_bounds__(1024) bolt_jit_kernel(\n"
+       "    const void* __restrict__ keyp, const unsigned char* __restrict__ kvalid, long long n, long long per,\n"
+       "    bolt::JitLits lits, bolt::JitCols cols, unsigned long long mask, int shift, int P,\n"
+       "    const unsigned int* __restrict__ counts, const unsigned int* __restrict__ base,\n"
+       "    long long* __restrict__ okeys, bolt::JpDst dst, unsigned int* __restrict__ actual) {\n"
+       "  using namespace bolt;\n"
+       "  constexpr int T = kJpBlock * " << R << ";\n"
+       "  __shared__ JpTileLds<T, " << np << "> L;\n";
+  emit_col_ptrs(o, sp.cols);
+  o << "  const int g = blockIdx.x;\n"
+       "  jp_tile_begin(L, P, g, counts, base);\n"
+       "  const long long lo = (long long)g * per;\n"
+       "  const long long hi = lo + per < n ? lo + per : n;\n";
+  const char* kt = ctype(sp.key_dtype);
+  for (int r = 0; r < R; r++) {
+    o << "  " << kt << " kraw" << r << " = 0;\n";
+    for (int c = 0; c < nc; c++) {
+      const std::string x = "x" + std::to_string(c) + "_" + std::to_string(r);
+      if (sp.cols[c].dtype == BOLT_DEC128) o << "  unsigned long long " << x << "_lo = 0, " << x << "_hi = 0;\n";
+      else o << "  " << ctype(sp.cols[c].dtype) << " " << x << " = 0;\n";
+    }
+  }
+  // loads of the tile starting at row `t` (every index clamped into the chunk)
+  auto loads = [&](const std::string& t, const char* ind) {
+    std::ostringstream l;
+    for (int r = 0; r < R; r++) {
+      const std::string ic = "ic" + std::to_string(r);
+      l << ind << "{\n"
+        << ind << "  const long long " << ic << "_ = " << t << " + " << 1024 * r << " + (long long)threadIdx.x;\n"
+        << ind << "  const long long " << ic << " = " << ic << "_ < hi ? " << ic << "_ : hi - 1;\n"
+        << ind << "  kraw" << r << " = __builtin_nontemporal_load((const " << kt << "*)keyp + " << ic << ");\n";
+      for (int c = 0; c < nc; c++) {
+        const std::string x = "x" + std::to_string(c) + "_" + std::to_string(r);
+        if (sp.cols[c].dtype == BOLT_DEC128)
+          l << ind << "  " << x << "_lo = c" << c << "[2 * " << ic << "]; " << x << "_hi = c" << c << "[2 * " << ic
+            << " + 1];\n";
+        else if (sp.cols[c].dtype == BOLT_BOOL)
+          l << ind << "  " << x << " = c" << c << "[" << ic << "];\n";
+        else
+          l << ind << "  " << x << " = __builtin_nontemporal_load(c" << c << " + " << ic << ");\n";
+      }
+      l << ind << "}\n";
+    }
+    return l.str();
+  };
+  o << "  if (lo < hi) {\n" << loads("lo", "    ") << "  }\n";
+  o << "  for (long long t0 = lo; t0 < hi; t0 += T) {\n"
+       "    long long k[" << R << "];\n"
+       "    int part[" << R << "];\n"
+       "    unsigned long long pv[" << R << "][" << (np > 0 ? np : 1) << "];\n";
+  for (int r = 0; r < R; r++) {
+    const std::string i = "i" + std::to_string(r);
+    const std::string keep = "keep" + std::to_string(r);
+    o << "    {\n"
+      << "      const long long " << i << " = t0 + " << 1024 * r << " + (long long)threadIdx.x;\n"
+      << "      bool " << keep << " = " << i << " < hi;\n";
+    if (sp.key_valid) o << "      " << keep << " = " << keep << " && vbit(kvalid, " << i << ");\n";
+    for (int t = 0; t < (int)sp.terms.size(); t++)
+      o << "      " << keep << " = " << keep << " && " << term_expr(sp.cols, sp.terms[t], t, r, i) << ";\n";
+    o << "      k[" << r << "] = (long long)kraw" << r << ";\n"
+      << "      part[" << r << "] = " << keep << " ? (int)((fmix64((unsigned long long)k[" << r
+      << "]) & mask) >> shift) : -1;\n";
+    for (int j = 0; j < np; j++) {
+      const int c = sp.pay[j];
+      const std::string x = "x" + std::to_string(c) + "_" + std::to_string(r);
+      std::string bits;
+      switch (sp.cols[c].dtype) {
+        case BOLT_F64: bits = "(unsigned long long)__double_as_longlong(" + x + ")"; break;
+        case BOLT_F32: bits = "(unsigned long long)__float_as_uint(" + x + ")"; break;
+        case BOLT_I64: bits = "(unsigned long long)" + x; break;
+        default: bits = "(unsigned long long)(unsigned int)" + x; break;
+      }
+      o << "      pv[" << r << "][" << j << "] = " << bits << ";\n";
+    }
+    o << "    }\n";
+  }
+  o << "    jp_tile_stage<T, " << np << ", " << R << ">(L, P, k, part, pv);\n"
+       "    if (t0 + T < hi) {\n" << loads("(t0 + T)", "      ") << "    }\n"
+       "    jp_tile_flush<T, " << np << ">(L, P, okeys, dst);\n"
+       "  }\n"
+       "  jp_tile_end<T, " << np << ">(L, P, g, counts, base, actual);\n"
+       "}\n";
+  return o.str();
+}
+
+std::string jp_scatter_jit_key(const JpJitSpec& sp) {
+  std::ostringstream k;
+  k << "jps:" << sp.items << ':' << sp.key_dtype << ':' << sp.key_valid << '|';
+  for (const JitSpec::Col& c : sp.cols) k << c.dtype << (c.has_valid ? 'v' : 'n') << ',';
+  k << '|';
+  for (const JitSpec::Term& t : sp.terms) k << t.col << '.' << t.op << ',';
+  k << '|';
+  for (int c : sp.pay) k << c << ',';
+  return k.str();
+}
+
+// Partitioned join, probe (the generated form of jp_probe_kernel in
+// join_part.hip).  Workgroup b serves XCD b % 8 (workgroups are dispatched to
+// the XCDs round-robin) and takes work items from that XCD's queue: an item
+// is `runs` consecutive chunk runs of one partition, the XCD's partitions in
+// order, so its workgroups probe one table region at a time, which stays in
+// its L2.  A lane's R rows' keys and carried columns are loaded together
+// (about half the rows match, so nearly every line of the runs is read
+// anyway), then their first table slots, then each row's chain.  The
+// columns of `sp` are the partitioned copies (no validity: rows with a NULL
+// input never take this form).
+std::string jit_jp_probe_source(const JitSpec& sp, int runs) {
+  std::ostringstream o;
+  const int R = sp.unroll;
+  const int nc = (int)sp.cols.size();
+  const int na = (int)sp.aggs.size();
+  o << kJitPrelude << "\n";
+  o << "extern \"C\" __global__ void __launch_bounds__(256) bolt_jit_kernel(\n"
+       "    const uint4* __restrict__ tbl, unsigned long long mask, const unsigned int* __restrict__ flags,\n"
+       "    const long long* __restrict__ pkeys, int P, int G, const unsigned int* __restrict__ counts,\n"
+       "    const unsigned int* __restrict__ base, const unsigned int* __restrict__ actual,\n"
+       "    unsigned int* __restrict__ queue, bolt::JitCols cols, bolt::Acc* __restrict__ parts) {\n"
+       "  using namespace bolt;\n";
+  emit_col_ptrs(o, sp.cols);
+  for (int a = 0; a < na; a++) o << "  Acc acc" << a << "; acc_init(acc" << a << ", " << sp.aggs[a].op << ");\n";
+  o << "  const bool first_only = flags[0] == 0u;\n"
+       "  constexpr int CG = " << runs << ";\n"
+       "  __shared__ unsigned int sh_item;\n"
+       "  const int x = blockIdx.x & 7;\n"
+       "  const unsigned int per_part = (unsigned int)(G / CG);\n"
+       "  const unsigned int items = (unsigned int)(P / 8) * per_part;\n"
+       "  for (;;) {\n"
+       "    if (threadIdx.x == 0) sh_item = atomicAdd(&queue[x], 1u);\n"
+       "    __syncthreads();\n"
+       "    const unsigned int item = sh_item;\n"
+       "    __syncthreads();\n"
+       "    if (item >= items) break;\n"
+       "    const int p = x + 8 * (int)(item / per_part);\n"
+       "    const int g0 = (int)(item % per_part) * CG;\n"
+       "    long long st[CG];\n"
+       "    unsigned int pre[CG + 1];\n"
+       "    pre[0] = 0u;\n"
+       "#pragma unroll\n"
+       "    for (int c = 0; c < CG; c++) {\n"
+       "      const long long cell = (long long)(g0 + c) * P + p;\n"
+       "      st[c] = (long long)base[p] + counts[cell];\n"
+       "      pre[c + 1] = pre[c] + actual[cell];\n"
+       "    }\n"
+       "    const unsigned int len = pre[CG];\n"
+       "    for (unsigned int r0 = 0; r0 < len; r0 += 256u * " << R << ") {\n";
+  for (int r = 0; r < R; r++) {
+    const std::string rs = std::to_string(r);
+    o << "      const unsigned int v" << rs << " = r0 + " << 256 * r << "u + threadIdx.x;\n"
+      << "      const bool in" << rs << " = v" << rs << " < len;\n"
+      << "      const unsigned int u" << rs << " = in" << rs << " ? v" << rs << " : len - 1u;\n"
+      << "      long long i" << rs << " = st[0] + u" << rs << ";\n"
+      << "#pragma unroll\n"
+      << "      for (int c = 1; c < CG; c++)\n"
+      << "        if (u" << rs << " >= pre[c]) i" << rs << " = st[c] + (u" << rs << " - pre[c]);\n"
+      << "      const long long k" << rs << " = __builtin_nontemporal_load(pkeys + i" << rs << ");\n";
+    for (int c = 0; c < nc; c++)
+      o << "      const " << ctype(sp.cols[c].dtype) << " x" << c << "_" << rs << " = __builtin_nontemporal_load(c" << c
+        << " + i" << rs << ");\n";
+    o << "      unsigned long long s" << rs << " = fmix64((unsigned long long)k" << rs << ") & mask;\n";
+  }
+  o << probe_chains(sp, R, [](int r) { return "in" + std::to_string(r); }, "      ");
+  o << "    }\n  }\n  __shared__ Acc sh[4];\n";
+  for (int a = 0; a < na; a++)
+    o << "  block_reduce_acc(acc" << a << ", " << sp.aggs[a].op << ", " << sp.aggs[a].dtype
+      << ", sh, 4, parts + (long long)blockIdx.x * " << na << " + " << a << ");\n";
   o << "}\n";
   return o.str();
 }
@@ -680,6 +915,10 @@ int jit_spec_from(const DevFilter& f, const AggList& aggs, bool probe, int key_d
   if (const char* u = getenv("BOLT_JIT_UNROLL")) {
     int v = atoi(u);
     if (v >= 1 && v <= 8) sp->unroll = v;
+  }
+  if (const char* u = getenv("BOLT_JIT_PROBE_SLOTS")) {
+    int v = atoi(u);
+    if (v >= 1 && v <= 8) sp->probe_slots = v;
   }
   auto col_index = [&](const void* d, const uint8_t* v, int dt) -> int {
     return jit_row_agg_col(d, v, dt, &sp->cols, cols);

@@ -34,21 +34,21 @@
 #include "jit.h"
 #include "join_dev.h"
 #include "reduce_common.h"
+// after the headers above, which bring in the HIP runtime and device_acc.h
+#include "jp_tile.h"
 
 namespace bolt {
 
-constexpr int kJpBlock = 1024;     // histogram and scatter workgroups (one per CU)
-constexpr int kJpGrid = 256;       // chunks: one per workgroup
-constexpr int kJpMaxParts = 1024;
-constexpr int kJpMaxPay = 2;       // probe columns carried with the key
+constexpr int kJpGrid = 256;       // chunks: one per histogram / scatter workgroup
 constexpr int kJpProbeBlock = 256;
 constexpr int kJpProbeUnroll = 4;  // rows in flight per lane in the probe
 constexpr int kJpProbeRuns = 4;    // chunk runs per probe work item (kJpGrid is a multiple)
 
+// Probe columns carried with the key: sources and their partitioned copies.
 struct JpPay {
   const void* src[kJpMaxPay];
-  void* dst[kJpMaxPay];
-  int32_t w[kJpMaxPay];  // 4 or 8 bytes
+  int32_t dtype[kJpMaxPay];
+  JpDst dst;
   int32_t n;
 };
 
@@ -121,33 +121,6 @@ __global__ __launch_bounds__(kJpMaxParts) void jp_base_kernel(const uint32_t* __
   if (t < P) base[t] = s[t] - v;
 }
 
-// Block-wide exclusive scan of P (<= kJpMaxParts) counters in LDS, in place
-// into `off` (cnt is left as is).  All kJpBlock threads take part.
-__device__ __forceinline__ void jp_block_scan(const uint32_t* cnt, uint32_t* off, int P, uint32_t* wsum) {
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const uint32_t v = t < P ? cnt[t] : 0u;
-  uint32_t incl = v;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t o = __shfl_up(incl, d);
-    if (lane >= d) incl += o;
-  }
-  if (lane == 63) wsum[wave] = incl;
-  __syncthreads();
-  if (t < 64) {
-    const uint32_t w = t < kJpBlock / 64 ? wsum[t] : 0u;
-    uint32_t wi = w;
-#pragma unroll
-    for (int d = 1; d < 16; d <<= 1) {
-      const uint32_t o = __shfl_up(wi, d);
-      if (lane >= d) wi += o;
-    }
-    if (t < kJpBlock / 64) wsum[t] = wi - w;
-  }
-  __syncthreads();
-  if (t < P) off[t] = wsum[wave] + incl - v;
-}
-
 // Chunk g, in tiles of kJpBlock * ITEMS rows: the rows that pass the filter
 // (and have a valid key) are ranked within their partition with LDS atomics,
 // staged in LDS in partition order, and written to the partitioned key and
@@ -162,74 +135,47 @@ __global__ __launch_bounds__(kJpBlock) void jp_scatter_kernel(const KeyT* __rest
                                                               int64_t* __restrict__ okeys,
                                                               uint32_t* __restrict__ actual) {
   constexpr int T = kJpBlock * ITEMS;
-  extern __shared__ __align__(16) uint8_t jp_lds[];
-  int64_t* skey = reinterpret_cast<int64_t*>(jp_lds);
-  uint64_t* spay = reinterpret_cast<uint64_t*>(jp_lds + (size_t)T * 8);  // [NPAY][T]
-  uint32_t* cur = reinterpret_cast<uint32_t*>(jp_lds + (size_t)T * 8 * (1 + NPAY));
-  uint32_t* tcnt = cur + kJpMaxParts;
-  uint32_t* toff = tcnt + kJpMaxParts;
-  uint16_t* spart = reinterpret_cast<uint16_t*>(toff + kJpMaxParts);
-  __shared__ uint32_t wsum[kJpBlock / 64];
+  __shared__ JpTileLds<T, NPAY> L;
   const int g = blockIdx.x;
-  for (int p = threadIdx.x; p < P; p += kJpBlock) {
-    cur[p] = base[p] + counts[(int64_t)g * P + p];
-    tcnt[p] = 0;
-  }
-  __syncthreads();
+  jp_tile_begin(L, P, g, counts, base);
   const int64_t lo = (int64_t)g * per;
   const int64_t hi = lo + per < n ? lo + per : n;
   for (int64_t t0 = lo; t0 < hi; t0 += T) {
-    int64_t k[ITEMS];
-    uint64_t pv[ITEMS][NPAY > 0 ? NPAY : 1];
+    i64_t k[ITEMS];
+    u64_t pv[ITEMS][NPAY > 0 ? NPAY : 1];
     int part[ITEMS];
-    uint32_t rank[ITEMS];
 #pragma unroll
     for (int u = 0; u < ITEMS; u++) {
       const int64_t i = t0 + u * kJpBlock + threadIdx.x;
-      const int64_t ic = i < hi ? i : hi - 1;
-      k[u] = jp_key(keys, ic);
+      k[u] = jp_key(keys, i < hi ? i : hi - 1);
+    }
+    // the width test is uniform, so each column's ITEMS loads issue together
 #pragma unroll
-      for (int c = 0; c < NPAY; c++)
-        pv[u][c] = pay.w[c] == 8 ? __builtin_nontemporal_load(static_cast<const uint64_t*>(pay.src[c]) + ic)
-                                 : (uint64_t)__builtin_nontemporal_load(static_cast<const uint32_t*>(pay.src[c]) + ic);
+    for (int c = 0; c < NPAY; c++) {
+      if (pay.dst.w[c] == 8) {
+#pragma unroll
+        for (int u = 0; u < ITEMS; u++) {
+          const int64_t i = t0 + u * kJpBlock + threadIdx.x;
+          pv[u][c] = __builtin_nontemporal_load(static_cast<const uint64_t*>(pay.src[c]) + (i < hi ? i : hi - 1));
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < ITEMS; u++) {
+          const int64_t i = t0 + u * kJpBlock + threadIdx.x;
+          pv[u][c] = __builtin_nontemporal_load(static_cast<const uint32_t*>(pay.src[c]) + (i < hi ? i : hi - 1));
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < ITEMS; u++) {
+      const int64_t i = t0 + u * kJpBlock + threadIdx.x;
       const bool keep = i < hi && bit_valid(kvalid, i) && eval_filter_nt<NT>(f, i);
       part[u] = keep ? (int)((fmix64((uint64_t)k[u]) & mask) >> shift) : -1;
     }
-#pragma unroll
-    for (int u = 0; u < ITEMS; u++) rank[u] = part[u] >= 0 ? atomicAdd(&tcnt[part[u]], 1u) : 0u;
-    __syncthreads();
-    jp_block_scan(tcnt, toff, P, wsum);
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < ITEMS; u++) {
-      if (part[u] < 0) continue;
-      const uint32_t j = toff[part[u]] + rank[u];
-      skey[j] = k[u];
-#pragma unroll
-      for (int c = 0; c < NPAY; c++) spay[c * T + j] = pv[u][c];
-      spart[j] = (uint16_t)part[u];
-    }
-    __syncthreads();
-    const uint32_t m = toff[P - 1] + tcnt[P - 1];
-    for (uint32_t j = threadIdx.x; j < m; j += kJpBlock) {
-      const int p = spart[j];
-      const uint64_t pos = (uint64_t)cur[p] + (j - toff[p]);
-      okeys[pos] = skey[j];
-#pragma unroll
-      for (int c = 0; c < NPAY; c++) {
-        if (pay.w[c] == 8) static_cast<uint64_t*>(pay.dst[c])[pos] = spay[c * T + j];
-        else static_cast<uint32_t*>(pay.dst[c])[pos] = (uint32_t)spay[c * T + j];
-      }
-    }
-    __syncthreads();
-    for (int p = threadIdx.x; p < P; p += kJpBlock) {
-      cur[p] += tcnt[p];
-      tcnt[p] = 0;
-    }
-    __syncthreads();
+    jp_tile_stage<T, NPAY, ITEMS>(L, P, k, part, pv);
+    jp_tile_flush<T, NPAY>(L, P, reinterpret_cast<i64_t*>(okeys), pay.dst);
   }
-  for (int p = threadIdx.x; p < P; p += kJpBlock)
-    actual[(int64_t)g * P + p] = cur[p] - (base[p] + counts[(int64_t)g * P + p]);
+  jp_tile_end<T, NPAY>(L, P, g, counts, base, actual);
 }
 
 // Probe.  Workgroup b serves XCD x = b % 8 (workgroups are dispatched to the
@@ -324,8 +270,10 @@ __global__ __launch_bounds__(kJpProbeBlock) void jp_probe_kernel(
 
 // Whether the partitioned form runs for this table and probe side.  It is
 // opt-in (BOLT_JOIN_PARTITIONED=1, where its limits allow): measured on the
-// 16M x 256M fused probe it takes 8.3 ms (hist 0.35, scatter 3.36, probe 4.58)
-// against 5.3 ms for the random probe, so the random probe stays the default.
+// 16M x 256M fused probe its generated kernels take 6.8 ms (hist 0.35,
+// scatter 2.72, probe 3.70) against 5.3 ms for the random probe, which stays
+// the default.  Both probes are bound by the L2 requests one CU's L1 keeps in
+// flight; the region-local one only shortens their latency.
 bool join_partitioned_wanted(const bolt_join_table_t* t, int64_t n_probe, const AggList& aggs) {
   const char* e = getenv("BOLT_JOIN_PARTITIONED");
   const bool forced_on = e && e[0] == '1';
@@ -352,13 +300,9 @@ static void launch_scatter(const bolt_column_t& key, int64_t per, const DevFilte
                            uint64_t mask, int shift, int P, const uint32_t* counts, const uint32_t* base,
                            int64_t* okeys, uint32_t* actual, hipStream_t s) {
   constexpr int ITEMS = NPAY <= 1 ? 8 : 4;
-  constexpr int T = kJpBlock * ITEMS;
-  const size_t lds = (size_t)T * 8 * (1 + NPAY) + 3 * kJpMaxParts * 4 + (size_t)T * 2;
-  auto* kern = jp_scatter_kernel<KeyT, NT, NPAY, ITEMS>;
-  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds);
-  kern<<<kJpGrid, kJpBlock, lds, s>>>(static_cast<const KeyT*>(key.data), key.validity, key.length, per, f, pay,
-                                      mask, shift, P, counts, base, okeys, actual);
+  jp_scatter_kernel<KeyT, NT, NPAY, ITEMS><<<kJpGrid, kJpBlock, 0, s>>>(
+      static_cast<const KeyT*>(key.data), key.validity, key.length, per, f, pay, mask, shift, P, counts, base, okeys,
+      actual);
 }
 
 template <typename KeyT, int NT>
@@ -382,6 +326,47 @@ static void launch_scatter_all(int nt, int npay, const bolt_column_t& key, int64
     case 2: launch_scatter_np<KeyT, 2>(npay, key, per, f, pay, mask, shift, P, counts, base, okeys, actual, s); break;
     default: launch_scatter_np<KeyT, -1>(npay, key, per, f, pay, mask, shift, P, counts, base, okeys, actual, s); break;
   }
+}
+
+// Shape of the generated scatter: filter columns and carried columns share one
+// column set, so a column that is both is loaded once.
+static int jp_scatter_spec(const DevFilter& f, const JpPay& pay, const bolt_column_t& key, JpJitSpec* sp,
+                           JitLits* lits, JitCols* cols) {
+  *sp = JpJitSpec();
+  memset(lits, 0, sizeof *lits);
+  memset(cols, 0, sizeof *cols);
+  sp->items = pay.n <= 1 ? 8 : 4;
+  if (const char* e = getenv("BOLT_JOIN_SCATTER_ITEMS")) {  // rows per lane per tile: 2, 4 or 8 (LDS permitting)
+    const int v = atoi(e);
+    if ((v == 2 || v == 4 || v == 8) && (v < 8 || pay.n <= 1)) sp->items = v;
+  }
+  sp->key_dtype = key.dtype == BOLT_I64 ? BOLT_I64 : BOLT_I32;
+  sp->key_valid = key.validity != nullptr;
+  for (int t = 0; t < f.n; t++) {
+    const int c = jit_row_agg_col(f.t[t].data, f.t[t].validity, f.t[t].dtype, &sp->cols, cols);
+    if (c < 0) return set_error(BOLT_ERR_UNSUPPORTED, "jit: too many columns");
+    sp->terms.push_back({c, f.t[t].op});
+    lits->i[t] = f.t[t].lit_i64;
+    lits->f[t] = f.t[t].lit_f64;
+    lits->hi[t] = f.t[t].lit_hi;
+  }
+  for (int j = 0; j < pay.n; j++) {
+    const int c = jit_row_agg_col(pay.src[j], nullptr, pay.dtype[j], &sp->cols, cols);
+    if (c < 0) return set_error(BOLT_ERR_UNSUPPORTED, "jit: too many columns");
+    sp->pay.push_back(c);
+  }
+  return BOLT_OK;
+}
+
+static std::string jp_probe_jit_key(const JitSpec& sp, int runs) {
+  std::string k = "jpp:" + std::to_string(runs) + ':' + std::to_string(sp.unroll) + ':' +
+                  std::to_string(sp.probe_slots) + ':' + std::to_string(sp.carry_dtype) + '|';
+  for (const JitSpec::Col& c : sp.cols) k += std::to_string(c.dtype) + (c.has_valid ? "v," : "n,");
+  k += '|';
+  for (const JitSpec::Agg& g : sp.aggs)
+    k += std::to_string(g.op) + '.' + std::to_string(g.src) + '.' + std::to_string(g.col) + '.' +
+         std::to_string(g.dtype) + ',';
+  return k;
 }
 
 // The partitioned probe + aggregate: `all` as make_agg_list built it over the
@@ -416,12 +401,13 @@ int join_probe_aggregate_partitioned(hipStream_t s, const bolt_join_table_t* t, 
     if (c == pay.n) {
       if (pay.n == kJpMaxPay) return set_error(BOLT_ERR_OTHER, "partitioned join: too many probe columns");
       pay.src[c] = d.data;
-      pay.w[c] = dtype_size(d.dtype);
-      BOLT_TRY(pbufs[c].alloc((size_t)((n * pay.w[c] + 7) / 8), s));
-      pay.dst[c] = pbufs[c].ptr;
+      pay.dtype[c] = d.dtype;
+      pay.dst.w[c] = dtype_size(d.dtype);
+      BOLT_TRY(pbufs[c].alloc((size_t)((n * pay.dst.w[c] + 7) / 8), s));
+      pay.dst.d[c] = pbufs[c].ptr;
       pay.n++;
     }
-    d.data = pay.dst[c];
+    d.data = pay.dst.d[c];
   }
   const int64_t per = (n + kJpGrid - 1) / kJpGrid;
   DevBuf<uint32_t> counts, actual, totals, base, queue;
@@ -443,6 +429,27 @@ int join_probe_aggregate_partitioned(hipStream_t s, const bolt_join_table_t* t, 
   if (pgrid < 8) pgrid = 8;
   BOLT_TRY(parts.alloc((size_t)pgrid * n_aggs, s));
   const int nt = f.n;
+  // generated scatter and probe (hipRTC), else the precompiled kernels
+  hipFunction_t jscatter = nullptr, jprobe = nullptr;
+  JpJitSpec ssp;
+  JitLits slits;
+  JitCols scols;
+  JitSpec psp;
+  JitLits plits;
+  JitCols pcols;
+  if (jit_enabled()) {
+    DevFilter none;
+    memset(&none, 0, sizeof none);
+    if (jp_scatter_spec(f, pay, key, &ssp, &slits, &scols) == BOLT_OK &&
+        jit_spec_from(none, pa, true, BOLT_I64, false, t->carry_dtype, &psp, &plits, &pcols) == BOLT_OK &&
+        jit_get_keyed(jp_scatter_jit_key(ssp), [&] { return jit_jp_scatter_source(ssp); }, &jscatter) == BOLT_OK &&
+        jit_get_keyed(jp_probe_jit_key(psp, kJpProbeRuns), [&] { return jit_jp_probe_source(psp, kJpProbeRuns); },
+                      &jprobe) == BOLT_OK) {
+    } else {
+      jscatter = jprobe = nullptr;
+    }
+    clear_error();
+  }
   prof_begin(s);
   if (key.dtype == BOLT_I64) {
     jp_hist_kernel<int64_t><<<kJpGrid, kJpBlock, 0, s>>>(static_cast<const int64_t*>(key.data), key.validity, n,
@@ -453,13 +460,39 @@ int join_probe_aggregate_partitioned(hipStream_t s, const bolt_join_table_t* t, 
   }
   jp_offsets_kernel<<<(P + 255) / 256, 256, 0, s>>>(counts.ptr, kJpGrid, P, totals.ptr);
   jp_base_kernel<<<1, kJpMaxParts, 0, s>>>(totals.ptr, P, base.ptr);
-  if (key.dtype == BOLT_I64)
+  if (jscatter) {
+    const void* kp = key.data;
+    const uint8_t* kv = key.validity;
+    long long nn = n, pp = per;
+    unsigned long long m = t->mask;
+    int sh = shift, PP = P;
+    JpDst dst = pay.dst;
+    void* args[] = {&kp, &kv, &nn, &pp, &slits, &scols, &m, &sh, &PP, &counts.ptr, &base.ptr, &okeys.ptr, &dst,
+                    &actual.ptr};
+    BOLT_HIP(hipModuleLaunchKernel(jscatter, kJpGrid, 1, 1, kJpBlock, 1, 1, 0, s, args, nullptr));
+  } else if (key.dtype == BOLT_I64) {
     launch_scatter_all<int64_t>(nt, pay.n, key, per, f, pay, t->mask, shift, P, counts.ptr, base.ptr, okeys.ptr,
                                 actual.ptr, s);
-  else
+  } else {
     launch_scatter_all<int32_t>(nt, pay.n, key, per, f, pay, t->mask, shift, P, counts.ptr, base.ptr, okeys.ptr,
                                 actual.ptr, s);
+  }
   BOLT_HIP(hipGetLastError());
+  if (jprobe) {
+    const uint4* tb = t->tbl;
+    unsigned long long m = t->mask;
+    const uint32_t* fl = t->flags;
+    int PP = P, GG = kJpGrid;
+    void* args[] = {&tb, &m, &fl, &okeys.ptr, &PP, &GG, &counts.ptr, &base.ptr, &actual.ptr, &queue.ptr, &pcols,
+                    &parts.ptr};
+    BOLT_HIP(hipModuleLaunchKernel(jprobe, pgrid, 1, 1, kJpProbeBlock, 1, 1, 0, s, args, nullptr));
+    for (int first = 0; first < n_aggs; first += 4) {
+      const int na = n_aggs - first < 4 ? n_aggs - first : 4;
+      BOLT_TRY(launch_finish(na, parts.ptr, (int)pgrid, n_aggs, first, agg_chunk(all, first, na), outs + first, s));
+    }
+    prof_end(s);
+    return BOLT_OK;
+  }
   for (int first = 0; first < n_aggs; first += 4) {
     const int na = n_aggs - first < 4 ? n_aggs - first : 4;
     AggList chunk = agg_chunk(pa, first, na);

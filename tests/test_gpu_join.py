@@ -190,17 +190,21 @@ def test_region_build_matches_atomic_build(gpu, monkeypatch, load, kernel_path):
     t.release()
 
 
+@pytest.mark.parametrize("jit", ["1", "0"])
 @pytest.mark.parametrize("key_dt", [N.I64, N.I32])
 @pytest.mark.parametrize("part_bits", ["3", "6", "10"])
-def test_partitioned_probe_aggregate_matches_oracle(gpu, kernel_path, monkeypatch, key_dt, part_bits):
+def test_partitioned_probe_aggregate_matches_oracle(gpu, kernel_path, monkeypatch, key_dt, part_bits, jit):
     """The radix-partitioned probe + aggregate against the oracle's host join:
     duplicate build keys, NULL probe keys, three filter terms over two
     columns, two probe columns carried with the key (Float64, Int32), the
-    carried build column, and five aggregates (two probe launches)."""
+    carried build column, and five aggregates (one generated probe, or two
+    launches of the precompiled one), through the hipRTC-generated scatter
+    and probe and through the precompiled kernels (BOLT_JIT=0)."""
     if kernel_path != "partitioned":
         pytest.skip("a test of the partitioned form")
+    monkeypatch.setenv("BOLT_JIT", jit)
     monkeypatch.setenv("BOLT_JOIN_PART_BITS", part_bits)
-    print(f"[case] partitioned key={key_dt} bits={part_bits}", flush=True)
+    print(f"[case] partitioned key={key_dt} bits={part_bits} jit={jit}", flush=True)
     rng = np.random.default_rng(int(part_bits) * 10 + key_dt)
     nb, npr = 400_000, 2_500_000
     np_dt = np.int64 if key_dt == N.I64 else np.int32
