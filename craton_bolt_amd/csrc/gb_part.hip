@@ -40,9 +40,20 @@ __device__ __forceinline__ uint32_t part_of(uint64_t key, int pbits) {
   return pbits == 0 ? 0u : (uint32_t)(mix64(key) >> (64 - pbits));
 }
 
+// Raw image of a 4- or 8-byte column value: the value `pre` loaded earlier
+// when the aggregate reads the column (pdata, pdtype) it was loaded from.
+__device__ __forceinline__ uint64_t raw_or_pre(const GAgg& g, int64_t row, const void* pdata, int pdtype,
+                                               uint64_t pre) {
+  if (g.data == pdata && g.dtype == pdtype) return pre;
+  return g.dtype == BOLT_I64 || g.dtype == BOLT_F64 ? static_cast<const uint64_t*>(g.data)[row]
+                                                    : (uint64_t) static_cast<const uint32_t*>(g.data)[row];
+}
+
 // Accumulator words of a group holding the single row `row` (the update of
-// update_slot applied to the identity).
-__device__ __forceinline__ void row_words(const GPlan& P, int64_t row, unsigned long long* dst, size_t stride) {
+// update_slot applied to the identity).  The value of column (pdata, pdtype)
+// at `row` may come preloaded in `pre` (pdata = nullptr: none).
+__device__ __forceinline__ void row_words(const GPlan& P, int64_t row, unsigned long long* dst, size_t stride,
+                                          const void* pdata = nullptr, int pdtype = 0, uint64_t pre = 0) {
   for (int a = 0; a < P.n; a++) {
     const GAgg& g = P.a[a];
     unsigned long long* w = dst + (size_t)g.w * stride;
@@ -53,8 +64,10 @@ __device__ __forceinline__ void row_words(const GPlan& P, int64_t row, unsigned 
     switch (g.kind) {
       case K_ISUM: {
         int64_t v = 0;
-        if (valid) v = g.dtype == BOLT_I64 ? static_cast<const int64_t*>(g.data)[row]
-                                           : (int64_t) static_cast<const int32_t*>(g.data)[row];
+        if (valid) {
+          const uint64_t r = raw_or_pre(g, row, pdata, pdtype, pre);
+          v = g.dtype == BOLT_I64 ? (int64_t)r : (int64_t)(int32_t)(uint32_t)r;
+        }
         w[0] = (uint64_t)v;
         w[stride] = (uint64_t)(v >> 63);
         break;
@@ -75,22 +88,25 @@ __device__ __forceinline__ void row_words(const GPlan& P, int64_t row, unsigned 
       case K_AVG:
       case K_VAR: {
         double v = 0.0;
-        if (valid)
-          v = g.dtype == BOLT_F64 ? static_cast<const double*>(g.data)[row]
-            : g.dtype == BOLT_F32 ? (double)static_cast<const float*>(g.data)[row]
-            : g.dtype == BOLT_I64 ? (double)static_cast<const int64_t*>(g.data)[row]
-                                  : (double)static_cast<const int32_t*>(g.data)[row];
+        if (valid) {
+          const uint64_t r = raw_or_pre(g, row, pdata, pdtype, pre);
+          v = g.dtype == BOLT_F64 ? as_f64(r)
+            : g.dtype == BOLT_F32 ? (double)__uint_as_float((uint32_t)r)
+            : g.dtype == BOLT_I64 ? (double)(int64_t)r
+                                  : (double)(int32_t)(uint32_t)r;
+        }
         w[0] = as_u64(v);
         break;
       }
       case K_MINMAX: {
         uint64_t k = g.op == BOLT_AGG_MIN ? ~0ull : 0ull;
         if (valid) {
+          const uint64_t r = raw_or_pre(g, row, pdata, pdtype, pre);
           switch (g.dtype) {
-            case BOLT_F64: k = f64_order_key(static_cast<const double*>(g.data)[row]); break;
-            case BOLT_F32: k = f64_order_key((double)static_cast<const float*>(g.data)[row]); break;
-            case BOLT_I64: k = (uint64_t) static_cast<const int64_t*>(g.data)[row] ^ 0x8000000000000000ull; break;
-            default: k = (uint64_t)(int64_t) static_cast<const int32_t*>(g.data)[row] ^ 0x8000000000000000ull; break;
+            case BOLT_F64: k = f64_order_key(as_f64(r)); break;
+            case BOLT_F32: k = f64_order_key((double)__uint_as_float((uint32_t)r)); break;
+            case BOLT_I64: k = r ^ 0x8000000000000000ull; break;
+            default: k = (uint64_t)(int64_t)(int32_t)(uint32_t)r ^ 0x8000000000000000ull; break;
           }
         }
         w[0] = k;
@@ -215,6 +231,20 @@ __device__ __forceinline__ void spilled_record(const PartSrc& src, const KeySpec
   row_words(P, (int64_t)src.rows[j], r + 1, 1);
 }
 
+// The first aggregate input the words of a spilled row read (a 4- or 8-byte
+// column), whose values a tile loads for all its spilled rows at once; -1 if
+// none.
+__device__ __forceinline__ int first_value_agg(const GPlan& P) {
+  for (int a = 0; a < P.n; a++) {
+    const GAgg& g = P.a[a];
+    if (g.op == BOLT_AGG_COUNT_STAR || g.kind == K_COUNT || g.kind == K_DSUM) continue;
+    if (g.dtype == BOLT_I64 || g.dtype == BOLT_F64 || g.dtype == BOLT_I32 || g.dtype == BOLT_F32 ||
+        g.dtype == BOLT_DATE32)
+      return a;
+  }
+  return -1;
+}
+
 // Merge records, one per item: the key then the W accumulator words, `stride`
 // = 1 + W 64-bit words apart.
 struct PartRecs {
@@ -307,6 +337,15 @@ __global__ __launch_bounds__(kStageBlock) void part_stage_kernel(PartSrc src, Ke
     __syncthreads();
     int rank[kStageMaxIpt];
     uint32_t dig[kStageMaxIpt];
+    // every item's partition id (PASS 1) or record key (PASS 2) first, with
+    // clamped indices, so the loads are in flight together
+    uint64_t raw[kStageMaxIpt];
+#pragma unroll
+    for (int u = 0; u < kStageMaxIpt; u++) {
+      int64_t i = t0 + (u < ipt ? u : 0) * kStageBlock + threadIdx.x;
+      i = i < hi ? i : hi - 1;
+      raw[u] = PASS == 1 ? (uint64_t)pid[i] : in[(uint64_t)i * stride];
+    }
 #pragma unroll
     for (int u = 0; u < kStageMaxIpt; u++) {
       rank[u] = -1;
@@ -316,7 +355,7 @@ __global__ __launch_bounds__(kStageBlock) void part_stage_kernel(PartSrc src, Ke
       if (i >= hi) continue;
       uint32_t p;
       if (PASS == 1) {
-        p = pid[i];
+        p = (uint32_t)raw[u];
         if (p == 0xFFFFu) {  // tagged partial record: merged straight into the global table
           const int64_t g = gslot(t, src.ptag[i], src.pkeys[i]);
           if (g >= 0) merge_slot(P, t.words, t.cap + 2, (uint32_t)g, src.pwords + i, (size_t)src.pcap);
@@ -324,7 +363,7 @@ __global__ __launch_bounds__(kStageBlock) void part_stage_kernel(PartSrc src, Ke
         }
         dig[u] = fb > 0 ? (p >> fb) : p;
       } else {
-        p = part_of(in[(uint64_t)i * stride], pbits);
+        p = part_of(raw[u], pbits);
         dig[u] = p & dmask;
       }
       rank[u] = (int)atomicAdd(&cnt[dig[u]], 1u);
@@ -342,6 +381,31 @@ __global__ __launch_bounds__(kStageBlock) void part_stage_kernel(PartSrc src, Ke
       gbase[threadIdx.x] = v ? atomicAdd(&cursor[dest], (unsigned long long)v) : 0ull;
     }
     __syncthreads();
+    // PASS 1: the spilled rows' ids, captured keys and first input value for
+    // all of this lane's items, loaded before any record is built (indices
+    // clamped, so the loads carry no branches and are in flight together)
+    int64_t srow[kStageMaxIpt];
+    uint64_t skey[kStageMaxIpt], sval[kStageMaxIpt];
+    const int va = PASS == 1 ? first_value_agg(P) : -1;
+    const int64_t n_spill = src.n_items - src.n_parts;
+    if (PASS == 1 && n_spill > 0) {
+#pragma unroll
+      for (int u = 0; u < kStageMaxIpt; u++) {
+        // slots past this tile's items per lane repeat item 0's (same lines)
+        int64_t j = t0 + (u < ipt ? u : 0) * kStageBlock + threadIdx.x - src.n_parts;
+        j = j < 0 ? 0 : (j >= n_spill ? n_spill - 1 : j);
+        srow[u] = (int64_t)src.rows[j];
+        skey[u] = src.keys ? src.keys[j] : 0ull;
+      }
+      if (va >= 0) {
+        const GAgg& g = P.a[va];
+        const bool w8 = g.dtype == BOLT_I64 || g.dtype == BOLT_F64;
+#pragma unroll
+        for (int u = 0; u < kStageMaxIpt; u++)
+          sval[u] = w8 ? static_cast<const uint64_t*>(g.data)[srow[u]]
+                       : (uint64_t) static_cast<const uint32_t*>(g.data)[srow[u]];
+      }
+    }
 #pragma unroll
     for (int u = 0; u < kStageMaxIpt; u++) {
       if (rank[u] < 0) continue;
@@ -356,7 +420,11 @@ __global__ __launch_bounds__(kStageBlock) void part_stage_kernel(PartSrc src, Ke
         r[0] = src.pkeys[i];
         for (int w = 0; w < W; w++) r[1 + w] = src.pwords[(size_t)w * src.pcap + i];
       } else {
-        spilled_record(src, ks, P, i - src.n_parts, r);
+        uint64_t key = skey[u];
+        if (!src.keys) (void)row_key(ks, srow[u], &key);
+        r[0] = key;
+        if (va >= 0) row_words(P, srow[u], r + 1, 1, P.a[va].data, P.a[va].dtype, sval[u]);
+        else row_words(P, srow[u], r + 1, 1);
       }
     }
     __syncthreads();
@@ -419,6 +487,34 @@ __global__ __launch_bounds__(kGbBlock) void part_reduce_kernel(GTable t, GPlan P
   // the lane waits on one HBM round trip per four records (the words then sit
   // in the lines the keys brought in)
   constexpr int kU = 4;
+  if (P.words <= 2) {
+    // narrow plans: each record's words are loaded with its key and merged
+    // from registers, so a lane waits on one round trip per four records
+    const uint64_t st = (uint64_t)recs.stride;
+    for (int64_t base = r0 + threadIdx.x; base < r1; base += (int64_t)kU * kGbBlock) {
+      unsigned long long keys[kU], v0[kU], v1[kU];
+#pragma unroll
+      for (int u = 0; u < kU; u++) {
+        int64_t r = base + (int64_t)u * kGbBlock;
+        r = r < r1 ? r : r1 - 1;
+        const unsigned long long* rec = recs.rec + (uint64_t)r * st;
+        keys[u] = rec[0];
+        v0[u] = rec[1];
+        v1[u] = P.words == 2 ? rec[2] : 0ull;
+      }
+#pragma unroll
+      for (int u = 0; u < kU; u++) {
+        if (base + (int64_t)u * kGbBlock >= r1) continue;
+        const int slot = gb_lds_slot(skeys, S, keys[u], &sused[2]);
+        if (slot >= 0) {
+          merge_slot2(P, swords, SS, (uint32_t)slot, v0[u], v1[u]);
+        } else {  // the partition holds more groups than the LDS table: straight to HBM
+          const int64_t g = gslot(t, 0, keys[u]);
+          if (g >= 0) merge_slot2(P, t.words, t.cap + 2, (uint32_t)g, v0[u], v1[u]);
+        }
+      }
+    }
+  } else
   for (int64_t base = r0 + threadIdx.x; base < r1; base += (int64_t)kU * kGbBlock) {
     unsigned long long keys[kU];
 #pragma unroll

@@ -202,6 +202,30 @@ __device__ __forceinline__ void merge_slot(const GPlan& P, unsigned long long* w
   }
 }
 
+// merge_slot for a plan of at most two words, whose words v0, v1 are in
+// registers (word k of the record is v0 for k = 0, else v1).
+__device__ __forceinline__ void merge_slot2(const GPlan& P, unsigned long long* words, size_t stride, uint32_t slot,
+                                            unsigned long long v0, unsigned long long v1) {
+  for (int a = 0; a < P.n; a++) {
+    const GAgg& g = P.a[a];
+    unsigned long long* w = words + (size_t)g.w * stride + slot;
+    const unsigned long long q0 = g.w == 0 ? v0 : v1;
+    if (g.nw >= 0 && g.nw != g.w) {
+      const unsigned long long c = g.nw == 0 ? v0 : v1;
+      if (c) atomicAdd(words + (size_t)g.nw * stride + slot, c);
+    }
+    switch (g.kind) {
+      case K_COUNT: if (q0) atomicAdd(w, q0); break;
+      case K_ISUM: atomic_add128(w, w + stride, q0, v1); break;  // two words: the sum is words 0 and 1
+      case K_FSUM: case K_AVG: case K_VAR: unsafeAtomicAdd(reinterpret_cast<double*>(w), as_f64(q0)); break;
+      case K_MINMAX:
+        if (g.op == BOLT_AGG_MIN) atomicMin(w, q0);
+        else atomicMax(w, q0);
+        break;
+    }
+  }
+}
+
 __device__ __forceinline__ unsigned long long word_identity(const GPlan& P, int w) {
   for (int a = 0; a < P.n; a++)
     if (P.a[a].kind == K_MINMAX && P.a[a].w == w) return P.a[a].op == BOLT_AGG_MIN ? ~0ull : 0ull;
