@@ -156,7 +156,6 @@ __global__ __launch_bounds__(256) void part_compact_spill_kernel(const uint32_t*
     if (skey) dkey[off + i] = skey[from];
   }
 }
-
 __global__ __launch_bounds__(kPartBlock) void part_hist_kernel(PartSrc src, KeySpec ks, int pbits, int64_t chunk,
                                                                uint16_t* __restrict__ pid,
                                                                uint32_t* __restrict__ hist,
@@ -339,12 +338,15 @@ __global__ __launch_bounds__(kStageBlock) void part_stage_kernel(PartSrc src, Ke
     uint32_t dig[kStageMaxIpt];
     // every item's partition id (PASS 1) or record key (PASS 2) first, with
     // clamped indices, so the loads are in flight together
-    uint64_t raw[kStageMaxIpt];
+    uint64_t raw[kStageMaxIpt], rw1[kStageMaxIpt], rw2[kStageMaxIpt];
+    const bool narrow = PASS == 2 && stride <= 3;  // PASS 2: a record of at most three words is loaded whole
 #pragma unroll
     for (int u = 0; u < kStageMaxIpt; u++) {
       int64_t i = t0 + (u < ipt ? u : 0) * kStageBlock + threadIdx.x;
       i = i < hi ? i : hi - 1;
       raw[u] = PASS == 1 ? (uint64_t)pid[i] : in[(uint64_t)i * stride];
+      rw1[u] = narrow && stride >= 2 ? in[(uint64_t)i * stride + 1] : 0ull;
+      rw2[u] = narrow && stride >= 3 ? in[(uint64_t)i * stride + 2] : 0ull;
     }
 #pragma unroll
     for (int u = 0; u < kStageMaxIpt; u++) {
@@ -413,7 +415,11 @@ __global__ __launch_bounds__(kStageBlock) void part_stage_kernel(PartSrc src, Ke
       const uint32_t sl = pre[dig[u]] + (uint32_t)rank[u];
       sdig[sl] = (uint8_t)dig[u];
       unsigned long long* r = stage + (size_t)sl * stride;
-      if (PASS == 2) {
+      if (PASS == 2 && narrow) {
+        r[0] = raw[u];
+        if (stride >= 2) r[1] = rw1[u];
+        if (stride >= 3) r[2] = rw2[u];
+      } else if (PASS == 2) {
         const unsigned long long* q = in + (uint64_t)i * stride;
         for (int w = 0; w < stride; w++) r[w] = q[w];
       } else if (i < src.n_parts) {
