@@ -87,17 +87,22 @@ static void account_free(void* p) {
   }
 }
 
-// ---- large-block cache ------------------------------------------------------
+// ---- block cache ------------------------------------------------------------
 // ROCm's stream-ordered allocator maps and unmaps large blocks on every
 // hipMallocAsync / hipFreeAsync even with the pool's release threshold at its
 // maximum (about 0.5 ms of host time per GiB on MI355X), which was a quarter
-// of a 256M-row GROUP BY call.  Blocks of 32 MiB and more freed outside graph
-// capture are kept per device, each with an event recorded on the freeing
-// stream, and handed to a later allocation of at most their size and more
-// than half of it, whose stream first waits for that event (so the reuse is
-// ordered after every use before the free, whatever the streams).  At most
-// 32 GiB is kept; an allocation that fails empties the cache and is retried.
-
+// of a 256M-row GROUP BY call, and each hipFreeAsync of a small block costs
+// 8-28 us of host time (about 70 us of a 1000-group GROUP BY call, whose
+// temporaries are freed before it returns).  Blocks freed outside graph
+// capture are therefore kept per device and handed to later allocations:
+// blocks under 32 MiB by power-of-two size class (a request is rounded up to
+// its class when allocated), larger ones to a request of at most their size
+// and more than half of it.  Each kept block carries an event recorded on the
+// freeing stream, and the stream that takes it first waits for that event, so
+// the reuse is ordered after every use before the free, whatever the streams;
+// the events come from a pool, so a free and a reuse cost one record and one
+// wait.  At most 32 GiB is kept; an allocation that fails empties the cache
+// and is retried.
 struct CachedBlock {
   void* p;
   size_t bytes;
@@ -105,11 +110,14 @@ struct CachedBlock {
   int dev;
 };
 static std::mutex g_cache_mu;
-static std::vector<CachedBlock> g_cache;
+static std::vector<CachedBlock> g_cache;                                    // blocks of kCacheMinBlock and more
+static std::unordered_map<uint64_t, std::vector<CachedBlock>> g_small;      // (device, class) -> blocks
 static std::unordered_map<void*, size_t> g_block_bytes;  // size of every block from hipMallocAsync
+static std::unordered_map<int, std::vector<hipEvent_t>> g_event_pool;       // device -> idle events
 static size_t g_cached_bytes = 0;
 constexpr size_t kCacheMinBlock = (size_t)32 << 20;
 constexpr size_t kCacheMaxBytes = (size_t)32 << 30;
+constexpr size_t kSmallClassMin = 256;
 
 static bool capturing(hipStream_t s) {
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -120,30 +128,74 @@ static bool capturing(hipStream_t s) {
   return cs != hipStreamCaptureStatusNone;
 }
 
+// The size a request is allocated at: its power-of-two class below
+// kCacheMinBlock, itself above.
+static size_t block_size_for(size_t bytes) {
+  if (bytes >= kCacheMinBlock) return bytes;
+  size_t c = kSmallClassMin;
+  while (c < bytes) c <<= 1;
+  return c;
+}
+
+static uint64_t small_key(int dev, size_t bytes) { return ((uint64_t)(uint32_t)dev << 40) | (uint64_t)bytes; }
+
+// An idle event of device `dev` (the caller holds no lock), or null.
+static hipEvent_t event_get(int dev) {
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    auto& pool = g_event_pool[dev];
+    if (!pool.empty()) {
+      const hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  return e;
+}
+
+// Called with g_cache_mu held.
+static void event_put_locked(int dev, hipEvent_t e) { g_event_pool[dev].push_back(e); }
+
 static void* cache_take(size_t bytes, hipStream_t s) {
-  if (bytes < kCacheMinBlock) return nullptr;
   int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
   CachedBlock got{};
   {
     std::lock_guard<std::mutex> lk(g_cache_mu);
-    size_t best = g_cache.size();
-    for (size_t i = 0; i < g_cache.size(); i++) {
-      const CachedBlock& b = g_cache[i];
-      if (b.dev == dev && b.bytes >= bytes && b.bytes / 2 < bytes &&
-          (best == g_cache.size() || b.bytes < g_cache[best].bytes))
-        best = i;
+    if (bytes < kCacheMinBlock) {
+      auto it = g_small.find(small_key(dev, bytes));
+      if (it == g_small.end() || it->second.empty()) return nullptr;
+      got = it->second.back();
+      it->second.pop_back();
+    } else {
+      size_t best = g_cache.size();
+      for (size_t i = 0; i < g_cache.size(); i++) {
+        const CachedBlock& b = g_cache[i];
+        if (b.dev == dev && b.bytes >= bytes && b.bytes / 2 < bytes &&
+            (best == g_cache.size() || b.bytes < g_cache[best].bytes))
+          best = i;
+      }
+      if (best == g_cache.size()) return nullptr;
+      got = g_cache[best];
+      g_cache.erase(g_cache.begin() + (std::ptrdiff_t)best);
     }
-    if (best == g_cache.size()) return nullptr;
-    got = g_cache[best];
     g_cached_bytes -= got.bytes;
-    g_cache.erase(g_cache.begin() + (std::ptrdiff_t)best);
   }
   if (hipStreamWaitEvent(s, got.freed, 0) != hipSuccess) {  // keep the order, or do not reuse
     (void)hipGetLastError();
     (void)hipEventSynchronize(got.freed);
   }
-  (void)hipEventDestroy(got.freed);
+  // the wait has taken the event's current record; the event can be recorded again
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  event_put_locked(dev, got.freed);
   return got.p;
 }
 
@@ -152,23 +204,26 @@ static bool cache_put(void* p, hipStream_t s) {
   {
     std::lock_guard<std::mutex> lk(g_cache_mu);
     const auto it = g_block_bytes.find(p);
-    if (it == g_block_bytes.end() || it->second < kCacheMinBlock || g_cached_bytes + it->second > kCacheMaxBytes)
-      return false;
+    if (it == g_block_bytes.end() || g_cached_bytes + it->second > kCacheMaxBytes) return false;
     bytes = it->second;
   }
   int dev = 0;
-  hipEvent_t ev = nullptr;
-  if (hipGetDevice(&dev) != hipSuccess || hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+  if (hipGetDevice(&dev) != hipSuccess) {
     (void)hipGetLastError();
     return false;
   }
+  const hipEvent_t ev = event_get(dev);
+  if (!ev) return false;
   if (hipEventRecord(ev, s) != hipSuccess) {
     (void)hipGetLastError();
-    (void)hipEventDestroy(ev);
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    event_put_locked(dev, ev);
     return false;
   }
   std::lock_guard<std::mutex> lk(g_cache_mu);
-  g_cache.push_back(CachedBlock{p, bytes, ev, dev});
+  const CachedBlock b{p, bytes, ev, dev};
+  if (bytes < kCacheMinBlock) g_small[small_key(dev, bytes)].push_back(b);
+  else g_cache.push_back(b);
   g_cached_bytes += bytes;
   return true;
 }
@@ -178,13 +233,17 @@ static void cache_flush() {
   {
     std::lock_guard<std::mutex> lk(g_cache_mu);
     all.swap(g_cache);
+    for (auto& kv : g_small)
+      for (const CachedBlock& b : kv.second) all.push_back(b);
+    g_small.clear();
     g_cached_bytes = 0;
     for (const CachedBlock& b : all) g_block_bytes.erase(b.p);
   }
   for (const CachedBlock& b : all) {
     (void)hipEventSynchronize(b.freed);
-    (void)hipEventDestroy(b.freed);
     (void)hipFree(b.p);
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    event_put_locked(b.dev, b.freed);
   }
 }
 
@@ -192,22 +251,25 @@ int dev_alloc(void** p, size_t bytes, hipStream_t s) {
   BOLT_CHECK_DEVICE();
   *p = nullptr;
   const bool cap = capturing(s);
-  if (!cap) *p = cache_take(bytes, s);
+  // outside capture a request takes its block size (class), so the block can
+  // be kept and reused; under capture it is allocated as asked and never kept
+  const size_t alloc_bytes = cap ? (bytes ? bytes : 1) : block_size_for(bytes ? bytes : 1);
+  if (!cap) *p = cache_take(alloc_bytes, s);
   if (!*p) {
-    hipError_t e = hipMallocAsync(p, bytes ? bytes : 1, s);
+    hipError_t e = hipMallocAsync(p, alloc_bytes, s);
     if (e != hipSuccess && !cap) {
       (void)hipGetLastError();
       cache_flush();
-      e = hipMallocAsync(p, bytes ? bytes : 1, s);
+      e = hipMallocAsync(p, alloc_bytes, s);
     }
     if (e != hipSuccess) {
       (void)hipGetLastError();
       return set_error(BOLT_ERR_MEMORY, "device allocation of %zu bytes failed: %s", bytes,
                        hipGetErrorString(e));
     }
-    if (!cap && bytes >= kCacheMinBlock) {
+    if (!cap) {
       std::lock_guard<std::mutex> lk(g_cache_mu);
-      g_block_bytes[*p] = bytes;
+      g_block_bytes[*p] = alloc_bytes;
     }
   }
   account_alloc(*p, bytes);
