@@ -823,6 +823,11 @@ extern "C" int bolt_groupby(bolt_stream_t stream, const bolt_column_t* keys, int
   const bool partitioned = use_lds && n > 0 &&
                            ((flags & BOLT_GROUPBY_FORCE_PARTITIONED) ||
                             (n_spill > (int64_t)(1 << 25) && est > 0.75 * (double)S));
+  // the partitioned tier writes each group once, with one claim, so its
+  // table is sized for at most half load (1.6x the padded estimate) rather than
+  // 2x: half the slots to initialise and for the extraction to scan
+  if (partitioned && !getenv("BOLT_GB_INITIAL_CAP"))
+    cap = next_pow2(std::max<uint64_t>(1024, (uint64_t)((double)want * 1.6)));
   GbPartitions parts;
   if (partitioned)
     BOLT_TRY(gb_part_build(s, ks, P, pkeys.ptr, ptag.ptr, pwords.ptr, (int64_t)pkeys.count, n_parts, spill.ptr,
