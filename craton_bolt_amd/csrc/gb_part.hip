@@ -11,7 +11,8 @@
 // there is no group cap (a partition whose groups overflow its LDS table
 // sends the rest to the global table directly):
 //
-//   compact   the workgroups' spill segments into one dense row list;
+//   prefix    of the workgroups' spill-segment counts (one workgroup); the
+//             spilled rows are then read in place in their segments;
 //   histogram every item (partial record or spilled row) gets a partition
 //             id from its key hash; per-workgroup counts in LDS, totals by
 //             one atomic per (workgroup, partition);
@@ -117,45 +118,88 @@ __device__ __forceinline__ void row_words(const GPlan& P, int64_t row, unsigned 
 }
 
 // The items: partial records [0, n_parts), then the spilled rows (PartSrc).
+// Segment of spilled row j: the last b with spre[b] <= j (so empty segments
+// are passed over), j < n_spill.
+__device__ __forceinline__ int spill_seg_find(const PartSrc& src, int64_t j) {
+  int lo = 0, hi = src.nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if ((int64_t)src.spre[mid] <= j) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// A thread's walk over the spill segments, for spilled rows visited in
+// non-decreasing order: the current segment's range [lo, hi) of j and its
+// base (position = base + j) stay in registers, so a row costs a compare
+// unless it crosses into a later segment.
+struct SpillWalk {
+  int b = -1;
+  int64_t lo = 0, hi = 0, base = 0;
+};
+
+__device__ __forceinline__ void spill_walk_set(const PartSrc& src, int b, SpillWalk* w) {
+  w->b = b;
+  w->lo = (int64_t)src.spre[b];
+  w->hi = b + 1 < src.nseg ? (int64_t)src.spre[b + 1] : INT64_MAX;
+  w->base = (int64_t)b * src.per - w->lo;
+}
+
+// Position of spilled row j in the spill segments.
+__device__ __forceinline__ int64_t spill_at(const PartSrc& src, int64_t j, SpillWalk* w) {
+  if (w->b < 0 || j < w->lo) spill_walk_set(src, spill_seg_find(src, j), w);
+  while (j >= w->hi) spill_walk_set(src, w->b + 1, w);
+  return w->base + j;
+}
+
+// Key and tag of item i; *w is the calling thread's spill-segment walk.
 __device__ __forceinline__ void item_key(const PartSrc& src, const KeySpec& ks, int64_t i, uint64_t* key,
-                                         uint8_t* tag) {
+                                         uint8_t* tag, SpillWalk* w) {
   if (i < src.n_parts) {
     *key = src.pkeys[i];
     *tag = src.ptag[i];
     return;
   }
+  const int64_t at = spill_at(src, i - src.n_parts, w);
   uint64_t k = 0;
   int kr = 1;
-  if (src.keys) k = src.keys[i - src.n_parts];
-  else kr = row_key(ks, (int64_t)src.rows[i - src.n_parts], &k);
+  if (src.keys) k = src.keys[at];
+  else kr = row_key(ks, (int64_t)src.rows[at], &k);
   // spilled rows carry regular keys; the NULL group is tagged for safety
   *tag = kr == 2 ? 1 : (k == kEmpty ? 2 : 0);
   *key = k;
 }
 
-// Spill segments [b * per, b * per + seg[b]) -> dense[prefix(b) ..] (and the
-// captured keys alongside, when there are any); K workgroups share a segment.
-// The segment's offset is a block sum of the counts before it.
-__global__ __launch_bounds__(256) void part_compact_spill_kernel(const uint32_t* __restrict__ spill,
-                                                                 const unsigned long long* __restrict__ skey,
-                                                                 int64_t per, const uint32_t* __restrict__ seg, int K,
-                                                                 uint32_t* __restrict__ dense,
-                                                                 unsigned long long* __restrict__ dkey) {
-  __shared__ uint64_t wsum[4];
-  const int b = blockIdx.x / K, k = blockIdx.x % K;
-  uint64_t part = 0;
-  for (int j = threadIdx.x; j < b; j += 256) part += seg[j];
-  for (int d = 32; d >= 1; d >>= 1) part += __shfl_xor(part, d);
-  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = part;
+// spre[b] = rows in the spill segments before b (spre[nseg] = all of them);
+// one workgroup.
+__global__ __launch_bounds__(kPartBlock) void part_spill_prefix_kernel(const uint32_t* __restrict__ seg, int nseg,
+                                                                       uint64_t* __restrict__ spre) {
+  __shared__ uint64_t wsum[kPartBlock / 64];
+  __shared__ uint64_t carry;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) carry = 0;
   __syncthreads();
-  const uint64_t off = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-  const uint32_t cnt = seg[b];
-  for (uint32_t i = (uint32_t)k * blockDim.x + threadIdx.x; i < cnt; i += (uint32_t)K * blockDim.x) {
-    const int64_t from = (int64_t)b * per + i;
-    dense[off + i] = spill[from];
-    if (skey) dkey[off + i] = skey[from];
+  for (int base = 0; base < nseg; base += kPartBlock) {
+    const int b = base + threadIdx.x;
+    const uint64_t v = b < nseg ? seg[b] : 0;
+    uint64_t inc = v;
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint64_t u = __shfl_up(inc, d);
+      if (lane >= d) inc += u;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    uint64_t before = carry;
+    for (int w = 0; w < wave; w++) before += wsum[w];
+    if (b < nseg) spre[b] = before + inc - v;
+    __syncthreads();
+    if (threadIdx.x == kPartBlock - 1) carry = before + inc;
+    __syncthreads();
   }
+  if (threadIdx.x == 0) spre[nseg] = carry;
 }
+
 __global__ __launch_bounds__(kPartBlock) void part_hist_kernel(PartSrc src, KeySpec ks, int pbits, int64_t chunk,
                                                                uint16_t* __restrict__ pid,
                                                                uint32_t* __restrict__ hist,
@@ -166,10 +210,11 @@ __global__ __launch_bounds__(kPartBlock) void part_hist_kernel(PartSrc src, KeyS
   __syncthreads();
   const int64_t i0 = (int64_t)blockIdx.x * chunk;
   const int64_t i1 = std::min<int64_t>(i0 + chunk, src.n_items);
+  SpillWalk sw;
   for (int64_t i = i0 + threadIdx.x; i < i1; i += kPartBlock) {
     uint64_t key;
     uint8_t tag;
-    item_key(src, ks, i, &key, &tag);
+    item_key(src, ks, i, &key, &tag, &sw);
     if (tag) {  // the NULL group / empty-marker key: merged straight into the global table by the scatter
       pid[i] = 0xFFFFu;
       continue;
@@ -223,11 +268,13 @@ __global__ __launch_bounds__(kPartBlock) void part_scan_kernel(const uint32_t* _
 // group), the key from the captured keys when there are any.
 __device__ __forceinline__ void spilled_record(const PartSrc& src, const KeySpec& ks, const GPlan& P, int64_t j,
                                                unsigned long long* r) {
+  SpillWalk w;
+  const int64_t at = spill_at(src, j, &w);
   uint64_t key = 0;
-  if (src.keys) key = src.keys[j];
-  else (void)row_key(ks, (int64_t)src.rows[j], &key);
+  if (src.keys) key = src.keys[at];
+  else (void)row_key(ks, (int64_t)src.rows[at], &key);
   r[0] = key;
-  row_words(P, (int64_t)src.rows[j], r + 1, 1);
+  row_words(P, (int64_t)src.rows[at], r + 1, 1);
 }
 
 // The first aggregate input the words of a spilled row read (a 4- or 8-byte
@@ -330,6 +377,7 @@ __global__ __launch_bounds__(kStageBlock) void part_stage_kernel(PartSrc src, Ke
     tile0 = blockIdx.x % K2;
     tstep = K2;
   }
+  SpillWalk swalk;  // PASS 1: this thread's walk over the spill segments (tiles ascend)
   for (int64_t tile = tile0; lo + tile * T < hi; tile += tstep) {
     const int64_t t0 = lo + tile * T;
     if (threadIdx.x < 64) cnt[threadIdx.x] = 0;
@@ -393,11 +441,13 @@ __global__ __launch_bounds__(kStageBlock) void part_stage_kernel(PartSrc src, Ke
     if (PASS == 1 && n_spill > 0) {
 #pragma unroll
       for (int u = 0; u < kStageMaxIpt; u++) {
-        // slots past this tile's items per lane repeat item 0's (same lines)
-        int64_t j = t0 + (u < ipt ? u : 0) * kStageBlock + threadIdx.x - src.n_parts;
+        // slots past this tile's items per lane repeat the last one (same
+        // lines), so j never decreases along u and the segment walk holds
+        int64_t j = t0 + (u < ipt ? u : ipt - 1) * kStageBlock + threadIdx.x - src.n_parts;
         j = j < 0 ? 0 : (j >= n_spill ? n_spill - 1 : j);
-        srow[u] = (int64_t)src.rows[j];
-        skey[u] = src.keys ? src.keys[j] : 0ull;
+        const int64_t at = spill_at(src, j, &swalk);
+        srow[u] = (int64_t)src.rows[at];
+        skey[u] = src.keys ? src.keys[at] : 0ull;
       }
       if (va >= 0) {
         const GAgg& g = P.a[va];
@@ -586,13 +636,12 @@ int gb_part_build(hipStream_t s, const KeySpec& ks, const GPlan& P, const unsign
   int pbits = gb_part_bits(est, S, bits_delta);
   const int np = 1 << pbits;
   out->np = np;
-  BOLT_TRY(out->dense.alloc((size_t)std::max<int64_t>(n_spill, 1), s));
-  if (spill_key) BOLT_TRY(out->dense_key.alloc((size_t)std::max<int64_t>(n_spill, 1), s));
-  const int K = 16;  // workgroups per spill segment
-  if (n_spill > 0)
-    part_compact_spill_kernel<<<nseg * K, 256, 0, s>>>(spill, spill_key, spill_per, spill_seg, K, out->dense.ptr,
-                                                        out->dense_key.ptr);
-  PartSrc src{pkeys, ptag, pwords, pcap, n_parts, out->dense.ptr, n_items, spill_key ? out->dense_key.ptr : nullptr};
+  // the spilled rows are read in place in the LDS tier's segments, through
+  // the prefix of the segments' counts
+  BOLT_TRY(out->spre.alloc((size_t)std::max(nseg, 1) + 1, s));
+  if (n_spill > 0) part_spill_prefix_kernel<<<1, kPartBlock, 0, s>>>(spill_seg, nseg, out->spre.ptr);
+  PartSrc src{pkeys, ptag,   pwords,   pcap,          n_parts, spill, n_items, spill_key,
+              spill_per, out->spre.ptr, nseg};
   const int nb = (int)std::min<int64_t>(512, std::max<int64_t>(1, (n_items + 65535) / 65536));
   const int64_t chunk = (n_items + nb - 1) / nb;
   DevBuf<uint32_t> total;

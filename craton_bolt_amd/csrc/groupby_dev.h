@@ -375,18 +375,23 @@ struct PartSrc {
   const unsigned long long* pwords;
   int64_t pcap;
   int64_t n_parts;
-  const uint32_t* rows;  // dense spilled row ids
+  const uint32_t* rows;  // spilled row ids, in the LDS tier's spill segments (see per / spre)
   int64_t n_items;
-  // dense spilled keys (PartOut::spill_key), or null: then they are read by row id
+  // spilled keys (PartOut::spill_key), or null: then they are read by row id
   const unsigned long long* keys;
+  // rows and keys sit in the LDS tier's spill segments: spilled row j is at
+  // b * per + (j - spre[b]) for the segment b holding it (spre: exclusive
+  // prefix of the segments' counts, nseg + 1 entries)
+  int64_t per;
+  const uint64_t* spre;
+  int nseg;
 };
 struct GbPartitions {
   int np = 0, pbits = 0, nb = 0, stride = 0;
   int64_t chunk = 0, recs_cap = 0;
   PartSrc src{};
   KeySpec ks{};
-  DevBuf<uint32_t> dense;
-  DevBuf<unsigned long long> dense_key;
+  DevBuf<uint64_t> spre;  // prefix of the spill segments' counts
   DevBuf<uint16_t> pid;
   DevBuf<uint32_t> hist;
   DevBuf<unsigned long long> cursor;
