@@ -211,6 +211,39 @@ __global__ __launch_bounds__(kPartBlock) void part_hist_kernel(PartSrc src, KeyS
   const int64_t i0 = (int64_t)blockIdx.x * chunk;
   const int64_t i1 = std::min<int64_t>(i0 + chunk, src.n_items);
   SpillWalk sw;
+  if (src.keys) {
+    // spilled keys captured: four items per lane, their loads issued together
+    // (a partial record reads spilled key 0 and a spilled row partial record
+    // 0, both in their buffers, and the selects keep what applies)
+    for (int64_t ib = i0 + threadIdx.x; ib < i1; ib += 4 * kPartBlock) {
+      uint64_t kk[4];
+      uint8_t tg[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        int64_t i = ib + (int64_t)u * kPartBlock;
+        i = i < i1 ? i : i1 - 1;
+        const bool sp = i >= src.n_parts;
+        const int64_t at = sp ? spill_at(src, i - src.n_parts, &sw) : 0;
+        const int64_t pi = sp ? 0 : i;
+        const uint64_t ks_ = src.keys[at], kp = src.pkeys[pi];
+        const uint8_t tp = src.ptag[pi];
+        kk[u] = sp ? ks_ : kp;
+        tg[u] = sp ? (uint8_t)(ks_ == kEmpty ? 2 : 0) : tp;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int64_t i = ib + (int64_t)u * kPartBlock;
+        if (i >= i1) continue;
+        if (tg[u]) {  // the NULL group / empty-marker key: merged straight into the global table by the scatter
+          pid[i] = 0xFFFFu;
+          continue;
+        }
+        const uint32_t p = part_of(kk[u], pbits);
+        pid[i] = (uint16_t)p;
+        atomicAdd(&lh[p], 1u);
+      }
+    }
+  } else
   for (int64_t i = i0 + threadIdx.x; i < i1; i += kPartBlock) {
     uint64_t key;
     uint8_t tag;
