@@ -526,6 +526,7 @@ struct bolt_gb_batch {
     if (h_plan) (void)hipHostFree(h_plan);
     if (own) {
       (void)hipStreamSynchronize(own);
+      cache_forget_stream(own);
       (void)hipStreamDestroy(own);
     }
   }

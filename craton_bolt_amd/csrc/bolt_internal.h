@@ -57,6 +57,9 @@ void prof_end(hipStream_t s);
 // Stream-ordered allocation from the device pool (accounted in pool stats).
 int dev_alloc(void** p, size_t bytes, hipStream_t s);
 int dev_free(void* p, hipStream_t s);
+// Before destroying stream s: blocks the device cache holds from frees on s are
+// then reused only once complete.
+void cache_forget_stream(hipStream_t s);
 
 // RAII device allocation freed (stream-ordered) on destruction.
 template <typename T>

@@ -9,6 +9,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cstring>
 #include <mutex>
@@ -98,16 +99,21 @@ static void account_free(void* p) {
 // blocks under 32 MiB by power-of-two size class (a request is rounded up to
 // its class when allocated), larger ones to a request of at most their size
 // and more than half of it.  Each kept block carries an event recorded on the
-// freeing stream, and the stream that takes it first waits for that event, so
-// the reuse is ordered after every use before the free, whatever the streams;
-// the events come from a pool, so a free and a reuse cost one record and one
-// wait.  At most 32 GiB is kept; an allocation that fails empties the cache
+// freeing stream.  A block is reused on the stream that freed it, where the
+// stream's own order puts the reuse after every use before the free, or on
+// another stream once that event has completed; no stream ever waits on
+// another's queued work through the cache, and no wait (a barrier in the
+// queue, which keeps kernels from overlapping) is added.  Blocks freed on a
+// stream destroyed through bolt_hip_stream_destroy lose the same-stream
+// shortcut, so a later stream with the same handle checks completion too.
+// The events come from a pool, so a free costs one record.  At most 32 GiB is kept; an allocation that fails empties the cache
 // and is retried.
 struct CachedBlock {
   void* p;
   size_t bytes;
   hipEvent_t freed;
   int dev;
+  hipStream_t stream;  // where it was freed
 };
 static std::mutex g_cache_mu;
 static std::vector<CachedBlock> g_cache;                                    // blocks of kCacheMinBlock and more
@@ -167,33 +173,59 @@ static void* cache_take(size_t bytes, hipStream_t s) {
     (void)hipGetLastError();
     return nullptr;
   }
+  // A block freed on this stream is taken first (its event is behind this
+  // stream's own order); one freed on another stream only once its event has
+  // completed, so that concurrent streams never wait on each other's queued
+  // work through the cache (a block still in flight elsewhere is left, and a
+  // new one allocated).
+  auto usable = [&](const CachedBlock& b, bool same_only) {
+    if (b.stream == s) return true;
+    if (same_only) return false;
+    const hipError_t q = hipEventQuery(b.freed);
+    if (q != hipSuccess) (void)hipGetLastError();
+    return q == hipSuccess;
+  };
   CachedBlock got{};
   {
     std::lock_guard<std::mutex> lk(g_cache_mu);
+    bool found = false;
     if (bytes < kCacheMinBlock) {
       auto it = g_small.find(small_key(dev, bytes));
       if (it == g_small.end() || it->second.empty()) return nullptr;
-      got = it->second.back();
-      it->second.pop_back();
-    } else {
-      size_t best = g_cache.size();
-      for (size_t i = 0; i < g_cache.size(); i++) {
-        const CachedBlock& b = g_cache[i];
-        if (b.dev == dev && b.bytes >= bytes && b.bytes / 2 < bytes &&
-            (best == g_cache.size() || b.bytes < g_cache[best].bytes))
-          best = i;
+      std::vector<CachedBlock>& v = it->second;
+      for (int pass = 0; pass < 2 && !found; pass++) {
+        // this stream's newest block first; else one of the eight oldest
+        // (the likeliest to have completed) freed elsewhere
+        const size_t lim = pass == 0 ? v.size() : std::min<size_t>(v.size(), 8);
+        for (size_t k = 0; k < lim && !found; k++) {
+          const size_t i = pass == 0 ? v.size() - 1 - k : k;
+          if (!usable(v[i], pass == 0)) continue;
+          got = v[i];
+          v.erase(v.begin() + (std::ptrdiff_t)i);
+          found = true;
+        }
       }
-      if (best == g_cache.size()) return nullptr;
-      got = g_cache[best];
-      g_cache.erase(g_cache.begin() + (std::ptrdiff_t)best);
+    } else {
+      for (int pass = 0; pass < 2 && !found; pass++) {
+        size_t best = g_cache.size();
+        for (size_t i = 0; i < g_cache.size(); i++) {
+          const CachedBlock& b = g_cache[i];
+          if (b.dev == dev && b.bytes >= bytes && b.bytes / 2 < bytes &&
+              (best == g_cache.size() || b.bytes < g_cache[best].bytes) && usable(b, pass == 0))
+            best = i;
+        }
+        if (best == g_cache.size()) continue;
+        got = g_cache[best];
+        g_cache.erase(g_cache.begin() + (std::ptrdiff_t)best);
+        found = true;
+      }
     }
+    if (!found) return nullptr;
     g_cached_bytes -= got.bytes;
   }
-  if (hipStreamWaitEvent(s, got.freed, 0) != hipSuccess) {  // keep the order, or do not reuse
-    (void)hipGetLastError();
-    (void)hipEventSynchronize(got.freed);
-  }
-  // the wait has taken the event's current record; the event can be recorded again
+  // no stream wait: a block is taken on the stream that freed it (ordered by
+  // that stream) or after its event completed.  A wait would put a barrier in
+  // the stream's queue and keep its kernels from overlapping.
   std::lock_guard<std::mutex> lk(g_cache_mu);
   event_put_locked(dev, got.freed);
   return got.p;
@@ -221,11 +253,22 @@ static bool cache_put(void* p, hipStream_t s) {
     return false;
   }
   std::lock_guard<std::mutex> lk(g_cache_mu);
-  const CachedBlock b{p, bytes, ev, dev};
+  const CachedBlock b{p, bytes, ev, dev, s};
   if (bytes < kCacheMinBlock) g_small[small_key(dev, bytes)].push_back(b);
   else g_cache.push_back(b);
   g_cached_bytes += bytes;
   return true;
+}
+
+// Blocks freed on `s` may only be reused once complete (the stream is going away).
+void cache_forget_stream(hipStream_t s) {
+  const hipStream_t gone = reinterpret_cast<hipStream_t>(~(uintptr_t)0);
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  for (CachedBlock& b : g_cache)
+    if (b.stream == s) b.stream = gone;
+  for (auto& kv : g_small)
+    for (CachedBlock& b : kv.second)
+      if (b.stream == s) b.stream = gone;
 }
 
 static void cache_flush() {
@@ -741,6 +784,7 @@ int bolt_hip_stream_create(bolt_stream_t* stream, unsigned int flags) {
 }
 int bolt_hip_stream_destroy(bolt_stream_t stream) {
   RT_CHECK();
+  bolt::cache_forget_stream((hipStream_t)stream);
   RT_HIP(hipStreamDestroy((hipStream_t)stream));
   return 0;
 }

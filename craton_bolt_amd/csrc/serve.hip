@@ -935,6 +935,7 @@ extern "C" int bolt_server_create(bolt_stream_t stream, const bolt_column_t* fac
     srv->d_lists.reset();
     srv->d_counts.reset();
     (void)hipStreamSynchronize(srv->s);
+    cache_forget_stream(srv->s);
     (void)hipStreamDestroy(srv->s);
   };
   for (int d = 0; d < n_dims; d++) {
@@ -1094,7 +1095,11 @@ extern "C" void bolt_server_destroy(bolt_server_t* srv) {
   srv->d_counts.reset();
   for (auto* t : srv->dim_tables) bolt_join_table_release(t);
   (void)hipStreamSynchronize(srv->s);
+  cache_forget_stream(srv->s);
   (void)hipStreamDestroy(srv->s);
-  if (srv->fs) (void)hipStreamDestroy(srv->fs);
+  if (srv->fs) {
+    cache_forget_stream(srv->fs);
+    (void)hipStreamDestroy(srv->fs);
+  }
   delete srv;
 }
