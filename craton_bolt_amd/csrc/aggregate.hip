@@ -482,7 +482,7 @@ static int bolt::aggregate_impl(hipStream_t s, const bolt_column_t* inputs, int3
     JitCols cols;
     hipFunction_t fn;
     if (jit_spec_from(f, all, false, 0, false, 0, &sp, &lits, &cols) == BOLT_OK &&
-        jit_get(jit_row_agg_source(sp), &fn) == BOLT_OK) {
+        jit_get_keyed(jit_row_agg_key(sp), [&] { return jit_row_agg_source(sp); }, &fn) == BOLT_OK) {
       BOLT_TRY(parts.alloc((size_t)grid * n_aggs, s));
       prof_begin(s);
       BOLT_TRY(jit_launch_row_agg(fn, grid, s, nullptr, 0, nullptr, n, nullptr, nullptr, lits, cols, parts.ptr, nullptr, 0));

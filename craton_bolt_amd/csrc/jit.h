@@ -76,6 +76,9 @@ bool jit_enabled();
 // Device helpers every generated kernel starts with (device_acc.h, gb_lds.h).
 const char* jit_prelude_text();
 std::string jit_row_agg_source(const JitSpec& sp);
+// Compact description of everything in `sp` that shapes jit_row_agg_source's
+// text, for jit_get_keyed.
+std::string jit_row_agg_key(const JitSpec& sp);
 int jit_get(const std::string& src, hipFunction_t* fn);
 int jit_spec_from(const DevFilter& f, const AggList& aggs, bool probe, int key_dtype, bool key_valid, int carry_dtype,
                   JitSpec* sp, JitLits* lits, JitCols* cols);

@@ -340,6 +340,18 @@ int jit_get_keyed(const std::string& key, const std::function<std::string()>& ge
   return BOLT_OK;
 }
 
+std::string jit_row_agg_key(const JitSpec& sp) {
+  std::ostringstream k;
+  k << "row:" << sp.probe << sp.bloom << ':' << sp.unroll << ':' << sp.probe_slots << ':' << sp.key_dtype << ':'
+    << sp.key_valid << ':' << sp.carry_dtype << '|';
+  for (const JitSpec::Col& c : sp.cols) k << c.dtype << (c.has_valid ? 'v' : 'n') << ',';
+  k << '|';
+  for (const JitSpec::Term& t : sp.terms) k << t.col << '.' << t.op << ',';
+  k << '|';
+  for (const JitSpec::Agg& g : sp.aggs) k << g.op << '.' << g.src << '.' << g.col << '.' << g.dtype << ',';
+  return k.str();
+}
+
 std::string gb_jit_key(const GbJitSpec& sp) {
   std::ostringstream k;
   k << "gb:" << sp.unroll << ':' << sp.n_keys << ':' << sp.t0 << ':' << sp.t1 << ':' << sp.kv0 << sp.kv1 << ':'

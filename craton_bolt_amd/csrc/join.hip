@@ -860,7 +860,7 @@ extern "C" int bolt_join_probe_aggregate(bolt_stream_t stream, const bolt_join_t
     const bool spec_ok = jit_spec_from(f, all, true, key.dtype == BOLT_I64 ? BOLT_I64 : BOLT_I32,
                                        key.validity != nullptr, t->carry_dtype, &sp, &lits, &cols) == BOLT_OK;
     sp.bloom = t->bloom != nullptr;
-    if (spec_ok && jit_get(jit_row_agg_source(sp), &fn) == BOLT_OK) {
+    if (spec_ok && jit_get_keyed(jit_row_agg_key(sp), [&] { return jit_row_agg_source(sp); }, &fn) == BOLT_OK) {
       BOLT_TRY(parts.alloc((size_t)grid * n_aggs, s));
       prof_begin(s);
       BOLT_TRY(jit_launch_row_agg(fn, grid, s, t->tbl, t->mask, t->flags, key.length, key.data, key.validity, lits, cols,
