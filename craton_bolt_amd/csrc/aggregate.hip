@@ -468,7 +468,9 @@ static int bolt::aggregate_impl(hipStream_t s, const bolt_column_t* inputs, int3
   AggList all;
   BOLT_TRY(make_agg_list(aggs, n_aggs, inputs, n_inputs, 0, &all));
 
-  unsigned grid = grid_for(n, kReduceBlock, kAggUnroll * 4, 256 * 8);
+  // at most 1024 workgroups: a streaming read of 2 GiB measured 6.8-6.9 TB/s
+  // at 1024 x 256 threads with nontemporal loads, 5.4-6.1 TB/s at 4096-16384
+  unsigned grid = stream_grid(n, kReduceBlock, kAggUnroll * 4, 1024);
   DevBuf<Acc> parts, outs;
   Acc* outs_p = scratch ? scratch->outs : nullptr;
   if (!outs_p) {

@@ -121,6 +121,17 @@ inline unsigned grid_for(int64_t work_items, int block, int per_thread, int max_
   return (unsigned)need;
 }
 
+// Grid of a streaming kernel: grid_for's, lowered to BOLT_STREAM_GRID when
+// that is set and smaller (a tuning knob; the kernels stride over the rows).
+inline unsigned stream_grid(int64_t work_items, int block, int per_thread, int max_blocks) {
+  unsigned g = grid_for(work_items, block, per_thread, max_blocks);
+  if (const char* e = getenv("BOLT_STREAM_GRID")) {
+    const long v = atol(e);
+    if (v >= 1 && (unsigned long)v < g) g = (unsigned)v;
+  }
+  return g;
+}
+
 // ---- device-side helpers --------------------------------------------------
 
 __device__ __forceinline__ bool bit_valid(const uint8_t* validity, int64_t i) {
