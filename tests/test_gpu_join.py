@@ -229,3 +229,29 @@ def test_partitioned_probe_aggregate_matches_oracle(gpu, kernel_path, monkeypatc
     assert got[2] == len(ol)
     assert got[3] == int(q[orr].max())
     assert got[4] == int(q[orr].astype(np.int64).sum())
+
+
+@pytest.mark.parametrize("jit", ["1", "0"])
+def test_partitioned_probe_aggregate_without_probe_columns(gpu, kernel_path, monkeypatch, jit):
+    """The partitioned form when no aggregate reads a probe column, so nothing
+    is carried with the key: COUNT(*), SUM and MAX of the build side's carried
+    column, behind a filter on a probe column; duplicate build keys."""
+    if kernel_path != "partitioned":
+        pytest.skip("a test of the partitioned form")
+    monkeypatch.setenv("BOLT_JIT", jit)
+    print(f"[case] partitioned, no carried probe column, jit={jit}", flush=True)
+    rng = np.random.default_rng(5)
+    nb, npr = 200_000, 1_500_000
+    bk = (rng.integers(0, 150_000, nb) * 7 + 3).astype(np.int64)
+    bp = rng.integers(-100, 100, nb).astype(np.int32)
+    pk = (rng.integers(0, 200_000, npr) * 7 + 3).astype(np.int64)
+    amt = rng.uniform(0, 100, npr)
+    t = JoinTable(DeviceColumn.from_numpy(bk), carry=DeviceColumn.from_numpy(bp))
+    got = t.probe_aggregate([DeviceColumn.from_numpy(pk), DeviceColumn.from_numpy(amt)], 0,
+                            [(N.COUNT_STAR, 0), (N.SUM, N.JOIN_CARRY), (N.MAX, N.JOIN_CARRY)],
+                            [Term(1, N.GT, 30.0)])
+    ol, orr = O.inner_join(O.HostCol(bk, O.I64), O.HostCol(pk, O.I64))
+    ol = ol[amt[orr] > 30.0]
+    assert got[0] == len(ol)
+    assert got[1] == int(bp[ol].astype(np.int64).sum())
+    assert got[2] == int(bp[ol].max())
